@@ -1,0 +1,383 @@
+// Test driver around the reference SNAPI library (oracle/_ref/libsnapi_ref.so).
+//
+// It plays the part of the reference worker (slave/slave.cc:117-146): build an
+// atmosphere from an `atmos{}` config block (atmos/acfg.cc:16), read its SPINOR
+// column (atmos/atmos_fio.cc:25), and call the reference's own entry points.
+// Besides the final observables it writes the intermediate state that the
+// reference holds between `nltepops()` and `formal()` so that the oracle and
+// the HIP path can be fed exactly the same inputs.
+//
+// Modes (all output is plain text on stdout, one `key value...` record a line):
+//   synth  <atmos.cfg> <lambda.txt> <el> <az> [every]
+//          per-depth state, per-line atomic data, op/em at every `every`-th
+//          wavelength, and the Stokes vector of atmosphere::obs_stokes
+//          (atmos/atmos_obs.cc:16-87).
+//   resp   <atmos.cfg> <lambda.txt> <el> <az>
+//          atmosphere::obs_stokes_responses (atmos/atmos_rad_synthesis.cc:166-399):
+//          Stokes plus d(IQUV)/d(q_p(z)) for the 7 physical parameters.
+//   lmstep <nrow> <npar> <lm> <seed>
+//          one damped normal-equation solve through the reference's cmlib
+//          routines (multiply_with_transpose, solve_svd) on a seeded J and r,
+//          in the order atmos/atmos_fit.cc:129-160 uses them.
+//   formal_scalar <atmos.cfg> <lambda.txt> <el> <az>
+//          unpolarised formal solution with approximate Lambda operator
+//          (plpar/atmos_ppbez_rts.cc:21-219) on the reference's continuum.
+//   nlte   <atmos.cfg> <el> <az>
+//          atmosphere::nltepops (atmos/atmos_pop.cc:67-240): converged level
+//          populations of every NLTE atom.
+//
+// This file is test infrastructure: it is compiled only against the reference
+// sources (oracle/ref.mk) and never linked into the product.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cmath>
+#include <unistd.h>
+
+// The driver reads the reference's protected state; it is compiled on its own
+// and only ever inspects, never changes, that state.
+#define protected public
+#define private public
+#include "types.h"
+#include "io.h"
+#include "mem.h"
+#include "mathtools.h"
+#include "uts.h"
+#include "atmos.h"
+#include "acfg.h"
+#include "atmol/atmol.h"
+#include "atmol/atom/atom.h"
+#include "atmol/mol/mol.h"
+#include "plpar/atmos_ppbez.h"
+#include "const.h"  // last: it defines one-letter macros (c, h, k, R, ...)
+#undef protected
+#undef private
+
+static char *slurp(const char *fn) {
+  FILE *f = fopen(fn, "r");
+  if (!f) { fprintf(stderr, "cannot open %s\n", fn); exit(2); }
+  fseek(f, 0, SEEK_END);
+  long n = ftell(f);
+  fseek(f, 0, SEEK_SET);
+  char *b = new char[n + 1];
+  size_t got = fread(b, 1, n, f);
+  b[got] = 0;
+  fclose(f);
+  return b;
+}
+
+// lambda file: one air wavelength per line, in Angstrom.
+static fp_t *read_lambda(const char *fn, int &n) {
+  FILE *f = fopen(fn, "r");
+  if (!f) { fprintf(stderr, "cannot open %s\n", fn); exit(2); }
+  fp_t buf[100000];
+  n = 0;
+  double v;
+  while (fscanf(f, "%lf", &v) == 1) buf[n++] = v * 1e-8;  // Angstrom -> cm
+  fclose(f);
+  fp_t *lam = new fp_t[n] - 1;  // 1-based, as jsub/cfg.cc:209-212 hands it over
+  for (int i = 1; i <= n; ++i) lam[i] = buf[i - 1];
+  return lam;
+}
+
+static atmosphere *make_atmos(const char *cfgfile, io_class &io) {
+  char *txt = slurp(cfgfile);
+  // the config file holds the body of an atmos{} block
+  acfg *cfg = new acfg(txt, io);
+  atmosphere *atm = atmos_new(cfg, io);
+  char wd[4096];
+  if (!getcwd(wd, sizeof(wd))) strcpy(wd, ".");
+  atm->init(wd, &io);
+  // Hand the column over the way the master hands it to a worker: pack it and
+  // rebuild it from the buffer (master/job.cc:63-86, slave/slave.cc:107-110).
+  int32_t sz = atm->size(io);
+  uint08_t *buf = new uint08_t[sz];
+  atm->pack(buf, 0, io);
+  int32_t offs = 0;
+  atmosphere *w = atmos_new(buf, offs, 0, io);
+  delete[] buf;
+  delete atm;
+  atm = w;
+  // SNAPI_REF_HE=1: re-integrate hydrostatic equilibrium on the tau grid with
+  // the reference's own equation of state (plpar/atmos_pp.cc:373-455) before
+  // anything else, so that Pg follows from T(tau) and log g alone.
+  if (getenv("SNAPI_REF_HE") && atoi(getenv("SNAPI_REF_HE"))) atm->enforce_hequilibrium();
+  return atm;
+}
+
+static void dump_state(atmosphere *a, fp_t ***Vr, fp_t ****Bv) {
+  int x1 = a->x1l, x2 = a->x2l;
+  printf("nd %d\n", a->x3h - a->x3l + 1);
+  printf("natm %d\n", a->natm);
+  printf("tau_grid %d\n", a->tau_grid);
+  for (int z = a->x3l; z <= a->x3h; ++z) {
+    printf("depth %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+           z, a->T[x1][x2][z], a->Ne[x1][x2][z], a->Nt[x1][x2][z], a->Vt[x1][x2][z],
+           Vr[x1][x2][z], Bv[1][x1][x2][z], Bv[2][x1][x2][z], Bv[3][x1][x2][z],
+           a->op_referent[x1][x2][z], a->tau_referent[x1][x2][z], a->rt_grid[z],
+           a->get_pop(x1, x2, z, 0, 0), a->x3[z]);
+  }
+  // per-atom line data and populations
+  for (int s = 0; s < a->natm; ++s) {
+    atom *at = dynamic_cast<atom *>(a->atml[s]);
+    if (!at) {
+      printf("species %d mol %s\n", s, a->atml[s]->id);
+      continue;
+    }
+    printf("species %d atom %s Z %d mass %.17g nlte %d\n", s, at->id, (int)at->Z, at->mass, (int)at->NLTE);
+    for (int zi = 0; zi <= at->Z; ++zi)
+      for (int l = 0; l < at->nl[zi]; ++l)
+        printf("level %d %d %d %.17g g %d j %d lqn %d glande %.17g\n", s, zi, l, at->ee[zi][l], (int)at->g[zi][l],
+               (int)at->j_qn[zi][l], (int)at->l_qn[zi][l], at->g_lande[zi][l]);
+    for (int zi = 0; zi <= at->Z; ++zi)
+      for (int i = 1; i < at->nl[zi]; ++i)
+        for (int ii = 0; ii < i; ++ii) {
+          if (!(at->A[zi][i][ii] > 1E0)) continue;
+          int tr = at->tmap[zi][i][ii];
+          fp_t lam = h * c / (at->ee[zi][i] - at->ee[zi][ii]);
+          printf("line %d %d %d %d tr %d lam %.17g A %.17g Blu %.17g Bul %.17g grad %.17g sigma %.17g alpha %.17g split %d\n",
+                 s, zi, ii, i, tr, lam, at->A[zi][i][ii], at->B[zi][ii][i], at->B[zi][i][ii],
+                 at->damp_rad(at->A[zi], ii, i), at->col_dam_cross_section[zi][i][ii],
+                 at->alpha_col_dam[zi][i][ii], at->split ? at->split[tr] : -1);
+          if (at->split && at->split[tr]) {
+            for (int m = 0; m < at->nm[tr][0]; ++m)
+              printf("zc %d %d p %.17g %.17g\n", s, tr, at->delta_lambda_p[tr][m], at->S_p[tr][m]);
+            for (int m = 0; m < at->nm[tr][1]; ++m)
+              printf("zc %d %d b %.17g %.17g\n", s, tr, at->delta_lambda_b[tr][m], at->S_b[tr][m]);
+            for (int m = 0; m < at->nm[tr][2]; ++m)
+              printf("zc %d %d r %.17g %.17g\n", s, tr, at->delta_lambda_r[tr][m], at->S_r[tr][m]);
+          }
+          for (int z = a->x3l; z <= a->x3h; ++z) {
+            fp_t T = a->T[x1][x2][z];
+            fp_t dld = at->broad_dop(lam, T, a->Vt[x1][x2][z]);
+            fp_t gc = at->damp_col(x1, x2, z, zi, i, ii, T, a->Ne[x1][x2][z], lam);
+            printf("lpop %d %d %d %.17g %.17g %.17g %.17g\n", s, tr, z, at->pop[x1][x2][z].n[zi][ii],
+                   at->pop[x1][x2][z].n[zi][i], dld, gc);
+          }
+        }
+  }
+}
+
+static void dump_opem(atmosphere *a, int nlambda, fp_t *lam_vac, fp_t ******op, fp_t *****em, int every) {
+  int x1 = a->x1l, x2 = a->x2l;
+  for (int l = 1; l <= nlambda; l += every)
+    for (int z = a->x3l; z <= a->x3h; ++z) {
+      printf("opem %d %d %.17g", l, z, lam_vac[l]);
+      for (int s = 1; s <= 4; ++s)
+        for (int sp = 1; sp <= 4; ++sp) printf(" %.17g", op[l][x1][x2][z][s][sp]);
+      for (int s = 1; s <= 4; ++s) printf(" %.17g", em[l][x1][x2][z][s]);
+      printf("\n");
+    }
+}
+
+// Background (every species but the line transitions) per depth at lambda index l:
+// atom bound-free and H- by calling each species with its lines removed is not
+// possible without editing the reference, so the driver records each species'
+// whole contribution separately instead.
+static void dump_species_opem(atmosphere *a, fp_t ***Vr, fp_t ****Bv, fp_t th, fp_t ph, int nlambda,
+                              fp_t *lam_vac, int every) {
+  int x1 = a->x1l, x2 = a->x2l;
+  for (int s = 0; s < a->natm; ++s) {
+    fp_t ******op = ft6dim(1, nlambda, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h, 1, 4, 1, 4);
+    fp_t *****em = ft5dim(1, nlambda, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h, 1, 4);
+    memset(op[1][x1][x2][a->x3l][1] + 1, 0, nlambda * (a->x3h - a->x3l + 1) * 16 * sizeof(fp_t));
+    memset(em[1][x1][x2][a->x3l] + 1, 0, nlambda * (a->x3h - a->x3l + 1) * 4 * sizeof(fp_t));
+    a->atml[s]->op_em_vector(a->T, a->Ne, Vr, a->Vt, Bv, th, ph, lam_vac, nlambda, op, em);
+    for (int l = 1; l <= nlambda; l += every)
+      for (int z = a->x3l; z <= a->x3h; ++z)
+        printf("spop %d %d %d %.17g %.17g\n", s, l, z, op[l][x1][x2][z][1][1], em[l][x1][x2][z][1]);
+    // bound-free part of an atom on its own (atom_rad_synthesis.cc:226-262)
+    if (atom *at = dynamic_cast<atom *>(a->atml[s])) {
+      memset(op[1][x1][x2][a->x3l][1] + 1, 0, nlambda * (a->x3h - a->x3l + 1) * 16 * sizeof(fp_t));
+      memset(em[1][x1][x2][a->x3l] + 1, 0, nlambda * (a->x3h - a->x3l + 1) * 4 * sizeof(fp_t));
+      at->boundfree_op_em_vector(a->T, a->Ne, Vr, th, ph, lam_vac, nlambda, op, em);
+      for (int l = 1; l <= nlambda; l += nlambda - 1)
+        for (int z = a->x3l; z <= a->x3h; ++z)
+          printf("bfop %d %d %d %.17g %.17g\n", s, l, z, op[l][x1][x2][z][1][1], em[l][x1][x2][z][1]);
+    }
+    del_ft6dim(op, 1, nlambda, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h, 1, 4, 1, 4);
+    del_ft5dim(em, 1, nlambda, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h, 1, 4);
+  }
+}
+
+static int mode_synth(int argc, char **argv, io_class &io) {
+  if (argc < 6) return 2;
+  atmosphere *a = make_atmos(argv[2], io);
+  int nlambda;
+  fp_t *lambda = read_lambda(argv[3], nlambda);
+  fp_t el = atof(argv[4]), az = atof(argv[5]);
+  int every = (argc > 6) ? atoi(argv[6]) : 10;
+  int x1 = a->x1l, x2 = a->x2l;
+
+  // The preamble of atmosphere::obs_stokes (atmos/atmos_obs.cc:23-58), step by
+  // step, so the state in between can be recorded.
+  a->set_grid(a->tau_grid);
+  a->boundary_condition_for_rt = -1;
+  a->popsetup();
+  a->compute_op_referent();
+  if (!a->tau_grid) a->compute_tau_referent();
+  a->nltepops();
+  a->atm_pop_setup();
+  a->atm_pop_fill();
+  fp_t ***Vr = a->project(a->Vx, a->Vy, a->Vz, el, az, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  fp_t ****Bv = a->transform(a->Bx, a->By, a->Bz, el, az, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  for (int s = 0; s < a->natm; ++s) {
+    a->atml[s]->rtsetup(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    a->atml[s]->zeeman_setup();
+  }
+  fp_t *lam_vac = airtovac(lambda + 1, nlambda) - 1;
+  printf("nlambda %d\n", nlambda);
+  for (int l = 1; l <= nlambda; ++l) printf("lambda %d %.17g %.17g\n", l, lambda[l], lam_vac[l]);
+  dump_state(a, Vr, Bv);
+  dump_species_opem(a, Vr, Bv, el, az, nlambda, lam_vac, every);
+
+  fp_t ******op = ft6dim(1, nlambda, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h, 1, 4, 1, 4);
+  fp_t *****em = ft5dim(1, nlambda, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h, 1, 4);
+  a->op_em_vector(Vr, Bv, el, az, lam_vac, nlambda, op, em);
+  dump_opem(a, nlambda, lam_vac, op, em, every);
+  fp_t ****S = ft4dim(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h, 1, 4);
+  for (int l = 1; l <= nlambda; ++l) {
+    if (a->tau_grid) a->normalize_to_referent_opacity(op[l], em[l]);
+    a->formal(a->rt_grid, S, 0, op[l], em[l], el, az, a->boundary_condition_for_rt);
+    printf("stokes_steps %d %.17g %.17g %.17g %.17g\n", l, S[x1][x2][a->x3l][1], S[x1][x2][a->x3l][2],
+           S[x1][x2][a->x3l][3], S[x1][x2][a->x3l][4]);
+    if (l == 1 || l == nlambda / 2 || (every > 0 && l % (every * 4) == 1))
+      for (int z = a->x3l; z <= a->x3h; ++z)
+        printf("sdepth %d %d %.17g %.17g %.17g %.17g\n", l, z, S[x1][x2][z][1], S[x1][x2][z][2], S[x1][x2][z][3],
+               S[x1][x2][z][4]);
+  }
+  for (int s = 0; s < a->natm; ++s) {
+    a->atml[s]->rtclean(0, 0, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    a->atml[s]->zeeman_clear();
+  }
+  a->popclean();
+
+  // And the entry point itself, end to end, for the final observable.
+  observable *o = a->obs_stokes(el, az, lambda, nlambda);
+  fp_t **Sv = o->get_S(1, 1);
+  for (int l = 1; l <= nlambda; ++l)
+    printf("stokes %d %.17g %.17g %.17g %.17g\n", l, Sv[1][l], Sv[2][l], Sv[3][l], Sv[4][l]);
+  delete o;
+  return 0;
+}
+
+static int mode_resp(int argc, char **argv, io_class &io) {
+  if (argc < 6) return 2;
+  atmosphere *a = make_atmos(argv[2], io);
+  int nlambda;
+  fp_t *lambda = read_lambda(argv[3], nlambda);
+  fp_t el = atof(argv[4]), az = atof(argv[5]);
+  a->set_grid(a->tau_grid);
+  int nd = a->x3h - a->x3l + 1;
+  fp_t ****rsp = ft4dim(1, 7, a->x3l, a->x3h, 1, nlambda, 1, 4);
+  // The response entry point of the fit path (atmos/atmos_rad_synthesis.cc:
+  // 166-399) with no node model: it fills the per-depth responses to the
+  // seven physical parameters T, Nt, vt, vLOS, |B|, incl, azi.
+  observable *o = a->obs_stokes_responses(el, az, lambda, nlambda, (fp_t ***)0, (model *)0, rsp);
+  fp_t **Sv = o->get_S(1, 1);
+  printf("nlambda %d nd %d\n", nlambda, nd);
+  for (int l = 1; l <= nlambda; ++l)
+    printf("stokes %d %.17g %.17g %.17g %.17g\n", l, Sv[1][l], Sv[2][l], Sv[3][l], Sv[4][l]);
+  for (int p = 1; p <= 7; ++p)
+    for (int z = a->x3l; z <= a->x3h; ++z)
+      for (int l = 1; l <= nlambda; ++l)
+        printf("resp %d %d %d %.17g %.17g %.17g %.17g\n", p, z, l, rsp[p][z][l][1], rsp[p][z][l][2], rsp[p][z][l][3],
+               rsp[p][z][l][4]);
+  return 0;
+}
+
+// xorshift-based deterministic generator shared with the Python side of the
+// tests (tests/golden/make_golden.py restates it).
+static unsigned long long g_state;
+static double urand() {
+  g_state ^= g_state << 13;
+  g_state ^= g_state >> 7;
+  g_state ^= g_state << 17;
+  return (double)(g_state >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static int mode_lmstep(int argc, char **argv, io_class &io) {
+  if (argc < 6) return 2;
+  int nrow = atoi(argv[2]), npar = atoi(argv[3]);
+  fp_t lm = atof(argv[4]);
+  g_state = strtoull(argv[5], 0, 10) * 2654435761ULL + 88172645463325252ULL;
+  fp_t **J = ft2dim(1, nrow, 1, npar);
+  fp_t *r = new fp_t[nrow] - 1;
+  for (int i = 1; i <= nrow; ++i)
+    for (int p = 1; p <= npar; ++p) J[i][p] = 2.0 * urand() - 1.0;
+  // make the last column nearly dependent on the first so the SVD cut matters
+  for (int i = 1; i <= nrow; ++i) J[i][npar] = J[i][1] * (1.0 + 1e-9 * (2.0 * urand() - 1.0));
+  for (int i = 1; i <= nrow; ++i) r[i] = 2.0 * urand() - 1.0;
+  printf("nrow %d npar %d lm %.17g\n", nrow, npar, lm);
+  for (int i = 1; i <= nrow; ++i) {
+    printf("J %d", i);
+    for (int p = 1; p <= npar; ++p) printf(" %.17g", J[i][p]);
+    printf(" r %.17g\n", r[i]);
+  }
+  // J^T J and J^T r exactly as atmos/atmos_fit.cc:129-146 forms them
+  fp_t **JT = transpose(J, nrow, npar);
+  fp_t **JTJ = multiply_with_transpose(J, nrow, npar);
+  fp_t *rhs = multiply_vector(JT, r, npar, nrow);
+  for (int i = 1; i <= npar; ++i) {
+    printf("JTJ %d", i);
+    for (int j = 1; j <= npar; ++j) printf(" %.17g", JTJ[i][j]);
+    printf(" rhs %.17g\n", rhs[i]);
+  }
+  for (int i = 1; i <= npar; ++i) JTJ[i][i] *= (lm + 1.0);
+  fp_t **corr = solve_svd(JTJ, &rhs - 1, 1, npar, 1);
+  printf("corr");
+  for (int i = 1; i <= npar; ++i) printf(" %.17g", corr[1][i]);
+  printf("\n");
+  return 0;
+}
+
+static int mode_formal_scalar(int argc, char **argv, io_class &io) {
+  if (argc < 6) return 2;
+  atmosphere *a = make_atmos(argv[2], io);
+  int nlambda;
+  fp_t *lambda = read_lambda(argv[3], nlambda);
+  fp_t el = atof(argv[4]), az = atof(argv[5]);
+  int x1 = a->x1l, x2 = a->x2l;
+  a->set_grid(a->tau_grid);
+  a->boundary_condition_for_rt = -1;
+  a->popsetup();
+  a->compute_op_referent();
+  if (!a->tau_grid) a->compute_tau_referent();
+  a->nltepops();
+  fp_t ***Vr = a->project(a->Vx, a->Vy, a->Vz, el, az, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  fp_t ****Bv = a->transform(a->Bx, a->By, a->Bz, el, az, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  fp_t ***S = ft3dim(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  fp_t ***L = ft3dim(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  printf("nd %d\n", a->x3h - a->x3l + 1);
+  for (int z = a->x3l; z <= a->x3h; ++z) printf("grid %d %.17g\n", z, a->rt_grid[z]);
+  for (int l = 1; l <= nlambda; ++l) {
+    fp_t lv = airtovac(lambda[l]);
+    fp_t ***op = a->opacity(a->T, a->Ne, Vr, a->Vt, Bv, el, az, lv);
+    fp_t ***em = a->emissivity(a->T, a->Ne, Vr, a->Vt, Bv, el, az, lv);
+    if (a->tau_grid) a->normalize_to_referent_opacity(op, em);
+    for (int z = a->x3l; z <= a->x3h; ++z)
+      printf("opem1 %d %d %.17g %.17g\n", l, z, op[x1][x2][z], em[x1][x2][z]);
+    a->formal(a->rt_grid, S, L, op, em, el, az, a->boundary_condition_for_rt);
+    for (int z = a->x3l; z <= a->x3h; ++z)
+      printf("sl %d %d %.17g %.17g\n", l, z, S[x1][x2][z], L[x1][x2][z]);
+    del_ft3dim(op, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    del_ft3dim(em, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  }
+  a->popclean();
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    fprintf(stderr, "usage: ref_driver synth|resp|lmstep|formal_scalar ...\n");
+    return 2;
+  }
+  io_class io;
+  io.reconf(0);
+  if (!strcmp(argv[1], "synth")) return mode_synth(argc, argv, io);
+  if (!strcmp(argv[1], "resp")) return mode_resp(argc, argv, io);
+  if (!strcmp(argv[1], "lmstep")) return mode_lmstep(argc, argv, io);
+  if (!strcmp(argv[1], "formal_scalar")) return mode_formal_scalar(argc, argv, io);
+  fprintf(stderr, "unknown mode %s\n", argv[1]);
+  return 2;
+}

@@ -1,0 +1,285 @@
+"""Writes the golden vectors under tests/golden/ by running the reference SNAPI
+code (built from /root/reference by oracle/ref.mk into oracle/_ref/).
+
+Run here only (the reference is not present where the GPU tests run):
+
+    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py
+
+Inputs are written into a scratch directory in the reference's own config
+formats (atmos{} block, atom{} files, SPINOR 1-D atmosphere): the atom files
+for the background species are read from the reference's cfg/ directory at
+generation time, the two-line Fe I atom is generated here from the line
+parameters of the 6301.5/6302.5 pair (see `fe_atom_cfg`).  Outputs are small
+.npz files of inputs and expected outputs; no reference source is stored.
+"""
+import math
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = os.environ.get("SNAPI_REFERENCE", "/root/reference")
+DRIVER = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+
+# CGS constants as the reference's cmlib/const.h states them.
+C_LIGHT = 2.99792458e10
+H_PLANCK = 6.6260687652e-27
+EV = 1.602189e-12
+M_E = 9.1093818872e-28
+E_CHARGE = 4.8032042510e-10
+
+
+def airtovac(lam_cm):
+    """Air -> vacuum wavelength, the refractive-index formula of
+    cmlib/mathtools.cc:1325-1331 (lambda in cm)."""
+    s = 1e4 / (lam_cm * 1e8)
+    n = 1.0 + 0.00008336624212083 + 0.02408926869968 / (130.1065924522 - s * s) \
+        + 0.0001599740894897 / (38.92568793293 - s * s)
+    return lam_cm * n
+
+
+# The two Fe I-like lines: air wavelength [A], lower-level energy [eV],
+# log gf, (2J, g, Lande g) of lower and upper level.
+LINES = [
+    dict(lam_air=6301.5, ee_low=3.65, loggf=-0.72, low=(4, 5, 1.83), up=(4, 5, 1.5)),
+    dict(lam_air=6302.5, ee_low=3.650001, loggf=-1.24, low=(2, 3, 2.5), up=(0, 1, 0.0)),
+]
+
+
+def fe_atom_cfg():
+    """A four-level Fe I atom in the reference's atom{} format (read by
+    atmos/atmol/atom/atomcfg.cc): two lower levels near 3.65 eV and two upper
+    levels placed so the line centres fall at the given air wavelengths.
+    A_ul follows from log gf = log(g_l f_lu) with
+    f_lu = (g_u/g_l) m_e c lambda^2 A_ul / (8 pi^2 e^2) (atom.cc:245-247).
+    The second lower level sits 1e-6 eV above the first so that no pair of
+    levels is degenerate.  Pairs without a line carry A = 1, the placeholder
+    the reference's own atom files use: atom::rtsetup (atom.cc:1516-1541)
+    needs a nonzero A between consecutive levels, and the bound-bound opacity
+    keeps only A > 1 (atom_rad_synthesis.cc:403).  Partition functions and bound-free data are taken
+    from the reference's cfg/iron_6300.cfg ion blocks."""
+    src = open(os.path.join(REF, "cfg", "iron_6300.cfg")).read()
+    # keep the ion-0 partition block and the ion 1/2 blocks of the template
+    i0 = src.index("partition{")
+    i1 = src.index("level{")
+    part0 = src[i0:i1]
+    k1 = src.index("CHARGE=1")
+    ion1 = src[src.rindex("ion{", 0, k1) :].rstrip()
+    assert ion1.endswith("}")
+    ion1 = ion1[:-1]  # drop the closing brace of atom{}
+    levels = []
+    ups = []
+    for ln in LINES:
+        lam_vac = airtovac(ln["lam_air"] * 1e-8)
+        de = H_PLANCK * C_LIGHT / lam_vac / EV
+        ups.append(ln["ee_low"] + de)
+    lo = [LINES[0], LINES[1]]
+    A = []
+    for ln, eu in zip(LINES, ups):
+        lam = H_PLANCK * C_LIGHT / ((eu - ln["ee_low"]) * EV)
+        gu = ln["up"][1]
+        gf = 10.0 ** ln["loggf"]
+        A.append(gf * 8.0 * math.pi ** 2 * E_CHARGE ** 2 / (M_E * C_LIGHT * lam * lam * gu))
+    bf = "      bf{\n        TYPE=PHY\n        LAMBDA= 100\n        VALUE=%s\n      }\n"
+
+    def level(n, ee, j2, g, gl, lqn, a_list, bfv):
+        s = "    level{\n      N=%d\n      EE=%.9f\n      G=%d\n      J=%d\n      L=%d\n      GL=%.4f\n" % (
+            n, ee, g, j2, lqn, gl)
+        if a_list is not None:
+            s += "      A=" + ", ".join("%.6e" % a for a in a_list) + "\n"
+        return s + bf % bfv + "    }\n"
+
+    levels.append(level(1, lo[0]["ee_low"], lo[0]["low"][0], lo[0]["low"][1], lo[0]["low"][2], 1, None, "0.092E-18"))
+    levels.append(level(2, lo[1]["ee_low"], lo[1]["low"][0], lo[1]["low"][1], lo[1]["low"][2], 1, [1.0], "0.092E-18"))
+    # upper levels in ascending energy: the 6302.5 upper level lies lower
+    levels.append(level(3, ups[1], LINES[1]["up"][0], LINES[1]["up"][1], LINES[1]["up"][2], 0, [1.0, A[1]],
+                        "0.092E-20"))
+    levels.append(level(4, ups[0], LINES[0]["up"][0], LINES[0]["up"][1], LINES[0]["up"][2], 0, [A[0], 1.0, 1.0],
+                        "0.092E-20"))
+    assert ups[1] < ups[0]
+    txt = ("atom{\n  NAME=Iron\n  ID=Fe\n  Z=2\n  MASS=55.854\n  ABUND=3.63e-05\n  NLTE=0\n"
+           "  ion{\n    CHARGE=0\n    EION=7.896\n    " + part0 + "".join(levels) + "  }\n  " + ion1.strip() + "\n}\n")
+    return txt
+
+
+BACKGROUND = ["hydrogen_3lvl.cfg", "helium.cfg", "carbon.cfg", "nitrogen.cfg", "oxygen.cfg", "calcium.cfg",
+              "silicon.cfg", "aluminium.cfg"]
+
+
+def cfg0_atmosphere(nd=61):
+    """The 61-point column: log tau500 uniform on [-6, 1], T(tau) =
+    4400 + 2000 max(0, -log tau - 3.5) + 3000 tau^0.5, B = 1000 G at 45/30 deg,
+    vmicro 1 km/s, vLOS 0.  Pg comes from the reference's hydrostatic
+    integration (SNAPI_REF_HE=1); the Pg/Pe columns here are starting values."""
+    lt = np.linspace(-6.0, 1.0, nd)
+    tau = 10.0 ** lt
+    T = 4400.0 + 2000.0 * np.maximum(0.0, -lt - 3.5) + 3000.0 * np.sqrt(tau)
+    rows = []
+    for i in range(nd):
+        pg = 1e4 * (tau[i] + 1e-3) ** 0.7
+        rows.append((lt[i], 1e7 * (1.0 - lt[i]), T[i], pg, pg * 1e-4, 0.0, 0.0, 1000.0, 1e5, 0.0,
+                     math.radians(45.0), math.radians(30.0)))
+    return rows
+
+
+def write_spinor(path, rows):
+    with open(path, "w") as f:
+        f.write("%d cfg0\n" % len(rows))
+        for r in rows:
+            f.write(" ".join("%.9e" % v for v in r) + "\n")
+
+
+def cfg0_lambda():
+    lam = []
+    for c in (6301.5, 6302.5):
+        lam.extend(np.round(c + np.linspace(-0.6, 0.6, 201), 6))
+    return np.array(lam)
+
+
+def setup_dir(d, rows, lam, he=True):
+    for a in BACKGROUND:
+        shutil.copy(os.path.join(REF, "cfg", a), os.path.join(d, a))
+    shutil.copy(os.path.join(REF, "cfg", "mol.cfg"), os.path.join(d, "mol.cfg"))
+    open(os.path.join(d, "iron_2line.cfg"), "w").write(fe_atom_cfg())
+    with open(os.path.join(d, "atoms.cfg"), "w") as f:
+        for a in BACKGROUND + ["iron_2line.cfg"]:
+            f.write("ATOM=%s\n" % a)
+    with open(os.path.join(d, "atmos.cfg"), "w") as f:
+        f.write("ID=cfg0\nTYPE=SPINOR\nFILE=atmos.dat\nGEOMETRY=PLANEPARALLEL\nRTSOLVER=Bezier\n"
+                "ATOM=atoms.cfg\nMOL=mol.cfg\nELECTRONS=0\nTAU_GRID=1\nLEVEL_POPS=0\n"
+                # dimensions given so the constructor allocates the arrays that
+                # read_spinor's resize() then frees (atmos/atmos.cc:61-73)
+                "NX=1\nNY=1\nNZ=%d\n" % len(rows))
+    write_spinor(os.path.join(d, "atmos.dat"), rows)
+    np.savetxt(os.path.join(d, "lambda.txt"), lam, fmt="%.6f")
+
+
+def run(d, args, he=True):
+    env = dict(os.environ)
+    env["SNAPI_REF_HE"] = "1" if he else "0"
+    out = subprocess.run([DRIVER] + args, cwd=d, env=env, capture_output=True, text=True, check=True)
+    return out.stdout.splitlines()
+
+
+def parse_synth(lines):
+    rec = {"depth": [], "lambda": [], "stokes": [], "stokes_steps": [], "opem": [], "spop": [], "line": [],
+           "zc": [], "lpop": [], "level": [], "species": [], "sdepth": [], "bfop": []}
+    for ln in lines:
+        t = ln.split()
+        if not t:
+            continue
+        k = t[0]
+        if k in rec:
+            rec[k].append(t[1:])
+    return rec
+
+
+def build_cfg0(d):
+    rows = cfg0_atmosphere()
+    lam = cfg0_lambda()
+    setup_dir(d, rows, lam)
+    rec = parse_synth(run(d, ["synth", "atmos.cfg", "lambda.txt", repr(math.pi), "0.0", "20"]))
+    dep = np.array([[float(x) for x in r[1:]] for r in rec["depth"]])
+    # columns: T Ne Nt Vt Vr |B| incl_deg azi_deg op_ref tau_ref rt_grid N_HI height
+    lamv = np.array([[float(r[1]), float(r[2])] for r in rec["lambda"]])
+    stokes = np.array([[float(x) for x in r[1:]] for r in rec["stokes"]])
+    steps = np.array([[float(x) for x in r[1:]] for r in rec["stokes_steps"]])
+    opem = np.array([[float(x) for x in r] for r in rec["opem"]])
+    spop = np.array([[float(x) for x in r] for r in rec["spop"]])
+    sdepth = np.array([[float(x) for x in r] for r in rec["sdepth"]])
+    bfop = np.array([[float(x) for x in r] for r in rec["bfop"]])
+    lines = []
+    for r in rec["line"]:
+        s, z, lo, up = (int(x) for x in r[:4])
+        kv = dict(zip(r[4::2], r[5::2]))
+        lines.append([s, z, lo, up, int(kv["tr"]), float(kv["lam"]), float(kv["A"]), float(kv["Blu"]),
+                      float(kv["Bul"]), float(kv["grad"]), float(kv["sigma"]), float(kv["alpha"]),
+                      int(kv["split"])])
+    zc = [[int(r[0]), int(r[1]), "pbr".index(r[2]), float(r[3]), float(r[4])] for r in rec["zc"]]
+    lpop = np.array([[float(x) for x in r] for r in rec["lpop"]])
+    species = [r for r in rec["species"]]
+    masses = {int(r[0]): float(r[r.index("mass") + 1]) for r in species if r[1] == "atom"}
+    levels = [[int(r[0]), int(r[1]), int(r[2]), float(r[3]), int(r[5]), int(r[7]), int(r[9]), float(r[11])]
+              for r in rec["level"]]
+    np.savez_compressed(
+        os.path.join(HERE, "cfg0_synth.npz"),
+        depth=dep, lam=lamv, stokes=stokes, stokes_steps=steps, opem=opem, spop=spop, sdepth=sdepth, bfop=bfop,
+        lines=np.array([l[:4] + [l[4]] + [0] * 0 for l in lines], dtype=np.int64),
+        line_data=np.array([l[5:12] for l in lines]), line_split=np.array([l[12] for l in lines]),
+        zc=np.array(zc, dtype=np.float64), lpop=lpop,
+        masses=np.array([[k, v] for k, v in sorted(masses.items())]),
+        levels=np.array(levels, dtype=np.float64),
+        atmos_rows=np.array(rows, dtype=np.float32))
+    print("cfg0_synth: nd=%d nlambda=%d lines=%d" % (len(dep), len(lamv), len(lines)))
+    return rows
+
+
+def build_resp(d):
+    rows = cfg0_atmosphere()
+    lam = np.round(6301.5 + np.linspace(-0.3, 0.3, 31), 6)
+    setup_dir(d, rows, lam)
+    lam_vac = np.array([airtovac(l * 1e-8) * 1e8 for l in lam])
+    out = run(d, ["resp", "atmos.cfg", "lambda.txt", repr(math.pi), "0.0"])
+    st, rs = [], []
+    for ln in out:
+        t = ln.split()
+        if t and t[0] == "stokes":
+            st.append([float(x) for x in t[1:]])
+        elif t and t[0] == "resp":
+            rs.append([float(x) for x in t[1:]])
+    rs = np.array(rs)
+    nd = len(rows)
+    nl = len(lam)
+    resp = np.zeros((7, nd, nl, 4))
+    for r in rs:
+        resp[int(r[0]) - 1, int(r[1]) - 1, int(r[2]) - 1, :] = r[3:]
+    np.savez_compressed(os.path.join(HERE, "cfg0_resp.npz"), lam_air=lam, lam_vac=lam_vac, stokes=np.array(st),
+                        resp=resp)
+    print("cfg0_resp: nlambda=%d" % nl)
+
+
+def build_lmstep(d):
+    recs = {}
+    for nrow, npar, lm, seed in ((240, 14, 1e-2, 1), (120, 14, 10.0, 2), (60, 7, 1e3, 3)):
+        out = run(d, ["lmstep", str(nrow), str(npar), repr(lm), str(seed)], he=False)
+        J, r, JTJ, rhs, corr = [], [], [], [], None
+        for ln in out:
+            t = ln.split()
+            if t[0] == "J":
+                J.append([float(x) for x in t[2:2 + npar]])
+                r.append(float(t[-1]))
+            elif t[0] == "JTJ":
+                JTJ.append([float(x) for x in t[2:2 + npar]])
+                rhs.append(float(t[-1]))
+            elif t[0] == "corr":
+                corr = [float(x) for x in t[1:]]
+        key = "s%d" % seed
+        recs[key + "_J"] = np.array(J)
+        recs[key + "_r"] = np.array(r)
+        recs[key + "_JTJ"] = np.array(JTJ)
+        recs[key + "_rhs"] = np.array(rhs)
+        recs[key + "_corr"] = np.array(corr)
+        recs[key + "_lm"] = np.array(lm)
+    np.savez_compressed(os.path.join(HERE, "lmstep.npz"), **recs)
+    print("lmstep: %d cases" % 3)
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        sys.exit("build the reference first: make -f oracle/ref.mk -j8")
+    d = tempfile.mkdtemp(prefix="snapi_golden_")
+    try:
+        build_cfg0(d)
+        build_resp(d)
+        build_lmstep(d)
+    finally:
+        shutil.rmtree(d)
+
+
+if __name__ == "__main__":
+    main()

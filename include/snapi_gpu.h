@@ -1,0 +1,186 @@
+/*
+ * snapi_gpu.h -- C ABI of the MI355X (gfx950) implementation of SNAPI's
+ * data-parallel hot path: Zeeman opacity build, polarized formal solution,
+ * per-depth responses (tangent) and their adjoint, the Levenberg-Marquardt
+ * normal-equation step, and the scalar formal solution with approximate
+ * Lambda operator used by the ALI iteration.
+ *
+ * Every entry point takes plain host pointers and sizes, copies inputs to the
+ * device, runs the HIP kernels and copies results back (the *_dev variants
+ * take device pointers and a hipStream_t given as void* instead and do no
+ * copies).  Return value: SNAPI_OK (0) or a negative snapi_status.
+ *
+ * Reference interfaces replaced (ivanzmilic/snapi):
+ *   snapi_synth_stokes*      atmosphere::obs_stokes                 atmos/atmos_obs.cc:16-87
+ *                            (atmosphere::op_em_vector              atmos/atmos_rad_synthesis.cc:20-72
+ *                             + atmos_ppbez::formal (polarized)     atmos/plpar/atmos_ppbez_rts.cc:918-1218)
+ *   snapi_stokes_responses*  atmosphere::obs_stokes_responses       atmos/atmos_rad_synthesis.cc:166-399
+ *                            (op_em_vector_plus_pert + formal_pert_numerical,
+ *                                                                   atmos/plpar/atmos_ppbez_rts.cc:1780-1857)
+ *   snapi_stokes_adjoint     (no counterpart: gradient J^T r of the chi^2 of
+ *                            atmos/atmos_fit.cc:229-248 in one reverse sweep)
+ *   snapi_lm_step            the damped normal-equation solve of
+ *                            atmosphere::stokes_lm_fit              atmos/atmos_fit.cc:129-160
+ *                            (multiply_with_transpose, solve_svd    cmlib/mathtools.cc:615-624, 344-387)
+ *   snapi_formal_scalar      atmos_ppbez::formal (scalar, with Lambda*)
+ *                                                                   atmos/plpar/atmos_ppbez_rts.cc:21-219
+ *   snapi_ali_accumulate     atom::add (Jbar, Lambda*, norm sums)   atmos/atmol/atom/atom.cc:1865-1927
+ */
+#ifndef SNAPI_GPU_H
+#define SNAPI_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum snapi_status {
+  SNAPI_OK = 0,
+  SNAPI_ERR_ARG = -1,         /* bad sizes or null pointers */
+  SNAPI_ERR_DEVICE = -2,      /* HIP runtime error */
+  SNAPI_ERR_UNSUPPORTED = -3, /* input outside what the kernels implement */
+  SNAPI_ERR_NONFINITE = -4    /* a result was NaN/Inf */
+};
+
+/* Precision of the compute path.
+ * SNAPI_FP64: every operation in double, the reference's arithmetic.
+ * SNAPI_MIXED: opacity build and formal solution in float with Taylor weights
+ *              for small optical-depth steps and compensated accumulation;
+ *              per-wavelength setup and the LM damping solve in double. */
+enum snapi_precision { SNAPI_FP64 = 0, SNAPI_MIXED = 1 };
+
+/*
+ * Column state ("column tile"): the per-depth quantities the reference's
+ * atmosphere object holds after popsetup()/compute_op_referent()/nltepops(),
+ * already projected on the ray (atmosphere::project/transform,
+ * atmos/atmos_rad.cc:15-51).  Layout state[col][field][z], depth index z=0 at
+ * the top of the atmosphere (reference x3l) and z=nd-1 at the bottom (x3h),
+ * nfield = SNAPI_NF_BASE + 2*nline.  Double for SNAPI_FP64, float for
+ * SNAPI_MIXED.
+ */
+enum snapi_field {
+  SNAPI_F_T = 0,     /* temperature [K] */
+  SNAPI_F_VT = 1,    /* microturbulent velocity [cm/s] */
+  SNAPI_F_VLOS = 2,  /* velocity projected on the ray [cm/s] */
+  SNAPI_F_BMAG = 3,  /* |B| [G] */
+  SNAPI_F_BINC = 4,  /* inclination of B to the ray [deg] */
+  SNAPI_F_BAZI = 5,  /* azimuth of B about the ray [deg] */
+  SNAPI_F_NE = 6,    /* electron density [cm^-3] */
+  SNAPI_F_NHI = 7,   /* neutral hydrogen density [cm^-3] (H- and van der Waals damping) */
+  SNAPI_F_OPREF = 8, /* referent continuum opacity at 500 nm [cm^-1] (TAU_GRID=1 normalisation) */
+  SNAPI_F_GRID = 9,  /* RT grid: tau_referent (negative, -10^logtau) or height [cm] */
+  SNAPI_F_BGOP = 10, /* wavelength-independent background opacity (Thomson + atomic bound-free) [cm^-1] */
+  SNAPI_F_BGEM = 11, /* its emissivity */
+  SNAPI_NF_BASE = 12 /* then per line l: SNAPI_NF_BASE+2l lower, +2l+1 upper level population */
+};
+
+#define SNAPI_MAX_LINES 16
+#define SNAPI_MAX_COMPONENTS 96
+#define SNAPI_MAX_ND 256
+#define SNAPI_MAX_NODES 16
+
+/* One bound-bound transition (atom::boundbound_op_em_vector,
+ * atmos/atmol/atom/atom_rad_synthesis.cc:396-505). */
+typedef struct {
+  double lam0;   /* vacuum line centre h c / (E_u - E_l) [cm] */
+  double aul;    /* A_ul [s^-1] */
+  double blu;    /* B_lu in the reference's units (atom.cc:238-243) */
+  double bul;    /* B_ul */
+  double grad;   /* radiative damping: sum of A out of both levels (atom.cc:926-935) */
+  double sigma;  /* van der Waals cross-section (atom::compute_damp_col) */
+  double alpha;  /* its velocity exponent */
+  double mass;   /* atomic mass [g] */
+  int split;     /* 1: Zeeman split (atom.cc:1650-1651); 0: scalar Voigt */
+  int ncomp[3];  /* number of pi, sigma_b, sigma_r components */
+  int comp0;     /* offset of this line's components in the component table */
+} snapi_line;
+
+/* Zeeman components: shift in units of lambda_B = 4.67e-5 lam0^2 |B| and
+ * relative strength (atom::zeeman_setup, atom.cc:1620-1773). */
+typedef struct {
+  double shift;
+  double strength;
+} snapi_component;
+
+typedef struct {
+  int nline;
+  int ncomp_total;
+  snapi_line line[SNAPI_MAX_LINES];
+  snapi_component comp[SNAPI_MAX_COMPONENTS];
+} snapi_line_table;
+
+/* Ray and grid options shared by the synthesis and response entry points. */
+typedef struct {
+  double theta;    /* reference 'el' (pi - theta): pi for mu = 1 */
+  double lam_mid;  /* wavelength of the Thomson emissivity, (lambda[1]+lambda[n])/2 (vacuum) [cm] */
+  int tau_grid;    /* 1: op/em normalised by op_ref and GRID is tau_referent */
+  int hminus;      /* 1: add H- bound-free + free-free (atmos/atmol/mol/mol.cc:379-412) */
+} snapi_ray;
+
+/* Library version / device query. */
+int snapi_version(void);
+int snapi_device_count(void);
+
+/*
+ * Polarized synthesis of a batch of columns on one wavelength grid.
+ *   state   [ncol][nfield][nd] (double for SNAPI_FP64, float for SNAPI_MIXED)
+ *   lam     [nlambda] vacuum wavelengths [cm] (double)
+ *   stokes  [ncol][nlambda][4] emergent I,Q,U,V (double for SNAPI_FP64, float for SNAPI_MIXED)
+ * One work-item per (column, wavelength); the column tile is staged in LDS.
+ */
+int snapi_synth_stokes(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
+                       const void *state, int nlambda, const double *lam, void *stokes);
+int snapi_synth_stokes_dev(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
+                           const void *state_dev, int nlambda, const double *lam_dev, void *stokes_dev,
+                           void *stream);
+
+/*
+ * Per-depth responses d(IQUV)/d q_p(z) of the emergent Stokes vector, the
+ * forward-mode (tangent) derivative of the synthesis above along the seven
+ * physical parameters of the reference: p = 0 T, 1 Nt, 2 vt, 3 vLOS, 4 |B|,
+ * 5 inclination, 6 azimuth (angles per radian).  dstate holds the
+ * thermodynamic derivatives the reference computes by finite differences of
+ * its equation of state (ne_lte_derivatives, level responses,
+ * compute_op_referent_derivative): [ncol][2][nfield][nd], slice 0 = d/dT,
+ * slice 1 = d/dNt, only the fields NE, NHI, OPREF and the populations are read.
+ *   resp [ncol][7][nd][nlambda][4]
+ * Only SNAPI_FP64 is implemented for this entry point.
+ */
+int snapi_stokes_responses(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
+                           const void *state, const void *dstate, int nlambda, const double *lam, void *stokes,
+                           void *resp);
+
+/*
+ * Gradient of chi^2/2 = sum_{l,s} w_s (S_obs - S)^2 / 2 w.r.t. the per-depth
+ * parameters in one reverse sweep (adjoint of the synthesis): the product
+ * J^T r of the per-depth Jacobian above with r = w_s (S - S_obs), without
+ * forming J.
+ *   obs   [ncol][nlambda][4]; wstokes[4]; grad [ncol][7][nd]
+ */
+int snapi_stokes_adjoint(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
+                         const void *state, const void *dstate, int nlambda, const double *lam, const double *obs,
+                         const double *wstokes, double *grad);
+
+/*
+ * One damped normal-equation solve per pixel (batched):
+ *   J [npix][nrow][npar] (row = (l-1)*4 + s as atmos_fit.cc:129-133), r [npix][nrow]
+ *   JTJ[npix][npar][npar], rhs[npix][npar] are written (undamped J^T J, J^T r),
+ *   corr[npix][npar] = pinv(JTJ * diag(1+lm)) rhs with singular values below
+ *   w_max/1e6 dropped (solve_svd semantics), in double.
+ * J^T J is accumulated on MFMA from float J (SNAPI_MIXED) or in double (SNAPI_FP64).
+ */
+int snapi_lm_step(int precision, int npix, int nrow, int npar, const void *J, const void *r, const double *lm,
+                  double *JTJ, double *rhs, double *corr);
+
+/*
+ * Scalar formal solution with the diagonal approximate Lambda operator for a
+ * batch of (column, frequency, ray) problems:
+ *   op, em [nprob][nd], grid [nprob][nd]; I, lstar [nprob][nd]
+ *   ct[nprob] = cos(theta) of each ray; boundary as the reference's (-1: S at the bottom).
+ */
+int snapi_formal_scalar(int nprob, int nd, const double *grid, const double *op, const double *em, const double *ct,
+                        double boundary, double *I, double *lstar);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SNAPI_GPU_H */

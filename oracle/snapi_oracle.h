@@ -1,0 +1,78 @@
+/*
+ * FP64 CPU restatement of SNAPI's hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the oracle the HIP path is checked against.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker or the timed CPU baseline; the product never calls it.
+ *
+ * Each function cites the reference file:line whose semantics it restates.
+ * How each one is pinned (golden vectors from the reference build, closed
+ * forms, invariants) is listed in its comment; functions with no such check
+ * say "parity unpinned".
+ */
+#ifndef SNAPI_ORACLE_H
+#define SNAPI_ORACLE_H
+
+#include "../include/snapi_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Humlicek W4 Voigt/Faraday-Voigt, cmlib/profiles.cc:7-37 (values only).
+ * Pinned: golden op/em of the reference build (every line evaluation goes
+ * through it), symmetry H(-v)=H(v), F(-v)=-F(v), and scipy's wofz to the
+ * published 1e-4 accuracy of W4. */
+void orc_voigt(double v, double a, double *H, double *F);
+
+/* Zeeman pattern of one transition, atom::zeeman_setup atom.cc:1620-1773.
+ * j2u, j2l: 2J of the upper/lower level.  Returns split flag; fills nm[3],
+ * shift[3][nm], str[3][nm] (rows: pi, sigma_b, sigma_r).
+ * Pinned: the reference's dumped pattern (tests/golden/cfg0_synth.npz 'zc'),
+ * strengths summing to 1 per group, and g_eff of the pattern. */
+int orc_zeeman(int j2u, int j2l, double gu, double gl, int nm[3], double shift[3][32], double str[3][32]);
+
+/* Planck function B_lambda, cmlib/mathtools.cc:1150-1152. */
+double orc_planck(double lam, double T);
+
+/* op[nd][4][4], em[nd][4] of one column at one vacuum wavelength:
+ * atmosphere::op_em_vector (atmos/atmos_rad_synthesis.cc:20-72) with the
+ * background of the column state, H- (mol.cc:379-412, 516-545, 609-632), and
+ * every line of the table (atom_rad_synthesis.cc:396-505), then the matrix
+ * symmetrisation of atmos_rad_synthesis.cc:56-68, then (tau_grid) division by
+ * op_ref (atmos/atmos_rad.cc:175-185).  state is one column [nfield][nd].
+ * Pinned: golden 'opem' (reference op/em at every 20th wavelength, all depths)
+ * and 'spop' (H- per wavelength). */
+void orc_opem(const snapi_line_table *lt, const snapi_ray *ray, int nd, const double *state, double lam,
+              int normalise, double *op, double *em);
+
+/* Polarized formal solution atmos_ppbez::formal, plpar/atmos_ppbez_rts.cc:
+ * 918-1218, with the reference's linear source-function weights (the w2 term
+ * vanishes for any step below 1e10) and the last point copied.
+ * op[nd][16] em[nd][4] grid[nd]; S[nd][4] out.  Returns 0, or -3 when a
+ * forward step exceeds the 1e10 switch of the quadratic branch.
+ * Pinned: golden 'stokes' and 'sdepth' (per-depth solution) of the reference
+ * build; constant source function gives I = S; linear source function on a
+ * semi-infinite grid gives Eddington-Barbier exactly. */
+int orc_formal_pol(int nd, const double *grid, const double *op, const double *em, double theta, double boundary,
+                   double *S);
+
+/* Scalar formal solution with diagonal Lambda*, plpar/atmos_ppbez_rts.cc:21-219.
+ * Pinned: golden 'sl' of the reference build; Lambda* equals the local
+ * weight w1; I = S for constant S. */
+int orc_formal_scalar(int nd, const double *grid, const double *op, const double *em, double theta, double boundary,
+                      double *I, double *L);
+
+/* obs_stokes for a batch of columns: state [ncol][nfield][nd], stokes [ncol][nlambda][4]. */
+int orc_synth(const snapi_line_table *lt, const snapi_ray *ray, int ncol, int nd, const double *state, int nlambda,
+              const double *lam, double *stokes, int nthreads);
+
+/* Per-depth responses (see snapi_stokes_responses), FP64 forward mode.
+ * resp [ncol][7][nd][nlambda][4]. */
+int orc_responses(const snapi_line_table *lt, const snapi_ray *ray, int ncol, int nd, const double *state,
+                  const double *dstate, int nlambda, const double *lam, double *stokes, double *resp, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,105 @@
+"""Loader of the FP64 CPU oracle (oracle/liboracle.so).  Test infrastructure:
+used only by tests, the smoke check and the CPU-baseline leg of bench.py."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from snapi_amd.abi import SnapiLineTable, SnapiRay, ptr
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(REPO, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-f", os.path.join(REPO, "oracle", "Makefile")], check=True)
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        L.orc_voigt.argtypes = [C.c_double, C.c_double, dp, dp]
+        L.orc_planck.restype = C.c_double
+        L.orc_planck.argtypes = [C.c_double, C.c_double]
+        L.orc_zeeman.restype = C.c_int
+        L.orc_zeeman.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int),
+                                 C.POINTER((C.c_double * 32) * 3), C.POINTER((C.c_double * 32) * 3)]
+        L.orc_opem.argtypes = [C.POINTER(SnapiLineTable), C.POINTER(SnapiRay), C.c_int, dp, C.c_double, C.c_int,
+                               dp, dp]
+        L.orc_formal_pol.restype = C.c_int
+        L.orc_formal_pol.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp]
+        L.orc_formal_scalar.restype = C.c_int
+        L.orc_formal_scalar.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp]
+        L.orc_synth.restype = C.c_int
+        L.orc_synth.argtypes = [C.POINTER(SnapiLineTable), C.POINTER(SnapiRay), C.c_int, C.c_int, dp, C.c_int, dp,
+                                dp, C.c_int]
+        if hasattr(L, "orc_responses"):
+            L.orc_responses.restype = C.c_int
+            L.orc_responses.argtypes = [C.POINTER(SnapiLineTable), C.POINTER(SnapiRay), C.c_int, C.c_int, dp, dp,
+                                        C.c_int, dp, dp, dp, C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def voigt(v, a):
+    H, F = C.c_double(), C.c_double()
+    lib().orc_voigt(float(v), float(a), C.byref(H), C.byref(F))
+    return H.value, F.value
+
+
+def zeeman(j2u, j2l, gu, gl):
+    nm = (C.c_int * 3)()
+    sh = ((C.c_double * 32) * 3)()
+    st = ((C.c_double * 32) * 3)()
+    split = lib().orc_zeeman(j2u, j2l, gu, gl, nm, C.byref(sh), C.byref(st))
+    return split, [[(sh[g][i], st[g][i]) for i in range(nm[g])] for g in range(3)]
+
+
+def opem(table, ray, state_col, lam, normalise=True):
+    nd = state_col.shape[1]
+    op = np.zeros((nd, 4, 4))
+    em = np.zeros((nd, 4))
+    lib().orc_opem(C.byref(table), C.byref(ray), nd, ptr(np.ascontiguousarray(state_col)), float(lam),
+                   int(normalise), ptr(op), ptr(em))
+    return op, em
+
+
+def formal_pol(grid, op, em, theta, boundary=-1.0):
+    nd = len(grid)
+    S = np.zeros((nd, 4))
+    rc = lib().orc_formal_pol(nd, ptr(np.ascontiguousarray(grid, dtype=np.float64)),
+                              ptr(np.ascontiguousarray(op)), ptr(np.ascontiguousarray(em)), theta, boundary, ptr(S))
+    return S, rc
+
+
+def formal_scalar(grid, op, em, theta, boundary=-1.0):
+    nd = len(grid)
+    I = np.zeros(nd)
+    Ls = np.zeros(nd)
+    rc = lib().orc_formal_scalar(nd, ptr(np.ascontiguousarray(grid, dtype=np.float64)),
+                                 ptr(np.ascontiguousarray(op, dtype=np.float64)),
+                                 ptr(np.ascontiguousarray(em, dtype=np.float64)), theta, boundary, ptr(I), ptr(Ls))
+    return I, Ls, rc
+
+
+def synth(table, ray, state, lam, nthreads=0):
+    ncol, nf, nd = state.shape
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    out = np.zeros((ncol, len(lam), 4))
+    rc = lib().orc_synth(C.byref(table), C.byref(ray), ncol, nd, ptr(st), len(lam), ptr(lam), ptr(out), nthreads)
+    return out, rc
+
+
+def responses(table, ray, state, dstate, lam, nthreads=0):
+    ncol, nf, nd = state.shape
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    ds = np.ascontiguousarray(dstate, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    out = np.zeros((ncol, len(lam), 4))
+    resp = np.zeros((ncol, 7, nd, len(lam), 4))
+    rc = lib().orc_responses(C.byref(table), C.byref(ray), ncol, nd, ptr(st), ptr(ds), len(lam), ptr(lam), ptr(out),
+                             ptr(resp), nthreads)
+    return out, resp, rc

@@ -1,0 +1,144 @@
+"""The FP64 oracle against the reference's golden vectors and against closed
+forms.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import _oracle
+from snapi_amd import abi
+
+
+def test_synthesis_matches_reference_golden(cfg0):
+    out, rc = _oracle.synth(cfg0["table"], cfg0["ray"], cfg0["state"], cfg0["lam_vac"])
+    assert rc == 0
+    ref = cfg0["stokes"]
+    ic = ref[:, 0].max()
+    # FP64 restatement of the same arithmetic: agreement to rounding
+    assert np.max(np.abs(out[0] - ref)) / ic < 1e-12
+    assert np.max(np.abs(out[0, :, 0] - ref[:, 0]) / ref[:, 0]) < 1e-12
+
+
+def test_opacity_matches_reference_golden(cfg0):
+    worst = 0.0
+    for row in cfg0["opem"]:
+        z = int(row[1]) - 1
+        op, em = _oracle.opem(cfg0["table"], cfg0["ray"], cfg0["state"][0], row[2], normalise=False)
+        rop = row[3:19].reshape(4, 4)
+        rem = row[19:23]
+        worst = max(worst, np.abs(op[z] - rop).max() / abs(rop[0, 0]), np.abs(em[z] - rem).max() / abs(rem[0]))
+    assert worst < 1e-12
+
+
+def test_depth_solution_matches_reference(cfg0):
+    """Per-depth Stokes of the formal solver at a few wavelengths."""
+    sd = cfg0["sdepth"]
+    for l in np.unique(sd[:, 0]).astype(int):
+        rows = sd[sd[:, 0] == l]
+        op, em = _oracle.opem(cfg0["table"], cfg0["ray"], cfg0["state"][0], cfg0["lam_vac"][l - 1])
+        S, rc = _oracle.formal_pol(cfg0["state"][0, abi.F_GRID], op, em, math.pi)
+        ref = rows[:, 2:6]
+        assert np.max(np.abs(S - ref)) / np.max(np.abs(ref[:, 0])) < 1e-12
+
+
+def test_zeeman_pattern_matches_reference(cfg0):
+    # 6301.5: J=2 -> J=2, g_l=1.83, g_u=1.5 ; 6302.5: J=0 upper, J=1 lower
+    for line in cfg0["lines"]:
+        if not line["split"]:
+            continue
+        if abs(line["lam0"] - 6.30329e-5) < 1e-9:
+            split, comps = _oracle.zeeman(4, 4, 1.5, 1.83)
+        else:
+            split, comps = _oracle.zeeman(0, 2, 0.0, 2.5)
+        assert split == 1
+        for grp in range(3):
+            mine = np.array(comps[grp])
+            ref = np.array(line["comps"][grp])
+            assert mine.shape == ref.shape
+            assert np.allclose(mine, ref, rtol=0, atol=1e-14)
+            assert abs(mine[:, 1].sum() - 1.0) < 1e-14
+
+
+def test_zeeman_effective_lande_factor():
+    """g_eff = sum over sigma_r of strength*shift for a normal-like pattern
+    equals the textbook value 0.5(g_u+g_l) + 0.25(g_u-g_l)(J_u(J_u+1)-J_l(J_l+1))."""
+    for (j2u, j2l, gu, gl) in ((4, 4, 1.5, 1.83), (0, 2, 0.0, 2.5), (2, 4, 1.1, 0.9), (6, 4, 1.2, 1.4)):
+        split, comps = _oracle.zeeman(j2u, j2l, gu, gl)
+        ju, jl = j2u / 2, j2l / 2
+        geff = 0.5 * (gu + gl) + 0.25 * (gu - gl) * (ju * (ju + 1) - jl * (jl + 1))
+        r = np.array(comps[2])
+        b = np.array(comps[1])
+        assert abs((r[:, 0] * r[:, 1]).sum() - geff) < 1e-12
+        assert abs((b[:, 0] * b[:, 1]).sum() + geff) < 1e-12
+
+
+def test_voigt_symmetry_and_limits():
+    for a in (1e-4, 0.01, 0.1, 0.5, 2.0):
+        for v in (0.0, 0.3, 1.0, 2.5, 4.0, 6.0, 10.0, 20.0):
+            h1, f1 = _oracle.voigt(v, a)
+            h2, f2 = _oracle.voigt(-v, a)
+            assert abs(h1 - h2) <= 1e-15 * max(1.0, abs(h1))
+            assert abs(f1 + f2) <= 1e-15 * max(1.0, abs(f1))
+    # a -> 0: H -> exp(-v^2)/sqrt(pi) (normalised profile)
+    for v in (0.0, 0.5, 1.0, 2.0):
+        h, _ = _oracle.voigt(v, 1e-8)
+        assert abs(h - math.exp(-v * v) / math.sqrt(math.pi)) < 1e-4
+
+
+def test_voigt_against_faddeeva():
+    """W4 is accurate to ~1e-4 of the Faddeeva function (Humlicek 1982)."""
+    sp = pytest.importorskip("scipy.special")
+    worst = 0.0
+    for a in (1e-3, 0.03, 0.3, 1.0):
+        for v in np.linspace(-12, 12, 241):
+            h, f = _oracle.voigt(v, a)
+            w = sp.wofz(complex(v, a)) / math.sqrt(math.pi)
+            worst = max(worst, abs(h - w.real), abs(f - w.imag))
+    assert worst < 1e-4
+
+
+def test_formal_constant_source():
+    """Constant source function, unpolarised: I = S at every depth."""
+    nd = 40
+    grid = -np.logspace(-5, 1, nd)
+    op = np.zeros((nd, 4, 4))
+    em = np.zeros((nd, 4))
+    for z in range(nd):
+        op[z] = np.eye(4) * (1.0 + 0.5 * z)
+        em[z, 0] = 3.0 * op[z, 0, 0]
+    S, rc = _oracle.formal_pol(grid, op, em, math.pi)
+    assert rc == 0
+    assert np.allclose(S[:, 0], 3.0, rtol=1e-12)
+    assert np.allclose(S[:, 1:], 0.0)
+
+
+def test_formal_linear_source_is_exact():
+    """Linear S(tau) = a + b tau with unit opacity on a tau grid: the linear
+    short-characteristics weights are exact, so I at the top point solved
+    equals a + b tau_1 + b (the boundary term decays as e^-tau_max)."""
+    nd = 60
+    tau = np.concatenate([[0.0], np.logspace(-4, 2, nd - 1)])
+    grid = -tau
+    a, b = 1.0, 2.0
+    op = np.zeros((nd, 4, 4))
+    em = np.zeros((nd, 4))
+    for z in range(nd):
+        op[z] = np.eye(4)
+        em[z, 0] = a + b * tau[z]
+    S, _ = _oracle.formal_pol(grid, op, em, math.pi)
+    # point 1 is the last one solved; I(tau) = a + b tau + b for a semi-infinite medium
+    expect = a + b * tau[1] + b
+    assert abs(S[1, 0] - expect) / expect < 1e-9
+
+
+def test_scalar_formal_lambda_operator():
+    nd = 30
+    tau = np.logspace(-4, 1.5, nd)
+    grid = -tau
+    op = np.ones(nd)
+    em = np.full(nd, 2.0)
+    I, L, rc = _oracle.formal_scalar(grid, op, em, math.pi)
+    assert np.allclose(I, 2.0, rtol=1e-12)
+    # Lambda* = w1 of the step arriving at each point; 1 at the bottom boundary
+    assert L[nd - 1] == 1.0
+    assert np.all((L[:-1] > 0) & (L[:-1] < 1))

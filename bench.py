@@ -1,0 +1,152 @@
+"""Throughput of the HIP hot path on one MI355X.
+
+Workload (one "step"): polarized LTE Stokes synthesis of the 512 x 512
+column cube, 71 depths, 301 wavelengths, mu = 1, two Fe I Zeeman lines plus
+the hydrogen lines and continuum of the reference atom set, in the
+mixed-precision path; inputs resident in HBM before the timed region.
+Value: lambda-point formal solves per second (column x wavelength x ray).
+
+Also reported: the FP64 GPU path on the same cube, the accuracy of the
+mixed path against the FP64 CPU oracle on a sample of columns, the roofline
+fraction of the synthesis kernel, and the oracle's own throughput on the
+host cores (cpu_baseline, FP64 port, OpenMP).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "lambda-point formal solves/s + LM pixel-iter/s at <=1e-4 Ic vs FP64 oracle"
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    sys.stderr.write("[bench] %s\n" % msg)
+    sys.stderr.flush()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--cpu-sample", type=int, default=4096, help="columns timed on the CPU oracle")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the hot path is per column and shards by column tiles; "
+              "runs on more than one GPU are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from snapi_amd import abi, api, flops, workloads
+
+    if not torch.cuda.is_available():
+        sys.exit("no GPU visible")
+    dev = torch.device("cuda:0")
+    tmpl = workloads.template_cfg1()
+    table, ray, lam = tmpl["table"], tmpl["ray"], tmpl["lam_vac"]
+    nx, ny = args.nx, args.ny
+    ncol = nx * ny
+    nlam = len(lam)
+    t0 = time.time()
+    state32 = workloads.cube_state(tmpl, nx, ny, seed=1, dtype=np.float32)
+    nf, nd = state32.shape[1], state32.shape[2]
+    log("cube %dx%d nd=%d nfield=%d nlambda=%d built in %.1fs" % (nx, ny, nd, nf, nlam, time.time() - t0))
+
+    st_d = torch.from_numpy(state32).to(dev)
+    lam_d = torch.from_numpy(np.ascontiguousarray(lam)).to(dev)
+    out_d = torch.empty((ncol, nlam, 4), dtype=torch.float32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+
+    def run_steps(n, state_t, out_t, prec):
+        for _ in range(n):
+            api.synth_stokes_dev(table, ray, state_t, lam_d, out_t, prec, stream)
+
+    def timed(state_t, out_t, prec, steps, warmup):
+        with torch.cuda.stream(stream):
+            run_steps(warmup, state_t, out_t, prec)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        with torch.cuda.stream(stream):
+            run_steps(steps, state_t, out_t, prec)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps  # ms per step = per launch
+
+    ms = timed(st_d, out_d, abi.SNAPI_MIXED, args.steps, args.warmup)
+    value = ncol * nlam / (ms * 1e-3)
+    log("mixed: %.3f ms/step, %.4g lambda-point solves/s" % (ms, value))
+
+    # algorithmic flops per lambda point on a sample of the cube
+    rng = np.random.default_rng(5)
+    samp = rng.choice(ncol, 64, replace=False)
+    fpl = flops.flops_per_lambda_point(table, ray, state32[samp].astype(np.float64), lam)
+    launch_flops = fpl * ncol * nlam
+    achieved = launch_flops / (ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "synth_kernel<float>", "flops_per_lambda_point": round(fpl, 1),
+            "ms_per_launch": round(ms, 4)}
+
+    fp64 = None
+    if not args.no_fp64:
+        st64_d = st_d.to(torch.float64)
+        out64_d = torch.empty((ncol, nlam, 4), dtype=torch.float64, device=dev)
+        ms64 = timed(st64_d, out64_d, abi.SNAPI_FP64, max(1, args.steps // 2), 1)
+        a64 = launch_flops / (ms64 * 1e-3) / 1e12
+        fp64 = {"value": ncol * nlam / (ms64 * 1e-3), "ms_per_step": round(ms64, 4),
+                "achieved_tflops": round(a64, 3), "frac_fp64_peak": round(a64 / FP64_PEAK_TFLOPS, 4)}
+        log("fp64: %.3f ms/step" % ms64)
+        del st64_d, out64_d
+
+    gpu_out = out_d.cpu().numpy()
+    cpu = None
+    acc = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import _oracle  # the FP64 CPU restatement: baseline and checker only
+        nthreads = max(1, min(16, os.cpu_count() or 1))
+        ns = min(args.cpu_sample, ncol)
+        idx = np.arange(ns)
+        st64 = state32[idx].astype(np.float64)
+        t0 = time.time()
+        ref, rc = _oracle.synth(table, ray, st64, lam, nthreads=nthreads)
+        dt = time.time() - t0
+        cpu = {"value": ns * nlam / dt, "unit": "lambda-point formal solves/s", "cores": nthreads, "kind": "port",
+               "sample": "%d columns x %d wavelengths of the same cube (FP64 oracle, OpenMP)" % (ns, nlam),
+               "seconds": round(dt, 2)}
+        ic = float(np.max(ref[..., 0]))
+        err = np.abs(gpu_out[idx].astype(np.float64) - ref) / ic
+        acc = {"max_err_ic": float(err.max()), "mean_err_ic": float(err.mean()), "columns": int(ns),
+               "reference": "FP64 CPU oracle"}
+        log("cpu oracle: %.4g lambda-points/s on %d threads; mixed max err %.3e Ic" % (cpu["value"], nthreads,
+                                                                                      err.max()))
+
+    line = {"metric": METRIC, "value": value, "unit": "lambda-point formal solves/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "LTE polarized Stokes synthesis, cube %dx%d columns x %d depths, %d wavelengths, "
+                                   "mu=1, Fe I 6301.5/6302.5 + H lines + H- continuum, mixed FP32/FP64"
+                                   % (nx, ny, nd, nlam),
+                       "columns": ncol, "depths": nd, "wavelengths": nlam, "lines": table.nline},
+            "roofline": roof, "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

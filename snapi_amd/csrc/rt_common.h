@@ -1,0 +1,212 @@
+// Device building blocks shared by the synthesis, response and adjoint
+// kernels: Humlicek W4 Voigt/Faraday-Voigt, short-characteristics weights,
+// the Fritsch-Butland opacity derivative and the pivot-free 4x4 solve.
+// Everything is templated on the arithmetic type R (double or float).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/snapi_gpu.h"
+
+namespace snapi {
+
+constexpr double kC = 2.99792458E10;        // cm/s      (cmlib/const.h:11)
+constexpr double kH = 6.6260687652E-27;     // erg s     (cmlib/const.h:12)
+constexpr double kK = 1.3806503E-16;        // erg/K     (cmlib/const.h:13)
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kSqrtPi = 1.77245386399759;  // as cmlib/profiles.cc:33
+constexpr double kThomson = 6.65E-25;
+
+template <typename R> __device__ __forceinline__ R rexp(R x);
+template <> __device__ __forceinline__ double rexp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float rexp<float>(float x) { return expf(x); }
+template <typename R> __device__ __forceinline__ R rsin(R x);
+template <> __device__ __forceinline__ double rsin<double>(double x) { return sin(x); }
+template <> __device__ __forceinline__ float rsin<float>(float x) { return sinf(x); }
+template <typename R> __device__ __forceinline__ R rsqrt_(R x);
+template <> __device__ __forceinline__ double rsqrt_<double>(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ float rsqrt_<float>(float x) { return sqrtf(x); }
+template <typename R> __device__ __forceinline__ R rexp2(R x);
+template <> __device__ __forceinline__ double rexp2<double>(double x) { return exp2(x); }
+template <> __device__ __forceinline__ float rexp2<float>(float x) { return exp2f(x); }
+
+template <typename R> struct cplx {
+  R re, im;
+};
+template <typename R> __device__ __forceinline__ cplx<R> cm(R a, R b) { return {a, b}; }
+template <typename R> __device__ __forceinline__ cplx<R> mul(cplx<R> a, cplx<R> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R> __device__ __forceinline__ cplx<R> addr(cplx<R> a, R s) { return {a.re + s, a.im}; }
+template <typename R> __device__ __forceinline__ cplx<R> rsub(R s, cplx<R> a) { return {s - a.re, -a.im}; }
+template <typename R> __device__ __forceinline__ cplx<R> scl(cplx<R> a, R s) { return {a.re * s, a.im * s}; }
+template <typename R> __device__ __forceinline__ cplx<R> cdiv(cplx<R> a, cplx<R> b) {
+  R d = R(1) / (b.re * b.re + b.im * b.im);
+  return {(a.re * b.re + a.im * b.im) * d, (a.im * b.re - a.re * b.im) * d};
+}
+
+// Humlicek (1982) W4 rational approximation of w(z), z = v + i a, in its four
+// regions; H = Re w / sqrt(pi), F = Im w / sqrt(pi).  Region 4 takes
+// cos(Im u) as +sqrt(1 - sin^2(Im u)), the form the reference evaluates
+// (cmlib/profiles.cc:7-37).  The region tests are uniform per lane.
+template <typename R> __device__ __forceinline__ void voigt(R v, R a, R &H, R &F) {
+  const R av = v < R(0) ? -v : v;
+  const R s = av + a;
+  const cplx<R> t = cm<R>(a, -v);
+  const cplx<R> u = cm<R>(a * a - v * v, R(-2) * a * v);
+  cplx<R> w;
+  if (s >= R(15)) {
+    w = cdiv(scl(t, R(0.5641896)), addr(u, R(0.5)));
+  } else if (s > R(5.5)) {
+    cplx<R> num = mul(t, addr(scl(u, R(0.5641896)), R(1.410474)));
+    cplx<R> den = addr(mul(u, addr(u, R(3.0))), R(0.75));
+    w = cdiv(num, den);
+  } else if (a > (R(0.195) * av - R(0.176))) {
+    cplx<R> num = addr(scl(t, R(0.5642236)), R(3.778987));
+    num = addr(mul(t, num), R(11.96482));
+    num = addr(mul(t, num), R(20.20933));
+    num = addr(mul(t, num), R(16.4955));
+    cplx<R> den = addr(t, R(6.699398));
+    den = addr(mul(t, den), R(21.69274));
+    den = addr(mul(t, den), R(39.27121));
+    den = addr(mul(t, den), R(38.82363));
+    den = addr(mul(t, den), R(16.4955));
+    w = cdiv(num, den);
+  } else {
+    cplx<R> p = rsub(R(1.320522), scl(u, R(0.56419)));
+    p = rsub(R(35.76683), mul(u, p));
+    p = rsub(R(219.0313), mul(u, p));
+    p = rsub(R(1540.787), mul(u, p));
+    p = rsub(R(3321.9905), mul(u, p));
+    p = rsub(R(36183.31), mul(u, p));
+    cplx<R> w4 = mul(t, p);
+    cplx<R> q = rsub(R(1.841439), u);
+    q = rsub(R(61.57037), mul(u, q));
+    q = rsub(R(364.2191), mul(u, q));
+    q = rsub(R(2186.181), mul(u, q));
+    q = rsub(R(9022.228), mul(u, q));
+    q = rsub(R(24322.84), mul(u, q));
+    q = rsub(R(32066.6), mul(u, q));
+    const R su = rsin<R>(u.im);
+    const R e = rexp<R>(u.re);
+    cplx<R> r = cdiv(w4, q);
+    w = cm<R>(e * rsqrt_<R>(R(1) - su * su) - r.re, e * su - r.im);
+  }
+  H = w.re * R(1.0 / kSqrtPi);
+  F = w.im * R(1.0 / kSqrtPi);
+}
+
+// Linear short-characteristics weights with the Taylor branch below
+// d tau = 1e-2 (plpar/atmos_ppbez_rts.cc:1067-1096).
+template <typename R> __device__ __forceinline__ void sc_weights(R tb, R &etb, R &w0, R &w1) {
+  if (tb < R(1E-2)) {
+    etb = R(1) - tb + tb * tb * R(0.5) - tb * tb * tb * R(1.0 / 6.0);
+    w0 = tb * (R(0.5) - tb * (R(1.0 / 3.0) - tb * R(0.125)));
+    w1 = R(0.5) * tb * (R(1) - tb * R(1.0 / 3.0) * (R(1) - tb * R(0.25)));
+  } else {
+    etb = rexp<R>(-tb);
+    const R itb = R(1) / tb;
+    w0 = itb - etb * (R(1) + itb);
+    w1 = R(1) - itb + etb * itb;
+  }
+}
+
+// Fritsch-Butland monotone derivative at an interior point
+// (atmos_ppbez_rts.cc:966-979).
+template <typename R> __device__ __forceinline__ R fb_deriv(R om, R o0, R op, R h1, R h2) {
+  const R d2 = (op - o0) / h2;
+  const R d1 = (o0 - om) / h1;
+  if (d1 * d2 > R(0)) {
+    const R al = (R(1) + h1 / (h1 + h2)) * R(1.0 / 3.0);
+    return d1 * d2 / (al * d1 + (R(1) - al) * d2);
+  }
+  return R(0);
+}
+
+// Reduced propagation matrix K/eta_I - 1 of the Stokes vector equation:
+//   [ 0   a   b   c ]   a,b,c = eta_Q,U,V / eta_I
+//   [ a   0   d  -e ]   d = rho_V / eta_I  (reference K[2][3])
+//   [ b  -d   0   f ]   e = rho_U / eta_I  (reference K[4][2])
+//   [ c   e  -f   0 ]   f = rho_Q / eta_I  (reference K[3][4])
+template <typename R> struct Kred {
+  R a, b, c, d, e, f;
+};
+
+// y = Kred * x
+template <typename R> __device__ __forceinline__ void kmul(const Kred<R> &k, const R x[4], R y[4]) {
+  y[0] = k.a * x[1] + k.b * x[2] + k.c * x[3];
+  y[1] = k.a * x[0] + k.d * x[2] - k.e * x[3];
+  y[2] = k.b * x[0] - k.d * x[1] + k.f * x[3];
+  y[3] = k.c * x[0] + k.e * x[1] - k.f * x[2];
+}
+
+// Solve (1 + w1 Kred) x = b with the pivot-free Crout factorisation of
+// cmlib/mathtools.cc:487-514 (unit upper, diagonal kept in L).
+template <typename R> __device__ __forceinline__ void solve_1pk(const Kred<R> &k, R w1, const R b[4], R x[4]) {
+  R A[4][4];
+  A[0][0] = R(1);       A[0][1] = w1 * k.a;   A[0][2] = w1 * k.b;   A[0][3] = w1 * k.c;
+  A[1][0] = w1 * k.a;   A[1][1] = R(1);       A[1][2] = w1 * k.d;   A[1][3] = -w1 * k.e;
+  A[2][0] = w1 * k.b;   A[2][1] = -w1 * k.d;  A[2][2] = R(1);       A[2][3] = w1 * k.f;
+  A[3][0] = w1 * k.c;   A[3][1] = w1 * k.e;   A[3][2] = -w1 * k.f;  A[3][3] = R(1);
+  R D[4][4];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+    for (int i = kk; i < 4; ++i) {
+      R sum = R(0);
+#pragma unroll
+      for (int p = 0; p < kk; ++p) sum += D[i][p] * D[p][kk];
+      D[i][kk] = A[i][kk] - sum;
+    }
+    const R inv = R(1) / D[kk][kk];
+#pragma unroll
+    for (int j = kk + 1; j < 4; ++j) {
+      R sum = R(0);
+#pragma unroll
+      for (int p = 0; p < kk; ++p) sum += D[kk][p] * D[p][j];
+      D[kk][j] = (A[kk][j] - sum) * inv;
+    }
+  }
+  R y[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    R sum = R(0);
+#pragma unroll
+    for (int p = 0; p < i; ++p) sum += D[i][p] * y[p];
+    y[i] = (b[i] - sum) / D[i][i];
+  }
+#pragma unroll
+  for (int i = 3; i >= 0; --i) {
+    R sum = R(0);
+#pragma unroll
+    for (int p = i + 1; p < 4; ++p) sum += D[i][p] * x[p];
+    x[i] = y[i] - sum;
+  }
+}
+
+// Line constants passed by value as a kernel argument (a few KB).
+struct KLine {
+  double lam0;      // vacuum line centre [cm]
+  double opf_l;     // B_lu h c / (4 pi lam0)
+  double opf_u;     // B_ul h c / (4 pi lam0)
+  double emf;       // A_ul h c / (4 pi lam0)
+  double sf;        // lam0^2 / (4 pi c)
+  double grad;      // radiative damping
+  double sigma;     // van der Waals cross-section
+  double alpha;     // its velocity exponent
+  double twok_m;    // 2 k / mass
+  double vmfac;     // 8 k / (pi mu), mu = 1.66e-24 / (1/1.008 + 1.66e-24/mass)
+  double lbfac;     // 4.67e-5 lam0^2  (lambda_B per Gauss)
+  int split;
+  int nc[3];
+  int c0;
+};
+struct KLines {
+  int nline;
+  int ncomp;
+  KLine l[SNAPI_MAX_LINES];
+  double shift[SNAPI_MAX_COMPONENTS];
+  double str[SNAPI_MAX_COMPONENTS];
+};
+
+}  // namespace snapi

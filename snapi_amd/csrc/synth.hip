@@ -1,0 +1,110 @@
+// Polarized synthesis on gfx950: Zeeman opacity build fused with the
+// Bezier (linear) short-characteristics formal solution, one work-item per
+// (column, wavelength), the depth recurrence kept in registers and the
+// column's per-depth quantities staged once in LDS.
+//
+// Restates atmosphere::obs_stokes (atmos/atmos_obs.cc:16-87):
+//   op_em_vector          atmos/atmos_rad_synthesis.cc:20-72
+//   boundbound_op_em_vector atmos/atmol/atom/atom_rad_synthesis.cc:396-505
+//   h_minus_mol::op_em_vector atmos/atmol/mol/mol.cc:379-412
+//   normalize_to_referent_opacity atmos/atmos_rad.cc:175-185
+//   atmos_ppbez::formal (polarized) atmos/plpar/atmos_ppbez_rts.cc:918-1218
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "rt_common.h"
+#include "synth_kernel.h"
+
+namespace snapi {
+
+template <typename R>
+__global__ __launch_bounds__(kSynthThreads) void synth_kernel(const KLines lines, const SynthArgs args,
+                                                             const R *__restrict__ state,
+                                                             const double *__restrict__ lam,
+                                                             R *__restrict__ stokes, int *__restrict__ status) {
+  extern __shared__ unsigned char smem_raw[];
+  R *sm = reinterpret_cast<R *>(smem_raw);
+  const int nd = args.nd;
+  const int col = blockIdx.x;
+  const R *st = state + (size_t)col * args.nfield * nd;
+  ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
+  stage_column<R, R>(lines, args, st, tile);
+  __syncthreads();
+  for (int l = threadIdx.x; l < args.nlambda; l += blockDim.x) {
+    R out[4];
+    int rc = solve_lambda<R>(lines, args, tile, lam[l], out);
+    R *o = stokes + ((size_t)col * args.nlambda + l) * 4;
+    o[0] = out[0];
+    o[1] = out[1];
+    o[2] = out[2];
+    o[3] = out[3];
+    if (rc) atomicOr(status, rc);
+  }
+}
+
+template <typename R>
+static int launch_synth(const KLines &kl, const SynthArgs &a, const void *state, const double *lam, void *stokes,
+                        int *status, hipStream_t stream) {
+  int threads = ((a.nlambda + 63) / 64) * 64;
+  if (threads > kSynthThreads) threads = kSynthThreads;
+  size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline);
+  hipLaunchKernelGGL(synth_kernel<R>, dim3(a.ncol), dim3(threads), smem, stream, kl, a,
+                     static_cast<const R *>(state), lam, static_cast<R *>(stokes), status);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+}  // namespace snapi
+
+using namespace snapi;
+
+extern "C" int snapi_synth_stokes_dev(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol,
+                                      int nd, const void *state_dev, int nlambda, const double *lam_dev,
+                                      void *stokes_dev, void *stream) {
+  KLines kl;
+  SynthArgs a;
+  int rc = prepare_synth(lines, ray, ncol, nd, nlambda, kl, a);
+  if (rc) return rc;
+  if (!state_dev || !lam_dev || !stokes_dev) return SNAPI_ERR_ARG;
+  int *status = device_status_slot();
+  if (!status) return SNAPI_ERR_DEVICE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == SNAPI_FP64)
+    return launch_synth<double>(kl, a, state_dev, lam_dev, stokes_dev, status, s);
+  if (precision == SNAPI_MIXED)
+    return launch_synth<float>(kl, a, state_dev, lam_dev, stokes_dev, status, s);
+  return SNAPI_ERR_ARG;
+}
+
+extern "C" int snapi_synth_stokes(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol,
+                                  int nd, const void *state, int nlambda, const double *lam, void *stokes) {
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
+  if (!state || !lam || !stokes || !lines || !ray || ncol <= 0 || nd < 2 || nlambda <= 0) return SNAPI_ERR_ARG;
+  const size_t esz = precision == SNAPI_FP64 ? sizeof(double) : sizeof(float);
+  const size_t nf = SNAPI_NF_BASE + 2 * (size_t)lines->nline;
+  const size_t sbytes = (size_t)ncol * nf * nd * esz;
+  const size_t obytes = (size_t)ncol * nlambda * 4 * esz;
+  void *ds = nullptr, *dout = nullptr;
+  double *dlam = nullptr;
+  int rc = SNAPI_OK;
+  if (hipMalloc(&ds, sbytes) != hipSuccess || hipMalloc(&dout, obytes) != hipSuccess ||
+      hipMalloc((void **)&dlam, nlambda * sizeof(double)) != hipSuccess) {
+    rc = SNAPI_ERR_DEVICE;
+  } else {
+    hipMemcpy(ds, state, sbytes, hipMemcpyHostToDevice);
+    hipMemcpy(dlam, lam, nlambda * sizeof(double), hipMemcpyHostToDevice);
+    reset_status();
+    rc = snapi_synth_stokes_dev(precision, lines, ray, ncol, nd, ds, nlambda, dlam, dout, nullptr);
+    if (rc == SNAPI_OK) {
+      if (hipDeviceSynchronize() != hipSuccess) rc = SNAPI_ERR_DEVICE;
+      else if (hipMemcpy(stokes, dout, obytes, hipMemcpyDeviceToHost) != hipSuccess) rc = SNAPI_ERR_DEVICE;
+      else rc = read_status();
+    }
+  }
+  if (ds) hipFree(ds);
+  if (dout) hipFree(dout);
+  if (dlam) hipFree(dlam);
+  return rc;
+}

@@ -1,0 +1,154 @@
+"""Column state from reference column dumps, and the synthetic cube workload.
+
+`column_from_dump` turns a column recorded from the reference code (the
+tests/golden/*.npz files: the state its atmosphere object holds after
+popsetup/compute_op_referent/nltepops) into this library's inputs: the line
+table, the column state [nfield][nd] and the wavelength grid.  This is the
+conversion a reference-side caller performs before calling the C ABI
+(atmos/atmos_obs.cc:23-58 computes the same quantities).
+
+`cube_state` draws the 512 x 512 x 71 LTE cube workload around the 71-depth
+template column: per-column temperature offset from a Gaussian random field
+(sigma 150 K, correlation length 8 px), |B| log-uniform on 10-3000 G,
+cos(inclination) uniform on [-1, 1], azimuth uniform on [0, 180) deg, vLOS
+N(0, 2 km/s) plus a gradient N(0, 0.5 km/s per decade of tau), vmicro
+uniform on [0, 2] km/s.  The thermodynamic fields of each column (electron
+and neutral-hydrogen densities, level populations, referent opacity,
+background) are the template's, rescaled with Boltzmann/Saha-type factors
+for the column's temperature: synthetic inputs of the right shape and
+magnitude, not a solution of the equation of state.
+"""
+import os
+
+import numpy as np
+
+from . import abi
+from .physics import THOMSON, planck, K_BOLTZ
+
+EV = 1.602189e-12
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+
+def column_from_dump(path):
+    d = np.load(path)
+    dep = d["depth"]
+    nd = dep.shape[0]
+    T, Ne, Nt, Vt, Vr, Bm, Bi, Ba, opref, tauref, grid, nhi, hgt = dep.T
+    lam_vac = d["lam"][:, 1]
+    lam_mid = 0.5 * (lam_vac[0] + lam_vac[-1])
+    masses = dict((int(k), v) for k, v in d["masses"])
+    levels = d["levels"]
+    lpop = d["lpop"]
+    zc = d["zc"]
+    lines, pops = [], []
+    for (s, z, lo, up, tr), ld, sp in zip(d["lines"], d["line_data"], d["line_split"]):
+        lam0, A, Blu, Bul, grad, sigma, alpha = ld
+        comps = [[], [], []]
+        for row in zc:
+            if int(row[0]) == s and int(row[1]) == tr:
+                comps[int(row[2])].append((row[3], row[4]))
+        elo = [r[3] for r in levels if int(r[0]) == s and int(r[1]) == z and int(r[2]) == lo][0]
+        lines.append(dict(lam0=lam0, aul=A, blu=Blu, bul=Bul, grad=grad, sigma=sigma, alpha=alpha,
+                          mass=masses[int(s)], split=int(sp), comps=comps, species=int(s), tr=int(tr),
+                          e_low=elo))
+        m = (lpop[:, 0] == s) & (lpop[:, 1] == tr)
+        rows = lpop[m]
+        rows = rows[np.argsort(rows[:, 2])]
+        pops.append((rows[:, 3], rows[:, 4]))
+    nf = abi.nfield(len(lines))
+    st = np.zeros((nf, nd))
+    st[abi.F_T] = T
+    st[abi.F_VT] = Vt
+    st[abi.F_VLOS] = Vr
+    st[abi.F_BMAG] = Bm
+    st[abi.F_BINC] = Bi
+    st[abi.F_BAZI] = Ba
+    st[abi.F_NE] = Ne
+    st[abi.F_NHI] = nhi
+    st[abi.F_OPREF] = opref
+    st[abi.F_GRID] = grid
+    bf = d["bfop"]
+    bfop = np.zeros(nd)
+    bfem = np.zeros(nd)
+    l1 = bf[:, 1].min()
+    for row in bf[bf[:, 1] == l1]:
+        bfop[int(row[2]) - 1] += row[3]
+        bfem[int(row[2]) - 1] += row[4]
+    thop = Ne * THOMSON
+    st[abi.F_BGOP] = thop + bfop
+    st[abi.F_BGEM] = thop * planck(lam_mid, T) + bfem
+    for i, (nl, nu) in enumerate(pops):
+        st[abi.NF_BASE + 2 * i] = nl
+        st[abi.NF_BASE + 2 * i + 1] = nu
+    ray = abi.SnapiRay(theta=np.pi, lam_mid=lam_mid, tau_grid=1, hminus=1)
+    return dict(lines=lines, table=abi.make_line_table(lines), ray=ray, state=st, lam_vac=lam_vac,
+                lam_air=d["lam"][:, 0], stokes=d["stokes"], bfop=bfop, bfem=bfem, logtau=np.log10(-tauref))
+
+
+def gaussian_random_field(nx, ny, sigma, corr, rng):
+    """White noise filtered with a Gaussian kernel of the given correlation
+    length (pixels), renormalised to standard deviation sigma."""
+    # drawn on a periodic grid at least 8 correlation lengths wide, then cut
+    mx, my = max(nx, int(8 * corr)), max(ny, int(8 * corr))
+    w = rng.standard_normal((mx, my))
+    kx = np.fft.fftfreq(mx)[:, None]
+    ky = np.fft.fftfreq(my)[None, :]
+    filt = np.exp(-2.0 * (np.pi * corr) ** 2 * (kx ** 2 + ky ** 2))
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * filt))
+    f -= f.mean()
+    f *= sigma / f.std()
+    return np.ascontiguousarray(f[:nx, :ny])
+
+
+def cube_state(template, nx, ny, seed=1, dtype=np.float32):
+    """[nx*ny][nfield][nd] column states of the cube workload (see module doc)."""
+    rng = np.random.default_rng(seed)
+    st0 = template["state"]
+    nf, nd = st0.shape
+    ncol = nx * ny
+    dT = gaussian_random_field(nx, ny, 150.0, 8.0, rng).reshape(ncol)
+    logB = rng.uniform(np.log10(10.0), np.log10(3000.0), ncol)
+    cosi = rng.uniform(-1.0, 1.0, ncol)
+    azi = rng.uniform(0.0, 180.0, ncol)
+    v0 = rng.normal(0.0, 2e5, ncol)
+    vg = rng.normal(0.0, 0.5e5, ncol)
+    vmic = rng.uniform(0.0, 2e5, ncol)
+    logtau = template["logtau"]
+    out = np.empty((ncol, nf, nd), dtype=np.float64)
+    T0 = st0[abi.F_T]
+    T = T0[None, :] + dT[:, None]
+    inv = 1.0 / (K_BOLTZ * T0[None, :]) - 1.0 / (K_BOLTZ * T)  # 1/kT0 - 1/kT
+    out[:] = st0[None]
+    out[:, abi.F_T] = T
+    out[:, abi.F_VT] = vmic[:, None]
+    out[:, abi.F_VLOS] = v0[:, None] + vg[:, None] * logtau[None, :]
+    out[:, abi.F_BMAG] = (10.0 ** logB)[:, None]
+    out[:, abi.F_BINC] = np.degrees(np.arccos(cosi))[:, None]
+    # the reference's ray-frame azimuth is atan(By/Bx): folded into (-90, 90)
+    out[:, abi.F_BAZI] = np.degrees(np.arctan(np.tan(np.radians(azi))))[:, None]
+    # Saha-type rescaling of the electron density (metal ionisation, ~7.9 eV / 2)
+    fne = np.exp(3.95 * EV * inv)
+    out[:, abi.F_NE] = st0[abi.F_NE][None] * fne
+    # referent (H-) opacity ~ Ne T^-1.5 10^(0.754 * 5040 / T)
+    fop = fne * (T0[None] / T) ** 1.5 * 10.0 ** (0.754 * 5040.0 * (1.0 / T - 1.0 / T0[None]))
+    out[:, abi.F_OPREF] = st0[abi.F_OPREF][None] * fop
+    lam_mid = template["ray"].lam_mid
+    thop = out[:, abi.F_NE] * THOMSON
+    out[:, abi.F_BGOP] = thop + template["bfop"][None] * fne
+    out[:, abi.F_BGEM] = thop * planck(lam_mid, T) + template["bfem"][None] * fne * planck(lam_mid, T) / planck(
+        lam_mid, T0[None])
+    for i, ln in enumerate(template["lines"]):
+        # Boltzmann factor of the level energy, times the neutral fraction change
+        fb = np.exp(ln["e_low"] * inv) / fne
+        out[:, abi.NF_BASE + 2 * i] = st0[abi.NF_BASE + 2 * i][None] * fb
+        out[:, abi.NF_BASE + 2 * i + 1] = st0[abi.NF_BASE + 2 * i + 1][None] * fb
+    return np.ascontiguousarray(out.astype(dtype))
+
+
+def template_cfg1():
+    return column_from_dump(os.path.join(GOLDEN, "cfg1_template.npz"))
+
+
+def cfg0_column():
+    return column_from_dump(os.path.join(GOLDEN, "cfg0_synth.npz"))

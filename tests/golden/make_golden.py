@@ -184,6 +184,12 @@ def build_cfg0(d):
     lam = cfg0_lambda()
     setup_dir(d, rows, lam)
     rec = parse_synth(run(d, ["synth", "atmos.cfg", "lambda.txt", repr(math.pi), "0.0", "20"]))
+    save_synth(rec, rows, "cfg0_synth.npz")
+    print("cfg0_synth: nd=%d nlambda=%d" % (len(rows), len(lam)))
+    return rows
+
+
+def save_synth(rec, rows, name):
     dep = np.array([[float(x) for x in r[1:]] for r in rec["depth"]])
     # columns: T Ne Nt Vt Vr |B| incl_deg azi_deg op_ref tau_ref rt_grid N_HI height
     lamv = np.array([[float(r[1]), float(r[2])] for r in rec["lambda"]])
@@ -207,7 +213,7 @@ def build_cfg0(d):
     levels = [[int(r[0]), int(r[1]), int(r[2]), float(r[3]), int(r[5]), int(r[7]), int(r[9]), float(r[11])]
               for r in rec["level"]]
     np.savez_compressed(
-        os.path.join(HERE, "cfg0_synth.npz"),
+        os.path.join(HERE, name),
         depth=dep, lam=lamv, stokes=stokes, stokes_steps=steps, opem=opem, spop=spop, sdepth=sdepth, bfop=bfop,
         lines=np.array([l[:4] + [l[4]] + [0] * 0 for l in lines], dtype=np.int64),
         line_data=np.array([l[5:12] for l in lines]), line_split=np.array([l[12] for l in lines]),
@@ -215,8 +221,22 @@ def build_cfg0(d):
         masses=np.array([[k, v] for k, v in sorted(masses.items())]),
         levels=np.array(levels, dtype=np.float64),
         atmos_rows=np.array(rows, dtype=np.float32))
-    print("cfg0_synth: nd=%d nlambda=%d lines=%d" % (len(dep), len(lamv), len(lines)))
-    return rows
+
+
+def cfg1_lambda():
+    """301 points, 8 mA apart, covering both lines: 6300.8 .. 6303.2 A."""
+    return np.round(6300.8 + 0.008 * np.arange(301), 6)
+
+
+def build_cfg1_template(d):
+    """The 71-depth column the cube workload is drawn around (same profile as
+    cfg0 on 71 points), its state and Stokes on the 301-point grid."""
+    rows = cfg0_atmosphere(71)
+    lam = cfg1_lambda()
+    setup_dir(d, rows, lam)
+    rec = parse_synth(run(d, ["synth", "atmos.cfg", "lambda.txt", repr(math.pi), "0.0", "1000"]))
+    save_synth(rec, rows, "cfg1_template.npz")
+    print("cfg1_template: nd=%d nlambda=%d" % (len(rows), len(lam)))
 
 
 def build_resp(d):
@@ -275,6 +295,7 @@ def main():
     d = tempfile.mkdtemp(prefix="snapi_golden_")
     try:
         build_cfg0(d)
+        build_cfg1_template(d)
         build_resp(d)
         build_lmstep(d)
     finally:

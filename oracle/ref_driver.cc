@@ -350,10 +350,14 @@ static int mode_formal_scalar(int argc, char **argv, io_class &io) {
   fp_t ***L = ft3dim(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
   printf("nd %d\n", a->x3h - a->x3l + 1);
   for (int z = a->x3l; z <= a->x3h; ++z) printf("grid %d %.17g\n", z, a->rt_grid[z]);
+  // the background opacity of the ALI loop (atmos/atmos_pop.cc:148-160,
+  // atmos/atmos_rad.cc:225-246) needs the transition maps of rtsetup
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->rtsetup(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
   for (int l = 1; l <= nlambda; ++l) {
     fp_t lv = airtovac(lambda[l]);
-    fp_t ***op = a->opacity(a->T, a->Ne, Vr, a->Vt, Bv, el, az, lv);
-    fp_t ***em = a->emissivity(a->T, a->Ne, Vr, a->Vt, Bv, el, az, lv);
+    for (int s = 0; s < a->natm; ++s) a->atml[s]->prof_init();
+    fp_t ***op = a->opacity_lte(a->T, a->Ne, Vr, a->Vt, Bv, el, az, lv);
+    fp_t ***em = a->emissivity_lte(a->T, a->Ne, Vr, a->Vt, Bv, el, az, lv);
     if (a->tau_grid) a->normalize_to_referent_opacity(op, em);
     for (int z = a->x3l; z <= a->x3h; ++z)
       printf("opem1 %d %d %.17g %.17g\n", l, z, op[x1][x2][z], em[x1][x2][z]);
@@ -363,6 +367,7 @@ static int mode_formal_scalar(int argc, char **argv, io_class &io) {
     del_ft3dim(op, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
     del_ft3dim(em, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
   }
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->rtclean(0, 0, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
   a->popclean();
   return 0;
 }

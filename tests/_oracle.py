@@ -103,3 +103,22 @@ def responses(table, ray, state, dstate, lam, nthreads=0):
     rc = lib().orc_responses(C.byref(table), C.byref(ray), ncol, nd, ptr(st), ptr(ds), len(lam), ptr(lam), ptr(out),
                              ptr(resp), nthreads)
     return out, resp, rc
+
+
+def lm_step(J, r, lm):
+    """One damped normal-equation solve as atmos/atmos_fit.cc:129-160 does it:
+    J^T J, J^T r, JTJ_ii *= (1 + lm), then solve_svd (cmlib/mathtools.cc:
+    344-387): pseudo-inverse dropping singular values with w_max/w > 1e6.
+    numpy's SVD is the textbook routine the reference's svdcmp computes."""
+    J = np.asarray(J, dtype=np.float64)
+    r = np.asarray(r, dtype=np.float64)
+    JTJ = J.T @ J
+    rhs = J.T @ r
+    M = JTJ.copy()
+    M[np.diag_indices_from(M)] *= (1.0 + lm)
+    U, w, Vt = np.linalg.svd(M)
+    wmax = w.max()
+    keep = (w != 0) & (wmax / np.where(w == 0, 1, w) <= 1e6)
+    winv = np.where(keep, 1.0 / np.where(w == 0, 1, w), 0.0)
+    corr = Vt.T @ (winv * (U.T @ rhs))
+    return JTJ, rhs, corr

@@ -263,6 +263,31 @@ def build_resp(d):
     print("cfg0_resp: nlambda=%d" % nl)
 
 
+def build_formal_scalar(d):
+    """Scalar formal solution + Lambda* of the reference on its own scalar
+    opacity (atmosphere::opacity/emissivity) for a few wavelengths, outgoing
+    and ingoing rays."""
+    rows = cfg0_atmosphere()
+    lam = np.array([6301.2, 6301.5, 6302.5, 6303.0])
+    setup_dir(d, rows, lam)
+    rec = {}
+    for tag, th in (("out", repr(math.pi)), ("in", "0.0")):
+        opem, sl, grid = [], [], []
+        for ln in run(d, ["formal_scalar", "atmos.cfg", "lambda.txt", th, "0.0"]):
+            t = ln.split()
+            if t[0] == "opem1":
+                opem.append([float(x) for x in t[1:]])
+            elif t[0] == "sl":
+                sl.append([float(x) for x in t[1:]])
+            elif t[0] == "grid":
+                grid.append(float(t[2]))
+        rec[tag + "_opem"] = np.array(opem)
+        rec[tag + "_sl"] = np.array(sl)
+        rec["grid"] = np.array(grid)
+    np.savez_compressed(os.path.join(HERE, "formal_scalar.npz"), **rec)
+    print("formal_scalar: %d wavelengths" % len(lam))
+
+
 def build_lmstep(d):
     recs = {}
     for nrow, npar, lm, seed in ((240, 14, 1e-2, 1), (120, 14, 10.0, 2), (60, 7, 1e3, 3)):
@@ -297,6 +322,7 @@ def main():
         build_cfg0(d)
         build_cfg1_template(d)
         build_resp(d)
+        build_formal_scalar(d)
         build_lmstep(d)
     finally:
         shutil.rmtree(d)

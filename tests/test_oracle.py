@@ -142,3 +142,41 @@ def test_scalar_formal_lambda_operator():
     # Lambda* = w1 of the step arriving at each point; 1 at the bottom boundary
     assert L[nd - 1] == 1.0
     assert np.all((L[:-1] > 0) & (L[:-1] < 1))
+
+
+def test_scalar_formal_matches_reference_golden():
+    import _golden
+    d = _golden.load("formal_scalar.npz")
+    grid = d["grid"]
+    nd = len(grid)
+    for tag, th in (("out", math.pi), ("in", 0.0)):
+        opem, sl = d[tag + "_opem"], d[tag + "_sl"]
+        for l in np.unique(opem[:, 0]).astype(int):
+            m = opem[:, 0] == l
+            op, em = opem[m, 2], opem[m, 3]
+            I, L, rc = _oracle.formal_scalar(grid, op, em, th)
+            ref = sl[sl[:, 0] == l]
+            scale = np.max(np.abs(ref[:, 2]))
+            assert np.max(np.abs(I - ref[:, 2])) / scale < 1e-12, (tag, l)
+            assert np.max(np.abs(L - ref[:, 3])) < 1e-12, (tag, l)
+
+
+def test_lm_step_matches_reference_golden():
+    import _golden
+    d = _golden.load("lmstep.npz")
+    for s in ("s1", "s2", "s3"):
+        JTJ, rhs, corr = _oracle.lm_step(d[s + "_J"], d[s + "_r"], float(d[s + "_lm"]))
+        assert np.allclose(JTJ, d[s + "_JTJ"], rtol=1e-12, atol=1e-12)
+        assert np.allclose(rhs, d[s + "_rhs"], rtol=1e-12, atol=1e-12)
+        ref = d[s + "_corr"]
+        assert np.max(np.abs(corr - ref)) / np.max(np.abs(ref)) < 1e-8, s
+
+
+def test_lm_step_linear_model_one_step():
+    """For a linear model r = J x_true and lm -> 0 the step is the
+    least-squares solution x_true (normal equations)."""
+    rng = np.random.default_rng(0)
+    J = rng.standard_normal((200, 9))
+    x = rng.standard_normal(9)
+    _, _, corr = _oracle.lm_step(J, J @ x, 0.0)
+    assert np.allclose(corr, x, rtol=1e-10)

@@ -27,20 +27,23 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int kNP = 16;  // padded parameter count (J columns + r + zero)
 
 // Lane l of a 16x16x4 MFMA holds A[l%16][l/16] and B[l/16][l%16]; for the
-// Gram matrix both are Jaug[row0 + l/16][l%16].  The 16x16 result holds, in
-// lane l, C[4*(l/16) + i][l%16] for i = 0..3.
+// Gram matrix both are Jaug[row0 + l/16][l%16].  Element i = 0..3 of the
+// result in lane l is C[row(l/16, i)][l%16]: rows 4*(l/16)+i for the f32
+// instruction, rows l/16 + 4*i for the f64 one.
 template <typename T> struct Mfma;
 template <> struct Mfma<float> {
   typedef floatx4 acc;
   static __device__ __forceinline__ acc step(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int sub, int i) { return 4 * sub + i; }
 };
 template <> struct Mfma<double> {
   typedef doublex4 acc;
   static __device__ __forceinline__ acc step(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int sub, int i) { return sub + 4 * i; }
 };
 
 template <typename T>
@@ -66,7 +69,7 @@ __global__ __launch_bounds__(256) void lm_normal_kernel(int npix, int nrow, int 
   double *G = JTJ + (size_t)wave * npar * npar;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int rr = 4 * sub + i;
+    const int rr = Mfma<T>::row(sub, i);
     const double val = (double)c[i];
     if (rr < npar && col < npar) G[rr * npar + col] = val;
     if (rr == npar && col < npar) rhs[(size_t)wave * npar + col] = val;  // (J^T r)_col = r^T J

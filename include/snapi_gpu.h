@@ -70,7 +70,8 @@ enum snapi_field {
   SNAPI_F_GRID = 9,  /* RT grid: tau_referent (negative, -10^logtau) or height [cm] */
   SNAPI_F_BGOP = 10, /* wavelength-independent background opacity (Thomson + atomic bound-free) [cm^-1] */
   SNAPI_F_BGEM = 11, /* its emissivity */
-  SNAPI_NF_BASE = 12 /* then per line l: SNAPI_NF_BASE+2l lower, +2l+1 upper level population */
+  SNAPI_F_NT = 12,   /* total particle density [cm^-3] (density responses) */
+  SNAPI_NF_BASE = 13 /* then per line l: SNAPI_NF_BASE+2l lower, +2l+1 upper level population */
 };
 
 #define SNAPI_MAX_LINES 16
@@ -114,6 +115,8 @@ typedef struct {
   double lam_mid;  /* wavelength of the Thomson emissivity, (lambda[1]+lambda[n])/2 (vacuum) [cm] */
   int tau_grid;    /* 1: op/em normalised by op_ref and GRID is tau_referent */
   int hminus;      /* 1: add H- bound-free + free-free (atmos/atmol/mol/mol.cc:379-412) */
+  int hminus_mid;  /* 1: H- at lam_mid for every wavelength, as the response path of the
+                      reference evaluates it (atmos/atmol/mol/mol.cc:414-469) */
 } snapi_ray;
 
 /* Library version / device query. */

@@ -261,12 +261,80 @@ static int mode_synth(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// The state and thermodynamic derivatives the response path works with,
+// recorded right where obs_stokes_responses (atmos/atmos_rad_synthesis.cc:
+// 166-399) builds its opacities: the same preamble (:173-190), then the
+// per-depth state, d/dT and d/dNt (p = 1, 2) of Ne, neutral H, the referent
+// opacity and the populations of every line level, and the bound-free
+// background.  ne_lte_derivatives (atmos/atmos_responses.cc:364-423) leaves T
+// raised by delta_T/2 and Ne, populations re-solved there, so the state is
+// recorded after it, as the response path sees it.
+static void dump_resp_state(atmosphere *a, fp_t el, fp_t az, fp_t *lambda, int nlambda) {
+  int x1 = a->x1l, x2 = a->x2l;
+  a->boundary_condition_for_rt = -1;
+  a->set_grid(a->tau_grid);
+  a->popsetup();
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->set_parent_atmosphere(a);
+  if (a->tau_grid) {
+    a->compute_op_referent();
+    a->compute_op_referent_derivative();
+  }
+  a->nltepops();
+  a->respsetup();
+  a->ne_lte_derivatives();
+  a->compute_nlte_population_responses(0);
+  fp_t ***Vr = a->project(a->Vx, a->Vy, a->Vz, el, az, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  fp_t ****Bv = a->transform(a->Bx, a->By, a->Bz, el, az, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  for (int s = 0; s < a->natm; ++s) {
+    a->atml[s]->rtsetup(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    a->atml[s]->zeeman_setup();
+  }
+  fp_t *lam_vac = airtovac(lambda + 1, nlambda) - 1;
+  for (int l = 1; l <= nlambda; ++l) printf("lambda %d %.17g %.17g\n", l, lambda[l], lam_vac[l]);
+  dump_state(a, Vr, Bv);
+  for (int z = a->x3l; z <= a->x3h; ++z) {
+    printf("thermo %d %.17g", z, a->Nt[x1][x2][z]);
+    for (int p = 1; p <= 2; ++p)
+      printf(" %.17g %.17g %.17g", a->Ne_lte_der[p][x1][x2][z], a->get_neutral_H_derivative_lte(p, x1, x2, z),
+             a->tau_grid ? a->op_referent_derivative[p][z][x1][x2][z] : 0.0);
+    printf("\n");
+  }
+  for (int s = 0; s < a->natm; ++s) {
+    atom *at = dynamic_cast<atom *>(a->atml[s]);
+    if (!at) continue;
+    for (int zi = 0; zi <= at->Z; ++zi)
+      for (int i = 1; i < at->nl[zi]; ++i)
+        for (int ii = 0; ii < i; ++ii) {
+          if (!(at->A[zi][i][ii] > 1E0)) continue;
+          for (int z = a->x3l; z <= a->x3h; ++z) {
+            printf("dpop %d %d %d %d %d", s, zi, ii, i, z);
+            for (int p = 1; p <= 2; ++p)
+              printf(" %.17g %.17g", at->level_responses[p][(z - a->x3l) * at->nmap + at->rmap[zi][ii] + 1][z],
+                     at->level_responses[p][(z - a->x3l) * at->nmap + at->rmap[zi][i] + 1][z]);
+            printf("\n");
+          }
+        }
+  }
+  dump_species_opem(a, Vr, Bv, el, az, nlambda, lam_vac, nlambda > 1 ? nlambda - 1 : 1);
+  for (int s = 0; s < a->natm; ++s) {
+    a->atml[s]->zeeman_clear();
+    a->atml[s]->rtclean(1, nlambda, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  }
+  a->respclean();
+  a->popclean();
+  if (a->tau_grid) a->delete_op_referent_derivative();
+}
+
 static int mode_resp(int argc, char **argv, io_class &io) {
   if (argc < 6) return 2;
   atmosphere *a = make_atmos(argv[2], io);
   int nlambda;
   fp_t *lambda = read_lambda(argv[3], nlambda);
   fp_t el = atof(argv[4]), az = atof(argv[5]);
+  dump_resp_state(a, el, az, lambda, nlambda);
+  // the entry point itself runs on a fresh copy of the same column
+  delete a;
+  a = make_atmos(argv[2], io);
   a->set_grid(a->tau_grid);
   int nd = a->x3h - a->x3l + 1;
   fp_t ****rsp = ft4dim(1, 7, a->x3l, a->x3h, 1, nlambda, 1, 4);

@@ -171,7 +171,7 @@ void orc_opem(const snapi_line_table *lt, const snapi_ray *ray, int nd, const do
     E[0] = ST(SNAPI_F_BGEM, z);
     if (ray->hminus) {
       double o, e;
-      hminus(lam, T, ST(SNAPI_F_NE, z), ST(SNAPI_F_NHI, z), &o, &e);
+      hminus(ray->hminus_mid ? ray->lam_mid : lam, T, ST(SNAPI_F_NE, z), ST(SNAPI_F_NHI, z), &o, &e);
       K[0] += o;
       E[0] += e;
     }

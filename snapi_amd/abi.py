@@ -16,8 +16,8 @@ SNAPI_ERR_NONFINITE = -4
 SNAPI_FP64 = 0
 SNAPI_MIXED = 1
 
-F_T, F_VT, F_VLOS, F_BMAG, F_BINC, F_BAZI, F_NE, F_NHI, F_OPREF, F_GRID, F_BGOP, F_BGEM = range(12)
-NF_BASE = 12
+F_T, F_VT, F_VLOS, F_BMAG, F_BINC, F_BAZI, F_NE, F_NHI, F_OPREF, F_GRID, F_BGOP, F_BGEM, F_NT = range(13)
+NF_BASE = 13
 
 MAX_LINES = 16
 MAX_COMPONENTS = 96
@@ -49,7 +49,8 @@ class SnapiLineTable(C.Structure):
 
 
 class SnapiRay(C.Structure):
-    _fields_ = [("theta", C.c_double), ("lam_mid", C.c_double), ("tau_grid", C.c_int), ("hminus", C.c_int)]
+    _fields_ = [("theta", C.c_double), ("lam_mid", C.c_double), ("tau_grid", C.c_int), ("hminus", C.c_int),
+                ("hminus_mid", C.c_int)]
 
 
 def nfield(nline):

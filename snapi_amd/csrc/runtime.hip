@@ -99,6 +99,8 @@ int prepare_synth(const snapi_line_table *lines, const snapi_ray *ray, int ncol,
   a.ct = ct;
   a.tau_grid = ray->tau_grid ? 1 : 0;
   a.hminus = ray->hminus ? 1 : 0;
+  a.hminus_mid = ray->hminus_mid ? 1 : 0;
+  a.lam_mid = ray->lam_mid;
   return SNAPI_OK;
 }
 

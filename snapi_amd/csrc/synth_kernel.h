@@ -13,8 +13,9 @@ constexpr double kLog2_10 = 3.321928094887362347870319429489390175864831393;
 
 struct SynthArgs {
   int ncol, nd, nlambda, nfield;
-  double ct;  // cos(theta) of the ray
-  int tau_grid, hminus;
+  double ct;       // cos(theta) of the ray
+  double lam_mid;  // H- wavelength when hminus_mid is set
+  int tau_grid, hminus, hminus_mid;
 };
 
 // LDS tile of one column: per-depth quantities that do not depend on the
@@ -113,10 +114,14 @@ template <typename R> struct LamConst {
   R plc, ple;           // Planck: 2 h c^2 / lambda^5 and hc/(k lambda) as the reference writes them
 };
 
-template <typename R> __device__ __forceinline__ void lambda_consts(const KLines &L, double lam, LamConst<R> &c) {
+template <typename R>
+__device__ __forceinline__ void lambda_consts(const KLines &L, const SynthArgs &a, double lam, LamConst<R> &c) {
   c.lam = lam;
   c.lamR = (R)lam;
-  const double la = lam * 1E8;
+  // the response path of the reference evaluates H- once, at the middle of
+  // the wavelength range (atmos/atmol/mol/mol.cc:420-423)
+  const double lh = a.hminus_mid ? a.lam_mid : lam;
+  const double la = lh * 1E8;
   // H- bound-free (mol.cc:612-617) and free-free (mol.cc:524-527), Gray
   const double ac[7] = {1.99654, -1.18267E-5, 2.64243E-6, -4.40524E-10, 3.23992E-14, -1.39568E-18, 2.78701E-23};
   double alpha = 0.0, p = 1.0;
@@ -129,9 +134,9 @@ template <typename R> __device__ __forceinline__ void lambda_consts(const KLines
   c.f0 = (R)(-2.2763 - 1.6850 * l + 0.76661 * l * l - 0.053346 * l * l * l);
   c.f1 = (R)(15.2827 - 9.2846 * l + 1.99381 * l * l - 0.142631 * l * l * l);
   c.f2 = (R)(-197.789 + 190.266 * l - 67.9775 * l * l + 10.6913 * l * l * l - 0.625151 * l * l * l * l);
-  c.hckl = (R)(kH * kC / lam / kK);
-  c.plc = (R)(2.0 * 6.62606957 * 8.9875517E-7 / lam / lam / lam / lam / lam);
-  c.ple = (R)(1.4387769 / lam);
+  c.hckl = (R)(kH * kC / lh / kK);
+  c.plc = (R)(2.0 * 6.62606957 * 8.9875517E-7 / lh / lh / lh / lh / lh);
+  c.ple = (R)(1.4387769 / lh);
 }
 
 // Opacity (7 independent matrix elements) and emissivity of one depth point,
@@ -218,7 +223,7 @@ template <typename R> __device__ __forceinline__ void reduce_ks(const R K[7], co
 template <typename R>
 __device__ int solve_lambda(const KLines &L, const SynthArgs &a, const ColumnTile<R> &t, double lam, R out[4]) {
   LamConst<R> c;
-  lambda_consts<R>(L, lam, c);
+  lambda_consts<R>(L, a, lam, c);
   const int nd = a.nd;
   const bool up = a.ct < 0.0;  // outgoing ray: from the bottom (z = nd-1) to the top
   const R mct = (R)(-a.ct);

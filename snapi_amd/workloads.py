@@ -68,6 +68,7 @@ def column_from_dump(path):
     st[abi.F_NHI] = nhi
     st[abi.F_OPREF] = opref
     st[abi.F_GRID] = grid
+    st[abi.F_NT] = Nt
     bf = d["bfop"]
     bfop = np.zeros(nd)
     bfem = np.zeros(nd)
@@ -144,6 +145,40 @@ def cube_state(template, nx, ny, seed=1, dtype=np.float32):
         out[:, abi.NF_BASE + 2 * i] = st0[abi.NF_BASE + 2 * i][None] * fb
         out[:, abi.NF_BASE + 2 * i + 1] = st0[abi.NF_BASE + 2 * i + 1][None] * fb
     return np.ascontiguousarray(out.astype(dtype))
+
+
+def response_column_from_dump(path):
+    """Column state as the reference's response path sees it, plus the
+    thermodynamic derivatives dstate[2][nfield][nd] (d/dT, d/dNt of Ne,
+    neutral H, referent opacity and line-level populations)."""
+    g = column_from_dump(path)
+    d = np.load(path)
+    st = g["state"]
+    nf, nd = st.shape
+    ds = np.zeros((2, nf, nd))
+    th = d["thermo"]  # z Nt [dNe dNH dkref] x 2
+    for row in th:
+        z = int(row[0]) - 1
+        for p in range(2):
+            ds[p, abi.F_NE, z] = row[2 + 3 * p]
+            ds[p, abi.F_NHI, z] = row[3 + 3 * p]
+            ds[p, abi.F_OPREF, z] = row[4 + 3 * p]
+    dp = d["dpop"]  # s zi lo up z [dlo dup] x 2
+    for i, ln in enumerate(g["lines"]):
+        rows = dp[dp[:, 0] == ln["species"]]
+        # match this line's lower/upper levels through the population dump order
+        s_lines = [l for l in g["lines"] if l["species"] == ln["species"]]
+        k = s_lines.index(ln)
+        keys = sorted(set((int(r[1]), int(r[2]), int(r[3])) for r in rows), key=lambda t: (t[0], t[2], t[1]))
+        zi, lo, up = keys[k]
+        sel = rows[(rows[:, 1] == zi) & (rows[:, 2] == lo) & (rows[:, 3] == up)]
+        sel = sel[np.argsort(sel[:, 4])]
+        for p in range(2):
+            ds[p, abi.NF_BASE + 2 * i] = sel[:, 5 + 2 * p]
+            ds[p, abi.NF_BASE + 2 * i + 1] = sel[:, 6 + 2 * p]
+    g["dstate"] = ds
+    g["resp"] = d["resp"]
+    return g
 
 
 def template_cfg1():

@@ -189,7 +189,7 @@ def build_cfg0(d):
     return rows
 
 
-def save_synth(rec, rows, name):
+def save_synth(rec, rows, name, extra=None):
     dep = np.array([[float(x) for x in r[1:]] for r in rec["depth"]])
     # columns: T Ne Nt Vt Vr |B| incl_deg azi_deg op_ref tau_ref rt_grid N_HI height
     lamv = np.array([[float(r[1]), float(r[2])] for r in rec["lambda"]])
@@ -220,7 +220,7 @@ def save_synth(rec, rows, name):
         zc=np.array(zc, dtype=np.float64), lpop=lpop,
         masses=np.array([[k, v] for k, v in sorted(masses.items())]),
         levels=np.array(levels, dtype=np.float64),
-        atmos_rows=np.array(rows, dtype=np.float32))
+        atmos_rows=np.array(rows, dtype=np.float32), **(extra or {}))
 
 
 def cfg1_lambda():
@@ -240,16 +240,21 @@ def build_cfg1_template(d):
 
 
 def build_resp(d):
+    """Per-depth responses of the 61-point column on 31 wavelengths around
+    6301.5 A: the state, thermodynamic derivatives and bound-free background
+    as obs_stokes_responses sees them, and its Stokes and 7 x nd responses."""
     rows = cfg0_atmosphere()
     lam = np.round(6301.5 + np.linspace(-0.3, 0.3, 31), 6)
     setup_dir(d, rows, lam)
-    lam_vac = np.array([airtovac(l * 1e-8) * 1e8 for l in lam])
     out = run(d, ["resp", "atmos.cfg", "lambda.txt", repr(math.pi), "0.0"])
-    st, rs = [], []
+    rec = parse_synth(out)
+    thermo, dpop, rs = [], [], []
     for ln in out:
         t = ln.split()
-        if t and t[0] == "stokes":
-            st.append([float(x) for x in t[1:]])
+        if t and t[0] == "thermo":
+            thermo.append([float(x) for x in t[1:]])
+        elif t and t[0] == "dpop":
+            dpop.append([float(x) for x in t[1:]])
         elif t and t[0] == "resp":
             rs.append([float(x) for x in t[1:]])
     rs = np.array(rs)
@@ -258,8 +263,7 @@ def build_resp(d):
     resp = np.zeros((7, nd, nl, 4))
     for r in rs:
         resp[int(r[0]) - 1, int(r[1]) - 1, int(r[2]) - 1, :] = r[3:]
-    np.savez_compressed(os.path.join(HERE, "cfg0_resp.npz"), lam_air=lam, lam_vac=lam_vac, stokes=np.array(st),
-                        resp=resp)
+    save_synth(rec, rows, "cfg0_resp.npz", extra=dict(thermo=np.array(thermo), dpop=np.array(dpop), resp=resp))
     print("cfg0_resp: nlambda=%d" % nl)
 
 

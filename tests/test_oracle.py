@@ -1,6 +1,7 @@
 """The FP64 oracle against the reference's golden vectors and against closed
 forms.  CPU only."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -180,3 +181,20 @@ def test_lm_step_linear_model_one_step():
     x = rng.standard_normal(9)
     _, _, corr = _oracle.lm_step(J, J @ x, 0.0)
     assert np.allclose(corr, x, rtol=1e-10)
+
+
+def test_responses_match_reference_golden():
+    """Per-depth responses of the oracle (the reference's own algorithm:
+    analytic local opacity derivatives + central difference through the
+    solver) against obs_stokes_responses of the reference build."""
+    from snapi_amd import workloads
+    g = workloads.response_column_from_dump(os.path.join(os.path.dirname(__file__), "golden", "cfg0_resp.npz"))
+    ray = abi.SnapiRay(theta=math.pi, lam_mid=g["ray"].lam_mid, tau_grid=1, hminus=1, hminus_mid=1)
+    st, resp, rc = _oracle.responses(g["table"], ray, g["state"][None], g["dstate"][None], g["lam_vac"])
+    assert rc == 0
+    assert np.max(np.abs(st[0] - g["stokes"])) / g["stokes"][:, 0].max() < 1e-12
+    for p in range(7):
+        ref = g["resp"][p]
+        scale = np.maximum(np.abs(ref).max(axis=(0, 1)), 1e-300)
+        err = np.max(np.abs(resp[0, p] - ref) / scale)
+        assert err < 1e-6, (p, err)

@@ -1,5 +1,7 @@
 """Python entry points for the LM normal-equation step and the scalar formal
 solver (host arrays in and out)."""
+import ctypes as C
+
 import numpy as np
 
 from . import abi
@@ -37,3 +39,40 @@ def formal_scalar(grid, op, em, ct, boundary=-1.0):
                                    abi.ptr(I), abi.ptr(L))
     check("snapi_formal_scalar", rc)
     return I, L
+
+
+def stokes_responses(table, ray, state, dstate, lam_vac, precision=abi.SNAPI_FP64):
+    """Emergent Stokes [ncol][nlambda][4] and per-depth responses
+    [ncol][7][nd][nlambda][4] (atmosphere::obs_stokes_responses,
+    atmos/atmos_rad_synthesis.cc:166-399), by the adjoint sweep."""
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    ds = np.ascontiguousarray(dstate, dtype=np.float64)
+    if st.ndim == 2:
+        st, ds = st[None], ds[None]
+    ncol, nf, nd = st.shape
+    lam = np.ascontiguousarray(lam_vac, dtype=np.float64)
+    stokes = np.zeros((ncol, len(lam), 4))
+    resp = np.zeros((ncol, 7, nd, len(lam), 4))
+    rc = lib().snapi_stokes_responses(precision, C.byref(table), C.byref(ray), ncol, nd, abi.vptr(st),
+                                      abi.vptr(ds), len(lam), abi.ptr(lam), abi.vptr(stokes), abi.vptr(resp))
+    check("snapi_stokes_responses", rc)
+    return stokes, resp
+
+
+def stokes_adjoint(table, ray, state, dstate, lam_vac, obs, wstokes=(1.0, 1.0, 1.0, 1.0),
+                   precision=abi.SNAPI_FP64):
+    """Gradient [ncol][7][nd] of chi^2/2 = sum w_s (S - S_obs)^2 / 2 with
+    respect to the per-depth parameters, in one reverse sweep."""
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    ds = np.ascontiguousarray(dstate, dtype=np.float64)
+    if st.ndim == 2:
+        st, ds = st[None], ds[None]
+    ncol, nf, nd = st.shape
+    lam = np.ascontiguousarray(lam_vac, dtype=np.float64)
+    ob = np.ascontiguousarray(obs, dtype=np.float64).reshape(ncol, len(lam), 4)
+    ws = np.ascontiguousarray(wstokes, dtype=np.float64)
+    grad = np.zeros((ncol, 7, nd))
+    rc = lib().snapi_stokes_adjoint(precision, C.byref(table), C.byref(ray), ncol, nd, abi.vptr(st), abi.vptr(ds),
+                                    len(lam), abi.ptr(lam), abi.ptr(ob), abi.ptr(ws), abi.ptr(grad))
+    check("snapi_stokes_adjoint", rc)
+    return grad

@@ -1,0 +1,633 @@
+// Per-depth response functions and the chi^2 gradient of the polarized
+// synthesis, by a hand-written reverse (adjoint) sweep of the formal
+// solution on gfx950.  One work-item per (column, wavelength):
+//   1. forward: opacity build + recurrence as in synthesis (synth_kernel.h),
+//      K, E and I of every depth point kept in a per-work-item scratch area;
+//   2. reverse: adjoints of I along the ray, back through the 4x4 solves, the
+//      short-characteristics weights, the optical-depth steps and the
+//      Fritsch-Butland derivative, giving dI_out/d(K, E) at every depth for
+//      up to four seeds (the four emergent Stokes components, or one seed
+//      w_s (S - S_obs) for the gradient);
+//   3. contraction: at every depth the local derivatives of (K, E) along the
+//      seven physical parameters (T, Nt, vt, vLOS, |B|, inclination,
+//      azimuth), restating the analytic part of the reference's
+//      op_em_vector_plus_pert (atmos/atmos_rad_synthesis.cc:76-162,
+//      atmos/atmol/atom/atom_rad_synthesis.cc:771-1026, atmos/atmol/mol/
+//      mol.cc:414-469, profile derivatives atom_rad.cc:611-664, 795-875) and
+//      the referent-opacity normalisation (atmos/atmos_rad.cc:187-201).
+// The reference propagates those local derivatives through the solver by a
+// central difference (formal_pert_numerical, plpar/atmos_ppbez_rts.cc:
+// 1708-1776); the adjoint gives the same derivative without the 2 x 7 x ND
+// extra formal solutions per wavelength.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "rt_common.h"
+#include "synth_kernel.h"
+
+namespace snapi {
+
+constexpr int kRespThreads = 128;
+constexpr int kFwd = 15;  // per depth: K[7], E[4], I[4]
+constexpr int kAdj = 11;  // per depth and seed: Kbar[7], Ebar[4]
+enum { SINK_RESP = 0, SINK_GRAD = 1 };
+
+struct RespArgs {
+  int nseed;          // 4 (responses) or 1 (gradient)
+  int sink;
+  int col0;           // first column of this launch
+  long long nwork;    // work-items of this launch (columns * nlambda)
+  const double *obs;  // [ncol][nlambda][4] (gradient)
+  double ws[4];       // Stokes weights (gradient)
+};
+
+// scratch: [depth k][field][work-item]
+struct Scratch {
+  double *p;
+  long long W;
+  int nd;
+  __device__ double &fwd(int k, int f, long long w) const { return p[((long long)k * kFwd + f) * W + w]; }
+  __device__ double &adj(int k, int f, int s, long long w) const {
+    return p[(long long)nd * kFwd * W + (((long long)k * kAdj + f) * 4 + s) * W + w];
+  }
+};
+
+// Local derivatives of the normalised opacity/emissivity of depth z along
+// the seven physical parameters.  dK[p][7] in the compact order
+// (eta_I, eta_Q, eta_U, eta_V, rho_Q, rho_U, rho_V), dE[p][4].
+__device__ void local_derivs(const KLines &L, const SynthArgs &a, const double *__restrict__ st,
+                             const double *__restrict__ dst, int z, double lam, double dK[7][7], double dE[7][4]) {
+  const int nd = a.nd, nf = a.nfield;
+#define S_(f) st[(size_t)(f)*nd + z]
+#define D_(p, f) dst[((size_t)(p)*nf + (f)) * nd + z]
+  for (int p = 0; p < 7; ++p) {
+    for (int i = 0; i < 7; ++i) dK[p][i] = 0.0;
+    for (int i = 0; i < 4; ++i) dE[p][i] = 0.0;
+  }
+  double K[7] = {0, 0, 0, 0, 0, 0, 0}, E[4] = {0, 0, 0, 0};
+  const double T = S_(SNAPI_F_T), Ne = S_(SNAPI_F_NE), nhi = S_(SNAPI_F_NHI), Nt = S_(SNAPI_F_NT);
+  const double dNe[2] = {D_(0, SNAPI_F_NE), D_(1, SNAPI_F_NE)};
+  const double dNH[2] = {D_(0, SNAPI_F_NHI), D_(1, SNAPI_F_NHI)};
+  const double lm = a.lam_mid;
+  auto planck = [](double l, double t) {
+    return 2.0 * 6.62606957 * 8.9875517E-7 / l / l / l / l / l / (exp(1.4387769 / l / t) - 1.0);
+  };
+  auto planck_dT = [&](double l, double t) {
+    const double x = 1.4387769 / l / t;
+    const double tmp = (x > 5.0) ? 1.0 : exp(x) / (exp(x) - 1.0);
+    return planck(l, t) * tmp * 1.4387769 / l / t / t;
+  };
+  // Thomson at the middle wavelength and the atomic bound-free background
+  const double opth = Ne * kThomson;
+  K[0] = S_(SNAPI_F_BGOP);
+  E[0] = S_(SNAPI_F_BGEM);
+  for (int p = 0; p < 2; ++p) {
+    const double o = dNe[p] * kThomson;
+    dK[p][0] += o;
+    dE[p][0] += planck(lm, T) * o + (p == 0 ? opth * planck_dT(lm, T) : 0.0);
+  }
+  if (a.hminus) {
+    const double lh = a.hminus_mid ? lm : lam;
+    const double ac[7] = {1.99654, -1.18267E-5, 2.64243E-6, -4.40524E-10, 3.23992E-14, -1.39568E-18, 2.78701E-23};
+    double alpha = 0.0, pw = 1.0;
+    for (int i = 0; i < 7; ++i) {
+      alpha += ac[i] * pw;
+      pw *= lh * 1E8;
+    }
+    const double TP = 5040. / T;
+    const double stim = exp(-kH * kC / lh / kK / T);
+    const double obf = 4.158E-28 * alpha * Ne * kK * T * pow(TP, 2.5) * pow(10.0, 0.754 * TP) * nhi * (1.0 - stim);
+    const double l = log10(lh * 1E8);
+    const double f0 = -2.2763 - 1.6850 * l + 0.76661 * l * l - 0.053346 * l * l * l;
+    const double f1 = 15.2827 - 9.2846 * l + 1.99381 * l * l - 0.142631 * l * l * l;
+    const double f2 = -197.789 + 190.266 * l - 67.9775 * l * l + 10.6913 * l * l * l - 0.625151 * l * l * l * l;
+    const double ltp = log10(TP);
+    const double ef = pow(10.0, f0 + f1 * ltp + f2 * ltp * ltp);
+    const double off = 1E-26 * Ne * kK * T * ef * nhi;
+    const double o = obf + off;
+    const double B = planck(lh, T);
+    K[0] += o;
+    E[0] += o * B;
+    double d[2];
+    d[0] = obf * (1.0 / T + 2.5 * 1.0 / TP * (-5040.0 / T / T)) + obf * dNe[0] / Ne + obf * dNH[0] / nhi +
+           obf * (-2.302585 * 0.754 * 5040.0 / T / T) + obf * (-stim * kH * kC / lh / kK / T / T) / (1.0 - stim);
+    d[1] = obf * dNe[1] / Ne + obf * dNH[1] / nhi;
+    const double dTP = -TP / T;
+    const double dltp = 1.0 / 2.302585 / TP * dTP;
+    const double def = ef * 2.302585 * (f1 * dltp + 2.0 * f2 * ltp * dltp);
+    const double cf = 1E-26 * kK;
+    d[0] += cf * Ne * ef * nhi + cf * T * Ne * nhi * def + cf * T * ef * (Ne * dNH[0] + nhi * dNe[0]);
+    d[1] += cf * T * ef * (Ne * dNH[1] + nhi * dNe[1]);
+    for (int p = 0; p < 2; ++p) {
+      dK[p][0] += d[p];
+      dE[p][0] += B * d[p] + (p == 0 ? o * planck_dT(lh, T) : 0.0);
+    }
+  }
+  const double vt = S_(SNAPI_F_VT), vl = S_(SNAPI_F_VLOS), bm = S_(SNAPI_F_BMAG);
+  double thb = S_(SNAPI_F_BINC) * kPi / 180.0, phb = S_(SNAPI_F_BAZI) * kPi / 180.0;
+  if (bm < 0.001) thb = phb = 0.0;
+  const double sp = sin(2.0 * phb), cp = cos(2.0 * phb), sn = sin(thb), cs = cos(thb);
+  for (int il = 0; il < L.nline; ++il) {
+    const KLine &ln = L.l[il];
+    const double l0 = ln.lam0;
+    const double nlo = S_(SNAPI_NF_BASE + 2 * il), nup = S_(SNAPI_NF_BASE + 2 * il + 1);
+    const double dvd = sqrt(ln.twok_m * T + vt * vt);
+    const double dld = dvd * l0 / kC;
+    const double vm = sqrt(ln.vmfac * T);
+    const double vpow = pow(vm / 1E6, -ln.alpha);
+    const double gc = nhi * vm * vpow * ln.sigma;
+    const double av = ln.sf * (ln.grad + gc) / dld;
+    const double opl = nlo * ln.opf_l - nup * ln.opf_u;
+    const double eml = nup * ln.emf;
+    double dopl[2], deml[2];
+    for (int p = 0; p < 2; ++p) {
+      dopl[p] = D_(p, SNAPI_NF_BASE + 2 * il) * ln.opf_l - D_(p, SNAPI_NF_BASE + 2 * il + 1) * ln.opf_u;
+      deml[p] = D_(p, SNAPI_NF_BASE + 2 * il + 1) * ln.emf;
+    }
+    const double ddld_T = l0 / (kC * dvd) * (0.5 * ln.twok_m);
+    const double ddld_vt = l0 / (kC * dvd) * vt;
+    const double vmd = 0.5 / vm * ln.vmfac;
+    double dgc_T = nhi * ln.sigma * (vmd * vpow - vm * ln.alpha * pow(vm / 1E6, -ln.alpha - 1.0) * vmd / 1E6);
+    dgc_T += dNH[0] * vm * vpow * ln.sigma;
+    const double a_T = -av / dld * ddld_T + dgc_T / dld * ln.sf;
+    const double a_N = (gc * 1E-3 / dld * ln.sf) / (1E-3 * Nt);
+    const double a_vt = -av / dld * ddld_vt;
+    const double lshift = lam * (1.0 + vl / kC);
+    if (!ln.split) {
+      double H, F, dH, dF;
+      const double x = (l0 - lshift) / dld;
+      voigt_d<double>(x, av, H, F, dH, dF);
+      K[0] += opl * H / dld;
+      E[0] += eml * H / dld;
+      const double xd[4] = {-x / dld * ddld_T, 0.0, -x / dld * ddld_vt, -lam / kC / dld};
+      const double ad[4] = {a_T, a_N, a_vt, 0.0};
+      const double dd[4] = {ddld_T, 0.0, ddld_vt, 0.0};
+      for (int p = 0; p < 4; ++p) {
+        const double g = (dH * xd[p] - dF * ad[p] - H / dld * dd[p]) / dld;
+        dK[p][0] += opl * g;
+        dE[p][0] += eml * g;
+      }
+      for (int p = 0; p < 2; ++p) {
+        dK[p][0] += dopl[p] * H / dld;
+        dE[p][0] += deml[p] * H / dld;
+      }
+      continue;
+    }
+    const double lB = ln.lbfac * bm;
+    double Hs[3] = {0, 0, 0}, Fs[3] = {0, 0, 0}, Hd[3][5], Fd[3][5];
+    for (int g = 0; g < 3; ++g)
+      for (int p = 0; p < 5; ++p) Hd[g][p] = Fd[g][p] = 0.0;
+    int c = ln.c0;
+    for (int g = 0; g < 3; ++g)
+      for (int m = 0; m < ln.nc[g]; ++m, ++c) {
+        const double sh = L.shift[c], sw = L.str[c];
+        const double x = (l0 + sh * lB - lshift) / dld;
+        double H, F, dH, dF;
+        voigt_d<double>(x, av, H, F, dH, dF);
+        Hs[g] += H * sw;
+        Fs[g] += F * sw;
+        const double xd[5] = {-x / dld * ddld_T, 0.0, -x / dld * ddld_vt, -lam / kC / dld, sh * ln.lbfac / dld};
+        const double ad[5] = {a_T, a_N, a_vt, 0.0, 0.0};
+        for (int p = 0; p < 5; ++p) {
+          Hd[g][p] += (dH * xd[p] - dF * ad[p]) * sw;
+          Fd[g][p] += (dF * xd[p] + dH * ad[p]) * sw;
+        }
+      }
+    for (int g = 0; g < 3; ++g) {
+      Hs[g] /= dld;
+      Fs[g] /= dld;
+      for (int p = 0; p < 5; ++p) {
+        Hd[g][p] /= dld;
+        Fd[g][p] /= dld;
+      }
+      Hd[g][0] -= ddld_T / dld * Hs[g];
+      Hd[g][2] -= ddld_vt / dld * Hs[g];
+      Fd[g][0] -= ddld_T / dld * Fs[g];
+      Fd[g][2] -= ddld_vt / dld * Fs[g];
+    }
+    // pattern geometry (atom_rad_synthesis.cc:486-497) applied to (H, F)
+    auto geom = [&](const double *H3, const double *F3, double fo, double fe, double *KK, double *EE) {
+      const double eI = 0.5 * (H3[0] * sn * sn + 0.5 * (H3[2] + H3[1]) * (1.0 + cs * cs));
+      const double hl = H3[0] - 0.5 * (H3[2] + H3[1]), fl = F3[0] - 0.5 * (F3[2] + F3[1]);
+      const double eQ = 0.5 * hl * sn * sn * cp, eU = 0.5 * hl * sn * sn * sp, eV = 0.5 * (H3[2] - H3[1]) * cs;
+      KK[0] += eI * fo;
+      KK[1] += eQ * fo;
+      KK[2] += eU * fo;
+      KK[3] += eV * fo;
+      KK[4] += 0.5 * fl * sn * sn * cp * fo;
+      KK[5] += 0.5 * fl * sn * sn * sp * fo;
+      KK[6] += 0.5 * (F3[2] - F3[1]) * cs * fo;
+      EE[0] += eI * fe;
+      EE[1] += eQ * fe;
+      EE[2] += eU * fe;
+      EE[3] += eV * fe;
+    };
+    geom(Hs, Fs, opl, eml, K, E);
+    for (int p = 0; p < 2; ++p) geom(Hs, Fs, dopl[p], deml[p], dK[p], dE[p]);
+    for (int p = 0; p < 5; ++p) {
+      const double H3[3] = {Hd[0][p], Hd[1][p], Hd[2][p]}, F3[3] = {Fd[0][p], Fd[1][p], Fd[2][p]};
+      geom(H3, F3, opl, eml, dK[p], dE[p]);
+    }
+    // inclination (rho_V term as the reference writes it, :999) and azimuth
+    const double Hp = Hs[0], Hb = Hs[1], Hr = Hs[2], Fp = Fs[0], Fb = Fs[1], Fr = Fs[2];
+    const double hl = Hp - 0.5 * (Hr + Hb), fl = Fp - 0.5 * (Fr + Fb);
+    const double gI = 0.5 * (Hp * 2.0 * sn * cs - 0.5 * (Hr + Hb) * (2.0 * cs * sn));
+    dK[5][0] += gI * opl;
+    dK[5][1] += 0.5 * hl * 2.0 * sn * cs * cp * opl;
+    dK[5][2] += 0.5 * hl * 2.0 * sn * cs * sp * opl;
+    dK[5][3] += -0.5 * (Hr - Hb) * sn * opl;
+    dK[5][4] += 0.5 * fl * 2.0 * cs * sn * cp * opl;
+    dK[5][5] += 0.5 * fl * 2.0 * cs * sn * sp * opl;
+    dK[5][6] += 0.5 * (Fr - Fb) * cs * opl;
+    dE[5][0] += gI * eml;
+    dE[5][1] += 0.5 * hl * 2.0 * sn * cs * cp * eml;
+    dE[5][2] += 0.5 * hl * 2.0 * sn * cs * sp * eml;
+    dE[5][3] += -0.5 * (Hr - Hb) * sn * eml;
+    dK[6][1] += -0.5 * hl * sn * sn * 2.0 * sp * opl;
+    dK[6][2] += 0.5 * hl * sn * sn * 2.0 * cp * opl;
+    dK[6][4] += -0.5 * fl * sn * sn * 2.0 * sp * opl;
+    dK[6][5] += 0.5 * fl * sn * sn * 2.0 * cp * opl;
+    dE[6][1] += -0.5 * hl * sn * sn * 2.0 * sp * eml;
+    dE[6][2] += 0.5 * hl * sn * sn * 2.0 * cp * eml;
+  }
+  if (a.tau_grid) {
+    const double r = S_(SNAPI_F_OPREF);
+    const double dr[2] = {D_(0, SNAPI_F_OPREF), D_(1, SNAPI_F_OPREF)};
+    for (int p = 0; p < 7; ++p) {
+      const double drp = p < 2 ? dr[p] : 0.0;
+      for (int i = 0; i < 7; ++i) dK[p][i] = dK[p][i] / r - K[i] * drp / r / r;
+      for (int i = 0; i < 4; ++i) dE[p][i] = dE[p][i] / r - E[i] * drp / r / r;
+    }
+  }
+#undef S_
+#undef D_
+}
+
+// derivatives of the linear weights with respect to the step (both branches)
+__device__ __forceinline__ void sc_weights_d(double tb, double &detb, double &dw0, double &dw1) {
+  if (tb < 1E-2) {
+    detb = -1.0 + tb - 0.5 * tb * tb;
+    dw0 = 0.5 - 2.0 * tb / 3.0 + 3.0 * tb * tb / 8.0;
+    dw1 = 0.5 - tb / 3.0 + tb * tb / 8.0;
+  } else {
+    const double e = exp(-tb), i = 1.0 / tb;
+    detb = -e;
+    dw0 = -i * i + e * (1.0 + i) + e * i * i;
+    dw1 = i * i - e * i - e * i * i;
+  }
+}
+
+// Fritsch-Butland derivative and its partials with respect to (om, o0, op)
+__device__ __forceinline__ double fb_deriv_d(double om, double o0, double op, double h1, double h2, double &gm,
+                                             double &g0, double &gp) {
+  const double d2 = (op - o0) / h2, d1 = (o0 - om) / h1;
+  gm = g0 = gp = 0.0;
+  if (d1 * d2 > 0) {
+    const double al = (1.0 + h1 / (h1 + h2)) / 3.0;
+    const double den = al * d1 + (1.0 - al) * d2;
+    const double D = d1 * d2 / den;
+    const double dD1 = d2 * d2 * (1.0 - al) / (den * den);
+    const double dD2 = d1 * d1 * al / (den * den);
+    gm = -dD1 / h1;
+    g0 = dD1 / h1 - dD2 / h2;
+    gp = dD2 / h2;
+    return D;
+  }
+  return 0.0;
+}
+
+// adjoint of y = K~ x with respect to the six reduced-matrix elements
+__device__ __forceinline__ void kmul_adj(const double yb[4], const double x[4], double kb[6]) {
+  kb[0] += yb[0] * x[1] + yb[1] * x[0];  // a
+  kb[1] += yb[0] * x[2] + yb[2] * x[0];  // b
+  kb[2] += yb[0] * x[3] + yb[3] * x[0];  // c
+  kb[3] += yb[1] * x[2] - yb[2] * x[1];  // d
+  kb[4] += -yb[1] * x[3] + yb[3] * x[1]; // e
+  kb[5] += yb[2] * x[3] - yb[3] * x[2];  // f
+}
+
+__global__ __launch_bounds__(kRespThreads) void resp_kernel(const KLines lines, const SynthArgs args,
+                                                           const RespArgs ra, const double *__restrict__ state,
+                                                           const double *__restrict__ dstate,
+                                                           const double *__restrict__ lam, Scratch scr,
+                                                           double *__restrict__ stokes, double *__restrict__ out,
+                                                           int *__restrict__ status) {
+  extern __shared__ unsigned char smem_raw[];
+  double *sm = reinterpret_cast<double *>(smem_raw);
+  const int nd = args.nd, nl = args.nlambda, nf = args.nfield;
+  const int cl = blockIdx.x;  // column within the launch
+  const int col = ra.col0 + cl;
+  const double *st = state + (size_t)col * nf * nd;
+  const double *dst = dstate + (size_t)col * 2 * nf * nd;
+  ColumnTile<double> tile = ColumnTile<double>::carve(sm, nd, lines.nline);
+  stage_column<double, double>(lines, args, st, tile);
+  __syncthreads();
+  const bool up = args.ct < 0.0;
+  const double mct = -args.ct;
+  auto zof = [&](int k) { return up ? nd - 1 - k : k; };
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    const long long w = (long long)cl * nl + l;
+    const double lm = lam[l];
+    LamConst<double> c;
+    lambda_consts<double>(lines, args, lm, c);
+    // ---------------- forward: opacity of every depth, then the recurrence
+    for (int k = 0; k < nd; ++k) {
+      double K[7], E[4];
+      opacity_at<double>(lines, args, tile, c, zof(k), K, E);
+      for (int i = 0; i < 7; ++i) scr.fwd(k, i, w) = K[i];
+      for (int i = 0; i < 4; ++i) scr.fwd(k, 7 + i, w) = E[i];
+      for (int i = 0; i < kAdj; ++i)
+        for (int s = 0; s < ra.nseed; ++s) scr.adj(k, i, s, w) = 0.0;
+    }
+    auto load = [&](int k, double K[7], double E[4]) {
+      for (int i = 0; i < 7; ++i) K[i] = scr.fwd(k, i, w);
+      for (int i = 0; i < 4; ++i) E[i] = scr.fwd(k, 7 + i, w);
+    };
+    auto hof = [&](int k) { return (tile.grid[zof(k)] - tile.grid[zof(k - 1)]) / mct; };
+    auto ooof = [&](int k) { return scr.fwd(k, 0, w); };
+    auto Dof = [&](int k) {  // D_k, 0 <= k <= nd-2
+      if (k == 0) return (ooof(1) - ooof(0)) / hof(1);
+      return fb_deriv<double>(ooof(k - 1), ooof(k), ooof(k + 1), hof(k), hof(k + 1));
+    };
+    {
+      double K0[7], E0[4];
+      load(0, K0, E0);
+      double I[4] = {up ? E0[0] / K0[0] : 0.0, 0.0, 0.0, 0.0};
+      for (int s = 0; s < 4; ++s) scr.fwd(0, 11 + s, w) = I[s];
+      for (int k = 1; k <= nd - 2; ++k) {
+        double Kp[7], Ep[4], Kc[7], Ec[4];
+        load(k - 1, Kp, Ep);
+        load(k, Kc, Ec);
+        Kred<double> kp, kc;
+        double Sp[4], Sc[4];
+        reduce_ks<double>(Kp, Ep, kp, Sp);
+        reduce_ks<double>(Kc, Ec, kc, Sc);
+        const double h = hof(k);
+        const double tb = h * 0.5 * (Kp[0] + Kc[0]) + h * h / 12.0 * (Dof(k - 1) - Dof(k));
+        double etb, w0, w1;
+        sc_weights<double>(tb, etb, w0, w1);
+        double kv[4], B[4], In[4];
+        kmul<double>(kp, I, kv);
+        for (int s = 0; s < 4; ++s) B[s] = -w0 * kv[s] + w0 * Sp[s] + w1 * Sc[s] + I[s] * etb;
+        solve_1pk<double>(kc, w1, B, In);
+        for (int s = 0; s < 4; ++s) {
+          I[s] = In[s];
+          scr.fwd(k, 11 + s, w) = In[s];
+        }
+      }
+      const int ko = nd >= 3 ? nd - 2 : 0;
+      double Io[4];
+      for (int s = 0; s < 4; ++s) Io[s] = scr.fwd(ko, 11 + s, w);
+      for (int s = 0; s < 4; ++s) stokes[((size_t)col * nl + l) * 4 + s] = Io[s];
+      if (!(fabs(Io[0]) < 1e300)) atomicOr(status, 2);
+      // ---------------- reverse sweep
+      const int ns = ra.nseed;
+      double Ib[4][4];  // [seed][component]
+      for (int s = 0; s < 4; ++s)
+        for (int q = 0; q < 4; ++q) Ib[s][q] = 0.0;
+      if (ra.sink == SINK_RESP) {
+        for (int s = 0; s < 4; ++s) Ib[s][s] = 1.0;
+      } else {
+        const double *ob = ra.obs + ((size_t)col * nl + l) * 4;
+        for (int q = 0; q < 4; ++q) Ib[0][q] = ra.ws[q] * (Io[q] - ob[q]);
+      }
+      // carried adjoints of the reduced quantities of depth k (from step k+1)
+      double Kr_b[4][6], S_b[4][4], D_b[4];
+      for (int s = 0; s < 4; ++s) {
+        for (int q = 0; q < 6; ++q) Kr_b[s][q] = 0.0;
+        for (int q = 0; q < 4; ++q) S_b[s][q] = 0.0;
+        D_b[s] = 0.0;
+      }
+      // converts the adjoint of (K~, S) of depth k into raw (K, E) adjoints
+      auto flush_reduced = [&](int k, const double Kr[4][6], const double Sb[4][4]) {
+        double K[7], E[4];
+        load(k, K, E);
+        const double iI = 1.0 / K[0];
+        // a..f <- K[1], K[2], K[3], K[6], K[5], K[4]
+        const int map[6] = {1, 2, 3, 6, 5, 4};
+        for (int s = 0; s < ns; ++s) {
+          double kI = 0.0;
+          for (int q = 0; q < 6; ++q) {
+            scr.adj(k, map[q], s, w) += Kr[s][q] * iI;
+            kI -= Kr[s][q] * K[map[q]] * iI * iI;
+          }
+          for (int q = 0; q < 4; ++q) {
+            scr.adj(k, 7 + q, s, w) += Sb[s][q] * iI;
+            kI -= Sb[s][q] * E[q] * iI * iI;
+          }
+          scr.adj(k, 0, s, w) += kI;
+        }
+      };
+      // propagates the adjoint of D_k into the opacities it is made of
+      auto flush_D = [&](int k, const double Db[4]) {
+        if (k == 0) {
+          const double h1 = hof(1);
+          for (int s = 0; s < ns; ++s) {
+            scr.adj(1, 0, s, w) += Db[s] / h1;
+            scr.adj(0, 0, s, w) -= Db[s] / h1;
+          }
+          return;
+        }
+        double gm, g0, gp;
+        fb_deriv_d(ooof(k - 1), ooof(k), ooof(k + 1), hof(k), hof(k + 1), gm, g0, gp);
+        for (int s = 0; s < ns; ++s) {
+          scr.adj(k - 1, 0, s, w) += Db[s] * gm;
+          scr.adj(k, 0, s, w) += Db[s] * g0;
+          scr.adj(k + 1, 0, s, w) += Db[s] * gp;
+        }
+      };
+      for (int k = nd - 2; k >= 1; --k) {
+        double Kp[7], Ep[4], Kc[7], Ec[4];
+        load(k - 1, Kp, Ep);
+        load(k, Kc, Ec);
+        Kred<double> kp, kc;
+        double Sp[4], Sc[4];
+        reduce_ks<double>(Kp, Ep, kp, Sp);
+        reduce_ks<double>(Kc, Ec, kc, Sc);
+        const double h = hof(k);
+        const double Dm = Dof(k - 1), D0 = Dof(k);
+        const double tb = h * 0.5 * (Kp[0] + Kc[0]) + h * h / 12.0 * (Dm - D0);
+        double etb, w0, w1, detb, dw0, dw1;
+        sc_weights<double>(tb, etb, w0, w1);
+        sc_weights_d(tb, detb, dw0, dw1);
+        double Ip[4], Ik[4];
+        for (int q = 0; q < 4; ++q) {
+          Ip[q] = scr.fwd(k - 1, 11 + q, w);
+          Ik[q] = scr.fwd(k, 11 + q, w);
+        }
+        double kvp[4], kvc[4];
+        kmul<double>(kp, Ip, kvp);
+        kmul<double>(kc, Ik, kvc);
+        // transposed system: (1 + w1 K~)^T = 1 + w1 K~' with d, e, f negated
+        const Kred<double> kcT = {kc.a, kc.b, kc.c, -kc.d, -kc.e, -kc.f};
+        const Kred<double> kpT = {kp.a, kp.b, kp.c, -kp.d, -kp.e, -kp.f};
+        double Krc_b[4][6], Sc_b[4][4], Krp_b[4][6], Sp_b[4][4], D0_b[4], Dm_b[4];
+        for (int s = 0; s < ns; ++s) {
+          double Bb[4];
+          solve_1pk<double>(kcT, w1, Ib[s], Bb);
+          // A = 1 + w1 K~_k :  Abar = -Bb (x) I_k
+          double kb[6] = {0, 0, 0, 0, 0, 0};
+          double nb[4] = {-w1 * Bb[0], -w1 * Bb[1], -w1 * Bb[2], -w1 * Bb[3]};
+          kmul_adj(nb, Ik, kb);
+          double w1b = -(Bb[0] * kvc[0] + Bb[1] * kvc[1] + Bb[2] * kvc[2] + Bb[3] * kvc[3]);
+          w1b += Bb[0] * Sc[0] + Bb[1] * Sc[1] + Bb[2] * Sc[2] + Bb[3] * Sc[3];
+          double w0b = 0.0, etbb = 0.0;
+          for (int q = 0; q < 4; ++q) {
+            w0b += Bb[q] * (Sp[q] - kvp[q]);
+            etbb += Bb[q] * Ip[q];
+          }
+          for (int q = 0; q < 6; ++q) Krc_b[s][q] = Kr_b[s][q] + kb[q];
+          for (int q = 0; q < 4; ++q) Sc_b[s][q] = S_b[s][q] + w1 * Bb[q];
+          // B = -w0 K~_{k-1} I_{k-1} + w0 S_{k-1} + w1 S_k + etb I_{k-1}
+          double vb[4] = {-w0 * Bb[0], -w0 * Bb[1], -w0 * Bb[2], -w0 * Bb[3]};
+          double kbp[6] = {0, 0, 0, 0, 0, 0};
+          kmul_adj(vb, Ip, kbp);
+          double Ipb[4];
+          kmul<double>(kpT, vb, Ipb);
+          for (int q = 0; q < 4; ++q) Ib[s][q] = Ipb[q] + etb * Bb[q];
+          for (int q = 0; q < 6; ++q) Krp_b[s][q] = kbp[q];
+          for (int q = 0; q < 4; ++q) Sp_b[s][q] = w0 * Bb[q];
+          const double tbb = w0b * dw0 + w1b * dw1 + etbb * detb;
+          // tb = h/2 (oo_{k-1} + oo_k) + h^2/12 (D_{k-1} - D_k)
+          scr.adj(k - 1, 0, s, w) += tbb * h * 0.5;
+          scr.adj(k, 0, s, w) += tbb * h * 0.5;
+          Dm_b[s] = tbb * h * h / 12.0;
+          D0_b[s] = D_b[s] - tbb * h * h / 12.0;
+        }
+        // depth k is complete for K~, S: flush; D_k is complete
+        flush_reduced(k, Krc_b, Sc_b);
+        flush_D(k, D0_b);
+        for (int s = 0; s < ns; ++s) {
+          for (int q = 0; q < 6; ++q) Kr_b[s][q] = Krp_b[s][q];
+          for (int q = 0; q < 4; ++q) S_b[s][q] = Sp_b[s][q];
+          D_b[s] = Dm_b[s];
+        }
+      }
+      // bottom point: boundary I_0 = (E_0[0]/K_0[0], 0, 0, 0) for the outgoing ray
+      if (nd >= 3) flush_D(0, D_b);
+      flush_reduced(0, Kr_b, S_b);
+      if (up) {
+        double K0b[7], E0b[4];
+        load(0, K0b, E0b);
+        for (int s = 0; s < ns; ++s) {
+          scr.adj(0, 7, s, w) += Ib[s][0] / K0b[0];
+          scr.adj(0, 0, s, w) -= Ib[s][0] * E0b[0] / (K0b[0] * K0b[0]);
+        }
+      }
+    }
+    // ---------------- contraction with the local derivatives
+    for (int k = 0; k < nd; ++k) {
+      const int z = zof(k);
+      double dK[7][7], dE[7][4];
+      local_derivs(lines, args, st, dst, z, lm, dK, dE);
+      for (int s = 0; s < ra.nseed; ++s) {
+        double Kb[7], Eb[4];
+        for (int i = 0; i < 7; ++i) Kb[i] = scr.adj(k, i, s, w);
+        for (int i = 0; i < 4; ++i) Eb[i] = scr.adj(k, 7 + i, s, w);
+        for (int p = 0; p < 7; ++p) {
+          double g = 0.0;
+          for (int i = 0; i < 7; ++i) g += Kb[i] * dK[p][i];
+          for (int i = 0; i < 4; ++i) g += Eb[i] * dE[p][i];
+          if (ra.sink == SINK_RESP)
+            out[((((size_t)col * 7 + p) * nd + z) * nl + l) * 4 + s] = g;
+          else
+            atomicAdd(&out[((size_t)col * 7 + p) * nd + z], g);
+        }
+      }
+    }
+  }
+}
+
+static int run_resp(const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd, const void *state,
+                    const void *dstate, int nlambda, const double *lam, void *stokes, double *out_host, int sink,
+                    const double *obs, const double *ws) {
+  KLines kl;
+  SynthArgs a;
+  int rc = prepare_synth(lines, ray, ncol, nd, nlambda, kl, a);
+  if (rc) return rc;
+  if (!state || !dstate || !lam || !stokes || !out_host) return SNAPI_ERR_ARG;
+  if (sink == SINK_GRAD && (!obs || !ws)) return SNAPI_ERR_ARG;
+  const size_t nf = a.nfield;
+  const size_t sb = (size_t)ncol * nf * nd * 8, db = 2 * sb, lb = (size_t)nlambda * 8;
+  const size_t stb = (size_t)ncol * nlambda * 4 * 8;
+  const size_t ob = sink == SINK_RESP ? (size_t)ncol * 7 * nd * nlambda * 4 * 8 : (size_t)ncol * 7 * nd * 8;
+  // columns per launch so that the scratch stays below ~2 GiB
+  const size_t per_col = (size_t)nlambda * nd * (kFwd + kAdj * 4) * 8;
+  size_t chunk_s = ((size_t)2 << 30) / (per_col > 0 ? per_col : 1);
+  if (chunk_s < 1) chunk_s = 1;
+  if (chunk_s > (size_t)ncol) chunk_s = (size_t)ncol;
+  const int chunk = (int)chunk_s;
+  double *ds = nullptr, *dd = nullptr, *dl = nullptr, *dstk = nullptr, *dout = nullptr, *dscr = nullptr,
+         *dobs = nullptr;
+  rc = SNAPI_OK;
+  if (hipMalloc((void **)&ds, sb) != hipSuccess || hipMalloc((void **)&dd, db) != hipSuccess ||
+      hipMalloc((void **)&dl, lb) != hipSuccess || hipMalloc((void **)&dstk, stb) != hipSuccess ||
+      hipMalloc((void **)&dout, ob) != hipSuccess || hipMalloc((void **)&dscr, per_col * chunk) != hipSuccess ||
+      (sink == SINK_GRAD && hipMalloc((void **)&dobs, stb) != hipSuccess)) {
+    rc = SNAPI_ERR_DEVICE;
+  } else {
+    hipMemcpy(ds, state, sb, hipMemcpyHostToDevice);
+    hipMemcpy(dd, dstate, db, hipMemcpyHostToDevice);
+    hipMemcpy(dl, lam, lb, hipMemcpyHostToDevice);
+    hipMemset(dout, 0, ob);
+    if (dobs) hipMemcpy(dobs, obs, stb, hipMemcpyHostToDevice);
+    reset_status();
+    RespArgs ra;
+    ra.nseed = sink == SINK_RESP ? 4 : 1;
+    ra.sink = sink;
+    ra.obs = dobs;
+    for (int s = 0; s < 4; ++s) ra.ws[s] = ws ? ws[s] : 1.0;
+    int threads = ((nlambda + 63) / 64) * 64;
+    if (threads > kRespThreads) threads = kRespThreads;
+    const size_t smem = ColumnTile<double>::bytes(nd, kl.nline);
+    int *status = device_status_slot();
+    for (int c0 = 0; c0 < ncol && rc == SNAPI_OK; c0 += chunk) {
+      const int nc = (ncol - c0 < chunk) ? ncol - c0 : chunk;
+      ra.col0 = c0;
+      ra.nwork = (long long)nc * nlambda;
+      Scratch scr{dscr, ra.nwork, nd};
+      hipLaunchKernelGGL(resp_kernel, dim3(nc), dim3(threads), smem, 0, kl, a, ra, ds, dd, dl, scr, dstk, dout,
+                         status);
+      if (hipGetLastError() != hipSuccess) rc = SNAPI_ERR_DEVICE;
+    }
+    if (rc == SNAPI_OK && hipDeviceSynchronize() != hipSuccess) rc = SNAPI_ERR_DEVICE;
+    if (rc == SNAPI_OK) {
+      hipMemcpy(stokes, dstk, stb, hipMemcpyDeviceToHost);
+      hipMemcpy(out_host, dout, ob, hipMemcpyDeviceToHost);
+      rc = read_status();
+    }
+  }
+  hipFree(ds);
+  hipFree(dd);
+  hipFree(dl);
+  hipFree(dstk);
+  hipFree(dout);
+  hipFree(dscr);
+  hipFree(dobs);
+  return rc;
+}
+
+}  // namespace snapi
+
+using namespace snapi;
+
+extern "C" int snapi_stokes_responses(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol,
+                                      int nd, const void *state, const void *dstate, int nlambda, const double *lam,
+                                      void *stokes, void *resp) {
+  if (precision != SNAPI_FP64) return SNAPI_ERR_UNSUPPORTED;
+  return run_resp(lines, ray, ncol, nd, state, dstate, nlambda, lam, stokes, static_cast<double *>(resp), SINK_RESP,
+                  nullptr, nullptr);
+}
+
+extern "C" int snapi_stokes_adjoint(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol,
+                                    int nd, const void *state, const void *dstate, int nlambda, const double *lam,
+                                    const double *obs, const double *wstokes, double *grad) {
+  if (precision != SNAPI_FP64) return SNAPI_ERR_UNSUPPORTED;
+  if (!grad) return SNAPI_ERR_ARG;
+  const size_t stb = (size_t)ncol * nlambda * 4;
+  double *tmp = new double[stb > 0 ? stb : 1];
+  int rc = run_resp(lines, ray, ncol, nd, state, dstate, nlambda, lam, tmp, grad, SINK_GRAD, obs, wstokes);
+  delete[] tmp;
+  return rc;
+}

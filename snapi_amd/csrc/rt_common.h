@@ -210,3 +210,98 @@ struct KLines {
 };
 
 }  // namespace snapi
+
+namespace snapi {
+
+// W4 values and the derivative of each region's expression with respect to
+// v (the analytic derivative of the approximant, as cmlib/profiles.cc:39-89
+// evaluates it); dH/da = -dF/dv and dF/da = dH/dv since w is analytic.
+template <typename R> __device__ __forceinline__ void voigt_d(R v, R a, R &H, R &F, R &dH, R &dF) {
+  const R av = v < R(0) ? -v : v;
+  const R s = av + a;
+  const cplx<R> t = cm<R>(a, -v), dt = cm<R>(R(0), R(-1));
+  const cplx<R> u = cm<R>(a * a - v * v, R(-2) * a * v), du = cm<R>(R(-2) * v, R(-2) * a);
+  cplx<R> w, dw;
+  if (s >= R(15)) {
+    const cplx<R> den = addr(u, R(0.5));
+    const cplx<R> num = scl(t, R(0.5641896));
+    w = cdiv(num, den);
+    const cplx<R> a1 = cdiv(scl(dt, R(0.5641896)), den);
+    const cplx<R> a2 = mul(cdiv(w, den), du);
+    dw = cm<R>(a1.re - a2.re, a1.im - a2.im);
+  } else if (s > R(5.5)) {
+    const cplx<R> num = addr(scl(u, R(0.5641896)), R(1.410474));
+    const cplx<R> den = addr(mul(u, addr(u, R(3.0))), R(0.75));
+    const cplx<R> tdu = mul(t, du);
+    w = cdiv(mul(t, num), den);
+    const cplx<R> a1 = cdiv(mul(dt, num), den);
+    const cplx<R> a2 = cdiv(scl(tdu, R(0.5641896)), den);
+    const cplx<R> a3 = mul(cdiv(cdiv(mul(tdu, num), den), den), addr(scl(u, R(2)), R(3.0)));
+    dw = cm<R>(a1.re + a2.re - a3.re, a1.im + a2.im - a3.im);
+  } else if (a > (R(0.195) * av - R(0.176))) {
+    cplx<R> up = addr(scl(t, R(0.5642236)), R(3.778987));
+    up = addr(mul(t, up), R(11.96482));
+    up = addr(mul(t, up), R(20.20933));
+    up = addr(mul(t, up), R(16.4955));
+    cplx<R> lo = addr(t, R(6.699398));
+    lo = addr(mul(t, lo), R(21.69274));
+    lo = addr(mul(t, lo), R(39.27121));
+    lo = addr(mul(t, lo), R(38.82363));
+    lo = addr(mul(t, lo), R(16.4955));
+    w = cdiv(up, lo);
+    // d(up)/dt and d(lo)/dt in Horner form
+    cplx<R> dup = addr(scl(t, R(0.5642236 * 4.0)), R(3.778987 * 3.0));
+    dup = addr(mul(t, dup), R(11.96482 * 2.0));
+    dup = addr(mul(t, dup), R(20.20933));
+    cplx<R> dlo = addr(scl(t, R(5.0)), R(6.699398 * 4.0));
+    dlo = addr(mul(t, dlo), R(21.69274 * 3.0));
+    dlo = addr(mul(t, dlo), R(39.27121 * 2.0));
+    dlo = addr(mul(t, dlo), R(38.82363));
+    const cplx<R> nn = mul(dup, lo), mm = mul(dlo, up);
+    dw = mul(dt, cdiv(cdiv(cm<R>(nn.re - mm.re, nn.im - mm.im), lo), lo));
+  } else {
+    cplx<R> P = rsub(R(1.320522), scl(u, R(0.56419)));
+    P = rsub(R(35.76683), mul(u, P));
+    P = rsub(R(219.0313), mul(u, P));
+    P = rsub(R(1540.787), mul(u, P));
+    P = rsub(R(3321.9905), mul(u, P));
+    P = rsub(R(36183.31), mul(u, P));
+    const cplx<R> w4 = mul(t, P);
+    cplx<R> dP = rsub(R(1.320522 * 5.0), scl(u, R(0.56419 * 6.0)));
+    dP = rsub(R(35.76683 * 4.0), mul(u, dP));
+    dP = rsub(R(219.0313 * 3.0), mul(u, dP));
+    dP = rsub(R(1540.787 * 2.0), mul(u, dP));
+    dP = rsub(R(3321.9905), mul(u, dP));
+    const cplx<R> g1 = mul(dt, P), g2 = mul(mul(t, du), dP);
+    const cplx<R> dw4 = cm<R>(g1.re - g2.re, g1.im - g2.im);
+    cplx<R> Q = rsub(R(1.841439), u);
+    Q = rsub(R(61.57037), mul(u, Q));
+    Q = rsub(R(364.2191), mul(u, Q));
+    Q = rsub(R(2186.181), mul(u, Q));
+    Q = rsub(R(9022.228), mul(u, Q));
+    Q = rsub(R(24322.84), mul(u, Q));
+    Q = rsub(R(32066.6), mul(u, Q));
+    cplx<R> dQ = rsub(R(1.841439 * 6.0), scl(u, R(7.0)));
+    dQ = rsub(R(61.57037 * 5.0), mul(u, dQ));
+    dQ = rsub(R(364.2191 * 4.0), mul(u, dQ));
+    dQ = rsub(R(2186.181 * 3.0), mul(u, dQ));
+    dQ = rsub(R(9022.228 * 2.0), mul(u, dQ));
+    dQ = rsub(R(24322.84), mul(u, dQ));
+    dQ = scl(mul(du, dQ), R(-1));
+    const R su = rsin<R>(u.im);
+    const R co = rsqrt_<R>(R(1) - su * su);
+    const R dsu = (R)cos((double)u.im) * du.im;
+    const R e = rexp<R>(u.re);
+    const cplx<R> r = cdiv(w4, Q);
+    w = cm<R>(e * co - r.re, e * su - r.im);
+    const cplx<R> n1 = mul(dw4, Q), n2 = mul(dQ, w4);
+    const cplx<R> q2 = cdiv(cm<R>(n1.re - n2.re, n1.im - n2.im), mul(Q, Q));
+    dw = cm<R>(e * du.re * co + e * (-su * du.im) - q2.re, e * du.re * su + e * dsu - q2.im);
+  }
+  H = w.re * R(1.0 / kSqrtPi);
+  F = w.im * R(1.0 / kSqrtPi);
+  dH = dw.re * R(1.0 / kSqrtPi);
+  dF = dw.im * R(1.0 / kSqrtPi);
+}
+
+}  // namespace snapi

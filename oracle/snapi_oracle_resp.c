@@ -378,6 +378,11 @@ static void opem_pert(const snapi_line_table *lt, const snapi_ray *ray, int nd, 
 #undef DEM
 }
 
+/* Step multiplier of the central differences (1 = the reference's steps);
+ * smaller values approach the exact derivative, for checks only. */
+static double g_fd_scale = 1.0;
+void orc_set_fd_scale(double s) { g_fd_scale = s; }
+
 int orc_responses(const snapi_line_table *lt, const snapi_ray *ray, int ncol, int nd, const double *state_all,
                   const double *dstate_all, int nlambda, const double *lam, double *stokes, double *resp,
                   int nthreads) {
@@ -402,9 +407,9 @@ int orc_responses(const snapi_line_table *lt, const snapi_ray *ray, int ncol, in
       for (int zk = 0; zk < nd; ++zk) {
         /* the caller scales Nt and angle perturbations before the difference
          * and divides the result by the same factor */
-        double sc = 1.0;
-        if (p == 1) sc = ST(SNAPI_F_NT, zk) * DELTA_NT_FRAC;
-        if (p == 5 || p == 6) sc = DELTA_ANGLE;
+        double sc = g_fd_scale;
+        if (p == 1) sc *= ST(SNAPI_F_NT, zk) * DELTA_NT_FRAC;
+        if (p == 5 || p == 6) sc *= DELTA_ANGLE;
         const double *dk = dop + ((size_t)p * nd + zk) * 16, *de = dem + ((size_t)p * nd + zk) * 4;
         double res[4];
         for (int sgn = 0; sgn < 2; ++sgn) {

@@ -35,6 +35,7 @@ def lib():
         L.orc_synth.restype = C.c_int
         L.orc_synth.argtypes = [C.POINTER(SnapiLineTable), C.POINTER(SnapiRay), C.c_int, C.c_int, dp, C.c_int, dp,
                                 dp, C.c_int]
+        L.orc_set_fd_scale.argtypes = [C.c_double]
         if hasattr(L, "orc_responses"):
             L.orc_responses.restype = C.c_int
             L.orc_responses.argtypes = [C.POINTER(SnapiLineTable), C.POINTER(SnapiRay), C.c_int, C.c_int, dp, dp,
@@ -93,15 +94,17 @@ def synth(table, ray, state, lam, nthreads=0):
     return out, rc
 
 
-def responses(table, ray, state, dstate, lam, nthreads=0):
+def responses(table, ray, state, dstate, lam, nthreads=0, fd_scale=1.0):
     ncol, nf, nd = state.shape
     st = np.ascontiguousarray(state, dtype=np.float64)
     ds = np.ascontiguousarray(dstate, dtype=np.float64)
     lam = np.ascontiguousarray(lam, dtype=np.float64)
     out = np.zeros((ncol, len(lam), 4))
     resp = np.zeros((ncol, 7, nd, len(lam), 4))
+    lib().orc_set_fd_scale(fd_scale)
     rc = lib().orc_responses(C.byref(table), C.byref(ray), ncol, nd, ptr(st), ptr(ds), len(lam), ptr(lam), ptr(out),
                              ptr(resp), nthreads)
+    lib().orc_set_fd_scale(1.0)
     return out, resp, rc
 
 

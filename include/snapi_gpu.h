@@ -137,31 +137,35 @@ int snapi_synth_stokes_dev(int precision, const snapi_line_table *lines, const s
                            void *stream);
 
 /*
- * Per-depth responses d(IQUV)/d q_p(z) of the emergent Stokes vector, the
- * forward-mode (tangent) derivative of the synthesis above along the seven
- * physical parameters of the reference: p = 0 T, 1 Nt, 2 vt, 3 vLOS, 4 |B|,
- * 5 inclination, 6 azimuth (angles per radian).  dstate holds the
- * thermodynamic derivatives the reference computes by finite differences of
- * its equation of state (ne_lte_derivatives, level responses,
- * compute_op_referent_derivative): [ncol][2][nfield][nd], slice 0 = d/dT,
- * slice 1 = d/dNt, only the fields NE, NHI, OPREF and the populations are read.
- *   resp [ncol][7][nd][nlambda][4]
- * Only SNAPI_FP64 is implemented for this entry point.
+ * Per-depth responses d(IQUV)/d q_p(z) of the emergent Stokes vector along
+ * the seven physical parameters of the reference: p = 0 T, 1 Nt, 2 vt,
+ * 3 vLOS, 4 |B|, 5 inclination, 6 azimuth (angles per radian), computed by a
+ * reverse (adjoint) sweep of the formal solution with the four emergent
+ * Stokes components as seeds.  dstate holds the thermodynamic derivatives
+ * the reference obtains by differencing its equation of state
+ * (ne_lte_derivatives, LTE level responses, compute_op_referent_derivative):
+ * [ncol][2][nfield][nd], slice 0 = d/dT, slice 1 = d/dNt; the fields NE,
+ * NHI, OPREF and the populations are read.  For the reference's response
+ * path set ray->hminus_mid = 1.
+ *   stokes [ncol][nlambda][4], resp [ncol][7][nd][nlambda][4]
+ * All arrays in the precision's type (double for SNAPI_FP64, float for
+ * SNAPI_MIXED), lam in double.  Valid for the mu = 1 ray, where the ray and
+ * atmosphere frames coincide (atmosphere::transform_responses is the
+ * identity there).
  */
 int snapi_stokes_responses(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
                            const void *state, const void *dstate, int nlambda, const double *lam, void *stokes,
                            void *resp);
 
 /*
- * Gradient of chi^2/2 = sum_{l,s} w_s (S_obs - S)^2 / 2 w.r.t. the per-depth
- * parameters in one reverse sweep (adjoint of the synthesis): the product
- * J^T r of the per-depth Jacobian above with r = w_s (S - S_obs), without
- * forming J.
- *   obs   [ncol][nlambda][4]; wstokes[4]; grad [ncol][7][nd]
+ * Gradient of chi^2/2 = sum_{l,s} w_s (S - S_obs)^2 / 2 with respect to the
+ * per-depth parameters in one reverse sweep (one seed w_s (S - S_obs)), i.e.
+ * J^T r without forming J.
+ *   obs [ncol][nlambda][4], grad [ncol][7][nd] (precision's type); wstokes[4] double
  */
 int snapi_stokes_adjoint(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
-                         const void *state, const void *dstate, int nlambda, const double *lam, const double *obs,
-                         const double *wstokes, double *grad);
+                         const void *state, const void *dstate, int nlambda, const double *lam, const void *obs,
+                         const double *wstokes, void *grad);
 
 /*
  * One damped normal-equation solve per pixel (batched):
@@ -173,6 +177,38 @@ int snapi_stokes_adjoint(int precision, const snapi_line_table *lines, const sna
  */
 int snapi_lm_step(int precision, int npix, int nrow, int npar, const void *J, const void *r, const double *lm,
                   double *JTJ, double *rhs, double *corr);
+
+/* Node model of the inversion (the reference's model class, atmos/atmos.h:
+ * 377-531): one entry per node, parameter code as the reference orders them
+ * (1 T, 2 vt, 3 vLOS, 4 |B|, 5 inclination [rad], 6 azimuth [rad]) and the
+ * node's position in log tau_500. */
+typedef struct {
+  int nnode;
+  int param[SNAPI_MAX_NODES];
+  double logtau[SNAPI_MAX_NODES];
+} snapi_node_model;
+
+enum snapi_jacobian {
+  SNAPI_JAC_ADJOINT = 0,     /* reverse sweep contracted with the node responses */
+  SNAPI_JAC_DIFFERENCES = 1  /* central differences of the node values through the synthesis (2n+1 syntheses) */
+};
+
+/*
+ * Batched Levenberg-Marquardt inversion (atmosphere::stokes_lm_fit,
+ * atmos/atmos_fit.cc:18-227) of npix pixels for the kinematic and magnetic
+ * node parameters (codes 2..6), the thermodynamic state of each pixel held
+ * as given; temperature nodes are rejected (SNAPI_ERR_UNSUPPORTED).  mu = 1 ray.
+ *   state [npix][nfield][nd], obs [npix][nlambda][4], mask [npix][nlambda] (or
+ *   null: all 1) in the precision's type; nodes [npix][nnode] in: start
+ *   values, out: solution; chi2 [npix], iterations [npix] out.
+ * nslot pixels are iterated at once; a finished pixel's slot is refilled from
+ * the queue of the remaining ones.  pixel_iterations (optional) returns the
+ * total number of LM iterations over all pixels.
+ */
+int snapi_invert(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                 const snapi_node_model *model, int npix, int nd, const void *state, int nlambda, const double *lam,
+                 const void *obs, const void *mask, const double *wstokes, double lm_start, int max_iter, int nslot,
+                 double *nodes, double *chi2, int *iterations, long long *pixel_iterations);
 
 /*
  * Scalar formal solution with the diagonal approximate Lambda operator for a

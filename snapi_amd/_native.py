@@ -36,7 +36,7 @@ def lib():
         L.snapi_synth_stokes_dev.argtypes = [ip, LT, RY, ip, ip, vp, ip, vp, vp, vp]
         for name, args in (
                 ("snapi_stokes_responses", [ip, LT, RY, ip, ip, vp, vp, ip, dp, vp, vp]),
-                ("snapi_stokes_adjoint", [ip, LT, RY, ip, ip, vp, vp, ip, dp, dp, dp, dp]),
+                ("snapi_stokes_adjoint", [ip, LT, RY, ip, ip, vp, vp, ip, dp, vp, dp, vp]),
                 ("snapi_lm_step", [ip, ip, ip, ip, vp, vp, dp, dp, dp, dp]),
                 ("snapi_formal_scalar", [ip, ip, dp, dp, dp, dp, C.c_double, dp, dp])):
             if hasattr(L, name):

@@ -45,14 +45,15 @@ def stokes_responses(table, ray, state, dstate, lam_vac, precision=abi.SNAPI_FP6
     """Emergent Stokes [ncol][nlambda][4] and per-depth responses
     [ncol][7][nd][nlambda][4] (atmosphere::obs_stokes_responses,
     atmos/atmos_rad_synthesis.cc:166-399), by the adjoint sweep."""
-    st = np.ascontiguousarray(state, dtype=np.float64)
-    ds = np.ascontiguousarray(dstate, dtype=np.float64)
+    dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
+    st = np.ascontiguousarray(state, dtype=dt)
+    ds = np.ascontiguousarray(dstate, dtype=dt)
     if st.ndim == 2:
         st, ds = st[None], ds[None]
     ncol, nf, nd = st.shape
     lam = np.ascontiguousarray(lam_vac, dtype=np.float64)
-    stokes = np.zeros((ncol, len(lam), 4))
-    resp = np.zeros((ncol, 7, nd, len(lam), 4))
+    stokes = np.zeros((ncol, len(lam), 4), dtype=dt)
+    resp = np.zeros((ncol, 7, nd, len(lam), 4), dtype=dt)
     rc = lib().snapi_stokes_responses(precision, C.byref(table), C.byref(ray), ncol, nd, abi.vptr(st),
                                       abi.vptr(ds), len(lam), abi.ptr(lam), abi.vptr(stokes), abi.vptr(resp))
     check("snapi_stokes_responses", rc)
@@ -63,16 +64,17 @@ def stokes_adjoint(table, ray, state, dstate, lam_vac, obs, wstokes=(1.0, 1.0, 1
                    precision=abi.SNAPI_FP64):
     """Gradient [ncol][7][nd] of chi^2/2 = sum w_s (S - S_obs)^2 / 2 with
     respect to the per-depth parameters, in one reverse sweep."""
-    st = np.ascontiguousarray(state, dtype=np.float64)
-    ds = np.ascontiguousarray(dstate, dtype=np.float64)
+    dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
+    st = np.ascontiguousarray(state, dtype=dt)
+    ds = np.ascontiguousarray(dstate, dtype=dt)
     if st.ndim == 2:
         st, ds = st[None], ds[None]
     ncol, nf, nd = st.shape
     lam = np.ascontiguousarray(lam_vac, dtype=np.float64)
-    ob = np.ascontiguousarray(obs, dtype=np.float64).reshape(ncol, len(lam), 4)
+    ob = np.ascontiguousarray(obs, dtype=dt).reshape(ncol, len(lam), 4)
     ws = np.ascontiguousarray(wstokes, dtype=np.float64)
-    grad = np.zeros((ncol, 7, nd))
+    grad = np.zeros((ncol, 7, nd), dtype=dt)
     rc = lib().snapi_stokes_adjoint(precision, C.byref(table), C.byref(ray), ncol, nd, abi.vptr(st), abi.vptr(ds),
-                                    len(lam), abi.ptr(lam), abi.ptr(ob), abi.ptr(ws), abi.ptr(grad))
+                                    len(lam), abi.ptr(lam), abi.vptr(ob), abi.ptr(ws), abi.vptr(grad))
     check("snapi_stokes_adjoint", rc)
     return grad

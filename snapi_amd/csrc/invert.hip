@@ -1,0 +1,544 @@
+// Batched Levenberg-Marquardt inversion of many pixels on one GPU.
+//
+// Restates atmosphere::stokes_lm_fit (atmos/atmos_fit.cc:18-227) for the
+// magnetic and kinematic node parameters (microturbulence, line-of-sight
+// velocity, |B|, inclination, azimuth), with the column's thermodynamic state
+// held fixed:
+//   - node values -> depth stratification by the reference's monotone cubic
+//     Bezier node interpolation (atmospheric_interpolation, cmlib/
+//     mathtools.cc:1208-1315) on log tau; node responses by its central
+//     difference of the interpolation (atmos_fit.cc:1250-1305);
+//   - Jacobian either by the adjoint sweep (responses.hip) contracted with
+//     the node responses, or by central differences of the node values
+//     through the synthesis (2 n + 1 syntheses);
+//   - scaled Jacobian and residual (scale_rf, calc_residual :229-260), J^T J
+//     on the matrix cores, multiplicative damping and the truncated-SVD solve
+//     in FP64 (lm.hip), corrections scaled back and bracketed
+//     (scale_corrections, model::bracket_parameter_values,
+//     atmos_model.cc:957-977);
+//   - accept if chi^2 decreases (lambda / sqrt(10), floor 1e-5), else
+//     lambda * sqrt(10) and the Jacobian is reused; stop after three accepted
+//     steps with relative chi^2 change below 1e-3 or at the iteration limit
+//     (:163-181).
+// Pixels live in a fixed set of slots; a pixel that finishes hands its slot
+// to the next pixel of a device-side queue, so uneven iteration counts keep
+// the launches full.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "lm_kernel.h"
+#include "resp_kernel.h"
+#include "rt_common.h"
+#include "synth_kernel.h"
+
+extern "C" int snapi_synth_stokes_dev(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol,
+                                      int nd, const void *state_dev, int nlambda, const double *lam_dev,
+                                      void *stokes_dev, void *stream);
+
+namespace snapi {
+
+// reference model parameter codes (atmos.h model, atmos_model.cc:630-710)
+enum { NP_T = 1, NP_VT = 2, NP_VS = 3, NP_B = 4, NP_THETA = 5, NP_PHI = 6 };
+
+struct NodeArgs {
+  int nnode;
+  int code[SNAPI_MAX_NODES];      // NP_*
+  double ltau[SNAPI_MAX_NODES];   // node positions, log tau_500
+  double scale[SNAPI_MAX_NODES];  // T 5000, vt 1e5, vs 1e5, B 1000, angles 1 (const.h:35-38)
+  double step[SNAPI_MAX_NODES];   // node-response steps: vt/vs 20 cm/s, B 0.01 G, angles 1e-3 rad (const.h:27-32)
+};
+
+// atmospheric_interpolation for a non-temperature parameter at log tau x:
+// cubic Bezier between nodes, zero end derivatives, interior derivatives by
+// the weighted harmonic mean of the neighbouring slopes, linear (here:
+// constant) extrapolation outside the node range.
+__device__ double interp_param(const NodeArgs &na, const double *v, int code, double x) {
+  double t[SNAPI_MAX_NODES], y[SNAPI_MAX_NODES], d[SNAPI_MAX_NODES];
+  int n = 0;
+  for (int i = 0; i < na.nnode; ++i)
+    if (na.code[i] == code) {
+      t[n] = na.ltau[i];
+      y[n] = v[i];
+      ++n;
+    }
+  if (n == 0) return 0.0;
+  if (n == 1) return y[0];
+  d[0] = 0.0;
+  d[n - 1] = 0.0;
+  for (int i = 1; i < n - 1; ++i) {
+    const double h1 = t[i] - t[i - 1], h2 = t[i + 1] - t[i];
+    const double d1 = (y[i] - y[i - 1]) / h1, d2 = (y[i + 1] - y[i]) / h2;
+    d[i] = 0.0;
+    if (d1 * d2 > 0) {
+      const double al = 0.3333333 * (1.0 + h2 / (h1 + h2));
+      d[i] = d1 * d2 / (al * d2 + (1.0 - al) * d1);
+    }
+  }
+  if (x < t[0]) return y[0] + d[0] * (x - t[0]);
+  if (x > t[n - 1]) return y[n - 1] + d[n - 1] * (x - t[n - 1]);
+  int ii = 0;
+  for (ii = 0; ii < n - 1; ++ii)
+    if (x <= t[ii + 1]) break;
+  const double hh = t[ii + 1] - t[ii];
+  const double E = y[ii] + hh / 3.0 * d[ii], F = y[ii + 1] - hh / 3.0 * d[ii + 1];
+  const double u = (x - t[ii]) / hh;
+  return (1.0 - u) * (1.0 - u) * (1.0 - u) * y[ii] + u * u * u * y[ii + 1] + 3.0 * u * (1.0 - u) * (1.0 - u) * E +
+         3.0 * u * u * (1.0 - u) * F;
+}
+
+// Field values of one depth from the node values, as the atmosphere stores
+// and projects them for the mu = 1 ray: B = |B| (sin th cos ph, sin th sin
+// ph, cos th), inclination acos(Bz/|B|), azimuth atan(By/Bx) (atmos/
+// atmos_rad.cc:25-51).  out: vt, vlos, |B|, inclination [deg], azimuth [deg].
+struct DepthFields {
+  double vt, vlos, bm, inc, azi;
+};
+__device__ DepthFields fields_at(const NodeArgs &na, const double *v, double x) {
+  DepthFields f;
+  f.vt = interp_param(na, v, NP_VT, x);
+  f.vlos = interp_param(na, v, NP_VS, x);
+  const double B = interp_param(na, v, NP_B, x);
+  const double th = interp_param(na, v, NP_THETA, x), ph = interp_param(na, v, NP_PHI, x);
+  const double bx = B * sin(th) * cos(ph), by = B * sin(th) * sin(ph), bz = B * cos(th);
+  f.bm = sqrt(bx * bx + by * by + bz * bz);
+  f.inc = acos(bz / f.bm) * 180.0 / kPi;
+  f.azi = atan(by / bx) * 180.0 / kPi;
+  return f;
+}
+
+// One workgroup per slot, threads over depth.  Writes the node-controlled
+// fields of state (and, with nmap, the node responses d q_p(z) / d node).
+template <typename R>
+__global__ void nodes_to_state_kernel(NodeArgs na, int nslot, int nd, int nf, const double *__restrict__ nodes,
+                                      size_t node_stride, R *__restrict__ state, size_t state_stride,
+                                      R *__restrict__ nmap, const int *__restrict__ active) {
+  const int slot = blockIdx.x;
+  if (active && !active[slot]) return;
+  const double *v = nodes + (size_t)slot * node_stride;
+  R *st = state + (size_t)slot * state_stride;
+  for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+    const double x = log10(-(double)st[SNAPI_F_GRID * nd + z]);
+    const DepthFields f = fields_at(na, v, x);
+    st[SNAPI_F_VT * nd + z] = (R)f.vt;
+    st[SNAPI_F_VLOS * nd + z] = (R)f.vlos;
+    st[SNAPI_F_BMAG * nd + z] = (R)f.bm;
+    st[SNAPI_F_BINC * nd + z] = (R)f.inc;
+    st[SNAPI_F_BAZI * nd + z] = (R)f.azi;
+    if (nmap) {
+      double vv[SNAPI_MAX_NODES];
+      for (int i = 0; i < na.nnode; ++i) vv[i] = v[i];
+      for (int n = 0; n < na.nnode; ++n) {
+        const double h = na.step[n];
+        vv[n] = v[n] + 0.5 * h;
+        const DepthFields a = fields_at(na, vv, x);
+        vv[n] = v[n] - 0.5 * h;
+        const DepthFields b = fields_at(na, vv, x);
+        vv[n] = v[n];
+        // the reference differences vt + vz + |B| + inc + azi (radians); only
+        // the node's own quantity changes
+        const double qa = a.vt + a.vlos + a.bm + (a.inc + a.azi) * kPi / 180.0;
+        const double qb = b.vt + b.vlos + b.bm + (b.inc + b.azi) * kPi / 180.0;
+        nmap[((size_t)slot * na.nnode + n) * nd + z] = (R)((qa - qb) / h);
+      }
+    }
+  }
+}
+
+// r = (S_obs - S) w_s mask (calc_residual), chi^2 = sum r^2 (calc_chisq)
+template <typename R>
+__global__ void residual_kernel(int nslot, int nl, const R *__restrict__ S, const R *__restrict__ obs,
+                                const R *__restrict__ mask, double w0, double w1, double w2, double w3,
+                                R *__restrict__ r, double *__restrict__ chi2, const int *__restrict__ active) {
+  const int slot = blockIdx.x;
+  if (active && !active[slot]) return;
+  __shared__ double red[256];
+  const double ws[4] = {w0, w1, w2, w3};
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nl * 4; i += blockDim.x) {
+    const int l = i >> 2, s = i & 3;
+    const double v = ((double)obs[(size_t)slot * nl * 4 + i] - (double)S[(size_t)slot * nl * 4 + i]) * ws[s] *
+                     (double)mask[(size_t)slot * nl + l];
+    if (r) r[(size_t)slot * nl * 4 + i] = (R)v;
+    acc += v * v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) chi2[slot] = red[0];
+}
+
+// Central-difference Jacobian from the (2n+1) syntheses of each slot.
+template <typename R>
+__global__ void fdjac_kernel(NodeArgs na, int nslot, int nl, const R *__restrict__ Sall, const double *__restrict__ hv,
+                             const R *__restrict__ mask, double w0, double w1, double w2, double w3,
+                             R *__restrict__ S, R *__restrict__ J, const int *__restrict__ active) {
+  const int slot = blockIdx.x;
+  if (active && !active[slot]) return;
+  const int nv = 2 * na.nnode + 1;
+  const double ws[4] = {w0, w1, w2, w3};
+  for (int i = threadIdx.x; i < nl * 4; i += blockDim.x) {
+    const int l = i >> 2, s = i & 3;
+    const size_t base = (size_t)slot * nv;
+    S[(size_t)slot * nl * 4 + i] = Sall[(base + 0) * nl * 4 + i];
+    const double mk = (double)mask[(size_t)slot * nl + l];
+    for (int n = 0; n < na.nnode; ++n) {
+      const double d = ((double)Sall[(base + 1 + 2 * n) * nl * 4 + i] - (double)Sall[(base + 2 + 2 * n) * nl * 4 + i]) /
+                       (2.0 * hv[(size_t)slot * na.nnode + n]);
+      J[((size_t)slot * nl * 4 + i) * na.nnode + n] = (R)(d * na.scale[n] * ws[s] * mk);
+    }
+  }
+}
+
+// node sets of the central differences: v = 0 base, 2n+1 / 2n+2 = node n +/- h
+__global__ void fd_nodes_kernel(NodeArgs na, int nslot, const double *__restrict__ nodes, double *__restrict__ fdn,
+                                double *__restrict__ hv, const int *__restrict__ active) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nslot || (active && !active[slot])) return;
+  const int nn = na.nnode, nv = 2 * nn + 1;
+  const double *v = nodes + (size_t)slot * nn;
+  for (int k = 0; k < nv; ++k)
+    for (int i = 0; i < nn; ++i) fdn[((size_t)slot * nv + k) * nn + i] = v[i];
+  for (int n = 0; n < nn; ++n) {
+    const double h = fmax(1e-3 * fabs(v[n]), na.step[n]);
+    hv[(size_t)slot * nn + n] = h;
+    fdn[((size_t)slot * nv + 1 + 2 * n) * nn + n] += h;
+    fdn[((size_t)slot * nv + 2 + 2 * n) * nn + n] -= h;
+  }
+}
+
+// bracket_parameter_values (atmos_model.cc:957-977)
+__device__ double bracket(int code, double v) {
+  switch (code) {
+    case NP_T: return fmin(fmax(v, 3400.0), 12000.0);
+    case NP_VT: return fmin(fmax(v, 1E2), 15E5);
+    case NP_VS: return fmin(fmax(v, -15E5), 15E5);
+    case NP_B: return fmin(fmax(v, 1.0), 1E4);
+  }
+  return v;
+}
+
+__global__ void trial_kernel(NodeArgs na, int nslot, const double *__restrict__ nodes, const double *__restrict__ corr,
+                             double *__restrict__ test, const int *__restrict__ active) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nslot || !active[slot]) return;
+  for (int n = 0; n < na.nnode; ++n) {
+    const size_t i = (size_t)slot * na.nnode + n;
+    test[i] = bracket(na.code[n], nodes[i] + corr[i] * na.scale[n]);
+  }
+}
+
+struct SlotState {
+  int *pix, *iter, *corrected, *ntrack, *active, *needj;
+  double *lm, *chi2, *chi2t, *track;
+};
+
+// accept / reject (atmos_fit.cc:163-181)
+__global__ void update_kernel(NodeArgs na, int nslot, int max_iter, SlotState ss, double *__restrict__ nodes,
+                              const double *__restrict__ test) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= nslot || !ss.active[slot]) return;
+  const double lmm = sqrt(10.0);
+  bool stop = false;
+  ss.iter[slot] += 1;
+  if (ss.chi2t[slot] < ss.chi2[slot]) {
+    for (int n = 0; n < na.nnode; ++n) nodes[(size_t)slot * na.nnode + n] = test[(size_t)slot * na.nnode + n];
+    ss.lm[slot] = fmax(ss.lm[slot] / lmm, 1E-5);
+    ss.corrected[slot] = 1;
+    double *tr = ss.track + (size_t)slot * 3;
+    int nt = ss.ntrack[slot];
+    if (nt < 3) {
+      tr[nt] = ss.chi2[slot];
+      ++nt;
+    } else {
+      tr[0] = tr[1];
+      tr[1] = tr[2];
+      tr[2] = ss.chi2[slot];
+    }
+    ss.ntrack[slot] = nt;
+    if (nt >= 3) {
+      const double change = fabs((tr[1] - tr[2]) / tr[2]);
+      if (change < 1E-3) stop = true;
+    }
+  } else {
+    ss.lm[slot] *= lmm;
+    ss.corrected[slot] = 0;
+  }
+  if (stop || ss.iter[slot] >= max_iter) ss.active[slot] = 0;
+  ss.needj[slot] = ss.active[slot] && ss.corrected[slot];
+}
+
+// Finished slots write their pixel's result and take the next pixel of the
+// queue (copying its state, observation, mask and start nodes).
+template <typename R>
+__global__ void refill_kernel(NodeArgs na, int nslot, int npix, int nf, int nd, int nl, int *__restrict__ queue,
+                              SlotState ss, const R *__restrict__ pstate, const R *__restrict__ pobs,
+                              const R *__restrict__ pmask, const double *__restrict__ pnodes0, double lm0,
+                              R *__restrict__ sstate, R *__restrict__ tstate, R *__restrict__ sobs,
+                              R *__restrict__ smask, double *__restrict__ snodes, double *__restrict__ out_nodes,
+                              double *__restrict__ out_chi2, int *__restrict__ out_iter) {
+  const int slot = blockIdx.x;
+  __shared__ int next;
+  if (ss.active[slot]) return;
+  if (threadIdx.x == 0) {
+    const int old = ss.pix[slot];
+    if (old >= 0) {
+      for (int n = 0; n < na.nnode; ++n) out_nodes[(size_t)old * na.nnode + n] = snodes[(size_t)slot * na.nnode + n];
+      out_chi2[old] = ss.chi2[slot];
+      out_iter[old] = ss.iter[slot];
+    }
+    next = atomicAdd(queue, 1);
+    if (next >= npix) next = -1;
+    ss.pix[slot] = next;
+    if (next >= 0) {
+      ss.iter[slot] = 0;
+      ss.corrected[slot] = 1;
+      ss.ntrack[slot] = 0;
+      ss.lm[slot] = lm0;
+      ss.active[slot] = 1;
+      ss.needj[slot] = 1;
+      for (int n = 0; n < na.nnode; ++n) snodes[(size_t)slot * na.nnode + n] = pnodes0[(size_t)next * na.nnode + n];
+    } else {
+      ss.needj[slot] = 0;
+    }
+  }
+  __syncthreads();
+  if (next < 0) return;
+  const size_t ns = (size_t)nf * nd;
+  for (size_t i = threadIdx.x; i < ns; i += blockDim.x) {
+    const R v = pstate[(size_t)next * ns + i];
+    sstate[(size_t)slot * ns + i] = v;
+    tstate[(size_t)slot * ns + i] = v;
+  }
+  for (int i = threadIdx.x; i < nl * 4; i += blockDim.x) sobs[(size_t)slot * nl * 4 + i] = pobs[(size_t)next * nl * 4 + i];
+  for (int i = threadIdx.x; i < nl; i += blockDim.x)
+    smask[(size_t)slot * nl + i] = pmask ? pmask[(size_t)next * nl + i] : R(1);
+}
+
+__global__ void count_active_kernel(int nslot, const int *__restrict__ active, int *__restrict__ count) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot < nslot && active[slot]) atomicAdd(count, 1);
+}
+
+#define HIPCHK(x)                          \
+  do {                                     \
+    if ((x) != hipSuccess) {               \
+      rc = SNAPI_ERR_DEVICE;               \
+      goto done;                           \
+    }                                      \
+  } while (0)
+
+template <typename R>
+static int invert_impl(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                       const NodeArgs &na, int npix, int nd, const void *pstate_h, int nlambda, const double *lam_h,
+                       const void *obs_h, const void *mask_h, const double *ws, double lm0, int max_iter, int nslot,
+                       double *nodes_io, double *chi2_out, int *iter_out, long long *stats) {
+  KLines kl;
+  SynthArgs a;
+  int rc = prepare_synth(lines, ray, 1, nd, nlambda, kl, a);
+  if (rc) return rc;
+  const int nf = a.nfield, nn = na.nnode, nl = nlambda, nv = 2 * nn + 1;
+  const size_t es = sizeof(R), ns = (size_t)nf * nd;
+  // device buffers
+  R *pstate = nullptr, *pobs = nullptr, *pmask = nullptr, *sstate = nullptr, *tstate = nullptr, *dzero = nullptr;
+  R *sobs = nullptr, *smask = nullptr, *S = nullptr, *St = nullptr, *J = nullptr, *r = nullptr, *nmap = nullptr;
+  R *fdstate = nullptr, *Sall = nullptr, *scr = nullptr;
+  double *dlam = nullptr, *pnodes0 = nullptr, *snodes = nullptr, *tnodes = nullptr, *JTJ = nullptr, *rhs = nullptr;
+  double *corr = nullptr, *onodes = nullptr, *ochi2 = nullptr, *fdn = nullptr, *hv = nullptr, *dbl = nullptr;
+  int *ints = nullptr, *queue = nullptr, *oiter = nullptr, *count = nullptr;
+  SlotState ss;
+  int chunk = nslot;
+  size_t per_col = 0;
+  int *status = device_status_slot();
+  if (!status) return SNAPI_ERR_DEVICE;
+  HIPCHK(hipMalloc((void **)&pstate, (size_t)npix * ns * es));
+  HIPCHK(hipMalloc((void **)&pobs, (size_t)npix * nl * 4 * es));
+  HIPCHK(hipMalloc((void **)&pmask, (size_t)npix * nl * es));
+  HIPCHK(hipMalloc((void **)&sstate, (size_t)nslot * ns * es));
+  HIPCHK(hipMalloc((void **)&tstate, (size_t)nslot * ns * es));
+  HIPCHK(hipMalloc((void **)&dzero, (size_t)nslot * 2 * ns * es));
+  HIPCHK(hipMalloc((void **)&sobs, (size_t)nslot * nl * 4 * es));
+  HIPCHK(hipMalloc((void **)&smask, (size_t)nslot * nl * es));
+  HIPCHK(hipMalloc((void **)&S, (size_t)nslot * nl * 4 * es));
+  HIPCHK(hipMalloc((void **)&St, (size_t)nslot * nl * 4 * es));
+  HIPCHK(hipMalloc((void **)&J, (size_t)nslot * nl * 4 * nn * es));
+  HIPCHK(hipMalloc((void **)&r, (size_t)nslot * nl * 4 * es));
+  HIPCHK(hipMalloc((void **)&nmap, (size_t)nslot * nn * nd * es));
+  HIPCHK(hipMalloc((void **)&dlam, (size_t)nl * 8));
+  HIPCHK(hipMalloc((void **)&pnodes0, (size_t)npix * nn * 8));
+  HIPCHK(hipMalloc((void **)&snodes, (size_t)nslot * nn * 8));
+  HIPCHK(hipMalloc((void **)&tnodes, (size_t)nslot * nn * 8));
+  HIPCHK(hipMalloc((void **)&JTJ, (size_t)nslot * nn * nn * 8));
+  HIPCHK(hipMalloc((void **)&rhs, (size_t)nslot * nn * 8));
+  HIPCHK(hipMalloc((void **)&corr, (size_t)nslot * nn * 8));
+  HIPCHK(hipMalloc((void **)&onodes, (size_t)npix * nn * 8));
+  HIPCHK(hipMalloc((void **)&ochi2, (size_t)npix * 8));
+  HIPCHK(hipMalloc((void **)&dbl, (size_t)nslot * 6 * 8));
+  HIPCHK(hipMalloc((void **)&ints, (size_t)nslot * 6 * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&queue, 2 * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&oiter, (size_t)npix * sizeof(int)));
+  count = queue + 1;
+  ss.pix = ints;
+  ss.iter = ints + nslot;
+  ss.corrected = ints + 2 * nslot;
+  ss.ntrack = ints + 3 * nslot;
+  ss.active = ints + 4 * nslot;
+  ss.needj = ints + 5 * nslot;
+  ss.lm = dbl;
+  ss.chi2 = dbl + nslot;
+  ss.chi2t = dbl + 2 * nslot;
+  ss.track = dbl + 3 * nslot;
+  if (jacobian == 0) {
+    per_col = resp_scratch_per_col(nl, nd, es);
+    size_t c = ((size_t)16 << 30) / (per_col ? per_col : 1);
+    if (c < 1) c = 1;
+    if (c > (size_t)nslot) c = nslot;
+    chunk = (int)c;
+    HIPCHK(hipMalloc((void **)&scr, per_col * chunk));
+  } else {
+    HIPCHK(hipMalloc((void **)&fdstate, (size_t)nslot * nv * ns * es));
+    HIPCHK(hipMalloc((void **)&Sall, (size_t)nslot * nv * nl * 4 * es));
+    HIPCHK(hipMalloc((void **)&fdn, (size_t)nslot * nv * nn * 8));
+    HIPCHK(hipMalloc((void **)&hv, (size_t)nslot * nn * 8));
+  }
+  HIPCHK(hipMemcpy(pstate, pstate_h, (size_t)npix * ns * es, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(pobs, obs_h, (size_t)npix * nl * 4 * es, hipMemcpyHostToDevice));
+  if (mask_h) HIPCHK(hipMemcpy(pmask, mask_h, (size_t)npix * nl * es, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dlam, lam_h, (size_t)nl * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(pnodes0, nodes_io, (size_t)npix * nn * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(dzero, 0, (size_t)nslot * 2 * ns * es));
+  HIPCHK(hipMemset(ints, 0, (size_t)nslot * 6 * sizeof(int)));
+  HIPCHK(hipMemset(dbl, 0, (size_t)nslot * 6 * 8));
+  HIPCHK(hipMemset(queue, 0, 2 * sizeof(int)));
+  {
+    std::vector<int> neg(nslot, -1);
+    HIPCHK(hipMemcpy(ss.pix, neg.data(), nslot * sizeof(int), hipMemcpyHostToDevice));
+  }
+  reset_status();
+  {
+    RespArgs ra = {};
+    ra.nseed = 4;
+    ra.sink = SINK_NODE;
+    for (int s = 0; s < 4; ++s) ra.ws[s] = ws[s];
+    ra.nnode = nn;
+    for (int n = 0; n < nn; ++n) {
+      // node parameter -> per-depth response index (vt 2, vLOS 3, |B| 4, incl 5, azi 6)
+      ra.pidx[n] = na.code[n] == NP_VT ? 2 : na.code[n] == NP_VS ? 3 : na.code[n] == NP_B ? 4 : na.code[n] == NP_THETA ? 5 : 6;
+      ra.nscale[n] = na.scale[n];
+    }
+    ra.nmap = nmap;
+    ra.mask = smask;
+    ra.active = ss.needj;
+    long long pixel_iters = 0;
+    for (int it = 0;; ++it) {
+      hipLaunchKernelGGL(refill_kernel<R>, dim3(nslot), dim3(256), 0, 0, na, nslot, npix, nf, nd, nl, queue, ss, pstate,
+                         pobs, mask_h ? pmask : (R *)nullptr, pnodes0, lm0, sstate, tstate, sobs, smask, snodes, onodes,
+                         ochi2, oiter);
+      HIPCHK(hipMemset(count, 0, sizeof(int)));
+      hipLaunchKernelGGL(count_active_kernel, dim3((nslot + 255) / 256), dim3(256), 0, 0, nslot, ss.active, count);
+      int nact = 0;
+      HIPCHK(hipMemcpy(&nact, count, sizeof(int), hipMemcpyDeviceToHost));
+      if (nact == 0) break;
+      pixel_iters += nact;
+      // model -> state and Jacobian where the model changed
+      hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, 0, na, nslot, nd, nf, snodes,
+                         (size_t)nn, sstate, ns, nmap, ss.needj);
+      if (jacobian == 0) {
+        rc = launch_resp<R>(kl, a, ra, nslot, sstate, dzero, dlam, S, J, scr, chunk, 0);
+        if (rc) goto done;
+      } else {
+        hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, 0, na, nslot, snodes, fdn, hv,
+                           ss.needj);
+        // (2n+1) copies of every slot's state with the node sets applied
+        for (int k = 0; k < nv; ++k) {
+          HIPCHK(hipMemcpy2DAsync(fdstate + (size_t)k * ns, nv * ns * es, sstate, ns * es, ns * es, nslot,
+                                  hipMemcpyDeviceToDevice, 0));
+          hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, 0, na, nslot, nd, nf,
+                             fdn + (size_t)k * nn, (size_t)nv * nn, fdstate + (size_t)k * ns, (size_t)nv * ns,
+                             (R *)nullptr, ss.needj);
+        }
+        // fdstate is [slot][node set]: one synthesis over nslot * (2n+1) columns
+        rc = snapi_synth_stokes_dev(precision, lines, ray, nslot * nv, nd, fdstate, nl, dlam, Sall, nullptr);
+        if (rc) goto done;
+        hipLaunchKernelGGL(fdjac_kernel<R>, dim3(nslot), dim3(256), 0, 0, na, nslot, nl, Sall, hv, smask, ws[0], ws[1],
+                           ws[2], ws[3], S, J, ss.needj);
+      }
+      // chi^2 of the current model (where it changed)
+      hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, 0, nslot, nl, S, sobs, smask, ws[0], ws[1],
+                         ws[2], ws[3], r, ss.chi2, ss.needj);
+      // normal equations and damped solve
+      launch_lm_normal<R>(nslot, nl * 4, nn, J, r, JTJ, rhs, nullptr, ss.active, 0);
+      launch_lm_solve(nslot, nn, JTJ, rhs, ss.lm, corr, ss.active, 0);
+      // trial model and its synthesis
+      hipLaunchKernelGGL(trial_kernel, dim3((nslot + 127) / 128), dim3(128), 0, 0, na, nslot, snodes, corr, tnodes,
+                         ss.active);
+      hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, 0, na, nslot, nd, nf, tnodes,
+                         (size_t)nn, tstate, ns, (R *)nullptr, ss.active);
+      rc = snapi_synth_stokes_dev(precision, lines, ray, nslot, nd, tstate, nl, dlam, St, nullptr);
+      if (rc) goto done;
+      hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, 0, nslot, nl, St, sobs, smask, ws[0], ws[1],
+                         ws[2], ws[3], (R *)nullptr, ss.chi2t, ss.active);
+      hipLaunchKernelGGL(update_kernel, dim3((nslot + 127) / 128), dim3(128), 0, 0, na, nslot, max_iter, ss, snodes,
+                         tnodes);
+      if (hipGetLastError() != hipSuccess) {
+        rc = SNAPI_ERR_DEVICE;
+        goto done;
+      }
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+      rc = SNAPI_ERR_DEVICE;
+      goto done;
+    }
+    HIPCHK(hipMemcpy(nodes_io, onodes, (size_t)npix * nn * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(chi2_out, ochi2, (size_t)npix * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(iter_out, oiter, (size_t)npix * sizeof(int), hipMemcpyDeviceToHost));
+    if (stats) stats[0] = pixel_iters;
+    rc = read_status();
+  }
+done:
+  hipFree(pstate); hipFree(pobs); hipFree(pmask); hipFree(sstate); hipFree(tstate); hipFree(dzero);
+  hipFree(sobs); hipFree(smask); hipFree(S); hipFree(St); hipFree(J); hipFree(r); hipFree(nmap);
+  hipFree(fdstate); hipFree(Sall); hipFree(scr); hipFree(dlam); hipFree(pnodes0); hipFree(snodes);
+  hipFree(tnodes); hipFree(JTJ); hipFree(rhs); hipFree(corr); hipFree(onodes); hipFree(ochi2); hipFree(fdn);
+  hipFree(hv); hipFree(dbl); hipFree(ints); hipFree(queue); hipFree(oiter);
+  return rc;
+}
+
+}  // namespace snapi
+
+using namespace snapi;
+
+extern "C" int snapi_invert(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                            const snapi_node_model *model, int npix, int nd, const void *state, int nlambda,
+                            const double *lam, const void *obs, const void *mask, const double *wstokes,
+                            double lm_start, int max_iter, int nslot, double *nodes, double *chi2, int *iterations,
+                            long long *pixel_iterations) {
+  if (!lines || !ray || !model || !state || !lam || !obs || !wstokes || !nodes || !chi2 || !iterations)
+    return SNAPI_ERR_ARG;
+  if (npix <= 0 || nd < 3 || nlambda <= 0 || max_iter <= 0) return SNAPI_ERR_ARG;
+  if (model->nnode <= 0 || model->nnode > SNAPI_MAX_NODES - 1) return SNAPI_ERR_ARG;
+  if (jacobian != SNAPI_JAC_ADJOINT && jacobian != SNAPI_JAC_DIFFERENCES) return SNAPI_ERR_ARG;
+  if (cos(ray->theta) > -0.999999) return SNAPI_ERR_UNSUPPORTED;  // node responses assume the mu = 1 ray
+  NodeArgs na = {};
+  na.nnode = model->nnode;
+  for (int n = 0; n < na.nnode; ++n) {
+    const int c = model->param[n];
+    if (c < NP_VT || c > NP_PHI) return SNAPI_ERR_UNSUPPORTED;  // temperature nodes need the equation of state
+    na.code[n] = c;
+    na.ltau[n] = model->logtau[n];
+    na.scale[n] = c == NP_VT ? 1E5 : c == NP_VS ? 1E5 : c == NP_B ? 1000.0 : 1.0;
+    na.step[n] = (c == NP_VT || c == NP_VS) ? 20.0 : c == NP_B ? 0.01 : 0.001;
+  }
+  if (nslot <= 0 || nslot > npix) nslot = npix;
+  if (precision == SNAPI_FP64)
+    return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
+                               lm_start, max_iter, nslot, nodes, chi2, iterations, pixel_iterations);
+  if (precision == SNAPI_MIXED)
+    return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
+                              lm_start, max_iter, nslot, nodes, chi2, iterations, pixel_iterations);
+  return SNAPI_ERR_ARG;
+}

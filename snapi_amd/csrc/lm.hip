@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "../../include/snapi_gpu.h"
+#include "lm_kernel.h"
 
 namespace snapi {
 
@@ -49,10 +50,12 @@ template <> struct Mfma<double> {
 template <typename T>
 __global__ __launch_bounds__(256) void lm_normal_kernel(int npix, int nrow, int npar, const T *__restrict__ J,
                                                        const T *__restrict__ r, double *__restrict__ JTJ,
-                                                       double *__restrict__ rhs, double *__restrict__ chi2) {
+                                                       double *__restrict__ rhs, double *__restrict__ chi2,
+                                                       const int *__restrict__ active) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= npix) return;  // whole wavefronts exit together
+  if (active && !active[wave]) return;
   const T *Jp = J + (size_t)wave * nrow * npar;
   const T *rp = r + (size_t)wave * nrow;
   const int col = lane & 15, sub = lane >> 4;
@@ -81,7 +84,7 @@ __global__ __launch_bounds__(256) void lm_normal_kernel(int npix, int nrow, int 
 // pseudo-inverse applied to rhs.
 __global__ __launch_bounds__(64) void lm_solve_kernel(int npix, int npar, const double *__restrict__ JTJ,
                                                      const double *__restrict__ rhs, const double *__restrict__ lm,
-                                                     double *__restrict__ corr) {
+                                                     double *__restrict__ corr, const int *__restrict__ active) {
   __shared__ double M[kNP][kNP + 1];
   __shared__ double Q[kNP][kNP + 1];
   __shared__ double rot[2];
@@ -89,6 +92,7 @@ __global__ __launch_bounds__(64) void lm_solve_kernel(int npix, int npar, const 
   const int pix = blockIdx.x;
   const int j = threadIdx.x;
   const int n = npar;
+  if (active && !active[pix]) return;  // uniform over the workgroup
   const double *G = JTJ + (size_t)pix * n * n;
   const double damp = 1.0 + lm[pix];
   for (int i = j; i < n * n; i += 64) {
@@ -166,6 +170,22 @@ __global__ __launch_bounds__(64) void lm_solve_kernel(int npix, int npar, const 
   }
 }
 
+template <typename T>
+void launch_lm_normal(int npix, int nrow, int npar, const T *J, const T *r, double *JTJ, double *rhs, double *chi2,
+                      const int *active, hipStream_t s) {
+  hipLaunchKernelGGL(lm_normal_kernel<T>, dim3((npix + 3) / 4), dim3(256), 0, s, npix, nrow, npar, J, r, JTJ, rhs,
+                     chi2, active);
+}
+template void launch_lm_normal<float>(int, int, int, const float *, const float *, double *, double *, double *,
+                                      const int *, hipStream_t);
+template void launch_lm_normal<double>(int, int, int, const double *, const double *, double *, double *, double *,
+                                       const int *, hipStream_t);
+
+void launch_lm_solve(int npix, int npar, const double *JTJ, const double *rhs, const double *lm, double *corr,
+                     const int *active, hipStream_t s) {
+  hipLaunchKernelGGL(lm_solve_kernel, dim3(npix), dim3(64), 0, s, npix, npar, JTJ, rhs, lm, corr, active);
+}
+
 }  // namespace snapi
 
 using namespace snapi;
@@ -189,14 +209,12 @@ extern "C" int snapi_lm_step(int precision, int npix, int nrow, int npar, const 
     hipMemcpy(dJ, J, nJ * es, hipMemcpyHostToDevice);
     hipMemcpy(dr, r, nr * es, hipMemcpyHostToDevice);
     hipMemcpy(dlm, lm, (size_t)npix * 8, hipMemcpyHostToDevice);
-    const int blocks = (npix + 3) / 4;
     if (precision == SNAPI_FP64)
-      hipLaunchKernelGGL(lm_normal_kernel<double>, dim3(blocks), dim3(256), 0, 0, npix, nrow, npar,
-                         (const double *)dJ, (const double *)dr, dG, drhs, (double *)nullptr);
+      launch_lm_normal<double>(npix, nrow, npar, (const double *)dJ, (const double *)dr, dG, drhs, nullptr, nullptr,
+                               0);
     else
-      hipLaunchKernelGGL(lm_normal_kernel<float>, dim3(blocks), dim3(256), 0, 0, npix, nrow, npar,
-                         (const float *)dJ, (const float *)dr, dG, drhs, (double *)nullptr);
-    hipLaunchKernelGGL(lm_solve_kernel, dim3(npix), dim3(64), 0, 0, npix, npar, dG, drhs, dlm, dcorr);
+      launch_lm_normal<float>(npix, nrow, npar, (const float *)dJ, (const float *)dr, dG, drhs, nullptr, nullptr, 0);
+    launch_lm_solve(npix, npar, dG, drhs, dlm, dcorr, nullptr, 0);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = SNAPI_ERR_DEVICE;
     else {
       hipMemcpy(JTJ, dG, nG * 8, hipMemcpyDeviceToHost);

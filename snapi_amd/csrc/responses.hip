@@ -25,30 +25,17 @@
 
 #include "rt_common.h"
 #include "synth_kernel.h"
+#include "resp_kernel.h"
 
 namespace snapi {
 
-constexpr int kRespThreads = 128;
-constexpr int kFwd = 15;  // per depth: K[7], E[4], I[4]
-constexpr int kAdj = 11;  // per depth and seed: Kbar[7], Ebar[4]
-enum { SINK_RESP = 0, SINK_GRAD = 1 };
-
-struct RespArgs {
-  int nseed;          // 4 (responses) or 1 (gradient)
-  int sink;
-  int col0;           // first column of this launch
-  long long nwork;    // work-items of this launch (columns * nlambda)
-  const double *obs;  // [ncol][nlambda][4] (gradient)
-  double ws[4];       // Stokes weights (gradient)
-};
-
 // scratch: [depth k][field][work-item]
-struct Scratch {
-  double *p;
+template <typename R> struct Scratch {
+  R *p;
   long long W;
   int nd;
-  __device__ double &fwd(int k, int f, long long w) const { return p[((long long)k * kFwd + f) * W + w]; }
-  __device__ double &adj(int k, int f, int s, long long w) const {
+  __device__ R &fwd(int k, int f, long long w) const { return p[((long long)k * kFwd + f) * W + w]; }
+  __device__ R &adj(int k, int f, int s, long long w) const {
     return p[(long long)nd * kFwd * W + (((long long)k * kAdj + f) * 4 + s) * W + w];
   }
 };
@@ -56,11 +43,12 @@ struct Scratch {
 // Local derivatives of the normalised opacity/emissivity of depth z along
 // the seven physical parameters.  dK[p][7] in the compact order
 // (eta_I, eta_Q, eta_U, eta_V, rho_Q, rho_U, rho_V), dE[p][4].
-__device__ void local_derivs(const KLines &L, const SynthArgs &a, const double *__restrict__ st,
-                             const double *__restrict__ dst, int z, double lam, double dK[7][7], double dE[7][4]) {
+template <typename S>
+__device__ void local_derivs(const KLines &L, const SynthArgs &a, const S *__restrict__ st, const S *__restrict__ dst,
+                             int z, double lam, double dK[7][7], double dE[7][4]) {
   const int nd = a.nd, nf = a.nfield;
-#define S_(f) st[(size_t)(f)*nd + z]
-#define D_(p, f) dst[((size_t)(p)*nf + (f)) * nd + z]
+#define S_(f) ((double)st[(size_t)(f)*nd + z])
+#define D_(p, f) ((double)dst[((size_t)(p)*nf + (f)) * nd + z])
   for (int p = 0; p < 7; ++p) {
     for (int i = 0; i < 7; ++i) dK[p][i] = 0.0;
     for (int i = 0; i < 4; ++i) dE[p][i] = 0.0;
@@ -265,82 +253,85 @@ __device__ void local_derivs(const KLines &L, const SynthArgs &a, const double *
 }
 
 // derivatives of the linear weights with respect to the step (both branches)
-__device__ __forceinline__ void sc_weights_d(double tb, double &detb, double &dw0, double &dw1) {
-  if (tb < 1E-2) {
-    detb = -1.0 + tb - 0.5 * tb * tb;
-    dw0 = 0.5 - 2.0 * tb / 3.0 + 3.0 * tb * tb / 8.0;
-    dw1 = 0.5 - tb / 3.0 + tb * tb / 8.0;
+template <typename R> __device__ __forceinline__ void sc_weights_d(R tb, R &detb, R &dw0, R &dw1) {
+  if (tb < R(1E-2)) {
+    detb = R(-1) + tb - R(0.5) * tb * tb;
+    dw0 = R(0.5) - R(2.0 / 3.0) * tb + R(3.0 / 8.0) * tb * tb;
+    dw1 = R(0.5) - tb * R(1.0 / 3.0) + tb * tb * R(0.125);
   } else {
-    const double e = exp(-tb), i = 1.0 / tb;
+    const R e = rexp<R>(-tb), i = R(1) / tb;
     detb = -e;
-    dw0 = -i * i + e * (1.0 + i) + e * i * i;
+    dw0 = -i * i + e * (R(1) + i) + e * i * i;
     dw1 = i * i - e * i - e * i * i;
   }
 }
 
 // Fritsch-Butland derivative and its partials with respect to (om, o0, op)
-__device__ __forceinline__ double fb_deriv_d(double om, double o0, double op, double h1, double h2, double &gm,
-                                             double &g0, double &gp) {
-  const double d2 = (op - o0) / h2, d1 = (o0 - om) / h1;
-  gm = g0 = gp = 0.0;
-  if (d1 * d2 > 0) {
-    const double al = (1.0 + h1 / (h1 + h2)) / 3.0;
-    const double den = al * d1 + (1.0 - al) * d2;
-    const double D = d1 * d2 / den;
-    const double dD1 = d2 * d2 * (1.0 - al) / (den * den);
-    const double dD2 = d1 * d1 * al / (den * den);
+template <typename R>
+__device__ __forceinline__ R fb_deriv_d(R om, R o0, R op, R h1, R h2, R &gm, R &g0, R &gp) {
+  const R d2 = (op - o0) / h2, d1 = (o0 - om) / h1;
+  gm = g0 = gp = R(0);
+  if (d1 * d2 > R(0)) {
+    const R al = (R(1) + h1 / (h1 + h2)) * R(1.0 / 3.0);
+    const R den = al * d1 + (R(1) - al) * d2;
+    const R D = d1 * d2 / den;
+    const R dD1 = d2 * d2 * (R(1) - al) / (den * den);
+    const R dD2 = d1 * d1 * al / (den * den);
     gm = -dD1 / h1;
     g0 = dD1 / h1 - dD2 / h2;
     gp = dD2 / h2;
     return D;
   }
-  return 0.0;
+  return R(0);
 }
 
 // adjoint of y = K~ x with respect to the six reduced-matrix elements
-__device__ __forceinline__ void kmul_adj(const double yb[4], const double x[4], double kb[6]) {
-  kb[0] += yb[0] * x[1] + yb[1] * x[0];  // a
-  kb[1] += yb[0] * x[2] + yb[2] * x[0];  // b
-  kb[2] += yb[0] * x[3] + yb[3] * x[0];  // c
-  kb[3] += yb[1] * x[2] - yb[2] * x[1];  // d
-  kb[4] += -yb[1] * x[3] + yb[3] * x[1]; // e
-  kb[5] += yb[2] * x[3] - yb[3] * x[2];  // f
+template <typename R> __device__ __forceinline__ void kmul_adj(const R yb[4], const R x[4], R kb[6]) {
+  kb[0] += yb[0] * x[1] + yb[1] * x[0];   // a
+  kb[1] += yb[0] * x[2] + yb[2] * x[0];   // b
+  kb[2] += yb[0] * x[3] + yb[3] * x[0];   // c
+  kb[3] += yb[1] * x[2] - yb[2] * x[1];   // d
+  kb[4] += -yb[1] * x[3] + yb[3] * x[1];  // e
+  kb[5] += yb[2] * x[3] - yb[3] * x[2];   // f
 }
 
+template <typename R>
 __global__ __launch_bounds__(kRespThreads) void resp_kernel(const KLines lines, const SynthArgs args,
-                                                           const RespArgs ra, const double *__restrict__ state,
-                                                           const double *__restrict__ dstate,
-                                                           const double *__restrict__ lam, Scratch scr,
-                                                           double *__restrict__ stokes, double *__restrict__ out,
+                                                           const RespArgs ra, const R *__restrict__ state,
+                                                           const R *__restrict__ dstate,
+                                                           const double *__restrict__ lam, Scratch<R> scr,
+                                                           R *__restrict__ stokes, R *__restrict__ out,
                                                            int *__restrict__ status) {
   extern __shared__ unsigned char smem_raw[];
-  double *sm = reinterpret_cast<double *>(smem_raw);
+  R *sm = reinterpret_cast<R *>(smem_raw);
   const int nd = args.nd, nl = args.nlambda, nf = args.nfield;
   const int cl = blockIdx.x;  // column within the launch
   const int col = ra.col0 + cl;
-  const double *st = state + (size_t)col * nf * nd;
-  const double *dst = dstate + (size_t)col * 2 * nf * nd;
-  ColumnTile<double> tile = ColumnTile<double>::carve(sm, nd, lines.nline);
-  stage_column<double, double>(lines, args, st, tile);
+  if (ra.active && !ra.active[col]) return;  // uniform over the workgroup
+  const R *st = state + (size_t)col * nf * nd;
+  const R *dst = dstate + (size_t)col * 2 * nf * nd;
+  ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
+  stage_column<R, R>(lines, args, st, tile);
   __syncthreads();
   const bool up = args.ct < 0.0;
-  const double mct = -args.ct;
+  const R mct = (R)(-args.ct);
+  const int ns = ra.nseed;
   auto zof = [&](int k) { return up ? nd - 1 - k : k; };
   for (int l = threadIdx.x; l < nl; l += blockDim.x) {
     const long long w = (long long)cl * nl + l;
     const double lm = lam[l];
-    LamConst<double> c;
-    lambda_consts<double>(lines, args, lm, c);
+    LamConst<R> c;
+    lambda_consts<R>(lines, args, lm, c);
     // ---------------- forward: opacity of every depth, then the recurrence
     for (int k = 0; k < nd; ++k) {
-      double K[7], E[4];
-      opacity_at<double>(lines, args, tile, c, zof(k), K, E);
+      R K[7], E[4];
+      opacity_at<R>(lines, args, tile, c, zof(k), K, E);
       for (int i = 0; i < 7; ++i) scr.fwd(k, i, w) = K[i];
       for (int i = 0; i < 4; ++i) scr.fwd(k, 7 + i, w) = E[i];
       for (int i = 0; i < kAdj; ++i)
-        for (int s = 0; s < ra.nseed; ++s) scr.adj(k, i, s, w) = 0.0;
+        for (int s = 0; s < ns; ++s) scr.adj(k, i, s, w) = R(0);
     }
-    auto load = [&](int k, double K[7], double E[4]) {
+    auto load = [&](int k, R K[7], R E[4]) {
       for (int i = 0; i < 7; ++i) K[i] = scr.fwd(k, i, w);
       for (int i = 0; i < 4; ++i) E[i] = scr.fwd(k, 7 + i, w);
     };
@@ -348,223 +339,266 @@ __global__ __launch_bounds__(kRespThreads) void resp_kernel(const KLines lines, 
     auto ooof = [&](int k) { return scr.fwd(k, 0, w); };
     auto Dof = [&](int k) {  // D_k, 0 <= k <= nd-2
       if (k == 0) return (ooof(1) - ooof(0)) / hof(1);
-      return fb_deriv<double>(ooof(k - 1), ooof(k), ooof(k + 1), hof(k), hof(k + 1));
+      return fb_deriv<R>(ooof(k - 1), ooof(k), ooof(k + 1), hof(k), hof(k + 1));
     };
+    R Io[4];
     {
-      double K0[7], E0[4];
+      R K0[7], E0[4];
       load(0, K0, E0);
-      double I[4] = {up ? E0[0] / K0[0] : 0.0, 0.0, 0.0, 0.0};
+      R I[4] = {up ? E0[0] / K0[0] : R(0), R(0), R(0), R(0)};
       for (int s = 0; s < 4; ++s) scr.fwd(0, 11 + s, w) = I[s];
       for (int k = 1; k <= nd - 2; ++k) {
-        double Kp[7], Ep[4], Kc[7], Ec[4];
+        R Kp[7], Ep[4], Kc[7], Ec[4];
         load(k - 1, Kp, Ep);
         load(k, Kc, Ec);
-        Kred<double> kp, kc;
-        double Sp[4], Sc[4];
-        reduce_ks<double>(Kp, Ep, kp, Sp);
-        reduce_ks<double>(Kc, Ec, kc, Sc);
-        const double h = hof(k);
-        const double tb = h * 0.5 * (Kp[0] + Kc[0]) + h * h / 12.0 * (Dof(k - 1) - Dof(k));
-        double etb, w0, w1;
-        sc_weights<double>(tb, etb, w0, w1);
-        double kv[4], B[4], In[4];
-        kmul<double>(kp, I, kv);
+        Kred<R> kp, kc;
+        R Sp[4], Sc[4];
+        reduce_ks<R>(Kp, Ep, kp, Sp);
+        reduce_ks<R>(Kc, Ec, kc, Sc);
+        const R h = hof(k);
+        const R tb = h * R(0.5) * (Kp[0] + Kc[0]) + h * h * R(1.0 / 12.0) * (Dof(k - 1) - Dof(k));
+        R etb, w0, w1;
+        sc_weights<R>(tb, etb, w0, w1);
+        R kv[4], B[4], In[4];
+        kmul<R>(kp, I, kv);
         for (int s = 0; s < 4; ++s) B[s] = -w0 * kv[s] + w0 * Sp[s] + w1 * Sc[s] + I[s] * etb;
-        solve_1pk<double>(kc, w1, B, In);
+        solve_1pk<R>(kc, w1, B, In);
         for (int s = 0; s < 4; ++s) {
           I[s] = In[s];
           scr.fwd(k, 11 + s, w) = In[s];
         }
       }
       const int ko = nd >= 3 ? nd - 2 : 0;
-      double Io[4];
       for (int s = 0; s < 4; ++s) Io[s] = scr.fwd(ko, 11 + s, w);
       for (int s = 0; s < 4; ++s) stokes[((size_t)col * nl + l) * 4 + s] = Io[s];
-      if (!(fabs(Io[0]) < 1e300)) atomicOr(status, 2);
-      // ---------------- reverse sweep
-      const int ns = ra.nseed;
-      double Ib[4][4];  // [seed][component]
-      for (int s = 0; s < 4; ++s)
-        for (int q = 0; q < 4; ++q) Ib[s][q] = 0.0;
-      if (ra.sink == SINK_RESP) {
-        for (int s = 0; s < 4; ++s) Ib[s][s] = 1.0;
-      } else {
-        const double *ob = ra.obs + ((size_t)col * nl + l) * 4;
-        for (int q = 0; q < 4; ++q) Ib[0][q] = ra.ws[q] * (Io[q] - ob[q]);
-      }
-      // carried adjoints of the reduced quantities of depth k (from step k+1)
-      double Kr_b[4][6], S_b[4][4], D_b[4];
-      for (int s = 0; s < 4; ++s) {
-        for (int q = 0; q < 6; ++q) Kr_b[s][q] = 0.0;
-        for (int q = 0; q < 4; ++q) S_b[s][q] = 0.0;
-        D_b[s] = 0.0;
-      }
-      // converts the adjoint of (K~, S) of depth k into raw (K, E) adjoints
-      auto flush_reduced = [&](int k, const double Kr[4][6], const double Sb[4][4]) {
-        double K[7], E[4];
-        load(k, K, E);
-        const double iI = 1.0 / K[0];
-        // a..f <- K[1], K[2], K[3], K[6], K[5], K[4]
-        const int map[6] = {1, 2, 3, 6, 5, 4};
-        for (int s = 0; s < ns; ++s) {
-          double kI = 0.0;
-          for (int q = 0; q < 6; ++q) {
-            scr.adj(k, map[q], s, w) += Kr[s][q] * iI;
-            kI -= Kr[s][q] * K[map[q]] * iI * iI;
-          }
-          for (int q = 0; q < 4; ++q) {
-            scr.adj(k, 7 + q, s, w) += Sb[s][q] * iI;
-            kI -= Sb[s][q] * E[q] * iI * iI;
-          }
-          scr.adj(k, 0, s, w) += kI;
+      if (!(fabs((double)Io[0]) < 1e300)) atomicOr(status, 2);
+    }
+    // ---------------- reverse sweep
+    R Ib[4][4];  // [seed][component]
+    for (int s = 0; s < 4; ++s)
+      for (int q = 0; q < 4; ++q) Ib[s][q] = R(0);
+    if (ra.sink == SINK_GRAD) {
+      const R *ob = static_cast<const R *>(ra.obs) + ((size_t)col * nl + l) * 4;
+      for (int q = 0; q < 4; ++q) Ib[0][q] = (R)ra.ws[q] * (Io[q] - ob[q]);
+    } else {
+      for (int s = 0; s < 4; ++s) Ib[s][s] = R(1);
+    }
+    // carried adjoints of the reduced quantities of depth k (from step k+1)
+    R Kr_b[4][6], S_b[4][4], D_b[4];
+    for (int s = 0; s < 4; ++s) {
+      for (int q = 0; q < 6; ++q) Kr_b[s][q] = R(0);
+      for (int q = 0; q < 4; ++q) S_b[s][q] = R(0);
+      D_b[s] = R(0);
+    }
+    // converts the adjoint of (K~, S) of depth k into raw (K, E) adjoints
+    auto flush_reduced = [&](int k, const R Kr[4][6], const R Sb[4][4]) {
+      R K[7], E[4];
+      load(k, K, E);
+      const R iI = R(1) / K[0];
+      const int map[6] = {1, 2, 3, 6, 5, 4};  // a..f <- K[1], K[2], K[3], K[6], K[5], K[4]
+      for (int s = 0; s < ns; ++s) {
+        R kI = R(0);
+        for (int q = 0; q < 6; ++q) {
+          scr.adj(k, map[q], s, w) += Kr[s][q] * iI;
+          kI -= Kr[s][q] * K[map[q]] * iI * iI;
         }
-      };
-      // propagates the adjoint of D_k into the opacities it is made of
-      auto flush_D = [&](int k, const double Db[4]) {
-        if (k == 0) {
-          const double h1 = hof(1);
-          for (int s = 0; s < ns; ++s) {
-            scr.adj(1, 0, s, w) += Db[s] / h1;
-            scr.adj(0, 0, s, w) -= Db[s] / h1;
-          }
-          return;
-        }
-        double gm, g0, gp;
-        fb_deriv_d(ooof(k - 1), ooof(k), ooof(k + 1), hof(k), hof(k + 1), gm, g0, gp);
-        for (int s = 0; s < ns; ++s) {
-          scr.adj(k - 1, 0, s, w) += Db[s] * gm;
-          scr.adj(k, 0, s, w) += Db[s] * g0;
-          scr.adj(k + 1, 0, s, w) += Db[s] * gp;
-        }
-      };
-      for (int k = nd - 2; k >= 1; --k) {
-        double Kp[7], Ep[4], Kc[7], Ec[4];
-        load(k - 1, Kp, Ep);
-        load(k, Kc, Ec);
-        Kred<double> kp, kc;
-        double Sp[4], Sc[4];
-        reduce_ks<double>(Kp, Ep, kp, Sp);
-        reduce_ks<double>(Kc, Ec, kc, Sc);
-        const double h = hof(k);
-        const double Dm = Dof(k - 1), D0 = Dof(k);
-        const double tb = h * 0.5 * (Kp[0] + Kc[0]) + h * h / 12.0 * (Dm - D0);
-        double etb, w0, w1, detb, dw0, dw1;
-        sc_weights<double>(tb, etb, w0, w1);
-        sc_weights_d(tb, detb, dw0, dw1);
-        double Ip[4], Ik[4];
         for (int q = 0; q < 4; ++q) {
-          Ip[q] = scr.fwd(k - 1, 11 + q, w);
-          Ik[q] = scr.fwd(k, 11 + q, w);
+          scr.adj(k, 7 + q, s, w) += Sb[s][q] * iI;
+          kI -= Sb[s][q] * E[q] * iI * iI;
         }
-        double kvp[4], kvc[4];
-        kmul<double>(kp, Ip, kvp);
-        kmul<double>(kc, Ik, kvc);
-        // transposed system: (1 + w1 K~)^T = 1 + w1 K~' with d, e, f negated
-        const Kred<double> kcT = {kc.a, kc.b, kc.c, -kc.d, -kc.e, -kc.f};
-        const Kred<double> kpT = {kp.a, kp.b, kp.c, -kp.d, -kp.e, -kp.f};
-        double Krc_b[4][6], Sc_b[4][4], Krp_b[4][6], Sp_b[4][4], D0_b[4], Dm_b[4];
-        for (int s = 0; s < ns; ++s) {
-          double Bb[4];
-          solve_1pk<double>(kcT, w1, Ib[s], Bb);
-          // A = 1 + w1 K~_k :  Abar = -Bb (x) I_k
-          double kb[6] = {0, 0, 0, 0, 0, 0};
-          double nb[4] = {-w1 * Bb[0], -w1 * Bb[1], -w1 * Bb[2], -w1 * Bb[3]};
-          kmul_adj(nb, Ik, kb);
-          double w1b = -(Bb[0] * kvc[0] + Bb[1] * kvc[1] + Bb[2] * kvc[2] + Bb[3] * kvc[3]);
-          w1b += Bb[0] * Sc[0] + Bb[1] * Sc[1] + Bb[2] * Sc[2] + Bb[3] * Sc[3];
-          double w0b = 0.0, etbb = 0.0;
-          for (int q = 0; q < 4; ++q) {
-            w0b += Bb[q] * (Sp[q] - kvp[q]);
-            etbb += Bb[q] * Ip[q];
-          }
-          for (int q = 0; q < 6; ++q) Krc_b[s][q] = Kr_b[s][q] + kb[q];
-          for (int q = 0; q < 4; ++q) Sc_b[s][q] = S_b[s][q] + w1 * Bb[q];
-          // B = -w0 K~_{k-1} I_{k-1} + w0 S_{k-1} + w1 S_k + etb I_{k-1}
-          double vb[4] = {-w0 * Bb[0], -w0 * Bb[1], -w0 * Bb[2], -w0 * Bb[3]};
-          double kbp[6] = {0, 0, 0, 0, 0, 0};
-          kmul_adj(vb, Ip, kbp);
-          double Ipb[4];
-          kmul<double>(kpT, vb, Ipb);
-          for (int q = 0; q < 4; ++q) Ib[s][q] = Ipb[q] + etb * Bb[q];
-          for (int q = 0; q < 6; ++q) Krp_b[s][q] = kbp[q];
-          for (int q = 0; q < 4; ++q) Sp_b[s][q] = w0 * Bb[q];
-          const double tbb = w0b * dw0 + w1b * dw1 + etbb * detb;
-          // tb = h/2 (oo_{k-1} + oo_k) + h^2/12 (D_{k-1} - D_k)
-          scr.adj(k - 1, 0, s, w) += tbb * h * 0.5;
-          scr.adj(k, 0, s, w) += tbb * h * 0.5;
-          Dm_b[s] = tbb * h * h / 12.0;
-          D0_b[s] = D_b[s] - tbb * h * h / 12.0;
-        }
-        // depth k is complete for K~, S: flush; D_k is complete
-        flush_reduced(k, Krc_b, Sc_b);
-        flush_D(k, D0_b);
-        for (int s = 0; s < ns; ++s) {
-          for (int q = 0; q < 6; ++q) Kr_b[s][q] = Krp_b[s][q];
-          for (int q = 0; q < 4; ++q) S_b[s][q] = Sp_b[s][q];
-          D_b[s] = Dm_b[s];
-        }
+        scr.adj(k, 0, s, w) += kI;
       }
-      // bottom point: boundary I_0 = (E_0[0]/K_0[0], 0, 0, 0) for the outgoing ray
-      if (nd >= 3) flush_D(0, D_b);
-      flush_reduced(0, Kr_b, S_b);
-      if (up) {
-        double K0b[7], E0b[4];
-        load(0, K0b, E0b);
+    };
+    // propagates the adjoint of D_k into the opacities it is made of
+    auto flush_D = [&](int k, const R Db[4]) {
+      if (k == 0) {
+        const R h1 = hof(1);
         for (int s = 0; s < ns; ++s) {
-          scr.adj(0, 7, s, w) += Ib[s][0] / K0b[0];
-          scr.adj(0, 0, s, w) -= Ib[s][0] * E0b[0] / (K0b[0] * K0b[0]);
+          scr.adj(1, 0, s, w) += Db[s] / h1;
+          scr.adj(0, 0, s, w) -= Db[s] / h1;
         }
+        return;
+      }
+      R gm, g0, gp;
+      fb_deriv_d<R>(ooof(k - 1), ooof(k), ooof(k + 1), hof(k), hof(k + 1), gm, g0, gp);
+      for (int s = 0; s < ns; ++s) {
+        scr.adj(k - 1, 0, s, w) += Db[s] * gm;
+        scr.adj(k, 0, s, w) += Db[s] * g0;
+        scr.adj(k + 1, 0, s, w) += Db[s] * gp;
+      }
+    };
+    for (int k = nd - 2; k >= 1; --k) {
+      R Kp[7], Ep[4], Kc[7], Ec[4];
+      load(k - 1, Kp, Ep);
+      load(k, Kc, Ec);
+      Kred<R> kp, kc;
+      R Sp[4], Sc[4];
+      reduce_ks<R>(Kp, Ep, kp, Sp);
+      reduce_ks<R>(Kc, Ec, kc, Sc);
+      const R h = hof(k);
+      const R Dm = Dof(k - 1), D0 = Dof(k);
+      const R tb = h * R(0.5) * (Kp[0] + Kc[0]) + h * h * R(1.0 / 12.0) * (Dm - D0);
+      R etb, w0, w1, detb, dw0, dw1;
+      sc_weights<R>(tb, etb, w0, w1);
+      sc_weights_d<R>(tb, detb, dw0, dw1);
+      R Ip[4], Ik[4];
+      for (int q = 0; q < 4; ++q) {
+        Ip[q] = scr.fwd(k - 1, 11 + q, w);
+        Ik[q] = scr.fwd(k, 11 + q, w);
+      }
+      R kvp[4], kvc[4];
+      kmul<R>(kp, Ip, kvp);
+      kmul<R>(kc, Ik, kvc);
+      // transposed system: (1 + w1 K~)^T = 1 + w1 K~' with d, e, f negated
+      const Kred<R> kcT = {kc.a, kc.b, kc.c, -kc.d, -kc.e, -kc.f};
+      const Kred<R> kpT = {kp.a, kp.b, kp.c, -kp.d, -kp.e, -kp.f};
+      R Krc_b[4][6], Sc_b[4][4], Krp_b[4][6], Sp_b[4][4], D0_b[4], Dm_b[4];
+      for (int s = 0; s < ns; ++s) {
+        R Bb[4];
+        solve_1pk<R>(kcT, w1, Ib[s], Bb);
+        // A = 1 + w1 K~_k :  Abar = -Bb (x) I_k
+        R kb[6] = {0, 0, 0, 0, 0, 0};
+        const R nb[4] = {-w1 * Bb[0], -w1 * Bb[1], -w1 * Bb[2], -w1 * Bb[3]};
+        kmul_adj<R>(nb, Ik, kb);
+        R w1b = -(Bb[0] * kvc[0] + Bb[1] * kvc[1] + Bb[2] * kvc[2] + Bb[3] * kvc[3]);
+        w1b += Bb[0] * Sc[0] + Bb[1] * Sc[1] + Bb[2] * Sc[2] + Bb[3] * Sc[3];
+        R w0b = R(0), etbb = R(0);
+        for (int q = 0; q < 4; ++q) {
+          w0b += Bb[q] * (Sp[q] - kvp[q]);
+          etbb += Bb[q] * Ip[q];
+        }
+        for (int q = 0; q < 6; ++q) Krc_b[s][q] = Kr_b[s][q] + kb[q];
+        for (int q = 0; q < 4; ++q) Sc_b[s][q] = S_b[s][q] + w1 * Bb[q];
+        // B = -w0 K~_{k-1} I_{k-1} + w0 S_{k-1} + w1 S_k + etb I_{k-1}
+        const R vb[4] = {-w0 * Bb[0], -w0 * Bb[1], -w0 * Bb[2], -w0 * Bb[3]};
+        R kbp[6] = {0, 0, 0, 0, 0, 0};
+        kmul_adj<R>(vb, Ip, kbp);
+        R Ipb[4];
+        kmul<R>(kpT, vb, Ipb);
+        for (int q = 0; q < 4; ++q) Ib[s][q] = Ipb[q] + etb * Bb[q];
+        for (int q = 0; q < 6; ++q) Krp_b[s][q] = kbp[q];
+        for (int q = 0; q < 4; ++q) Sp_b[s][q] = w0 * Bb[q];
+        const R tbb = w0b * dw0 + w1b * dw1 + etbb * detb;
+        // tb = h/2 (oo_{k-1} + oo_k) + h^2/12 (D_{k-1} - D_k)
+        scr.adj(k - 1, 0, s, w) += tbb * h * R(0.5);
+        scr.adj(k, 0, s, w) += tbb * h * R(0.5);
+        Dm_b[s] = tbb * h * h * R(1.0 / 12.0);
+        D0_b[s] = D_b[s] - tbb * h * h * R(1.0 / 12.0);
+      }
+      // depth k is complete for K~, S; D_k is complete
+      flush_reduced(k, Krc_b, Sc_b);
+      flush_D(k, D0_b);
+      for (int s = 0; s < ns; ++s) {
+        for (int q = 0; q < 6; ++q) Kr_b[s][q] = Krp_b[s][q];
+        for (int q = 0; q < 4; ++q) S_b[s][q] = Sp_b[s][q];
+        D_b[s] = Dm_b[s];
       }
     }
-    // ---------------- contraction with the local derivatives
+    // bottom point: boundary I_0 = (E_0[0]/K_0[0], 0, 0, 0) for the outgoing ray
+    if (nd >= 3) flush_D(0, D_b);
+    flush_reduced(0, Kr_b, S_b);
+    if (up) {
+      R K0b[7], E0b[4];
+      load(0, K0b, E0b);
+      for (int s = 0; s < ns; ++s) {
+        scr.adj(0, 7, s, w) += Ib[s][0] / K0b[0];
+        scr.adj(0, 0, s, w) -= Ib[s][0] * E0b[0] / (K0b[0] * K0b[0]);
+      }
+    }
+    // ---------------- contraction with the local derivatives; node sinks
+    // keep g[s][p] of every depth in the adjoint area of the scratch
     for (int k = 0; k < nd; ++k) {
       const int z = zof(k);
       double dK[7][7], dE[7][4];
-      local_derivs(lines, args, st, dst, z, lm, dK, dE);
-      for (int s = 0; s < ra.nseed; ++s) {
+      local_derivs<R>(lines, args, st, dst, z, lm, dK, dE);
+      for (int s = 0; s < ns; ++s) {
         double Kb[7], Eb[4];
-        for (int i = 0; i < 7; ++i) Kb[i] = scr.adj(k, i, s, w);
-        for (int i = 0; i < 4; ++i) Eb[i] = scr.adj(k, 7 + i, s, w);
+        for (int i = 0; i < 7; ++i) Kb[i] = (double)scr.adj(k, i, s, w);
+        for (int i = 0; i < 4; ++i) Eb[i] = (double)scr.adj(k, 7 + i, s, w);
         for (int p = 0; p < 7; ++p) {
           double g = 0.0;
           for (int i = 0; i < 7; ++i) g += Kb[i] * dK[p][i];
           for (int i = 0; i < 4; ++i) g += Eb[i] * dE[p][i];
           if (ra.sink == SINK_RESP)
-            out[((((size_t)col * 7 + p) * nd + z) * nl + l) * 4 + s] = g;
+            out[((((size_t)col * 7 + p) * nd + z) * nl + l) * 4 + s] = (R)g;
+          else if (ra.sink == SINK_GRAD)
+            atomicAdd(&out[((size_t)col * 7 + p) * nd + z], (R)g);
           else
-            atomicAdd(&out[((size_t)col * 7 + p) * nd + z], g);
+            scr.adj(k, p, s, w) = (R)g;  // entries 0..6 of depth k are no longer read
         }
       }
+    }
+    if (ra.sink == SINK_NODE) {
+      const R *nm = static_cast<const R *>(ra.nmap) + (size_t)col * ra.nnode * nd;
+      const R mk = ra.mask ? static_cast<const R *>(ra.mask)[(size_t)col * nl + l] : R(1);
+      for (int s = 0; s < 4; ++s)
+        for (int n = 0; n < ra.nnode; ++n) {
+          const int p = ra.pidx[n];
+          R acc = R(0);
+          for (int k = 0; k < nd; ++k) acc += scr.adj(k, p, s, w) * nm[(size_t)n * nd + zof(k)];
+          out[((size_t)col * nl * 4 + (size_t)l * 4 + s) * ra.nnode + n] = acc * (R)(ra.nscale[n] * ra.ws[s]) * mk;
+        }
     }
   }
 }
 
-static int run_resp(const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd, const void *state,
-                    const void *dstate, int nlambda, const double *lam, void *stokes, double *out_host, int sink,
-                    const double *obs, const double *ws) {
+template <typename R>
+int launch_resp(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, const void *ds_, const void *dd_,
+                       const double *dl, void *dstk_, void *dout_, void *dscr_, int chunk, hipStream_t stream) {
+  int threads = ((a.nlambda + 63) / 64) * 64;
+  if (threads > kRespThreads) threads = kRespThreads;
+  const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline);
+  int *status = device_status_slot();
+  if (!status) return SNAPI_ERR_DEVICE;
+  for (int c0 = 0; c0 < ncol; c0 += chunk) {
+    const int nc = (ncol - c0 < chunk) ? ncol - c0 : chunk;
+    ra.col0 = c0;
+    ra.nwork = (long long)nc * a.nlambda;
+    Scratch<R> scr{static_cast<R *>(dscr_), ra.nwork, a.nd};
+    hipLaunchKernelGGL(resp_kernel<R>, dim3(nc), dim3(threads), smem, stream, kl, a, ra, static_cast<const R *>(ds_),
+                       static_cast<const R *>(dd_), dl, scr, static_cast<R *>(dstk_), static_cast<R *>(dout_),
+                       status);
+    if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;
+  }
+  return SNAPI_OK;
+}
+
+// bytes of scratch per column
+size_t resp_scratch_per_col(int nlambda, int nd, size_t esz) {
+  return (size_t)nlambda * nd * (kFwd + kAdj * 4) * esz;
+}
+
+template int launch_resp<double>(const KLines &, const SynthArgs &, RespArgs, int, const void *, const void *,
+                                 const double *, void *, void *, void *, int, hipStream_t);
+template int launch_resp<float>(const KLines &, const SynthArgs &, RespArgs, int, const void *, const void *,
+                                const double *, void *, void *, void *, int, hipStream_t);
+
+static int run_resp(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
+                    const void *state, const void *dstate, int nlambda, const double *lam, void *stokes,
+                    void *out_host, int sink, const void *obs, const double *ws) {
   KLines kl;
   SynthArgs a;
   int rc = prepare_synth(lines, ray, ncol, nd, nlambda, kl, a);
   if (rc) return rc;
   if (!state || !dstate || !lam || !stokes || !out_host) return SNAPI_ERR_ARG;
   if (sink == SINK_GRAD && (!obs || !ws)) return SNAPI_ERR_ARG;
+  const size_t es = precision == SNAPI_FP64 ? 8 : 4;
   const size_t nf = a.nfield;
-  const size_t sb = (size_t)ncol * nf * nd * 8, db = 2 * sb, lb = (size_t)nlambda * 8;
-  const size_t stb = (size_t)ncol * nlambda * 4 * 8;
-  const size_t ob = sink == SINK_RESP ? (size_t)ncol * 7 * nd * nlambda * 4 * 8 : (size_t)ncol * 7 * nd * 8;
-  // columns per launch so that the scratch stays below ~2 GiB
-  const size_t per_col = (size_t)nlambda * nd * (kFwd + kAdj * 4) * 8;
+  const size_t sb = (size_t)ncol * nf * nd * es, db = 2 * sb, lb = (size_t)nlambda * 8;
+  const size_t stb = (size_t)ncol * nlambda * 4 * es;
+  const size_t ob = (sink == SINK_RESP ? (size_t)ncol * 7 * nd * nlambda * 4 : (size_t)ncol * 7 * nd) * es;
+  const size_t per_col = resp_scratch_per_col(nlambda, nd, es);
   size_t chunk_s = ((size_t)2 << 30) / (per_col > 0 ? per_col : 1);
   if (chunk_s < 1) chunk_s = 1;
   if (chunk_s > (size_t)ncol) chunk_s = (size_t)ncol;
   const int chunk = (int)chunk_s;
-  double *ds = nullptr, *dd = nullptr, *dl = nullptr, *dstk = nullptr, *dout = nullptr, *dscr = nullptr,
-         *dobs = nullptr;
+  void *ds = nullptr, *dd = nullptr, *dstk = nullptr, *dout = nullptr, *dscr = nullptr, *dobs = nullptr;
+  double *dl = nullptr;
   rc = SNAPI_OK;
-  if (hipMalloc((void **)&ds, sb) != hipSuccess || hipMalloc((void **)&dd, db) != hipSuccess ||
-      hipMalloc((void **)&dl, lb) != hipSuccess || hipMalloc((void **)&dstk, stb) != hipSuccess ||
-      hipMalloc((void **)&dout, ob) != hipSuccess || hipMalloc((void **)&dscr, per_col * chunk) != hipSuccess ||
-      (sink == SINK_GRAD && hipMalloc((void **)&dobs, stb) != hipSuccess)) {
+  if (hipMalloc(&ds, sb) != hipSuccess || hipMalloc(&dd, db) != hipSuccess ||
+      hipMalloc((void **)&dl, lb) != hipSuccess || hipMalloc(&dstk, stb) != hipSuccess ||
+      hipMalloc(&dout, ob) != hipSuccess || hipMalloc(&dscr, per_col * chunk) != hipSuccess ||
+      (sink == SINK_GRAD && hipMalloc(&dobs, stb) != hipSuccess)) {
     rc = SNAPI_ERR_DEVICE;
   } else {
     hipMemcpy(ds, state, sb, hipMemcpyHostToDevice);
@@ -573,24 +607,15 @@ static int run_resp(const snapi_line_table *lines, const snapi_ray *ray, int nco
     hipMemset(dout, 0, ob);
     if (dobs) hipMemcpy(dobs, obs, stb, hipMemcpyHostToDevice);
     reset_status();
-    RespArgs ra;
-    ra.nseed = sink == SINK_RESP ? 4 : 1;
+    RespArgs ra = {};
+    ra.nseed = sink == SINK_GRAD ? 1 : 4;
     ra.sink = sink;
     ra.obs = dobs;
     for (int s = 0; s < 4; ++s) ra.ws[s] = ws ? ws[s] : 1.0;
-    int threads = ((nlambda + 63) / 64) * 64;
-    if (threads > kRespThreads) threads = kRespThreads;
-    const size_t smem = ColumnTile<double>::bytes(nd, kl.nline);
-    int *status = device_status_slot();
-    for (int c0 = 0; c0 < ncol && rc == SNAPI_OK; c0 += chunk) {
-      const int nc = (ncol - c0 < chunk) ? ncol - c0 : chunk;
-      ra.col0 = c0;
-      ra.nwork = (long long)nc * nlambda;
-      Scratch scr{dscr, ra.nwork, nd};
-      hipLaunchKernelGGL(resp_kernel, dim3(nc), dim3(threads), smem, 0, kl, a, ra, ds, dd, dl, scr, dstk, dout,
-                         status);
-      if (hipGetLastError() != hipSuccess) rc = SNAPI_ERR_DEVICE;
-    }
+    if (precision == SNAPI_FP64)
+      rc = launch_resp<double>(kl, a, ra, ncol, ds, dd, dl, dstk, dout, dscr, chunk, 0);
+    else
+      rc = launch_resp<float>(kl, a, ra, ncol, ds, dd, dl, dstk, dout, dscr, chunk, 0);
     if (rc == SNAPI_OK && hipDeviceSynchronize() != hipSuccess) rc = SNAPI_ERR_DEVICE;
     if (rc == SNAPI_OK) {
       hipMemcpy(stokes, dstk, stb, hipMemcpyDeviceToHost);
@@ -615,19 +640,19 @@ using namespace snapi;
 extern "C" int snapi_stokes_responses(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol,
                                       int nd, const void *state, const void *dstate, int nlambda, const double *lam,
                                       void *stokes, void *resp) {
-  if (precision != SNAPI_FP64) return SNAPI_ERR_UNSUPPORTED;
-  return run_resp(lines, ray, ncol, nd, state, dstate, nlambda, lam, stokes, static_cast<double *>(resp), SINK_RESP,
-                  nullptr, nullptr);
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
+  return run_resp(precision, lines, ray, ncol, nd, state, dstate, nlambda, lam, stokes, resp, SINK_RESP, nullptr,
+                  nullptr);
 }
 
 extern "C" int snapi_stokes_adjoint(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol,
                                     int nd, const void *state, const void *dstate, int nlambda, const double *lam,
-                                    const double *obs, const double *wstokes, double *grad) {
-  if (precision != SNAPI_FP64) return SNAPI_ERR_UNSUPPORTED;
+                                    const void *obs, const double *wstokes, void *grad) {
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
   if (!grad) return SNAPI_ERR_ARG;
   const size_t stb = (size_t)ncol * nlambda * 4;
   double *tmp = new double[stb > 0 ? stb : 1];
-  int rc = run_resp(lines, ray, ncol, nd, state, dstate, nlambda, lam, tmp, grad, SINK_GRAD, obs, wstokes);
+  int rc = run_resp(precision, lines, ray, ncol, nd, state, dstate, nlambda, lam, tmp, grad, SINK_GRAD, obs, wstokes);
   delete[] tmp;
   return rc;
 }

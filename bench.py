@@ -11,6 +11,13 @@ mixed path against the FP64 CPU oracle on a sample of columns, the roofline
 fraction of the synthesis kernel, and the oracle's own throughput on the
 host cores (cpu_baseline, FP64 port, OpenMP).
 
+Second leg ("lm"): batched Levenberg-Marquardt inversion of 256 x 256 pixels
+drawn as the cube (observations: FP64 GPU synthesis plus Gaussian noise of
+1e-3 Ic; per-pixel masks keeping 120-301 wavelengths; 7 kinematic/magnetic
+nodes started +/-20 % from truth; at most 30 iterations), mixed precision
+with adjoint Jacobians, against the FP64 path with central-difference
+Jacobians on 8192 of the pixels: LM pixel-iterations per second of each.
+
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -44,6 +51,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096, help="columns timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--lm-side", type=int, default=256, help="inversion leg: side of the pixel square")
+    ap.add_argument("--lm-fd-pixels", type=int, default=8192, help="pixels of the FP64 difference-Jacobian run")
+    ap.add_argument("--no-lm", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the hot path is per column and shards by column tiles; "
@@ -137,6 +147,10 @@ def main():
         log("cpu oracle: %.4g lambda-points/s on %d threads; mixed max err %.3e Ic" % (cpu["value"], nthreads,
                                                                                       err.max()))
 
+    lm = None
+    if not args.no_lm:
+        lm = lm_leg(torch, dev, tmpl, lam_d, stream, args.lm_side, args.lm_fd_pixels)
+
     line = {"metric": METRIC, "value": value, "unit": "lambda-point formal solves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
@@ -144,8 +158,92 @@ def main():
                                    "mu=1, Fe I 6301.5/6302.5 + H lines + H- continuum, mixed FP32/FP64"
                                    % (nx, ny, nd, nlam),
                        "columns": ncol, "depths": nd, "wavelengths": nlam, "lines": table.nline},
-            "roofline": roof, "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64}
+            "roofline": roof, "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64, "lm": lm}
     print(json.dumps(line))
+
+
+def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):
+    from snapi_amd import abi, api, api_extra, invert, workloads
+    table, ray = tmpl["table"], tmpl["ray"]
+    nlam = lam_d.shape[0]
+    t0 = time.time()
+    prob = workloads.inversion_problem(tmpl, side, side, seed=3)
+    npix = prob["state"].shape[0]
+    nn = len(prob["nodes"])
+    model = invert.node_model(prob["nodes"])
+    ws = (1.0, 4.0, 4.0, 2.0)
+    st64 = torch.from_numpy(prob["state"]).to(dev)
+    obs64 = torch.empty((npix, nlam, 4), dtype=torch.float64, device=dev)
+    api.synth_stokes_dev(table, ray, st64, lam_d, obs64, abi.SNAPI_FP64, stream)
+    torch.cuda.synchronize()
+    ic = obs64[:, :, 0].amax(dim=1)
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    noise = torch.randn(obs64.shape, dtype=torch.float64, device=dev, generator=g) * (1e-3 * ic)[:, None, None]
+    obs64 += noise
+    mask64 = torch.from_numpy(prob["mask"]).to(dev)
+    start = torch.from_numpy(prob["start"]).to(dev)
+    # chi^2 expected from the noise alone: sum over kept points of (w_s sigma)^2
+    wsq = sum(w * w for w in ws)
+    chi2_noise = (mask64.sum(dim=1) * wsq * (1e-3 * ic) ** 2)
+    log("inversion workload: %d pixels, %d nodes, built in %.1fs" % (npix, nn, time.time() - t0))
+
+    def run(n, precision, jacobian, label):
+        dt_ = torch.float32 if precision == abi.SNAPI_MIXED else torch.float64
+        st_t = st64[:n].to(dt_).contiguous()
+        ob_t = obs64[:n].to(dt_).contiguous()
+        mk_t = mask64[:n].to(dt_).contiguous()
+        nodes = torch.empty((n, nn), dtype=torch.float64, device=dev)
+        chi2 = torch.empty(n, dtype=torch.float64, device=dev)
+        iters = torch.empty(n, dtype=torch.int32, device=dev)
+        # warm-up on a few pixels (kernel loading), not timed
+        invert.invert_dev(table, ray, model, st_t[:64], lam_d, ob_t[:64], mk_t[:64], start[:64], nodes[:64],
+                          chi2[:64], iters[:64], wstokes=ws, max_iter=2, precision=precision, jacobian=jacobian,
+                          stream=stream)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        total = invert.invert_dev(table, ray, model, st_t, lam_d, ob_t, mk_t, start[:n], nodes, chi2, iters,
+                                  wstokes=ws, max_iter=30, precision=precision, jacobian=jacobian, stream=stream)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t1
+        it = iters.cpu().numpy()
+        red = (chi2 / chi2_noise[:n]).cpu().numpy()
+        nd_ = nodes.cpu().numpy()
+        tr = prob["truth"][:n]
+        strong = tr[:, 3] > 300.0
+        berr = np.median(np.abs(nd_[strong, 3] - tr[strong, 3]) / tr[strong, 3])
+        out = {"pixels": n, "pixel_iterations": int(total), "seconds": round(sec, 3),
+               "pixel_iter_per_s": total / sec, "mean_iterations": float(it.mean()),
+               "at_iteration_limit": float(np.mean(it >= 30)), "median_chi2_over_noise": float(np.median(red)),
+               "median_B_rel_err_above_300G": float(berr)}
+        log("%s: %d pixel-iterations in %.2fs = %.4g /s; mean iterations %.1f; median chi2/noise %.3g"
+            % (label, total, sec, total / sec, it.mean(), np.median(red)))
+        return out
+
+    mixed = run(npix, abi.SNAPI_MIXED, 0, "mixed adjoint")
+    fd = run(min(nfd, npix), abi.SNAPI_FP64, 1, "fp64 differences")
+    # Jacobian error of the mixed path vs the FP64 path (per-depth responses,
+    # 8 columns, every 10th wavelength)
+    lam_h = lam_d.cpu().numpy()[::10]
+    stc = prob["state"][:8]
+    nf_, nd_ = stc.shape[1:]
+    dz = np.zeros((8, 2, nf_, nd_))
+    _, r64 = api_extra.stokes_responses(table, ray, stc, dz, lam_h, precision=abi.SNAPI_FP64)
+    _, r32 = api_extra.stokes_responses(table, ray, stc.astype(np.float32), dz.astype(np.float32), lam_h,
+                                        precision=abi.SNAPI_MIXED)
+    jerr = 0.0
+    for c in range(8):
+        for p in range(2, 7):
+            sc = np.abs(r64[c, p]).max(axis=(0, 1))
+            sc = np.maximum(sc, 1e-3 * sc.max())
+            jerr = max(jerr, float(np.max(np.abs(r32[c, p].astype(np.float64) - r64[c, p]) / sc)))
+    return {"workload": "batched LM inversion, %d pixels of the cube, 71 depths, %d wavelengths masked to "
+                        "120-%d per pixel, %d nodes (2 vLOS, 1 vmicro, 2 B, 1 incl, 1 azi), noise 1e-3 Ic, "
+                        "start +/-20 %%, <=30 iterations" % (npix, nlam, nlam, nn),
+            "value": mixed["pixel_iter_per_s"], "unit": "LM pixel-iterations/s",
+            "mixed_adjoint": mixed, "fp64_differences": fd,
+            "speedup_vs_fp64_differences": mixed["pixel_iter_per_s"] / fd["pixel_iter_per_s"],
+            "jacobian_rel_err_mixed_vs_fp64": jerr}
 
 
 if __name__ == "__main__":

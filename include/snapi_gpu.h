@@ -211,6 +211,18 @@ int snapi_invert(int precision, int jacobian, const snapi_line_table *lines, con
                  double *nodes, double *chi2, int *iterations, long long *pixel_iterations);
 
 /*
+ * snapi_invert with the arrays already in device memory (state_dev, lam_dev,
+ * obs_dev, mask_dev as above; nodes0_dev [npix][nnode] start values,
+ * nodes_dev, chi2_dev, iterations_dev out), all work on `stream` (a
+ * hipStream_t; null = the default stream).  Same semantics and errors.
+ */
+int snapi_invert_dev(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                     const snapi_node_model *model, int npix, int nd, const void *state_dev, int nlambda,
+                     const double *lam_dev, const void *obs_dev, const void *mask_dev, const double *wstokes,
+                     double lm_start, int max_iter, int nslot, const double *nodes0_dev, double *nodes_dev,
+                     double *chi2_dev, int *iterations_dev, long long *pixel_iterations, void *stream);
+
+/*
  * Scalar formal solution with the diagonal approximate Lambda operator for a
  * batch of (column, frequency, ray) problems:
  *   op, em [nprob][nd], grid [nprob][nd]; I, lstar [nprob][nd]

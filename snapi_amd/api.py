@@ -33,6 +33,13 @@ def synth_stokes_dev(table, ray, state_t, lam_t, out_t, precision=abi.SNAPI_FP64
     `stream` (a torch.cuda.Stream or None for the current stream)."""
     import torch
     ncol, nf, nd = state_t.shape
+    nl = lam_t.shape[0]
+    dt = torch.float64 if precision == abi.SNAPI_FP64 else torch.float32
+    for t, d, shp in ((state_t, dt, (ncol, abi.nfield(table.nline), nd)), (lam_t, torch.float64, (nl,)),
+                      (out_t, dt, (ncol, nl, 4))):
+        if t.dtype != d or tuple(t.shape) != shp or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("synth_stokes_dev: expected a contiguous device tensor %s of shape %s, got %s %s"
+                             % (d, shp, t.dtype, tuple(t.shape)))
     s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
     rc = lib().snapi_synth_stokes_dev(precision, C.byref(table), C.byref(ray), ncol, nd,
                                       C.c_void_p(state_t.data_ptr()), lam_t.shape[0], C.c_void_p(lam_t.data_ptr()),

@@ -336,7 +336,8 @@ template <typename R>
 static int invert_impl(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
                        const NodeArgs &na, int npix, int nd, const void *pstate_h, int nlambda, const double *lam_h,
                        const void *obs_h, const void *mask_h, const double *ws, double lm0, int max_iter, int nslot,
-                       double *nodes_io, double *chi2_out, int *iter_out, long long *stats) {
+                       const double *nodes0, double *nodes_out, double *chi2_out, int *iter_out, long long *stats,
+                       bool dev, hipStream_t st) {
   KLines kl;
   SynthArgs a;
   int rc = prepare_synth(lines, ray, 1, nd, nlambda, kl, a);
@@ -350,14 +351,38 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   double *dlam = nullptr, *pnodes0 = nullptr, *snodes = nullptr, *tnodes = nullptr, *JTJ = nullptr, *rhs = nullptr;
   double *corr = nullptr, *onodes = nullptr, *ochi2 = nullptr, *fdn = nullptr, *hv = nullptr, *dbl = nullptr;
   int *ints = nullptr, *queue = nullptr, *oiter = nullptr, *count = nullptr;
+  // caller's buffers (device variant) or copies of the host arrays
+  void *own[8] = {};
   SlotState ss;
   int chunk = nslot;
   size_t per_col = 0;
   int *status = device_status_slot();
   if (!status) return SNAPI_ERR_DEVICE;
-  HIPCHK(hipMalloc((void **)&pstate, (size_t)npix * ns * es));
-  HIPCHK(hipMalloc((void **)&pobs, (size_t)npix * nl * 4 * es));
-  HIPCHK(hipMalloc((void **)&pmask, (size_t)npix * nl * es));
+  if (dev) {
+    pstate = (R *)pstate_h;
+    pobs = (R *)obs_h;
+    pmask = (R *)mask_h;
+    dlam = (double *)lam_h;
+    pnodes0 = (double *)nodes0;
+    onodes = nodes_out;
+    ochi2 = chi2_out;
+    oiter = iter_out;
+  } else {
+    HIPCHK(hipMalloc((void **)&pstate, (size_t)npix * ns * es));
+    HIPCHK(hipMalloc((void **)&pobs, (size_t)npix * nl * 4 * es));
+    HIPCHK(hipMalloc((void **)&pmask, (size_t)npix * nl * es));
+    HIPCHK(hipMalloc((void **)&dlam, (size_t)nl * 8));
+    HIPCHK(hipMalloc((void **)&pnodes0, (size_t)npix * nn * 8));
+    HIPCHK(hipMalloc((void **)&onodes, (size_t)npix * nn * 8));
+    HIPCHK(hipMalloc((void **)&ochi2, (size_t)npix * 8));
+    HIPCHK(hipMalloc((void **)&oiter, (size_t)npix * sizeof(int)));
+    own[0] = pstate; own[1] = pobs; own[2] = pmask; own[3] = dlam;
+    own[4] = pnodes0; own[5] = onodes; own[6] = ochi2; own[7] = oiter;
+  }
+  if (!pstate || !pobs || !dlam || !pnodes0 || !onodes || !ochi2 || !oiter) {
+    rc = SNAPI_ERR_ARG;
+    goto done;
+  }
   HIPCHK(hipMalloc((void **)&sstate, (size_t)nslot * ns * es));
   HIPCHK(hipMalloc((void **)&tstate, (size_t)nslot * ns * es));
   HIPCHK(hipMalloc((void **)&dzero, (size_t)nslot * 2 * ns * es));
@@ -368,19 +393,14 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   HIPCHK(hipMalloc((void **)&J, (size_t)nslot * nl * 4 * nn * es));
   HIPCHK(hipMalloc((void **)&r, (size_t)nslot * nl * 4 * es));
   HIPCHK(hipMalloc((void **)&nmap, (size_t)nslot * nn * nd * es));
-  HIPCHK(hipMalloc((void **)&dlam, (size_t)nl * 8));
-  HIPCHK(hipMalloc((void **)&pnodes0, (size_t)npix * nn * 8));
   HIPCHK(hipMalloc((void **)&snodes, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&tnodes, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&JTJ, (size_t)nslot * nn * nn * 8));
   HIPCHK(hipMalloc((void **)&rhs, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&corr, (size_t)nslot * nn * 8));
-  HIPCHK(hipMalloc((void **)&onodes, (size_t)npix * nn * 8));
-  HIPCHK(hipMalloc((void **)&ochi2, (size_t)npix * 8));
   HIPCHK(hipMalloc((void **)&dbl, (size_t)nslot * 6 * 8));
   HIPCHK(hipMalloc((void **)&ints, (size_t)nslot * 6 * sizeof(int)));
   HIPCHK(hipMalloc((void **)&queue, 2 * sizeof(int)));
-  HIPCHK(hipMalloc((void **)&oiter, (size_t)npix * sizeof(int)));
   count = queue + 1;
   ss.pix = ints;
   ss.iter = ints + nslot;
@@ -405,18 +425,21 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     HIPCHK(hipMalloc((void **)&fdn, (size_t)nslot * nv * nn * 8));
     HIPCHK(hipMalloc((void **)&hv, (size_t)nslot * nn * 8));
   }
-  HIPCHK(hipMemcpy(pstate, pstate_h, (size_t)npix * ns * es, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(pobs, obs_h, (size_t)npix * nl * 4 * es, hipMemcpyHostToDevice));
-  if (mask_h) HIPCHK(hipMemcpy(pmask, mask_h, (size_t)npix * nl * es, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(dlam, lam_h, (size_t)nl * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(pnodes0, nodes_io, (size_t)npix * nn * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(dzero, 0, (size_t)nslot * 2 * ns * es));
-  HIPCHK(hipMemset(ints, 0, (size_t)nslot * 6 * sizeof(int)));
-  HIPCHK(hipMemset(dbl, 0, (size_t)nslot * 6 * 8));
-  HIPCHK(hipMemset(queue, 0, 2 * sizeof(int)));
+  if (!dev) {
+    HIPCHK(hipMemcpy(pstate, pstate_h, (size_t)npix * ns * es, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pobs, obs_h, (size_t)npix * nl * 4 * es, hipMemcpyHostToDevice));
+    if (mask_h) HIPCHK(hipMemcpy(pmask, mask_h, (size_t)npix * nl * es, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dlam, lam_h, (size_t)nl * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pnodes0, nodes0, (size_t)npix * nn * 8, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemsetAsync(dzero, 0, (size_t)nslot * 2 * ns * es, st));
+  HIPCHK(hipMemsetAsync(ints, 0, (size_t)nslot * 6 * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(dbl, 0, (size_t)nslot * 6 * 8, st));
+  HIPCHK(hipMemsetAsync(queue, 0, 2 * sizeof(int), st));
   {
     std::vector<int> neg(nslot, -1);
-    HIPCHK(hipMemcpy(ss.pix, neg.data(), nslot * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(ss.pix, neg.data(), nslot * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
   }
   reset_status();
   {
@@ -435,76 +458,80 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     ra.active = ss.needj;
     long long pixel_iters = 0;
     for (int it = 0;; ++it) {
-      hipLaunchKernelGGL(refill_kernel<R>, dim3(nslot), dim3(256), 0, 0, na, nslot, npix, nf, nd, nl, queue, ss, pstate,
+      hipLaunchKernelGGL(refill_kernel<R>, dim3(nslot), dim3(256), 0, st, na, nslot, npix, nf, nd, nl, queue, ss, pstate,
                          pobs, mask_h ? pmask : (R *)nullptr, pnodes0, lm0, sstate, tstate, sobs, smask, snodes, onodes,
                          ochi2, oiter);
-      HIPCHK(hipMemset(count, 0, sizeof(int)));
-      hipLaunchKernelGGL(count_active_kernel, dim3((nslot + 255) / 256), dim3(256), 0, 0, nslot, ss.active, count);
+      HIPCHK(hipMemsetAsync(count, 0, sizeof(int), st));
+      hipLaunchKernelGGL(count_active_kernel, dim3((nslot + 255) / 256), dim3(256), 0, st, nslot, ss.active, count);
       int nact = 0;
-      HIPCHK(hipMemcpy(&nact, count, sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpyAsync(&nact, count, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
       if (nact == 0) break;
       pixel_iters += nact;
       // model -> state and Jacobian where the model changed
-      hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, 0, na, nslot, nd, nf, snodes,
+      hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, snodes,
                          (size_t)nn, sstate, ns, nmap, ss.needj);
       if (jacobian == 0) {
-        rc = launch_resp<R>(kl, a, ra, nslot, sstate, dzero, dlam, S, J, scr, chunk, 0);
+        rc = launch_resp<R>(kl, a, ra, nslot, sstate, dzero, dlam, S, J, scr, chunk, st);
         if (rc) goto done;
       } else {
-        hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, 0, na, nslot, snodes, fdn, hv,
+        hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, fdn, hv,
                            ss.needj);
         // (2n+1) copies of every slot's state with the node sets applied
         for (int k = 0; k < nv; ++k) {
           HIPCHK(hipMemcpy2DAsync(fdstate + (size_t)k * ns, nv * ns * es, sstate, ns * es, ns * es, nslot,
-                                  hipMemcpyDeviceToDevice, 0));
-          hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, 0, na, nslot, nd, nf,
+                                  hipMemcpyDeviceToDevice, st));
+          hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf,
                              fdn + (size_t)k * nn, (size_t)nv * nn, fdstate + (size_t)k * ns, (size_t)nv * ns,
                              (R *)nullptr, ss.needj);
         }
         // fdstate is [slot][node set]: one synthesis over nslot * (2n+1) columns
-        rc = snapi_synth_stokes_dev(precision, lines, ray, nslot * nv, nd, fdstate, nl, dlam, Sall, nullptr);
+        rc = snapi_synth_stokes_dev(precision, lines, ray, nslot * nv, nd, fdstate, nl, dlam, Sall, st);
         if (rc) goto done;
-        hipLaunchKernelGGL(fdjac_kernel<R>, dim3(nslot), dim3(256), 0, 0, na, nslot, nl, Sall, hv, smask, ws[0], ws[1],
+        hipLaunchKernelGGL(fdjac_kernel<R>, dim3(nslot), dim3(256), 0, st, na, nslot, nl, Sall, hv, smask, ws[0], ws[1],
                            ws[2], ws[3], S, J, ss.needj);
       }
       // chi^2 of the current model (where it changed)
-      hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, 0, nslot, nl, S, sobs, smask, ws[0], ws[1],
+      hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, S, sobs, smask, ws[0], ws[1],
                          ws[2], ws[3], r, ss.chi2, ss.needj);
       // normal equations and damped solve
-      launch_lm_normal<R>(nslot, nl * 4, nn, J, r, JTJ, rhs, nullptr, ss.active, 0);
-      launch_lm_solve(nslot, nn, JTJ, rhs, ss.lm, corr, ss.active, 0);
+      launch_lm_normal<R>(nslot, nl * 4, nn, J, r, JTJ, rhs, nullptr, ss.active, st);
+      launch_lm_solve(nslot, nn, JTJ, rhs, ss.lm, corr, ss.active, st);
       // trial model and its synthesis
-      hipLaunchKernelGGL(trial_kernel, dim3((nslot + 127) / 128), dim3(128), 0, 0, na, nslot, snodes, corr, tnodes,
+      hipLaunchKernelGGL(trial_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, corr, tnodes,
                          ss.active);
-      hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, 0, na, nslot, nd, nf, tnodes,
+      hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, tnodes,
                          (size_t)nn, tstate, ns, (R *)nullptr, ss.active);
-      rc = snapi_synth_stokes_dev(precision, lines, ray, nslot, nd, tstate, nl, dlam, St, nullptr);
+      rc = snapi_synth_stokes_dev(precision, lines, ray, nslot, nd, tstate, nl, dlam, St, st);
       if (rc) goto done;
-      hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, 0, nslot, nl, St, sobs, smask, ws[0], ws[1],
+      hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, St, sobs, smask, ws[0], ws[1],
                          ws[2], ws[3], (R *)nullptr, ss.chi2t, ss.active);
-      hipLaunchKernelGGL(update_kernel, dim3((nslot + 127) / 128), dim3(128), 0, 0, na, nslot, max_iter, ss, snodes,
+      hipLaunchKernelGGL(update_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, max_iter, ss, snodes,
                          tnodes);
       if (hipGetLastError() != hipSuccess) {
         rc = SNAPI_ERR_DEVICE;
         goto done;
       }
     }
-    if (hipDeviceSynchronize() != hipSuccess) {
+    if (hipStreamSynchronize(st) != hipSuccess) {
       rc = SNAPI_ERR_DEVICE;
       goto done;
     }
-    HIPCHK(hipMemcpy(nodes_io, onodes, (size_t)npix * nn * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(chi2_out, ochi2, (size_t)npix * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(iter_out, oiter, (size_t)npix * sizeof(int), hipMemcpyDeviceToHost));
+    if (!dev) {
+      HIPCHK(hipMemcpy(nodes_out, onodes, (size_t)npix * nn * 8, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(chi2_out, ochi2, (size_t)npix * 8, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(iter_out, oiter, (size_t)npix * sizeof(int), hipMemcpyDeviceToHost));
+    }
     if (stats) stats[0] = pixel_iters;
     rc = read_status();
   }
 done:
-  hipFree(pstate); hipFree(pobs); hipFree(pmask); hipFree(sstate); hipFree(tstate); hipFree(dzero);
+  for (void *p : own) hipFree(p);
+  hipFree(sstate); hipFree(tstate); hipFree(dzero);
   hipFree(sobs); hipFree(smask); hipFree(S); hipFree(St); hipFree(J); hipFree(r); hipFree(nmap);
-  hipFree(fdstate); hipFree(Sall); hipFree(scr); hipFree(dlam); hipFree(pnodes0); hipFree(snodes);
-  hipFree(tnodes); hipFree(JTJ); hipFree(rhs); hipFree(corr); hipFree(onodes); hipFree(ochi2); hipFree(fdn);
-  hipFree(hv); hipFree(dbl); hipFree(ints); hipFree(queue); hipFree(oiter);
+  hipFree(fdstate); hipFree(Sall); hipFree(scr); hipFree(snodes);
+  hipFree(tnodes); hipFree(JTJ); hipFree(rhs); hipFree(corr); hipFree(fdn);
+  hipFree(hv); hipFree(dbl); hipFree(ints); hipFree(queue);
   return rc;
 }
 
@@ -512,18 +539,14 @@ done:
 
 using namespace snapi;
 
-extern "C" int snapi_invert(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
-                            const snapi_node_model *model, int npix, int nd, const void *state, int nlambda,
-                            const double *lam, const void *obs, const void *mask, const double *wstokes,
-                            double lm_start, int max_iter, int nslot, double *nodes, double *chi2, int *iterations,
-                            long long *pixel_iterations) {
-  if (!lines || !ray || !model || !state || !lam || !obs || !wstokes || !nodes || !chi2 || !iterations)
-    return SNAPI_ERR_ARG;
+static int setup_nodes(const snapi_line_table *lines, const snapi_ray *ray, const snapi_node_model *model, int npix,
+                       int nd, int nlambda, int jacobian, int max_iter, NodeArgs &na) {
+  if (!lines || !ray || !model) return SNAPI_ERR_ARG;
   if (npix <= 0 || nd < 3 || nlambda <= 0 || max_iter <= 0) return SNAPI_ERR_ARG;
   if (model->nnode <= 0 || model->nnode > SNAPI_MAX_NODES - 1) return SNAPI_ERR_ARG;
   if (jacobian != SNAPI_JAC_ADJOINT && jacobian != SNAPI_JAC_DIFFERENCES) return SNAPI_ERR_ARG;
   if (cos(ray->theta) > -0.999999) return SNAPI_ERR_UNSUPPORTED;  // node responses assume the mu = 1 ray
-  NodeArgs na = {};
+  na = NodeArgs{};
   na.nnode = model->nnode;
   for (int n = 0; n < na.nnode; ++n) {
     const int c = model->param[n];
@@ -533,12 +556,48 @@ extern "C" int snapi_invert(int precision, int jacobian, const snapi_line_table 
     na.scale[n] = c == NP_VT ? 1E5 : c == NP_VS ? 1E5 : c == NP_B ? 1000.0 : 1.0;
     na.step[n] = (c == NP_VT || c == NP_VS) ? 20.0 : c == NP_B ? 0.01 : 0.001;
   }
+  return SNAPI_OK;
+}
+
+extern "C" int snapi_invert(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                            const snapi_node_model *model, int npix, int nd, const void *state, int nlambda,
+                            const double *lam, const void *obs, const void *mask, const double *wstokes,
+                            double lm_start, int max_iter, int nslot, double *nodes, double *chi2, int *iterations,
+                            long long *pixel_iterations) {
+  if (!state || !lam || !obs || !wstokes || !nodes || !chi2 || !iterations) return SNAPI_ERR_ARG;
+  NodeArgs na;
+  int rc = setup_nodes(lines, ray, model, npix, nd, nlambda, jacobian, max_iter, na);
+  if (rc) return rc;
   if (nslot <= 0 || nslot > npix) nslot = npix;
   if (precision == SNAPI_FP64)
     return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
-                               lm_start, max_iter, nslot, nodes, chi2, iterations, pixel_iterations);
+                               lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0);
   if (precision == SNAPI_MIXED)
     return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
-                              lm_start, max_iter, nslot, nodes, chi2, iterations, pixel_iterations);
+                              lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0);
+  return SNAPI_ERR_ARG;
+}
+
+extern "C" int snapi_invert_dev(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                                const snapi_node_model *model, int npix, int nd, const void *state_dev, int nlambda,
+                                const double *lam_dev, const void *obs_dev, const void *mask_dev,
+                                const double *wstokes, double lm_start, int max_iter, int nslot,
+                                const double *nodes0_dev, double *nodes_dev, double *chi2_dev, int *iterations_dev,
+                                long long *pixel_iterations, void *stream) {
+  if (!state_dev || !lam_dev || !obs_dev || !wstokes || !nodes0_dev || !nodes_dev || !chi2_dev || !iterations_dev)
+    return SNAPI_ERR_ARG;
+  NodeArgs na;
+  int rc = setup_nodes(lines, ray, model, npix, nd, nlambda, jacobian, max_iter, na);
+  if (rc) return rc;
+  if (nslot <= 0 || nslot > npix) nslot = npix;
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == SNAPI_FP64)
+    return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
+                               mask_dev, wstokes, lm_start, max_iter, nslot, nodes0_dev, nodes_dev, chi2_dev,
+                               iterations_dev, pixel_iterations, true, st);
+  if (precision == SNAPI_MIXED)
+    return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
+                              mask_dev, wstokes, lm_start, max_iter, nslot, nodes0_dev, nodes_dev, chi2_dev,
+                              iterations_dev, pixel_iterations, true, st);
   return SNAPI_ERR_ARG;
 }

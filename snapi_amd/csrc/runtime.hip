@@ -27,7 +27,8 @@ int *device_status_slot() {
 
 void reset_status() {
   int *s = device_status_slot();
-  if (s) hipMemset(s, 0, sizeof(int));
+  // completed before returning, so work on any stream sees the cleared flag
+  if (s && hipMemset(s, 0, sizeof(int)) == hipSuccess) (void)hipStreamSynchronize(nullptr);
 }
 
 int read_status() {

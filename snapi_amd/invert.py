@@ -1,4 +1,6 @@
-"""Batched LM inversion (snapi_invert) and the node model it uses."""
+"""Batched LM inversion (snapi_invert, snapi_invert_dev) and the node model it
+uses.  lm_start defaults to the reference's observable start_lambda (1.0,
+cmlib/obs.cc:50)."""
 import ctypes as C
 
 import numpy as np
@@ -23,12 +25,14 @@ def node_model(nodes):
     return m
 
 
-def invert(table, ray, model, state, lam_vac, obs, nodes0, mask=None, wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1e3,
+def invert(table, ray, model, state, lam_vac, obs, nodes0, mask=None, wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0,
            max_iter=30, nslot=0, precision=abi.SNAPI_FP64, jacobian=0):
     """Returns (nodes [npix][nnode], chi2 [npix], iterations [npix], total pixel-iterations)."""
     dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
     st = np.ascontiguousarray(state, dtype=dt)
     npix, nf, nd = st.shape
+    if nf != abi.nfield(table.nline):
+        raise ValueError("state has %d fields, line table needs %d" % (nf, abi.nfield(table.nline)))
     lam = np.ascontiguousarray(lam_vac, dtype=np.float64)
     ob = np.ascontiguousarray(obs, dtype=dt).reshape(npix, len(lam), 4)
     mk = None if mask is None else np.ascontiguousarray(mask, dtype=dt).reshape(npix, len(lam))
@@ -49,3 +53,42 @@ def invert(table, ray, model, state, lam_vac, obs, nodes0, mask=None, wstokes=(1
            abi.ptr(nodes), abi.ptr(chi2), iters.ctypes.data_as(C.POINTER(C.c_int)), C.byref(total))
     check("snapi_invert", rc)
     return nodes, chi2, iters, total.value
+
+
+def invert_dev(table, ray, model, state_t, lam_t, obs_t, mask_t, nodes0_t, nodes_t, chi2_t, iters_t,
+               wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0, precision=abi.SNAPI_MIXED,
+               jacobian=0, stream=None):
+    """snapi_invert_dev on torch device tensors (state [npix][nfield][nd],
+    obs [npix][nlambda][4], mask [npix][nlambda] or None, in the precision's
+    type; lam float64; nodes0/nodes float64 [npix][nnode]; chi2 float64;
+    iterations int32).  Returns the total number of pixel-iterations."""
+    import torch
+    npix, nf, nd = state_t.shape
+    nl = lam_t.shape[0]
+    nn = model.nnode
+    dt = torch.float64 if precision == abi.SNAPI_FP64 else torch.float32
+    want = [(state_t, dt, (npix, abi.nfield(table.nline), nd)), (lam_t, torch.float64, (nl,)),
+            (obs_t, dt, (npix, nl, 4)), (nodes0_t, torch.float64, (npix, nn)), (nodes_t, torch.float64, (npix, nn)),
+            (chi2_t, torch.float64, (npix,)), (iters_t, torch.int32, (npix,))]
+    if mask_t is not None:
+        want.append((mask_t, dt, (npix, nl)))
+    for t, d, shp in want:
+        if t.dtype != d or tuple(t.shape) != shp or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("invert_dev: expected a contiguous device tensor %s of shape %s, got %s %s"
+                             % (d, shp, t.dtype, tuple(t.shape)))
+    ws = np.ascontiguousarray(wstokes, dtype=np.float64)
+    total = C.c_longlong(0)
+    L = lib()
+    f = L.snapi_invert_dev
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay),
+                  C.POINTER(SnapiNodeModel), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                  C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                  C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]
+    sp = None if stream is None else C.c_void_p(stream.cuda_stream)
+    rc = f(precision, jacobian, C.byref(table), C.byref(ray), C.byref(model), npix, nd, state_t.data_ptr(), nl,
+           lam_t.data_ptr(), obs_t.data_ptr(), None if mask_t is None else mask_t.data_ptr(), abi.ptr(ws), lm_start,
+           max_iter, nslot, nodes0_t.data_ptr(), nodes_t.data_ptr(), chi2_t.data_ptr(), iters_t.data_ptr(),
+           C.byref(total), sp)
+    check("snapi_invert_dev", rc)
+    return total.value

@@ -147,6 +147,45 @@ def cube_state(template, nx, ny, seed=1, dtype=np.float32):
     return np.ascontiguousarray(out.astype(dtype))
 
 
+# the non-thermodynamic node set of the 14-node inversion workload (7 T
+# nodes, 2 vLOS, 1 vmicro, 2 B, 1 inclination, 1 azimuth), in log tau_500
+INVERSION_NODES = [("vt", -2.0), ("vlos", -4.0), ("vlos", 0.0), ("B", -4.0), ("B", 0.0), ("theta", -2.0),
+                   ("phi", -2.0)]
+
+
+def inversion_problem(template, nx, ny, seed=3):
+    """Pixels of the batched-inversion workload: cube columns (as cube_state),
+    their node values (truth), starting values perturbed by up to +/-20 %,
+    and per-pixel wavelength masks keeping a random 120..nlambda subset.
+    Returns dict(state [npix][nfield][nd] float64, truth, start [npix][7],
+    mask [npix][nlambda] float64)."""
+    st = cube_state(template, nx, ny, seed=seed, dtype=np.float64)
+    npix = st.shape[0]
+    rng = np.random.default_rng(seed + 1000)
+    logtau = template["logtau"]
+    truth = np.empty((npix, len(INVERSION_NODES)))
+    for k, (p, lt) in enumerate(INVERSION_NODES):
+        z = int(np.argmin(np.abs(logtau - lt)))
+        if p == "vt":
+            truth[:, k] = st[:, abi.F_VT, z]
+        elif p == "vlos":
+            truth[:, k] = st[:, abi.F_VLOS, z]
+        elif p == "B":
+            truth[:, k] = st[:, abi.F_BMAG, z]
+        elif p == "theta":
+            truth[:, k] = np.radians(st[:, abi.F_BINC, z])
+        else:
+            truth[:, k] = np.radians(st[:, abi.F_BAZI, z])
+    start = truth * rng.uniform(0.8, 1.2, truth.shape)
+    nl = len(template["lam_vac"])
+    mask = np.zeros((npix, nl))
+    keep = rng.integers(120, nl + 1, npix)
+    order = np.argsort(rng.random((npix, nl)), axis=1)
+    for p in range(npix):
+        mask[p, order[p, :keep[p]]] = 1.0
+    return dict(state=st, truth=truth, start=start, mask=mask, nodes=INVERSION_NODES)
+
+
 def response_column_from_dump(path):
     """Column state as the reference's response path sees it, plus the
     thermodynamic derivatives dstate[2][nfield][nd] (d/dT, d/dNt of Ne,

@@ -440,6 +440,31 @@ static int mode_formal_scalar(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// Node interpolation (cmlib/mathtools.cc:1208-1315) for a seeded node set:
+// interp <nnode> <is_temp> <seed>; grid of 71 log tau points from -6 to 1.
+static int mode_interp(int argc, char **argv, io_class &io) {
+  if (argc < 5) return 2;
+  int nn = atoi(argv[2]), is_temp = atoi(argv[3]);
+  g_state = strtoull(argv[4], 0, 10) * 2654435761ULL + 88172645463325252ULL;
+  const int nd = 71;
+  fp_t *nt = new fp_t[nn] - 1, *nv = new fp_t[nn] - 1;
+  fp_t *grid = new fp_t[nd] - 1, *q = new fp_t[nd] - 1;
+  // ascending node positions inside [-5.5, 0.5], values of either sign
+  fp_t pos = -5.5;
+  for (int i = 1; i <= nn; ++i) {
+    pos += (6.0 / nn) * (0.3 + 0.7 * urand());
+    nt[i] = pos;
+    nv[i] = is_temp ? 4000.0 + 3000.0 * urand() : 2.0 * urand() - 1.0;
+  }
+  for (int z = 1; z <= nd; ++z) grid[z] = -6.0 + 7.0 * (z - 1) / (nd - 1);
+  atmospheric_interpolation(nt, nv, nn, grid, q, 1, nd, is_temp);
+  printf("nodes");
+  for (int i = 1; i <= nn; ++i) printf(" %.17g %.17g", nt[i], nv[i]);
+  printf("\n");
+  for (int z = 1; z <= nd; ++z) printf("q %.17g %.17g\n", grid[z], q[z]);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr, "usage: ref_driver synth|resp|lmstep|formal_scalar ...\n");
@@ -451,6 +476,7 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "resp")) return mode_resp(argc, argv, io);
   if (!strcmp(argv[1], "lmstep")) return mode_lmstep(argc, argv, io);
   if (!strcmp(argv[1], "formal_scalar")) return mode_formal_scalar(argc, argv, io);
+  if (!strcmp(argv[1], "interp")) return mode_interp(argc, argv, io);
   fprintf(stderr, "unknown mode %s\n", argv[1]);
   return 2;
 }

@@ -318,6 +318,27 @@ def build_lmstep(d):
     print("lmstep: %d cases" % 3)
 
 
+def build_interp(d):
+    """Non-temperature node interpolation (is_temp 0) for 1..5 nodes."""
+    recs = {}
+    for nn, seed in ((1, 11), (2, 12), (3, 13), (4, 14), (5, 15)):
+        out = run(d, ["interp", str(nn), "0", str(seed)], he=False)
+        q = []
+        for ln in out:
+            t = ln.split()
+            if t[0] == "nodes":
+                v = [float(x) for x in t[1:]]
+                recs["n%d_tau" % nn] = np.array(v[0::2])
+                recs["n%d_val" % nn] = np.array(v[1::2])
+            elif t[0] == "q":
+                q.append([float(t[1]), float(t[2])])
+        q = np.array(q)
+        recs["n%d_grid" % nn] = q[:, 0]
+        recs["n%d_q" % nn] = q[:, 1]
+    np.savez_compressed(os.path.join(HERE, "interp.npz"), **recs)
+    print("interp: 5 node sets")
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
@@ -328,6 +349,7 @@ def main():
         build_resp(d)
         build_formal_scalar(d)
         build_lmstep(d)
+        build_interp(d)
     finally:
         shutil.rmtree(d)
 

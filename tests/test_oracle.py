@@ -198,3 +198,13 @@ def test_responses_match_reference_golden():
         scale = np.maximum(np.abs(ref).max(axis=(0, 1)), 1e-300)
         err = np.max(np.abs(resp[0, p] - ref) / scale)
         assert err < 1e-6, (p, err)
+
+
+def test_node_interpolation_matches_reference_golden():
+    """Bezier node interpolation of a non-temperature parameter
+    (atmospheric_interpolation) against the reference on 1..5 seeded nodes."""
+    import _golden
+    g = _golden.load("interp.npz")
+    for n in range(1, 6):
+        q = _oracle._interp_param(g["n%d_tau" % n], g["n%d_val" % n], g["n%d_grid" % n])
+        assert np.max(np.abs(q - g["n%d_q" % n])) < 1e-14, n

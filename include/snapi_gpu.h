@@ -43,9 +43,10 @@ enum snapi_status {
 
 /* Precision of the compute path.
  * SNAPI_FP64: every operation in double, the reference's arithmetic.
- * SNAPI_MIXED: opacity build and formal solution in float with Taylor weights
- *              for small optical-depth steps and compensated accumulation;
- *              per-wavelength setup and the LM damping solve in double. */
+ * SNAPI_MIXED: opacity build and formal solution in float (Taylor weights
+ *              for optical-depth steps below 1e-2); the wavelength-independent
+ *              per-depth setup, the line-centre offsets and the LM damping
+ *              solve in double. */
 enum snapi_precision { SNAPI_FP64 = 0, SNAPI_MIXED = 1 };
 
 /*

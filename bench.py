@@ -15,8 +15,9 @@ Second leg ("lm"): batched Levenberg-Marquardt inversion of 256 x 256 pixels
 drawn as the cube (observations: FP64 GPU synthesis plus Gaussian noise of
 1e-3 Ic; per-pixel masks keeping 120-301 wavelengths; 7 kinematic/magnetic
 nodes started +/-20 % from truth; at most 30 iterations), mixed precision
-with adjoint Jacobians, against the FP64 path with central-difference
-Jacobians on 8192 of the pixels: LM pixel-iterations per second of each.
+with tangent-sweep Jacobians, against the FP64 path with central-difference
+Jacobians on 8192 of the pixels (and the mixed adjoint-sweep Jacobian on the
+same 8192): LM pixel-iterations per second of each.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -220,8 +221,9 @@ def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):
             % (label, total, sec, total / sec, it.mean(), np.median(red)))
         return out
 
-    mixed = run(npix, abi.SNAPI_MIXED, 0, "mixed adjoint")
+    mixed = run(npix, abi.SNAPI_MIXED, 2, "mixed tangent")
     fd = run(min(nfd, npix), abi.SNAPI_FP64, 1, "fp64 differences")
+    adj = run(min(nfd, npix), abi.SNAPI_MIXED, 0, "mixed adjoint")
     # Jacobian error of the mixed path vs the FP64 path (per-depth responses,
     # 8 columns, every 10th wavelength)
     lam_h = lam_d.cpu().numpy()[::10]
@@ -241,7 +243,7 @@ def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):
                         "120-%d per pixel, %d nodes (2 vLOS, 1 vmicro, 2 B, 1 incl, 1 azi), noise 1e-3 Ic, "
                         "start +/-20 %%, <=30 iterations" % (npix, nlam, nlam, nn),
             "value": mixed["pixel_iter_per_s"], "unit": "LM pixel-iterations/s",
-            "mixed_adjoint": mixed, "fp64_differences": fd,
+            "mixed_tangent": mixed, "fp64_differences": fd, "mixed_adjoint": adj,
             "speedup_vs_fp64_differences": mixed["pixel_iter_per_s"] / fd["pixel_iter_per_s"],
             "jacobian_rel_err_mixed_vs_fp64": jerr}
 

@@ -191,7 +191,8 @@ typedef struct {
 
 enum snapi_jacobian {
   SNAPI_JAC_ADJOINT = 0,     /* reverse sweep contracted with the node responses */
-  SNAPI_JAC_DIFFERENCES = 1  /* central differences of the node values through the synthesis (2n+1 syntheses) */
+  SNAPI_JAC_DIFFERENCES = 1, /* central differences of the node values through the synthesis (2n+1 syntheses) */
+  SNAPI_JAC_TANGENT = 2      /* forward-mode sweep carrying every node direction (no per-depth storage) */
 };
 
 /*
@@ -210,6 +211,21 @@ int snapi_invert(int precision, int jacobian, const snapi_line_table *lines, con
                  const snapi_node_model *model, int npix, int nd, const void *state, int nlambda, const double *lam,
                  const void *obs, const void *mask, const double *wstokes, double lm_start, int max_iter, int nslot,
                  double *nodes, double *chi2, int *iterations, long long *pixel_iterations);
+
+/*
+ * Emergent Stokes vector and node Jacobian of npix pixels at given node
+ * values (atmosphere::obs_stokes_responses_to_nodes, atmos/atmos_fit.cc:
+ * 1205-1352, for node parameter codes 2..6; mu = 1 ray).  The node-controlled
+ * fields of state are replaced by the node interpolation first.
+ *   state [npix][nfield][nd] and the outputs in the precision's type;
+ *   nodes [npix][nnode] double;
+ *   stokes [npix][nlambda][4], J [npix][nlambda*4][nnode] = dS_s(lambda_l)/d node_n
+ *   (per cm/s, G or rad; row l*4+s).
+ * jacobian selects the method (snapi_jacobian).
+ */
+int snapi_node_jacobian(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                        const snapi_node_model *model, int npix, int nd, const void *state, const double *nodes,
+                        int nlambda, const double *lam, void *stokes, void *J);
 
 /*
  * snapi_invert with the arrays already in device memory (state_dev, lam_dev,

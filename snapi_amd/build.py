@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsnapi_gpu.so")
-SOURCES = ["runtime.hip", "synth.hip", "responses.hip", "lm.hip", "formal_scalar.hip", "invert.hip"]
+SOURCES = ["runtime.hip", "synth.hip", "responses.hip", "tangent.hip", "lm.hip", "formal_scalar.hip", "invert.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # packed-FP32 instructions stay off: hipcc would otherwise form
@@ -19,7 +19,10 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
 
 
 def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    missing = [p for p in srcs if not os.path.exists(p)]
+    if missing:
+        raise FileNotFoundError("HIP sources missing: %s" % missing)
     tmp = LIB + ".tmp"
     cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs
     if verbose:

@@ -8,9 +8,10 @@
 //     Bezier node interpolation (atmospheric_interpolation, cmlib/
 //     mathtools.cc:1208-1315) on log tau; node responses by its central
 //     difference of the interpolation (atmos_fit.cc:1250-1305);
-//   - Jacobian either by the adjoint sweep (responses.hip) contracted with
-//     the node responses, or by central differences of the node values
-//     through the synthesis (2 n + 1 syntheses);
+//   - Jacobian by the forward-mode sweep over the node directions
+//     (tangent.hip), by the adjoint sweep (responses.hip) contracted with the
+//     node responses, or by central differences of the node values through
+//     the synthesis (2 n + 1 syntheses);
 //   - scaled Jacobian and residual (scale_rf, calc_residual :229-260), J^T J
 //     on the matrix cores, multiplicative damping and the truncated-SVD solve
 //     in FP64 (lm.hip), corrections scaled back and bracketed
@@ -319,6 +320,10 @@ __global__ void refill_kernel(NodeArgs na, int nslot, int npix, int nf, int nd, 
     smask[(size_t)slot * nl + i] = pmask ? pmask[(size_t)next * nl + i] : R(1);
 }
 
+template <typename R> __global__ void fill_kernel(size_t n, R v, R *__restrict__ p) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
 __global__ void count_active_kernel(int nslot, const int *__restrict__ active, int *__restrict__ count) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot < nslot && active[slot]) atomicAdd(count, 1);
@@ -412,14 +417,14 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   ss.chi2 = dbl + nslot;
   ss.chi2t = dbl + 2 * nslot;
   ss.track = dbl + 3 * nslot;
-  if (jacobian == 0) {
+  if (jacobian == SNAPI_JAC_ADJOINT) {
     per_col = resp_scratch_per_col(nl, nd, es);
     size_t c = ((size_t)16 << 30) / (per_col ? per_col : 1);
     if (c < 1) c = 1;
     if (c > (size_t)nslot) c = nslot;
     chunk = (int)c;
     HIPCHK(hipMalloc((void **)&scr, per_col * chunk));
-  } else {
+  } else if (jacobian == SNAPI_JAC_DIFFERENCES) {
     HIPCHK(hipMalloc((void **)&fdstate, (size_t)nslot * nv * ns * es));
     HIPCHK(hipMalloc((void **)&Sall, (size_t)nslot * nv * nl * 4 * es));
     HIPCHK(hipMalloc((void **)&fdn, (size_t)nslot * nv * nn * 8));
@@ -471,8 +476,11 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
       // model -> state and Jacobian where the model changed
       hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, snodes,
                          (size_t)nn, sstate, ns, nmap, ss.needj);
-      if (jacobian == 0) {
+      if (jacobian == SNAPI_JAC_ADJOINT) {
         rc = launch_resp<R>(kl, a, ra, nslot, sstate, dzero, dlam, S, J, scr, chunk, st);
+        if (rc) goto done;
+      } else if (jacobian == SNAPI_JAC_TANGENT) {
+        rc = launch_tangent<R>(kl, a, ra, nslot, sstate, dlam, S, J, st);
         if (rc) goto done;
       } else {
         hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, fdn, hv,
@@ -535,6 +543,92 @@ done:
   return rc;
 }
 
+// Stokes vector and raw node Jacobian dS/dnode of npix pixels at given node
+// values (obs_stokes_responses_to_nodes, atmos/atmos_fit.cc:1205-1352, for
+// the kinematic and magnetic nodes).
+template <typename R>
+static int node_jacobian_impl(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                              NodeArgs na, int npix, int nd, const void *state_h, const double *nodes_h, int nlambda,
+                              const double *lam_h, void *stokes_h, void *J_h) {
+  KLines kl;
+  SynthArgs a;
+  int rc = prepare_synth(lines, ray, 1, nd, nlambda, kl, a);
+  if (rc) return rc;
+  const int nf = a.nfield, nn = na.nnode, nl = nlambda, nv = 2 * nn + 1;
+  const size_t es = sizeof(R), ns = (size_t)nf * nd;
+  for (int n = 0; n < nn; ++n) na.scale[n] = 1.0;  // unscaled derivatives
+  R *dstate = nullptr, *S = nullptr, *J = nullptr, *nmap = nullptr, *dzero = nullptr, *scr = nullptr;
+  R *fdstate = nullptr, *Sall = nullptr, *ones = nullptr;
+  double *dlam = nullptr, *dnodes = nullptr, *fdn = nullptr, *hv = nullptr;
+  RespArgs ra = {};
+  HIPCHK(hipMalloc((void **)&dstate, (size_t)npix * ns * es));
+  HIPCHK(hipMalloc((void **)&S, (size_t)npix * nl * 4 * es));
+  HIPCHK(hipMalloc((void **)&J, (size_t)npix * nl * 4 * nn * es));
+  HIPCHK(hipMalloc((void **)&nmap, (size_t)npix * nn * nd * es));
+  HIPCHK(hipMalloc((void **)&dlam, (size_t)nl * 8));
+  HIPCHK(hipMalloc((void **)&dnodes, (size_t)npix * nn * 8));
+  HIPCHK(hipMemcpy(dstate, state_h, (size_t)npix * ns * es, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dlam, lam_h, (size_t)nl * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dnodes, nodes_h, (size_t)npix * nn * 8, hipMemcpyHostToDevice));
+  reset_status();
+  hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, nf, dnodes, (size_t)nn,
+                     dstate, ns, nmap, (const int *)nullptr);
+  ra.nseed = 4;
+  ra.sink = SINK_NODE;
+  for (int s = 0; s < 4; ++s) ra.ws[s] = 1.0;
+  ra.nnode = nn;
+  for (int n = 0; n < nn; ++n) {
+    ra.pidx[n] = na.code[n] == NP_VT ? 2 : na.code[n] == NP_VS ? 3 : na.code[n] == NP_B ? 4 : na.code[n] == NP_THETA ? 5 : 6;
+    ra.nscale[n] = 1.0;
+  }
+  ra.nmap = nmap;
+  if (jacobian == SNAPI_JAC_TANGENT) {
+    rc = launch_tangent<R>(kl, a, ra, npix, dstate, dlam, S, J, 0);
+    if (rc) goto done;
+  } else if (jacobian == SNAPI_JAC_ADJOINT) {
+    const size_t per_col = resp_scratch_per_col(nl, nd, es);
+    size_t c = ((size_t)16 << 30) / (per_col ? per_col : 1);
+    if (c < 1) c = 1;
+    if (c > (size_t)npix) c = npix;
+    HIPCHK(hipMalloc((void **)&scr, per_col * c));
+    HIPCHK(hipMalloc((void **)&dzero, (size_t)npix * 2 * ns * es));
+    HIPCHK(hipMemset(dzero, 0, (size_t)npix * 2 * ns * es));
+    rc = launch_resp<R>(kl, a, ra, npix, dstate, dzero, dlam, S, J, scr, (int)c, 0);
+    if (rc) goto done;
+  } else {
+    HIPCHK(hipMalloc((void **)&fdstate, (size_t)npix * nv * ns * es));
+    HIPCHK(hipMalloc((void **)&Sall, (size_t)npix * nv * nl * 4 * es));
+    HIPCHK(hipMalloc((void **)&fdn, (size_t)npix * nv * nn * 8));
+    HIPCHK(hipMalloc((void **)&hv, (size_t)npix * nn * 8));
+    HIPCHK(hipMalloc((void **)&ones, (size_t)npix * nl * es));
+    hipLaunchKernelGGL(fill_kernel<R>, dim3(256), dim3(256), 0, 0, (size_t)npix * nl, R(1), ones);
+    hipLaunchKernelGGL(fd_nodes_kernel, dim3((npix + 127) / 128), dim3(128), 0, 0, na, npix, dnodes, fdn, hv,
+                       (const int *)nullptr);
+    for (int k = 0; k < nv; ++k) {
+      HIPCHK(hipMemcpy2DAsync(fdstate + (size_t)k * ns, nv * ns * es, dstate, ns * es, ns * es, npix,
+                              hipMemcpyDeviceToDevice, 0));
+      hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, nf,
+                         fdn + (size_t)k * nn, (size_t)nv * nn, fdstate + (size_t)k * ns, (size_t)nv * ns,
+                         (R *)nullptr, (const int *)nullptr);
+    }
+    rc = snapi_synth_stokes_dev(precision, lines, ray, npix * nv, nd, fdstate, nl, dlam, Sall, nullptr);
+    if (rc) goto done;
+    hipLaunchKernelGGL(fdjac_kernel<R>, dim3(npix), dim3(256), 0, 0, na, npix, nl, Sall, hv, ones, 1.0, 1.0, 1.0,
+                       1.0, S, J, (const int *)nullptr);
+  }
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    rc = SNAPI_ERR_DEVICE;
+    goto done;
+  }
+  HIPCHK(hipMemcpy(stokes_h, S, (size_t)npix * nl * 4 * es, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(J_h, J, (size_t)npix * nl * 4 * nn * es, hipMemcpyDeviceToHost));
+  rc = read_status();
+done:
+  hipFree(dstate); hipFree(S); hipFree(J); hipFree(nmap); hipFree(dzero); hipFree(scr); hipFree(fdstate);
+  hipFree(Sall); hipFree(ones); hipFree(dlam); hipFree(dnodes); hipFree(fdn); hipFree(hv);
+  return rc;
+}
+
 }  // namespace snapi
 
 using namespace snapi;
@@ -544,7 +638,8 @@ static int setup_nodes(const snapi_line_table *lines, const snapi_ray *ray, cons
   if (!lines || !ray || !model) return SNAPI_ERR_ARG;
   if (npix <= 0 || nd < 3 || nlambda <= 0 || max_iter <= 0) return SNAPI_ERR_ARG;
   if (model->nnode <= 0 || model->nnode > SNAPI_MAX_NODES - 1) return SNAPI_ERR_ARG;
-  if (jacobian != SNAPI_JAC_ADJOINT && jacobian != SNAPI_JAC_DIFFERENCES) return SNAPI_ERR_ARG;
+  if (jacobian != SNAPI_JAC_ADJOINT && jacobian != SNAPI_JAC_DIFFERENCES && jacobian != SNAPI_JAC_TANGENT)
+    return SNAPI_ERR_ARG;
   if (cos(ray->theta) > -0.999999) return SNAPI_ERR_UNSUPPORTED;  // node responses assume the mu = 1 ray
   na = NodeArgs{};
   na.nnode = model->nnode;
@@ -599,5 +694,21 @@ extern "C" int snapi_invert_dev(int precision, int jacobian, const snapi_line_ta
     return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
                               mask_dev, wstokes, lm_start, max_iter, nslot, nodes0_dev, nodes_dev, chi2_dev,
                               iterations_dev, pixel_iterations, true, st);
+  return SNAPI_ERR_ARG;
+}
+
+extern "C" int snapi_node_jacobian(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                                   const snapi_node_model *model, int npix, int nd, const void *state,
+                                   const double *nodes, int nlambda, const double *lam, void *stokes, void *J) {
+  if (!state || !nodes || !lam || !stokes || !J) return SNAPI_ERR_ARG;
+  NodeArgs na;
+  int rc = setup_nodes(lines, ray, model, npix, nd, nlambda, jacobian, 1, na);
+  if (rc) return rc;
+  if (precision == SNAPI_FP64)
+    return node_jacobian_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state, nodes, nlambda, lam,
+                                      stokes, J);
+  if (precision == SNAPI_MIXED)
+    return node_jacobian_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state, nodes, nlambda, lam,
+                                     stokes, J);
   return SNAPI_ERR_ARG;
 }

@@ -26,6 +26,9 @@ template <> __device__ __forceinline__ float rsin<float>(float x) { return sinf(
 template <typename R> __device__ __forceinline__ R rsqrt_(R x);
 template <> __device__ __forceinline__ double rsqrt_<double>(double x) { return sqrt(x); }
 template <> __device__ __forceinline__ float rsqrt_<float>(float x) { return sqrtf(x); }
+template <typename R> __device__ __forceinline__ R rcos(R x);
+template <> __device__ __forceinline__ double rcos<double>(double x) { return cos(x); }
+template <> __device__ __forceinline__ float rcos<float>(float x) { return cosf(x); }
 template <typename R> __device__ __forceinline__ R rexp2(R x);
 template <> __device__ __forceinline__ double rexp2<double>(double x) { return exp2(x); }
 template <> __device__ __forceinline__ float rexp2<float>(float x) { return exp2f(x); }
@@ -140,15 +143,19 @@ template <typename R> __device__ __forceinline__ void kmul(const Kred<R> &k, con
   y[3] = k.c * x[0] + k.e * x[1] - k.f * x[2];
 }
 
-// Solve (1 + w1 Kred) x = b with the pivot-free Crout factorisation of
-// cmlib/mathtools.cc:487-514 (unit upper, diagonal kept in L).
-template <typename R> __device__ __forceinline__ void solve_1pk(const Kred<R> &k, R w1, const R b[4], R x[4]) {
+// Pivot-free Crout factorisation of 1 + w1 Kred (cmlib/mathtools.cc:487-514:
+// unit upper triangle, diagonal kept in L) and the solve with it; the factor
+// is shared by the forward solve and the tangent solves of one depth.
+template <typename R> struct LU4 {
+  R D[4][4];
+};
+template <typename R> __device__ __forceinline__ void factor_1pk(const Kred<R> &k, R w1, LU4<R> &f) {
   R A[4][4];
   A[0][0] = R(1);       A[0][1] = w1 * k.a;   A[0][2] = w1 * k.b;   A[0][3] = w1 * k.c;
   A[1][0] = w1 * k.a;   A[1][1] = R(1);       A[1][2] = w1 * k.d;   A[1][3] = -w1 * k.e;
   A[2][0] = w1 * k.b;   A[2][1] = -w1 * k.d;  A[2][2] = R(1);       A[2][3] = w1 * k.f;
   A[3][0] = w1 * k.c;   A[3][1] = w1 * k.e;   A[3][2] = -w1 * k.f;  A[3][3] = R(1);
-  R D[4][4];
+  R (&D)[4][4] = f.D;
 #pragma unroll
   for (int kk = 0; kk < 4; ++kk) {
 #pragma unroll
@@ -167,6 +174,9 @@ template <typename R> __device__ __forceinline__ void solve_1pk(const Kred<R> &k
       D[kk][j] = (A[kk][j] - sum) * inv;
     }
   }
+}
+template <typename R> __device__ __forceinline__ void solve_lu(const LU4<R> &f, const R b[4], R x[4]) {
+  const R (&D)[4][4] = f.D;
   R y[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -182,6 +192,12 @@ template <typename R> __device__ __forceinline__ void solve_1pk(const Kred<R> &k
     for (int p = i + 1; p < 4; ++p) sum += D[i][p] * x[p];
     x[i] = y[i] - sum;
   }
+}
+// Solve (1 + w1 Kred) x = b.
+template <typename R> __device__ __forceinline__ void solve_1pk(const Kred<R> &k, R w1, const R b[4], R x[4]) {
+  LU4<R> f;
+  factor_1pk<R>(k, w1, f);
+  solve_lu<R>(f, b, x);
 }
 
 // Line constants passed by value as a kernel argument (a few KB).
@@ -290,12 +306,13 @@ template <typename R> __device__ __forceinline__ void voigt_d(R v, R a, R &H, R 
     dQ = scl(mul(du, dQ), R(-1));
     const R su = rsin<R>(u.im);
     const R co = rsqrt_<R>(R(1) - su * su);
-    const R dsu = (R)cos((double)u.im) * du.im;
+    const R dsu = rcos<R>(u.im) * du.im;
     const R e = rexp<R>(u.re);
     const cplx<R> r = cdiv(w4, Q);
     w = cm<R>(e * co - r.re, e * su - r.im);
     const cplx<R> n1 = mul(dw4, Q), n2 = mul(dQ, w4);
-    const cplx<R> q2 = cdiv(cm<R>(n1.re - n2.re, n1.im - n2.im), mul(Q, Q));
+    // (n1 - n2) / Q^2 as two divisions: |Q|^4 overflows float for |u| ~ 30
+    const cplx<R> q2 = cdiv(cdiv(cm<R>(n1.re - n2.re, n1.im - n2.im), Q), Q);
     dw = cm<R>(e * du.re * co + e * (-su * du.im) - q2.re, e * du.re * su + e * dsu - q2.im);
   }
   H = w.re * R(1.0 / kSqrtPi);

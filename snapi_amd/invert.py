@@ -92,3 +92,27 @@ def invert_dev(table, ray, model, state_t, lam_t, obs_t, mask_t, nodes0_t, nodes
            C.byref(total), sp)
     check("snapi_invert_dev", rc)
     return total.value
+
+
+def node_jacobian(table, ray, model, state, nodes, lam_vac, precision=abi.SNAPI_FP64, jacobian=2):
+    """(stokes [npix][nlambda][4], J [npix][nlambda*4][nnode]) of snapi_node_jacobian."""
+    dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
+    st = np.ascontiguousarray(state, dtype=dt)
+    if st.ndim == 2:
+        st = st[None]
+    npix, nf, nd = st.shape
+    if nf != abi.nfield(table.nline):
+        raise ValueError("state has %d fields, line table needs %d" % (nf, abi.nfield(table.nline)))
+    nv = np.ascontiguousarray(nodes, dtype=np.float64).reshape(npix, model.nnode)
+    lam = np.ascontiguousarray(lam_vac, dtype=np.float64)
+    stokes = np.zeros((npix, len(lam), 4), dtype=dt)
+    J = np.zeros((npix, len(lam) * 4, model.nnode), dtype=dt)
+    f = lib().snapi_node_jacobian
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay),
+                  C.POINTER(SnapiNodeModel), C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int,
+                  C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+    rc = f(precision, jacobian, C.byref(table), C.byref(ray), C.byref(model), npix, nd, abi.vptr(st), abi.ptr(nv),
+           len(lam), abi.ptr(lam), abi.vptr(stokes), abi.vptr(J))
+    check("snapi_node_jacobian", rc)
+    return stokes, J

@@ -182,3 +182,40 @@ def nodes_to_state(model_nodes, values, state):
     st[abi.F_BINC] = np.degrees(np.arccos(bz / bm))
     st[abi.F_BAZI] = np.degrees(np.arctan(by / bx))
     return st
+
+
+_NODE_FIELD = {"vt": (1, 1.0), "vlos": (2, 1.0), "B": (3, 1.0), "theta": (4, np.pi / 180.0), "phi": (5, np.pi / 180.0)}
+_NODE_STEP = {"vt": 20.0, "vlos": 20.0, "B": 0.01, "theta": 1e-3, "phi": 1e-3}
+
+
+def node_map(model_nodes, values, state):
+    """d q_p(z) / d node_n by the central difference of the node
+    interpolation with the reference's steps (atmos/atmos_fit.cc:1250-1305;
+    const.h:27-32), angles in radians.  [nnode][nd]."""
+    out = np.zeros((len(model_nodes), state.shape[1]))
+    for n, (p, _) in enumerate(model_nodes):
+        f, sc = _NODE_FIELD[p]
+        h = _NODE_STEP[p]
+        vp = np.array(values, dtype=np.float64)
+        vm = vp.copy()
+        vp[n] += 0.5 * h
+        vm[n] -= 0.5 * h
+        out[n] = (nodes_to_state(model_nodes, vp, state)[f] - nodes_to_state(model_nodes, vm, state)[f]) * sc / h
+    return out
+
+
+def node_jacobian(table, ray, model_nodes, values, state, lam, fd_scale=1.0):
+    """Stokes and dS/dnode [nlambda*4][nnode] of one pixel the reference's way:
+    per-depth responses (local analytic derivatives propagated by central
+    differences through the solver) contracted with the node map."""
+    from snapi_amd import abi
+    st = nodes_to_state(model_nodes, values, state)
+    nf, nd = st.shape
+    ds = np.zeros((1, 2, nf, nd))
+    S, R, rc = responses(table, ray, st[None], ds, lam, fd_scale=fd_scale)
+    pidx = {"vt": 2, "vlos": 3, "B": 4, "theta": 5, "phi": 6}
+    nm = node_map(model_nodes, values, state)
+    J = np.zeros((len(lam) * 4, len(model_nodes)))
+    for n, (p, _) in enumerate(model_nodes):
+        J[:, n] = np.einsum("zls,z->ls", R[0, pidx[p]], nm[n]).reshape(-1)
+    return S[0], J

@@ -34,7 +34,8 @@ def problem():
     return dict(t=t, truth=truth, states=states, lam=lam, obs=obs, start=start)
 
 
-@pytest.mark.parametrize("precision,jacobian", [(abi.SNAPI_FP64, 0), (abi.SNAPI_MIXED, 0), (abi.SNAPI_FP64, 1)])
+@pytest.mark.parametrize("precision,jacobian", [(abi.SNAPI_FP64, 0), (abi.SNAPI_MIXED, 0), (abi.SNAPI_FP64, 1),
+                                               (abi.SNAPI_FP64, 2), (abi.SNAPI_MIXED, 2)])
 def test_inversion_recovers_truth(problem, precision, jacobian):
     p = problem
     print("inversion: %d pixels, %d nodes, %d wavelengths, precision %d, jacobian %d" % (

@@ -109,8 +109,17 @@ def main():
     fpl = flops.flops_per_lambda_point(table, ray, state32[samp].astype(np.float64), lam)
     launch_flops = fpl * ncol * nlam
     achieved = launch_flops / (ms * 1e-3) / 1e12
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs of this
+    # kernel on this workload (scaled by a plain copy's reading)
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "synth_traffic.json")
+    if os.path.exists(tf):
+        t = json.load(open(tf))
+        if (t["columns"], t["depths"], t["wavelengths"]) == (ncol, nd, nlam):
+            traffic = t["bytes_per_launch"]
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+            "algorithmic_bytes": int(ncol * nf * nd * 4 + ncol * nlam * 16),
             "kernel": "synth_kernel<float>", "flops_per_lambda_point": round(fpl, 1),
             "ms_per_launch": round(ms, 4)}
 
@@ -164,7 +173,7 @@ def main():
 
 
 def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):
-    from snapi_amd import abi, api, api_extra, invert, workloads
+    from snapi_amd import abi, api, invert, workloads
     table, ray = tmpl["table"], tmpl["ray"]
     nlam = lam_d.shape[0]
     t0 = time.time()
@@ -224,21 +233,15 @@ def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):
     mixed = run(npix, abi.SNAPI_MIXED, 2, "mixed tangent")
     fd = run(min(nfd, npix), abi.SNAPI_FP64, 1, "fp64 differences")
     adj = run(min(nfd, npix), abi.SNAPI_MIXED, 0, "mixed adjoint")
-    # Jacobian error of the mixed path vs the FP64 path (per-depth responses,
-    # 8 columns, every 10th wavelength)
+    # node-Jacobian error of the mixed tangent sweep against the FP64 one
+    # (8 pixels at their true node values, every 10th wavelength), relative to
+    # the largest |dS/dnode| of each pixel and node
     lam_h = lam_d.cpu().numpy()[::10]
-    stc = prob["state"][:8]
-    nf_, nd_ = stc.shape[1:]
-    dz = np.zeros((8, 2, nf_, nd_))
-    _, r64 = api_extra.stokes_responses(table, ray, stc, dz, lam_h, precision=abi.SNAPI_FP64)
-    _, r32 = api_extra.stokes_responses(table, ray, stc.astype(np.float32), dz.astype(np.float32), lam_h,
-                                        precision=abi.SNAPI_MIXED)
-    jerr = 0.0
-    for c in range(8):
-        for p in range(2, 7):
-            sc = np.abs(r64[c, p]).max(axis=(0, 1))
-            sc = np.maximum(sc, 1e-3 * sc.max())
-            jerr = max(jerr, float(np.max(np.abs(r32[c, p].astype(np.float64) - r64[c, p]) / sc)))
+    st8, v8 = prob["state"][:8], prob["truth"][:8]
+    _, j64 = invert.node_jacobian(table, ray, model, st8, v8, lam_h, abi.SNAPI_FP64, 2)
+    _, j32 = invert.node_jacobian(table, ray, model, st8.astype(np.float32), v8, lam_h, abi.SNAPI_MIXED, 2)
+    sc = np.maximum(np.abs(j64).max(axis=1, keepdims=True), 1e-300)
+    jerr = float(np.max(np.abs(j32.astype(np.float64) - j64) / sc))
     return {"workload": "batched LM inversion, %d pixels of the cube, 71 depths, %d wavelengths masked to "
                         "120-%d per pixel, %d nodes (2 vLOS, 1 vmicro, 2 B, 1 incl, 1 azi), noise 1e-3 Ic, "
                         "start +/-20 %%, <=30 iterations" % (npix, nlam, nlam, nn),

@@ -494,7 +494,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
                              (R *)nullptr, ss.needj);
         }
         // fdstate is [slot][node set]: one synthesis over nslot * (2n+1) columns
-        rc = snapi_synth_stokes_dev(precision, lines, ray, nslot * nv, nd, fdstate, nl, dlam, Sall, st);
+        rc = synth_dev_masked(precision, lines, ray, nslot * nv, nd, fdstate, nl, dlam, Sall, ss.needj, nv, st);
         if (rc) goto done;
         hipLaunchKernelGGL(fdjac_kernel<R>, dim3(nslot), dim3(256), 0, st, na, nslot, nl, Sall, hv, smask, ws[0], ws[1],
                            ws[2], ws[3], S, J, ss.needj);
@@ -510,7 +510,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
                          ss.active);
       hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, tnodes,
                          (size_t)nn, tstate, ns, (R *)nullptr, ss.active);
-      rc = snapi_synth_stokes_dev(precision, lines, ray, nslot, nd, tstate, nl, dlam, St, st);
+      rc = synth_dev_masked(precision, lines, ray, nslot, nd, tstate, nl, dlam, St, ss.active, 1, st);
       if (rc) goto done;
       hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, St, sobs, smask, ws[0], ws[1],
                          ws[2], ws[3], (R *)nullptr, ss.chi2t, ss.active);

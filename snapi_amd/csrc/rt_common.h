@@ -33,6 +33,12 @@ template <typename R> __device__ __forceinline__ R rexp2(R x);
 template <> __device__ __forceinline__ double rexp2<double>(double x) { return exp2(x); }
 template <> __device__ __forceinline__ float rexp2<float>(float x) { return exp2f(x); }
 
+// Reciprocal: IEEE division in double; the hardware v_rcp_f32 (1 ulp) in
+// float, where the correctly rounded division costs about ten instructions.
+template <typename R> __device__ __forceinline__ R rrcp(R x);
+template <> __device__ __forceinline__ double rrcp<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ float rrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <typename R> struct cplx {
   R re, im;
 };
@@ -44,7 +50,7 @@ template <typename R> __device__ __forceinline__ cplx<R> addr(cplx<R> a, R s) { 
 template <typename R> __device__ __forceinline__ cplx<R> rsub(R s, cplx<R> a) { return {s - a.re, -a.im}; }
 template <typename R> __device__ __forceinline__ cplx<R> scl(cplx<R> a, R s) { return {a.re * s, a.im * s}; }
 template <typename R> __device__ __forceinline__ cplx<R> cdiv(cplx<R> a, cplx<R> b) {
-  R d = R(1) / (b.re * b.re + b.im * b.im);
+  R d = rrcp<R>(b.re * b.re + b.im * b.im);
   return {(a.re * b.re + a.im * b.im) * d, (a.im * b.re - a.re * b.im) * d};
 }
 
@@ -108,7 +114,7 @@ template <typename R> __device__ __forceinline__ void sc_weights(R tb, R &etb, R
     w1 = R(0.5) * tb * (R(1) - tb * R(1.0 / 3.0) * (R(1) - tb * R(0.25)));
   } else {
     etb = rexp<R>(-tb);
-    const R itb = R(1) / tb;
+    const R itb = rrcp<R>(tb);
     w0 = itb - etb * (R(1) + itb);
     w1 = R(1) - itb + etb * itb;
   }
@@ -117,11 +123,11 @@ template <typename R> __device__ __forceinline__ void sc_weights(R tb, R &etb, R
 // Fritsch-Butland monotone derivative at an interior point
 // (atmos_ppbez_rts.cc:966-979).
 template <typename R> __device__ __forceinline__ R fb_deriv(R om, R o0, R op, R h1, R h2) {
-  const R d2 = (op - o0) / h2;
-  const R d1 = (o0 - om) / h1;
+  const R d2 = (op - o0) * rrcp<R>(h2);
+  const R d1 = (o0 - om) * rrcp<R>(h1);
   if (d1 * d2 > R(0)) {
-    const R al = (R(1) + h1 / (h1 + h2)) * R(1.0 / 3.0);
-    return d1 * d2 / (al * d1 + (R(1) - al) * d2);
+    const R al = (R(1) + h1 * rrcp<R>(h1 + h2)) * R(1.0 / 3.0);
+    return d1 * d2 * rrcp<R>(al * d1 + (R(1) - al) * d2);
   }
   return R(0);
 }
@@ -148,6 +154,7 @@ template <typename R> __device__ __forceinline__ void kmul(const Kred<R> &k, con
 // is shared by the forward solve and the tangent solves of one depth.
 template <typename R> struct LU4 {
   R D[4][4];
+  R id[4];  // reciprocals of the diagonal of L
 };
 template <typename R> __device__ __forceinline__ void factor_1pk(const Kred<R> &k, R w1, LU4<R> &f) {
   R A[4][4];
@@ -165,7 +172,8 @@ template <typename R> __device__ __forceinline__ void factor_1pk(const Kred<R> &
       for (int p = 0; p < kk; ++p) sum += D[i][p] * D[p][kk];
       D[i][kk] = A[i][kk] - sum;
     }
-    const R inv = R(1) / D[kk][kk];
+    const R inv = rrcp<R>(D[kk][kk]);
+    f.id[kk] = inv;
 #pragma unroll
     for (int j = kk + 1; j < 4; ++j) {
       R sum = R(0);
@@ -183,7 +191,7 @@ template <typename R> __device__ __forceinline__ void solve_lu(const LU4<R> &f, 
     R sum = R(0);
 #pragma unroll
     for (int p = 0; p < i; ++p) sum += D[i][p] * y[p];
-    y[i] = (b[i] - sum) / D[i][i];
+    y[i] = (b[i] - sum) * f.id[i];
   }
 #pragma unroll
   for (int i = 3; i >= 0; --i) {

@@ -24,11 +24,13 @@ template <typename R>
 __global__ __launch_bounds__(kSynthThreads) void synth_kernel(const KLines lines, const SynthArgs args,
                                                              const R *__restrict__ state,
                                                              const double *__restrict__ lam,
-                                                             R *__restrict__ stokes, int *__restrict__ status) {
+                                                             R *__restrict__ stokes, int *__restrict__ status,
+                                                             const int *__restrict__ active, int active_div) {
   extern __shared__ unsigned char smem_raw[];
   R *sm = reinterpret_cast<R *>(smem_raw);
   const int nd = args.nd;
   const int col = blockIdx.x;
+  if (active && !active[col / active_div]) return;  // uniform over the workgroup
   const R *st = state + (size_t)col * args.nfield * nd;
   ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
   stage_column<R, R>(lines, args, st, tile);
@@ -47,13 +49,34 @@ __global__ __launch_bounds__(kSynthThreads) void synth_kernel(const KLines lines
 
 template <typename R>
 static int launch_synth(const KLines &kl, const SynthArgs &a, const void *state, const double *lam, void *stokes,
-                        int *status, hipStream_t stream) {
+                        int *status, hipStream_t stream, const int *active = nullptr, int active_div = 1) {
   int threads = ((a.nlambda + 63) / 64) * 64;
   if (threads > kSynthThreads) threads = kSynthThreads;
   size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline);
   hipLaunchKernelGGL(synth_kernel<R>, dim3(a.ncol), dim3(threads), smem, stream, kl, a,
-                     static_cast<const R *>(state), lam, static_cast<R *>(stokes), status);
+                     static_cast<const R *>(state), lam, static_cast<R *>(stokes), status, active,
+                     active_div > 0 ? active_div : 1);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+// Synthesis of the columns whose flag active[col / active_div] is set (the
+// others keep their output untouched): the trial and difference syntheses
+// of the batched inversion, where finished slots are skipped.
+int synth_dev_masked(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
+                     const void *state_dev, int nlambda, const double *lam_dev, void *stokes_dev, const int *active,
+                     int active_div, hipStream_t stream) {
+  KLines kl;
+  SynthArgs a;
+  int rc = prepare_synth(lines, ray, ncol, nd, nlambda, kl, a);
+  if (rc) return rc;
+  if (!state_dev || !lam_dev || !stokes_dev) return SNAPI_ERR_ARG;
+  int *status = device_status_slot();
+  if (!status) return SNAPI_ERR_DEVICE;
+  if (precision == SNAPI_FP64)
+    return launch_synth<double>(kl, a, state_dev, lam_dev, stokes_dev, status, stream, active, active_div);
+  if (precision == SNAPI_MIXED)
+    return launch_synth<float>(kl, a, state_dev, lam_dev, stokes_dev, status, stream, active, active_div);
+  return SNAPI_ERR_ARG;
 }
 
 }  // namespace snapi

@@ -151,7 +151,7 @@ __device__ __forceinline__ void opacity_at(const KLines &L, const SynthArgs &a, 
     const R stim = R(1) - rexp<R>(-c.hckl * tinv);
     const R Lt = t.ltp[z];
     const R o = c.hbf * t.cbf[z] * stim + t.cff[z] * rexp2<R>(R(kLog2_10) * (c.f0 + c.f1 * Lt + c.f2 * Lt * Lt));
-    const R B = c.plc / (rexp<R>(c.ple * tinv) - R(1));
+    const R B = c.plc * rrcp<R>(rexp<R>(c.ple * tinv) - R(1));
     KI += o;
     EI += o * B;
   }
@@ -207,7 +207,7 @@ __device__ __forceinline__ void opacity_at(const KLines &L, const SynthArgs &a, 
 }
 
 template <typename R> __device__ __forceinline__ void reduce_ks(const R K[7], const R E[4], Kred<R> &k, R S[4]) {
-  const R i = R(1) / K[0];
+  const R i = rrcp<R>(K[0]);
   k.a = K[1] * i; k.b = K[2] * i; k.c = K[3] * i;
   k.d = K[6] * i; k.e = K[5] * i; k.f = K[4] * i;
   S[0] = E[0] * i; S[1] = E[1] * i; S[2] = E[2] * i; S[3] = E[3] * i;
@@ -286,6 +286,10 @@ __device__ int solve_lambda(const KLines &L, const SynthArgs &a, const ColumnTil
 // the sizes the kernels support.
 int prepare_synth(const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd, int nlambda, KLines &kl,
                   SynthArgs &a);
+// Synthesis restricted to the columns with active[col / active_div] set.
+int synth_dev_masked(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
+                     const void *state_dev, int nlambda, const double *lam_dev, void *stokes_dev, const int *active,
+                     int active_div, hipStream_t stream);
 // A per-device int the kernels OR their status bits into.
 int *device_status_slot();
 void reset_status();

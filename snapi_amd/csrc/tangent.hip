@@ -31,7 +31,7 @@
 
 namespace snapi {
 
-constexpr int kTanThreads = 256;
+constexpr int kTanThreads = 64;
 enum { Q_VT = 0, Q_VL = 1, Q_B = 2, Q_TH = 3, Q_PH = 4, NQ = 5 };
 
 // LDS tile of the derivative factors of one column, after the ColumnTile.
@@ -89,7 +89,7 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
     const R stim = R(1) - rexp<R>(-c.hckl * tinv);
     const R Lt = t.ltp[z];
     const R o = c.hbf * t.cbf[z] * stim + t.cff[z] * rexp2<R>(R(kLog2_10) * (c.f0 + c.f1 * Lt + c.f2 * Lt * Lt));
-    const R B = c.plc / (rexp<R>(c.ple * tinv) - R(1));
+    const R B = c.plc * rrcp<R>(rexp<R>(c.ple * tinv) - R(1));
     KI += o;
     EI += o * B;
   }
@@ -238,7 +238,7 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
 template <typename R>
 __device__ __forceinline__ void reduced_u(const R G[NQ][11], R eta, const Kred<R> &k, const R S[4], const R I[4],
                                           unsigned qmask, R u[NQ][4], R ds0[NQ]) {
-  const R ie = R(1) / eta;
+  const R ie = rrcp<R>(eta);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     if (!(qmask & (1u << q))) {
@@ -279,7 +279,7 @@ template <typename R> __device__ __forceinline__ void sc_weights_dt(R tb, R &det
     dw0 = R(0.5) - R(2.0 / 3.0) * tb + R(3.0 / 8.0) * tb * tb;
     dw1 = R(0.5) - tb * R(1.0 / 3.0) + tb * tb * R(0.125);
   } else {
-    const R e = rexp<R>(-tb), i = R(1) / tb;
+    const R e = rexp<R>(-tb), i = rrcp<R>(tb);
     detb = -e;
     dw0 = -i * i + e * (R(1) + i) + e * i * i;
     dw1 = i * i - e * i - e * i * i;
@@ -288,17 +288,19 @@ template <typename R> __device__ __forceinline__ void sc_weights_dt(R tb, R &det
 
 template <typename R>
 __device__ __forceinline__ R fb_deriv_p(R om, R o0, R op, R h1, R h2, R &gm, R &g0, R &gp) {
-  const R d2 = (op - o0) / h2, d1 = (o0 - om) / h1;
+  const R ih1 = rrcp<R>(h1), ih2 = rrcp<R>(h2);
+  const R d2 = (op - o0) * ih2, d1 = (o0 - om) * ih1;
   gm = g0 = gp = R(0);
   if (d1 * d2 > R(0)) {
-    const R al = (R(1) + h1 / (h1 + h2)) * R(1.0 / 3.0);
+    const R al = (R(1) + h1 * rrcp<R>(h1 + h2)) * R(1.0 / 3.0);
     const R den = al * d1 + (R(1) - al) * d2;
-    const R dD1 = d2 * d2 * (R(1) - al) / (den * den);
-    const R dD2 = d1 * d1 * al / (den * den);
-    gm = -dD1 / h1;
-    g0 = dD1 / h1 - dD2 / h2;
-    gp = dD2 / h2;
-    return d1 * d2 / den;
+    const R iden = rrcp<R>(den);
+    const R dD1 = d2 * d2 * (R(1) - al) * iden * iden;
+    const R dD2 = d1 * d1 * al * iden * iden;
+    gm = -dD1 * ih1;
+    g0 = dD1 * ih1 - dD2 * ih2;
+    gp = dD2 * ih2;
+    return d1 * d2 * iden;
   }
   return R(0);
 }

@@ -248,6 +248,109 @@ int snapi_invert_dev(int precision, int jacobian, const snapi_line_table *lines,
 int snapi_formal_scalar(int nprob, int nd, const double *grid, const double *op, const double *em, const double *ct,
                         double boundary, double *I, double *lstar);
 
+/* ---------------------------------------------------------------------------
+ * ALI (approximate Lambda iteration) for one NLTE atom in plane-parallel
+ * columns: atmosphere::nltepops (atmos/atmos_pop.cc:67-240) with
+ * atom::add (atmos/atmol/atom/atom.cc:1865-1927) and atom::newpops
+ * (atmos/atmol/atom/atom_pop.cc:27-135).
+ *
+ * Levels are the atom's mapped levels (atom::nmap order: every level of every
+ * ionisation stage, the last one being the ground level of the final stage).
+ * Transitions are the atom's own list (atom::tmap order): lines between levels
+ * of one stage, and one bound-free transition from every level except the
+ * last to the ground level of the next stage.
+ */
+#define SNAPI_ALI_MAX_LEVELS 16
+#define SNAPI_ALI_MAX_TRANS 64
+enum snapi_transition_kind { SNAPI_TR_LINE = 0, SNAPI_TR_BF = 1 };
+typedef struct {
+  int kind;      /* SNAPI_TR_LINE or SNAPI_TR_BF */
+  int lower;     /* mapped level index (bound-free: the ionising level) */
+  int upper;     /* mapped level index (bound-free: ground level of the next stage) */
+  double lam0;   /* line: h c / (E_u - E_l) [cm] */
+  double aul;    /* line: A_ul [s^-1] */
+  double blu;    /* line: B_lu as the reference stores it (atom.cc:237-243) */
+  double bul;    /* line: B_ul */
+  double energy; /* line: E_u - E_l [erg] */
+  double gratio; /* bound-free: g_lower / g_continuum */
+  double de;     /* bound-free: ionisation energy of the stage minus E_lower [erg] */
+} snapi_ali_transition;
+typedef struct {
+  int nlevel;
+  int ntrans;
+  double mass;   /* atomic mass [g] */
+  snapi_ali_transition tr[SNAPI_ALI_MAX_TRANS];
+} snapi_ali_atom;
+
+/* Per-depth column fields of the ALI entry points: colstate[col][field][z]. */
+enum snapi_ali_field {
+  SNAPI_A_T = 0,     /* temperature [K] */
+  SNAPI_A_VT = 1,    /* microturbulent velocity [cm/s] */
+  SNAPI_A_NE = 2,    /* electron density [cm^-3] */
+  SNAPI_A_OPREF = 3, /* referent opacity (normalisation of the formal solution when tau_grid) */
+  SNAPI_A_GRID = 4,  /* RT grid (tau_referent, negative, or height) */
+  SNAPI_A_NA = 5,    /* total population of the atom (conservation equation) */
+  SNAPI_A_NF = 6
+};
+
+/* Frequency grid, rays and the fixed (population-independent) inputs. */
+typedef struct {
+  int nlam;               /* wavelength points */
+  const double *lam;      /* [nlam] vacuum wavelengths [cm] */
+  const double *lamw;     /* [nlam] integration weights */
+  const double *fudge;    /* [nlam] opacity fudge of the bound-free terms */
+  const double *sigma;    /* [ntrans][nlam] bound-free cross-sections (line rows unused) */
+  int nray;
+  const double *theta;    /* [nray] ray angles as the reference's anglesetup gives them */
+  const double *wray;     /* [nray] angular weights (bin) */
+  int bg_per_ray;         /* 1: bgop/bgem hold one set per ray; 0: one set for all rays */
+  double boundary;        /* boundary condition of the formal solution (-1: S at the bottom) */
+  int tau_grid;           /* 1: opacities normalised by SNAPI_A_OPREF in the formal solution */
+} snapi_ali_grid;
+
+/*
+ * One ALI accumulation pass with the given populations: for every ray and
+ * wavelength the opacity (background + the atom's bound-free and lines with
+ * Voigt profiles), the scalar formal solution with Lambda*, and the sums of
+ * atom::add.
+ *   colstate [ncol][SNAPI_A_NF][nd], vproj [ncol][nray][nd] velocity along
+ *   each ray, gamma [ncol][ntrans][nd] total damping of each line,
+ *   bgop, bgem [ncol][nray or 1][nd][nlam] background opacity/emissivity,
+ *   pops [ncol][nd][nlevel];
+ *   acc [ncol][nd][ntrans][4] out: J (lower->upper), J (upper->lower),
+ *   Lambda*, profile norm (atom::Jb, Ju, Ls, norm).  All double.
+ * precision selects the formal solution's arithmetic; sums are in double.
+ */
+int snapi_ali_accumulate(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol, int nd,
+                         const double *colstate, const double *vproj, const double *gamma, const double *bgop,
+                         const double *bgem, const double *pops, double *acc);
+
+/*
+ * New populations from the accumulated sums (atom::newpops): rate matrix with
+ * the local approximate operator (alo = 1) or plain Lambda iteration
+ * (alo = 0), collision rates coll [ncol][nd][nlevel][nlevel] (rate i -> j),
+ * the conservation equation replacing the row of the most populated level,
+ * pivot-free Crout solve in double.
+ *   newpops [ncol][nd][nlevel] out; relchange [ncol] out: max over depth and
+ *   level of |new/old - 1|, or -1 where a population came out negative.
+ */
+int snapi_ali_newpops(const snapi_ali_atom *atom, int ncol, int nd, const double *colstate, const double *coll,
+                      const double *pops, const double *acc, int alo, double *newpops, double *relchange);
+
+/*
+ * The whole iteration on the device (atmosphere::nltepops): accumulate,
+ * new populations with the approximate operator, plain Lambda iteration for
+ * a column whose populations came out negative, stop a column when its
+ * relative change is below tol or after max_iter iterations.
+ *   pops [ncol][nd][nlevel] in: start populations, out: result;
+ *   iterations [ncol], relchange [ncol] (last value; -1: negative populations
+ *   with both methods, the column keeps its previous populations) out.
+ */
+int snapi_ali_solve(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol, int nd,
+                    const double *colstate, const double *vproj, const double *gamma, const double *bgop,
+                    const double *bgem, const double *coll, double tol, int max_iter, double *pops, int *iterations,
+                    double *relchange);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,9 +22,12 @@
 //   formal_scalar <atmos.cfg> <lambda.txt> <el> <az>
 //          unpolarised formal solution with approximate Lambda operator
 //          (plpar/atmos_ppbez_rts.cc:21-219) on the reference's continuum.
-//   nlte   <atmos.cfg> <el> <az>
-//          atmosphere::nltepops (atmos/atmos_pop.cc:67-240): converged level
-//          populations of every NLTE atom.
+//   nlte   <atmos.cfg> [max_iter]
+//          atmosphere::nltepops (atmos/atmos_pop.cc:67-240) for the NLTE atom:
+//          the inputs of one ALI iteration, its J/Lambda*/norm sums and new
+//          populations, and the converged populations.
+//   interp <nnode> <is_temp> <seed>
+//          atmospheric_interpolation on a seeded node set.
 //
 // This file is test infrastructure: it is compiled only against the reference
 // sources (oracle/ref.mk) and never linked into the product.
@@ -440,6 +443,183 @@ static int mode_formal_scalar(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// NLTE populations of the active atom (atmosphere::nltepops, atmos/
+// atmos_pop.cc:67-240).  The ALI loop is driven here with the reference's own
+// functions so that every input of one iteration (frequency grid, rays,
+// background opacity, profiles' damping, collision rates, cross-sections)
+// and its results (J, Lambda*, norm sums after atom::add; the populations of
+// atom::newpops) can be written; the converged populations of the
+// reference's own nltepops() on a fresh atmosphere are written last.
+static int mode_nlte(int argc, char **argv, io_class &io) {
+  if (argc < 3) return 2;
+  int maxit = (argc > 3) ? atoi(argv[3]) : 100;
+  atmosphere *a = make_atmos(argv[2], io);
+  const int x1 = a->x1l, x2 = a->x2l;
+  a->set_grid(a->tau_grid);
+  a->boundary_condition_for_rt = -1;
+  a->popsetup();
+  a->compute_op_referent();
+  if (!a->tau_grid) a->compute_tau_referent();
+  // --- nltepops set-up (atmos_pop.cc:71-131)
+  a->ltepops();
+  fp_t *lambda = 0;
+  int32_t nlambda = 0;
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->rtsetup(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  for (int s = 0; s < a->natm; ++s)
+    lambda = a->atml[s]->getlambda(lambda, nlambda, a->T[x1][x2][a->x3h], a->Nt[x1][x2][a->x3h], a->Ne[x1][x2][a->x3h]);
+  int ntp;
+  fp_t *th, *ph;
+  fp_t *bin = a->anglesetup(th, ph, ntp);
+  fp_t *lambda_w = new fp_t[nlambda];
+  if (nlambda >= 2) {
+    lambda_w[0] = (lambda[1] - lambda[0]) * 0.5;
+    lambda_w[nlambda - 1] = (lambda[nlambda - 1] - lambda[nlambda - 2]) * 0.5;
+    for (int l = 1; l < nlambda - 1; ++l) lambda_w[l] = (lambda[l + 1] - lambda[l - 1]) * 0.5;
+  } else if (nlambda)
+    lambda_w[0] = 1.0;
+  if (a->tau_grid) a->compute_op_referent();
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->compute_active_population(a->T, a->Ne);
+  atom *at = 0;
+  int sa = -1;
+  for (int s = 0; s < a->natm; ++s) {
+    atom *t = dynamic_cast<atom *>(a->atml[s]);
+    if (t && t->NLTE) {
+      at = t;
+      sa = s;
+    }
+  }
+  if (!at) return 3;
+  const int nd = a->x3h - a->x3l + 1;
+  printf("nd %d\nntp %d\nnlam %d\nboundary %.17g\ntau_grid %d\n", nd, ntp, nlambda, a->boundary_condition_for_rt,
+         a->tau_grid);
+  for (int tp = 1; tp <= ntp; ++tp) printf("ray %d %.17g %.17g\n", tp, th[tp], bin[tp]);
+  for (int l = 0; l < nlambda; ++l)
+    printf("lam %d %.17g %.17g %.17g\n", l, lambda[l], lambda_w[l], a->get_opacity_fudge(lambda[l]));
+  printf("atom %d nmap %d ntr %d Z %d mass %.17g\n", sa, at->nmap, at->ntr, (int)at->Z, at->mass);
+  for (int i = 0; i < at->nmap; ++i)
+    printf("level %d %d %d %.17g %d\n", i, (int)at->zmap[i], (int)at->lmap[i], at->ee[at->zmap[i]][at->lmap[i]],
+           (int)at->g[at->zmap[i]][at->lmap[i]]);
+  for (int tr = 1; tr <= (int)at->ntr; ++tr) {
+    const int z = at->inverse_tmap[tr][2], lo = at->inverse_tmap[tr][3], up = at->inverse_tmap[tr][4];
+    if (up < at->nl[z]) {
+      const fp_t lam0 = h * c / (at->ee[z][up] - at->ee[z][lo]);
+      printf("trans %d line %d %d %.17g %.17g %.17g %.17g %.17g\n", tr, (int)at->rmap[z][lo], (int)at->rmap[z][up],
+             lam0, at->A[z][up][lo], at->B[z][lo][up], at->B[z][up][lo], at->ee[z][up] - at->ee[z][lo]);
+    } else {
+      printf("trans %d bf %d %d %.17g %.17g\n", tr, (int)at->rmap[z][lo], (int)at->rmap[z + 1][0],
+             (fp_t)at->g[z][lo] / (fp_t)at->g[z + 1][0], at->ip[z] - at->ee[z][lo]);
+      for (int l = 0; l < nlambda; ++l)
+        printf("sigma %d %d %.17g\n", tr, l, at->bf[z][lo] ? at->bf[z][lo]->U(lambda[l]) : 0.0);
+    }
+  }
+  for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) {
+    const fp_t T = a->T[x1][x2][x3i], ne = a->Ne[x1][x2][x3i];
+    printf("depth %d %.17g %.17g %.17g %.17g %.17g %.17g\n", x3i, T, a->Vt[x1][x2][x3i], ne,
+           a->op_referent[x1][x2][x3i], a->rt_grid[x3i], at->pop[x1][x2][x3i].Na);
+    for (int i = 0; i < at->nmap; ++i)
+      printf("pop0 %d %d %.17g\n", x3i, i, at->pop[x1][x2][x3i].n[at->zmap[i]][at->lmap[i]]);
+    for (int tr = 1; tr <= (int)at->ntr; ++tr) {
+      const int z = at->inverse_tmap[tr][2], lo = at->inverse_tmap[tr][3], up = at->inverse_tmap[tr][4];
+      if (up < at->nl[z]) {
+        const fp_t lam0 = h * c / (at->ee[z][up] - at->ee[z][lo]);
+        const fp_t ar = at->damp_rad(at->A[z], lo, up);
+        const fp_t gc = at->damp_col(x1, x2, x3i, z, up, lo, T, ne, lam0);
+        printf("gamma %d %d %.17g\n", tr, x3i, ar + gc * turn_on_damping);
+      }
+    }
+    // collision rates between mapped levels, as atom::newpops uses them (atom_pop.cc:51-105)
+    for (int i = 0; i < at->nmap - 1; ++i) {
+      const int l = at->lmap[i], z = at->zmap[i];
+      for (int ll = 0; ll < at->nl[z]; ++ll) {
+        if (ll == l) continue;
+        printf("coll %d %d %d %.17g\n", x3i, i, (int)at->rmap[z][ll], at->C_ij(z, l, ll, T, ne));
+      }
+      const int ic = at->rmap[z + 1][0];
+      printf("coll %d %d %d %.17g\n", x3i, i, ic, at->C_i_cont(z, l, T, ne));
+      printf("coll %d %d %d %.17g\n", x3i, ic, i, at->C_cont_i(z, l, T, ne));
+    }
+  }
+  // --- the ALI loop (atmos_pop.cc:133-205)
+  fp_t ***S = ft3dim(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  fp_t ***L = ft3dim(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  fp_t relative_change = 1.0;
+  int iter;
+  for (iter = 0; iter < maxit; ++iter) {
+    for (int s = 0; s < a->natm; ++s) a->atml[s]->rtinit();
+    for (int tp = 1; tp <= ntp; ++tp) {
+      fp_t ***Vr = a->project(a->Vx, a->Vy, a->Vz, th[tp], ph[tp], a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+      fp_t ****Bv = a->transform(a->Bx, a->By, a->Bz, th[tp], ph[tp], a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+      if (iter == 0)
+        for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) printf("vproj %d %d %.17g\n", tp, x3i, Vr[x1][x2][x3i]);
+      for (int l = 0; l < nlambda; ++l) {
+        for (int s = 0; s < a->natm; ++s) a->atml[s]->prof_init();
+        fp_t ***op = a->opacity_lte(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]);
+        fp_t ***em = a->emissivity_lte(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]);
+        if (iter == 0) {
+          // the active atom's population-independent part (free-free, Rayleigh)
+          fp_t ***ff = at->freefree_op(a->T, a->Ne, Vr, lambda[l]);
+          fp_t ***ry = at->rayleigh_op(lambda[l]);
+          for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) {
+            const fp_t p = ff[x1][x2][x3i] + ry[x1][x2][x3i];
+            printf("bg %d %d %d %.17g %.17g\n", tp, l, x3i, op[x1][x2][x3i] + p,
+                   em[x1][x2][x3i] + p * Planck_f(lambda[l], a->T[x1][x2][x3i]));
+          }
+          del_ft3dim(ff, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+          del_ft3dim(ry, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+        }
+        op = a->add(a->opacity_active_only(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]), op, a->x1l, a->x1h,
+                    a->x2l, a->x2h, a->x3l, a->x3h);
+        em = a->add(a->emissivity_active_only(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]), em, a->x1l,
+                    a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+        if (a->tau_grid) a->normalize_to_referent_opacity(op, em);
+        a->formal(a->rt_grid, S, L, op, em, th[tp], ph[tp], a->boundary_condition_for_rt);
+        if (a->tau_grid) a->de_normalize(op, em);
+        for (int s = 0; s < a->natm; ++s) a->atml[s]->add(S, L, op, lambda[l], lambda_w[l], bin[tp]);
+        del_ft3dim(em, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+        del_ft3dim(op, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+      }
+      del_ft4dim(Bv, 1, 3, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+      del_ft3dim(Vr, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    }
+    if (iter == 0) {
+      for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) {
+        for (int tr = 1; tr <= (int)at->ntr; ++tr)
+          printf("acc %d %d %.17g %.17g %.17g %.17g\n", x3i, tr, at->Jb[x1][x2][x3i][tr], at->Ju[x1][x2][x3i][tr],
+                 at->Ls[x1][x2][x3i][tr], at->norm[x1][x2][x3i][tr]);
+        fp_t *np = at->newpops(x1, x2, x3i, 1);
+        for (int i = 0; i < at->nmap; ++i) printf("newpop %d %d %.17g\n", x3i, i, np[i]);
+        delete[] np;
+      }
+    }
+    relative_change = a->newpops(a->T, a->Nt, a->Ne, lambda, nlambda, 1);
+    int alo = 1;
+    if (relative_change < 0.0) {
+      alo = 0;
+      relative_change = a->newpops(a->T, a->Nt, a->Ne, lambda, nlambda, 0);
+    }
+    printf("iter %d %.17g %d\n", iter, relative_change, alo);
+    if (relative_change < 0.0) break;
+    if (relative_change < 1E-2) break;
+  }
+  for (int x3i = a->x3l; x3i <= a->x3h; ++x3i)
+    for (int i = 0; i < at->nmap; ++i)
+      printf("final %d %d %.17g\n", x3i, i, at->pop[x1][x2][x3i].n[at->zmap[i]][at->lmap[i]]);
+  // --- the reference's own nltepops on a fresh atmosphere
+  atmosphere *b = make_atmos(argv[2], io);
+  b->set_grid(b->tau_grid);
+  b->boundary_condition_for_rt = -1;
+  b->popsetup();
+  b->compute_op_referent();
+  if (!b->tau_grid) b->compute_tau_referent();
+  int rv = b->nltepops();
+  atom *bt = dynamic_cast<atom *>(b->atml[sa]);
+  printf("ref_rv %d\n", rv);
+  for (int x3i = b->x3l; x3i <= b->x3h; ++x3i)
+    for (int i = 0; i < bt->nmap; ++i)
+      printf("ref_final %d %d %.17g\n", x3i, i, bt->pop[x1][x2][x3i].n[bt->zmap[i]][bt->lmap[i]]);
+  return 0;
+}
+
 // Node interpolation (cmlib/mathtools.cc:1208-1315) for a seeded node set:
 // interp <nnode> <is_temp> <seed>; grid of 71 log tau points from -6 to 1.
 static int mode_interp(int argc, char **argv, io_class &io) {
@@ -477,6 +657,7 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "lmstep")) return mode_lmstep(argc, argv, io);
   if (!strcmp(argv[1], "formal_scalar")) return mode_formal_scalar(argc, argv, io);
   if (!strcmp(argv[1], "interp")) return mode_interp(argc, argv, io);
+  if (!strcmp(argv[1], "nlte")) return mode_nlte(argc, argv, io);
   fprintf(stderr, "unknown mode %s\n", argv[1]);
   return 2;
 }

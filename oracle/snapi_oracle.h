@@ -72,6 +72,18 @@ int orc_synth(const snapi_line_table *lt, const snapi_ray *ray, int ncol, int nd
 int orc_responses(const snapi_line_table *lt, const snapi_ray *ray, int ncol, int nd, const double *state,
                   const double *dstate, int nlambda, const double *lam, double *stokes, double *resp, int nthreads);
 
+/* ALI for one NLTE atom (snapi_oracle_ali.c); single column for the first
+ * two, ncol columns for the solve.  Layouts as the snapi_ali_* entry points. */
+int orc_ali_accumulate(const snapi_ali_atom *at, const snapi_ali_grid *g, int nd, const double *colstate,
+                       const double *vproj, const double *gamma, const double *bgop, const double *bgem,
+                       const double *pops, double *acc);
+double orc_ali_newpops(const snapi_ali_atom *at, int nd, const double *colstate, const double *coll,
+                       const double *pops, const double *acc, int alo, double *newpops);
+int orc_ali_solve(const snapi_ali_atom *at, const snapi_ali_grid *g, int ncol, int nd, const double *colstate,
+                  const double *vproj, const double *gamma, const double *bgop, const double *bgem,
+                  const double *coll, double tol, int max_iter, double *pops, int *iterations, double *relchange,
+                  int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,59 @@ class SnapiRay(C.Structure):
                 ("hminus_mid", C.c_int)]
 
 
+# ALI (snapi_ali_*): per-depth fields, transitions, atom, grid
+A_T, A_VT, A_NE, A_OPREF, A_GRID, A_NA, A_NF = range(7)
+ALI_MAX_LEVELS = 16
+ALI_MAX_TRANS = 64
+TR_LINE, TR_BF = 0, 1
+
+
+class SnapiAliTransition(C.Structure):
+    _fields_ = [("kind", C.c_int), ("lower", C.c_int), ("upper", C.c_int), ("lam0", C.c_double),
+                ("aul", C.c_double), ("blu", C.c_double), ("bul", C.c_double), ("energy", C.c_double),
+                ("gratio", C.c_double), ("de", C.c_double)]
+
+
+class SnapiAliAtom(C.Structure):
+    _fields_ = [("nlevel", C.c_int), ("ntrans", C.c_int), ("mass", C.c_double),
+                ("tr", SnapiAliTransition * ALI_MAX_TRANS)]
+
+
+class SnapiAliGrid(C.Structure):
+    _fields_ = [("nlam", C.c_int), ("lam", C.POINTER(C.c_double)), ("lamw", C.POINTER(C.c_double)),
+                ("fudge", C.POINTER(C.c_double)), ("sigma", C.POINTER(C.c_double)), ("nray", C.c_int),
+                ("theta", C.POINTER(C.c_double)), ("wray", C.POINTER(C.c_double)), ("bg_per_ray", C.c_int),
+                ("boundary", C.c_double), ("tau_grid", C.c_int)]
+
+
+def make_ali_atom(nlevel, mass, trans):
+    """trans: list of dicts (kind, lower, upper and, for lines, lam0, aul, blu,
+    bul, energy; for bound-free, gratio, de) in the atom's transition order."""
+    a = SnapiAliAtom()
+    a.nlevel = nlevel
+    a.ntrans = len(trans)
+    a.mass = mass
+    for i, t in enumerate(trans):
+        for k, v in t.items():
+            setattr(a.tr[i], k, v)
+    return a
+
+
+def make_ali_grid(lam, lamw, fudge, sigma, theta, wray, bg_per_ray, boundary=-1.0, tau_grid=1):
+    """The arrays are kept alive on the returned structure (attribute _keep)."""
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (lam, lamw, fudge, sigma, theta, wray)]
+    g = SnapiAliGrid()
+    g.nlam = len(arrs[0])
+    g.lam, g.lamw, g.fudge, g.sigma = (ptr(x) for x in arrs[:4])
+    g.nray = len(arrs[4])
+    g.theta, g.wray = ptr(arrs[4]), ptr(arrs[5])
+    g.bg_per_ray = int(bg_per_ray)
+    g.boundary = boundary
+    g.tau_grid = int(tau_grid)
+    g._keep = arrs
+    return g
+
+
 def nfield(nline):
     return NF_BASE + 2 * nline
 

@@ -219,3 +219,55 @@ def node_jacobian(table, ray, model_nodes, values, state, lam, fd_scale=1.0):
     for n, (p, _) in enumerate(model_nodes):
         J[:, n] = np.einsum("zls,z->ls", R[0, pidx[p]], nm[n]).reshape(-1)
     return S[0], J
+
+
+def _ali_lib():
+    from snapi_amd import abi
+    L = lib()
+    if not getattr(L, "_ali_ready", False):
+        AT, GR = C.POINTER(abi.SnapiAliAtom), C.POINTER(abi.SnapiAliGrid)
+        dp = C.POINTER(C.c_double)
+        L.orc_ali_accumulate.argtypes = [AT, GR, C.c_int, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_ali_newpops.argtypes = [AT, C.c_int, dp, dp, dp, dp, C.c_int, dp]
+        L.orc_ali_newpops.restype = C.c_double
+        L.orc_ali_solve.argtypes = [AT, GR, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int, dp,
+                                    C.POINTER(C.c_int), dp, C.c_int]
+        L._ali_ready = True
+    return L
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def ali_accumulate(p, pops):
+    """One accumulation pass of one column; p from _golden.nlte()-like dicts."""
+    L = _ali_lib()
+    nd, nt = p["nd"], p["ntrans"]
+    acc = np.zeros((nd, nt, 4))
+    args = [_c(p[k]) for k in ("colstate", "vproj", "gamma", "bgop", "bgem")]
+    pp = _c(pops)
+    L.orc_ali_accumulate(C.byref(p["atom"]), C.byref(p["grid"]), nd, *[ptr(a) for a in args], ptr(pp), ptr(acc))
+    return acc
+
+
+def ali_newpops(p, pops, acc, alo=1):
+    L = _ali_lib()
+    nd, nm = p["nd"], p["nlevel"]
+    out = np.zeros((nd, nm))
+    cs, co, pp, ac = _c(p["colstate"]), _c(p["coll"]), _c(pops), _c(acc)
+    rc = L.orc_ali_newpops(C.byref(p["atom"]), nd, ptr(cs), ptr(co), ptr(pp), ptr(ac), alo, ptr(out))
+    return out, rc
+
+
+def ali_solve(p, pops, tol, max_iter, ncol=1, nthreads=0):
+    """Columns stacked along a leading axis of every per-column array (ncol)."""
+    L = _ali_lib()
+    nd = p["nd"]
+    pp = _c(pops).copy()
+    it = np.zeros(ncol, dtype=np.int32)
+    rc = np.zeros(ncol)
+    args = [_c(p[k]) for k in ("colstate", "vproj", "gamma", "bgop", "bgem", "coll")]
+    L.orc_ali_solve(C.byref(p["atom"]), C.byref(p["grid"]), ncol, nd, *[ptr(a) for a in args], tol, max_iter,
+                    ptr(pp), it.ctypes.data_as(C.POINTER(C.c_int)), ptr(rc), nthreads)
+    return pp, it, rc

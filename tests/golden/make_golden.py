@@ -339,6 +339,107 @@ def build_interp(d):
     print("interp: 5 node sets")
 
 
+NLTE_ATOM = "calcium_ir_rh.cfg"
+
+
+def build_nlte(d):
+    """A Ca II 5-level + continuum NLTE atom (the reference's calcium_ir_rh.cfg)
+    in the cfg0 column with the background set: inputs of one ALI iteration,
+    its sums and new populations, the iteration history and the converged
+    populations of the reference's nltepops (atmos/atmos_pop.cc:67-240)."""
+    bg = [a for a in BACKGROUND if a != "calcium.cfg"]
+    for a in bg + [NLTE_ATOM]:
+        shutil.copy(os.path.join(REF, "cfg", a), os.path.join(d, a))
+    shutil.copy(os.path.join(REF, "cfg", "mol.cfg"), os.path.join(d, "mol.cfg"))
+    with open(os.path.join(d, "atoms.cfg"), "w") as f:
+        for a in bg + [NLTE_ATOM]:
+            f.write("ATOM=%s\n" % a)
+    rows = cfg0_atmosphere()
+    with open(os.path.join(d, "atmos.cfg"), "w") as f:
+        f.write("ID=cfg0\nTYPE=SPINOR\nFILE=atmos.dat\nGEOMETRY=PLANEPARALLEL\nRTSOLVER=Bezier\n"
+                "ATOM=atoms.cfg\nMOL=mol.cfg\nELECTRONS=0\nTAU_GRID=1\nLEVEL_POPS=0\n"
+                "NX=1\nNY=1\nNZ=%d\n" % len(rows))
+    write_spinor(os.path.join(d, "atmos.dat"), rows)
+    out = run(d, ["nlte", "atmos.cfg", "100"])
+    r = {}
+    trans, sig, lam, rays, depth, pop0, gam, coll, vproj, bg, acc, newpop, iters, final, ref = \
+        [], [], [], [], [], [], [], [], [], [], [], [], [], [], []
+    for ln in out:
+        t = ln.split()
+        k = t[0]
+        if k in ("nd", "ntp", "nlam", "tau_grid", "ref_rv"):
+            r[k] = int(t[1])
+        elif k == "boundary":
+            r[k] = float(t[1])
+        elif k == "atom":
+            r["nmap"], r["ntr"], r["mass"] = int(t[3]), int(t[5]), float(t[9])
+        elif k == "ray":
+            rays.append([float(t[2]), float(t[3])])
+        elif k == "lam":
+            lam.append([float(x) for x in t[2:5]])
+        elif k == "trans":
+            if t[2] == "line":
+                trans.append([0, int(t[3]), int(t[4])] + [float(x) for x in t[5:10]] + [0.0, 0.0])
+            else:
+                trans.append([1, int(t[3]), int(t[4]), 0, 0, 0, 0, 0, float(t[5]), float(t[6])])
+        elif k == "sigma":
+            sig.append([int(t[1]), int(t[2]), float(t[3])])
+        elif k == "depth":
+            depth.append([float(x) for x in t[2:8]])
+        elif k == "pop0":
+            pop0.append([int(t[1]), int(t[2]), float(t[3])])
+        elif k == "gamma":
+            gam.append([int(t[1]), int(t[2]), float(t[3])])
+        elif k == "coll":
+            coll.append([int(t[1]), int(t[2]), int(t[3]), float(t[4])])
+        elif k == "vproj":
+            vproj.append([int(t[1]), int(t[2]), float(t[3])])
+        elif k == "bg":
+            bg.append([int(t[1]), int(t[2]), int(t[3]), float(t[4]), float(t[5])])
+        elif k == "acc":
+            acc.append([int(t[1]), int(t[2])] + [float(x) for x in t[3:7]])
+        elif k == "newpop":
+            newpop.append([int(t[1]), int(t[2]), float(t[3])])
+        elif k == "iter":
+            iters.append([int(t[1]), float(t[2]), int(t[3])])
+        elif k == "final":
+            final.append([int(t[1]), int(t[2]), float(t[3])])
+        elif k == "ref_final":
+            ref.append([int(t[1]), int(t[2]), float(t[3])])
+    nd, nl, nt, nm, ntp = r["nd"], r["nlam"], r["ntr"], r["nmap"], r["ntp"]
+    z0 = 1  # the reference's depth index starts at x3l = 1
+
+    def grid3(rows, shape, idx):
+        a = np.zeros(shape)
+        for row in rows:
+            a[idx(row)] = row[-1]
+        return a
+    rec = dict(nd=nd, nlam=nl, ntr=nt, nmap=nm, ntp=ntp, mass=r["mass"], boundary=r["boundary"],
+               tau_grid=r["tau_grid"], ref_rv=r["ref_rv"], rays=np.array(rays), lam=np.array(lam),
+               trans=np.array(trans), depth=np.array(depth))
+    rec["sigma"] = grid3(sig, (nt, nl), lambda w: (w[0] - 1, w[1]))
+    rec["pop0"] = grid3(pop0, (nd, nm), lambda w: (w[0] - z0, w[1]))
+    rec["gamma"] = grid3(gam, (nt, nd), lambda w: (w[0] - 1, w[1] - z0))
+    rec["coll"] = grid3(coll, (nd, nm, nm), lambda w: (w[0] - z0, w[1], w[2]))
+    rec["vproj"] = grid3(vproj, (ntp, nd), lambda w: (w[0] - 1, w[1] - z0))
+    b = np.array(bg)
+    bgop = np.zeros((ntp, nd, nl))
+    bgem = np.zeros((ntp, nd, nl))
+    bgop[b[:, 0].astype(int) - 1, b[:, 2].astype(int) - z0, b[:, 1].astype(int)] = b[:, 3]
+    bgem[b[:, 0].astype(int) - 1, b[:, 2].astype(int) - z0, b[:, 1].astype(int)] = b[:, 4]
+    rec["bgop"], rec["bgem"] = bgop, bgem
+    a = np.zeros((nd, nt, 4))
+    for w in acc:
+        a[w[0] - z0, w[1] - 1] = w[2:6]
+    rec["acc"] = a
+    rec["newpop"] = grid3(newpop, (nd, nm), lambda w: (w[0] - z0, w[1]))
+    rec["iters"] = np.array(iters)
+    rec["final"] = grid3(final, (nd, nm), lambda w: (w[0] - z0, w[1]))
+    rec["ref_final"] = grid3(ref, (nd, nm), lambda w: (w[0] - z0, w[1]))
+    np.savez_compressed(os.path.join(HERE, "nlte.npz"), **rec)
+    print("nlte: %d depths, %d wavelengths, %d rays, %d iterations" % (nd, nl, ntp, len(iters)))
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
@@ -350,6 +451,7 @@ def main():
         build_formal_scalar(d)
         build_lmstep(d)
         build_interp(d)
+        build_nlte(d)
     finally:
         shutil.rmtree(d)
 

@@ -208,3 +208,61 @@ def test_node_interpolation_matches_reference_golden():
     for n in range(1, 6):
         q = _oracle._interp_param(g["n%d_tau" % n], g["n%d_val" % n], g["n%d_grid" % n])
         assert np.max(np.abs(q - g["n%d_q" % n])) < 1e-14, n
+
+
+def test_ali_fixture_replicates_reference_nltepops():
+    """The golden loop (driven with the reference's functions) ends where the
+    reference's own nltepops ends."""
+    import _golden
+    p = _golden.nlte()
+    assert np.array_equal(p["final"], p["ref_final"])
+
+
+def test_ali_accumulation_matches_reference():
+    """J, Lambda* and norm sums of one ALI pass (atom::add) at LTE start
+    populations."""
+    import _golden
+    p = _golden.nlte()
+    acc = _oracle.ali_accumulate(p, p["pop0"])
+    ref = p["acc"]
+    for q, name in enumerate(("Jlu", "Jul", "Lstar", "norm")):
+        scale = np.maximum(np.abs(ref[..., q]).max(axis=0), 1e-300)
+        err = np.max(np.abs(acc[..., q] - ref[..., q]) / scale)
+        assert err < 1e-10, (name, err)
+
+
+def test_ali_newpops_matches_reference():
+    """Rate matrix with the local operator, conservation row, Crout solve
+    (atom::newpops) from the reference's own sums."""
+    import _golden
+    p = _golden.nlte()
+    new, rc = _oracle.ali_newpops(p, p["pop0"], p["acc"], 1)
+    assert np.max(np.abs(new / p["newpop"] - 1.0)) < 1e-10
+    assert abs(rc - p["iters"][0, 1]) / p["iters"][0, 1] < 1e-10
+
+
+def test_ali_solve_matches_reference_nltepops():
+    import _golden
+    p = _golden.nlte()
+    pops, it, rc = _oracle.ali_solve(p, p["pop0"], 1e-2, 100)
+    assert it[0] == len(p["iters"])
+    assert np.max(np.abs(pops / p["ref_final"] - 1.0)) < 1e-9
+    assert abs(rc[0] - p["iters"][-1, 1]) < 1e-9
+
+
+def test_ali_collision_dominated_limit():
+    """Collisions dominating every radiative rate drive the populations to
+    the Boltzmann/Saha values that zero every net collisional rate
+    (detailed balance): scaling all collision rates up by 1e12 must leave a
+    population vector n with C^T n = 0 at each depth."""
+    import _golden
+    p = dict(_golden.nlte())
+    p["coll"] = p["coll"] * 1e12
+    acc = _oracle.ali_accumulate(p, p["pop0"])
+    new, rc = _oracle.ali_newpops(p, p["pop0"], acc, 1)
+    nd, nm = p["nd"], p["nlevel"]
+    for z in range(0, nd, 10):
+        Cm = p["coll"][z]
+        net = Cm.T @ new[z] - Cm.sum(axis=1) * new[z]
+        scale = (Cm.sum(axis=1) * new[z]).max()
+        assert np.max(np.abs(net)) / scale < 1e-8, z

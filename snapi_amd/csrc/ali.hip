@@ -1,0 +1,779 @@
+// ALI (approximate Lambda iteration) for one NLTE atom on gfx950.
+//
+// Restates atmosphere::nltepops (atmos/atmos_pop.cc:67-240):
+//   - accumulation: for every ray and wavelength, the scalar opacity and
+//     emissivity (the population-independent background given, plus the
+//     atom's bound-free terms atom_rad.cc:312-437 and Voigt lines
+//     atom_rad.cc:439-537 with the current populations), the scalar
+//     short-characteristics formal solution with the diagonal Lambda*
+//     (plpar/atmos_ppbez_rts.cc:21-219) and the sums of atom::add
+//     (atmos/atmol/atom/atom.cc:1865-1927);
+//   - new populations: atom::newpops (atom_pop.cc:27-135), rate matrix with
+//     the local approximate operator (R_ij_local_ALO, atom_rad.cc:1732-1757)
+//     or plain Lambda iteration (R_ij), bound-free R_i_cont / R_cont_i
+//     (:1842-1854), conservation row for the most populated level, the
+//     pivot-free Crout solve (cmlib/mathtools.cc:487-514), in double;
+//   - iteration: Lambda-iteration fallback for negative populations, stop
+//     at a relative change below tol (atmos_pop.cc:198-214).
+// One workgroup per column, work-items over wavelengths; the depth
+// recurrence of every wavelength advances in lock-step so that the sums of
+// each depth are reduced across the workgroup (wavefront shuffles, then
+// LDS) and added to the column's accumulators by the workgroup that owns the
+// column.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rt_common.h"
+#include "synth_kernel.h"
+
+namespace snapi {
+
+constexpr int kAliThreads = 256;
+constexpr int kAliMaxLines = 16;  // lines whose profiles a work-item carries between depths
+constexpr int kAliMaxBf = 16;
+constexpr double kSaha = 2.0706615382E-16;  // cmlib/const.h:24
+
+// Atom constants as kernel arguments.
+struct AliAtom {
+  int nlevel, ntrans, nline, nbf;
+  double mass;
+  int line_t[kAliMaxLines], line_lo[kAliMaxLines], line_up[kAliMaxLines];
+  double lam0[kAliMaxLines], aul[kAliMaxLines], blu[kAliMaxLines], bul[kAliMaxLines], energy[kAliMaxLines];
+  int bf_t[kAliMaxBf], bf_lo[kAliMaxBf], bf_up[kAliMaxBf];
+  double gratio[kAliMaxBf], de[kAliMaxBf];
+  // rate matrix structure
+  int stage[SNAPI_ALI_MAX_LEVELS];
+  int bf_of[SNAPI_ALI_MAX_LEVELS];                           // bound-free slot of a level or -1
+  signed char pair_line[SNAPI_ALI_MAX_LEVELS][SNAPI_ALI_MAX_LEVELS];  // line slot between two levels or -1
+};
+
+struct AliGridArgs {
+  int nlam, nray, bg_per_ray, tau_grid;
+  double boundary;
+  double ct[16], wray[16];
+};
+
+struct AliCol {  // device pointers of the per-column inputs
+  const double *cs, *vproj, *gamma, *bgop, *bgem, *pops;
+  const double *lam, *lamw, *fudge, *sigma;
+  double *acc;
+  const int *active;
+};
+
+template <typename R> struct AliTile {
+  R *grid, *opref, *tinv, *vr;          // per depth (vr: velocity along the current ray / c)
+  R *idld, *a, *lof, *emf, *accf;       // per line and depth
+  R *nlo, *e1;                          // per bound-free and depth
+  static __host__ __device__ size_t bytes(int nd, int nline, int nbf) {
+    return sizeof(R) * (size_t)nd * (4 + 5 * nline + 2 * nbf);
+  }
+  static __device__ AliTile carve(R *p, int nd, int nline, int nbf) {
+    AliTile t;
+    t.grid = p; p += nd;
+    t.opref = p; p += nd;
+    t.tinv = p; p += nd;
+    t.vr = p; p += nd;
+    t.idld = p; p += (size_t)nd * nline;
+    t.a = p; p += (size_t)nd * nline;
+    t.lof = p; p += (size_t)nd * nline;
+    t.emf = p; p += (size_t)nd * nline;
+    t.accf = p; p += (size_t)nd * nline;
+    t.nlo = p; p += (size_t)nd * nbf;
+    t.e1 = p;
+    return t;
+  }
+};
+
+template <typename R>
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// opacity, emissivity and line profiles of one depth for one wavelength
+template <typename R>
+__device__ __forceinline__ void ali_opem(const AliAtom &A, const AliTile<R> &t, int nd, int z, double lam, R lamR,
+                                         R hcl, R plc, R ple, R fud, const R sg[kAliMaxBf], R bo, R be, R &op,
+                                         R &em, R prof[kAliMaxLines]) {
+  R o = bo, e = be;
+  const R ex = rexp<R>(-hcl * t.tinv[z]);  // exp(-h c / (lambda k T))
+  const R B = plc * rrcp<R>(rexp<R>(ple * t.tinv[z]) - R(1));
+#pragma unroll
+  for (int b = 0; b < kAliMaxBf; ++b) {
+    if (b >= A.nbf) continue;
+    const R sig = sg[b];
+    if (sig == R(0)) continue;
+    const int i = b * nd + z;
+    o += sig * (t.nlo[i] - t.e1[i] * ex) * fud;
+    e += sig * t.e1[i] * (R(1) - ex) * B * fud;
+  }
+  const R hc4l = R(kH * kC / (4.0 * kPi)) * rrcp<R>(lamR);
+#pragma unroll
+  for (int j = 0; j < kAliMaxLines; ++j) {
+    prof[j] = R(0);
+    if (j >= A.nline) continue;
+    const int i = j * nd + z;
+    const R x = (R)(A.lam0[j] - lam) * t.idld[i] - lamR * t.vr[z] * t.idld[i];
+    R H, F;
+    voigt<R>(x, t.a[i], H, F);
+    const R p = t.idld[i] * H;
+    prof[j] = p;
+    o += p * t.lof[i] * hc4l;
+    e += p * t.emf[i] * hc4l;
+  }
+  op = o;
+  em = e;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, const AliGridArgs G, int nd, AliCol P) {
+  extern __shared__ unsigned char smem_raw[];
+  const int col = blockIdx.x;
+  if (P.active && !P.active[col]) return;  // uniform over the workgroup
+  const int nline = A.nline, nbf = A.nbf, nlam = G.nlam;
+  const int nq = 3 * nline + 2 * nbf;
+  const int nwave = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  double *red = reinterpret_cast<double *>(smem_raw);  // [2][nwave][nq]
+  R *sm = reinterpret_cast<R *>(smem_raw + sizeof(double) * 2 * nwave * nq);
+  AliTile<R> t = AliTile<R>::carve(sm, nd, nline, nbf);
+  const double *cs = P.cs + (size_t)col * SNAPI_A_NF * nd;
+  const double *pops = P.pops + (size_t)col * nd * A.nlevel;
+  double *acc = P.acc + (size_t)col * nd * A.ntrans * 4;
+  // per-depth and per-(transition, depth) quantities of this column
+  for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+    const double T = cs[SNAPI_A_T * nd + z];
+    t.grid[z] = (R)cs[SNAPI_A_GRID * nd + z];
+    t.opref[z] = (R)cs[SNAPI_A_OPREF * nd + z];
+    t.tinv[z] = (R)(1.0 / T);
+  }
+  for (int i = threadIdx.x; i < nd * nline; i += blockDim.x) {
+    const int j = i / nd, z = i - j * nd;
+    const double T = cs[SNAPI_A_T * nd + z], vt = cs[SNAPI_A_VT * nd + z];
+    const double l0 = A.lam0[j];
+    const double dld = sqrt(2.0 * kK * T / A.mass + vt * vt) * l0 / kC;
+    const double sf = l0 * l0 / (4.0 * kPi * kC);
+    const double *n = pops + (size_t)z * A.nlevel;
+    const double nlo = n[A.line_lo[j]], nup = n[A.line_up[j]];
+    t.idld[i] = (R)(1.0 / dld);
+    t.a[i] = (R)(sf * P.gamma[((size_t)col * A.ntrans + A.line_t[j]) * nd + z] / dld);
+    t.lof[i] = (R)(nlo * A.blu[j] * (1.0 - nup / nlo * A.bul[j] / A.blu[j]));
+    t.emf[i] = (R)(nup * A.aul[j]);
+    t.accf[i] = (R)((nlo * A.blu[j] - nup * A.bul[j]) * A.energy[j] * 0.25 / kPi);
+  }
+  for (int i = threadIdx.x; i < nd * nbf; i += blockDim.x) {
+    const int b = i / nd, z = i - b * nd;
+    const double T = cs[SNAPI_A_T * nd + z], ne = cs[SNAPI_A_NE * nd + z];
+    const double *n = pops + (size_t)z * A.nlevel;
+    t.nlo[i] = (R)n[A.bf_lo[b]];
+    t.e1[i] = (R)(n[A.bf_up[b]] * ne * kSaha * pow(T, -1.5) * A.gratio[b] * exp(A.de[b] / kK / T));
+  }
+  for (int r = 0; r < G.nray; ++r) {
+    __syncthreads();  // the previous ray's readers of vr are done
+    for (int z = threadIdx.x; z < nd; z += blockDim.x)
+      t.vr[z] = (R)(P.vproj[((size_t)col * G.nray + r) * nd + z] / kC);
+    __syncthreads();
+    const double ct = G.ct[r];
+    const R mct = (R)(-ct);
+    const bool down = ct > 0.0;  // enters at the top
+    auto zof = [&](int k) { return down ? k : nd - 1 - k; };
+    const size_t bgo = ((size_t)col * (G.bg_per_ray ? G.nray : 1) + (G.bg_per_ray ? r : 0)) * nd * nlam;
+    const double *bop = P.bgop + bgo, *bem = P.bgem + bgo;
+    for (int l0 = 0; l0 < nlam; l0 += blockDim.x) {
+      const int l = l0 + threadIdx.x;
+      const bool valid = l < nlam;
+      const int lc = valid ? l : nlam - 1;  // lanes past the grid compute a copy and add nothing
+      const double lam = P.lam[lc];
+      const R lamR = (R)lam;
+      const R hcl = (R)(kH * kC / lam / kK);
+      const R plc = (R)(2.0 * 6.62606957 * 8.9875517E-7 / lam / lam / lam / lam / lam);
+      const R ple = (R)(1.4387769 / lam);
+      const R fud = (R)P.fudge[lc];
+      const double ww = valid ? P.lamw[lc] * G.wray[r] : 0.0;
+      // per-wavelength constants: cross-sections, 2 h c^2 / lambda^5, h c / (lambda k)
+      R sg[kAliMaxBf];
+      double sgd[kAliMaxBf];
+#pragma unroll
+      for (int b = 0; b < kAliMaxBf; ++b) {
+        sgd[b] = b < nbf ? P.sigma[(size_t)A.bf_t[b] * nlam + lc] : 0.0;
+        sg[b] = (R)sgd[b];
+      }
+      const double c5 = 2.0 * kH * kC * kC / pow(lam, 5.0);
+      const double hclk = kH * kC / lam / kK;
+      R profc[kAliMaxLines], profn[kAliMaxLines];
+      R opc, emc, opn, emn;
+      // depth 0 and the look-ahead depth 1
+      const int z0 = zof(0), z1 = zof(1);
+      ali_opem<R>(A, t, nd, z0, lam, lamR, hcl, plc, ple, fud, sg, (R)bop[(size_t)z0 * nlam + lc],
+                  (R)bem[(size_t)z0 * nlam + lc], opc, emc, profc);
+      ali_opem<R>(A, t, nd, z1, lam, lamR, hcl, plc, ple, fud, sg, (R)bop[(size_t)z1 * nlam + lc],
+                  (R)bem[(size_t)z1 * nlam + lc], opn, emn, profn);
+      const bool tg = G.tau_grid != 0;
+      R oom = tg ? opc / t.opref[z0] : opc, oo0 = tg ? opn / t.opref[z1] : opn;
+      R Sm = tg ? (emc / t.opref[z0]) / oom : emc / oom;
+      R S0 = tg ? (emn / t.opref[z1]) / oo0 : emn / oo0;
+      R h0 = (t.grid[z1] - t.grid[z0]) / mct;
+      R Dm = (oo0 - oom) / h0;
+      R I = down ? R(0) : Sm;
+      if (!down && G.boundary >= 0.0) I = (R)G.boundary;
+      R Ls = R(0);
+      if (!down) Ls = (G.boundary == -1.0) ? R(1) : R(0);
+      for (int k = 0; k < nd; ++k) {
+        const int z = zof(k);
+        R opd = opc;  // the total opacity of depth z (de-normalised as the reference does)
+        if (k > 0) {
+          // optical-depth step k-1 -> k (D of the last point one-sided)
+          R D0, oop = R(0), hp = R(1), Sp = R(0), opp = R(0), emp = R(0);
+          R profp[kAliMaxLines];
+          if (k < nd - 1) {
+            const int zn = zof(k + 1);
+            ali_opem<R>(A, t, nd, zn, lam, lamR, hcl, plc, ple, fud, sg,
+                        (R)bop[(size_t)zn * nlam + lc], (R)bem[(size_t)zn * nlam + lc], opp, emp, profp);
+            oop = tg ? opp / t.opref[zn] : opp;
+            Sp = tg ? (emp / t.opref[zn]) / oop : emp / oop;
+            hp = (t.grid[zn] - t.grid[z]) / mct;
+            D0 = fb_deriv<R>(oom, oo0, oop, h0, hp);
+          } else {
+            D0 = (oo0 - oom) / h0;
+          }
+          const R C = R(0.5) * ((oo0 - h0 * R(0.5) * D0) + (oom + h0 * R(0.5) * Dm));
+          const R tb = h0 * R(1.0 / 3.0) * (oom + oo0 + C);
+          R etb, w0, w1;
+          sc_weights<R>(tb, etb, w0, w1);
+          I = I * etb + w0 * Sm + w1 * S0;
+          Ls = w1;
+          opd = tg ? oo0 * t.opref[z] : opn;
+#pragma unroll
+          for (int j = 0; j < kAliMaxLines; ++j) profc[j] = profn[j];
+          // shift the window: depth k+1 becomes the look-ahead of the next step
+          oom = oo0;
+          oo0 = oop;
+          Sm = S0;
+          S0 = Sp;
+          Dm = D0;
+          h0 = hp;
+          opn = opp;
+#pragma unroll
+          for (int j = 0; j < kAliMaxLines; ++j) profn[j] = profp[j];
+        }
+        // contributions of this wavelength to the sums of depth z
+        const int buf = k & 1;
+        double *rb = red + (size_t)buf * nwave * nq;
+        const R iop = rrcp<R>(opd);
+        int q = 0;
+#pragma unroll
+        for (int j = 0; j < kAliMaxLines; ++j) {
+          if (j >= nline) continue;
+          const R p = profc[j];
+          const R cf = p * (t.accf[j * nd + z] * iop) * R(0.25 / kPi);
+          const double cJ = wave_sum<R>(ww * (double)(cf * I));
+          const double cL = wave_sum<R>(ww * (double)(cf * Ls));
+          const double cN = wave_sum<R>(ww * (double)(p * R(0.25 / kPi)));
+          if (lane == 0) {
+            rb[wave * nq + q] = cJ;
+            rb[wave * nq + q + 1] = cL;
+            rb[wave * nq + q + 2] = cN;
+          }
+          q += 3;
+        }
+        const double exd = nbf ? exp(-hclk * (double)t.tinv[z]) : 0.0;
+#pragma unroll
+        for (int b = 0; b < kAliMaxBf; ++b) {
+          if (b >= nbf) continue;
+          const double s = ww * sgd[b] * lam / kH / kC;
+          const double Id = (double)I;
+          const double cm = wave_sum<R>(s * Id);
+          const double cn = wave_sum<R>(s * (Id + c5) * exd);
+          if (lane == 0) {
+            rb[wave * nq + q] = cm;
+            rb[wave * nq + q + 1] = cn;
+          }
+          q += 2;
+        }
+        __syncthreads();
+        for (int qq = threadIdx.x; qq < nq; qq += blockDim.x) {
+          double s = 0.0;
+          for (int w = 0; w < nwave; ++w) s += rb[w * nq + qq];
+          int tr, slot;
+          if (qq < 3 * nline) {
+            tr = A.line_t[qq / 3];
+            slot = qq % 3;
+          } else {
+            tr = A.bf_t[(qq - 3 * nline) / 2];
+            slot = (qq - 3 * nline) % 2;
+          }
+          double *ap = acc + ((size_t)z * A.ntrans + tr) * 4;
+          if (qq < 3 * nline) {
+            if (slot == 0) {
+              ap[0] += s;
+              ap[1] += s;
+            } else if (slot == 1) {
+              ap[2] += s;
+            } else {
+              ap[3] += s;
+            }
+          } else {
+            ap[slot] += s;
+          }
+        }
+        // the other buffer is written next; this one is read until the
+        // barrier of the following depth
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// New populations of one (column, depth): thread per depth, matrix in LDS.
+__global__ void ali_newpops_kernel(const AliAtom A, int ncol, int nd, const double *__restrict__ cs_all,
+                                   const double *__restrict__ coll, const double *__restrict__ pops,
+                                   const double *__restrict__ acc, int alo, double *__restrict__ newp,
+                                   double *__restrict__ rc_col, int *__restrict__ neg_col,
+                                   const int *__restrict__ mask) {
+  extern __shared__ double lds[];
+  const int n = A.nlevel, nt = A.ntrans;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)ncol * nd) return;
+  const int col = (int)(gid / nd), z = (int)(gid % nd);
+  if (mask && !mask[col]) return;
+  double *M = lds + (size_t)threadIdx.x * (n * n + 2 * n);
+  double *b = M + n * n, *y = b + n;
+  const double *p = pops + ((size_t)col * nd + z) * n;
+  const double *A4 = acc + ((size_t)col * nd + z) * nt * 4;
+  const double *C = coll + ((size_t)col * nd + z) * n * n;
+  const double *cs = cs_all + (size_t)col * SNAPI_A_NF * nd;
+  const double T = cs[SNAPI_A_T * nd + z], ne = cs[SNAPI_A_NE * nd + z];
+  for (int i = 0; i < n * n; ++i) M[i] = 0.0;
+  for (int i = 0; i < n; ++i) b[i] = 0.0;
+  for (int i = 0; i < n - 1; ++i) {
+    for (int j = 0; j < n; ++j) {
+      if (A.stage[j] != A.stage[i] || j == i) continue;
+      const int s = A.pair_line[i][j];
+      double Rout = 0.0, Rin = 0.0;
+      if (s >= 0) {
+        const int t = A.line_t[s];
+        const double JJ = A4[t * 4 + 0] / A4[t * 4 + 3], LL = A4[t * 4 + 2] / A4[t * 4 + 3];
+        const bool idown = (i == A.line_up[s]);  // i -> j is a downward transition
+        const double nu = idown ? p[i] : p[j], nl = idown ? p[j] : p[i];
+        const double Sl = nu * A.aul[s] / (nl * A.blu[s] - nu * A.bul[s]);
+        const double Ao = idown ? A.aul[s] : 0.0, Bo = idown ? A.bul[s] : A.blu[s];
+        const double Ai = idown ? 0.0 : A.aul[s], Bi = idown ? A.blu[s] : A.bul[s];
+        if (alo) {
+          Rout = Ao * (1.0 - LL) + Bo * (JJ - LL * Sl);
+          Rin = Ai * (1.0 - LL) + Bi * (JJ - LL * Sl);
+        } else {
+          Rout = Ao + Bo * JJ;
+          Rin = Ai + Bi * JJ;
+        }
+      }
+      M[i * n + i] -= (Rout + C[i * n + j]);
+      M[i * n + j] += (Rin + C[j * n + i]);
+    }
+    const int bs = A.bf_of[i];
+    if (bs >= 0) {
+      const int t = A.bf_t[bs], ic = A.bf_up[bs];
+      double R = A4[t * 4 + 0];
+      M[i * n + i] -= (R + C[i * n + ic]);
+      M[ic * n + i] += (R + C[i * n + ic]);
+      R = A4[t * 4 + 1] * A.gratio[bs] * exp(A.de[bs] / kK / T) * pow(T, -1.5) * ne * kSaha;
+      M[i * n + ic] += (R + C[ic * n + i]);
+      M[ic * n + ic] -= (R + C[ic * n + i]);
+    }
+  }
+  int rep = n - 1;
+  double maxpop = p[rep];
+  for (int i = 0; i < n - 1; ++i)
+    if (p[i] > maxpop) {
+      maxpop = p[i];
+      rep = i;
+    }
+  for (int j = 0; j < n; ++j) M[rep * n + j] = 1.0;
+  b[rep] = cs[SNAPI_A_NA * nd + z];
+  // pivot-free Crout in place: L (with diagonal) below, unit U above
+  for (int k = 0; k < n; ++k) {
+    for (int i = k; i < n; ++i) {
+      double sum = 0.0;
+      for (int q = 0; q < k; ++q) sum += M[i * n + q] * M[q * n + k];
+      M[i * n + k] -= sum;
+    }
+    for (int j = k + 1; j < n; ++j) {
+      double sum = 0.0;
+      for (int q = 0; q < k; ++q) sum += M[k * n + q] * M[q * n + j];
+      M[k * n + j] = (M[k * n + j] - sum) / M[k * n + k];
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    double sum = 0.0;
+    for (int q = 0; q < i; ++q) sum += M[i * n + q] * y[q];
+    y[i] = (b[i] - sum) / M[i * n + i];
+  }
+  double *x = newp + ((size_t)col * nd + z) * n;
+  for (int i = n - 1; i >= 0; --i) {
+    double sum = 0.0;
+    for (int q = i + 1; q < n; ++q) sum += M[i * n + q] * x[q];
+    x[i] = y[i] - sum;
+  }
+  double mx = 0.0;
+  int neg = 0;
+  for (int i = 0; i < n; ++i) {
+    const double r = x[i] / p[i] - 1.0;
+    if (r < -1.0) neg = 1;
+    if (!(fabs(r) <= mx)) mx = fabs(r);  // a NaN ends up as the maximum
+  }
+  // non-negative doubles order like their bit patterns
+  atomicMax(reinterpret_cast<unsigned long long *>(rc_col + col), (unsigned long long)__double_as_longlong(mx));
+  if (neg) atomicOr(neg_col + col, 1);
+}
+
+// per-column bookkeeping of the iteration
+__global__ void ali_update_kernel(int ncol, int nd, int n, int max_iter, double tol, int *__restrict__ active,
+                                  int *__restrict__ retry, const double *__restrict__ rc, const int *__restrict__ neg,
+                                  const double *__restrict__ newp, double *__restrict__ pops, int *__restrict__ iters,
+                                  double *__restrict__ rc_out, int stage) {
+  const int col = blockIdx.x;
+  if (!active[col]) return;
+  if (stage == 0) {  // after the approximate-operator solve: columns to redo with Lambda iteration
+    if (threadIdx.x == 0) retry[col] = neg[col];
+    return;
+  }
+  __syncthreads();
+  const bool bad = neg[col] != 0;
+  if (!bad)
+    for (int i = threadIdx.x; i < nd * n; i += blockDim.x) pops[(size_t)col * nd * n + i] = newp[(size_t)col * nd * n + i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    iters[col] += 1;
+    rc_out[col] = bad ? -1.0 : rc[col];
+    if (bad || rc[col] < tol || iters[col] >= max_iter) active[col] = 0;
+  }
+}
+
+__global__ void ali_reset_kernel(int ncol, const int *__restrict__ mask, double *__restrict__ rc,
+                                 int *__restrict__ neg) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol || (mask && !mask[col])) return;
+  rc[col] = 0.0;
+  neg[col] = 0;
+}
+
+__global__ void ali_zero_kernel(size_t n_per_col, int ncol, const int *__restrict__ mask, double *__restrict__ p) {
+  const size_t n = n_per_col * (size_t)ncol;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    if (!mask || mask[i / n_per_col]) p[i] = 0.0;
+}
+
+__global__ void ali_count_kernel(int ncol, const int *__restrict__ active, int *__restrict__ count) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col < ncol && active[col]) atomicAdd(count, 1);
+}
+
+// ---------------------------------------------------------------- host side
+static int convert_atom(const snapi_ali_atom *a, AliAtom &A) {
+  if (!a || a->nlevel < 2 || a->nlevel > SNAPI_ALI_MAX_LEVELS || a->ntrans < 1 || a->ntrans > SNAPI_ALI_MAX_TRANS)
+    return SNAPI_ERR_ARG;
+  memset(&A, 0, sizeof(A));
+  A.nlevel = a->nlevel;
+  A.ntrans = a->ntrans;
+  A.mass = a->mass;
+  for (int i = 0; i < SNAPI_ALI_MAX_LEVELS; ++i) {
+    A.bf_of[i] = -1;
+    for (int j = 0; j < SNAPI_ALI_MAX_LEVELS; ++j) A.pair_line[i][j] = -1;
+  }
+  for (int t = 0; t < a->ntrans; ++t) {
+    const snapi_ali_transition &tr = a->tr[t];
+    if (tr.lower < 0 || tr.lower >= a->nlevel || tr.upper < 0 || tr.upper >= a->nlevel || tr.lower == tr.upper)
+      return SNAPI_ERR_ARG;
+    if (tr.kind == SNAPI_TR_LINE) {
+      if (A.nline >= kAliMaxLines) return SNAPI_ERR_UNSUPPORTED;
+      const int j = A.nline++;
+      A.line_t[j] = t;
+      A.line_lo[j] = tr.lower;
+      A.line_up[j] = tr.upper;
+      A.lam0[j] = tr.lam0;
+      A.aul[j] = tr.aul;
+      A.blu[j] = tr.blu;
+      A.bul[j] = tr.bul;
+      A.energy[j] = tr.energy;
+      A.pair_line[tr.lower][tr.upper] = A.pair_line[tr.upper][tr.lower] = (signed char)j;
+      if (!(tr.blu > 0.0) || !(tr.lam0 > 0.0)) return SNAPI_ERR_ARG;
+    } else if (tr.kind == SNAPI_TR_BF) {
+      if (A.nbf >= kAliMaxBf || A.bf_of[tr.lower] >= 0) return SNAPI_ERR_UNSUPPORTED;
+      const int b = A.nbf++;
+      A.bf_t[b] = t;
+      A.bf_lo[b] = tr.lower;
+      A.bf_up[b] = tr.upper;
+      A.gratio[b] = tr.gratio;
+      A.de[b] = tr.de;
+      A.bf_of[tr.lower] = b;
+    } else {
+      return SNAPI_ERR_ARG;
+    }
+  }
+  // every level but the last ionises; the stage ends at a level whose
+  // continuum is the next level
+  for (int i = 0; i < a->nlevel - 1; ++i)
+    if (A.bf_of[i] < 0) return SNAPI_ERR_UNSUPPORTED;
+  if (A.bf_of[a->nlevel - 1] >= 0) return SNAPI_ERR_UNSUPPORTED;
+  int s = 0;
+  for (int i = 0; i < a->nlevel; ++i) {
+    A.stage[i] = s;
+    if (i + 1 < a->nlevel && A.bf_up[A.bf_of[i]] == i + 1) ++s;
+  }
+  return SNAPI_OK;
+}
+
+static int convert_grid(const snapi_ali_grid *g, AliGridArgs &G) {
+  if (!g || g->nlam < 1 || g->nray < 1 || g->nray > 16 || !g->lam || !g->lamw || !g->fudge || !g->sigma ||
+      !g->theta || !g->wray)
+    return SNAPI_ERR_ARG;
+  G.nlam = g->nlam;
+  G.nray = g->nray;
+  G.bg_per_ray = g->bg_per_ray ? 1 : 0;
+  G.tau_grid = g->tau_grid ? 1 : 0;
+  G.boundary = g->boundary;
+  for (int r = 0; r < g->nray; ++r) {
+    G.ct[r] = cos(g->theta[r]);
+    G.wray[r] = g->wray[r];
+    if (fabs(G.ct[r]) < 1e-6) return SNAPI_ERR_ARG;
+  }
+  return SNAPI_OK;
+}
+
+static size_t acc_smem(int precision, const AliAtom &A, int nd, int threads) {
+  const int nq = 3 * A.nline + 2 * A.nbf;
+  const size_t es = precision == SNAPI_FP64 ? 8 : 4;
+  return sizeof(double) * 2 * (threads / 64) * nq + es * (size_t)nd * (4 + 5 * A.nline + 2 * A.nbf);
+}
+
+static int launch_acc(int precision, const AliAtom &A, const AliGridArgs &G, int ncol, int nd, const AliCol &P,
+                      hipStream_t st) {
+  const int threads = kAliThreads;
+  const size_t smem = acc_smem(precision, A, nd, threads);
+  if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;
+  if (precision == SNAPI_FP64)
+    hipLaunchKernelGGL(ali_acc_kernel<double>, dim3(ncol), dim3(threads), smem, st, A, G, nd, P);
+  else
+    hipLaunchKernelGGL(ali_acc_kernel<float>, dim3(ncol), dim3(threads), smem, st, A, G, nd, P);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+static int newpops_threads(int n) {
+  const size_t per = sizeof(double) * (size_t)(n * n + 2 * n);
+  int th = (int)(65536 / per);
+  if (th > 128) th = 128;
+  th = (th / 32) * 32;
+  return th < 32 ? 32 : th;
+}
+
+static int launch_newpops(const AliAtom &A, int ncol, int nd, const double *cs, const double *coll,
+                          const double *pops, const double *acc, int alo, double *newp, double *rc, int *neg,
+                          const int *mask, hipStream_t st) {
+  const int th = newpops_threads(A.nlevel);
+  const size_t smem = sizeof(double) * (size_t)th * (A.nlevel * A.nlevel + 2 * A.nlevel);
+  const long long nthr = (long long)ncol * nd;
+  hipLaunchKernelGGL(ali_reset_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, ncol, mask, rc, neg);
+  hipLaunchKernelGGL(ali_newpops_kernel, dim3((unsigned)((nthr + th - 1) / th)), dim3(th), smem, st, A, ncol, nd, cs,
+                     coll, pops, acc, alo, newp, rc, neg, mask);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+#define ACHK(x)                \
+  do {                         \
+    if ((x) != hipSuccess) {   \
+      rc = SNAPI_ERR_DEVICE;   \
+      goto done;               \
+    }                          \
+  } while (0)
+
+struct AliBuffers {
+  double *cs = nullptr, *vproj = nullptr, *gamma = nullptr, *bgop = nullptr, *bgem = nullptr, *pops = nullptr;
+  double *lam = nullptr, *lamw = nullptr, *fudge = nullptr, *sigma = nullptr, *acc = nullptr;
+  void release() {
+    double *p[] = {cs, vproj, gamma, bgop, bgem, pops, lam, lamw, fudge, sigma, acc};
+    for (double *q : p) hipFree(q);
+  }
+};
+
+static int upload_inputs(const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol, int nd,
+                         const double *colstate, const double *vproj, const double *gamma, const double *bgop,
+                         const double *bgem, const double *pops, AliBuffers &B, AliCol &P) {
+  const size_t nlam = grid->nlam, nray = grid->nray, nt = atom->ntrans;
+  const size_t nbg = (size_t)ncol * (grid->bg_per_ray ? nray : 1) * nd * nlam;
+  struct {
+    double **dst;
+    const void *src;
+    size_t n;
+  } up[] = {{&B.cs, colstate, (size_t)ncol * SNAPI_A_NF * nd},
+            {&B.vproj, vproj, (size_t)ncol * nray * nd},
+            {&B.gamma, gamma, (size_t)ncol * nt * nd},
+            {&B.bgop, bgop, nbg},
+            {&B.bgem, bgem, nbg},
+            {&B.pops, pops, (size_t)ncol * nd * atom->nlevel},
+            {&B.lam, grid->lam, nlam},
+            {&B.lamw, grid->lamw, nlam},
+            {&B.fudge, grid->fudge, nlam},
+            {&B.sigma, grid->sigma, nt * nlam}};
+  for (auto &u : up) {
+    if (hipMalloc((void **)u.dst, u.n * sizeof(double)) != hipSuccess) return SNAPI_ERR_DEVICE;
+    if (hipMemcpy(*u.dst, u.src, u.n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return SNAPI_ERR_DEVICE;
+  }
+  if (hipMalloc((void **)&B.acc, (size_t)ncol * nd * nt * 4 * sizeof(double)) != hipSuccess) return SNAPI_ERR_DEVICE;
+  P.cs = B.cs;
+  P.vproj = B.vproj;
+  P.gamma = B.gamma;
+  P.bgop = B.bgop;
+  P.bgem = B.bgem;
+  P.pops = B.pops;
+  P.lam = B.lam;
+  P.lamw = B.lamw;
+  P.fudge = B.fudge;
+  P.sigma = B.sigma;
+  P.acc = B.acc;
+  P.active = nullptr;
+  return SNAPI_OK;
+}
+
+static bool bad_inputs(const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol, int nd, const void *a,
+                       const void *b, const void *c, const void *d, const void *e) {
+  return !atom || !grid || ncol <= 0 || nd < 3 || nd > SNAPI_MAX_ND || !a || !b || !c || !d || !e;
+}
+
+}  // namespace snapi
+
+using namespace snapi;
+
+extern "C" int snapi_ali_accumulate(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol,
+                                    int nd, const double *colstate, const double *vproj, const double *gamma,
+                                    const double *bgop, const double *bgem, const double *pops, double *acc) {
+  if (bad_inputs(atom, grid, ncol, nd, colstate, vproj, gamma, bgop, bgem) || !pops || !acc) return SNAPI_ERR_ARG;
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
+  AliAtom A;
+  AliGridArgs G;
+  int rc = convert_atom(atom, A);
+  if (rc) return rc;
+  rc = convert_grid(grid, G);
+  if (rc) return rc;
+  AliBuffers B;
+  AliCol P;
+  rc = upload_inputs(atom, grid, ncol, nd, colstate, vproj, gamma, bgop, bgem, pops, B, P);
+  if (rc) goto done;
+  ACHK(hipMemset(B.acc, 0, (size_t)ncol * nd * atom->ntrans * 4 * sizeof(double)));
+  rc = launch_acc(precision, A, G, ncol, nd, P, 0);
+  if (rc) goto done;
+  ACHK(hipDeviceSynchronize());
+  ACHK(hipMemcpy(acc, B.acc, (size_t)ncol * nd * atom->ntrans * 4 * sizeof(double), hipMemcpyDeviceToHost));
+done:
+  B.release();
+  return rc;
+}
+
+extern "C" int snapi_ali_newpops(const snapi_ali_atom *atom, int ncol, int nd, const double *colstate,
+                                 const double *coll, const double *pops, const double *acc, int alo, double *newpops,
+                                 double *relchange) {
+  if (!atom || ncol <= 0 || nd < 1 || !colstate || !coll || !pops || !acc || !newpops || !relchange)
+    return SNAPI_ERR_ARG;
+  AliAtom A;
+  int rc = convert_atom(atom, A);
+  if (rc) return rc;
+  const int n = atom->nlevel, nt = atom->ntrans;
+  double *dcs = nullptr, *dcoll = nullptr, *dp = nullptr, *dacc = nullptr, *dnew = nullptr, *drc = nullptr;
+  int *dneg = nullptr;
+  std::vector<int> neg(ncol);
+  ACHK(hipMalloc((void **)&dcs, (size_t)ncol * SNAPI_A_NF * nd * 8));
+  ACHK(hipMalloc((void **)&dcoll, (size_t)ncol * nd * n * n * 8));
+  ACHK(hipMalloc((void **)&dp, (size_t)ncol * nd * n * 8));
+  ACHK(hipMalloc((void **)&dacc, (size_t)ncol * nd * nt * 4 * 8));
+  ACHK(hipMalloc((void **)&dnew, (size_t)ncol * nd * n * 8));
+  ACHK(hipMalloc((void **)&drc, (size_t)ncol * 8));
+  ACHK(hipMalloc((void **)&dneg, (size_t)ncol * sizeof(int)));
+  ACHK(hipMemcpy(dcs, colstate, (size_t)ncol * SNAPI_A_NF * nd * 8, hipMemcpyHostToDevice));
+  ACHK(hipMemcpy(dcoll, coll, (size_t)ncol * nd * n * n * 8, hipMemcpyHostToDevice));
+  ACHK(hipMemcpy(dp, pops, (size_t)ncol * nd * n * 8, hipMemcpyHostToDevice));
+  ACHK(hipMemcpy(dacc, acc, (size_t)ncol * nd * nt * 4 * 8, hipMemcpyHostToDevice));
+  rc = launch_newpops(A, ncol, nd, dcs, dcoll, dp, dacc, alo ? 1 : 0, dnew, drc, dneg, nullptr, 0);
+  if (rc) goto done;
+  ACHK(hipDeviceSynchronize());
+  ACHK(hipMemcpy(newpops, dnew, (size_t)ncol * nd * n * 8, hipMemcpyDeviceToHost));
+  ACHK(hipMemcpy(relchange, drc, (size_t)ncol * 8, hipMemcpyDeviceToHost));
+  ACHK(hipMemcpy(neg.data(), dneg, (size_t)ncol * sizeof(int), hipMemcpyDeviceToHost));
+  for (int c = 0; c < ncol; ++c)
+    if (neg[c]) relchange[c] = -1.0;
+done:
+  hipFree(dcs); hipFree(dcoll); hipFree(dp); hipFree(dacc); hipFree(dnew); hipFree(drc); hipFree(dneg);
+  return rc;
+}
+
+extern "C" int snapi_ali_solve(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol, int nd,
+                               const double *colstate, const double *vproj, const double *gamma, const double *bgop,
+                               const double *bgem, const double *coll, double tol, int max_iter, double *pops,
+                               int *iterations, double *relchange) {
+  if (bad_inputs(atom, grid, ncol, nd, colstate, vproj, gamma, bgop, bgem) || !coll || !pops || !iterations ||
+      !relchange || max_iter < 1)
+    return SNAPI_ERR_ARG;
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
+  AliAtom A;
+  AliGridArgs G;
+  int rc = convert_atom(atom, A);
+  if (rc) return rc;
+  rc = convert_grid(grid, G);
+  if (rc) return rc;
+  const int n = atom->nlevel, nt = atom->ntrans;
+  AliBuffers B;
+  AliCol P;
+  double *dcoll = nullptr, *dnew = nullptr, *drc = nullptr, *drco = nullptr;
+  int *dint = nullptr;
+  int *active = nullptr, *retry = nullptr, *neg = nullptr, *iters = nullptr, *count = nullptr;
+  rc = upload_inputs(atom, grid, ncol, nd, colstate, vproj, gamma, bgop, bgem, pops, B, P);
+  if (rc) goto done;
+  ACHK(hipMalloc((void **)&dcoll, (size_t)ncol * nd * n * n * 8));
+  ACHK(hipMemcpy(dcoll, coll, (size_t)ncol * nd * n * n * 8, hipMemcpyHostToDevice));
+  ACHK(hipMalloc((void **)&dnew, (size_t)ncol * nd * n * 8));
+  ACHK(hipMalloc((void **)&drc, (size_t)ncol * 8));
+  ACHK(hipMalloc((void **)&drco, (size_t)ncol * 8));
+  ACHK(hipMalloc((void **)&dint, ((size_t)ncol * 4 + 1) * sizeof(int)));
+  active = dint;
+  retry = dint + ncol;
+  neg = dint + 2 * ncol;
+  iters = dint + 3 * ncol;
+  count = dint + 4 * ncol;
+  {
+    std::vector<int> ones(ncol, 1);
+    ACHK(hipMemcpy(active, ones.data(), (size_t)ncol * sizeof(int), hipMemcpyHostToDevice));
+    ACHK(hipMemset(iters, 0, (size_t)ncol * sizeof(int)));
+    ACHK(hipMemset(drco, 0, (size_t)ncol * 8));
+  }
+  P.active = active;
+  for (int it = 0; it < max_iter; ++it) {
+    int nact = 0;
+    ACHK(hipMemset(count, 0, sizeof(int)));
+    hipLaunchKernelGGL(ali_count_kernel, dim3((ncol + 255) / 256), dim3(256), 0, 0, ncol, active, count);
+    ACHK(hipMemcpy(&nact, count, sizeof(int), hipMemcpyDeviceToHost));
+    if (nact == 0) break;
+    hipLaunchKernelGGL(ali_zero_kernel, dim3(1024), dim3(256), 0, 0, (size_t)nd * nt * 4, ncol, active, B.acc);
+    rc = launch_acc(precision, A, G, ncol, nd, P, 0);
+    if (rc) goto done;
+    rc = launch_newpops(A, ncol, nd, B.cs, dcoll, B.pops, B.acc, 1, dnew, drc, neg, active, 0);
+    if (rc) goto done;
+    hipLaunchKernelGGL(ali_update_kernel, dim3(ncol), dim3(64), 0, 0, ncol, nd, n, max_iter, tol, active, retry, drc,
+                       neg, dnew, B.pops, iters, drco, 0);
+    rc = launch_newpops(A, ncol, nd, B.cs, dcoll, B.pops, B.acc, 0, dnew, drc, neg, retry, 0);
+    if (rc) goto done;
+    hipLaunchKernelGGL(ali_update_kernel, dim3(ncol), dim3(256), 0, 0, ncol, nd, n, max_iter, tol, active, retry,
+                       drc, neg, dnew, B.pops, iters, drco, 1);
+    if (hipGetLastError() != hipSuccess) {
+      rc = SNAPI_ERR_DEVICE;
+      goto done;
+    }
+  }
+  ACHK(hipDeviceSynchronize());
+  ACHK(hipMemcpy(pops, B.pops, (size_t)ncol * nd * n * 8, hipMemcpyDeviceToHost));
+  ACHK(hipMemcpy(iterations, iters, (size_t)ncol * sizeof(int), hipMemcpyDeviceToHost));
+  ACHK(hipMemcpy(relchange, drco, (size_t)ncol * 8, hipMemcpyDeviceToHost));
+done:
+  B.release();
+  hipFree(dcoll); hipFree(dnew); hipFree(drc); hipFree(drco); hipFree(dint);
+  return rc;
+}

@@ -129,6 +129,40 @@ __device__ __forceinline__ void ali_opem(const AliAtom &A, const AliTile<R> &t, 
   em = e;
 }
 
+// Normalised opacity and source function of one depth.  In double exactly as
+// the reference forms them (op / op_ref, then (em / op_ref) / (op / op_ref)).
+// In float the wavelength grid's far-UV points (opacities up to 1e32 cm^-1
+// against referent opacities down to 1e-14 cm^-1) would overflow: the
+// normalised opacity saturates at 1e20, where every step is optically thick
+// and the solution equals the local source function, and S = em / op is
+// formed from the raw values.
+template <typename R> struct AliNorm;
+template <> struct AliNorm<double> {
+  static __device__ __forceinline__ void apply(bool tg, double op, double em, double opref, double &oo, double &S) {
+    oo = tg ? op / opref : op;
+    S = tg ? (em / opref) / oo : em / oo;
+  }
+  static __device__ __forceinline__ double fb(double om, double o0, double op, double h1, double h2) {
+    return fb_deriv<double>(om, o0, op, h1, h2);
+  }
+};
+template <> struct AliNorm<float> {
+  static __device__ __forceinline__ void apply(bool tg, float op, float em, float opref, float &oo, float &S) {
+    oo = fminf(tg ? op * rrcp<float>(opref) : op, 1e20f);
+    S = em * rrcp<float>(op);
+  }
+  // Fritsch-Butland derivative in harmonic form, 1 / (al / d2 + (1 - al) / d1),
+  // whose products cannot overflow for the saturated opacities
+  static __device__ __forceinline__ float fb(float om, float o0, float op, float h1, float h2) {
+    const float d2 = (op - o0) * rrcp<float>(h2), d1 = (o0 - om) * rrcp<float>(h1);
+    if ((d1 > 0.0f && d2 > 0.0f) || (d1 < 0.0f && d2 < 0.0f)) {
+      const float al = (1.0f + h1 * rrcp<float>(h1 + h2)) * (1.0f / 3.0f);
+      return rrcp<float>(al * rrcp<float>(d2) + (1.0f - al) * rrcp<float>(d1));
+    }
+    return 0.0f;
+  }
+};
+
 template <typename R>
 __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, const AliGridArgs G, int nd, AliCol P) {
   extern __shared__ unsigned char smem_raw[];
@@ -212,9 +246,9 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
       ali_opem<R>(A, t, nd, z1, lam, lamR, hcl, plc, ple, fud, sg, (R)bop[(size_t)z1 * nlam + lc],
                   (R)bem[(size_t)z1 * nlam + lc], opn, emn, profn);
       const bool tg = G.tau_grid != 0;
-      R oom = tg ? opc / t.opref[z0] : opc, oo0 = tg ? opn / t.opref[z1] : opn;
-      R Sm = tg ? (emc / t.opref[z0]) / oom : emc / oom;
-      R S0 = tg ? (emn / t.opref[z1]) / oo0 : emn / oo0;
+      R oom, oo0, Sm, S0;
+      AliNorm<R>::apply(tg, opc, emc, t.opref[z0], oom, Sm);
+      AliNorm<R>::apply(tg, opn, emn, t.opref[z1], oo0, S0);
       R h0 = (t.grid[z1] - t.grid[z0]) / mct;
       R Dm = (oo0 - oom) / h0;
       R I = down ? R(0) : Sm;
@@ -232,10 +266,9 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
             const int zn = zof(k + 1);
             ali_opem<R>(A, t, nd, zn, lam, lamR, hcl, plc, ple, fud, sg,
                         (R)bop[(size_t)zn * nlam + lc], (R)bem[(size_t)zn * nlam + lc], opp, emp, profp);
-            oop = tg ? opp / t.opref[zn] : opp;
-            Sp = tg ? (emp / t.opref[zn]) / oop : emp / oop;
+            AliNorm<R>::apply(tg, opp, emp, t.opref[zn], oop, Sp);
             hp = (t.grid[zn] - t.grid[z]) / mct;
-            D0 = fb_deriv<R>(oom, oo0, oop, h0, hp);
+            D0 = AliNorm<R>::fb(oom, oo0, oop, h0, hp);
           } else {
             D0 = (oo0 - oom) / h0;
           }
@@ -245,7 +278,7 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
           sc_weights<R>(tb, etb, w0, w1);
           I = I * etb + w0 * Sm + w1 * S0;
           Ls = w1;
-          opd = tg ? oo0 * t.opref[z] : opn;
+          opd = (tg && sizeof(R) == 8) ? oo0 * t.opref[z] : opn;
 #pragma unroll
           for (int j = 0; j < kAliMaxLines; ++j) profc[j] = profn[j];
           // shift the window: depth k+1 becomes the look-ahead of the next step
@@ -268,7 +301,8 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
         for (int j = 0; j < kAliMaxLines; ++j) {
           if (j >= nline) continue;
           const R p = profc[j];
-          const R cf = p * (t.accf[j * nd + z] * iop) * R(0.25 / kPi);
+          // line_factor = (n_l B_lu - n_u B_ul) phi E / (4 pi op); weight phi line_factor / (4 pi)
+          const R cf = p * (p * t.accf[j * nd + z] * iop) * R(0.25 / kPi);
           const double cJ = wave_sum<R>(ww * (double)(cf * I));
           const double cL = wave_sum<R>(ww * (double)(cf * Ls));
           const double cN = wave_sum<R>(ww * (double)(p * R(0.25 / kPi)));

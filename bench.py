@@ -19,6 +19,13 @@ with tangent-sweep Jacobians, against the FP64 path with central-difference
 Jacobians on 8192 of the pixels (and the mixed adjoint-sweep Jacobian on the
 same 8192): LM pixel-iterations per second of each.
 
+Third leg ("ali"): NLTE populations of a Ca II 5-level + continuum atom by
+ALI in 64 x 64 plane-parallel columns of 81 depths (chromospheric rise to
+8000 K, +/-10 % perturbations), 220 wavelengths, 5 Gauss-Radau directions per
+hemisphere, iterated to a relative change below 1e-4: formal solves per second
+inside the iteration, mixed precision, against the FP64 GPU path on 512 of the
+columns and the FP64 oracle on the host cores.
+
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -55,6 +62,8 @@ def main():
     ap.add_argument("--lm-side", type=int, default=256, help="inversion leg: side of the pixel square")
     ap.add_argument("--lm-fd-pixels", type=int, default=8192, help="pixels of the FP64 difference-Jacobian run")
     ap.add_argument("--no-lm", action="store_true")
+    ap.add_argument("--ali-side", type=int, default=64, help="ALI leg: side of the column square")
+    ap.add_argument("--no-ali", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the hot path is per column and shards by column tiles; "
@@ -161,6 +170,10 @@ def main():
     if not args.no_lm:
         lm = lm_leg(torch, dev, tmpl, lam_d, stream, args.lm_side, args.lm_fd_pixels)
 
+    alires = None
+    if not args.no_ali:
+        alires = ali_leg(torch, dev, stream, args.ali_side, not args.no_cpu)
+
     line = {"metric": METRIC, "value": value, "unit": "lambda-point formal solves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
@@ -168,8 +181,67 @@ def main():
                                    "mu=1, Fe I 6301.5/6302.5 + H lines + H- continuum, mixed FP32/FP64"
                                    % (nx, ny, nd, nlam),
                        "columns": ncol, "depths": nd, "wavelengths": nlam, "lines": table.nline},
-            "roofline": roof, "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64, "lm": lm}
+            "roofline": roof, "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64, "lm": lm, "ali": alires}
     print(json.dumps(line))
+
+
+def ali_leg(torch, dev, stream, side, with_cpu):
+    from snapi_amd import abi, ali, workloads
+    t0 = time.time()
+    p = workloads.ali_problem(side, side)
+    ncol = side * side
+    nd = p["colstate"].shape[2]
+    nlam, nray = p["nlam"], p["nray"]
+    log("ALI workload: %d columns x %d depths, %d wavelengths, %d rays, built in %.1fs"
+        % (ncol, nd, nlam, nray, time.time() - t0))
+    T = {k: torch.from_numpy(np.ascontiguousarray(p[k])).to(dev)
+         for k in ("colstate", "vproj", "gamma", "bgop", "bgem", "coll")}
+
+    def run(n, precision, tol=1e-4, max_iter=300):
+        pops = torch.from_numpy(p["pops"][:n].copy()).to(dev)
+        it = torch.zeros(n, dtype=torch.int32, device=dev)
+        rc = torch.zeros(n, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ali.solve_dev(p["atom"], p["grid"], T["colstate"][:n], T["vproj"][:n], T["gamma"][:n], T["bgop"][:n],
+                      T["bgem"][:n], T["coll"][:n], pops, it, rc, tol=tol, max_iter=max_iter, precision=precision,
+                      stream=stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t1, pops.cpu().numpy(), it.cpu().numpy(), rc.cpu().numpy()
+
+    run(16, abi.SNAPI_MIXED, max_iter=2)  # kernel loading, not timed
+    sec, pm, itm, rcm = run(ncol, abi.SNAPI_MIXED)
+    nsol = float(itm.sum()) * nray * nlam
+    mixed = {"columns": ncol, "seconds": round(sec, 3), "column_iterations": int(itm.sum()),
+             "formal_solves_per_s": nsol / sec, "column_iter_per_s": float(itm.sum()) / sec,
+             "mean_iterations": float(itm.mean()), "not_converged": int(np.sum((rcm < 0) | (rcm >= 1e-4)))}
+    log("ALI mixed: %d column-iterations in %.2fs = %.4g formal solves/s; mean iterations %.1f"
+        % (itm.sum(), sec, nsol / sec, itm.mean()))
+    n64 = min(512, ncol)
+    sec64, p64, it64, rc64 = run(n64, abi.SNAPI_FP64)
+    fp64 = {"columns": n64, "seconds": round(sec64, 3), "formal_solves_per_s": float(it64.sum()) * nray * nlam / sec64,
+            "mean_iterations": float(it64.mean())}
+    err = float(np.max(np.abs(pm[:n64] / p64 - 1.0)))
+    log("ALI fp64: %d columns in %.2fs; mixed vs fp64 populations max rel %.2e" % (n64, sec64, err))
+    cpu = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import _oracle  # the FP64 CPU restatement: baseline only
+        nthreads = max(1, min(16, os.cpu_count() or 1))
+        nc = 16
+        q = {k: np.ascontiguousarray(p[k][:nc]) for k in ("colstate", "vproj", "gamma", "bgop", "bgem", "coll")}
+        q.update(atom=p["atom"], grid=p["grid"], nd=nd, ntrans=p["atom"].ntrans, nlevel=p["atom"].nlevel)
+        t1 = time.time()
+        _, itc, _ = _oracle.ali_solve(q, p["pops"][:nc], 1e-4, 300, ncol=nc, nthreads=nthreads)
+        dt = time.time() - t1
+        cpu = {"value": float(itc.sum()) * nray * nlam / dt, "unit": "ALI formal solves/s", "cores": nthreads,
+               "kind": "port", "sample": "%d columns of the same workload to 1e-4 (FP64 oracle, OpenMP)" % nc,
+               "seconds": round(dt, 2)}
+        log("ALI cpu oracle: %.4g formal solves/s on %d threads" % (cpu["value"], nthreads))
+    return {"workload": "NLTE Ca II 5 levels + continuum by ALI, %d columns x %d depths, %d wavelengths, %d rays, "
+                        "to relative change 1e-4" % (ncol, nd, nlam, nray),
+            "value": mixed["formal_solves_per_s"], "unit": "ALI formal solves/s (column x ray x wavelength)",
+            "mixed": mixed, "fp64_gpu": fp64, "populations_rel_err_mixed_vs_fp64": err, "cpu_baseline": cpu}
 
 
 def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):

@@ -351,6 +351,14 @@ int snapi_ali_solve(int precision, const snapi_ali_atom *atom, const snapi_ali_g
                     const double *bgem, const double *coll, double tol, int max_iter, double *pops, int *iterations,
                     double *relchange);
 
+/* snapi_ali_solve on device-resident arrays (colstate, vproj, gamma, bgop,
+ * bgem, coll, pops, iterations, relchange: device pointers; the grid's
+ * arrays stay host arrays and are copied), all work on `stream`. */
+int snapi_ali_solve_dev(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol, int nd,
+                        const double *colstate_dev, const double *vproj_dev, const double *gamma_dev,
+                        const double *bgop_dev, const double *bgem_dev, const double *coll_dev, double tol,
+                        int max_iter, double *pops_dev, int *iterations_dev, double *relchange_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,3 +110,33 @@ def solve(atom, grid, colstate, vproj, gamma, bgop, bgem, coll, pops, tol=1e-2, 
                            it.ctypes.data_as(C.POINTER(C.c_int)), abi.ptr(rcv))
     check("snapi_ali_solve", rc)
     return pp, it, rcv
+
+
+def solve_dev(atom, grid, colstate_t, vproj_t, gamma_t, bgop_t, bgem_t, coll_t, pops_t, iters_t, relchange_t,
+              tol=1e-4, max_iter=200, precision=abi.SNAPI_MIXED, stream=None):
+    """snapi_ali_solve_dev on torch float64 device tensors (int32 for the
+    iteration counts); the grid's arrays are host arrays."""
+    import torch
+    L = lib()
+    _declare(L)
+    f = L.snapi_ali_solve_dev
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.POINTER(abi.SnapiAliAtom), C.POINTER(abi.SnapiAliGrid), C.c_int, C.c_int] + \
+        [C.c_void_p] * 6 + [C.c_double, C.c_int] + [C.c_void_p] * 4
+    ncol, nf, nd = colstate_t.shape
+    nb = grid.nray if grid.bg_per_ray else 1
+    want = [(colstate_t, torch.float64, (ncol, abi.A_NF, nd)), (vproj_t, torch.float64, (ncol, grid.nray, nd)),
+            (gamma_t, torch.float64, (ncol, atom.ntrans, nd)), (bgop_t, torch.float64, (ncol, nb, nd, grid.nlam)),
+            (bgem_t, torch.float64, (ncol, nb, nd, grid.nlam)),
+            (coll_t, torch.float64, (ncol, nd, atom.nlevel, atom.nlevel)),
+            (pops_t, torch.float64, (ncol, nd, atom.nlevel)), (iters_t, torch.int32, (ncol,)),
+            (relchange_t, torch.float64, (ncol,))]
+    for t, d, shp in want:
+        if t.dtype != d or tuple(t.shape) != shp or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("solve_dev: expected a contiguous device tensor %s of shape %s, got %s %s"
+                             % (d, shp, t.dtype, tuple(t.shape)))
+    sp = None if stream is None else C.c_void_p(stream.cuda_stream)
+    rc = f(precision, C.byref(atom), C.byref(grid), ncol, nd, colstate_t.data_ptr(), vproj_t.data_ptr(),
+           gamma_t.data_ptr(), bgop_t.data_ptr(), bgem_t.data_ptr(), coll_t.data_ptr(), tol, max_iter,
+           pops_t.data_ptr(), iters_t.data_ptr(), relchange_t.data_ptr(), sp)
+    check("snapi_ali_solve_dev", rc)

@@ -468,11 +468,11 @@ __global__ void ali_update_kernel(int ncol, int nd, int n, int max_iter, double 
                                   const double *__restrict__ newp, double *__restrict__ pops, int *__restrict__ iters,
                                   double *__restrict__ rc_out, int stage) {
   const int col = blockIdx.x;
-  if (!active[col]) return;
   if (stage == 0) {  // after the approximate-operator solve: columns to redo with Lambda iteration
-    if (threadIdx.x == 0) retry[col] = neg[col];
+    if (threadIdx.x == 0) retry[col] = active[col] ? neg[col] : 0;
     return;
   }
+  if (!active[col]) return;
   __syncthreads();
   const bool bad = neg[col] != 0;
   if (!bad)
@@ -740,6 +740,62 @@ done:
   return rc;
 }
 
+// The iteration on device-resident inputs (P: heavy arrays and the grid's
+// arrays on the device, P.acc a scratch of ncol * nd * ntrans * 4 doubles).
+static int ali_solve_core(int precision, const AliAtom &A, const AliGridArgs &G, int ncol, int nd, AliCol P,
+                          const double *dcoll, double *dpops, int *diters, double *drco, double tol, int max_iter,
+                          hipStream_t st) {
+  const int n = A.nlevel, nt = A.ntrans;
+  int rc = SNAPI_OK;
+  double *dnew = nullptr, *drc = nullptr;
+  int *dint = nullptr;
+  int *active, *retry, *neg, *count;
+  ACHK(hipMalloc((void **)&dnew, (size_t)ncol * nd * n * 8));
+  ACHK(hipMalloc((void **)&drc, (size_t)ncol * 8));
+  ACHK(hipMalloc((void **)&dint, ((size_t)ncol * 3 + 1) * sizeof(int)));
+  active = dint;
+  retry = dint + ncol;
+  neg = dint + 2 * ncol;
+  count = dint + 3 * ncol;
+  {
+    std::vector<int> ones(ncol, 1);
+    ACHK(hipMemcpyAsync(active, ones.data(), (size_t)ncol * sizeof(int), hipMemcpyHostToDevice, st));
+    ACHK(hipMemsetAsync(retry, 0, (size_t)ncol * sizeof(int), st));
+    ACHK(hipMemsetAsync(diters, 0, (size_t)ncol * sizeof(int), st));
+    ACHK(hipMemsetAsync(drco, 0, (size_t)ncol * 8, st));
+    ACHK(hipStreamSynchronize(st));
+  }
+  P.active = active;
+  P.pops = dpops;
+  for (int it = 0; it < max_iter; ++it) {
+    int nact = 0;
+    ACHK(hipMemsetAsync(count, 0, sizeof(int), st));
+    hipLaunchKernelGGL(ali_count_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, ncol, active, count);
+    ACHK(hipMemcpyAsync(&nact, count, sizeof(int), hipMemcpyDeviceToHost, st));
+    ACHK(hipStreamSynchronize(st));
+    if (nact == 0) break;
+    hipLaunchKernelGGL(ali_zero_kernel, dim3(1024), dim3(256), 0, st, (size_t)nd * nt * 4, ncol, active, P.acc);
+    rc = launch_acc(precision, A, G, ncol, nd, P, st);
+    if (rc) goto done;
+    rc = launch_newpops(A, ncol, nd, P.cs, dcoll, dpops, P.acc, 1, dnew, drc, neg, active, st);
+    if (rc) goto done;
+    hipLaunchKernelGGL(ali_update_kernel, dim3(ncol), dim3(64), 0, st, ncol, nd, n, max_iter, tol, active, retry, drc,
+                       neg, dnew, dpops, diters, drco, 0);
+    rc = launch_newpops(A, ncol, nd, P.cs, dcoll, dpops, P.acc, 0, dnew, drc, neg, retry, st);
+    if (rc) goto done;
+    hipLaunchKernelGGL(ali_update_kernel, dim3(ncol), dim3(256), 0, st, ncol, nd, n, max_iter, tol, active, retry,
+                       drc, neg, dnew, dpops, diters, drco, 1);
+    if (hipGetLastError() != hipSuccess) {
+      rc = SNAPI_ERR_DEVICE;
+      goto done;
+    }
+  }
+  ACHK(hipStreamSynchronize(st));
+done:
+  hipFree(dnew); hipFree(drc); hipFree(dint);
+  return rc;
+}
+
 extern "C" int snapi_ali_solve(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol, int nd,
                                const double *colstate, const double *vproj, const double *gamma, const double *bgop,
                                const double *bgem, const double *coll, double tol, int max_iter, double *pops,
@@ -754,60 +810,69 @@ extern "C" int snapi_ali_solve(int precision, const snapi_ali_atom *atom, const 
   if (rc) return rc;
   rc = convert_grid(grid, G);
   if (rc) return rc;
-  const int n = atom->nlevel, nt = atom->ntrans;
+  const int n = atom->nlevel;
   AliBuffers B;
   AliCol P;
-  double *dcoll = nullptr, *dnew = nullptr, *drc = nullptr, *drco = nullptr;
-  int *dint = nullptr;
-  int *active = nullptr, *retry = nullptr, *neg = nullptr, *iters = nullptr, *count = nullptr;
+  double *dcoll = nullptr, *drco = nullptr;
+  int *diters = nullptr;
   rc = upload_inputs(atom, grid, ncol, nd, colstate, vproj, gamma, bgop, bgem, pops, B, P);
   if (rc) goto done;
   ACHK(hipMalloc((void **)&dcoll, (size_t)ncol * nd * n * n * 8));
   ACHK(hipMemcpy(dcoll, coll, (size_t)ncol * nd * n * n * 8, hipMemcpyHostToDevice));
-  ACHK(hipMalloc((void **)&dnew, (size_t)ncol * nd * n * 8));
-  ACHK(hipMalloc((void **)&drc, (size_t)ncol * 8));
   ACHK(hipMalloc((void **)&drco, (size_t)ncol * 8));
-  ACHK(hipMalloc((void **)&dint, ((size_t)ncol * 4 + 1) * sizeof(int)));
-  active = dint;
-  retry = dint + ncol;
-  neg = dint + 2 * ncol;
-  iters = dint + 3 * ncol;
-  count = dint + 4 * ncol;
-  {
-    std::vector<int> ones(ncol, 1);
-    ACHK(hipMemcpy(active, ones.data(), (size_t)ncol * sizeof(int), hipMemcpyHostToDevice));
-    ACHK(hipMemset(iters, 0, (size_t)ncol * sizeof(int)));
-    ACHK(hipMemset(drco, 0, (size_t)ncol * 8));
-  }
-  P.active = active;
-  for (int it = 0; it < max_iter; ++it) {
-    int nact = 0;
-    ACHK(hipMemset(count, 0, sizeof(int)));
-    hipLaunchKernelGGL(ali_count_kernel, dim3((ncol + 255) / 256), dim3(256), 0, 0, ncol, active, count);
-    ACHK(hipMemcpy(&nact, count, sizeof(int), hipMemcpyDeviceToHost));
-    if (nact == 0) break;
-    hipLaunchKernelGGL(ali_zero_kernel, dim3(1024), dim3(256), 0, 0, (size_t)nd * nt * 4, ncol, active, B.acc);
-    rc = launch_acc(precision, A, G, ncol, nd, P, 0);
-    if (rc) goto done;
-    rc = launch_newpops(A, ncol, nd, B.cs, dcoll, B.pops, B.acc, 1, dnew, drc, neg, active, 0);
-    if (rc) goto done;
-    hipLaunchKernelGGL(ali_update_kernel, dim3(ncol), dim3(64), 0, 0, ncol, nd, n, max_iter, tol, active, retry, drc,
-                       neg, dnew, B.pops, iters, drco, 0);
-    rc = launch_newpops(A, ncol, nd, B.cs, dcoll, B.pops, B.acc, 0, dnew, drc, neg, retry, 0);
-    if (rc) goto done;
-    hipLaunchKernelGGL(ali_update_kernel, dim3(ncol), dim3(256), 0, 0, ncol, nd, n, max_iter, tol, active, retry,
-                       drc, neg, dnew, B.pops, iters, drco, 1);
-    if (hipGetLastError() != hipSuccess) {
-      rc = SNAPI_ERR_DEVICE;
-      goto done;
-    }
-  }
-  ACHK(hipDeviceSynchronize());
+  ACHK(hipMalloc((void **)&diters, (size_t)ncol * sizeof(int)));
+  rc = ali_solve_core(precision, A, G, ncol, nd, P, dcoll, B.pops, diters, drco, tol, max_iter, 0);
+  if (rc) goto done;
   ACHK(hipMemcpy(pops, B.pops, (size_t)ncol * nd * n * 8, hipMemcpyDeviceToHost));
-  ACHK(hipMemcpy(iterations, iters, (size_t)ncol * sizeof(int), hipMemcpyDeviceToHost));
+  ACHK(hipMemcpy(iterations, diters, (size_t)ncol * sizeof(int), hipMemcpyDeviceToHost));
   ACHK(hipMemcpy(relchange, drco, (size_t)ncol * 8, hipMemcpyDeviceToHost));
 done:
   B.release();
-  hipFree(dcoll); hipFree(dnew); hipFree(drc); hipFree(drco); hipFree(dint);
+  hipFree(dcoll); hipFree(drco); hipFree(diters);
+  return rc;
+}
+
+extern "C" int snapi_ali_solve_dev(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol,
+                                   int nd, const double *colstate_dev, const double *vproj_dev,
+                                   const double *gamma_dev, const double *bgop_dev, const double *bgem_dev,
+                                   const double *coll_dev, double tol, int max_iter, double *pops_dev,
+                                   int *iterations_dev, double *relchange_dev, void *stream) {
+  if (bad_inputs(atom, grid, ncol, nd, colstate_dev, vproj_dev, gamma_dev, bgop_dev, bgem_dev) || !coll_dev ||
+      !pops_dev || !iterations_dev || !relchange_dev || max_iter < 1)
+    return SNAPI_ERR_ARG;
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
+  AliAtom A;
+  AliGridArgs G;
+  int rc = convert_atom(atom, A);
+  if (rc) return rc;
+  rc = convert_grid(grid, G);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const size_t nlam = grid->nlam, nt = atom->ntrans;
+  double *g = nullptr, *acc = nullptr;
+  AliCol P;
+  ACHK(hipMalloc((void **)&g, (3 * nlam + nt * nlam) * 8));
+  ACHK(hipMalloc((void **)&acc, (size_t)ncol * nd * nt * 4 * 8));
+  ACHK(hipMemcpyAsync(g, grid->lam, nlam * 8, hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(g + nlam, grid->lamw, nlam * 8, hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(g + 2 * nlam, grid->fudge, nlam * 8, hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(g + 3 * nlam, grid->sigma, nt * nlam * 8, hipMemcpyHostToDevice, st));
+  P.cs = colstate_dev;
+  P.vproj = vproj_dev;
+  P.gamma = gamma_dev;
+  P.bgop = bgop_dev;
+  P.bgem = bgem_dev;
+  P.pops = pops_dev;
+  P.lam = g;
+  P.lamw = g + nlam;
+  P.fudge = g + 2 * nlam;
+  P.sigma = g + 3 * nlam;
+  P.acc = acc;
+  P.active = nullptr;
+  rc = ali_solve_core(precision, A, G, ncol, nd, P, coll_dev, pops_dev, iterations_dev, relchange_dev, tol, max_iter,
+                      st);
+done:
+  hipFree(g);
+  hipFree(acc);
   return rc;
 }

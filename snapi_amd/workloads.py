@@ -186,6 +186,116 @@ def inversion_problem(template, nx, ny, seed=3):
     return dict(state=st, truth=truth, start=start, mask=mask, nodes=INVERSION_NODES)
 
 
+def ali_template():
+    """The NLTE column recorded from the reference (tests/golden/nlte.npz):
+    a Ca II 5-level + continuum atom (the reference's calcium_ir_rh.cfg) in
+    the 61-depth photosphere, with every ALI input as the reference computes
+    it (frequency grid, rays, background, damping, collision rates, LTE start
+    populations)."""
+    d = np.load(os.path.join(GOLDEN, "nlte.npz"))
+    trans = []
+    for row in d["trans"]:
+        kind, lo, up = int(row[0]), int(row[1]), int(row[2])
+        t = dict(kind=kind, lower=lo, upper=up)
+        if kind == abi.TR_LINE:
+            t.update(lam0=row[3], aul=row[4], blu=row[5], bul=row[6], energy=row[7])
+        else:
+            t.update(gratio=row[8], de=row[9])
+        trans.append(t)
+    dep = d["depth"]
+    cs = np.zeros((abi.A_NF, int(d["nd"])))
+    cs[abi.A_T], cs[abi.A_VT], cs[abi.A_NE] = dep[:, 0], dep[:, 1], dep[:, 2]
+    cs[abi.A_OPREF], cs[abi.A_GRID], cs[abi.A_NA] = dep[:, 3], dep[:, 4], dep[:, 5]
+    return dict(nlevel=int(d["nmap"]), mass=float(d["mass"]), trans=trans, lam=d["lam"], rays=d["rays"],
+                sigma=d["sigma"], colstate=cs, vproj=d["vproj"], gamma=d["gamma"], bgop=d["bgop"],
+                bgem=d["bgem"], coll=d["coll"], pop0=d["pop0"], boundary=float(d["boundary"]))
+
+
+def gauss_radau_mu(n):
+    """n-point Gauss-Radau nodes and weights on mu in (0, 1], mu = 1 included."""
+    from numpy.polynomial import legendre as leg
+    # interior nodes: roots of (P_{n-1}(x) + P_n(x)) / (1 + x) on [-1, 1], x = -1 fixed
+    c = np.zeros(n + 1)
+    c[n - 1] = 1.0
+    c[n] = 1.0
+    r = np.sort(np.real(leg.legroots(c)))
+    x = np.concatenate([[-1.0], r[np.abs(r + 1.0) > 1e-9]])[:n]
+    pn1 = leg.legval(x, np.eye(n)[n - 1])
+    w = (1.0 - x) / (n * n * pn1 * pn1)
+    w[0] = 2.0 / (n * n)
+    return (1.0 - x) / 2.0, w / 2.0  # mapped to mu in (0, 1], weights summing to 1
+
+
+def ali_problem(nx, ny, nd=81, nlam=220, nmu=5, seed=9):
+    """The ALI workload: nx * ny plane-parallel columns of nd depths drawn
+    around the reference's NLTE column, resampled in log tau, with a
+    chromospheric temperature rise towards 8000 K above log tau = -4 and
+    seeded +/-10 % perturbations of temperature, electron density and
+    background opacity; nlam wavelengths taken from the reference's NLTE grid;
+    nmu Gauss-Radau directions in each hemisphere (2 nmu rays).  The
+    emissivity follows the temperature through the Planck function; collision
+    rates and start populations are the template's at each depth.  Returns
+    dict(atom, grid, colstate, vproj, gamma, bgop, bgem, coll, pops) with one
+    background set for all rays."""
+    t = ali_template()
+    rng = np.random.default_rng(seed)
+    nd0 = t["colstate"].shape[1]
+    x0 = np.log10(-t["colstate"][abi.A_GRID])
+    x = np.linspace(x0[0], x0[-1], nd)
+
+    def rs(a, log=True):  # resample the last axis from the template depths
+        a = np.asarray(a, dtype=np.float64)
+        flat = a.reshape(-1, nd0)
+        out = np.empty((flat.shape[0], nd))
+        for i, row in enumerate(flat):
+            if log and np.all(row > 0):
+                out[i] = 10.0 ** np.interp(x, x0, np.log10(row))
+            else:
+                out[i] = np.interp(x, x0, row)
+        return out.reshape(a.shape[:-1] + (nd,))
+    li = np.unique(np.round(np.linspace(0, len(t["lam"]) - 1, nlam)).astype(int))
+    lam = t["lam"][li, 0]
+    lamw = np.empty_like(lam)
+    lamw[0] = (lam[1] - lam[0]) * 0.5
+    lamw[-1] = (lam[-1] - lam[-2]) * 0.5
+    lamw[1:-1] = (lam[2:] - lam[:-2]) * 0.5
+    mu, wmu = gauss_radau_mu(nmu)
+    theta = np.concatenate([np.arccos(mu), np.pi - np.arccos(mu)])
+    wray = np.concatenate([wmu, wmu]) * 2.0 * np.pi
+    ncol = nx * ny
+    cs0 = rs(t["colstate"])
+    cs0[abi.A_GRID] = -10.0 ** x
+    T0 = cs0[abi.A_T]
+    rise = np.clip((-4.0 - x) / 2.0, 0.0, 1.0)  # 0 below log tau = -4, 1 above -6
+    Tch = T0 + rise * np.maximum(8000.0 - T0, 0.0)
+    cs = np.repeat(cs0[None], ncol, 0)
+    fT = rng.uniform(0.9, 1.1, (ncol, 1))
+    cs[:, abi.A_T] = Tch[None] * fT
+    cs[:, abi.A_NE] *= rng.uniform(0.9, 1.1, (ncol, 1))
+    bg0 = rs(t["bgop"][0].T).T[:, li]  # [nd][nlam] (the template's rays share the background at rest)
+    be0 = rs(np.maximum(t["bgem"][0].T, 1e-300)).T[:, li]
+    bgop = np.repeat(bg0[None, None], ncol, 0) * rng.uniform(0.9, 1.1, (ncol, 1, 1, 1))
+    # emissivity = opacity x Planck at the column's temperature, scaled from the template's
+    # B(lam, T) / B(lam, T0) = (e^a0 - 1) / (e^a - 1), a = h c / (lam k T), in a form
+    # that neither overflows nor divides 0 by 0 in the far UV
+    a = 1.4387769 / lam[None, None, :] / cs[:, abi.A_T][:, :, None]
+    a0 = 1.4387769 / lam[None, None, :] / cs0[abi.A_T][None, :, None]
+    ratio = np.exp(a0 - a) * (-np.expm1(-a0)) / (-np.expm1(-a))
+    bgem = np.ascontiguousarray(np.repeat(be0[None, None], ncol, 0) * ratio[:, None] * (bgop / bg0[None, None]))
+    trans = t["trans"]
+    atom = abi.make_ali_atom(t["nlevel"], t["mass"], trans)
+    sig = t["sigma"][:, li]
+    grid = abi.make_ali_grid(lam, lamw, np.ones_like(lam), sig, theta, wray, 0, t["boundary"], 1)
+    vproj = np.zeros((ncol, len(theta), nd))
+    gamma = np.repeat(rs(t["gamma"])[None], ncol, 0)
+    coll = np.repeat(rs(np.moveaxis(t["coll"], 0, -1), log=False)[None], ncol, 0)
+    coll = np.ascontiguousarray(np.moveaxis(coll, -1, 1))
+    pops = np.ascontiguousarray(np.repeat(rs(t["pop0"].T).T[None], ncol, 0))
+    return dict(atom=atom, grid=grid, colstate=np.ascontiguousarray(cs), vproj=vproj,
+                gamma=np.ascontiguousarray(gamma), bgop=np.ascontiguousarray(bgop), bgem=bgem, coll=coll,
+                pops=pops, nlam=len(lam), nray=len(theta))
+
+
 def response_column_from_dump(path):
     """Column state as the reference's response path sees it, plus the
     thermodynamic derivatives dstate[2][nfield][nd] (d/dT, d/dNt of Ne,

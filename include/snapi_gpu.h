@@ -24,7 +24,12 @@
  *                            (multiply_with_transpose, solve_svd    cmlib/mathtools.cc:615-624, 344-387)
  *   snapi_formal_scalar      atmos_ppbez::formal (scalar, with Lambda*)
  *                                                                   atmos/plpar/atmos_ppbez_rts.cc:21-219
+ *   snapi_node_jacobian      atmosphere::obs_stokes_responses_to_nodes
+ *                                                                   atmos/atmos_fit.cc:1205-1352
+ *   snapi_invert*            atmosphere::stokes_lm_fit              atmos/atmos_fit.cc:18-227
  *   snapi_ali_accumulate     atom::add (Jbar, Lambda*, norm sums)   atmos/atmol/atom/atom.cc:1865-1927
+ *   snapi_ali_newpops        atom::newpops                          atmos/atmol/atom/atom_pop.cc:27-135
+ *   snapi_ali_solve*         atmosphere::nltepops (one NLTE atom)   atmos/atmos_pop.cc:67-240
  */
 #ifndef SNAPI_GPU_H
 #define SNAPI_GPU_H

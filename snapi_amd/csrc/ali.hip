@@ -34,6 +34,7 @@ namespace snapi {
 constexpr int kAliThreads = 256;
 constexpr int kAliMaxLines = 16;  // lines whose profiles a work-item carries between depths
 constexpr int kAliMaxBf = 16;
+constexpr int kAliSlots = 64;  // reduced sums per depth: 3 per line + 2 per bound-free <= 64
 constexpr double kSaha = 2.0706615382E-16;  // cmlib/const.h:24
 
 // Atom constants as kernel arguments.
@@ -86,13 +87,6 @@ template <typename R> struct AliTile {
     return t;
   }
 };
-
-template <typename R>
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // opacity, emissivity and line profiles of one depth for one wavelength
 template <typename R>
@@ -169,10 +163,9 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
   const int col = blockIdx.x;
   if (P.active && !P.active[col]) return;  // uniform over the workgroup
   const int nline = A.nline, nbf = A.nbf, nlam = G.nlam;
-  const int nq = 3 * nline + 2 * nbf;
   const int nwave = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x & 63;
-  double *red = reinterpret_cast<double *>(smem_raw);  // [2][nwave][nq]
-  R *sm = reinterpret_cast<R *>(smem_raw + sizeof(double) * 2 * nwave * nq);
+  double *red = reinterpret_cast<double *>(smem_raw);  // [2][nwave][kAliSlots]
+  R *sm = reinterpret_cast<R *>(smem_raw + sizeof(double) * 2 * nwave * kAliSlots);
   AliTile<R> t = AliTile<R>::carve(sm, nd, nline, nbf);
   const double *cs = P.cs + (size_t)col * SNAPI_A_NF * nd;
   const double *pops = P.pops + (size_t)col * nd * A.nlevel;
@@ -292,26 +285,25 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
 #pragma unroll
           for (int j = 0; j < kAliMaxLines; ++j) profn[j] = profp[j];
         }
-        // contributions of this wavelength to the sums of depth z
+        // contributions of this wavelength to the sums of depth z, in fixed
+        // slots: line j at 3j (J), 3j+1 (Lambda*), 3j+2 (norm); bound-free b
+        // at 62-2b (photoionisation) and 63-2b (recombination)
         const int buf = k & 1;
-        double *rb = red + (size_t)buf * nwave * nq;
+        double *rb = red + (size_t)buf * nwave * kAliSlots;
         const R iop = rrcp<R>(opd);
-        int q = 0;
+        R v[kAliSlots];
+#pragma unroll
+        for (int q = 0; q < kAliSlots; ++q) v[q] = R(0);
+        const R wwr = (R)ww;
 #pragma unroll
         for (int j = 0; j < kAliMaxLines; ++j) {
           if (j >= nline) continue;
           const R p = profc[j];
           // line_factor = (n_l B_lu - n_u B_ul) phi E / (4 pi op); weight phi line_factor / (4 pi)
-          const R cf = p * (p * t.accf[j * nd + z] * iop) * R(0.25 / kPi);
-          const double cJ = wave_sum<R>(ww * (double)(cf * I));
-          const double cL = wave_sum<R>(ww * (double)(cf * Ls));
-          const double cN = wave_sum<R>(ww * (double)(p * R(0.25 / kPi)));
-          if (lane == 0) {
-            rb[wave * nq + q] = cJ;
-            rb[wave * nq + q + 1] = cL;
-            rb[wave * nq + q + 2] = cN;
-          }
-          q += 3;
+          const R cf = wwr * p * (p * t.accf[j * nd + z] * iop) * R(0.25 / kPi);
+          v[3 * j] = cf * I;
+          v[3 * j + 1] = cf * Ls;
+          v[3 * j + 2] = wwr * p * R(0.25 / kPi);
         }
         const double exd = nbf ? exp(-hclk * (double)t.tinv[z]) : 0.0;
 #pragma unroll
@@ -319,26 +311,37 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
           if (b >= nbf) continue;
           const double s = ww * sgd[b] * lam / kH / kC;
           const double Id = (double)I;
-          const double cm = wave_sum<R>(s * Id);
-          const double cn = wave_sum<R>(s * (Id + c5) * exd);
-          if (lane == 0) {
-            rb[wave * nq + q] = cm;
-            rb[wave * nq + q + 1] = cn;
-          }
-          q += 2;
+          v[kAliSlots - 2 - 2 * b] = (R)(s * Id);
+          v[kAliSlots - 1 - 2 * b] = (R)(s * (Id + c5) * exd);
         }
+        // butterfly reduction over the wavefront: at each step a lane keeps
+        // half of its slots and adds its partner's copy of them; after six
+        // steps lane L holds the wavefront's sum of slot L
+#pragma unroll
+        for (int o = 32, half = kAliSlots / 2; o >= 1; o >>= 1, half >>= 1) {
+          const bool upper = (lane & o) != 0;
+#pragma unroll
+          for (int q = 0; q < half; ++q) {
+            const R lo = v[q], hi = v[q + half];
+            const R recv = __shfl_xor(upper ? lo : hi, o, 64);
+            v[q] = (upper ? hi : lo) + recv;
+          }
+        }
+        rb[wave * kAliSlots + lane] = (double)v[0];
         __syncthreads();
-        for (int qq = threadIdx.x; qq < nq; qq += blockDim.x) {
-          double s = 0.0;
-          for (int w = 0; w < nwave; ++w) s += rb[w * nq + qq];
-          int tr, slot;
+        for (int qq = threadIdx.x; qq < kAliSlots; qq += blockDim.x) {
+          int tr = -1, slot = 0;
           if (qq < 3 * nline) {
             tr = A.line_t[qq / 3];
             slot = qq % 3;
-          } else {
-            tr = A.bf_t[(qq - 3 * nline) / 2];
-            slot = (qq - 3 * nline) % 2;
+          } else if (qq >= kAliSlots - 2 * nbf) {
+            const int r2 = kAliSlots - 1 - qq;
+            tr = A.bf_t[r2 / 2];
+            slot = (r2 & 1) ? 0 : 1;
           }
+          if (tr < 0) continue;
+          double s = 0.0;
+          for (int w = 0; w < nwave; ++w) s += rb[w * kAliSlots + qq];
           double *ap = acc + ((size_t)z * A.ntrans + tr) * 4;
           if (qq < 3 * nline) {
             if (slot == 0) {
@@ -577,16 +580,15 @@ static int convert_grid(const snapi_ali_grid *g, AliGridArgs &G) {
 }
 
 static size_t acc_smem(int precision, const AliAtom &A, int nd, int threads) {
-  const int nq = 3 * A.nline + 2 * A.nbf;
   const size_t es = precision == SNAPI_FP64 ? 8 : 4;
-  return sizeof(double) * 2 * (threads / 64) * nq + es * (size_t)nd * (4 + 5 * A.nline + 2 * A.nbf);
+  return sizeof(double) * 2 * (threads / 64) * kAliSlots + es * (size_t)nd * (4 + 5 * A.nline + 2 * A.nbf);
 }
 
 static int launch_acc(int precision, const AliAtom &A, const AliGridArgs &G, int ncol, int nd, const AliCol &P,
                       hipStream_t st) {
   const int threads = kAliThreads;
   const size_t smem = acc_smem(precision, A, nd, threads);
-  if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;
+  if (smem > 65536 || 3 * A.nline + 2 * A.nbf > kAliSlots) return SNAPI_ERR_UNSUPPORTED;
   if (precision == SNAPI_FP64)
     hipLaunchKernelGGL(ali_acc_kernel<double>, dim3(ncol), dim3(threads), smem, st, A, G, nd, P);
   else

@@ -30,6 +30,9 @@
  *   snapi_ali_accumulate     atom::add (Jbar, Lambda*, norm sums)   atmos/atmol/atom/atom.cc:1865-1927
  *   snapi_ali_newpops        atom::newpops                          atmos/atmol/atom/atom_pop.cc:27-135
  *   snapi_ali_solve*         atmosphere::nltepops (one NLTE atom)   atmos/atmos_pop.cc:67-240
+ *   snapi_chemeq*            atmosphere::ltepops: chemeq + atom::lte atmos/atmos_pop.cc:30-45,
+ *                                                                   atmos/atmos_chm.cc:28-363,
+ *                                                                   atmos/atmol/atom/atom.cc:779-787, 892-924
  */
 #ifndef SNAPI_GPU_H
 #define SNAPI_GPU_H
@@ -363,6 +366,103 @@ int snapi_ali_solve_dev(int precision, const snapi_ali_atom *atom, const snapi_a
                         const double *colstate_dev, const double *vproj_dev, const double *gamma_dev,
                         const double *bgop_dev, const double *bgem_dev, const double *coll_dev, double tol,
                         int max_iter, double *pops_dev, int *iterations_dev, double *relchange_dev, void *stream);
+
+/*
+ * Chemical equilibrium and LTE populations, per depth point, in double.
+ *
+ * The species table mirrors what the reference's atom and molecule objects
+ * hold for atmosphere::chemeq: per element the number abundance, the
+ * ionisation energies and one partition function per ionisation stage
+ * (pf/cpf/trav/irwin, atmos/atmol/atom/partf.cc:14-358); per molecule its
+ * components and its dissociation-constant form (eqc_cnst/eqc_poly/eqc_hmin,
+ * atmos/atmol/mol/moleqc.cc); and the levels whose Boltzmann populations
+ * (atom::lte, atmos/atmol/atom/atom.cc:779-787) are wanted.
+ */
+#define SNAPI_CHEM_MAX_ELEMENTS 14
+#define SNAPI_CHEM_MAX_STAGES 10
+#define SNAPI_CHEM_MAX_TERMS 6
+#define SNAPI_CHEM_MAX_COEFS 8
+#define SNAPI_CHEM_MAX_MOLECULES 32
+#define SNAPI_CHEM_MAX_COMPONENTS 4
+#define SNAPI_CHEM_MAX_COMP_STAGE 2 /* component stages 0 and 1 (after clamping at 0) */
+#define SNAPI_CHEM_MAX_LEVELS 256
+
+enum snapi_pf_type { SNAPI_PF_NONE = 0, SNAPI_PF_CONST = 1, SNAPI_PF_TRAV = 2, SNAPI_PF_IRWIN = 3 };
+enum snapi_eqc_type { SNAPI_EQC_NONE = 0, SNAPI_EQC_CONST = 1, SNAPI_EQC_POLY = 2, SNAPI_EQC_HMIN = 3 };
+
+typedef struct {
+  int type;   /* snapi_pf_type; NONE: U = 1 */
+  int nterm;  /* TRAV: asymptotic terms; IRWIN: polynomial coefficients */
+  double value;  /* CONST: U */
+  double charge; /* TRAV: effective charge seen by the outer electron (stage charge + 1) */
+  double g0;     /* TRAV: statistical weight of the ground term */
+  double g2[SNAPI_CHEM_MAX_TERMS];   /* TRAV per term: weight of its asymptotic sum */
+  double xion[SNAPI_CHEM_MAX_TERMS]; /* series limit [eV] */
+  double xl[SNAPI_CHEM_MAX_TERMS];   /* first principal quantum number of the asymptotic sum */
+  int ncoef[SNAPI_CHEM_MAX_TERMS];   /* explicit levels of the term */
+  double gamma[SNAPI_CHEM_MAX_TERMS][SNAPI_CHEM_MAX_COEFS]; /* their excitation energies [eV] */
+  double alpha[SNAPI_CHEM_MAX_TERMS][SNAPI_CHEM_MAX_COEFS]; /* their statistical weights */
+  double irwin[SNAPI_CHEM_MAX_COEFS]; /* IRWIN: ln U = sum_i irwin[i] (ln T)^i */
+} snapi_partition;
+
+typedef struct {
+  int nstage;       /* ionisation stages (the reference's Z + 1) */
+  double abundance; /* number abundance, any common scale */
+  double eion[SNAPI_CHEM_MAX_STAGES]; /* ionisation energy of stage z -> z+1 [erg], z < nstage-1 */
+  snapi_partition pf[SNAPI_CHEM_MAX_STAGES];
+} snapi_chem_element;
+
+typedef struct {
+  int ncomp;
+  int element[SNAPI_CHEM_MAX_COMPONENTS]; /* 0: electron; 1..nelement: element in table order */
+  int stage[SNAPI_CHEM_MAX_COMPONENTS];   /* charge index as the reference stores it (0 neutral,
+                                             1 singly ionised, -1 the electron); the molecule's
+                                             charge is their sum */
+  int eqc;   /* snapi_eqc_type */
+  int ncoef; /* POLY: log10 K = sum_i coef[i] (5040/T)^i; CONST: K = coef[0] */
+  double coef[SNAPI_CHEM_MAX_COEFS];
+} snapi_chem_molecule;
+
+typedef struct {
+  int element, stage; /* element 1..nelement */
+  double g;           /* statistical weight; 0 marks a placeholder whose population is the stage's */
+  double energy;      /* excitation energy [erg] */
+} snapi_chem_level;
+
+typedef struct {
+  int nelement, nmolecule, nlevel;
+  snapi_chem_element element[SNAPI_CHEM_MAX_ELEMENTS];
+  snapi_chem_molecule molecule[SNAPI_CHEM_MAX_MOLECULES];
+  snapi_chem_level level[SNAPI_CHEM_MAX_LEVELS];
+} snapi_chem_table;
+
+/*
+ * Chemical equilibrium (atmosphere::chemeq, atmos/atmos_chm.cc:28-363) at
+ * every depth point of ncol columns, then LTE level populations (atom::lte,
+ * atom.cc:779-787), as atmosphere::ltepops (atmos/atmos_pop.cc:30-45) runs
+ * them.  Newton iteration on (Ne, N_1..N_nelement) with ionisation fractions
+ * from atom::ionfrc (atom.cc:892-924) and molecules in dissociation
+ * equilibrium; at most the reference's 51 Newton steps; converged when every
+ * |defect/N| <= 1e-4.  The reference's Ne input only enters its dissociation
+ * constants, none of whose forms depend on it, so there is none here.
+ *   T, Nt [ncol][nd] in (K, cm^-3);
+ *   ne [ncol][nd] out;
+ *   stage [ncol][nstage_total][nd] out (may be NULL): N of every ionisation
+ *     stage, elements in table order, stages ascending;
+ *   molecule [ncol][nmolecule][nd] out (may be NULL);
+ *   level [ncol][nlevel][nd] out (may be NULL);
+ *   iterations [ncol][nd] out (may be NULL): Newton steps taken, -1 where the
+ *     linear system was singular (the reference exits there).
+ */
+int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, const double *T, const double *Nt, double *ne,
+                 double *stage, double *molecule, double *level, int *iterations);
+
+/* snapi_chemeq on device-resident arrays (T, Nt and the outputs: device
+ * pointers; the table stays a host struct and is copied), all work on
+ * `stream`. */
+int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev, const double *Nt_dev,
+                     double *ne_dev, double *stage_dev, double *molecule_dev, double *level_dev,
+                     int *iterations_dev, void *stream);
 
 #ifdef __cplusplus
 }

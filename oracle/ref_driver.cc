@@ -28,6 +28,9 @@
 //          populations, and the converged populations.
 //   interp <nnode> <is_temp> <seed>
 //          atmospheric_interpolation on a seeded node set.
+//   chemeq <atmos.cfg>
+//          the species table, then chemical equilibrium and LTE populations
+//          (atmosphere::ltepops, atmos/atmos_pop.cc:30-45) on every depth.
 //
 // This file is test infrastructure: it is compiled only against the reference
 // sources (oracle/ref.mk) and never linked into the product.
@@ -43,6 +46,12 @@
 #define private public
 #include "types.h"
 #include "io.h"
+#include "atomcfg.h"
+// the partition-function classes keep their parameters in the default
+// (private) section of a `class`; read them as struct members
+#define class struct
+#include "partf.h"
+#undef class
 #include "mem.h"
 #include "mathtools.h"
 #include "uts.h"
@@ -645,6 +654,122 @@ static int mode_interp(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// Chemical equilibrium (atmos/atmos_chm.cc:28-363) and LTE level populations
+// (atmos/atmol/atom/atom.cc:763-787) on every depth point of the column:
+//   chemeq <atmos.cfg>
+// The species table (elements with ionisation energies, partition-function
+// parameters and levels; molecules with components and dissociation-constant
+// forms) is written first, then per depth: T, Nt, the incoming Ne, partition
+// functions and ionisation fractions at that Ne, dissociation constants, and
+// after atmosphere::ltepops the final Ne, stage and level populations and the
+// H- density.
+static int mode_chemeq(int argc, char **argv, io_class &io) {
+  if (argc < 3) return 2;
+  atmosphere *a = make_atmos(argv[2], io);
+  const int x1 = a->x1l, x2 = a->x2l;
+  a->popsetup();
+  atom **el = new atom *[a->natm];
+  int nel = 0;
+  for (int s = 0; s < a->natm; ++s) {
+    int nc;
+    if (!a->atml[s]->components(nc)) el[nel++] = dynamic_cast<atom *>(a->atml[s]);
+  }
+  printf("nd %d\nnelement %d\n", a->x3h - a->x3l + 1, nel);
+  for (int ie = 0; ie < nel; ++ie) {
+    atom *t = el[ie];
+    printf("element %d %d %.17g\n", ie + 1, (int)t->Z, t->abund);
+    for (int z = 0; z < t->Z; ++z) printf("eion %d %d %.17g\n", ie + 1, z, t->ip[z]);
+    for (int z = 0; z <= t->Z; ++z) {
+      pf *p = t->partf[z];
+      printf("pf %d %d %d", ie + 1, z, (int)p->type);
+      if (p->type == PF_TYPE_CONST) printf(" %.17g", ((cpf *)p)->val);
+      if (p->type == PF_TYPE_IRWIN) {
+        irwin *w = (irwin *)p;
+        printf(" %d", w->n);
+        for (int i = 0; i < w->n; ++i) printf(" %.17g", w->a[i]);
+      }
+      if (p->type == PF_TYPE_TRAV) {
+        trav *w = (trav *)p;
+        printf(" %.17g %.17g %d", w->z, w->g0, w->n);
+      }
+      printf("\n");
+      if (p->type == PF_TYPE_TRAV) {
+        trav *w = (trav *)p;
+        for (int i = 0; i < w->n; ++i) {
+          printf("term %d %d %d %.17g %.17g %.17g %d %d", ie + 1, z, i, w->g2[i], w->ee[i], w->ll[i], (int)w->asym[i],
+                 w->m[i]);
+          for (int l = 0; l < w->m[i]; ++l) printf(" %.17g %.17g", w->g[i][l], w->a[i][l]);
+          printf("\n");
+        }
+      }
+      for (int l = 0; l < t->nl[z]; ++l)
+        printf("lvl %d %d %d %d %.17g\n", ie + 1, z, l, (int)t->g[z][l], t->ee[z][l]);
+    }
+  }
+  int nmol = 0;
+  for (int s = 0; s < a->natm; ++s) {
+    int nc;
+    ion *cmp = a->atml[s]->components(nc);
+    if (!cmp) continue;
+    mol *m = (mol *)a->atml[s];
+    eqc *K = m->K[0];
+    printf("mol %d %d %d", ++nmol, nc, (int)K->type);
+    if (K->type == EQC_TYPE_CNST) printf(" 1 %.17g", ((eqc_cnst *)K)->coef);
+    if (K->type == EQC_TYPE_POLY) {
+      eqc_poly *pk = (eqc_poly *)K;
+      printf(" %d", (int)pk->nc);
+      for (int i = 0; i < pk->nc; ++i) printf(" %.17g", pk->cfs[i]);
+    }
+    if (K->type == EQC_TYPE_HMIN) printf(" 0");
+    printf("\n");
+    for (int i = 0; i < nc; ++i) {
+      int idx = 0;
+      for (int ie = 0; ie < nel; ++ie)
+        if (cmp[i].atom == el[ie]) idx = ie + 1;
+      printf("comp %d %d %d %d\n", nmol, i, idx, (int)cmp[i].z);
+    }
+  }
+  fp_t *fk = 0, *df = 0;
+  int nk = 0;
+  for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) {
+    const fp_t T = a->T[x1][x2][x3i], Nt = a->Nt[x1][x2][x3i], ne = a->Ne[x1][x2][x3i];
+    printf("point %d %.17g %.17g %.17g\n", x3i, T, Nt, ne);
+    for (int ie = 0; ie < nel; ++ie) {
+      atom *t = el[ie];
+      for (int z = 0; z <= t->Z; ++z)
+        printf("U %d %d %d %.17g %.17g\n", x3i, ie + 1, z, t->partf[z]->U(T, ne, io), t->partf[z]->dU(T, ne, io));
+      t->ionfrc(T, ne, fk, df, nk);
+      for (int z = 0; z < nk; ++z) printf("frc %d %d %d %.17g %.17g\n", x3i, ie + 1, z, fk[z], df[z]);
+      delete[] fk;
+      delete[] df;
+      fk = df = 0;
+    }
+    int m = 0;
+    for (int s = 0; s < a->natm; ++s) {
+      int nc;
+      if (a->atml[s]->components(nc)) printf("K %d %d %.17g\n", x3i, ++m, ((mol *)a->atml[s])->dissoc(T, ne));
+    }
+  }
+  a->ltepops();
+  for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) {
+    printf("ne %d %.17g\n", x3i, a->Ne[x1][x2][x3i]);
+    for (int ie = 0; ie < nel; ++ie) {
+      atom *t = el[ie];
+      pps &p = t->pop[x1][x2][x3i];
+      printf("Nt %d %d %.17g\n", x3i, ie + 1, p.Nt);
+      for (int z = 0; z <= t->Z; ++z) {
+        printf("N %d %d %d %.17g\n", x3i, ie + 1, z, p.N[z]);
+        for (int l = 0; l < t->nl[z]; ++l) printf("n %d %d %d %d %.17g\n", x3i, ie + 1, z, l, p.n[z][l]);
+      }
+    }
+    for (int s = 0; s < a->natm; ++s) {
+      h_minus_mol *hm = dynamic_cast<h_minus_mol *>(a->atml[s]);
+      if (hm) printf("hmin %d %.17g\n", x3i, hm->N[x1][x2][x3i]);
+    }
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr, "usage: ref_driver synth|resp|lmstep|formal_scalar ...\n");
@@ -658,6 +783,7 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "formal_scalar")) return mode_formal_scalar(argc, argv, io);
   if (!strcmp(argv[1], "interp")) return mode_interp(argc, argv, io);
   if (!strcmp(argv[1], "nlte")) return mode_nlte(argc, argv, io);
+  if (!strcmp(argv[1], "chemeq")) return mode_chemeq(argc, argv, io);
   fprintf(stderr, "unknown mode %s\n", argv[1]);
   return 2;
 }

@@ -84,6 +84,16 @@ int orc_ali_solve(const snapi_ali_atom *at, const snapi_ali_grid *g, int ncol, i
                   const double *coll, double tol, int max_iter, double *pops, int *iterations, double *relchange,
                   int nthreads);
 
+/* Chemical equilibrium and LTE populations (snapi_oracle_chem.c); layouts as
+ * snapi_chemeq.  orc_chemeq_point returns the Newton steps taken or -1. */
+void orc_chem_partition(const snapi_partition *pf, double T, double ne, double *U, double *dU);
+void orc_chem_ionfrc(const snapi_chem_element *el, double T, double ne, double *fk, double *df);
+double orc_chem_dissoc(const snapi_chem_molecule *m, double T);
+int orc_chemeq_point(const snapi_chem_table *tb, double T, double Nt, double *ne, double *stage, double *mol,
+                     double *lev);
+int orc_chemeq(const snapi_chem_table *tb, int ncol, int nd, const double *T, const double *Nt, double *ne,
+               double *stage, double *mol, double *lev, int *iterations, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

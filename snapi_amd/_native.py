@@ -25,6 +25,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("HIP library %s is missing: build it with `python -m snapi_amd.build` "
                                "(or __graft_entry__.build())" % LIB_PATH)
+        # torch ships its own HIP runtime under the same soname; loaded first,
+        # it is the one the library binds to, so that torch device tensors and
+        # the library share one runtime.  Loaded after the library, torch would
+        # bring up a second runtime that finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
         LT, RY = C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay)

@@ -106,6 +106,108 @@ def make_ali_grid(lam, lamw, fudge, sigma, theta, wray, bg_per_ray, boundary=-1.
     return g
 
 
+# Chemical equilibrium (snapi_chemeq*): species table
+CHEM_MAX_ELEMENTS = 14
+CHEM_MAX_STAGES = 10
+CHEM_MAX_TERMS = 6
+CHEM_MAX_COEFS = 8
+CHEM_MAX_MOLECULES = 32
+CHEM_MAX_COMPONENTS = 4
+CHEM_MAX_LEVELS = 256
+PF_NONE, PF_CONST, PF_TRAV, PF_IRWIN = range(4)
+EQC_NONE, EQC_CONST, EQC_POLY, EQC_HMIN = range(4)
+
+
+class SnapiPartition(C.Structure):
+    _fields_ = [("type", C.c_int), ("nterm", C.c_int), ("value", C.c_double), ("charge", C.c_double),
+                ("g0", C.c_double), ("g2", C.c_double * CHEM_MAX_TERMS), ("xion", C.c_double * CHEM_MAX_TERMS),
+                ("xl", C.c_double * CHEM_MAX_TERMS), ("ncoef", C.c_int * CHEM_MAX_TERMS),
+                ("gamma", (C.c_double * CHEM_MAX_COEFS) * CHEM_MAX_TERMS),
+                ("alpha", (C.c_double * CHEM_MAX_COEFS) * CHEM_MAX_TERMS),
+                ("irwin", C.c_double * CHEM_MAX_COEFS)]
+
+
+class SnapiChemElement(C.Structure):
+    _fields_ = [("nstage", C.c_int), ("abundance", C.c_double), ("eion", C.c_double * CHEM_MAX_STAGES),
+                ("pf", SnapiPartition * CHEM_MAX_STAGES)]
+
+
+class SnapiChemMolecule(C.Structure):
+    _fields_ = [("ncomp", C.c_int), ("element", C.c_int * CHEM_MAX_COMPONENTS),
+                ("stage", C.c_int * CHEM_MAX_COMPONENTS), ("eqc", C.c_int), ("ncoef", C.c_int),
+                ("coef", C.c_double * CHEM_MAX_COEFS)]
+
+
+class SnapiChemLevel(C.Structure):
+    _fields_ = [("element", C.c_int), ("stage", C.c_int), ("g", C.c_double), ("energy", C.c_double)]
+
+
+class SnapiChemTable(C.Structure):
+    _fields_ = [("nelement", C.c_int), ("nmolecule", C.c_int), ("nlevel", C.c_int),
+                ("element", SnapiChemElement * CHEM_MAX_ELEMENTS),
+                ("molecule", SnapiChemMolecule * CHEM_MAX_MOLECULES),
+                ("level", SnapiChemLevel * CHEM_MAX_LEVELS)]
+
+
+def make_chem_table(d, levels=True):
+    """Species table from arrays shaped as tests/golden/chemeq.npz holds them:
+    per element Z, abund, eion, pf_* and term_*/irwin parameters; per molecule
+    mol_eqc/mol_ncoef/mol_coef/mol_ncomp/mol_elem/mol_stage; levels
+    (element 0-based, stage, level, g) with level_ee [erg]."""
+    t = SnapiChemTable()
+    nel = len(d["Z"])
+    t.nelement = nel
+    for e in range(nel):
+        el = t.element[e]
+        el.nstage = int(d["Z"][e]) + 1
+        el.abundance = float(d["abund"][e])
+        for z in range(el.nstage - 1):
+            el.eion[z] = float(d["eion"][e, z])
+        for z in range(el.nstage):
+            pf = el.pf[z]
+            pf.type = int(d["pf_type"][e, z])
+            pf.nterm = int(d["pf_nterm"][e, z])
+            pf.value = float(d["pf_value"][e, z])
+            pf.charge = float(d["pf_charge"][e, z])
+            pf.g0 = float(d["pf_g0"][e, z])
+            for i in range(CHEM_MAX_TERMS if pf.type == PF_TRAV else 0):
+                pf.g2[i] = float(d["term_g2"][e, z, i])
+                pf.xion[i] = float(d["term_xion"][e, z, i])
+                pf.xl[i] = float(d["term_xl"][e, z, i])
+                pf.ncoef[i] = int(d["term_ncoef"][e, z, i])
+                for c in range(CHEM_MAX_COEFS):
+                    pf.gamma[i][c] = float(d["term_gamma"][e, z, i, c])
+                    pf.alpha[i][c] = float(d["term_alpha"][e, z, i, c])
+            if pf.type == PF_IRWIN:
+                for c in range(pf.nterm):
+                    pf.irwin[c] = float(d["irwin"][e, z, c])
+    nmol = len(d["mol_eqc"])
+    t.nmolecule = nmol
+    for m in range(nmol):
+        mo = t.molecule[m]
+        mo.ncomp = int(d["mol_ncomp"][m])
+        mo.eqc = int(d["mol_eqc"][m])
+        mo.ncoef = int(d["mol_ncoef"][m])
+        for c in range(mo.ncoef):
+            mo.coef[c] = float(d["mol_coef"][m, c])
+        for i in range(mo.ncomp):
+            mo.element[i] = int(d["mol_elem"][m, i])
+            mo.stage[i] = int(d["mol_stage"][m, i])
+    if levels:
+        lv = d["levels"]
+        t.nlevel = len(lv)
+        for j in range(len(lv)):
+            t.level[j].element = int(lv[j, 0]) + 1
+            t.level[j].stage = int(lv[j, 1])
+            t.level[j].g = float(lv[j, 3])
+            t.level[j].energy = float(d["level_ee"][j])
+    return t
+
+
+def chem_stage_count(table):
+    return sum(table.element[e].nstage for e in range(table.nelement))
+
+
 def nfield(nline):
     return NF_BASE + 2 * nline
 

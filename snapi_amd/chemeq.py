@@ -1,0 +1,87 @@
+"""Chemical equilibrium and LTE populations (snapi_chemeq, snapi_chemeq_dev):
+atmosphere::ltepops (atmos/atmos_pop.cc:30-45) -- atmosphere::chemeq
+(atmos/atmos_chm.cc:28-363) then atom::lte (atmos/atmol/atom/atom.cc:779-787)
+-- at every depth point of a batch of columns, in double.
+
+The species table is an abi.SnapiChemTable (abi.make_chem_table builds one
+from per-species arrays).  Arrays (double, C order):
+  T, Nt       [ncol][nd]            temperature [K], total particle density [cm^-3]
+  ne          [ncol][nd]            electron density
+  stage       [ncol][nstage][nd]    every ionisation stage of every element, table order
+  molecule    [ncol][nmolecule][nd]
+  level       [ncol][nlevel][nd]    LTE populations of the table's levels
+  iterations  [ncol][nd]            Newton steps taken (-1: singular system)
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._native import lib, check
+
+_dp = C.POINTER(C.c_double)
+
+
+def _declare(L):
+    if getattr(L, "_chem_declared", False):
+        return
+    TB = C.POINTER(abi.SnapiChemTable)
+    L.snapi_chemeq.argtypes = [TB, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int)]
+    L.snapi_chemeq.restype = C.c_int
+    L.snapi_chemeq_dev.argtypes = [TB, C.c_int, C.c_int] + [C.c_void_p] * 8
+    L.snapi_chemeq_dev.restype = C.c_int
+    L._chem_declared = True
+
+
+def _shape(T, Nt):
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    Nt = np.ascontiguousarray(Nt, dtype=np.float64)
+    if T.ndim == 1:
+        T, Nt = T[None], Nt[None]
+    if T.shape != Nt.shape or T.ndim != 2:
+        raise ValueError("T and Nt must both be [ncol][nd], got %s and %s" % (T.shape, Nt.shape))
+    return T, Nt
+
+
+def chemeq(table, T, Nt):
+    """dict(ne, stage, molecule, level, iterations) for host arrays T, Nt."""
+    L = lib()
+    _declare(L)
+    T, Nt = _shape(T, Nt)
+    ncol, nd = T.shape
+    ne = np.zeros((ncol, nd))
+    st = np.zeros((ncol, abi.chem_stage_count(table), nd))
+    mo = np.zeros((ncol, table.nmolecule, nd))
+    lv = np.zeros((ncol, table.nlevel, nd))
+    it = np.zeros((ncol, nd), dtype=np.int32)
+    rc = L.snapi_chemeq(C.byref(table), ncol, nd, abi.ptr(T), abi.ptr(Nt), abi.ptr(ne), abi.ptr(st),
+                        abi.ptr(mo) if table.nmolecule else None, abi.ptr(lv) if table.nlevel else None,
+                        it.ctypes.data_as(C.POINTER(C.c_int)))
+    check("snapi_chemeq", rc)
+    return dict(ne=ne, stage=st, molecule=mo, level=lv, iterations=it)
+
+
+def chemeq_dev(table, T_t, Nt_t, ne_t, stage_t=None, molecule_t=None, level_t=None, iterations_t=None,
+               stream=None):
+    """snapi_chemeq_dev on torch device tensors (float64; int32 iterations);
+    optional outputs may be None."""
+    L = lib()
+    _declare(L)
+    import torch
+    if T_t.dim() != 2:
+        raise ValueError("T must be [ncol][nd]")
+    ncol, nd = T_t.shape
+    want = [(T_t, torch.float64, (ncol, nd)), (Nt_t, torch.float64, (ncol, nd)), (ne_t, torch.float64, (ncol, nd)),
+            (stage_t, torch.float64, (ncol, abi.chem_stage_count(table), nd)),
+            (molecule_t, torch.float64, (ncol, table.nmolecule, nd)),
+            (level_t, torch.float64, (ncol, table.nlevel, nd)), (iterations_t, torch.int32, (ncol, nd))]
+    for t, d, shp in want:
+        if t is None:
+            continue
+        if t.dtype != d or tuple(t.shape) != shp or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("chemeq_dev: expected a contiguous device tensor %s of shape %s, got %s %s"
+                             % (d, shp, t.dtype, tuple(t.shape)))
+    ptrs = [None if t is None else t.data_ptr() for t, _, _ in want]
+    sp = None if stream is None else C.c_void_p(stream.cuda_stream)
+    rc = L.snapi_chemeq_dev(C.byref(table), ncol, nd, *ptrs, sp)
+    check("snapi_chemeq_dev", rc)

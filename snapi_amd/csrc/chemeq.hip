@@ -1,0 +1,627 @@
+// Chemical equilibrium and LTE level populations on gfx950, in double.
+//
+// Restates atmosphere::ltepops (atmos/atmos_pop.cc:30-45) for every depth
+// point: atmosphere::chemeq (atmos/atmos_chm.cc:28-363) -- a starting
+// electron density from ionisation alone, averaging to 1 % agreement, then
+// Newton steps on (Ne, N_1..N_n) with ionisation fractions from
+// atom::ionfrc (atmos/atmol/atom/atom.cc:892-924), partition functions
+// (atmos/atmol/atom/partf.cc:14-358) and molecules in dissociation
+// equilibrium (atmos/atmol/mol/moleqc.cc) -- then atom::lte
+// (atmos/atmol/atom/atom.cc:779-787) for the requested levels.
+//
+// Layout: a team of 16 lanes (a quarter wavefront) owns one depth point and
+// keeps the whole Newton system in registers.  Lane j <= n owns unknown j
+// (0: electrons, a: element a) and column j of the Jacobian; lane a also
+// evaluates element a's partition functions and ionisation fractions; lane
+// n+1 holds the right-hand side; molecule m is evaluated on lane m mod 16.
+// Sums over elements and molecules are gathered with team shuffles in the
+// reference's order, and the system is solved by Gaussian elimination with
+// partial pivoting across the team's columns (the reference's SVD solve
+// gives the same solution for every system it accepts; it exits on a
+// singular one, here the point reports -1 steps).  The temperature-only
+// parts of the partition functions and Saha factors are evaluated once per
+// point.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "rt_common.h"
+
+namespace {
+
+constexpr int kTeam = 16;
+constexpr int kChemThreads = 256;
+constexpr int kMaxAverage = 1000;  // passes of the starting-value averaging
+constexpr int kMaxS = SNAPI_CHEM_MAX_STAGES;
+constexpr int kMaxE = SNAPI_CHEM_MAX_ELEMENTS;
+constexpr int kMaxM = SNAPI_CHEM_MAX_MOLECULES;
+constexpr int kMaxC = SNAPI_CHEM_MAX_COMPONENTS;
+static_assert(kMaxE + 2 <= kTeam, "unknowns and right-hand side must fit a team");
+static_assert(kMaxM <= 2 * kTeam, "two molecule rounds per team");
+
+// CGS constants as cmlib/const.h states them.
+constexpr double kPi = 3.141592653589793238462643383279502884197169399375105;
+constexpr double kH = 6.6260687652E-27;
+constexpr double kK = 1.3806503E-16;
+constexpr double kMe = 9.1093818872E-28;
+constexpr double kE = 4.8032042510E-10;
+constexpr double kRyd = 1.097373156853955E5;
+constexpr double kE0 = 7.957747154594775607E-02;
+
+// Table-derived constants, computed on the host.
+struct ChemDerived {
+  int nst;                            // ionisation stages over all elements
+  int soff[kMaxE];                    // first stage of each element in the stage output
+  double travf;                       // highest-level constant of trav::U (partf.cc:222)
+  double q[kMaxM];                    // molecular charge: sum of component charge indices
+  double mul[kMaxM][kTeam];           // multiplicity of unknown j (0: electrons) in molecule m
+  int celem[kMaxM][kMaxC];            // component unknown index
+  int cstage[kMaxM][kMaxC];           // component stage clamped at 0
+};
+
+__device__ __forceinline__ double tshfl(double v, int src) { return __shfl(v, src, kTeam); }
+__device__ __forceinline__ int tshfl(int v, int src) { return __shfl(v, src, kTeam); }
+
+// Fischel & Sparks (1971) simplified eqs. (26)/(27) and the derivative in p
+// (trav::qas_f, trav::dqas_f, partf.cc:261-288).
+__device__ __forceinline__ double qas_f(double l, double p, double dz) {
+  if (p > l) {
+    const double llm = l - 1.0;
+    return ((1.3333333 * p + 0.5) * p + 0.16666667 + 1.33333333 * dz) * p - 0.4 * dz * dz / p -
+           ((0.33333333 * llm - 0.5) * llm - 0.16666667 - dz) * llm + 0.5 * dz * dz / l;
+  }
+  const double axl2 = dz / (l * l);
+  return (1.0 + axl2 * (0.33333333 + 0.1 * axl2)) * p * p * p * p / l;
+}
+
+__device__ __forceinline__ double dqas_f(double l, double p, double dz) {
+  if (p > l) return ((3.0 * 1.3333333 * p + 1.0) * p + 0.16666667 + 1.33333333 * dz) + 0.4 * dz * dz / (p * p);
+  const double axl2 = dz / (l * l);
+  return 4.0 * (1.0 + axl2 * (0.33333333 + 0.1 * axl2)) * p * p * p / l;
+}
+
+// Temperature-only part of one stage's partition function: the whole U for
+// NONE (1), CONST and IRWIN; g0 plus the explicit levels for TRAV.
+__device__ double pf_base(const snapi_partition &pf, double T, double x) {
+  if (pf.type == SNAPI_PF_CONST) return pf.value;
+  if (pf.type == SNAPI_PF_IRWIN) {
+    const double lnT = log(T);
+    double s = 0.0;
+    for (int i = 0; i < pf.nterm; ++i) s += pf.irwin[i] * pow(lnT, (double)i);
+    return exp(s);
+  }
+  if (pf.type != SNAPI_PF_TRAV) return 1.0;
+  double sum = pf.g0;
+  for (int i = 0; i < pf.nterm; ++i)
+    for (int l = 0; l < pf.ncoef[i]; ++l) sum += pf.alpha[i][l] * exp(-2.302585 * pf.gamma[i][l] * x);
+  return sum;
+}
+
+// U and dU/dne of one stage at electron density ne (r6 = ne^(1/6),
+// r76 = ne^(7/6)); trav::dU keeps only the last term (partf.cc:236-250).
+__device__ __forceinline__ void pf_eval(const snapi_partition &pf, double base, double x, double f, double r6,
+                                        double r76, double &U, double &dU) {
+  U = base;
+  dU = 0.0;
+  if (pf.type != SNAPI_PF_TRAV) return;
+  const double z = pf.charge;
+  const double p = z * f / r6;
+  const double alpha = 31.321 * z * z * x;
+  const int n = pf.nterm;
+  for (int i = 0; i < n; ++i) {
+    const double c = pf.g2[i] * exp(-2.302585 * pf.xion[i] * x);
+    U += c * qas_f(pf.xl[i], p, alpha);
+    if (i == n - 1) dU = -c * dqas_f(pf.xl[i], p, alpha) * z * f / (6.0 * r76);
+  }
+}
+
+// Raw sums of atom::ionfrc (atom.cc:892-924) over one element's stages:
+// with the unnormalised fractions fk_z (fk_0 = 1) and derivatives df_z,
+// fsum = sum fk, dfsum = sum df, ez = sum z fk, dz = sum z df, and fk_1,
+// df_1.  The normalised values follow as fk/fsum and (df - dfsum fk/fsum)/fsum.
+struct IonSums {
+  double fsum, dfsum, ez, dz, f1, d1;
+};
+
+// base/sx: the element's temperature-only partition-function parts and Saha
+// factors, per stage, in this thread's LDS slots.  With store, fk_{z+1}
+// overwrites the Saha factor of stage z once used.
+__device__ IonSums ionfrc(const snapi_chem_element &el, int ns, const double *base, double *sx, double x, double fct,
+                          double f, double ne, bool store) {
+  const double r6 = pow(ne, 1.0 / 6.0), r76 = ne * r6;
+  double Up, dUp;
+  pf_eval(el.pf[0], base[0], x, f, r6, r76, Up, dUp);
+  dUp /= Up;
+  IonSums s{1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double fk = 1.0, dfv = 0.0;
+  for (int z = 0; z < ns - 1; ++z) {
+    double Un, dUn;
+    pf_eval(el.pf[z + 1], base[(z + 1) * kChemThreads], x, f, r6, r76, Un, dUn);
+    dUn /= Un;
+    const double fz = fct * (Un / Up) * sx[z * kChemThreads] / ne;
+    const double fd = fz * ((dUn) - (dUp * ne + 1.0) / ne);
+    dfv = dfv * fz + fk * fd;
+    fk = fk * fz;
+    s.fsum += fk;
+    s.dfsum += dfv;
+    s.ez += (z + 1) * fk;
+    s.dz += (z + 1) * dfv;
+    if (z == 0) {
+      s.f1 = fk;
+      s.d1 = dfv;
+    }
+    if (store) sx[z * kChemThreads] = fk;
+    Up = Un;
+    dUp = dUn;
+  }
+  return s;
+}
+
+// Dissociation constant (eqc_cnst / eqc_poly / eqc_hmin, moleqc.cc).
+__device__ double dissoc(const snapi_chem_molecule &m, double T) {
+  const double x = 5040.0 / T;
+  if (m.eqc == SNAPI_EQC_CONST) return m.coef[0];
+  if (m.eqc == SNAPI_EQC_POLY) {
+    double s = m.coef[m.ncoef - 1];
+    for (int i = m.ncoef - 2; i >= 0; --i) s = m.coef[i] + x * s;
+    return exp(2.302585093 * s);
+  }
+  if (m.eqc == SNAPI_EQC_HMIN) {
+    const double s = 0.1249 + (2.5 * log(T) / 2.302585) - 0.747 * x;
+    return exp(2.302585093 * s);
+  }
+  return 0.0;
+}
+
+// Phases of one point's computation; every phase starts with the ionisation
+// fractions of all elements at the phase's electron density.
+enum : int { kAtOne = 0, kAtNt = 1, kAverage = 2, kNewton = 3, kFinal = 4 };
+
+__global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_table *__restrict__ tb,
+                                                              const ChemDerived *__restrict__ dv, long npt,
+                                                              const double *__restrict__ Tin,
+                                                              const double *__restrict__ Ntin, int nd,
+                                                              double *__restrict__ ne_out,
+                                                              double *__restrict__ stage_out,
+                                                              double *__restrict__ mol_out,
+                                                              double *__restrict__ lev_out, int *__restrict__ it_out) {
+  // per-thread slots [stage][thread]: temperature-only partition-function
+  // part and Saha factor exp(-chi/kT) of this lane's element
+  __shared__ double base_s[kMaxS * kChemThreads];
+  __shared__ double sx_s[kMaxS * kChemThreads];
+  const int tid = threadIdx.x;
+  const int lane = tid & (kTeam - 1);
+  const long pt = ((long)blockIdx.x * blockDim.x + tid) / kTeam;
+  if (pt >= npt) return;  // the whole team leaves together; no block barrier follows
+  double *base = base_s + tid, *sx = sx_s + tid;
+  const int na = tb->nelement, nm = tb->nmolecule;
+  const double T = Tin[pt], Nt = Ntin[pt];
+  const double x = 5040.0 / T, kT = kK * T, f = dv->travf;
+  const double fct = 2.0 * pow((2.0 * kPi * kMe * kK * T) / (kH * kH), 1.5);
+  const bool is_el = lane >= 1 && lane <= na;
+  const bool is_unk = lane <= na;
+  const snapi_chem_element &el = tb->element[is_el ? lane - 1 : 0];
+  const int ns = is_el ? el.nstage : 1;
+  const double ab = is_el ? el.abundance : 0.0;
+  const double ab1 = tb->element[0].abundance;
+  if (is_el) {
+    for (int z = 0; z < ns; ++z) {
+      base[z * kChemThreads] = pf_base(el.pf[z], T, x);
+      if (z < ns - 1) sx[z * kChemThreads] = exp(-el.eion[z] / (kK * T));
+    }
+  }
+  double Km[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int m = r * kTeam + lane;
+    Km[r] = (m < nm) ? dissoc(tb->molecule[m], T) : 1.0;
+  }
+  double S = 0.0;
+  for (int a = 0; a < na; ++a) S += tb->element[a].abundance;
+
+  const int rhs = na + 1;
+  int phase = kAtOne, pass = 0, steps = 0, iter = 0;
+  double ne_eval = 1.0, F1 = 0.0, N0 = 0.0, Nown = 0.0;
+  IonSums s{1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (;;) {
+    if (is_el) s = ionfrc(el, ns, base, sx, x, fct, f, ne_eval, phase == kFinal);
+    if (phase == kFinal) break;
+    const double rf = 1.0 / s.fsum;
+    if (phase == kAtOne || phase == kAtNt) {
+      // starting electron density from ionisation alone (atmos_chm.cc:127-154)
+      const double v = ab * (s.ez * rf);
+      double F = 0.0;
+      for (int a = 1; a <= na; ++a) F += tshfl(v, a);
+      if (phase == kAtOne) {
+        F1 = F;
+        ne_eval = Nt;
+        phase = kAtNt;
+        continue;
+      }
+      const double aa = F / (S * Nt);
+      double bb = F1 / S - aa;
+      const double cc = -bb * Nt;
+      bb += 1.0 - aa * Nt;
+      const double root = (-bb + sqrt(bb * bb - 4.0 * aa * cc)) / (2.0 * aa);
+      N0 = (root > 1.0) ? root : 1.0;
+      ne_eval = N0;
+      phase = kAverage;
+      continue;
+    }
+    if (phase == kAverage) {
+      // average until the electron count agrees to 1 % (atmos_chm.cc:157-169);
+      // the reference has no limit, a point still outside 1 % after
+      // kMaxAverage passes continues from the current estimate
+      Nown = is_el ? (Nt - N0) * ab / S : 0.0;
+      const double sl = (s.ez * rf) * Nown;
+      double esum = 0.0;
+      for (int a = 1; a <= na; ++a) esum += tshfl(sl, a);
+      N0 = 0.5 * (N0 + esum);
+      if (!((1E-2 * (N0 + esum)) < fabs(N0 - esum)) || ++pass >= kMaxAverage) {
+        phase = kNewton;
+        if (lane == 0) Nown = N0;
+      }
+      ne_eval = N0;
+      continue;
+    }
+
+    // one Newton-Raphson step at Ne = N0 (atmos_chm.cc:179-305)
+    N0 = ne_eval;
+    const double N1 = tshfl(Nown, 1);
+    const double eown = s.ez * rf;
+    const double tdef = Nown * eown;
+    const double uder = Nown * ((s.dz - s.dfsum * s.ez * rf) * rf);
+    const double fk0 = is_el ? rf : 1.0, fk1 = is_el ? s.f1 * rf : 0.0;
+    const double df0 = is_el ? (0.0 - s.dfsum * rf) * rf : 0.0;
+    const double df1 = is_el ? (s.d1 - s.dfsum * s.f1 * rf) * rf : 0.0;
+    // molecules: number densities and the electron-density factor of dm[m][0]
+    double nmr[2], dmr[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int m = r * kTeam + lane;
+      const bool ok = m < nm;
+      const int mm = ok ? m : 0;
+      double nmv = 1.0 / (Km[r] * kT), d0f = 0.0;
+#pragma unroll
+      for (int c = 0; c < kMaxC; ++c) {
+        const bool use = ok && c < tb->molecule[mm].ncomp;
+        const int a = use ? dv->celem[mm][c] : 0;
+        const int zc = use ? dv->cstage[mm][c] : 0;
+        const double Na = tshfl(Nown, a);
+        const double f0 = tshfl(fk0, a), f1 = tshfl(fk1, a), g0 = tshfl(df0, a), g1 = tshfl(df1, a);
+        if (use) {
+          const double fz = zc ? f1 : f0, gz = zc ? g1 : g0;
+          nmv *= fz * Na * kT;
+          if (a == 0) d0f += 1.0 / Na;
+          d0f += gz / fz;
+        }
+      }
+      nmr[r] = ok ? nmv : 0.0;
+      dmr[r] = ok ? nmv * d0f : 0.0;
+    }
+    // defects
+    double d0 = -N0;
+    for (int a = 1; a <= na; ++a) d0 += tshfl(tdef, a);
+    double d1 = N0 - Nt;
+    for (int a = 1; a <= na; ++a) d1 += tshfl(Nown, a);
+    double Nr = N1, Nc = Nown;
+    for (int m = 0; m < nm; ++m) {
+      const double nmv = tshfl(m < kTeam ? nmr[0] : nmr[1], m & (kTeam - 1));
+      d0 += dv->q[m] * nmv;
+      d1 += nmv;
+      Nr += nmv * dv->mul[m][1];
+      Nc += nmv * dv->mul[m][lane];
+    }
+    const double down = (lane == 0) ? d0 : (lane == 1) ? d1 : ab * Nr - ab1 * Nc;
+    // Jacobian column of this lane (rows 0..n), or the right-hand side
+    double col[kTeam];
+    {
+      double A00 = -1.0;
+      for (int a = 1; a <= na; ++a) A00 += tshfl(uder, a);
+      double acc0 = (lane == 0) ? A00 : eown, acc1 = 1.0, dNr = 0.0;
+      double dNc[kTeam];
+#pragma unroll
+      for (int a = 0; a < kTeam; ++a) dNc[a] = 0.0;
+      const double inv = 1.0 / Nown;
+      for (int m = 0; m < nm; ++m) {
+        const double nmv = tshfl(m < kTeam ? nmr[0] : nmr[1], m & (kTeam - 1));
+        const double d0m = tshfl(m < kTeam ? dmr[0] : dmr[1], m & (kTeam - 1));
+        double dmj;
+        if (lane == 0) {
+          dmj = d0m;
+        } else {
+          double sj = 0.0;
+          for (int c = 0; c < tb->molecule[m].ncomp; ++c)
+            if (dv->celem[m][c] == lane) sj += inv;
+          dmj = sj * nmv;
+        }
+        acc0 += dv->q[m] * dmj;
+        acc1 += dmj;
+        dNr += dmj * dv->mul[m][1];
+#pragma unroll
+        for (int a = 2; a < kTeam; ++a) dNc[a] += dmj * dv->mul[m][a];
+      }
+      if (lane == 1) dNr += 1.0;
+      col[0] = acc0;
+      col[1] = acc1;
+#pragma unroll
+      for (int a = 2; a < kTeam; ++a) {
+        const double dn = dNc[a] + ((lane == a) ? 1.0 : 0.0);
+        const double aba = (a <= na) ? tb->element[(a <= na ? a : 1) - 1].abundance : 0.0;
+        col[a] = aba * dNr - ab1 * dn;
+      }
+      // the right-hand side lane collects the defects
+#pragma unroll
+      for (int a = 0; a < kTeam; ++a) {
+        const double da = tshfl(down, a);
+        if (lane == rhs) col[a] = (a <= na) ? da : 0.0;
+      }
+      if (lane > rhs) {
+#pragma unroll
+        for (int a = 0; a < kTeam; ++a) col[a] = 0.0;
+      }
+    }
+    // Gaussian elimination with partial pivoting over the team's columns
+    bool singular = false;
+#pragma unroll
+    for (int k = 0; k < kTeam - 1; ++k) {
+      if (k <= na) {
+        int p = k;
+        double best = fabs(col[k]);
+#pragma unroll
+        for (int r = k + 1; r < kTeam; ++r)
+          if (r <= na && fabs(col[r]) > best) {
+            best = fabs(col[r]);
+            p = r;
+          }
+        p = tshfl(p, k);
+#pragma unroll
+        for (int r = k + 1; r < kTeam; ++r)
+          if (r == p) {
+            const double t = col[k];
+            col[k] = col[r];
+            col[r] = t;
+          }
+        const double piv = tshfl(col[k], k);
+        if (!(fabs(piv) > 0.0) || !isfinite(piv)) singular = true;
+#pragma unroll
+        for (int r = k + 1; r < kTeam; ++r) {
+          if (r <= na) {
+            const double l = tshfl(col[r], k) / piv;
+            if (lane > k) col[r] -= l * col[k];
+          }
+        }
+      }
+    }
+    if (singular) {
+      steps = -1;
+      phase = kFinal;
+      continue;
+    }
+    // back substitution: v_k = (rhs_k - sum_{j>k} U_kj v_j) / U_kk
+    double myv = 0.0;
+#pragma unroll
+    for (int k = kTeam - 2; k >= 0; --k) {
+      if (k <= na) {
+        double sk = col[k];  // meaningful on the right-hand side lane
+#pragma unroll
+        for (int j = k + 1; j < kTeam - 1; ++j)
+          if (j <= na) sk -= tshfl(col[k] * myv, j);
+        const double ukk = tshfl(col[k], k);
+        const double vk = tshfl(sk / ukk, rhs);
+        if (lane == k) myv = vk;
+      }
+    }
+    if (is_unk) {
+      Nown -= myv;
+      if (Nown < 1E-2) Nown = 1E-2;
+    }
+    ++steps;
+    // the reference counts two per step (one inside a log call) and stops above 100
+    iter += 2;
+    const bool bad = is_unk && (fabs(down / Nown) > 1E-4);
+    const unsigned long long b = __ballot(bad);
+    const unsigned long long team = 0xFFFFull << (tid & 63 & ~(kTeam - 1));
+    if (iter > 100 || !(b & team)) {
+      // outputs use the fractions of this step's electron density
+      phase = kFinal;
+      continue;
+    }
+    ne_eval = tshfl(Nown, 0);
+  }
+
+  // outputs: the final N with the last step's fractions (atom::pupdate, atom.cc:1449-1453)
+  const long c = pt / nd, zd = pt % nd;
+  N0 = tshfl(Nown, 0);
+  const double rf = 1.0 / s.fsum;
+  if (lane == 0) {
+    ne_out[pt] = N0;
+    if (it_out) it_out[pt] = steps;
+  }
+  if (stage_out && is_el) {
+    const int off = dv->soff[lane - 1], nst = dv->nst;
+    for (int z = 0; z < ns; ++z)
+      stage_out[((long)c * nst + off + z) * nd + zd] = Nown * ((z ? sx[(z - 1) * kChemThreads] : 1.0) * rf);
+  }
+  if (mol_out) {
+    const double fk0 = is_el ? rf : 1.0, fk1 = is_el ? s.f1 * rf : 0.0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int m = r * kTeam + lane;
+      const bool ok = m < nm;
+      const int mm = ok ? m : 0;
+      double nmv = 1.0 / (Km[r] * kT);
+#pragma unroll
+      for (int cc2 = 0; cc2 < kMaxC; ++cc2) {
+        const bool use = ok && cc2 < tb->molecule[mm].ncomp;
+        const int a = use ? dv->celem[mm][cc2] : 0;
+        const int zc = use ? dv->cstage[mm][cc2] : 0;
+        const double Na = tshfl(Nown, a), f0 = tshfl(fk0, a), f1 = tshfl(fk1, a);
+        if (use) nmv *= (zc ? f1 : f0) * Na * kT;
+      }
+      if (ok) mol_out[((long)c * nm + m) * nd + zd] = nmv;
+    }
+  }
+  if (lev_out && is_el) {
+    const int nl = tb->nlevel;
+    const double r6 = pow(N0, 1.0 / 6.0), r76 = N0 * r6;
+    for (int l = 0; l < nl; ++l) {
+      const snapi_chem_level &L = tb->level[l];
+      if (L.element != lane) continue;
+      const int z = L.stage;
+      const double Nz = Nown * ((z ? sx[(z - 1) * kChemThreads] : 1.0) * rf);
+      double U, dU;
+      pf_eval(el.pf[z], base[z * kChemThreads], x, f, r6, r76, U, dU);
+      lev_out[((long)c * nl + l) * nd + zd] = Nz * ((L.g != 0.0) ? L.g * exp(-L.energy / kT) / U : 1.0);
+    }
+  }
+}
+
+// Checks the table against the kernel's limits and derives its constants.
+int derive(const snapi_chem_table *tb, ChemDerived &d) {
+  if (tb->nelement < 1 || tb->nelement > kMaxE || tb->nmolecule < 0 || tb->nmolecule > kMaxM ||
+      tb->nlevel < 0 || tb->nlevel > SNAPI_CHEM_MAX_LEVELS)
+    return SNAPI_ERR_UNSUPPORTED;
+  memset(&d, 0, sizeof(d));
+  int nst = 0;
+  for (int a = 0; a < tb->nelement; ++a) {
+    const snapi_chem_element &el = tb->element[a];
+    if (el.nstage < 1 || el.nstage > kMaxS || !(el.abundance >= 0.0)) return SNAPI_ERR_UNSUPPORTED;
+    for (int z = 0; z < el.nstage - 1; ++z)
+      if (!std::isfinite(el.eion[z])) return SNAPI_ERR_UNSUPPORTED;
+    for (int z = 0; z < el.nstage; ++z) {
+      const snapi_partition &pf = el.pf[z];
+      if (pf.type < SNAPI_PF_NONE || pf.type > SNAPI_PF_IRWIN) return SNAPI_ERR_UNSUPPORTED;
+      if (pf.type == SNAPI_PF_TRAV) {
+        if (pf.nterm < 0 || pf.nterm > SNAPI_CHEM_MAX_TERMS) return SNAPI_ERR_UNSUPPORTED;
+        for (int i = 0; i < pf.nterm; ++i)
+          if (pf.ncoef[i] < 0 || pf.ncoef[i] > SNAPI_CHEM_MAX_COEFS) return SNAPI_ERR_UNSUPPORTED;
+      }
+      if (pf.type == SNAPI_PF_IRWIN && (pf.nterm < 0 || pf.nterm > SNAPI_CHEM_MAX_COEFS))
+        return SNAPI_ERR_UNSUPPORTED;
+    }
+    d.soff[a] = nst;
+    nst += el.nstage;
+  }
+  d.nst = nst;
+  if (!(tb->element[0].abundance > 0.0)) return SNAPI_ERR_UNSUPPORTED;
+  for (int m = 0; m < tb->nmolecule; ++m) {
+    const snapi_chem_molecule &M = tb->molecule[m];
+    if (M.ncomp < 1 || M.ncomp > kMaxC) return SNAPI_ERR_UNSUPPORTED;
+    if (M.eqc == SNAPI_EQC_POLY && (M.ncoef < 1 || M.ncoef > SNAPI_CHEM_MAX_COEFS)) return SNAPI_ERR_UNSUPPORTED;
+    if (M.eqc < SNAPI_EQC_NONE || M.eqc > SNAPI_EQC_HMIN) return SNAPI_ERR_UNSUPPORTED;
+    for (int i = 0; i < M.ncomp; ++i) {
+      const int e = M.element[i], z = M.stage[i] > 0 ? M.stage[i] : 0;
+      if (e < 0 || e > tb->nelement || z >= SNAPI_CHEM_MAX_COMP_STAGE) return SNAPI_ERR_UNSUPPORTED;
+      if (e > 0 && z >= tb->element[e - 1].nstage) return SNAPI_ERR_UNSUPPORTED;
+      if (e == 0 && z != 0) return SNAPI_ERR_UNSUPPORTED;
+      d.celem[m][i] = e;
+      d.cstage[m][i] = z;
+      d.mul[m][e] += 1.0;
+      d.q[m] += (double)M.stage[i];
+    }
+  }
+  for (int l = 0; l < tb->nlevel; ++l) {
+    const snapi_chem_level &L = tb->level[l];
+    if (L.element < 1 || L.element > tb->nelement || L.stage < 0 ||
+        L.stage >= tb->element[L.element - 1].nstage)
+      return SNAPI_ERR_UNSUPPORTED;
+  }
+  const double er = kE * kRyd * kRyd;
+  d.travf = 1.405169521381833474E+05 * ((4.0 * kPi * kE0) / (er * er));
+  return SNAPI_OK;
+}
+
+int launch_chemeq(const snapi_chem_table *dtb, const ChemDerived *ddv, long npt, int nd, const double *T,
+                  const double *Nt, double *ne, double *stage, double *mol, double *lev, int *it, hipStream_t st) {
+  const long threads = npt * kTeam;
+  const long blocks = (threads + kChemThreads - 1) / kChemThreads;
+  if (blocks > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(chemeq_kernel, dim3((unsigned)blocks), dim3(kChemThreads), 0, st, dtb, ddv, npt, T, Nt, nd, ne,
+                     stage, mol, lev, it);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+#define CCHK(x)              \
+  do {                       \
+    if ((x) != hipSuccess) { \
+      rc = SNAPI_ERR_DEVICE; \
+      goto done;             \
+    }                        \
+  } while (0)
+
+// Copies the table and its derived constants into one device buffer.
+int upload_table(const snapi_chem_table *tb, const ChemDerived &d, void **buf, hipStream_t st) {
+  *buf = nullptr;
+  const size_t n = sizeof(snapi_chem_table) + sizeof(ChemDerived);
+  if (hipMalloc(buf, n) != hipSuccess) return SNAPI_ERR_DEVICE;
+  if (hipMemcpyAsync(*buf, tb, sizeof(snapi_chem_table), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync((char *)*buf + sizeof(snapi_chem_table), &d, sizeof(ChemDerived), hipMemcpyHostToDevice, st) !=
+          hipSuccess) {
+    hipFree(*buf);
+    *buf = nullptr;
+    return SNAPI_ERR_DEVICE;
+  }
+  return SNAPI_OK;
+}
+
+}  // namespace
+
+extern "C" int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,
+                                const double *Nt_dev, double *ne_dev, double *stage_dev, double *molecule_dev,
+                                double *level_dev, int *iterations_dev, void *stream) {
+  if (!table || ncol <= 0 || nd <= 0 || !T_dev || !Nt_dev || !ne_dev) return SNAPI_ERR_ARG;
+  ChemDerived d;
+  int rc = derive(table, d);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  void *buf = nullptr;
+  rc = upload_table(table, d, &buf, st);
+  if (rc) return rc;
+  rc = launch_chemeq((const snapi_chem_table *)buf,
+                     (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), (long)ncol * nd, nd, T_dev,
+                     Nt_dev, ne_dev, stage_dev, table->nmolecule ? molecule_dev : nullptr,
+                     table->nlevel ? level_dev : nullptr, iterations_dev, st);
+  // the table buffer is freed once the kernel that reads it has finished
+  if (hipStreamSynchronize(st) != hipSuccess && rc == SNAPI_OK) rc = SNAPI_ERR_DEVICE;
+  hipFree(buf);
+  return rc;
+}
+
+extern "C" int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, const double *T, const double *Nt,
+                            double *ne, double *stage, double *molecule, double *level, int *iterations) {
+  if (!table || ncol <= 0 || nd <= 0 || !T || !Nt || !ne) return SNAPI_ERR_ARG;
+  ChemDerived d;
+  int rc = derive(table, d);
+  if (rc) return rc;
+  const size_t np = (size_t)ncol * nd;
+  const size_t nst = (size_t)d.nst, nm = (size_t)table->nmolecule, nl = (size_t)table->nlevel;
+  double *dT = nullptr, *dNt = nullptr, *dne = nullptr, *dst = nullptr, *dmo = nullptr, *dlv = nullptr;
+  int *dit = nullptr;
+  void *buf = nullptr;
+  CCHK(hipMalloc((void **)&dT, np * 8));
+  CCHK(hipMalloc((void **)&dNt, np * 8));
+  CCHK(hipMalloc((void **)&dne, np * 8));
+  if (stage) CCHK(hipMalloc((void **)&dst, np * nst * 8));
+  if (molecule && nm) CCHK(hipMalloc((void **)&dmo, np * nm * 8));
+  if (level && nl) CCHK(hipMalloc((void **)&dlv, np * nl * 8));
+  if (iterations) CCHK(hipMalloc((void **)&dit, np * sizeof(int)));
+  CCHK(hipMemcpy(dT, T, np * 8, hipMemcpyHostToDevice));
+  CCHK(hipMemcpy(dNt, Nt, np * 8, hipMemcpyHostToDevice));
+  rc = upload_table(table, d, &buf, nullptr);
+  if (rc) goto done;
+  rc = launch_chemeq((const snapi_chem_table *)buf,
+                     (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), (long)np, nd, dT, dNt, dne,
+                     dst, dmo, dlv, dit, nullptr);
+  if (rc) goto done;
+  CCHK(hipDeviceSynchronize());
+  CCHK(hipMemcpy(ne, dne, np * 8, hipMemcpyDeviceToHost));
+  if (dst) CCHK(hipMemcpy(stage, dst, np * nst * 8, hipMemcpyDeviceToHost));
+  if (dmo) CCHK(hipMemcpy(molecule, dmo, np * nm * 8, hipMemcpyDeviceToHost));
+  if (dlv) CCHK(hipMemcpy(level, dlv, np * nl * 8, hipMemcpyDeviceToHost));
+  if (dit) CCHK(hipMemcpy(iterations, dit, np * sizeof(int), hipMemcpyDeviceToHost));
+done:
+  hipFree(dT); hipFree(dNt); hipFree(dne); hipFree(dst); hipFree(dmo); hipFree(dlv); hipFree(dit); hipFree(buf);
+  return rc;
+}

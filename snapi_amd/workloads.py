@@ -336,3 +336,23 @@ def template_cfg1():
 
 def cfg0_column():
     return column_from_dump(os.path.join(GOLDEN, "cfg0_synth.npz"))
+
+
+def chemeq_problem(nx, ny, seed=5):
+    """Depth points of the thermodynamics workload: the species table recorded
+    from the reference (eight background atoms, the two-line Fe atom, 13
+    molecules, tests/golden/chemeq.npz) and nx*ny columns of the cfg0 column's
+    61 depths (T, Nt after the reference's hydrostatic integration), with a
+    Gaussian random field of 150 K rms added to T and a per-column 0.1 dex rms
+    scaling of Nt.  Returns dict(table, T [ncol][nd], Nt [ncol][nd])."""
+    d = np.load(os.path.join(GOLDEN, "chemeq.npz"))
+    table = abi.make_chem_table(d)
+    n0, n1 = (int(x) for x in d["ncolumn_points"])
+    T0, Nt0 = d["T"][n0:n0 + n1], d["Nt"][n0:n0 + n1]
+    rng = np.random.default_rng(seed)
+    ncol = nx * ny
+    dT = gaussian_random_field(nx, ny, 150.0, 8.0, rng).reshape(ncol)
+    dn = rng.normal(0.0, 0.1, ncol)
+    T = np.ascontiguousarray(T0[None, :] + dT[:, None])
+    Nt = np.ascontiguousarray(Nt0[None, :] * 10.0 ** dn[:, None])
+    return dict(table=table, T=T, Nt=Nt)

@@ -49,3 +49,17 @@ def nlte():
                 vproj=d["vproj"], gamma=d["gamma"], bgop=d["bgop"], bgem=d["bgem"], coll=d["coll"],
                 pop0=d["pop0"], acc=d["acc"], newpop=d["newpop"], iters=d["iters"], final=d["final"],
                 ref_final=d["ref_final"])
+
+
+def chemeq():
+    """The reference's chemical equilibrium on 133 depth points (a T/Pg sweep
+    and the cfg0 column): species table arrays, per-point inputs, the
+    reference's partition functions, fractions, dissociation constants and
+    results.  Adds 'table' (abi.SnapiChemTable) and 'stage_ref'
+    [npoint][nstage] (stages of all elements in table order)."""
+    from snapi_amd import abi
+    d = dict(load("chemeq.npz"))
+    d["table"] = abi.make_chem_table(d)
+    t = d["table"]
+    d["stage_ref"] = np.concatenate([d["N"][:, e, :t.element[e].nstage] for e in range(t.nelement)], axis=1)
+    return d

@@ -271,3 +271,53 @@ def ali_solve(p, pops, tol, max_iter, ncol=1, nthreads=0):
     L.orc_ali_solve(C.byref(p["atom"]), C.byref(p["grid"]), ncol, nd, *[ptr(a) for a in args], tol, max_iter,
                     ptr(pp), it.ctypes.data_as(C.POINTER(C.c_int)), ptr(rc), nthreads)
     return pp, it, rc
+
+
+def _chem_lib():
+    from snapi_amd import abi
+    L = lib()
+    if not getattr(L, "_chem_ready", False):
+        TB = C.POINTER(abi.SnapiChemTable)
+        dp = C.POINTER(C.c_double)
+        L.orc_chem_partition.argtypes = [C.POINTER(abi.SnapiPartition), C.c_double, C.c_double, dp, dp]
+        L.orc_chem_ionfrc.argtypes = [C.POINTER(abi.SnapiChemElement), C.c_double, C.c_double, dp, dp]
+        L.orc_chem_dissoc.argtypes = [C.POINTER(abi.SnapiChemMolecule), C.c_double]
+        L.orc_chem_dissoc.restype = C.c_double
+        L.orc_chemeq_point.argtypes = [TB, C.c_double, C.c_double, dp, dp, dp, dp]
+        L.orc_chemeq_point.restype = C.c_int
+        L.orc_chemeq.argtypes = [TB, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_int), C.c_int]
+        L._chem_ready = True
+    return L
+
+
+def chem_partition(pf, T, ne):
+    U, dU = C.c_double(), C.c_double()
+    _chem_lib().orc_chem_partition(C.byref(pf), float(T), float(ne), C.byref(U), C.byref(dU))
+    return U.value, dU.value
+
+
+def chem_ionfrc(element, T, ne):
+    fk = np.zeros(element.nstage)
+    df = np.zeros(element.nstage)
+    _chem_lib().orc_chem_ionfrc(C.byref(element), float(T), float(ne), ptr(fk), ptr(df))
+    return fk, df
+
+
+def chem_dissoc(molecule, T):
+    return _chem_lib().orc_chem_dissoc(C.byref(molecule), float(T))
+
+
+def chemeq(table, T, Nt, nthreads=0):
+    """T, Nt [ncol][nd] -> ne, stage [ncol][nstage][nd], molecule, level, iterations."""
+    from snapi_amd.abi import chem_stage_count
+    L = _chem_lib()
+    T, Nt = _c(T), _c(Nt)
+    ncol, nd = T.shape
+    ne = np.zeros((ncol, nd))
+    st = np.zeros((ncol, chem_stage_count(table), nd))
+    mo = np.zeros((ncol, table.nmolecule, nd))
+    lv = np.zeros((ncol, max(table.nlevel, 1), nd))
+    it = np.zeros((ncol, nd), dtype=np.int32)
+    L.orc_chemeq(C.byref(table), ncol, nd, ptr(T), ptr(Nt), ptr(ne), ptr(st), ptr(mo), ptr(lv),
+                 it.ctypes.data_as(C.POINTER(C.c_int)), nthreads if nthreads else (os.cpu_count() or 1))
+    return ne, st, mo, lv[:, :table.nlevel], it

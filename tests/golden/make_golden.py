@@ -440,6 +440,138 @@ def build_nlte(d):
     print("nlte: %d depths, %d wavelengths, %d rays, %d iterations" % (nd, nl, ntp, len(iters)))
 
 
+CHEM_MAXS, CHEM_MAXT, CHEM_MAXC = 10, 6, 8
+
+
+def chemeq_grid_atmosphere():
+    """A column whose depth points sweep temperature and gas pressure
+    independently: 12 temperatures 3000-40000 K times 6 pressures
+    1e-1-1e6 dyn cm^-2 (SPINOR rows; Pe = 1e-4 Pg is only a starting value)."""
+    rows = []
+    temps = [3000.0, 3500.0, 4000.0, 4500.0, 5000.0, 5800.0, 6500.0, 8000.0, 10000.0, 15000.0, 25000.0, 40000.0]
+    pres = [1e-1, 1e1, 1e3, 1e4, 1e5, 1e6]
+    i = 0
+    for T in temps:
+        for pg in pres:
+            lt = -7.0 + 0.1 * i
+            rows.append((lt, 1e7 * (1.0 - lt), T, pg, pg * 1e-4, 0.0, 0.0, 0.0, 1e5, 0.0, 0.0, 0.0))
+            i += 1
+    return rows
+
+
+def parse_chemeq(out):
+    """Parses the driver's `chemeq` records into dense arrays (element index
+    1-based in the records, 0-based in the arrays)."""
+    S, T_, C = CHEM_MAXS, CHEM_MAXT, CHEM_MAXC
+    recs = [ln.split() for ln in out if ln.strip()]
+    nel = next(int(t[1]) for t in recs if t[0] == "nelement")
+    nmol = sum(1 for t in recs if t[0] == "mol")
+    pts = [t for t in recs if t[0] == "point"]
+    npt = len(pts)
+    pidx = {int(t[1]): i for i, t in enumerate(pts)}
+    r = dict(
+        Z=np.zeros(nel, np.int32), abund=np.zeros(nel), eion=np.full((nel, S), np.nan),
+        pf_type=np.zeros((nel, S), np.int32), pf_value=np.zeros((nel, S)), pf_charge=np.zeros((nel, S)),
+        pf_g0=np.zeros((nel, S)), pf_nterm=np.zeros((nel, S), np.int32),
+        term_g2=np.zeros((nel, S, T_)), term_xion=np.zeros((nel, S, T_)), term_xl=np.zeros((nel, S, T_)),
+        term_ncoef=np.zeros((nel, S, T_), np.int32), term_gamma=np.zeros((nel, S, T_, C)),
+        term_alpha=np.zeros((nel, S, T_, C)), irwin=np.zeros((nel, S, C)),
+        mol_eqc=np.zeros(nmol, np.int32), mol_ncoef=np.zeros(nmol, np.int32), mol_coef=np.zeros((nmol, C)),
+        mol_ncomp=np.zeros(nmol, np.int32), mol_elem=np.zeros((nmol, 4), np.int32),
+        mol_stage=np.zeros((nmol, 4), np.int32),
+        T=np.zeros(npt), Nt=np.zeros(npt), ne_in=np.zeros(npt),
+        U=np.zeros((npt, nel, S)), dU=np.zeros((npt, nel, S)), fk=np.zeros((npt, nel, S)),
+        df=np.zeros((npt, nel, S)), K=np.zeros((npt, nmol)), ne=np.zeros(npt), Nel=np.zeros((npt, nel)),
+        N=np.zeros((npt, nel, S)), hmin=np.zeros(npt))
+    lvl, pops = [], {}
+    for t in recs:
+        k = t[0]
+        if k == "element":
+            e = int(t[1]) - 1
+            r["Z"][e], r["abund"][e] = int(t[2]), float(t[3])
+        elif k == "eion":
+            r["eion"][int(t[1]) - 1, int(t[2])] = float(t[3])
+        elif k == "pf":
+            e, z, ty = int(t[1]) - 1, int(t[2]), int(t[3])
+            r["pf_type"][e, z] = ty
+            if ty == 1:
+                r["pf_value"][e, z] = float(t[4])
+            elif ty == 3:
+                n = int(t[4])
+                r["pf_nterm"][e, z] = n
+                r["irwin"][e, z, :n] = [float(x) for x in t[5:5 + n]]
+            elif ty == 2:
+                r["pf_charge"][e, z], r["pf_g0"][e, z], r["pf_nterm"][e, z] = float(t[4]), float(t[5]), int(t[6])
+        elif k == "term":
+            e, z, i = int(t[1]) - 1, int(t[2]), int(t[3])
+            r["term_g2"][e, z, i], r["term_xion"][e, z, i], r["term_xl"][e, z, i] = map(float, t[4:7])
+            m = int(t[8])
+            r["term_ncoef"][e, z, i] = m
+            v = [float(x) for x in t[9:9 + 2 * m]]
+            r["term_gamma"][e, z, i, :m] = v[0::2]
+            r["term_alpha"][e, z, i, :m] = v[1::2]
+        elif k == "lvl":
+            lvl.append([int(t[1]) - 1, int(t[2]), int(t[3]), int(t[4]), float(t[5])])
+        elif k == "mol":
+            m = int(t[1]) - 1
+            r["mol_ncomp"][m], r["mol_eqc"][m], r["mol_ncoef"][m] = int(t[2]), int(t[3]), int(t[4])
+            r["mol_coef"][m, :int(t[4])] = [float(x) for x in t[5:5 + int(t[4])]]
+        elif k == "comp":
+            m, i = int(t[1]) - 1, int(t[2])
+            r["mol_elem"][m, i], r["mol_stage"][m, i] = int(t[3]), int(t[4])
+        elif k == "point":
+            i = pidx[int(t[1])]
+            r["T"][i], r["Nt"][i], r["ne_in"][i] = float(t[2]), float(t[3]), float(t[4])
+        elif k == "U":
+            i, e, z = pidx[int(t[1])], int(t[2]) - 1, int(t[3])
+            r["U"][i, e, z], r["dU"][i, e, z] = float(t[4]), float(t[5])
+        elif k == "frc":
+            i, e, z = pidx[int(t[1])], int(t[2]) - 1, int(t[3])
+            r["fk"][i, e, z], r["df"][i, e, z] = float(t[4]), float(t[5])
+        elif k == "K":
+            r["K"][pidx[int(t[1])], int(t[2]) - 1] = float(t[3])
+        elif k == "ne":
+            r["ne"][pidx[int(t[1])]] = float(t[2])
+        elif k == "Nt":
+            r["Nel"][pidx[int(t[1])], int(t[2]) - 1] = float(t[3])
+        elif k == "N":
+            r["N"][pidx[int(t[1])], int(t[2]) - 1, int(t[3])] = float(t[4])
+        elif k == "n":
+            pops[(pidx[int(t[1])], int(t[2]) - 1, int(t[3]), int(t[4]))] = float(t[5])
+        elif k == "hmin":
+            r["hmin"][pidx[int(t[1])]] = float(t[2])
+    r["levels"] = np.array([row[:4] for row in lvl], np.int32)
+    r["level_ee"] = np.array([row[4] for row in lvl])
+    n = np.zeros((npt, len(lvl)))
+    for j, row in enumerate(lvl):
+        for i in range(npt):
+            n[i, j] = pops[(i, row[0], row[1], row[2])]
+    r["n"] = n
+    return r
+
+
+def build_chemeq(d):
+    """Chemical equilibrium and LTE populations (atmos/atmos_chm.cc:28-363,
+    atom.cc:763-787) for the cfg0 species set (eight background atoms, the
+    two-line Fe atom and the reference's 13 molecules) on two columns: the
+    temperature/pressure sweep of `chemeq_grid_atmosphere` and the cfg0
+    column after the reference's hydrostatic integration."""
+    recs = []
+    for rows, he in ((chemeq_grid_atmosphere(), False), (cfg0_atmosphere(), True)):
+        setup_dir(d, rows, cfg0_lambda())
+        recs.append(parse_chemeq(run(d, ["chemeq", "atmos.cfg"], he=he)))
+    a, b = recs
+    pointwise = ("T", "Nt", "ne_in", "U", "dU", "fk", "df", "K", "ne", "Nel", "N", "hmin", "n")
+    out = {k: (np.concatenate([a[k], b[k]]) if k in pointwise else a[k]) for k in a}
+    for k in a:
+        if k not in pointwise:
+            assert np.array_equal(np.nan_to_num(a[k]), np.nan_to_num(b[k])), k
+    out["ncolumn_points"] = np.array([len(a["T"]), len(b["T"])], np.int32)
+    np.savez_compressed(os.path.join(HERE, "chemeq.npz"), **out)
+    print("chemeq: %d elements, %d molecules, %d levels, %d points" %
+          (len(a["Z"]), len(a["mol_eqc"]), len(a["levels"]), len(out["T"])))
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
@@ -452,6 +584,7 @@ def main():
         build_lmstep(d)
         build_interp(d)
         build_nlte(d)
+        build_chemeq(d)
     finally:
         shutil.rmtree(d)
 

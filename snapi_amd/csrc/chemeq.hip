@@ -50,10 +50,25 @@ constexpr double kRyd = 1.097373156853955E5;
 constexpr double kE0 = 7.957747154594775607E-02;
 
 // Table-derived constants, computed on the host.
+// Stage descriptor of a Traving partition function: its asymptotic terms
+// are nterm consecutive entries from toff of the element's term list.
+struct StageDesc {
+  int nterm;     // -1: the whole U is temperature-only (NONE, CONST, IRWIN)
+  int toff;
+  double zf;     // effective charge times the highest-level constant
+  double rzf;    // its reciprocal
+  double z2;     // 31.321 times the effective charge squared
+};
+
 struct ChemDerived {
   int nst;                            // ionisation stages over all elements
   int soff[kMaxE];                    // first stage of each element in the stage output
+  int roff[kMaxE];                    // element's slots in a point's LDS region
+  int nslot;                          // doubles of a point's LDS region
   double travf;                       // highest-level constant of trav::U (partf.cc:222)
+  StageDesc sd[kMaxE][kMaxS];
+  double xl[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];  // first quantum number of each term
+  double rxl[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS]; // its reciprocal
   double q[kMaxM];                    // molecular charge: sum of component charge indices
   double mul[kMaxM][kTeam];           // multiplicity of unknown j (0: electrons) in molecule m
   int celem[kMaxM][kMaxC];            // component unknown index
@@ -64,21 +79,22 @@ __device__ __forceinline__ double tshfl(double v, int src) { return __shfl(v, sr
 __device__ __forceinline__ int tshfl(int v, int src) { return __shfl(v, src, kTeam); }
 
 // Fischel & Sparks (1971) simplified eqs. (26)/(27) and the derivative in p
-// (trav::qas_f, trav::dqas_f, partf.cc:261-288).
-__device__ __forceinline__ double qas_f(double l, double p, double dz) {
+// (trav::qas_f, trav::dqas_f, partf.cc:261-288), with the divisions by p and
+// l taken as products with their reciprocals rp and rl.
+__device__ __forceinline__ double qas_f(double l, double rl, double p, double rp, double dz) {
   if (p > l) {
     const double llm = l - 1.0;
-    return ((1.3333333 * p + 0.5) * p + 0.16666667 + 1.33333333 * dz) * p - 0.4 * dz * dz / p -
-           ((0.33333333 * llm - 0.5) * llm - 0.16666667 - dz) * llm + 0.5 * dz * dz / l;
+    return ((1.3333333 * p + 0.5) * p + 0.16666667 + 1.33333333 * dz) * p - 0.4 * dz * dz * rp -
+           ((0.33333333 * llm - 0.5) * llm - 0.16666667 - dz) * llm + 0.5 * dz * dz * rl;
   }
-  const double axl2 = dz / (l * l);
-  return (1.0 + axl2 * (0.33333333 + 0.1 * axl2)) * p * p * p * p / l;
+  const double axl2 = dz * (rl * rl);
+  return (1.0 + axl2 * (0.33333333 + 0.1 * axl2)) * p * p * p * p * rl;
 }
 
-__device__ __forceinline__ double dqas_f(double l, double p, double dz) {
-  if (p > l) return ((3.0 * 1.3333333 * p + 1.0) * p + 0.16666667 + 1.33333333 * dz) + 0.4 * dz * dz / (p * p);
-  const double axl2 = dz / (l * l);
-  return 4.0 * (1.0 + axl2 * (0.33333333 + 0.1 * axl2)) * p * p * p / l;
+__device__ __forceinline__ double dqas_f(double l, double rl, double p, double rp, double dz) {
+  if (p > l) return ((3.0 * 1.3333333 * p + 1.0) * p + 0.16666667 + 1.33333333 * dz) + 0.4 * dz * dz * (rp * rp);
+  const double axl2 = dz * (rl * rl);
+  return 4.0 * (1.0 + axl2 * (0.33333333 + 0.1 * axl2)) * p * p * p * rl;
 }
 
 // Temperature-only part of one stage's partition function: the whole U for
@@ -98,21 +114,39 @@ __device__ double pf_base(const snapi_partition &pf, double T, double x) {
   return sum;
 }
 
-// U and dU/dne of one stage at electron density ne (r6 = ne^(1/6),
-// r76 = ne^(7/6)); trav::dU keeps only the last term (partf.cc:236-250).
-__device__ __forceinline__ void pf_eval(const snapi_partition &pf, double base, double x, double f, double r6,
-                                        double r76, double &U, double &dU) {
+// Electron-density powers and reciprocals shared by every stage of one
+// evaluation: ne^(1/6) enters the highest bound level p = z f / ne^(1/6),
+// ne^(7/6) the derivative of trav::U.
+struct NePow {
+  double ne, rne, r6, rr6, k76;  // ne, 1/ne, ne^(1/6), ne^(-1/6), 1/(6 ne^(7/6))
+};
+
+__device__ __forceinline__ NePow ne_powers(double ne) {
+  NePow w;
+  w.ne = ne;
+  w.rne = 1.0 / ne;
+  w.r6 = cbrt(sqrt(ne));
+  w.rr6 = 1.0 / w.r6;
+  w.k76 = w.rne * w.rr6 * (1.0 / 6.0);
+  return w;
+}
+
+// U (and with deriv dU/dne) of one stage from its temperature-only part
+// `base` and, for a Traving stage, the temperature-only weights c of its
+// asymptotic terms; trav::dU keeps only the last term (partf.cc:236-250).
+template <bool kDeriv>
+__device__ __forceinline__ void pf_eval(const StageDesc &d, const double *xl, const double *rxl, double base,
+                                        const double *c, double x, const NePow &w, double &U, double &dU) {
   U = base;
   dU = 0.0;
-  if (pf.type != SNAPI_PF_TRAV) return;
-  const double z = pf.charge;
-  const double p = z * f / r6;
-  const double alpha = 31.321 * z * z * x;
-  const int n = pf.nterm;
+  const int n = d.nterm;
+  if (n < 0) return;
+  const double p = d.zf * w.rr6, rp = w.r6 * d.rzf;
+  const double alpha = d.z2 * x;
   for (int i = 0; i < n; ++i) {
-    const double c = pf.g2[i] * exp(-2.302585 * pf.xion[i] * x);
-    U += c * qas_f(pf.xl[i], p, alpha);
-    if (i == n - 1) dU = -c * dqas_f(pf.xl[i], p, alpha) * z * f / (6.0 * r76);
+    const double l = xl[d.toff + i], rl = rxl[d.toff + i], ci = c[d.toff + i];
+    U += ci * qas_f(l, rl, p, rp, alpha);
+    if (kDeriv && i == n - 1) dU = -ci * dqas_f(l, rl, p, rp, alpha) * d.zf * w.k76;
   }
 }
 
@@ -124,36 +158,44 @@ struct IonSums {
   double fsum, dfsum, ez, dz, f1, d1;
 };
 
-// base/sx: the element's temperature-only partition-function parts and Saha
-// factors, per stage, in this thread's LDS slots.  With store, fk_{z+1}
-// overwrites the Saha factor of stage z once used.
-__device__ IonSums ionfrc(const snapi_chem_element &el, int ns, const double *base, double *sx, double x, double fct,
-                          double f, double ne, bool store) {
-  const double r6 = pow(ne, 1.0 / 6.0), r76 = ne * r6;
+// The element's slots in the point's LDS region R: base[ns] (temperature-only
+// part of U per stage), sx[ns-1] (Saha factor exp(-chi/kT) per stage), then
+// the Traving term weights.  Without kDeriv only fsum, ez and f1 are formed;
+// with store, fk_{z+1} overwrites sx[z] once used.
+template <bool kDeriv>
+__device__ IonSums ionfrc(const StageDesc *sd, const double *xl, const double *rxl, int ns, double *R, double x,
+                          double fct, double ne, bool store) {
+  const NePow w = ne_powers(ne);
+  double *sx = R + ns;
+  const double *c = R + 2 * ns - 1;
   double Up, dUp;
-  pf_eval(el.pf[0], base[0], x, f, r6, r76, Up, dUp);
-  dUp /= Up;
+  pf_eval<kDeriv>(sd[0], xl, rxl, R[0], c, x, w, Up, dUp);
+  double rUp = 1.0 / Up;
+  dUp *= rUp;
   IonSums s{1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double fk = 1.0, dfv = 0.0;
   for (int z = 0; z < ns - 1; ++z) {
     double Un, dUn;
-    pf_eval(el.pf[z + 1], base[(z + 1) * kChemThreads], x, f, r6, r76, Un, dUn);
-    dUn /= Un;
-    const double fz = fct * (Un / Up) * sx[z * kChemThreads] / ne;
-    const double fd = fz * ((dUn) - (dUp * ne + 1.0) / ne);
-    dfv = dfv * fz + fk * fd;
+    pf_eval<kDeriv>(sd[z + 1], xl, rxl, R[z + 1], c, x, w, Un, dUn);
+    const double rUn = 1.0 / Un;
+    const double fz = fct * (Un * rUp) * sx[z] * w.rne;
+    if (kDeriv) {
+      dUn *= rUn;
+      const double fd = fz * ((dUn) - (dUp * ne + 1.0) * w.rne);
+      dfv = dfv * fz + fk * fd;
+      s.dfsum += dfv;
+      s.dz += (z + 1) * dfv;
+      dUp = dUn;
+    }
     fk = fk * fz;
     s.fsum += fk;
-    s.dfsum += dfv;
     s.ez += (z + 1) * fk;
-    s.dz += (z + 1) * dfv;
     if (z == 0) {
       s.f1 = fk;
       s.d1 = dfv;
     }
-    if (store) sx[z * kChemThreads] = fk;
-    Up = Un;
-    dUp = dUn;
+    if (store) sx[z] = fk;
+    rUp = rUn;
   }
   return s;
 }
@@ -186,29 +228,45 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
                                                               double *__restrict__ stage_out,
                                                               double *__restrict__ mol_out,
                                                               double *__restrict__ lev_out, int *__restrict__ it_out) {
-  // per-thread slots [stage][thread]: temperature-only partition-function
-  // part and Saha factor exp(-chi/kT) of this lane's element
-  __shared__ double base_s[kMaxS * kChemThreads];
-  __shared__ double sx_s[kMaxS * kChemThreads];
+  // table descriptors shared by the workgroup, then one region per point
+  __shared__ StageDesc sd_s[kMaxE][kMaxS];
+  __shared__ double xl_s[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];
+  __shared__ double rxl_s[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];
+  extern __shared__ double region_s[];
   const int tid = threadIdx.x;
+  const int na = tb->nelement, nm = tb->nmolecule;
+  for (int i = tid; i < na * kMaxS; i += kChemThreads) sd_s[i / kMaxS][i % kMaxS] = dv->sd[i / kMaxS][i % kMaxS];
+  for (int i = tid; i < na * kMaxS * SNAPI_CHEM_MAX_TERMS; i += kChemThreads) {
+    const int a = i / (kMaxS * SNAPI_CHEM_MAX_TERMS), t = i % (kMaxS * SNAPI_CHEM_MAX_TERMS);
+    xl_s[a][t] = dv->xl[a][t];
+    rxl_s[a][t] = dv->rxl[a][t];
+  }
+  __syncthreads();
   const int lane = tid & (kTeam - 1);
   const long pt = ((long)blockIdx.x * blockDim.x + tid) / kTeam;
-  if (pt >= npt) return;  // the whole team leaves together; no block barrier follows
-  double *base = base_s + tid, *sx = sx_s + tid;
-  const int na = tb->nelement, nm = tb->nmolecule;
+  if (pt >= npt) return;  // the whole team leaves together; no barrier follows
   const double T = Tin[pt], Nt = Ntin[pt];
-  const double x = 5040.0 / T, kT = kK * T, f = dv->travf;
+  const double x = 5040.0 / T, kT = kK * T;
   const double fct = 2.0 * pow((2.0 * kPi * kMe * kK * T) / (kH * kH), 1.5);
   const bool is_el = lane >= 1 && lane <= na;
   const bool is_unk = lane <= na;
-  const snapi_chem_element &el = tb->element[is_el ? lane - 1 : 0];
+  const int ea = is_el ? lane - 1 : 0;
+  const snapi_chem_element &el = tb->element[ea];
   const int ns = is_el ? el.nstage : 1;
   const double ab = is_el ? el.abundance : 0.0;
   const double ab1 = tb->element[0].abundance;
+  const StageDesc *sd = sd_s[ea];
+  const double *xl = xl_s[ea], *rxl = rxl_s[ea];
+  double *R = region_s + (tid / kTeam) * dv->nslot + dv->roff[ea];
   if (is_el) {
+    // temperature-only parts: explicit levels of every stage, Saha factors,
+    // and the weights of the Traving asymptotic terms
     for (int z = 0; z < ns; ++z) {
-      base[z * kChemThreads] = pf_base(el.pf[z], T, x);
-      if (z < ns - 1) sx[z * kChemThreads] = exp(-el.eion[z] / (kK * T));
+      const snapi_partition &pf = el.pf[z];
+      R[z] = pf_base(pf, T, x);
+      if (z < ns - 1) R[ns + z] = exp(-el.eion[z] / (kK * T));
+      if (pf.type == SNAPI_PF_TRAV)
+        for (int i = 0; i < pf.nterm; ++i) R[2 * ns - 1 + sd[z].toff + i] = pf.g2[i] * exp(-2.302585 * pf.xion[i] * x);
     }
   }
   double Km[2];
@@ -225,7 +283,12 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
   double ne_eval = 1.0, F1 = 0.0, N0 = 0.0, Nown = 0.0;
   IonSums s{1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (;;) {
-    if (is_el) s = ionfrc(el, ns, base, sx, x, fct, f, ne_eval, phase == kFinal);
+    if (is_el) {
+      if (phase == kNewton)
+        s = ionfrc<true>(sd, xl, rxl, ns, R, x, fct, ne_eval, false);
+      else
+        s = ionfrc<false>(sd, xl, rxl, ns, R, x, fct, ne_eval, phase == kFinal);
+    }
     if (phase == kFinal) break;
     const double rf = 1.0 / s.fsum;
     if (phase == kAtOne || phase == kAtNt) {
@@ -442,7 +505,7 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
   if (stage_out && is_el) {
     const int off = dv->soff[lane - 1], nst = dv->nst;
     for (int z = 0; z < ns; ++z)
-      stage_out[((long)c * nst + off + z) * nd + zd] = Nown * ((z ? sx[(z - 1) * kChemThreads] : 1.0) * rf);
+      stage_out[((long)c * nst + off + z) * nd + zd] = Nown * ((z ? R[ns + z - 1] : 1.0) * rf);
   }
   if (mol_out) {
     const double fk0 = is_el ? rf : 1.0, fk1 = is_el ? s.f1 * rf : 0.0;
@@ -465,14 +528,14 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
   }
   if (lev_out && is_el) {
     const int nl = tb->nlevel;
-    const double r6 = pow(N0, 1.0 / 6.0), r76 = N0 * r6;
+    const NePow w = ne_powers(N0);
     for (int l = 0; l < nl; ++l) {
       const snapi_chem_level &L = tb->level[l];
       if (L.element != lane) continue;
       const int z = L.stage;
-      const double Nz = Nown * ((z ? sx[(z - 1) * kChemThreads] : 1.0) * rf);
+      const double Nz = Nown * ((z ? R[ns + z - 1] : 1.0) * rf);
       double U, dU;
-      pf_eval(el.pf[z], base[z * kChemThreads], x, f, r6, r76, U, dU);
+      pf_eval<false>(sd[z], xl, rxl, R[z], R + 2 * ns - 1, x, w, U, dU);
       lev_out[((long)c * nl + l) * nd + zd] = Nz * ((L.g != 0.0) ? L.g * exp(-L.energy / kT) / U : 1.0);
     }
   }
@@ -505,6 +568,32 @@ int derive(const snapi_chem_table *tb, ChemDerived &d) {
     nst += el.nstage;
   }
   d.nst = nst;
+  const double er = kE * kRyd * kRyd;
+  d.travf = 1.405169521381833474E+05 * ((4.0 * kPi * kE0) / (er * er));
+  int slot = 0;
+  for (int a = 0; a < tb->nelement; ++a) {
+    const snapi_chem_element &el = tb->element[a];
+    int nt = 0;
+    for (int z = 0; z < el.nstage; ++z) {
+      const snapi_partition &pf = el.pf[z];
+      StageDesc &sd = d.sd[a][z];
+      sd.nterm = -1;
+      if (pf.type != SNAPI_PF_TRAV) continue;
+      sd.nterm = pf.nterm;
+      sd.toff = nt;
+      sd.zf = pf.charge * d.travf;
+      sd.rzf = 1.0 / sd.zf;
+      sd.z2 = 31.321 * pf.charge * pf.charge;
+      for (int i = 0; i < pf.nterm; ++i) {
+        d.xl[a][nt + i] = pf.xl[i];
+        d.rxl[a][nt + i] = 1.0 / pf.xl[i];
+      }
+      nt += pf.nterm;
+    }
+    d.roff[a] = slot;
+    slot += 2 * el.nstage - 1 + nt;
+  }
+  d.nslot = slot;
   if (!(tb->element[0].abundance > 0.0)) return SNAPI_ERR_UNSUPPORTED;
   for (int m = 0; m < tb->nmolecule; ++m) {
     const snapi_chem_molecule &M = tb->molecule[m];
@@ -528,18 +617,18 @@ int derive(const snapi_chem_table *tb, ChemDerived &d) {
         L.stage >= tb->element[L.element - 1].nstage)
       return SNAPI_ERR_UNSUPPORTED;
   }
-  const double er = kE * kRyd * kRyd;
-  d.travf = 1.405169521381833474E+05 * ((4.0 * kPi * kE0) / (er * er));
   return SNAPI_OK;
 }
 
-int launch_chemeq(const snapi_chem_table *dtb, const ChemDerived *ddv, long npt, int nd, const double *T,
+int launch_chemeq(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot, long npt, int nd, const double *T,
                   const double *Nt, double *ne, double *stage, double *mol, double *lev, int *it, hipStream_t st) {
   const long threads = npt * kTeam;
   const long blocks = (threads + kChemThreads - 1) / kChemThreads;
   if (blocks > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(chemeq_kernel, dim3((unsigned)blocks), dim3(kChemThreads), 0, st, dtb, ddv, npt, T, Nt, nd, ne,
-                     stage, mol, lev, it);
+  const size_t lds = (size_t)(kChemThreads / kTeam) * nslot * sizeof(double);
+  if (lds > 48 * 1024) return SNAPI_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(chemeq_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, npt, T, Nt, nd,
+                     ne, stage, mol, lev, it);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
@@ -580,7 +669,7 @@ extern "C" int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd,
   rc = upload_table(table, d, &buf, st);
   if (rc) return rc;
   rc = launch_chemeq((const snapi_chem_table *)buf,
-                     (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), (long)ncol * nd, nd, T_dev,
+                     (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), d.nslot, (long)ncol * nd, nd, T_dev,
                      Nt_dev, ne_dev, stage_dev, table->nmolecule ? molecule_dev : nullptr,
                      table->nlevel ? level_dev : nullptr, iterations_dev, st);
   // the table buffer is freed once the kernel that reads it has finished
@@ -612,7 +701,7 @@ extern "C" int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, con
   rc = upload_table(table, d, &buf, nullptr);
   if (rc) goto done;
   rc = launch_chemeq((const snapi_chem_table *)buf,
-                     (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), (long)np, nd, dT, dNt, dne,
+                     (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), d.nslot, (long)np, nd, dT, dNt, dne,
                      dst, dmo, dlv, dit, nullptr);
   if (rc) goto done;
   CCHK(hipDeviceSynchronize());

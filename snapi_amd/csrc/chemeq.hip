@@ -24,7 +24,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 #include "rt_common.h"
 
@@ -33,6 +36,7 @@ namespace {
 constexpr int kTeam = 16;
 constexpr int kChemThreads = 256;
 constexpr int kMaxAverage = 1000;  // passes of the starting-value averaging
+constexpr int kMaxSched = 12;      // Traving stage evaluations per lane
 constexpr int kMaxS = SNAPI_CHEM_MAX_STAGES;
 constexpr int kMaxE = SNAPI_CHEM_MAX_ELEMENTS;
 constexpr int kMaxM = SNAPI_CHEM_MAX_MOLECULES;
@@ -64,7 +68,10 @@ struct ChemDerived {
   int nst;                            // ionisation stages over all elements
   int soff[kMaxE];                    // first stage of each element in the stage output
   int roff[kMaxE];                    // element's slots in a point's LDS region
+  int uoff[kMaxE];                    // U, 1/U and dlnU/dne tables within them
   int nslot;                          // doubles of a point's LDS region
+  int nsched[kTeam];                  // Traving stages evaluated by each lane of a team
+  int sched[kTeam][kMaxSched];        // element * kMaxS + stage
   double travf;                       // highest-level constant of trav::U (partf.cc:222)
   StageDesc sd[kMaxE][kMaxS];
   double xl[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];  // first quantum number of each term
@@ -158,30 +165,55 @@ struct IonSums {
   double fsum, dfsum, ez, dz, f1, d1;
 };
 
-// The element's slots in the point's LDS region R: base[ns] (temperature-only
-// part of U per stage), sx[ns-1] (Saha factor exp(-chi/kT) per stage), then
-// the Traving term weights.  Without kDeriv only fsum, ez and f1 are formed;
-// with store, fk_{z+1} overwrites sx[z] once used.
+// Makes one lane's LDS writes visible to the other lanes of its wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// An element's slots in the point's LDS region R: base[ns] (temperature-only
+// part of U per stage), sx[ns-1] (Saha factor exp(-chi/kT) per stage), the
+// Traving term weights, then from uoff the tables U[ns], 1/U[ns] and
+// dlnU/dne[ns] at the current electron density.
+
+// The lanes of a team evaluate the Traving stages of all elements (the
+// schedule spreads their terms evenly over the 16 lanes) into the U tables.
 template <bool kDeriv>
-__device__ IonSums ionfrc(const StageDesc *sd, const double *xl, const double *rxl, int ns, double *R, double x,
-                          double fct, double ne, bool store) {
-  const NePow w = ne_powers(ne);
+__device__ __forceinline__ void eval_stages(const ChemDerived *dv, const StageDesc (*sd)[kMaxS],
+                                            const double (*xl)[kMaxS * SNAPI_CHEM_MAX_TERMS],
+                                            const double (*rxl)[kMaxS * SNAPI_CHEM_MAX_TERMS], const int *nsched,
+                                            const int (*sched)[kMaxSched], const int *ns_of, int lane, double *team,
+                                            double x, const NePow &w) {
+  const int n = nsched[lane];
+  for (int k = 0; k < n; ++k) {
+    const int a = sched[lane][k] / kMaxS, z = sched[lane][k] % kMaxS, ns = ns_of[a];
+    double *Ra = team + dv->roff[a];
+    double U, dU;
+    pf_eval<kDeriv>(sd[a][z], xl[a], rxl[a], Ra[z], Ra + 2 * ns - 1, x, w, U, dU);
+    double *Ut = Ra + dv->uoff[a];
+    const double rU = 1.0 / U;
+    Ut[z] = U;
+    Ut[ns + z] = rU;
+    if (kDeriv) Ut[2 * ns + z] = dU * rU;
+  }
+}
+
+// atom::ionfrc (atom.cc:892-924) from the U tables: the unnormalised
+// fractions' recursion over the stages.  Without kDeriv only fsum, ez and
+// f1 are formed; with store, fk_{z+1} overwrites sx[z] once used.
+template <bool kDeriv>
+__device__ IonSums ionfrc(int ns, double *R, const double *Ut, double fct, const NePow &w, bool store) {
   double *sx = R + ns;
-  const double *c = R + 2 * ns - 1;
-  double Up, dUp;
-  pf_eval<kDeriv>(sd[0], xl, rxl, R[0], c, x, w, Up, dUp);
-  double rUp = 1.0 / Up;
-  dUp *= rUp;
+  double rUp = Ut[ns], dUp = kDeriv ? Ut[2 * ns] : 0.0;
   IonSums s{1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double fk = 1.0, dfv = 0.0;
   for (int z = 0; z < ns - 1; ++z) {
-    double Un, dUn;
-    pf_eval<kDeriv>(sd[z + 1], xl, rxl, R[z + 1], c, x, w, Un, dUn);
-    const double rUn = 1.0 / Un;
+    const double Un = Ut[z + 1];
     const double fz = fct * (Un * rUp) * sx[z] * w.rne;
     if (kDeriv) {
-      dUn *= rUn;
-      const double fd = fz * ((dUn) - (dUp * ne + 1.0) * w.rne);
+      const double dUn = Ut[2 * ns + z + 1];
+      const double fd = fz * ((dUn) - (dUp * w.ne + 1.0) * w.rne);
       dfv = dfv * fz + fk * fd;
       s.dfsum += dfv;
       s.dz += (z + 1) * dfv;
@@ -195,7 +227,7 @@ __device__ IonSums ionfrc(const StageDesc *sd, const double *xl, const double *r
       s.d1 = dfv;
     }
     if (store) sx[z] = fk;
-    rUp = rUn;
+    rUp = Ut[ns + z + 1];
   }
   return s;
 }
@@ -232,6 +264,7 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
   __shared__ StageDesc sd_s[kMaxE][kMaxS];
   __shared__ double xl_s[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];
   __shared__ double rxl_s[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];
+  __shared__ int nsched_s[kTeam], sched_s[kTeam][kMaxSched], ns_s[kMaxE];
   extern __shared__ double region_s[];
   const int tid = threadIdx.x;
   const int na = tb->nelement, nm = tb->nmolecule;
@@ -241,6 +274,11 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
     xl_s[a][t] = dv->xl[a][t];
     rxl_s[a][t] = dv->rxl[a][t];
   }
+  if (tid < kTeam) {
+    nsched_s[tid] = dv->nsched[tid];
+    for (int k = 0; k < kMaxSched; ++k) sched_s[tid][k] = dv->sched[tid][k];
+  }
+  if (tid < na) ns_s[tid] = tb->element[tid].nstage;
   __syncthreads();
   const int lane = tid & (kTeam - 1);
   const long pt = ((long)blockIdx.x * blockDim.x + tid) / kTeam;
@@ -257,7 +295,9 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
   const double ab1 = tb->element[0].abundance;
   const StageDesc *sd = sd_s[ea];
   const double *xl = xl_s[ea], *rxl = rxl_s[ea];
-  double *R = region_s + (tid / kTeam) * dv->nslot + dv->roff[ea];
+  double *team = region_s + (tid / kTeam) * dv->nslot;
+  double *R = team + dv->roff[ea];
+  double *Ut = R + dv->uoff[ea];
   if (is_el) {
     // temperature-only parts: explicit levels of every stage, Saha factors,
     // and the weights of the Traving asymptotic terms
@@ -265,10 +305,17 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
       const snapi_partition &pf = el.pf[z];
       R[z] = pf_base(pf, T, x);
       if (z < ns - 1) R[ns + z] = exp(-el.eion[z] / (kK * T));
-      if (pf.type == SNAPI_PF_TRAV)
+      if (pf.type == SNAPI_PF_TRAV) {
         for (int i = 0; i < pf.nterm; ++i) R[2 * ns - 1 + sd[z].toff + i] = pf.g2[i] * exp(-2.302585 * pf.xion[i] * x);
+      } else {
+        // U of the other stages depends on T alone
+        Ut[z] = R[z];
+        Ut[ns + z] = 1.0 / R[z];
+        Ut[2 * ns + z] = 0.0;
+      }
     }
   }
+  wave_lds_sync();
   double Km[2];
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
@@ -283,11 +330,24 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
   double ne_eval = 1.0, F1 = 0.0, N0 = 0.0, Nown = 0.0;
   IonSums s{1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (;;) {
-    if (is_el) {
-      if (phase == kNewton)
-        s = ionfrc<true>(sd, xl, rxl, ns, R, x, fct, ne_eval, false);
-      else
-        s = ionfrc<false>(sd, xl, rxl, ns, R, x, fct, ne_eval, phase == kFinal);
+    // The four teams of a wavefront run their Newton steps together: a team
+    // that has its starting value waits while another team of its wavefront
+    // is still averaging, so that the Newton code is issued once per step
+    // for all of them rather than once per team.
+    const bool before = __ballot(phase < kNewton) != 0;
+    if (phase == kNewton && before) continue;
+    {
+      const NePow w = ne_powers(ne_eval);
+      if (phase == kNewton) {
+        eval_stages<true>(dv, sd_s, xl_s, rxl_s, nsched_s, sched_s, ns_s, lane, team, x, w);
+        wave_lds_sync();
+        if (is_el) s = ionfrc<true>(ns, R, Ut, fct, w, false);
+      } else {
+        eval_stages<false>(dv, sd_s, xl_s, rxl_s, nsched_s, sched_s, ns_s, lane, team, x, w);
+        wave_lds_sync();
+        if (is_el) s = ionfrc<false>(ns, R, Ut, fct, w, phase == kFinal);
+      }
+      wave_lds_sync();
     }
     if (phase == kFinal) break;
     const double rf = 1.0 / s.fsum;
@@ -591,9 +651,27 @@ int derive(const snapi_chem_table *tb, ChemDerived &d) {
       nt += pf.nterm;
     }
     d.roff[a] = slot;
-    slot += 2 * el.nstage - 1 + nt;
+    d.uoff[a] = 2 * el.nstage - 1 + nt;
+    slot += 5 * el.nstage - 1 + nt;
   }
   d.nslot = slot;
+  // Traving stages to lanes, longest first onto the least-loaded lane
+  std::vector<std::pair<int, int>> st;  // (terms, element * kMaxS + stage)
+  for (int a = 0; a < tb->nelement; ++a)
+    for (int z = 0; z < tb->element[a].nstage; ++z)
+      if (d.sd[a][z].nterm >= 0) st.push_back({d.sd[a][z].nterm + 1, a * kMaxS + z});
+  std::stable_sort(st.begin(), st.end(), [](const std::pair<int, int> &u, const std::pair<int, int> &v) {
+    return u.first > v.first;
+  });
+  int load[kTeam] = {0};
+  for (const auto &e : st) {
+    int best = 0;
+    for (int l = 1; l < kTeam; ++l)
+      if (load[l] < load[best]) best = l;
+    if (d.nsched[best] >= kMaxSched) return SNAPI_ERR_UNSUPPORTED;
+    d.sched[best][d.nsched[best]++] = e.second;
+    load[best] += e.first;
+  }
   if (!(tb->element[0].abundance > 0.0)) return SNAPI_ERR_UNSUPPORTED;
   for (int m = 0; m < tb->nmolecule; ++m) {
     const snapi_chem_molecule &M = tb->molecule[m];
@@ -626,7 +704,7 @@ int launch_chemeq(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot
   const long blocks = (threads + kChemThreads - 1) / kChemThreads;
   if (blocks > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
   const size_t lds = (size_t)(kChemThreads / kTeam) * nslot * sizeof(double);
-  if (lds > 48 * 1024) return SNAPI_ERR_UNSUPPORTED;
+  if (lds > 44 * 1024) return SNAPI_ERR_UNSUPPORTED;  // with the static tables, within 64 KiB
   hipLaunchKernelGGL(chemeq_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, npt, T, Nt, nd,
                      ne, stage, mol, lev, it);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;

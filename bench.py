@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--no-lm", action="store_true")
     ap.add_argument("--ali-side", type=int, default=64, help="ALI leg: side of the column square")
     ap.add_argument("--no-ali", action="store_true")
+    ap.add_argument("--chem-side", type=int, default=256, help="chemeq leg: side of the column square")
+    ap.add_argument("--no-chem", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the hot path is per column and shards by column tiles; "
@@ -174,6 +176,10 @@ def main():
     if not args.no_ali:
         alires = ali_leg(torch, dev, stream, args.ali_side, not args.no_cpu)
 
+    chem = None
+    if not args.no_chem:
+        chem = chemeq_leg(torch, dev, stream, args.chem_side, not args.no_cpu)
+
     line = {"metric": METRIC, "value": value, "unit": "lambda-point formal solves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
@@ -181,8 +187,69 @@ def main():
                                    "mu=1, Fe I 6301.5/6302.5 + H lines + H- continuum, mixed FP32/FP64"
                                    % (nx, ny, nd, nlam),
                        "columns": ncol, "depths": nd, "wavelengths": nlam, "lines": table.nline},
-            "roofline": roof, "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64, "lm": lm, "ali": alires}
+            "roofline": roof, "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64, "lm": lm, "ali": alires,
+            "chemeq": chem}
     print(json.dumps(line))
+
+
+def chemeq_leg(torch, dev, stream, side, with_cpu):
+    from snapi_amd import abi, chemeq, workloads
+    t0 = time.time()
+    p = workloads.chemeq_problem(side, side)
+    tb = p["table"]
+    ncol, nd = p["T"].shape
+    npt = ncol * nd
+    nst = abi.chem_stage_count(tb)
+    log("chemeq workload: %d columns x %d depths, %d elements (%d stages), %d molecules, %d levels, built in %.1fs"
+        % (ncol, nd, tb.nelement, nst, tb.nmolecule, tb.nlevel, time.time() - t0))
+    T = torch.from_numpy(p["T"]).to(dev)
+    Nt = torch.from_numpy(p["Nt"]).to(dev)
+    ne = torch.empty((ncol, nd), dtype=torch.float64, device=dev)
+    st = torch.empty((ncol, nst, nd), dtype=torch.float64, device=dev)
+    mo = torch.empty((ncol, tb.nmolecule, nd), dtype=torch.float64, device=dev)
+    lv = torch.empty((ncol, tb.nlevel, nd), dtype=torch.float64, device=dev)
+    it = torch.empty((ncol, nd), dtype=torch.int32, device=dev)
+
+    def call():
+        chemeq.chemeq_dev(tb, T, Nt, ne, st, mo, lv, it, stream=stream)
+
+    call()  # kernel loading, not timed
+    torch.cuda.synchronize()
+    reps = 5
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    itn = it.cpu().numpy()
+    res = {"points": npt, "ms_per_call": round(ms, 3), "points_per_s": npt / (ms * 1e-3),
+           "mean_newton_steps": float(itn.mean()), "max_newton_steps": int(itn.max()),
+           "singular": int(np.sum(itn < 0))}
+    log("chemeq: %d points in %.2f ms = %.4g points/s; Newton steps mean %.2f max %d"
+        % (npt, ms, res["points_per_s"], itn.mean(), itn.max()))
+    cpu = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import _oracle  # the FP64 CPU restatement: baseline and checker only
+        nthreads = max(1, min(16, os.cpu_count() or 1))
+        nc = min(ncol, 2048)
+        t1 = time.time()
+        ne_c, st_c, mo_c, lv_c, it_c = _oracle.chemeq(tb, p["T"][:nc], p["Nt"][:nc], nthreads=nthreads)
+        dt = time.time() - t1
+        cpu = {"value": nc * nd / dt, "unit": "depth points/s", "cores": nthreads, "kind": "port",
+               "sample": "%d columns x %d depths of the same workload (FP64 oracle, OpenMP)" % (nc, nd),
+               "seconds": round(dt, 2)}
+        g_ne = ne[:nc].cpu().numpy()
+        res["ne_rel_err_vs_oracle"] = float(np.max(np.abs(g_ne / ne_c - 1.0)))
+        res["steps_equal_fraction"] = float(np.mean(itn[:nc] == it_c))
+        log("chemeq cpu oracle: %.4g points/s on %d threads; GPU vs oracle Ne max rel %.2e, steps equal %.4f"
+            % (cpu["value"], nthreads, res["ne_rel_err_vs_oracle"], res["steps_equal_fraction"]))
+    return {"workload": "chemical equilibrium + LTE populations, %d columns x %d depths, %d elements, %d molecules, "
+                        "%d levels, FP64" % (ncol, nd, tb.nelement, tb.nmolecule, tb.nlevel),
+            "value": res["points_per_s"], "unit": "depth points/s", "fp64": res, "cpu_baseline": cpu}
 
 
 def ali_leg(torch, dev, stream, side, with_cpu):

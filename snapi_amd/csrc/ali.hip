@@ -89,15 +89,15 @@ template <typename R> struct AliTile {
 };
 
 // opacity, emissivity and line profiles of one depth for one wavelength
-template <typename R>
+template <typename R, int kL, int kB>
 __device__ __forceinline__ void ali_opem(const AliAtom &A, const AliTile<R> &t, int nd, int z, double lam, R lamR,
-                                         R hcl, R plc, R ple, R fud, const R sg[kAliMaxBf], R bo, R be, R &op,
-                                         R &em, R prof[kAliMaxLines]) {
+                                         R hcl, R plc, R ple, R fud, const R sg[kB], R bo, R be, R &op, R &em,
+                                         R prof[kL]) {
   R o = bo, e = be;
   const R ex = rexp<R>(-hcl * t.tinv[z]);  // exp(-h c / (lambda k T))
   const R B = plc * rrcp<R>(rexp<R>(ple * t.tinv[z]) - R(1));
 #pragma unroll
-  for (int b = 0; b < kAliMaxBf; ++b) {
+  for (int b = 0; b < kB; ++b) {
     if (b >= A.nbf) continue;
     const R sig = sg[b];
     if (sig == R(0)) continue;
@@ -107,7 +107,7 @@ __device__ __forceinline__ void ali_opem(const AliAtom &A, const AliTile<R> &t, 
   }
   const R hc4l = R(kH * kC / (4.0 * kPi)) * rrcp<R>(lamR);
 #pragma unroll
-  for (int j = 0; j < kAliMaxLines; ++j) {
+  for (int j = 0; j < kL; ++j) {
     prof[j] = R(0);
     if (j >= A.nline) continue;
     const int i = j * nd + z;
@@ -157,15 +157,22 @@ template <> struct AliNorm<float> {
   }
 };
 
-template <typename R>
+// With kLds the column's sums stay in LDS, [depth][3 per line, 2 per
+// bound-free], and reach the accumulators in global memory once, after the
+// last ray; the additions happen in the same order either way.  kL and kB
+// bound the atom's lines and bound-free transitions the work-items carry in
+// registers, kS (32 or 64) the reduced slots per depth.
+template <typename R, bool kLds, int kL, int kB, int kS>
 __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, const AliGridArgs G, int nd, AliCol P) {
   extern __shared__ unsigned char smem_raw[];
   const int col = blockIdx.x;
   if (P.active && !P.active[col]) return;  // uniform over the workgroup
   const int nline = A.nline, nbf = A.nbf, nlam = G.nlam;
   const int nwave = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x & 63;
-  double *red = reinterpret_cast<double *>(smem_raw);  // [2][nwave][kAliSlots]
-  R *sm = reinterpret_cast<R *>(smem_raw + sizeof(double) * 2 * nwave * kAliSlots);
+  double *red = reinterpret_cast<double *>(smem_raw);  // [2][nwave][kS]
+  const int nsum = 3 * nline + 2 * nbf;
+  double *sums = red + 2 * nwave * kS;  // [nd][nsum] with kLds
+  R *sm = reinterpret_cast<R *>(sums + (kLds ? (size_t)nd * nsum : 0));
   AliTile<R> t = AliTile<R>::carve(sm, nd, nline, nbf);
   const double *cs = P.cs + (size_t)col * SNAPI_A_NF * nd;
   const double *pops = P.pops + (size_t)col * nd * A.nlevel;
@@ -191,6 +198,8 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
     t.emf[i] = (R)(nup * A.aul[j]);
     t.accf[i] = (R)((nlo * A.blu[j] - nup * A.bul[j]) * A.energy[j] * 0.25 / kPi);
   }
+  if (kLds)
+    for (int i = threadIdx.x; i < nd * nsum; i += blockDim.x) sums[i] = 0.0;
   for (int i = threadIdx.x; i < nd * nbf; i += blockDim.x) {
     const int b = i / nd, z = i - b * nd;
     const double T = cs[SNAPI_A_T * nd + z], ne = cs[SNAPI_A_NE * nd + z];
@@ -221,22 +230,23 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
       const R fud = (R)P.fudge[lc];
       const double ww = valid ? P.lamw[lc] * G.wray[r] : 0.0;
       // per-wavelength constants: cross-sections, 2 h c^2 / lambda^5, h c / (lambda k)
-      R sg[kAliMaxBf];
-      double sgd[kAliMaxBf];
+      R sg[kB];
+      double sw[kB];  // weight x cross-section x lambda / (h c) of the photo-/recombination integrals
 #pragma unroll
-      for (int b = 0; b < kAliMaxBf; ++b) {
-        sgd[b] = b < nbf ? P.sigma[(size_t)A.bf_t[b] * nlam + lc] : 0.0;
-        sg[b] = (R)sgd[b];
+      for (int b = 0; b < kB; ++b) {
+        const double sgd = b < nbf ? P.sigma[(size_t)A.bf_t[b] * nlam + lc] : 0.0;
+        sg[b] = (R)sgd;
+        sw[b] = ww * sgd * lam / kH / kC;
       }
       const double c5 = 2.0 * kH * kC * kC / pow(lam, 5.0);
       const double hclk = kH * kC / lam / kK;
-      R profc[kAliMaxLines], profn[kAliMaxLines];
+      R profc[kL], profn[kL];
       R opc, emc, opn, emn;
       // depth 0 and the look-ahead depth 1
       const int z0 = zof(0), z1 = zof(1);
-      ali_opem<R>(A, t, nd, z0, lam, lamR, hcl, plc, ple, fud, sg, (R)bop[(size_t)z0 * nlam + lc],
+      ali_opem<R, kL, kB>(A, t, nd, z0, lam, lamR, hcl, plc, ple, fud, sg, (R)bop[(size_t)z0 * nlam + lc],
                   (R)bem[(size_t)z0 * nlam + lc], opc, emc, profc);
-      ali_opem<R>(A, t, nd, z1, lam, lamR, hcl, plc, ple, fud, sg, (R)bop[(size_t)z1 * nlam + lc],
+      ali_opem<R, kL, kB>(A, t, nd, z1, lam, lamR, hcl, plc, ple, fud, sg, (R)bop[(size_t)z1 * nlam + lc],
                   (R)bem[(size_t)z1 * nlam + lc], opn, emn, profn);
       const bool tg = G.tau_grid != 0;
       R oom, oo0, Sm, S0;
@@ -254,10 +264,10 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
         if (k > 0) {
           // optical-depth step k-1 -> k (D of the last point one-sided)
           R D0, oop = R(0), hp = R(1), Sp = R(0), opp = R(0), emp = R(0);
-          R profp[kAliMaxLines];
+          R profp[kL];
           if (k < nd - 1) {
             const int zn = zof(k + 1);
-            ali_opem<R>(A, t, nd, zn, lam, lamR, hcl, plc, ple, fud, sg,
+            ali_opem<R, kL, kB>(A, t, nd, zn, lam, lamR, hcl, plc, ple, fud, sg,
                         (R)bop[(size_t)zn * nlam + lc], (R)bem[(size_t)zn * nlam + lc], opp, emp, profp);
             AliNorm<R>::apply(tg, opp, emp, t.opref[zn], oop, Sp);
             hp = (t.grid[zn] - t.grid[z]) / mct;
@@ -273,7 +283,7 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
           Ls = w1;
           opd = (tg && sizeof(R) == 8) ? oo0 * t.opref[z] : opn;
 #pragma unroll
-          for (int j = 0; j < kAliMaxLines; ++j) profc[j] = profn[j];
+          for (int j = 0; j < kL; ++j) profc[j] = profn[j];
           // shift the window: depth k+1 becomes the look-ahead of the next step
           oom = oo0;
           oo0 = oop;
@@ -283,20 +293,20 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
           h0 = hp;
           opn = opp;
 #pragma unroll
-          for (int j = 0; j < kAliMaxLines; ++j) profn[j] = profp[j];
+          for (int j = 0; j < kL; ++j) profn[j] = profp[j];
         }
         // contributions of this wavelength to the sums of depth z, in fixed
         // slots: line j at 3j (J), 3j+1 (Lambda*), 3j+2 (norm); bound-free b
-        // at 62-2b (photoionisation) and 63-2b (recombination)
+        // at kS-2-2b (photoionisation) and kS-1-2b (recombination)
         const int buf = k & 1;
-        double *rb = red + (size_t)buf * nwave * kAliSlots;
+        double *rb = red + (size_t)buf * nwave * kS;
         const R iop = rrcp<R>(opd);
-        R v[kAliSlots];
+        R v[kS];
 #pragma unroll
-        for (int q = 0; q < kAliSlots; ++q) v[q] = R(0);
+        for (int q = 0; q < kS; ++q) v[q] = R(0);
         const R wwr = (R)ww;
 #pragma unroll
-        for (int j = 0; j < kAliMaxLines; ++j) {
+        for (int j = 0; j < kL; ++j) {
           if (j >= nline) continue;
           const R p = profc[j];
           // line_factor = (n_l B_lu - n_u B_ul) phi E / (4 pi op); weight phi line_factor / (4 pi)
@@ -305,20 +315,33 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
           v[3 * j + 1] = cf * Ls;
           v[3 * j + 2] = wwr * p * R(0.25 / kPi);
         }
-        const double exd = nbf ? exp(-hclk * (double)t.tinv[z]) : 0.0;
+        // exp(-h c / (lambda k T)): in double on the double path, in float
+        // (whose result the float path rounds to anyway) on the mixed one
+        double exd = 0.0;
+        if (nbf) {
+          if constexpr (sizeof(R) == 8)
+            exd = exp(-hclk * (double)t.tinv[z]);
+          else
+            exd = (double)rexp<R>(-hcl * t.tinv[z]);
+        }
 #pragma unroll
-        for (int b = 0; b < kAliMaxBf; ++b) {
+        for (int b = 0; b < kB; ++b) {
           if (b >= nbf) continue;
-          const double s = ww * sgd[b] * lam / kH / kC;
+          const double s = sw[b];
           const double Id = (double)I;
-          v[kAliSlots - 2 - 2 * b] = (R)(s * Id);
-          v[kAliSlots - 1 - 2 * b] = (R)(s * (Id + c5) * exd);
+          v[kS - 2 - 2 * b] = (R)(s * Id);
+          v[kS - 1 - 2 * b] = (R)(s * (Id + c5) * exd);
         }
         // butterfly reduction over the wavefront: at each step a lane keeps
-        // half of its slots and adds its partner's copy of them; after six
-        // steps lane L holds the wavefront's sum of slot L
+        // half of its slots and adds its partner's copy of them; at the end
+        // lane L holds the wavefront's sum of slot L mod kS (with 32 slots the
+        // two half-wavefronts are added first)
+        if (kS < 64) {
 #pragma unroll
-        for (int o = 32, half = kAliSlots / 2; o >= 1; o >>= 1, half >>= 1) {
+          for (int q = 0; q < kS; ++q) v[q] += __shfl_xor(v[q], 32, 64);
+        }
+#pragma unroll
+        for (int o = kS / 2, half = kS / 2; o >= 1; o >>= 1, half >>= 1) {
           const bool upper = (lane & o) != 0;
 #pragma unroll
           for (int q = 0; q < half; ++q) {
@@ -327,21 +350,26 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
             v[q] = (upper ? hi : lo) + recv;
           }
         }
-        rb[wave * kAliSlots + lane] = (double)v[0];
+        if (lane < kS) rb[wave * kS + lane] = (double)v[0];
         __syncthreads();
-        for (int qq = threadIdx.x; qq < kAliSlots; qq += blockDim.x) {
+        for (int qq = threadIdx.x; qq < kS; qq += blockDim.x) {
           int tr = -1, slot = 0;
           if (qq < 3 * nline) {
             tr = A.line_t[qq / 3];
             slot = qq % 3;
-          } else if (qq >= kAliSlots - 2 * nbf) {
-            const int r2 = kAliSlots - 1 - qq;
+          } else if (qq >= kS - 2 * nbf) {
+            const int r2 = kS - 1 - qq;
             tr = A.bf_t[r2 / 2];
             slot = (r2 & 1) ? 0 : 1;
           }
           if (tr < 0) continue;
           double s = 0.0;
-          for (int w = 0; w < nwave; ++w) s += rb[w * kAliSlots + qq];
+          for (int w = 0; w < nwave; ++w) s += rb[w * kS + qq];
+          if (kLds) {
+            const int c = qq < 3 * nline ? qq : 3 * nline + 2 * ((kS - 1 - qq) / 2) + slot;
+            sums[(size_t)z * nsum + c] += s;
+            continue;
+          }
           double *ap = acc + ((size_t)z * A.ntrans + tr) * 4;
           if (qq < 3 * nline) {
             if (slot == 0) {
@@ -360,6 +388,27 @@ __global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, c
         // barrier of the following depth
       }
       __syncthreads();
+    }
+  }
+  if (kLds) {
+    for (int i = threadIdx.x; i < nd * nsum; i += blockDim.x) {
+      const int z = i / nsum, c = i - z * nsum;
+      const double v = sums[i];
+      if (c < 3 * nline) {
+        double *ap = acc + ((size_t)z * A.ntrans + A.line_t[c / 3]) * 4;
+        const int slot = c % 3;
+        if (slot == 0) {
+          ap[0] += v;
+          ap[1] += v;
+        } else if (slot == 1) {
+          ap[2] += v;
+        } else {
+          ap[3] += v;
+        }
+      } else {
+        const int b = (c - 3 * nline) / 2, slot = (c - 3 * nline) & 1;
+        acc[((size_t)z * A.ntrans + A.bf_t[b]) * 4 + slot] += v;
+      }
     }
   }
 }
@@ -579,20 +628,41 @@ static int convert_grid(const snapi_ali_grid *g, AliGridArgs &G) {
   return SNAPI_OK;
 }
 
-static size_t acc_smem(int precision, const AliAtom &A, int nd, int threads) {
+// Atoms of at most kSmallL lines and kSmallB bound-free transitions run the
+// variant with 32 reduced slots and shorter register arrays.
+constexpr int kSmallL = 6, kSmallB = 6;
+
+static size_t acc_smem(int precision, const AliAtom &A, int nd, int threads, bool lds_sums) {
   const size_t es = precision == SNAPI_FP64 ? 8 : 4;
-  return sizeof(double) * 2 * (threads / 64) * kAliSlots + es * (size_t)nd * (4 + 5 * A.nline + 2 * A.nbf);
+  return sizeof(double) * 2 * (threads / 64) * kAliSlots + es * (size_t)nd * (4 + 5 * A.nline + 2 * A.nbf) +
+         (lds_sums ? sizeof(double) * (size_t)nd * (3 * A.nline + 2 * A.nbf) : 0);
 }
 
 static int launch_acc(int precision, const AliAtom &A, const AliGridArgs &G, int ncol, int nd, const AliCol &P,
                       hipStream_t st) {
   const int threads = kAliThreads;
-  const size_t smem = acc_smem(precision, A, nd, threads);
-  if (smem > 65536 || 3 * A.nline + 2 * A.nbf > kAliSlots) return SNAPI_ERR_UNSUPPORTED;
-  if (precision == SNAPI_FP64)
-    hipLaunchKernelGGL(ali_acc_kernel<double>, dim3(ncol), dim3(threads), smem, st, A, G, nd, P);
-  else
-    hipLaunchKernelGGL(ali_acc_kernel<float>, dim3(ncol), dim3(threads), smem, st, A, G, nd, P);
+  if (3 * A.nline + 2 * A.nbf > kAliSlots) return SNAPI_ERR_UNSUPPORTED;
+  // the column's sums in LDS when they fit beside the tile, else in global memory
+  const bool lds = acc_smem(precision, A, nd, threads, true) <= 65536;
+  const size_t smem = acc_smem(precision, A, nd, threads, lds);
+  if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;
+  const bool small = A.nline <= kSmallL && A.nbf <= kSmallB && 3 * A.nline + 2 * A.nbf <= 32;
+  const dim3 g(ncol), b(threads);
+  if (precision == SNAPI_FP64) {
+    if (small && lds)
+      hipLaunchKernelGGL((ali_acc_kernel<double, true, kSmallL, kSmallB, 32>), g, b, smem, st, A, G, nd, P);
+    else if (lds)
+      hipLaunchKernelGGL((ali_acc_kernel<double, true, kAliMaxLines, kAliMaxBf, 64>), g, b, smem, st, A, G, nd, P);
+    else
+      hipLaunchKernelGGL((ali_acc_kernel<double, false, kAliMaxLines, kAliMaxBf, 64>), g, b, smem, st, A, G, nd, P);
+  } else {
+    if (small && lds)
+      hipLaunchKernelGGL((ali_acc_kernel<float, true, kSmallL, kSmallB, 32>), g, b, smem, st, A, G, nd, P);
+    else if (lds)
+      hipLaunchKernelGGL((ali_acc_kernel<float, true, kAliMaxLines, kAliMaxBf, 64>), g, b, smem, st, A, G, nd, P);
+    else
+      hipLaunchKernelGGL((ali_acc_kernel<float, false, kAliMaxLines, kAliMaxBf, 64>), g, b, smem, st, A, G, nd, P);
+  }
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 

@@ -30,6 +30,9 @@
  *   snapi_ali_accumulate     atom::add (Jbar, Lambda*, norm sums)   atmos/atmol/atom/atom.cc:1865-1927
  *   snapi_ali_newpops        atom::newpops                          atmos/atmol/atom/atom_pop.cc:27-135
  *   snapi_ali_solve*         atmosphere::nltepops (one NLTE atom)   atmos/atmos_pop.cc:67-240
+ *   snapi_formal_responses*  atmos_ppbez::formal_with_responses    atmos/plpar/atmos_ppbez_rts.cc:349-778
+ *   snapi_response_solve     the Crout solves of atom::compute_nlte_population_responses
+ *                                                                   atmos/atmol/atom/atom_responses.cc:171-200
  *   snapi_chemeq*            atmosphere::ltepops: chemeq + atom::lte atmos/atmos_pop.cc:30-45,
  *                                                                   atmos/atmos_chm.cc:28-363,
  *                                                                   atmos/atmol/atom/atom.cc:779-787, 892-924
@@ -456,6 +459,40 @@ typedef struct {
  */
 int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, const double *T, const double *Nt, double *ne,
                  double *stage, double *molecule, double *level, int *iterations);
+
+/*
+ * NLTE population responses: the two numerical kernels of
+ * atom::compute_nlte_population_responses (atmos/atmol/atom/atom_responses.cc:
+ * 26-204) as atmosphere::compute_nlte_population_responses
+ * (atmos/atmos_responses.cc:13-157) drives them.
+ *
+ * snapi_formal_responses: atmos_ppbez::formal_with_responses
+ * (atmos/plpar/atmos_ppbez_rts.cc:349-778), the scalar formal solution of one
+ * ray with the response of every intensity to every opacity and emissivity,
+ * with the reference's linear weights (its Bezier branch needs a forward step
+ * above 1e10) and Taylor weights below a step of 1e-2.
+ *   grid [ncol][nd]; op, em [ncol][nprob][nd], normalised as the reference
+ *   passes them; theta [nprob] (ray angle of each problem, radians);
+ *   boundary: -1 (I = S at the bottom) or a fixed intensity;
+ *   I, L [ncol][nprob][nd] out (intensity, Lambda*);
+ *   Rop, Rem [ncol][nprob][nd][nd] out: dI_z/dop_z', dI_z/dem_z'.
+ * 2 <= nd <= 256.
+ */
+int snapi_formal_responses(int ncol, int nprob, int nd, const double *grid, const double *op, const double *em,
+                           const double *theta, double boundary, double *I, double *L, double *Rop, double *Rem);
+int snapi_formal_responses_dev(int ncol, int nprob, int nd, const double *grid_dev, const double *op_dev,
+                               const double *em_dev, const double *theta_dev, double boundary, double *I_dev,
+                               double *L_dev, double *Rop_dev, double *Rem_dev, void *stream);
+
+/*
+ * snapi_response_solve: the Crout factorisation without pivoting and its
+ * solves (cmlib/mathtools.cc:487-514) that turn the assembled response
+ * matrix and right-hand sides into level responses
+ * (atom_responses.cc:171-200), for nsys independent systems, in double with
+ * the reference's operation order.
+ *   M [nsys][n][n]; B, X [nsys][nrhs][n] (one right-hand side per row).
+ */
+int snapi_response_solve(int nsys, int n, int nrhs, const double *M, const double *B, double *X);
 
 /* snapi_chemeq on device-resident arrays (T, Nt and the outputs: device
  * pointers; the table stays a host struct and is copied), all work on

@@ -31,6 +31,10 @@
 //   chemeq <atmos.cfg>
 //          the species table, then chemical equilibrium and LTE populations
 //          (atmosphere::ltepops, atmos/atmos_pop.cc:30-45) on every depth.
+//   nlteresp <atmos.cfg> [every]
+//          NLTE population responses of the active atom: its linear system,
+//          right-hand sides and solution, and formal_with_responses inputs
+//          and outputs at selected wavelengths.
 //
 // This file is test infrastructure: it is compiled only against the reference
 // sources (oracle/ref.mk) and never linked into the product.
@@ -770,6 +774,88 @@ static int mode_chemeq(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// NLTE population responses of the active atom (atmosphere::
+// compute_nlte_population_responses, atmos/atmos_responses.cc:13-157, as
+// obs_scalar_responses prepares it, atmos_obs.cc:168-181):
+//   nlteresp <atmos.cfg> [every]
+// First the atom's assembled linear system, its three right-hand sides per
+// perturbed depth and the level responses it solves for; then, at every
+// `every`-th wavelength of its NLTE grid and for both rays, the inputs and
+// outputs of atmos_ppbez::formal_with_responses (plpar/atmos_ppbez_rts.cc:
+// 349-778): normalised opacity and emissivity, intensity and the responses
+// of every intensity to every opacity and emissivity.
+static int mode_nlteresp(int argc, char **argv, io_class &io) {
+  if (argc < 3) return 2;
+  const int every = (argc > 3) ? atoi(argv[3]) : 40;
+  atmosphere *a = make_atmos(argv[2], io);
+  const int x1 = a->x1l, x2 = a->x2l;
+  a->set_grid(a->tau_grid);
+  a->boundary_condition_for_rt = -1;
+  a->popsetup();
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->set_parent_atmosphere(a);
+  if (a->tau_grid) a->compute_op_referent_derivative();
+  a->nltepops();
+  a->respsetup();
+  a->ne_lte_derivatives();
+  a->compute_nlte_population_responses(0);
+  atom *at = 0;
+  for (int s = 0; s < a->natm; ++s) {
+    atom *t = dynamic_cast<atom *>(a->atml[s]);
+    if (t && t->NLTE) at = t;
+  }
+  if (!at) return 3;
+  const int nd = a->x3h - a->x3l + 1, nm = at->nmap, n = nd * nm;
+  printf("nd %d\nnmap %d\ntau_grid %d\nboundary %.17g\n", nd, nm, a->tau_grid, a->boundary_condition_for_rt);
+  for (int r = 1; r <= n; ++r)
+    for (int cc = 1; cc <= n; ++cc)
+      if (at->response_matrix[r][cc] != 0.0) printf("M %d %d %.17g\n", r - 1, cc - 1, at->response_matrix[r][cc]);
+  for (int kk = 1; kk <= nd; ++kk)
+    for (int r = 1; r <= n; ++r)
+      printf("beta %d %d %.17g %.17g %.17g\n", kk - 1, r - 1, at->beta_Temp[kk][r], at->beta_density[kk][r],
+             at->beta_v_micro[kk][r]);
+  for (int p = 1; p <= 3; ++p)
+    for (int r = 1; r <= n; ++r)
+      for (int kk = 1; kk <= nd; ++kk) printf("resp %d %d %d %.17g\n", p, r - 1, kk - 1, at->level_responses[p][r][kk]);
+  // formal_with_responses at selected wavelengths of the NLTE grid
+  fp_t *lambda = 0;
+  int32_t nlambda = 0;
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->rtsetup(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  for (int s = 0; s < a->natm; ++s)
+    lambda = a->atml[s]->getlambda(lambda, nlambda, a->T[x1][x2][a->x3h], a->Nt[x1][x2][a->x3h], a->Ne[x1][x2][a->x3h]);
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->rtinit();
+  int ntp;
+  fp_t *th, *ph;
+  a->anglesetup(th, ph, ntp);
+  fp_t ***S = ft3dim(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  fp_t **Rop = ft2dim(a->x3l, a->x3h, a->x3l, a->x3h);
+  fp_t **Rem = ft2dim(a->x3l, a->x3h, a->x3l, a->x3h);
+  for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) printf("grid %d %.17g\n", x3i - a->x3l, a->rt_grid[x3i]);
+  int prob = 0;
+  for (int tp = 1; tp <= ntp; ++tp) {
+    fp_t ***Vr = a->project(a->Vx, a->Vy, a->Vz, th[tp], ph[tp], a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    fp_t ****Bv = a->transform(a->Bx, a->By, a->Bz, th[tp], ph[tp], a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    for (int l = 0; l < nlambda; l += every) {
+      for (int s = 0; s < a->natm; ++s) a->atml[s]->prof_init();
+      fp_t ***op = a->opacity(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]);
+      fp_t ***em = a->emissivity(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]);
+      if (a->tau_grid) a->normalize_to_referent_opacity(op, em);
+      a->formal_with_responses(a->rt_grid, S, 0, Rop, Rem, op, em, th[tp], ph[tp], a->boundary_condition_for_rt);
+      printf("prob %d %d %d %.17g %.17g\n", prob, tp, l, th[tp], lambda[l]);
+      for (int x3i = a->x3l; x3i <= a->x3h; ++x3i)
+        printf("opem %d %d %.17g %.17g %.17g\n", prob, x3i - a->x3l, op[x1][x2][x3i], em[x1][x2][x3i], S[x1][x2][x3i]);
+      for (int z = a->x3l; z <= a->x3h; ++z)
+        for (int zp = a->x3l; zp <= a->x3h; ++zp)
+          printf("R %d %d %d %.17g %.17g\n", prob, z - a->x3l, zp - a->x3l, Rop[z][zp], Rem[z][zp]);
+      del_ft3dim(em, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+      del_ft3dim(op, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+      ++prob;
+    }
+    del_ft4dim(Bv, 1, 3, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    del_ft3dim(Vr, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr, "usage: ref_driver synth|resp|lmstep|formal_scalar ...\n");
@@ -784,6 +870,7 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "interp")) return mode_interp(argc, argv, io);
   if (!strcmp(argv[1], "nlte")) return mode_nlte(argc, argv, io);
   if (!strcmp(argv[1], "chemeq")) return mode_chemeq(argc, argv, io);
+  if (!strcmp(argv[1], "nlteresp")) return mode_nlteresp(argc, argv, io);
   fprintf(stderr, "unknown mode %s\n", argv[1]);
   return 2;
 }

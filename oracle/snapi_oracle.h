@@ -94,6 +94,14 @@ int orc_chemeq_point(const snapi_chem_table *tb, double T, double Nt, double *ne
 int orc_chemeq(const snapi_chem_table *tb, int ncol, int nd, const double *T, const double *Nt, double *ne,
                double *stage, double *mol, double *lev, int *iterations, int nthreads);
 
+/* NLTE population responses (snapi_oracle_nlteresp.c): formal solution with
+ * the responses of every intensity to every opacity and emissivity of one
+ * ray (Rop, Rem [nd][nd]); and the pivot-free Crout solve of an n x n system
+ * for nrhs right-hand sides (B, X [nrhs][n]). */
+int orc_formal_responses(int nd, const double *grid, const double *op, const double *em, double theta,
+                         double boundary, double *I, double *L, double *Rop, double *Rem);
+int orc_response_solve(int n, int nrhs, const double *M, const double *B, double *X);
+
 #ifdef __cplusplus
 }
 #endif

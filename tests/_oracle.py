@@ -321,3 +321,34 @@ def chemeq(table, T, Nt, nthreads=0):
     L.orc_chemeq(C.byref(table), ncol, nd, ptr(T), ptr(Nt), ptr(ne), ptr(st), ptr(mo), ptr(lv),
                  it.ctypes.data_as(C.POINTER(C.c_int)), nthreads if nthreads else (os.cpu_count() or 1))
     return ne, st, mo, lv[:, :table.nlevel], it
+
+
+def formal_responses(grid, op, em, theta, boundary=-1.0):
+    """One ray: (I, L, Rop, Rem) with Rop, Rem [nd][nd]."""
+    L_ = lib()
+    dp = C.POINTER(C.c_double)
+    L_.orc_formal_responses.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp, dp, dp]
+    L_.orc_formal_responses.restype = C.c_int
+    g, o, e = _c(grid), _c(op), _c(em)
+    nd = len(g)
+    I = np.zeros(nd)
+    Ls = np.zeros(nd)
+    Rop = np.zeros((nd, nd))
+    Rem = np.zeros((nd, nd))
+    rc = L_.orc_formal_responses(nd, ptr(g), ptr(o), ptr(e), float(theta), float(boundary), ptr(I), ptr(Ls), ptr(Rop),
+                                 ptr(Rem))
+    assert rc == 0
+    return I, Ls, Rop, Rem
+
+
+def response_solve(M, B):
+    """Pivot-free Crout solve of M X^T = B^T: B, X [nrhs][n]."""
+    L_ = lib()
+    dp = C.POINTER(C.c_double)
+    L_.orc_response_solve.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+    L_.orc_response_solve.restype = C.c_int
+    M, B = _c(M), _c(B)
+    n = M.shape[0]
+    X = np.zeros_like(B)
+    L_.orc_response_solve(n, B.shape[0], ptr(M), ptr(B), ptr(X))
+    return X

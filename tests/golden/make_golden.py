@@ -572,6 +572,80 @@ def build_chemeq(d):
           (len(a["Z"]), len(a["mol_eqc"]), len(a["levels"]), len(out["T"])))
 
 
+def nlte_dir(d):
+    """Config directory of the NLTE column: the background set without Ca,
+    the reference's Ca II 5-level + continuum atom, the cfg0 atmosphere."""
+    bg = [a for a in BACKGROUND if a != "calcium.cfg"]
+    for a in bg + [NLTE_ATOM]:
+        shutil.copy(os.path.join(REF, "cfg", a), os.path.join(d, a))
+    shutil.copy(os.path.join(REF, "cfg", "mol.cfg"), os.path.join(d, "mol.cfg"))
+    with open(os.path.join(d, "atoms.cfg"), "w") as f:
+        for a in bg + [NLTE_ATOM]:
+            f.write("ATOM=%s\n" % a)
+    rows = cfg0_atmosphere()
+    with open(os.path.join(d, "atmos.cfg"), "w") as f:
+        f.write("ID=cfg0\nTYPE=SPINOR\nFILE=atmos.dat\nGEOMETRY=PLANEPARALLEL\nRTSOLVER=Bezier\n"
+                "ATOM=atoms.cfg\nMOL=mol.cfg\nELECTRONS=0\nTAU_GRID=1\nLEVEL_POPS=0\n"
+                "NX=1\nNY=1\nNZ=%d\n" % len(rows))
+    write_spinor(os.path.join(d, "atmos.dat"), rows)
+
+
+def build_nlteresp(d):
+    """NLTE population responses of the Ca II atom in the cfg0 column
+    (atmosphere::compute_nlte_population_responses after nltepops): the
+    assembled (nd*nmap)^2 system, its right-hand sides for T, density and
+    vmicro at every depth and the level responses; and formal_with_responses
+    at every 74th wavelength of the NLTE grid for both rays."""
+    nlte_dir(d)
+    out = run(d, ["nlteresp", "atmos.cfg", "74"])
+    head = {}
+    Ms, betas, resps, grid, probs, opem, R = [], [], [], [], [], [], []
+    for ln in out:
+        t = ln.split()
+        k = t[0]
+        if k in ("nd", "nmap", "tau_grid"):
+            head[k] = int(t[1])
+        elif k == "boundary":
+            head[k] = float(t[1])
+        elif k == "M":
+            Ms.append((int(t[1]), int(t[2]), float(t[3])))
+        elif k == "beta":
+            betas.append((int(t[1]), int(t[2]), float(t[3]), float(t[4]), float(t[5])))
+        elif k == "resp":
+            resps.append((int(t[1]), int(t[2]), int(t[3]), float(t[4])))
+        elif k == "grid":
+            grid.append(float(t[2]))
+        elif k == "prob":
+            probs.append((int(t[2]), int(t[3]), float(t[4]), float(t[5])))
+        elif k == "opem":
+            opem.append((int(t[1]), int(t[2]), float(t[3]), float(t[4]), float(t[5])))
+        elif k == "R":
+            R.append((int(t[1]), int(t[2]), int(t[3]), float(t[4]), float(t[5])))
+    nd, nm = head["nd"], head["nmap"]
+    n = nd * nm
+    M = np.zeros((n, n))
+    for r, c, v in Ms:
+        M[r, c] = v
+    beta = np.zeros((3, nd, n))
+    for kk, r, a, b, c in betas:
+        beta[:, kk, r] = (a, b, c)
+    resp = np.zeros((3, n, nd))
+    for p_, r, kk, v in resps:
+        resp[p_ - 1, r, kk] = v
+    npr = len(probs)
+    oe = np.zeros((npr, 3, nd))
+    for pr, z, a, b, c in opem:
+        oe[pr, :, z] = (a, b, c)
+    RR = np.zeros((npr, 2, nd, nd))
+    for pr, z, zp, a, b in R:
+        RR[pr, :, z, zp] = (a, b)
+    rec = dict(nd=nd, nmap=nm, tau_grid=head["tau_grid"], boundary=head["boundary"], M=M, beta=beta, resp=resp,
+               grid=np.array(grid), prob=np.array(probs), op=oe[:, 0], em=oe[:, 1], I=oe[:, 2], Rop=RR[:, 0],
+               Rem=RR[:, 1])
+    np.savez_compressed(os.path.join(HERE, "nlteresp.npz"), **rec)
+    print("nlteresp: system %d x %d, %d formal problems" % (n, n, npr))
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
@@ -585,6 +659,7 @@ def main():
         build_interp(d)
         build_nlte(d)
         build_chemeq(d)
+        build_nlteresp(d)
     finally:
         shutil.rmtree(d)
 

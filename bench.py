@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--no-ali", action="store_true")
     ap.add_argument("--chem-side", type=int, default=256, help="chemeq leg: side of the column square")
     ap.add_argument("--no-chem", action="store_true")
+    ap.add_argument("--nlteresp-cols", type=int, default=64, help="NLTE-response leg: columns of the formal pass")
+    ap.add_argument("--nlteresp-systems", type=int, default=64, help="NLTE-response leg: systems solved")
+    ap.add_argument("--no-nlteresp", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the hot path is per column and shards by column tiles; "
@@ -180,6 +183,10 @@ def main():
     if not args.no_chem:
         chem = chemeq_leg(torch, dev, stream, args.chem_side, not args.no_cpu)
 
+    nresp = None
+    if not args.no_nlteresp:
+        nresp = nlteresp_leg(torch, dev, stream, args.nlteresp_cols, args.nlteresp_systems, not args.no_cpu)
+
     line = {"metric": METRIC, "value": value, "unit": "lambda-point formal solves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
@@ -188,7 +195,7 @@ def main():
                                    % (nx, ny, nd, nlam),
                        "columns": ncol, "depths": nd, "wavelengths": nlam, "lines": table.nline},
             "roofline": roof, "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64, "lm": lm, "ali": alires,
-            "chemeq": chem}
+            "chemeq": chem, "nlte_responses": nresp}
     print(json.dumps(line))
 
 
@@ -250,6 +257,87 @@ def chemeq_leg(torch, dev, stream, side, with_cpu):
     return {"workload": "chemical equilibrium + LTE populations, %d columns x %d depths, %d elements, %d molecules, "
                         "%d levels, FP64" % (ncol, nd, tb.nelement, tb.nmolecule, tb.nlevel),
             "value": res["points_per_s"], "unit": "depth points/s", "fp64": res, "cpu_baseline": cpu}
+
+
+def nlteresp_leg(torch, dev, stream, ncol, nsys, with_cpu):
+    """The two numerical kernels of the NLTE population responses on the Ca II
+    column recorded from the reference (tests/golden/nlteresp.npz): the
+    formal solution with dI/dop and dI/dem for 2 rays x 444 wavelengths per
+    column (the reference's NLTE grid), inputs perturbed per column; and the
+    366 x 366 response-system solve with its 183 right-hand sides."""
+    from snapi_amd import nlteresp
+    d = np.load(os.path.join(REPO, "tests", "golden", "nlteresp.npz"))
+    nd, npr0 = int(d["nd"]), len(d["prob"])
+    rng = np.random.default_rng(12)
+    nprob = 888
+    pick = np.arange(nprob) % npr0
+    theta = torch.from_numpy(np.ascontiguousarray(d["prob"][pick, 2])).to(dev)
+    grid = torch.from_numpy(np.repeat(d["grid"][None], ncol, axis=0)).to(dev)
+    f = np.exp(rng.normal(0.0, 0.1, (ncol, nprob, nd)))
+    op = torch.from_numpy(np.ascontiguousarray(d["op"][pick][None] * f)).to(dev)
+    em = torch.from_numpy(np.ascontiguousarray(d["em"][pick][None] * f)).to(dev)
+    I = torch.empty((ncol, nprob, nd), dtype=torch.float64, device=dev)
+    L = torch.empty_like(I)
+    Rop = torch.empty((ncol, nprob, nd, nd), dtype=torch.float64, device=dev)
+    Rem = torch.empty_like(Rop)
+
+    def call():
+        nlteresp.formal_responses_dev(grid, op, em, theta, I, L, Rop, Rem, stream=stream)
+
+    call()
+    torch.cuda.synchronize()
+    reps = 5
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    probs = ncol * nprob
+    # HBM bytes per problem: op, em read, I, L written, both nd x nd matrices written
+    byt = probs * (4 * nd + 2 * nd * nd) * 8.0
+    formal = {"problems": probs, "depths": nd, "ms_per_launch": round(ms, 3), "problems_per_s": probs / (ms * 1e-3),
+              "roofline": {"bound": "hbm", "achieved": byt / (ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                           "frac": byt / (ms * 1e-3) / 8e12, "algorithmic_bytes": byt}}
+    log("formal_with_responses: %d problems x %d depths in %.3f ms = %.4g problems/s, %.0f GB/s"
+        % (probs, nd, ms, probs / (ms * 1e-3), byt / (ms * 1e-3) / 1e9))
+    n = d["M"].shape[0]
+    M = d["M"][None] * (1.0 + 1e-3 * rng.standard_normal((nsys, n, n)))
+    B = np.repeat(d["beta"].reshape(3 * nd, n)[None], nsys, axis=0)
+    nlteresp.response_solve(M[:1], B[:1])  # kernel loading, not timed
+    t0 = time.perf_counter()
+    X = nlteresp.response_solve(M, B)
+    wall = time.perf_counter() - t0
+    flop = nsys * (2.0 * n ** 3 / 3.0 + 2.0 * n * n * B.shape[1])
+    solve = {"systems": nsys, "n": n, "rhs": int(B.shape[1]), "seconds_with_copies": round(wall, 4),
+             "systems_per_s": nsys / wall, "fp64_gflop_per_s": flop / wall / 1e9}
+    log("response solve: %d systems of %d x %d with %d right-hand sides in %.3f s (host copies included)"
+        % (nsys, n, n, B.shape[1], wall))
+    cpu = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import _oracle  # the FP64 CPU restatement: baseline and checker only
+        t1 = time.time()
+        nref = 32
+        for p in range(nref):
+            _oracle.formal_responses(d["grid"], d["op"][p % npr0], d["em"][p % npr0], d["prob"][p % npr0, 2])
+        dt = time.time() - t1
+        t2 = time.time()
+        Xo = _oracle.response_solve(M[0], B[0])
+        dts = time.time() - t2
+        cpu = {"value": nref / dt, "unit": "formal_with_responses problems/s", "cores": 1, "kind": "port",
+               "sample": "%d problems of the same column (FP64 oracle, one thread); one system solve %.3f s"
+                         % (nref, dts), "seconds": round(dt, 3), "solve_seconds": round(dts, 3)}
+        solve["rel_err_vs_oracle"] = float(np.max(np.abs(X[0] - Xo)) / np.abs(Xo).max())
+        log("nlteresp cpu oracle: %.4g problems/s, one solve %.3f s; GPU solve vs oracle %.2e"
+            % (nref / dt, dts, solve["rel_err_vs_oracle"]))
+    return {"workload": "NLTE population responses, Ca II 5 levels + continuum in the 61-depth column: "
+                        "formal_with_responses for %d columns x %d (ray, wavelength) problems; %d solves of the "
+                        "%d x %d response system" % (ncol, nprob, nsys, n, n),
+            "value": formal["problems_per_s"], "unit": "formal_with_responses problems/s", "formal": formal,
+            "solve": solve, "cpu_baseline": cpu}
 
 
 def ali_leg(torch, dev, stream, side, with_cpu):

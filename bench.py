@@ -216,9 +216,10 @@ def chemeq_leg(torch, dev, stream, side, with_cpu):
     mo = torch.empty((ncol, tb.nmolecule, nd), dtype=torch.float64, device=dev)
     lv = torch.empty((ncol, tb.nlevel, nd), dtype=torch.float64, device=dev)
     it = torch.empty((ncol, nd), dtype=torch.int32, device=dev)
+    kr = torch.empty((ncol, nd), dtype=torch.float64, device=dev)
 
     def call():
-        chemeq.chemeq_dev(tb, T, Nt, ne, st, mo, lv, it, stream=stream)
+        chemeq.chemeq_dev(tb, T, Nt, ne, st, mo, lv, it, stream=stream, opref_t=kr)
 
     call()  # kernel loading, not timed
     torch.cuda.synchronize()
@@ -244,7 +245,8 @@ def chemeq_leg(torch, dev, stream, side, with_cpu):
         nthreads = max(1, min(16, os.cpu_count() or 1))
         nc = min(ncol, 2048)
         t1 = time.time()
-        ne_c, st_c, mo_c, lv_c, it_c = _oracle.chemeq(tb, p["T"][:nc], p["Nt"][:nc], nthreads=nthreads)
+        ne_c, st_c, mo_c, lv_c, it_c, kr_c = _oracle.chemeq(tb, p["T"][:nc], p["Nt"][:nc], nthreads=nthreads,
+                                                            with_opref=True)
         dt = time.time() - t1
         cpu = {"value": nc * nd / dt, "unit": "depth points/s", "cores": nthreads, "kind": "port",
                "sample": "%d columns x %d depths of the same workload (FP64 oracle, OpenMP)" % (nc, nd),
@@ -252,9 +254,10 @@ def chemeq_leg(torch, dev, stream, side, with_cpu):
         g_ne = ne[:nc].cpu().numpy()
         res["ne_rel_err_vs_oracle"] = float(np.max(np.abs(g_ne / ne_c - 1.0)))
         res["steps_equal_fraction"] = float(np.mean(itn[:nc] == it_c))
+        res["opref_rel_err_vs_oracle"] = float(np.max(np.abs(kr[:nc].cpu().numpy() / kr_c - 1.0)))
         log("chemeq cpu oracle: %.4g points/s on %d threads; GPU vs oracle Ne max rel %.2e, steps equal %.4f"
             % (cpu["value"], nthreads, res["ne_rel_err_vs_oracle"], res["steps_equal_fraction"]))
-    return {"workload": "chemical equilibrium + LTE populations, %d columns x %d depths, %d elements, %d molecules, "
+    return {"workload": "chemical equilibrium + LTE populations + referent continuum opacity, %d columns x %d depths, %d elements, %d molecules, "
                         "%d levels, FP64" % (ncol, nd, tb.nelement, tb.nmolecule, tb.nlevel),
             "value": res["points_per_s"], "unit": "depth points/s", "fp64": res, "cpu_baseline": cpu}
 

@@ -389,6 +389,7 @@ int snapi_ali_solve_dev(int precision, const snapi_ali_atom *atom, const snapi_a
 #define SNAPI_CHEM_MAX_COMPONENTS 4
 #define SNAPI_CHEM_MAX_COMP_STAGE 2 /* component stages 0 and 1 (after clamping at 0) */
 #define SNAPI_CHEM_MAX_LEVELS 256
+#define SNAPI_CHEM_MAX_BF 64
 
 enum snapi_pf_type { SNAPI_PF_NONE = 0, SNAPI_PF_CONST = 1, SNAPI_PF_TRAV = 2, SNAPI_PF_IRWIN = 3 };
 enum snapi_eqc_type { SNAPI_EQC_NONE = 0, SNAPI_EQC_CONST = 1, SNAPI_EQC_POLY = 2, SNAPI_EQC_HMIN = 3 };
@@ -432,11 +433,26 @@ typedef struct {
   double energy;      /* excitation energy [erg] */
 } snapi_chem_level;
 
+/* A bound-free transition of the continuum opacity at the referent
+ * wavelength (atom::opacity_continuum, atmos/atmol/atom/atom_rad.cc:151-187):
+ * from a level of stage `stage` to the ground level of stage+1. */
 typedef struct {
-  int nelement, nmolecule, nlevel;
+  int element, stage;     /* element 1..nelement */
+  double g, energy;       /* lower level: statistical weight, excitation energy [erg] */
+  double g_up, energy_up; /* ground level of the next stage (g_up = 0: a placeholder whose
+                             population is the stage's) */
+  double sigma;           /* cross-section at lambda_ref [cm^2] */
+} snapi_chem_bf;
+
+typedef struct {
+  int nelement, nmolecule, nlevel, nbf;
+  int hminus;        /* 1: the referent opacity includes H- bound-free + free-free of
+                        element 1's neutral stage (h_minus_mol::opacity_continuum) */
+  double lambda_ref; /* referent wavelength [cm] */
   snapi_chem_element element[SNAPI_CHEM_MAX_ELEMENTS];
   snapi_chem_molecule molecule[SNAPI_CHEM_MAX_MOLECULES];
   snapi_chem_level level[SNAPI_CHEM_MAX_LEVELS];
+  snapi_chem_bf bf[SNAPI_CHEM_MAX_BF];
 } snapi_chem_table;
 
 /*
@@ -454,11 +470,16 @@ typedef struct {
  *     stage, elements in table order, stages ascending;
  *   molecule [ncol][nmolecule][nd] out (may be NULL);
  *   level [ncol][nlevel][nd] out (may be NULL);
+ *   opref [ncol][nd] out (may be NULL): continuum opacity at lambda_ref
+ *     (atmosphere::opacity_continuum, atmos/atmos_rad.cc:286-294): Thomson,
+ *     the table's bound-free transitions with stimulated emission, H- if
+ *     hminus; the referent opacity of compute_op_referent
+ *     (plpar/atmos_ppbez_rts.cc:268-281) for lambda_ref = airtovac(5000 A);
  *   iterations [ncol][nd] out (may be NULL): Newton steps taken, -1 where the
  *     linear system was singular (the reference exits there).
  */
 int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, const double *T, const double *Nt, double *ne,
-                 double *stage, double *molecule, double *level, int *iterations);
+                 double *stage, double *molecule, double *level, double *opref, int *iterations);
 
 /*
  * NLTE population responses: the two numerical kernels of
@@ -498,7 +519,7 @@ int snapi_response_solve(int nsys, int n, int nrhs, const double *M, const doubl
  * pointers; the table stays a host struct and is copied), all work on
  * `stream`. */
 int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev, const double *Nt_dev,
-                     double *ne_dev, double *stage_dev, double *molecule_dev, double *level_dev,
+                     double *ne_dev, double *stage_dev, double *molecule_dev, double *level_dev, double *opref_dev,
                      int *iterations_dev, void *stream);
 
 #ifdef __cplusplus

@@ -671,6 +671,13 @@ static int mode_chemeq(int argc, char **argv, io_class &io) {
   if (argc < 3) return 2;
   atmosphere *a = make_atmos(argv[2], io);
   const int x1 = a->x1l, x2 = a->x2l;
+  // the hydrostatic integration's result when make_atmos ran it
+  // (atmos_pp::enforce_hequilibrium, plpar/atmos_pp.cc:372-455)
+  if (getenv("SNAPI_REF_HE") && atoi(getenv("SNAPI_REF_HE")))
+    for (int x3i = a->x3l; x3i <= a->x3h; ++x3i)
+      printf("he %d %.17g %.17g %.17g %.17g %.17g %.17g\n", x3i, a->T[x1][x2][x3i],
+             log10(-a->tau_referent[x1][x2][x3i]), a->Nt[x1][x2][x3i], a->Ne[x1][x2][x3i],
+             a->op_referent[x1][x2][x3i], a->rho[x1][x2][x3i]);
   a->popsetup();
   atom **el = new atom *[a->natm];
   int nel = 0;
@@ -709,7 +716,18 @@ static int mode_chemeq(int argc, char **argv, io_class &io) {
       for (int l = 0; l < t->nl[z]; ++l)
         printf("lvl %d %d %d %d %.17g\n", ie + 1, z, l, (int)t->g[z][l], t->ee[z][l]);
     }
+    // bound-free cross-sections at the referent wavelength (atom::opacity_continuum,
+    // atmos/atmol/atom/atom_rad.cc:151-187)
+    const fp_t lr = airtovac(lambda_referent);
+    for (int z = 0; z < t->Z; ++z)
+      for (int l = 0; l < t->nl[z]; ++l) {
+        const fp_t sg = (t->bf[z][l]) ? t->bf[z][l]->U(lr) : 0.0, sg0 = (t->bf[z][l]) ? t->bf[z][l]->U(lambda_referent) : 0.0;
+        if (sg != 0.0 || sg0 != 0.0)
+          printf("bf %d %d %d %d %.17g %d %.17g %.17g %.17g\n", ie + 1, z, l, (int)t->g[z][l], t->ee[z][l],
+                 (int)t->g[z + 1][0], t->ee[z + 1][0], sg, sg0);
+      }
   }
+  printf("lref %.17g %.17g\n", airtovac(lambda_referent), (fp_t)lambda_referent);
   int nmol = 0;
   for (int s = 0; s < a->natm; ++s) {
     int nc;
@@ -771,6 +789,10 @@ static int mode_chemeq(int argc, char **argv, io_class &io) {
       if (hm) printf("hmin %d %.17g\n", x3i, hm->N[x1][x2][x3i]);
     }
   }
+  // referent opacity after chemeq + LTE at every depth (atmos_ppbez::compute_op_referent,
+  // plpar/atmos_ppbez_rts.cc:268-281)
+  a->compute_op_referent();
+  for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) printf("opref %d %.17g\n", x3i, a->op_referent[x1][x2][x3i]);
   return 0;
 }
 

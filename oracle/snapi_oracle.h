@@ -90,9 +90,14 @@ void orc_chem_partition(const snapi_partition *pf, double T, double ne, double *
 void orc_chem_ionfrc(const snapi_chem_element *el, double T, double ne, double *fk, double *df);
 double orc_chem_dissoc(const snapi_chem_molecule *m, double T);
 int orc_chemeq_point(const snapi_chem_table *tb, double T, double Nt, double *ne, double *stage, double *mol,
-                     double *lev);
+                     double *lev, double *opref);
+double orc_chem_opref(const snapi_chem_table *tb, double T, double ne, const double *stage_pop);
+int orc_hequilibrium(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, double *Nt,
+                     double *ne, double *opref, double *rho, int *iters);
+int orc_hequilibrium_batch(const snapi_chem_table *tb, int ncol, int nd, const double *T, const double *logtau,
+                           double *Nt, double *ne, double *opref, double *rho, int *iters, int nthreads);
 int orc_chemeq(const snapi_chem_table *tb, int ncol, int nd, const double *T, const double *Nt, double *ne,
-               double *stage, double *mol, double *lev, int *iterations, int nthreads);
+               double *stage, double *mol, double *lev, double *opref, int *iterations, int nthreads);
 
 /* NLTE population responses (snapi_oracle_nlteresp.c): formal solution with
  * the responses of every intensity to every opacity and emissivity of one

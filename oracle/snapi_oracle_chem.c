@@ -8,6 +8,8 @@
  *                         mol::dissoc, atmos/atmol/mol/mol.cc:159-162
  *   equilibrium           atmosphere::chemeq, atmos/atmos_chm.cc:28-363
  *   level populations     atom::lte, atmos/atmol/atom/atom.cc:779-787
+ *   referent opacity      atmosphere::opacity_continuum, atmos/atmos_rad.cc:286-294
+ *   hydrostatic eq.       atmos_pp::enforce_hequilibrium, atmos/plpar/atmos_pp.cc:372-455
  *
  * The reference solves each Newton system by SVD (cmlib svdcmp) and exits
  * when a singular value falls below 1e-30 of the largest; for every system it
@@ -18,7 +20,11 @@
  * fractions, dissociation constants, electron densities, stage and level
  * populations and H- densities on 133 depth points
  * (tests/golden/chemeq.npz); the pure-hydrogen Saha limit in closed form;
- * charge and particle conservation at the result.
+ * charge and particle conservation at the result; the referent opacity
+ * against the reference's compute_op_referent on the same points and the LTE
+ * detailed-balance form of its bound-free term; the hydrostatic march against
+ * the reference's enforce_hequilibrium on the cfg0 column (Nt, Ne, opacity and
+ * density at every depth) and against the pressure balance it integrates.
  */
 #include <math.h>
 #include <string.h>
@@ -33,6 +39,8 @@
 #define C_E 4.8032042510E-10
 #define C_RYD 1.097373156853955E5
 #define C_E0 7.957747154594775607E-02
+#define C_C 2.99792458E10
+#define C_SAHA 2.0706615382E-16
 
 #define MAXE SNAPI_CHEM_MAX_ELEMENTS
 #define MAXS SNAPI_CHEM_MAX_STAGES
@@ -170,11 +178,67 @@ static int solve_pivot(int n, double *A, double *d, double *v) {
   return 0;
 }
 
+/* LTE population of a level (atom::lte, atom.cc:779-787). */
+static double lte_level(const snapi_chem_element *el, int stage, double g, double energy, double T, double ne,
+                        double Nstage) {
+  double U, dU;
+  orc_chem_partition(&el->pf[stage], T, ne, &U, &dU);
+  return Nstage * ((g != 0.0) ? g * exp(-energy / (C_K * T)) / U : 1.0);
+}
+
+/* Continuum opacity at the table's referent wavelength once chemeq and LTE
+ * are done: atmosphere::opacity_continuum (atmos_rad.cc:286-294) -- Thomson
+ * 6.65e-25 Ne, every element's bound-free transitions with stimulated
+ * emission in table order (atom::opacity_continuum, atom_rad.cc:151-187,
+ * whose free-free term is computed and then discarded), then H- bound-free
+ * and free-free of element 1's neutral stage (h_minus_mol::opacity_continuum,
+ * mol.cc:669-695).  stage_pop: every stage of every element, table order. */
+double orc_chem_opref(const snapi_chem_table *tb, double T, double ne, const double *stage_pop) {
+  const double lam = tb->lambda_ref;
+  double op = 6.65E-25 * ne;
+  int off[MAXE + 1];
+  off[1] = 0;
+  for (int a = 1; a < tb->nelement; ++a) off[a + 1] = off[a] + tb->element[a - 1].nstage;
+  for (int a = 1; a <= tb->nelement; ++a) {
+    const snapi_chem_element *el = &tb->element[a - 1];
+    double part = 0.0;
+    for (int j = 0; j < tb->nbf; ++j) {
+      const snapi_chem_bf *b = &tb->bf[j];
+      if (b->element != a || b->sigma == 0.0) continue;
+      const int z = b->stage;
+      const double nl = lte_level(el, z, b->g, b->energy, T, ne, stage_pop[off[a] + z]);
+      const double nu = lte_level(el, z + 1, b->g_up, b->energy_up, T, ne, stage_pop[off[a] + z + 1]);
+      const double pop_mod = nu * ne * C_SAHA * pow(T, -1.5) * b->g / b->g_up * exp((el->eion[z] - b->energy) / C_K / T);
+      part += b->sigma * (nl - pop_mod * exp(-C_H * C_C / lam / C_K / T));
+    }
+    op += part;
+  }
+  if (tb->hminus) {
+    const double NH = stage_pop[0];
+    static const double a[7] = {1.99654, -1.18267E-5, 2.64243E-6, -4.40524E-10, 3.23992E-14, -1.39568E-18,
+                                2.78701E-23};
+    double alpha = 0.0;
+    for (int i = 0; i < 7; ++i) alpha += a[i] * pow(lam * 1E8, i);
+    double T_P = 5040. / T;
+    const double op_bf = 4.158E-28 * alpha * ne * C_K * T * pow(T_P, 2.5) * pow(10.0, 0.754 * T_P) * NH *
+                         (1.0 - exp(-C_H * C_C / lam / C_K / T));
+    const double l = log10(lam * 1E8);
+    const double f_0 = -2.2763 - 1.6850 * l + 0.76661 * l * l - 0.053346 * l * l * l;
+    const double f_1 = 15.2827 - 9.2846 * l + 1.99381 * l * l - 0.142631 * l * l * l;
+    const double f_2 = -197.789 + 190.266 * l - 67.9775 * l * l + 10.6913 * l * l * l - 0.625151 * l * l * l * l;
+    T_P = 5040. / T;
+    const double op_ff =
+        1E-26 * ne * C_K * T * pow(10.0, f_0 + f_1 * log10(T_P) + f_2 * log10(T_P) * log10(T_P)) * NH;
+    op += op_bf + op_ff;
+  }
+  return op;
+}
+
 /* One depth point: atmosphere::chemeq (atmos_chm.cc:28-363) then atom::lte
  * (atom.cc:779-787).  Outputs as snapi_chemeq for one point (stage, mol,
  * lev may be NULL).  Returns the Newton steps taken, or -1 (singular). */
-int orc_chemeq_point(const snapi_chem_table *tb, double T, double Nt, double *ne_out, double *stage, double *mol,
-                     double *lev) {
+static int chemeq_point(const snapi_chem_table *tb, double T, double Nt, double *ne_out, double *stage,
+                        double *mol, double *lev, double *opref, double *n1_total) {
   const int na = tb->nelement, nm = tb->nmolecule, n1 = na + 1;
   const double kT = C_K * T;
   double abund[MAXE + 1], N[MAXE + 1], d[MAXE + 1], v[MAXE + 1], A[(MAXE + 1) * (MAXE + 1)];
@@ -314,11 +378,15 @@ int orc_chemeq_point(const snapi_chem_table *tb, double T, double Nt, double *ne
   *ne_out = N[0];
   // stage populations: the final N with the last iteration's fractions
   // (atom::pupdate, atom.cc:1449-1453)
-  if (stage) {
+  double st_all[MAXE * MAXS];
+  {
     int s = 0;
     for (int a = 1; a <= na; ++a)
-      for (int z = 0; z < nk[a]; ++z) stage[s++] = N[a] * fk[a][z];
+      for (int z = 0; z < nk[a]; ++z) st_all[s++] = N[a] * fk[a][z];
+    if (stage) memcpy(stage, st_all, sizeof(double) * s);
   }
+  if (opref) *opref = orc_chem_opref(tb, T, N[0], st_all);
+  if (n1_total) *n1_total = N[1];
   if (mol) {
     for (int m = 0; m < nm; ++m) {
       const snapi_chem_molecule *M = &tb->molecule[m];
@@ -343,8 +411,13 @@ int orc_chemeq_point(const snapi_chem_table *tb, double T, double Nt, double *ne
   return steps;
 }
 
+int orc_chemeq_point(const snapi_chem_table *tb, double T, double Nt, double *ne_out, double *stage, double *mol,
+                     double *lev, double *opref) {
+  return chemeq_point(tb, T, Nt, ne_out, stage, mol, lev, opref, NULL);
+}
+
 int orc_chemeq(const snapi_chem_table *tb, int ncol, int nd, const double *T, const double *Nt, double *ne,
-               double *stage, double *mol, double *lev, int *iterations, int nthreads) {
+               double *stage, double *mol, double *lev, double *opref, int *iterations, int nthreads) {
   int nst = 0;
   for (int a = 0; a < tb->nelement; ++a) nst += tb->element[a].nstage;
   const int nm = tb->nmolecule, nl = tb->nlevel;
@@ -352,9 +425,10 @@ int orc_chemeq(const snapi_chem_table *tb, int ncol, int nd, const double *T, co
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
   for (long p = 0; p < np; ++p) {
     const long c = p / nd, z = p % nd;
-    double st[MAXE * MAXS], mo[MAXM], le[SNAPI_CHEM_MAX_LEVELS], e;
-    const int it = orc_chemeq_point(tb, T[p], Nt[p], &e, st, mo, le);
+    double st[MAXE * MAXS], mo[MAXM], le[SNAPI_CHEM_MAX_LEVELS], e, kr;
+    const int it = orc_chemeq_point(tb, T[p], Nt[p], &e, st, mo, le, &kr);
     ne[p] = e;
+    if (opref) opref[p] = kr;
     if (iterations) iterations[p] = it;
     if (stage)
       for (int s = 0; s < nst; ++s) stage[(c * nst + s) * nd + z] = st[s];
@@ -364,4 +438,80 @@ int orc_chemeq(const snapi_chem_table *tb, int ncol, int nd, const double *T, co
       for (int l = 0; l < nl; ++l) lev[(c * nl + l) * nd + z] = le[l];
   }
   return 0;
+}
+
+/* Hydrostatic equilibrium of one column on its optical-depth grid
+ * (atmos_pp::enforce_hequilibrium, atmos/plpar/atmos_pp.cc:372-455): the top
+ * point from a 0.3 dyn/cm^2 guess, iterated on Nt = 10^logtau rho g / kappa / kT;
+ * then downwards, each point's pressure step g / kappa_mean * dtau with
+ * kappa_mean the geometric mean of opacity per mass of the point and the one
+ * above, iterated up to 20 times and left when the correction falls below
+ * 1e-2 of the TOP point's Nt; every point's opacity and density are those of
+ * its last iteration (before its last correction), and the final state is
+ * chemical equilibrium at the final Nt.  rho = 1.4 * 1.67e-24 times the total
+ * (free) density of element 1.  iters[z]: iterations run at depth z. */
+#define C_GRAV 274.88E2
+int orc_hequilibrium(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, double *Nt,
+                     double *ne, double *opref, double *rho, int *iters) {
+  if (nd < 1 || !(tb->lambda_ref > 0.0)) return -1;
+  const int max_iter = 20;
+  const double break_me = 1E-2;
+  double e, op_prev = 0.0, rho_prev = 0.0;
+  {
+    Nt[0] = 0.3 / C_K / T[0];
+    int it = 0;
+    for (; it < max_iter;) {
+      double op, nh;
+      chemeq_point(tb, T[0], Nt[0], &e, NULL, NULL, NULL, &op, &nh);
+      const double r = nh * 1.4 * 1.67E-24;
+      op_prev = op;
+      rho_prev = r;
+      const double dN = (pow(10.0, logtau[0]) * r * C_GRAV / op) / C_K / T[0] - Nt[0];
+      Nt[0] += dN;
+      ++it;
+      if (fabs(dN / Nt[0]) < break_me) break;
+    }
+    if (iters) iters[0] = it;
+  }
+  for (int z = 1; z < nd; ++z) {
+    const double delta_tau = pow(10, logtau[z]) - pow(10, logtau[z - 1]);
+    double delta_p = rho_prev * C_GRAV / op_prev * delta_tau;
+    Nt[z] = (Nt[z - 1] * C_K * T[z - 1] + delta_p) / C_K / T[z];
+    double op = op_prev, r = rho_prev;
+    int it = 0;
+    for (; it < max_iter;) {
+      double nh;
+      chemeq_point(tb, T[z], Nt[z], &e, NULL, NULL, NULL, &op, &nh);
+      r = nh * 1.4 * 1.67E-24;
+      const double kappa_mean = sqrt(op * op_prev / r / rho_prev);
+      delta_p = C_GRAV / kappa_mean * delta_tau;
+      const double dN = (Nt[z - 1] * C_K * T[z - 1] + delta_p) / C_K / T[z] - Nt[z];
+      Nt[z] += dN;
+      ++it;
+      if (fabs(dN / Nt[0]) < break_me) break;
+    }
+    op_prev = op;
+    rho_prev = r;
+    if (iters) iters[z] = it;
+  }
+  for (int z = 0; z < nd; ++z) {
+    double op, nh;
+    chemeq_point(tb, T[z], Nt[z], &e, NULL, NULL, NULL, &op, &nh);
+    if (ne) ne[z] = e;
+    if (opref) opref[z] = op;
+    if (rho) rho[z] = nh * 1.4 * 1.67E-24;
+  }
+  return 0;
+}
+
+int orc_hequilibrium_batch(const snapi_chem_table *tb, int ncol, int nd, const double *T, const double *logtau,
+                           double *Nt, double *ne, double *opref, double *rho, int *iters, int nthreads) {
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) reduction(| : bad)
+  for (int c = 0; c < ncol; ++c) {
+    const long o = (long)c * nd;
+    bad |= orc_hequilibrium(tb, nd, T + o, logtau + o, Nt + o, ne ? ne + o : NULL, opref ? opref + o : NULL,
+                            rho ? rho + o : NULL, iters ? iters + o : NULL) != 0;
+  }
+  return bad ? -1 : 0;
 }

@@ -114,6 +114,7 @@ CHEM_MAX_COEFS = 8
 CHEM_MAX_MOLECULES = 32
 CHEM_MAX_COMPONENTS = 4
 CHEM_MAX_LEVELS = 256
+CHEM_MAX_BF = 64
 PF_NONE, PF_CONST, PF_TRAV, PF_IRWIN = range(4)
 EQC_NONE, EQC_CONST, EQC_POLY, EQC_HMIN = range(4)
 
@@ -142,18 +143,29 @@ class SnapiChemLevel(C.Structure):
     _fields_ = [("element", C.c_int), ("stage", C.c_int), ("g", C.c_double), ("energy", C.c_double)]
 
 
+class SnapiChemBf(C.Structure):
+    _fields_ = [("element", C.c_int), ("stage", C.c_int), ("g", C.c_double), ("energy", C.c_double),
+                ("g_up", C.c_double), ("energy_up", C.c_double), ("sigma", C.c_double)]
+
+
 class SnapiChemTable(C.Structure):
-    _fields_ = [("nelement", C.c_int), ("nmolecule", C.c_int), ("nlevel", C.c_int),
+    _fields_ = [("nelement", C.c_int), ("nmolecule", C.c_int), ("nlevel", C.c_int), ("nbf", C.c_int),
+                ("hminus", C.c_int), ("lambda_ref", C.c_double),
                 ("element", SnapiChemElement * CHEM_MAX_ELEMENTS),
                 ("molecule", SnapiChemMolecule * CHEM_MAX_MOLECULES),
-                ("level", SnapiChemLevel * CHEM_MAX_LEVELS)]
+                ("level", SnapiChemLevel * CHEM_MAX_LEVELS),
+                ("bf", SnapiChemBf * CHEM_MAX_BF)]
 
 
-def make_chem_table(d, levels=True):
+def make_chem_table(d, levels=True, referent=0):
     """Species table from arrays shaped as tests/golden/chemeq.npz holds them:
     per element Z, abund, eion, pf_* and term_*/irwin parameters; per molecule
     mol_eqc/mol_ncoef/mol_coef/mol_ncomp/mol_elem/mol_stage; levels
-    (element 0-based, stage, level, g) with level_ee [erg]."""
+    (element 0-based, stage, level, g) with level_ee [erg]; the bound-free
+    transitions bf_int/bf_real with their cross-sections at the referent
+    wavelength lref[referent] (0: airtovac(5000 A), as compute_op_referent
+    uses it; 1: 5000 A, as the hydrostatic integration does); H- included when
+    a molecule has the HMIN form."""
     t = SnapiChemTable()
     nel = len(d["Z"])
     t.nelement = nel
@@ -193,6 +205,16 @@ def make_chem_table(d, levels=True):
         for i in range(mo.ncomp):
             mo.element[i] = int(d["mol_elem"][m, i])
             mo.stage[i] = int(d["mol_stage"][m, i])
+    t.lambda_ref = float(d["lref"][referent]) if "lref" in d else 5000e-8
+    t.hminus = int(any(int(e) == EQC_HMIN for e in d["mol_eqc"]))
+    if "bf_int" in d:
+        bi, br = d["bf_int"], d["bf_real"]
+        t.nbf = len(bi)
+        for j in range(len(bi)):
+            b = t.bf[j]
+            b.element, b.stage = int(bi[j, 0]) + 1, int(bi[j, 1])
+            b.g, b.energy, b.g_up = float(bi[j, 3]), float(br[j, 0]), float(bi[j, 4])
+            b.energy_up, b.sigma = float(br[j, 1]), float(br[j, 2 + referent])
     if levels:
         lv = d["levels"]
         t.nlevel = len(lv)

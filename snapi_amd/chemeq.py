@@ -10,6 +10,8 @@ from per-species arrays).  Arrays (double, C order):
   stage       [ncol][nstage][nd]    every ionisation stage of every element, table order
   molecule    [ncol][nmolecule][nd]
   level       [ncol][nlevel][nd]    LTE populations of the table's levels
+  opref       [ncol][nd]            continuum opacity at the table's lambda_ref
+                                    (atmosphere::opacity_continuum, atmos/atmos_rad.cc:286-294)
   iterations  [ncol][nd]            Newton steps taken (-1: singular system)
 """
 import ctypes as C
@@ -26,9 +28,9 @@ def _declare(L):
     if getattr(L, "_chem_declared", False):
         return
     TB = C.POINTER(abi.SnapiChemTable)
-    L.snapi_chemeq.argtypes = [TB, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int)]
+    L.snapi_chemeq.argtypes = [TB, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int)]
     L.snapi_chemeq.restype = C.c_int
-    L.snapi_chemeq_dev.argtypes = [TB, C.c_int, C.c_int] + [C.c_void_p] * 8
+    L.snapi_chemeq_dev.argtypes = [TB, C.c_int, C.c_int] + [C.c_void_p] * 9
     L.snapi_chemeq_dev.restype = C.c_int
     L._chem_declared = True
 
@@ -44,7 +46,8 @@ def _shape(T, Nt):
 
 
 def chemeq(table, T, Nt):
-    """dict(ne, stage, molecule, level, iterations) for host arrays T, Nt."""
+    """dict(ne, stage, molecule, level, opref, iterations) for host arrays T, Nt;
+    opref is None when the table has no referent wavelength."""
     L = lib()
     _declare(L)
     T, Nt = _shape(T, Nt)
@@ -54,15 +57,16 @@ def chemeq(table, T, Nt):
     mo = np.zeros((ncol, table.nmolecule, nd))
     lv = np.zeros((ncol, table.nlevel, nd))
     it = np.zeros((ncol, nd), dtype=np.int32)
+    kr = np.zeros((ncol, nd)) if table.lambda_ref > 0 else None
     rc = L.snapi_chemeq(C.byref(table), ncol, nd, abi.ptr(T), abi.ptr(Nt), abi.ptr(ne), abi.ptr(st),
                         abi.ptr(mo) if table.nmolecule else None, abi.ptr(lv) if table.nlevel else None,
-                        it.ctypes.data_as(C.POINTER(C.c_int)))
+                        None if kr is None else abi.ptr(kr), it.ctypes.data_as(C.POINTER(C.c_int)))
     check("snapi_chemeq", rc)
-    return dict(ne=ne, stage=st, molecule=mo, level=lv, iterations=it)
+    return dict(ne=ne, stage=st, molecule=mo, level=lv, opref=kr, iterations=it)
 
 
 def chemeq_dev(table, T_t, Nt_t, ne_t, stage_t=None, molecule_t=None, level_t=None, iterations_t=None,
-               stream=None):
+               stream=None, opref_t=None):
     """snapi_chemeq_dev on torch device tensors (float64; int32 iterations);
     optional outputs may be None."""
     L = lib()
@@ -74,7 +78,8 @@ def chemeq_dev(table, T_t, Nt_t, ne_t, stage_t=None, molecule_t=None, level_t=No
     want = [(T_t, torch.float64, (ncol, nd)), (Nt_t, torch.float64, (ncol, nd)), (ne_t, torch.float64, (ncol, nd)),
             (stage_t, torch.float64, (ncol, abi.chem_stage_count(table), nd)),
             (molecule_t, torch.float64, (ncol, table.nmolecule, nd)),
-            (level_t, torch.float64, (ncol, table.nlevel, nd)), (iterations_t, torch.int32, (ncol, nd))]
+            (level_t, torch.float64, (ncol, table.nlevel, nd)), (opref_t, torch.float64, (ncol, nd)),
+            (iterations_t, torch.int32, (ncol, nd))]
     for t, d, shp in want:
         if t is None:
             continue

@@ -52,6 +52,8 @@ constexpr double kMe = 9.1093818872E-28;
 constexpr double kE = 4.8032042510E-10;
 constexpr double kRyd = 1.097373156853955E5;
 constexpr double kE0 = 7.957747154594775607E-02;
+constexpr double kC = 2.99792458E10;
+constexpr double kSaha = 2.0706615382E-16;  // h^3 / (2 pi me k)^1.5 / 2
 
 // Table-derived constants, computed on the host.
 // Stage descriptor of a Traving partition function: its asymptotic terms
@@ -248,6 +250,25 @@ __device__ double dissoc(const snapi_chem_molecule &m, double T) {
   return 0.0;
 }
 
+// H- bound-free (Gray's polynomial cross-section times the Saha ratio) and
+// free-free opacity per neutral hydrogen atom at wavelength lam [cm]
+// (h_minus_mol::opacity_continuum, mol.cc:669-695).
+__device__ double hminus_opacity(double T, double ne, double NH, double lam) {
+  const double a[7] = {1.99654, -1.18267E-5, 2.64243E-6, -4.40524E-10, 3.23992E-14, -1.39568E-18, 2.78701E-23};
+  double alpha = 0.0;
+  for (int i = 0; i < 7; ++i) alpha += a[i] * pow(lam * 1E8, (double)i);
+  const double T_P = 5040. / T;
+  const double op_bf = 4.158E-28 * alpha * ne * kK * T * pow(T_P, 2.5) * pow(10.0, 0.754 * T_P) * NH *
+                       (1.0 - exp(-kH * kC / lam / kK / T));
+  const double l = log10(lam * 1E8);
+  const double f_0 = -2.2763 - 1.6850 * l + 0.76661 * l * l - 0.053346 * l * l * l;
+  const double f_1 = 15.2827 - 9.2846 * l + 1.99381 * l * l - 0.142631 * l * l * l;
+  const double f_2 = -197.789 + 190.266 * l - 67.9775 * l * l + 10.6913 * l * l * l - 0.625151 * l * l * l * l;
+  const double lt = log10(T_P);
+  const double op_ff = 1E-26 * ne * kK * T * pow(10.0, f_0 + f_1 * lt + f_2 * lt * lt) * NH;
+  return op_bf + op_ff;
+}
+
 // Phases of one point's computation; every phase starts with the ionisation
 // fractions of all elements at the phase's electron density.
 enum : int { kAtOne = 0, kAtNt = 1, kAverage = 2, kNewton = 3, kFinal = 4 };
@@ -259,7 +280,8 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
                                                               double *__restrict__ ne_out,
                                                               double *__restrict__ stage_out,
                                                               double *__restrict__ mol_out,
-                                                              double *__restrict__ lev_out, int *__restrict__ it_out) {
+                                                              double *__restrict__ lev_out,
+                                                              double *__restrict__ opref_out, int *__restrict__ it_out) {
   // table descriptors shared by the workgroup, then one region per point
   __shared__ StageDesc sd_s[kMaxE][kMaxS];
   __shared__ double xl_s[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];
@@ -599,6 +621,42 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
       lev_out[((long)c * nl + l) * nd + zd] = Nz * ((L.g != 0.0) ? L.g * exp(-L.energy / kT) / U : 1.0);
     }
   }
+  // continuum opacity at the referent wavelength (atmosphere::opacity_continuum,
+  // atmos_rad.cc:286-294): Thomson scattering, each element's bound-free
+  // transitions with stimulated emission (atom::opacity_continuum,
+  // atom_rad.cc:151-187; its free-free part is discarded there), then H-
+  if (opref_out) {
+    const double lam = tb->lambda_ref;
+    double part = 0.0;
+    if (is_el) {
+      const NePow w = ne_powers(N0);
+      const double stim = exp(-kH * kC / lam / kK / T), tp = pow(T, -1.5);
+      for (int j = 0; j < tb->nbf; ++j) {
+        const snapi_chem_bf &b = tb->bf[j];
+        if (b.element != lane || b.sigma == 0.0) continue;
+        const int z = b.stage;
+        double Ul, Uu, dU;
+        pf_eval<false>(sd[z], xl, rxl, R[z], R + 2 * ns - 1, x, w, Ul, dU);
+        pf_eval<false>(sd[z + 1], xl, rxl, R[z + 1], R + 2 * ns - 1, x, w, Uu, dU);
+        const double nl = Nown * ((z ? R[ns + z - 1] : 1.0) * rf) *
+                          ((b.g != 0.0) ? b.g * exp(-b.energy / kT) / Ul : 1.0);
+        const double nu = Nown * (R[ns + z] * rf) * ((b.g_up != 0.0) ? b.g_up * exp(-b.energy_up / kT) / Uu : 1.0);
+        const double pop_mod = nu * N0 * kSaha * tp * b.g / b.g_up * exp((el.eion[z] - b.energy) / kK / T);
+        part += b.sigma * (nl - pop_mod * stim);
+      }
+    }
+    double op = 6.65E-25 * N0;
+#pragma unroll
+    for (int a = 1; a < kTeam; ++a) {
+      const double pa = tshfl(part, a);
+      if (a <= na) op += pa;
+    }
+    const double NH = tshfl(Nown * rf, 1);  // neutral hydrogen: element 1, stage 0
+    if (lane == 0) {
+      if (tb->hminus) op += hminus_opacity(T, N0, NH, lam);
+      opref_out[pt] = op;
+    }
+  }
 }
 
 // Checks the table against the kernel's limits and derives its constants.
@@ -695,18 +753,27 @@ int derive(const snapi_chem_table *tb, ChemDerived &d) {
         L.stage >= tb->element[L.element - 1].nstage)
       return SNAPI_ERR_UNSUPPORTED;
   }
+  if (tb->nbf < 0 || tb->nbf > SNAPI_CHEM_MAX_BF) return SNAPI_ERR_UNSUPPORTED;
+  for (int j = 0; j < tb->nbf; ++j) {
+    const snapi_chem_bf &b = tb->bf[j];
+    // a transition needs the stage above it
+    if (b.element < 1 || b.element > tb->nelement || b.stage < 0 ||
+        b.stage + 1 >= tb->element[b.element - 1].nstage)
+      return SNAPI_ERR_UNSUPPORTED;
+  }
   return SNAPI_OK;
 }
 
 int launch_chemeq(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot, long npt, int nd, const double *T,
-                  const double *Nt, double *ne, double *stage, double *mol, double *lev, int *it, hipStream_t st) {
+                  const double *Nt, double *ne, double *stage, double *mol, double *lev, double *opref, int *it,
+                  hipStream_t st) {
   const long threads = npt * kTeam;
   const long blocks = (threads + kChemThreads - 1) / kChemThreads;
   if (blocks > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
   const size_t lds = (size_t)(kChemThreads / kTeam) * nslot * sizeof(double);
   if (lds > 44 * 1024) return SNAPI_ERR_UNSUPPORTED;  // with the static tables, within 64 KiB
   hipLaunchKernelGGL(chemeq_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, npt, T, Nt, nd,
-                     ne, stage, mol, lev, it);
+                     ne, stage, mol, lev, opref, it);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
@@ -737,8 +804,9 @@ int upload_table(const snapi_chem_table *tb, const ChemDerived &d, void **buf, h
 
 extern "C" int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,
                                 const double *Nt_dev, double *ne_dev, double *stage_dev, double *molecule_dev,
-                                double *level_dev, int *iterations_dev, void *stream) {
+                                double *level_dev, double *opref_dev, int *iterations_dev, void *stream) {
   if (!table || ncol <= 0 || nd <= 0 || !T_dev || !Nt_dev || !ne_dev) return SNAPI_ERR_ARG;
+  if (opref_dev && !(table->lambda_ref > 0.0)) return SNAPI_ERR_ARG;
   ChemDerived d;
   int rc = derive(table, d);
   if (rc) return rc;
@@ -749,7 +817,7 @@ extern "C" int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd,
   rc = launch_chemeq((const snapi_chem_table *)buf,
                      (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), d.nslot, (long)ncol * nd, nd, T_dev,
                      Nt_dev, ne_dev, stage_dev, table->nmolecule ? molecule_dev : nullptr,
-                     table->nlevel ? level_dev : nullptr, iterations_dev, st);
+                     table->nlevel ? level_dev : nullptr, opref_dev, iterations_dev, st);
   // the table buffer is freed once the kernel that reads it has finished
   if (hipStreamSynchronize(st) != hipSuccess && rc == SNAPI_OK) rc = SNAPI_ERR_DEVICE;
   hipFree(buf);
@@ -757,14 +825,17 @@ extern "C" int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd,
 }
 
 extern "C" int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, const double *T, const double *Nt,
-                            double *ne, double *stage, double *molecule, double *level, int *iterations) {
+                            double *ne, double *stage, double *molecule, double *level, double *opref,
+                            int *iterations) {
   if (!table || ncol <= 0 || nd <= 0 || !T || !Nt || !ne) return SNAPI_ERR_ARG;
+  if (opref && !(table->lambda_ref > 0.0)) return SNAPI_ERR_ARG;
   ChemDerived d;
   int rc = derive(table, d);
   if (rc) return rc;
   const size_t np = (size_t)ncol * nd;
   const size_t nst = (size_t)d.nst, nm = (size_t)table->nmolecule, nl = (size_t)table->nlevel;
-  double *dT = nullptr, *dNt = nullptr, *dne = nullptr, *dst = nullptr, *dmo = nullptr, *dlv = nullptr;
+  double *dT = nullptr, *dNt = nullptr, *dne = nullptr, *dst = nullptr, *dmo = nullptr, *dlv = nullptr,
+         *dkr = nullptr;
   int *dit = nullptr;
   void *buf = nullptr;
   CCHK(hipMalloc((void **)&dT, np * 8));
@@ -773,6 +844,7 @@ extern "C" int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, con
   if (stage) CCHK(hipMalloc((void **)&dst, np * nst * 8));
   if (molecule && nm) CCHK(hipMalloc((void **)&dmo, np * nm * 8));
   if (level && nl) CCHK(hipMalloc((void **)&dlv, np * nl * 8));
+  if (opref) CCHK(hipMalloc((void **)&dkr, np * 8));
   if (iterations) CCHK(hipMalloc((void **)&dit, np * sizeof(int)));
   CCHK(hipMemcpy(dT, T, np * 8, hipMemcpyHostToDevice));
   CCHK(hipMemcpy(dNt, Nt, np * 8, hipMemcpyHostToDevice));
@@ -780,15 +852,16 @@ extern "C" int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, con
   if (rc) goto done;
   rc = launch_chemeq((const snapi_chem_table *)buf,
                      (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), d.nslot, (long)np, nd, dT, dNt, dne,
-                     dst, dmo, dlv, dit, nullptr);
+                     dst, dmo, dlv, dkr, dit, nullptr);
   if (rc) goto done;
   CCHK(hipDeviceSynchronize());
   CCHK(hipMemcpy(ne, dne, np * 8, hipMemcpyDeviceToHost));
   if (dst) CCHK(hipMemcpy(stage, dst, np * nst * 8, hipMemcpyDeviceToHost));
   if (dmo) CCHK(hipMemcpy(molecule, dmo, np * nm * 8, hipMemcpyDeviceToHost));
   if (dlv) CCHK(hipMemcpy(level, dlv, np * nl * 8, hipMemcpyDeviceToHost));
+  if (dkr) CCHK(hipMemcpy(opref, dkr, np * 8, hipMemcpyDeviceToHost));
   if (dit) CCHK(hipMemcpy(iterations, dit, np * sizeof(int), hipMemcpyDeviceToHost));
 done:
-  hipFree(dT); hipFree(dNt); hipFree(dne); hipFree(dst); hipFree(dmo); hipFree(dlv); hipFree(dit); hipFree(buf);
+  hipFree(dT); hipFree(dNt); hipFree(dne); hipFree(dst); hipFree(dmo); hipFree(dlv); hipFree(dkr); hipFree(dit); hipFree(buf);
   return rc;
 }

@@ -283,9 +283,12 @@ def _chem_lib():
         L.orc_chem_ionfrc.argtypes = [C.POINTER(abi.SnapiChemElement), C.c_double, C.c_double, dp, dp]
         L.orc_chem_dissoc.argtypes = [C.POINTER(abi.SnapiChemMolecule), C.c_double]
         L.orc_chem_dissoc.restype = C.c_double
-        L.orc_chemeq_point.argtypes = [TB, C.c_double, C.c_double, dp, dp, dp, dp]
+        L.orc_chemeq_point.argtypes = [TB, C.c_double, C.c_double, dp, dp, dp, dp, dp]
         L.orc_chemeq_point.restype = C.c_int
-        L.orc_chemeq.argtypes = [TB, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_int), C.c_int]
+        L.orc_chemeq.argtypes = [TB, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_int), C.c_int]
+        L.orc_hequilibrium_batch.argtypes = [TB, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_int),
+                                             C.c_int]
+        L.orc_hequilibrium_batch.restype = C.c_int
         L._chem_ready = True
     return L
 
@@ -307,8 +310,9 @@ def chem_dissoc(molecule, T):
     return _chem_lib().orc_chem_dissoc(C.byref(molecule), float(T))
 
 
-def chemeq(table, T, Nt, nthreads=0):
-    """T, Nt [ncol][nd] -> ne, stage [ncol][nstage][nd], molecule, level, iterations."""
+def chemeq(table, T, Nt, nthreads=0, with_opref=False):
+    """T, Nt [ncol][nd] -> ne, stage [ncol][nstage][nd], molecule, level, iterations
+    (and the referent opacity [ncol][nd] last with with_opref)."""
     from snapi_amd.abi import chem_stage_count
     L = _chem_lib()
     T, Nt = _c(T), _c(Nt)
@@ -318,8 +322,11 @@ def chemeq(table, T, Nt, nthreads=0):
     mo = np.zeros((ncol, table.nmolecule, nd))
     lv = np.zeros((ncol, max(table.nlevel, 1), nd))
     it = np.zeros((ncol, nd), dtype=np.int32)
-    L.orc_chemeq(C.byref(table), ncol, nd, ptr(T), ptr(Nt), ptr(ne), ptr(st), ptr(mo), ptr(lv),
+    kr = np.zeros((ncol, nd))
+    L.orc_chemeq(C.byref(table), ncol, nd, ptr(T), ptr(Nt), ptr(ne), ptr(st), ptr(mo), ptr(lv), ptr(kr),
                  it.ctypes.data_as(C.POINTER(C.c_int)), nthreads if nthreads else (os.cpu_count() or 1))
+    if with_opref:
+        return ne, st, mo, lv[:, :table.nlevel], it, kr
     return ne, st, mo, lv[:, :table.nlevel], it
 
 
@@ -352,3 +359,18 @@ def response_solve(M, B):
     X = np.zeros_like(B)
     L_.orc_response_solve(n, B.shape[0], ptr(M), ptr(B), ptr(X))
     return X
+
+
+def hequilibrium(table, T, logtau, nthreads=0):
+    """T, logtau [ncol][nd] (top first) -> Nt, ne, opref, rho, iterations [ncol][nd]."""
+    L = _chem_lib()
+    T, logtau = _c(T), _c(logtau)
+    if T.ndim == 1:
+        T, logtau = T[None], logtau[None]
+    ncol, nd = T.shape
+    out = [np.zeros((ncol, nd)) for _ in range(4)]
+    it = np.zeros((ncol, nd), dtype=np.int32)
+    rc = L.orc_hequilibrium_batch(C.byref(table), ncol, nd, ptr(T), ptr(logtau), *[ptr(o) for o in out],
+                                  it.ctypes.data_as(C.POINTER(C.c_int)), nthreads if nthreads else (os.cpu_count() or 1))
+    assert rc == 0
+    return (*out, it)

@@ -482,8 +482,8 @@ def parse_chemeq(out):
         T=np.zeros(npt), Nt=np.zeros(npt), ne_in=np.zeros(npt),
         U=np.zeros((npt, nel, S)), dU=np.zeros((npt, nel, S)), fk=np.zeros((npt, nel, S)),
         df=np.zeros((npt, nel, S)), K=np.zeros((npt, nmol)), ne=np.zeros(npt), Nel=np.zeros((npt, nel)),
-        N=np.zeros((npt, nel, S)), hmin=np.zeros(npt))
-    lvl, pops = [], {}
+        N=np.zeros((npt, nel, S)), hmin=np.zeros(npt), opref=np.zeros(npt))
+    lvl, pops, bfs, he = [], {}, [], []
     for t in recs:
         k = t[0]
         if k == "element":
@@ -540,6 +540,20 @@ def parse_chemeq(out):
             pops[(pidx[int(t[1])], int(t[2]) - 1, int(t[3]), int(t[4]))] = float(t[5])
         elif k == "hmin":
             r["hmin"][pidx[int(t[1])]] = float(t[2])
+        elif k == "opref":
+            r["opref"][pidx[int(t[1])]] = float(t[2])
+        elif k == "bf":
+            bfs.append([int(t[1]) - 1, int(t[2]), int(t[3]), int(t[4]), float(t[5]), int(t[6]), float(t[7]),
+                        float(t[8]), float(t[9])])
+        elif k == "lref":
+            r["lref"] = np.array([float(t[1]), float(t[2])])
+        elif k == "he":
+            he.append([float(x) for x in t[2:8]])
+    # bound-free entries: element (0-based), stage, level, g, energy, g and energy of the next
+    # stage's ground level, cross-section at airtovac(5000 A) and at 5000 A
+    r["bf_int"] = np.array([[b[0], b[1], b[2], b[3], b[5]] for b in bfs], np.int32).reshape(-1, 5)
+    r["bf_real"] = np.array([[b[4], b[6], b[7], b[8]] for b in bfs]).reshape(-1, 4)
+    r["he"] = np.array(he).reshape(-1, 6)  # T, log tau, Nt, Ne, op_ref (5000 A), rho after the integration
     r["levels"] = np.array([row[:4] for row in lvl], np.int32)
     r["level_ee"] = np.array([row[4] for row in lvl])
     n = np.zeros((npt, len(lvl)))
@@ -561,10 +575,11 @@ def build_chemeq(d):
         setup_dir(d, rows, cfg0_lambda())
         recs.append(parse_chemeq(run(d, ["chemeq", "atmos.cfg"], he=he)))
     a, b = recs
-    pointwise = ("T", "Nt", "ne_in", "U", "dU", "fk", "df", "K", "ne", "Nel", "N", "hmin", "n")
+    pointwise = ("T", "Nt", "ne_in", "U", "dU", "fk", "df", "K", "ne", "Nel", "N", "hmin", "n", "opref")
     out = {k: (np.concatenate([a[k], b[k]]) if k in pointwise else a[k]) for k in a}
+    out["he"] = b["he"]
     for k in a:
-        if k not in pointwise:
+        if k not in pointwise and k != "he":
             assert np.array_equal(np.nan_to_num(a[k]), np.nan_to_num(b[k])), k
     out["ncolumn_points"] = np.array([len(a["T"]), len(b["T"])], np.int32)
     np.savez_compressed(os.path.join(HERE, "chemeq.npz"), **out)

@@ -37,19 +37,33 @@ def test_library_exports_every_declared_symbol(libpath):
 
 
 def test_struct_layout_matches_header():
-    """ctypes mirror sizes agree with the C layout compiled by the host compiler."""
+    """ctypes mirror sizes and field offsets agree with the C layout compiled
+    by the host compiler."""
     import subprocess
     import tempfile
     from snapi_amd import abi
-    src = ('#include <stdio.h>\n#include "%s/include/snapi_gpu.h"\nint main(){printf("%%zu %%zu %%zu %%zu\\n",'
-           'sizeof(snapi_line),sizeof(snapi_component),sizeof(snapi_line_table),sizeof(snapi_ray));return 0;}\n'
-           % REPO)
+    pairs = [("snapi_line", abi.SnapiLine), ("snapi_component", abi.SnapiComponent),
+             ("snapi_line_table", abi.SnapiLineTable), ("snapi_ray", abi.SnapiRay),
+             ("snapi_ali_transition", abi.SnapiAliTransition), ("snapi_ali_atom", abi.SnapiAliAtom),
+             ("snapi_ali_grid", abi.SnapiAliGrid), ("snapi_partition", abi.SnapiPartition),
+             ("snapi_chem_element", abi.SnapiChemElement), ("snapi_chem_molecule", abi.SnapiChemMolecule),
+             ("snapi_chem_level", abi.SnapiChemLevel), ("snapi_chem_bf", abi.SnapiChemBf),
+             ("snapi_chem_table", abi.SnapiChemTable)]
+    lines = []
+    want = []
+    for cname, cls in pairs:
+        lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        want.append(C.sizeof(cls))
+        for fname, _ in cls._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, fname))
+            want.append(getattr(cls, fname).offset)
+    src = ('#include <stdio.h>\n#include <stddef.h>\n#include "%s/include/snapi_gpu.h"\nint main(){%s return 0;}\n'
+           % (REPO, "".join(lines)))
     with tempfile.TemporaryDirectory() as d:
         open(os.path.join(d, "t.c"), "w").write(src)
         subprocess.run(["gcc", "-o", os.path.join(d, "t"), os.path.join(d, "t.c")], check=True)
         out = subprocess.run([os.path.join(d, "t")], capture_output=True, text=True, check=True).stdout.split()
-    assert [int(x) for x in out] == [C.sizeof(abi.SnapiLine), C.sizeof(abi.SnapiComponent),
-                                     C.sizeof(abi.SnapiLineTable), C.sizeof(abi.SnapiRay)]
+    assert [int(x) for x in out] == want
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

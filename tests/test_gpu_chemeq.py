@@ -2,7 +2,8 @@
 the reference's own results (tests/golden/chemeq.npz, 133 depth points) and
 against the oracle on perturbed columns and on a wide temperature/pressure
 sweep; the device-array entry point against the host one; the pure-hydrogen
-Saha limit; a table without molecules or levels.
+Saha limit; a table without molecules or levels.  The continuum opacity at
+the referent wavelength is checked with the populations.
 
 Tolerance: everything is double precision; the GPU sums the same terms in a
 different association and its exp/pow differ from glibc's in the last bit,
@@ -58,13 +59,14 @@ def test_chemeq_matches_reference(ref):
     nel = ref["Nel"].sum(axis=1)[None]
     compare(tb, got, ref["ne"][None], ref["stage_ref"].T[None], None, ref["n"].T[None], nel, 1e-9)
     e_hm = np.max(rel(got["molecule"][0, 0], ref["hmin"], 1e-300))
-    print("  H- %.2e" % e_hm)
-    assert e_hm < 1e-9
+    e_kr = np.max(rel(got["opref"][0], ref["opref"], 1e-300))
+    print("  H- %.2e  referent opacity %.2e" % (e_hm, e_kr))
+    assert e_hm < 1e-9 and e_kr < 1e-9
 
 
 def oracle_all(tb, T, Nt):
-    ne, st, mo, lv, it = _oracle.chemeq(tb, T, Nt)
-    return dict(ne=ne, stage=st, molecule=mo, level=lv, iterations=it)
+    ne, st, mo, lv, it, kr = _oracle.chemeq(tb, T, Nt, with_opref=True)
+    return dict(ne=ne, stage=st, molecule=mo, level=lv, iterations=it, opref=kr)
 
 
 def check_against_oracle(tb, T, Nt, label):
@@ -78,8 +80,12 @@ def check_against_oracle(tb, T, Nt, label):
     assert (got["iterations"] >= 1).all()
     nel = want["stage"].sum(axis=1)
     compare(tb, got, want["ne"], want["stage"], want["molecule"], want["level"], nel, 1e-9, same)
+    e_kr = rel(got["opref"], want["opref"], 1e-300)
+    print("  referent opacity %.2e" % e_kr[same].max(), flush=True)
+    assert e_kr[same].max() < 1e-9
     if not same.all():
         compare(tb, got, want["ne"], want["stage"], want["molecule"], want["level"], nel, 1e-4, ~same)
+        assert e_kr[~same].max() < 1e-4
     return got
 
 
@@ -110,10 +116,12 @@ def test_chemeq_dev_equals_host(ref):
     mo_t = torch.empty((ncol, tb.nmolecule, nd), dtype=torch.float64, device=dev)
     lv_t = torch.empty((ncol, tb.nlevel, nd), dtype=torch.float64, device=dev)
     it_t = torch.empty((ncol, nd), dtype=torch.int32, device=dev)
+    kr_t = torch.empty((ncol, nd), dtype=torch.float64, device=dev)
     print("chemeq_dev vs chemeq: %d x %d" % (ncol, nd), flush=True)
     s = torch.cuda.Stream()
-    chemeq.chemeq_dev(tb, T_t, Nt_t, ne_t, st_t, mo_t, lv_t, it_t, stream=s)
+    chemeq.chemeq_dev(tb, T_t, Nt_t, ne_t, st_t, mo_t, lv_t, it_t, stream=s, opref_t=kr_t)
     s.synchronize()
+    assert np.array_equal(kr_t.cpu().numpy(), host["opref"])
     assert np.array_equal(ne_t.cpu().numpy(), host["ne"])
     assert np.array_equal(st_t.cpu().numpy(), host["stage"])
     assert np.array_equal(mo_t.cpu().numpy(), host["molecule"])
@@ -163,5 +171,10 @@ def test_unsupported_tables_are_refused(ref):
         chemeq.chemeq(bad, T, Nt)
     bad = copy.deepcopy(ref["table"])
     bad.element[0].nstage = abi.CHEM_MAX_STAGES + 1
+    with pytest.raises(SnapiError):
+        chemeq.chemeq(bad, T, Nt)
+    # a bound-free transition from the highest stage has no stage above it
+    bad = copy.deepcopy(ref["table"])
+    bad.bf[0].stage = bad.element[bad.bf[0].element - 1].nstage - 1
     with pytest.raises(SnapiError):
         chemeq.chemeq(bad, T, Nt)

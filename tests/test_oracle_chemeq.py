@@ -129,3 +129,51 @@ def test_conservation_at_the_result(ref):
     print("particle conservation %.2e, charge conservation %.2e" % (e_p, e_c))
     assert e_p < 1e-10 and e_c < 1e-4
     assert atoms_in_mol.min() >= 1
+
+
+def test_referent_opacity_matches_reference(ref):
+    """Continuum opacity at airtovac(5000 A) after chemeq: the reference's
+    compute_op_referent values on the same 133 points (Thomson, bound-free of
+    H n=3 and two Fe I levels, H- bound-free and free-free)."""
+    t = ref["table"]
+    assert t.hminus == 1 and t.nbf == 3 and abs(t.lambda_ref - ref["lref"][0]) == 0.0
+    *_, kr = _oracle.chemeq(t, ref["T"][None], ref["Nt"][None], with_opref=True)
+    err = np.max(np.abs(kr[0] / ref["opref"] - 1.0))
+    print("referent opacity vs reference: max rel %.2e" % err)
+    assert err < 1e-12
+
+
+def test_bound_free_opacity_is_detailed_balance(ref):
+    """With H- left out, the opacity is Thomson plus sum sigma n_l (1 - e^(-hc/lambda kT)):
+    the Saha-Boltzmann population that the stimulated-emission term uses
+    (pop_mod) equals n_l itself in LTE, so the bound-free term reduces to the
+    textbook LTE form.  A wrong sign on the stimulated term, a wrong Saha
+    constant or exponent, or the upper stage's population taken in place of its
+    ground level fails this.  The identity holds to the last Newton step: the
+    stage populations carry the fractions of that step's electron density and
+    pop_mod the final one, which differ by less than the 1e-4 convergence
+    tolerance, scaled by the stimulated-emission factor (below 0.5 here); a
+    sign error on that term alone moves the result by 2 e^(-hc/lambda kT),
+    above 1e-3 at the hotter points."""
+    import copy
+    t = copy.deepcopy(ref["table"])
+    t.hminus = 0
+    T, Nt = ref["T"][None], ref["Nt"][None]
+    ne, st, mo, lv, it, kr = _oracle.chemeq(t, T, Nt, with_opref=True)
+    soff = np.concatenate([[0], np.cumsum([t.element[e].nstage for e in range(t.nelement)])])
+    lam = t.lambda_ref
+    want = 6.65E-25 * ne[0]
+    for j in range(t.nbf):
+        b = t.bf[j]
+        N = st[0, soff[b.element - 1] + b.stage]
+        U = np.array([_oracle.chem_partition(t.element[b.element - 1].pf[b.stage], T[0, i], ne[0, i])[0]
+                      for i in range(T.shape[1])])
+        nl = N * b.g * np.exp(-b.energy / (K_B * T[0])) / U
+        want = want + b.sigma * nl * (1.0 - np.exp(-H_P * 2.99792458E10 / (lam * K_B * T[0])))
+    err = np.max(np.abs(kr[0] / want - 1.0))
+    print("bound-free opacity vs LTE closed form: max rel %.2e" % err)
+    assert err < 1e-5
+    # no transitions and no H-: Thomson scattering alone
+    t.nbf = 0
+    *_, kr0 = _oracle.chemeq(t, T, Nt, with_opref=True)
+    assert np.allclose(kr0[0], 6.65E-25 * ne[0], rtol=1e-15, atol=0)

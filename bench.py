@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--no-ali", action="store_true")
     ap.add_argument("--chem-side", type=int, default=256, help="chemeq leg: side of the column square")
     ap.add_argument("--no-chem", action="store_true")
+    ap.add_argument("--he-side", type=int, default=128, help="hydrostatic-equilibrium leg: side of the column square")
+    ap.add_argument("--no-he", action="store_true")
     ap.add_argument("--nlteresp-cols", type=int, default=64, help="NLTE-response leg: columns of the formal pass")
     ap.add_argument("--nlteresp-systems", type=int, default=64, help="NLTE-response leg: systems solved")
     ap.add_argument("--no-nlteresp", action="store_true")
@@ -183,6 +185,10 @@ def main():
     if not args.no_chem:
         chem = chemeq_leg(torch, dev, stream, args.chem_side, not args.no_cpu)
 
+    he = None
+    if not args.no_he:
+        he = hequilibrium_leg(torch, dev, stream, args.he_side, not args.no_cpu)
+
     nresp = None
     if not args.no_nlteresp:
         nresp = nlteresp_leg(torch, dev, stream, args.nlteresp_cols, args.nlteresp_systems, not args.no_cpu)
@@ -195,7 +201,7 @@ def main():
                                    % (nx, ny, nd, nlam),
                        "columns": ncol, "depths": nd, "wavelengths": nlam, "lines": table.nline},
             "roofline": roof, "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64, "lm": lm, "ali": alires,
-            "chemeq": chem, "nlte_responses": nresp}
+            "chemeq": chem, "hequilibrium": he, "nlte_responses": nresp}
     print(json.dumps(line))
 
 
@@ -260,6 +266,62 @@ def chemeq_leg(torch, dev, stream, side, with_cpu):
     return {"workload": "chemical equilibrium + LTE populations + referent continuum opacity, %d columns x %d depths, %d elements, %d molecules, "
                         "%d levels, FP64" % (ncol, nd, tb.nelement, tb.nmolecule, tb.nlevel),
             "value": res["points_per_s"], "unit": "depth points/s", "fp64": res, "cpu_baseline": cpu}
+
+
+def hequilibrium_leg(torch, dev, stream, side, with_cpu):
+    from snapi_amd import abi, chemeq, workloads
+    p = workloads.hequilibrium_problem(side, side)
+    tb = p["table"]
+    ncol, nd = p["T"].shape
+    log("hequilibrium workload: %d columns x %d depths" % (ncol, nd))
+    T = torch.from_numpy(p["T"]).to(dev)
+    lt = torch.from_numpy(p["logtau"]).to(dev)
+    f = lambda: torch.empty((ncol, nd), dtype=torch.float64, device=dev)
+    Nt, ne, kr, rho = f(), f(), f(), f()
+    it = torch.empty((ncol, nd), dtype=torch.int32, device=dev)
+
+    def call():
+        chemeq.hequilibrium_dev(tb, T, lt, Nt, ne, kr, rho, iterations_t=it, stream=stream)
+
+    call()  # kernel loading, not timed
+    torch.cuda.synchronize()
+    reps = 3
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    itn = it.cpu().numpy()
+    solves = int(itn.sum()) + ncol * nd  # march iterations plus the final pass
+    res = {"columns": ncol, "depths": nd, "ms_per_call": round(ms, 3), "columns_per_s": ncol / (ms * 1e-3),
+           "chemeq_solves": solves, "solves_per_s": solves / (ms * 1e-3),
+           "mean_iterations": float(itn.mean()), "max_iterations": int(itn.max())}
+    log("hequilibrium: %d columns in %.2f ms = %.4g columns/s (%.4g chemical equilibria/s); iterations mean %.2f max %d"
+        % (ncol, ms, res["columns_per_s"], res["solves_per_s"], itn.mean(), itn.max()))
+    cpu = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import _oracle  # the FP64 CPU restatement: baseline and checker only
+        nthreads = max(1, min(16, os.cpu_count() or 1))
+        nc = min(ncol, 2048)
+        t1 = time.time()
+        Nt_c, ne_c, kr_c, rho_c, it_c = _oracle.hequilibrium(tb, p["T"][:nc], p["logtau"][:nc], nthreads=nthreads)
+        dt = time.time() - t1
+        cpu = {"value": nc / dt, "unit": "columns/s", "cores": nthreads, "kind": "port",
+               "sample": "%d columns x %d depths of the same workload (FP64 oracle, OpenMP over columns)" % (nc, nd),
+               "seconds": round(dt, 2)}
+        same = np.all(itn[:nc] == it_c, axis=1)
+        g = Nt[:nc].cpu().numpy()
+        res["Nt_rel_err_vs_oracle"] = float(np.max(np.abs(g[same] / Nt_c[same] - 1.0))) if same.any() else None
+        res["columns_equal_iterations"] = float(same.mean())
+        log("hequilibrium cpu oracle: %.4g columns/s on %d threads; GPU vs oracle Nt max rel %s, equal iterations %.4f"
+            % (cpu["value"], nthreads, res["Nt_rel_err_vs_oracle"], same.mean()))
+    return {"workload": "hydrostatic equilibrium (chemical equilibrium + referent opacity per iteration), "
+                        "%d columns x %d depths, FP64" % (ncol, nd),
+            "value": res["columns_per_s"], "unit": "columns/s", "fp64": res, "cpu_baseline": cpu}
 
 
 def nlteresp_leg(torch, dev, stream, ncol, nsys, with_cpu):

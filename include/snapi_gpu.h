@@ -36,6 +36,7 @@
  *   snapi_chemeq*            atmosphere::ltepops: chemeq + atom::lte atmos/atmos_pop.cc:30-45,
  *                                                                   atmos/atmos_chm.cc:28-363,
  *                                                                   atmos/atmol/atom/atom.cc:779-787, 892-924
+ *   snapi_hequilibrium*      atmos_pp::enforce_hequilibrium         atmos/plpar/atmos_pp.cc:372-455
  */
 #ifndef SNAPI_GPU_H
 #define SNAPI_GPU_H
@@ -521,6 +522,36 @@ int snapi_response_solve(int nsys, int n, int nrhs, const double *M, const doubl
 int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev, const double *Nt_dev,
                      double *ne_dev, double *stage_dev, double *molecule_dev, double *level_dev, double *opref_dev,
                      int *iterations_dev, void *stream);
+
+/*
+ * Hydrostatic equilibrium: atmos_pp::enforce_hequilibrium
+ * (atmos/plpar/atmos_pp.cc:372-455) for a batch of columns on their
+ * referent optical-depth grids, top first.  The total particle density of
+ * the top point is iterated from a 0.3 dyn/cm^2 guess on
+ * Nt = 10^logtau rho g / kappa / kT; every deeper point takes the pressure
+ * step g / kappa_mean * dtau (kappa_mean: geometric mean of the two points'
+ * opacity per unit mass), iterated with chemical equilibrium at each new Nt
+ * up to 20 times and until a correction is below 1e-2 of the top point's Nt;
+ * g = 274.88e2 cm/s^2 (cmlib/const.h:25), rho = 1.4 * 1.67e-24 g times the
+ * total density of element 1.  The outputs are chemical equilibrium at the
+ * final Nt, as the reference's last pass (atmos_pp.cc:449-454) leaves them.
+ * The opacity is taken at the table's lambda_ref, which must be set
+ * (enforce_hequilibrium uses 5000 A as it stands, without airtovac).
+ *   T, logtau [ncol][nd] in: temperature [K], log10 of the referent optical depth;
+ *   Nt [ncol][nd] out: total particle density [cm^-3];
+ *   ne, opref, rho [ncol][nd] out (may be NULL): electron density, opacity at
+ *     lambda_ref [cm^-1], mass density [g cm^-3];
+ *   stage, molecule, level out (may be NULL): as snapi_chemeq;
+ *   iterations [ncol][nd] out (may be NULL): iterations at each depth (1..20).
+ */
+int snapi_hequilibrium(const snapi_chem_table *table, int ncol, int nd, const double *T, const double *logtau,
+                       double *Nt, double *ne, double *opref, double *rho, double *stage, double *molecule,
+                       double *level, int *iterations);
+/* snapi_hequilibrium on device-resident arrays, all work on `stream`. */
+int snapi_hequilibrium_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,
+                           const double *logtau_dev, double *Nt_dev, double *ne_dev, double *opref_dev,
+                           double *rho_dev, double *stage_dev, double *molecule_dev, double *level_dev,
+                           int *iterations_dev, void *stream);
 
 #ifdef __cplusplus
 }

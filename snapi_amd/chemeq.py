@@ -13,6 +13,12 @@ from per-species arrays).  Arrays (double, C order):
   opref       [ncol][nd]            continuum opacity at the table's lambda_ref
                                     (atmosphere::opacity_continuum, atmos/atmos_rad.cc:286-294)
   iterations  [ncol][nd]            Newton steps taken (-1: singular system)
+
+Hydrostatic equilibrium (snapi_hequilibrium, snapi_hequilibrium_dev):
+atmos_pp::enforce_hequilibrium (atmos/plpar/atmos_pp.cc:372-455) on columns
+of temperature and log referent optical depth, top first; Nt, ne, opref and
+rho [ncol][nd] come out at the final state, iterations counts the HE
+iterations at each depth.
 """
 import ctypes as C
 
@@ -32,6 +38,10 @@ def _declare(L):
     L.snapi_chemeq.restype = C.c_int
     L.snapi_chemeq_dev.argtypes = [TB, C.c_int, C.c_int] + [C.c_void_p] * 9
     L.snapi_chemeq_dev.restype = C.c_int
+    L.snapi_hequilibrium.argtypes = [TB, C.c_int, C.c_int] + [_dp] * 9 + [C.POINTER(C.c_int)]
+    L.snapi_hequilibrium.restype = C.c_int
+    L.snapi_hequilibrium_dev.argtypes = [TB, C.c_int, C.c_int] + [C.c_void_p] * 11
+    L.snapi_hequilibrium_dev.restype = C.c_int
     L._chem_declared = True
 
 
@@ -90,3 +100,55 @@ def chemeq_dev(table, T_t, Nt_t, ne_t, stage_t=None, molecule_t=None, level_t=No
     sp = None if stream is None else C.c_void_p(stream.cuda_stream)
     rc = L.snapi_chemeq_dev(C.byref(table), ncol, nd, *ptrs, sp)
     check("snapi_chemeq_dev", rc)
+
+
+def hequilibrium(table, T, logtau, populations=False):
+    """dict(Nt, ne, opref, rho, iterations) for host arrays T, logtau
+    [ncol][nd] (top first); with populations also stage, molecule, level."""
+    L = lib()
+    _declare(L)
+    T, logtau = _shape(T, logtau)
+    ncol, nd = T.shape
+    out = {k: np.zeros((ncol, nd)) for k in ("Nt", "ne", "opref", "rho")}
+    out["iterations"] = np.zeros((ncol, nd), dtype=np.int32)
+    st = mo = lv = None
+    if populations:
+        st = out["stage"] = np.zeros((ncol, abi.chem_stage_count(table), nd))
+        mo = out["molecule"] = np.zeros((ncol, table.nmolecule, nd))
+        lv = out["level"] = np.zeros((ncol, table.nlevel, nd))
+    rc = L.snapi_hequilibrium(C.byref(table), ncol, nd, abi.ptr(T), abi.ptr(logtau), abi.ptr(out["Nt"]),
+                              abi.ptr(out["ne"]), abi.ptr(out["opref"]), abi.ptr(out["rho"]),
+                              None if st is None else abi.ptr(st),
+                              abi.ptr(mo) if (mo is not None and table.nmolecule) else None,
+                              abi.ptr(lv) if (lv is not None and table.nlevel) else None,
+                              out["iterations"].ctypes.data_as(C.POINTER(C.c_int)))
+    check("snapi_hequilibrium", rc)
+    return out
+
+
+def hequilibrium_dev(table, T_t, logtau_t, Nt_t, ne_t=None, opref_t=None, rho_t=None, stage_t=None,
+                     molecule_t=None, level_t=None, iterations_t=None, stream=None):
+    """snapi_hequilibrium_dev on torch device tensors (float64; int32
+    iterations); optional outputs may be None."""
+    L = lib()
+    _declare(L)
+    import torch
+    if T_t.dim() != 2:
+        raise ValueError("T must be [ncol][nd]")
+    ncol, nd = T_t.shape
+    pt = (ncol, nd)
+    want = [(T_t, torch.float64, pt), (logtau_t, torch.float64, pt), (Nt_t, torch.float64, pt),
+            (ne_t, torch.float64, pt), (opref_t, torch.float64, pt), (rho_t, torch.float64, pt),
+            (stage_t, torch.float64, (ncol, abi.chem_stage_count(table), nd)),
+            (molecule_t, torch.float64, (ncol, table.nmolecule, nd)),
+            (level_t, torch.float64, (ncol, table.nlevel, nd)), (iterations_t, torch.int32, pt)]
+    for t, d, shp in want:
+        if t is None:
+            continue
+        if t.dtype != d or tuple(t.shape) != shp or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("hequilibrium_dev: expected a contiguous device tensor %s of shape %s, got %s %s"
+                             % (d, shp, t.dtype, tuple(t.shape)))
+    ptrs = [None if t is None else t.data_ptr() for t, _, _ in want]
+    sp = None if stream is None else C.c_void_p(stream.cuda_stream)
+    rc = L.snapi_hequilibrium_dev(C.byref(table), ncol, nd, *ptrs, sp)
+    check("snapi_hequilibrium_dev", rc)

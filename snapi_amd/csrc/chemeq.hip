@@ -174,11 +174,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// An element's slots in the point's LDS region R: base[ns] (temperature-only
-// part of U per stage), sx[ns-1] (Saha factor exp(-chi/kT) per stage), the
-// Traving term weights, then from uoff the tables U[ns], 1/U[ns] and
-// dlnU/dne[ns] at the current electron density.
-
 // The lanes of a team evaluate the Traving stages of all elements (the
 // schedule spreads their terms evenly over the 16 lanes) into the U tables.
 template <bool kDeriv>
@@ -203,9 +198,9 @@ __device__ __forceinline__ void eval_stages(const ChemDerived *dv, const StageDe
 
 // atom::ionfrc (atom.cc:892-924) from the U tables: the unnormalised
 // fractions' recursion over the stages.  Without kDeriv only fsum, ez and
-// f1 are formed; with store, fk_{z+1} overwrites sx[z] once used.
+// f1 are formed; with store, fk_{z+1} is kept in frac[z].
 template <bool kDeriv>
-__device__ IonSums ionfrc(int ns, double *R, const double *Ut, double fct, const NePow &w, bool store) {
+__device__ IonSums ionfrc(int ns, double *R, const double *Ut, double *frac, double fct, const NePow &w, bool store) {
   double *sx = R + ns;
   double rUp = Ut[ns], dUp = kDeriv ? Ut[2 * ns] : 0.0;
   IonSums s{1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -228,7 +223,7 @@ __device__ IonSums ionfrc(int ns, double *R, const double *Ut, double fct, const
       s.f1 = fk;
       s.d1 = dfv;
     }
-    if (store) sx[z] = fk;
+    if (store) frac[z] = fk;
     rUp = Ut[ns + z + 1];
   }
   return s;
@@ -273,53 +268,96 @@ __device__ double hminus_opacity(double T, double ne, double NH, double lam) {
 // fractions of all elements at the phase's electron density.
 enum : int { kAtOne = 0, kAtNt = 1, kAverage = 2, kNewton = 3, kFinal = 4 };
 
-__global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_table *__restrict__ tb,
-                                                              const ChemDerived *__restrict__ dv, long npt,
-                                                              const double *__restrict__ Tin,
-                                                              const double *__restrict__ Ntin, int nd,
-                                                              double *__restrict__ ne_out,
-                                                              double *__restrict__ stage_out,
-                                                              double *__restrict__ mol_out,
-                                                              double *__restrict__ lev_out,
-                                                              double *__restrict__ opref_out, int *__restrict__ it_out) {
-  // table descriptors shared by the workgroup, then one region per point
-  __shared__ StageDesc sd_s[kMaxE][kMaxS];
-  __shared__ double xl_s[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];
-  __shared__ double rxl_s[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];
-  __shared__ int nsched_s[kTeam], sched_s[kTeam][kMaxSched], ns_s[kMaxE];
-  extern __shared__ double region_s[];
-  const int tid = threadIdx.x;
-  const int na = tb->nelement, nm = tb->nmolecule;
-  for (int i = tid; i < na * kMaxS; i += kChemThreads) sd_s[i / kMaxS][i % kMaxS] = dv->sd[i / kMaxS][i % kMaxS];
+// Table descriptors that every workgroup copies into LDS.
+struct ChemShared {
+  StageDesc sd[kMaxE][kMaxS];
+  double xl[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];
+  double rxl[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];
+  int nsched[kTeam], sched[kTeam][kMaxSched], ns[kMaxE];
+};
+
+__device__ __forceinline__ void load_shared(ChemShared &sh, const snapi_chem_table *__restrict__ tb,
+                                            const ChemDerived *__restrict__ dv, int tid) {
+  const int na = tb->nelement;
+  for (int i = tid; i < na * kMaxS; i += kChemThreads) sh.sd[i / kMaxS][i % kMaxS] = dv->sd[i / kMaxS][i % kMaxS];
   for (int i = tid; i < na * kMaxS * SNAPI_CHEM_MAX_TERMS; i += kChemThreads) {
     const int a = i / (kMaxS * SNAPI_CHEM_MAX_TERMS), t = i % (kMaxS * SNAPI_CHEM_MAX_TERMS);
-    xl_s[a][t] = dv->xl[a][t];
-    rxl_s[a][t] = dv->rxl[a][t];
+    sh.xl[a][t] = dv->xl[a][t];
+    sh.rxl[a][t] = dv->rxl[a][t];
   }
   if (tid < kTeam) {
-    nsched_s[tid] = dv->nsched[tid];
-    for (int k = 0; k < kMaxSched; ++k) sched_s[tid][k] = dv->sched[tid][k];
+    sh.nsched[tid] = dv->nsched[tid];
+    for (int k = 0; k < kMaxSched; ++k) sh.sched[tid][k] = dv->sched[tid][k];
   }
-  if (tid < na) ns_s[tid] = tb->element[tid].nstage;
+  if (tid < na) sh.ns[tid] = tb->element[tid].nstage;
   __syncthreads();
-  const int lane = tid & (kTeam - 1);
-  const long pt = ((long)blockIdx.x * blockDim.x + tid) / kTeam;
-  if (pt >= npt) return;  // the whole team leaves together; no barrier follows
-  const double T = Tin[pt], Nt = Ntin[pt];
-  const double x = 5040.0 / T, kT = kK * T;
-  const double fct = 2.0 * pow((2.0 * kPi * kMe * kK * T) / (kH * kH), 1.5);
-  const bool is_el = lane >= 1 && lane <= na;
-  const bool is_unk = lane <= na;
-  const int ea = is_el ? lane - 1 : 0;
-  const snapi_chem_element &el = tb->element[ea];
-  const int ns = is_el ? el.nstage : 1;
-  const double ab = is_el ? el.abundance : 0.0;
-  const double ab1 = tb->element[0].abundance;
-  const StageDesc *sd = sd_s[ea];
-  const double *xl = xl_s[ea], *rxl = rxl_s[ea];
-  double *team = region_s + (tid / kTeam) * dv->nslot;
-  double *R = team + dv->roff[ea];
-  double *Ut = R + dv->uoff[ea];
+}
+
+// One lane's place in its team: lane 0 the electrons, lanes 1..na the
+// elements, lane na+1 the right-hand side of the Newton system; an element
+// lane's slots in the team's LDS region: base[ns] (temperature-only part of
+// U per stage), sx[ns-1] (Saha factor exp(-chi/kT) per stage), the Traving
+// term weights, then from uoff the tables U[ns], 1/U[ns], dlnU/dne[ns] at
+// the current electron density and the final unnormalised fractions
+// fk_1..fk_(ns-1).
+struct ChemTeam {
+  int tid, lane, na, nm, ea, ns;
+  bool is_el, is_unk;
+  double ab, ab1, S;
+  double *team, *R, *Ut, *frac;
+  const StageDesc *sd;
+  const double *xl, *rxl;
+};
+
+__device__ __forceinline__ ChemTeam make_team(const snapi_chem_table *__restrict__ tb,
+                                              const ChemDerived *__restrict__ dv, const ChemShared &sh,
+                                              double *region, int tid) {
+  ChemTeam cx;
+  cx.tid = tid;
+  cx.lane = tid & (kTeam - 1);
+  cx.na = tb->nelement;
+  cx.nm = tb->nmolecule;
+  cx.is_el = cx.lane >= 1 && cx.lane <= cx.na;
+  cx.is_unk = cx.lane <= cx.na;
+  cx.ea = cx.is_el ? cx.lane - 1 : 0;
+  cx.ns = cx.is_el ? tb->element[cx.ea].nstage : 1;
+  cx.ab = cx.is_el ? tb->element[cx.ea].abundance : 0.0;
+  cx.ab1 = tb->element[0].abundance;
+  double S = 0.0;
+  for (int a = 0; a < cx.na; ++a) S += tb->element[a].abundance;
+  cx.S = S;
+  cx.sd = sh.sd[cx.ea];
+  cx.xl = sh.xl[cx.ea];
+  cx.rxl = sh.rxl[cx.ea];
+  cx.team = region + (tid / kTeam) * dv->nslot;
+  cx.R = cx.team + dv->roff[cx.ea];
+  cx.Ut = cx.R + dv->uoff[cx.ea];
+  cx.frac = cx.Ut + 3 * cx.ns;
+  return cx;
+}
+
+// Temperature-dependent state of a team's point.
+struct ChemTemp {
+  double T, x, kT, fct, Km[2];
+};
+
+// Writes the temperature-only parts of every element's partition functions
+// and Saha factors into the team's LDS region; the dissociation constants of
+// the lane's molecules (m = lane, lane + 16) stay in registers.
+__device__ __forceinline__ ChemTemp set_temperature(const snapi_chem_table *__restrict__ tb, const ChemTeam &cx,
+                                                    double T) {
+  ChemTemp ct;
+  ct.T = T;
+  const double x = 5040.0 / T;
+  ct.x = x;
+  ct.kT = kK * T;
+  ct.fct = 2.0 * pow((2.0 * kPi * kMe * kK * T) / (kH * kH), 1.5);
+  const int lane = cx.lane, ns = cx.ns, nm = cx.nm;
+  const bool is_el = cx.is_el;
+  const snapi_chem_element &el = tb->element[cx.ea];
+  const StageDesc *sd = cx.sd;
+  double *R = cx.R, *Ut = cx.Ut;
+  double *Km = ct.Km;
   if (is_el) {
     // temperature-only parts: explicit levels of every stage, Saha factors,
     // and the weights of the Traving asymptotic terms
@@ -338,15 +376,39 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
     }
   }
   wave_lds_sync();
-  double Km[2];
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int m = r * kTeam + lane;
     Km[r] = (m < nm) ? dissoc(tb->molecule[m], T) : 1.0;
   }
-  double S = 0.0;
-  for (int a = 0; a < na; ++a) S += tb->element[a].abundance;
+  return ct;
+}
 
+// Result of one point's equilibrium.
+struct ChemSol {
+  double N0, Nown, rf;  // electron density, the lane's unknown, 1 / sum of its fractions
+  IonSums s;
+  int steps;
+};
+
+// atmosphere::chemeq (atmos_chm.cc:28-363) at one point: starting value from
+// ionisation alone, averaging, then Newton steps; the last phase evaluates
+// the fractions at the final electron density's predecessor and stores them.
+__device__ __forceinline__ ChemSol solve_point(const snapi_chem_table *__restrict__ tb,
+                                               const ChemDerived *__restrict__ dv, const ChemShared &sh,
+                                               const ChemTeam &cx, const ChemTemp &ct, double Nt) {
+  const int tid = cx.tid, lane = cx.lane, na = cx.na, nm = cx.nm, ns = cx.ns;
+  const bool is_el = cx.is_el, is_unk = cx.is_unk;
+  const double ab = cx.ab, ab1 = cx.ab1, S = cx.S;
+  const double x = ct.x, kT = ct.kT, fct = ct.fct;
+  const double *Km = ct.Km;
+  double *team = cx.team, *R = cx.R, *Ut = cx.Ut, *frac = cx.frac;
+  const auto &sd_s = sh.sd;
+  const auto &xl_s = sh.xl;
+  const auto &rxl_s = sh.rxl;
+  const auto &nsched_s = sh.nsched;
+  const auto &sched_s = sh.sched;
+  const auto &ns_s = sh.ns;
   const int rhs = na + 1;
   int phase = kAtOne, pass = 0, steps = 0, iter = 0;
   double ne_eval = 1.0, F1 = 0.0, N0 = 0.0, Nown = 0.0;
@@ -363,11 +425,11 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
       if (phase == kNewton) {
         eval_stages<true>(dv, sd_s, xl_s, rxl_s, nsched_s, sched_s, ns_s, lane, team, x, w);
         wave_lds_sync();
-        if (is_el) s = ionfrc<true>(ns, R, Ut, fct, w, false);
+        if (is_el) s = ionfrc<true>(ns, R, Ut, frac, fct, w, false);
       } else {
         eval_stages<false>(dv, sd_s, xl_s, rxl_s, nsched_s, sched_s, ns_s, lane, team, x, w);
         wave_lds_sync();
-        if (is_el) s = ionfrc<false>(ns, R, Ut, fct, w, phase == kFinal);
+        if (is_el) s = ionfrc<false>(ns, R, Ut, frac, fct, w, phase == kFinal);
       }
       wave_lds_sync();
     }
@@ -576,18 +638,96 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
     ne_eval = tshfl(Nown, 0);
   }
 
+  ChemSol sol;
+  sol.N0 = tshfl(Nown, 0);
+  sol.Nown = Nown;
+  sol.rf = 1.0 / s.fsum;
+  sol.s = s;
+  sol.steps = steps;
+  return sol;
+}
+
+// Continuum opacity at the referent wavelength once a point is solved
+// (atmosphere::opacity_continuum, atmos_rad.cc:286-294): Thomson scattering,
+// each element's bound-free transitions with stimulated emission
+// (atom::opacity_continuum, atom_rad.cc:151-187; its free-free part is
+// discarded there), then H- on element 1's neutral stage.  Every lane of the
+// team returns the sum.
+__device__ __forceinline__ double referent_opacity(const snapi_chem_table *__restrict__ tb, const ChemTeam &cx,
+                                                   const ChemTemp &ct, const ChemSol &sol) {
+  const double lam = tb->lambda_ref, T = ct.T, kT = ct.kT, N0 = sol.N0, Nown = sol.Nown, rf = sol.rf;
+  const int lane = cx.lane, ns = cx.ns;
+  const double *R = cx.R, *frac = cx.frac;
+  double part = 0.0;
+  if (cx.is_el) {
+    const NePow w = ne_powers(N0);
+    const double stim = exp(-kH * kC / lam / kK / T), tp = pow(T, -1.5);
+    const snapi_chem_element &el = tb->element[cx.ea];
+    for (int j = 0; j < tb->nbf; ++j) {
+      const snapi_chem_bf &b = tb->bf[j];
+      if (b.element != lane || b.sigma == 0.0) continue;
+      const int z = b.stage;
+      double Ul, Uu, dU;
+      pf_eval<false>(cx.sd[z], cx.xl, cx.rxl, R[z], R + 2 * ns - 1, ct.x, w, Ul, dU);
+      pf_eval<false>(cx.sd[z + 1], cx.xl, cx.rxl, R[z + 1], R + 2 * ns - 1, ct.x, w, Uu, dU);
+      const double nl = Nown * ((z ? frac[z - 1] : 1.0) * rf) * ((b.g != 0.0) ? b.g * exp(-b.energy / kT) / Ul : 1.0);
+      const double nu = Nown * (frac[z] * rf) * ((b.g_up != 0.0) ? b.g_up * exp(-b.energy_up / kT) / Uu : 1.0);
+      const double pop_mod = nu * N0 * kSaha * tp * b.g / b.g_up * exp((el.eion[z] - b.energy) / kK / T);
+      part += b.sigma * (nl - pop_mod * stim);
+    }
+  }
+  double op = 6.65E-25 * N0;
+#pragma unroll
+  for (int a = 1; a < kTeam; ++a) {
+    const double pa = tshfl(part, a);
+    if (a <= cx.na) op += pa;
+  }
+  const double NH = tshfl(Nown * rf, 1);  // neutral hydrogen: element 1, stage 0
+  if (tb->hminus) op += hminus_opacity(T, N0, NH, lam);
+  return op;
+}
+
+// Mass density from the total density of element 1, 1.4 hydrogen masses per
+// hydrogen atom (atmos_pp.cc:404), in the reference's operation order.
+__device__ __forceinline__ double mass_density(double NH) { return NH * 1.4 * 1.67E-24; }
+
+__global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_table *__restrict__ tb,
+                                                              const ChemDerived *__restrict__ dv, long npt,
+                                                              const double *__restrict__ Tin,
+                                                              const double *__restrict__ Ntin, int nd,
+                                                              double *__restrict__ ne_out,
+                                                              double *__restrict__ stage_out,
+                                                              double *__restrict__ mol_out,
+                                                              double *__restrict__ lev_out,
+                                                              double *__restrict__ opref_out,
+                                                              double *__restrict__ rho_out, int *__restrict__ it_out) {
+  __shared__ ChemShared sh;
+  extern __shared__ double region_s[];
+  const int tid = threadIdx.x;
+  load_shared(sh, tb, dv, tid);
+  const ChemTeam cx = make_team(tb, dv, sh, region_s, tid);
+  const long pt = ((long)blockIdx.x * blockDim.x + tid) / kTeam;
+  if (pt >= npt) return;  // the whole team leaves together; no barrier follows
+  const ChemTemp ct = set_temperature(tb, cx, Tin[pt]);
+  const ChemSol sol = solve_point(tb, dv, sh, cx, ct, Ntin[pt]);
+  const int lane = cx.lane, ns = cx.ns, nm = cx.nm;
+  const bool is_el = cx.is_el;
+  const double x = ct.x, kT = ct.kT;
+  const double *Km = ct.Km, *R = cx.R, *frac = cx.frac;
+  const StageDesc *sd = cx.sd;
+  const double *xl = cx.xl, *rxl = cx.rxl;
   // outputs: the final N with the last step's fractions (atom::pupdate, atom.cc:1449-1453)
   const long c = pt / nd, zd = pt % nd;
-  N0 = tshfl(Nown, 0);
-  const double rf = 1.0 / s.fsum;
+  const double N0 = sol.N0, Nown = sol.Nown, rf = sol.rf;
+  const IonSums &s = sol.s;
   if (lane == 0) {
-    ne_out[pt] = N0;
-    if (it_out) it_out[pt] = steps;
+    if (ne_out) ne_out[pt] = N0;
+    if (it_out) it_out[pt] = sol.steps;
   }
   if (stage_out && is_el) {
     const int off = dv->soff[lane - 1], nst = dv->nst;
     for (int z = 0; z < ns; ++z)
-      stage_out[((long)c * nst + off + z) * nd + zd] = Nown * ((z ? R[ns + z - 1] : 1.0) * rf);
+      stage_out[((long)c * nst + off + z) * nd + zd] = Nown * ((z ? frac[z - 1] : 1.0) * rf);
   }
   if (mol_out) {
     const double fk0 = is_el ? rf : 1.0, fk1 = is_el ? s.f1 * rf : 0.0;
@@ -615,47 +755,85 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
       const snapi_chem_level &L = tb->level[l];
       if (L.element != lane) continue;
       const int z = L.stage;
-      const double Nz = Nown * ((z ? R[ns + z - 1] : 1.0) * rf);
+      const double Nz = Nown * ((z ? frac[z - 1] : 1.0) * rf);
       double U, dU;
       pf_eval<false>(sd[z], xl, rxl, R[z], R + 2 * ns - 1, x, w, U, dU);
       lev_out[((long)c * nl + l) * nd + zd] = Nz * ((L.g != 0.0) ? L.g * exp(-L.energy / kT) / U : 1.0);
     }
   }
-  // continuum opacity at the referent wavelength (atmosphere::opacity_continuum,
-  // atmos_rad.cc:286-294): Thomson scattering, each element's bound-free
-  // transitions with stimulated emission (atom::opacity_continuum,
-  // atom_rad.cc:151-187; its free-free part is discarded there), then H-
   if (opref_out) {
-    const double lam = tb->lambda_ref;
-    double part = 0.0;
-    if (is_el) {
-      const NePow w = ne_powers(N0);
-      const double stim = exp(-kH * kC / lam / kK / T), tp = pow(T, -1.5);
-      for (int j = 0; j < tb->nbf; ++j) {
-        const snapi_chem_bf &b = tb->bf[j];
-        if (b.element != lane || b.sigma == 0.0) continue;
-        const int z = b.stage;
-        double Ul, Uu, dU;
-        pf_eval<false>(sd[z], xl, rxl, R[z], R + 2 * ns - 1, x, w, Ul, dU);
-        pf_eval<false>(sd[z + 1], xl, rxl, R[z + 1], R + 2 * ns - 1, x, w, Uu, dU);
-        const double nl = Nown * ((z ? R[ns + z - 1] : 1.0) * rf) *
-                          ((b.g != 0.0) ? b.g * exp(-b.energy / kT) / Ul : 1.0);
-        const double nu = Nown * (R[ns + z] * rf) * ((b.g_up != 0.0) ? b.g_up * exp(-b.energy_up / kT) / Uu : 1.0);
-        const double pop_mod = nu * N0 * kSaha * tp * b.g / b.g_up * exp((el.eion[z] - b.energy) / kK / T);
-        part += b.sigma * (nl - pop_mod * stim);
+    const double op = referent_opacity(tb, cx, ct, sol);
+    if (lane == 0) opref_out[pt] = op;
+  }
+  if (rho_out && lane == 1) rho_out[pt] = mass_density(Nown);
+}
+
+constexpr double kGrav = 274.88E2;  // gravitational acceleration at the solar surface [cm/s^2]
+constexpr int kHeMaxIter = 20;      // iterations per depth (atmos_pp.cc:385)
+
+// Hydrostatic equilibrium of a batch of columns, one team per column
+// (atmos_pp::enforce_hequilibrium, atmos_pp.cc:372-455).  The top point
+// starts from 0.3 dyn/cm^2 and iterates Nt = 10^logtau rho g / kappa / kT
+// until a correction is below 1e-2 of Nt; each deeper point starts from the
+// pressure step rho g / kappa * dtau of the point above and iterates the step
+// g / kappa_mean * dtau, kappa_mean the geometric mean of the two points'
+// opacity per unit mass, until a correction is below 1e-2 of the TOP point's
+// Nt, for at most 20 iterations each.  A point's opacity and density for the
+// next step are those of its last chemical equilibrium, evaluated before its
+// last correction.  The temperature-only parts are set once per depth and kept
+// in LDS through that depth's iterations.  Writes the final Nt and the
+// iterations per depth; chemical equilibrium at the final Nt follows in
+// chemeq_kernel.
+__global__ __launch_bounds__(kChemThreads) void hequilibrium_kernel(const snapi_chem_table *__restrict__ tb,
+                                                                    const ChemDerived *__restrict__ dv, long ncol,
+                                                                    int nd, const double *__restrict__ Tin,
+                                                                    const double *__restrict__ logtau,
+                                                                    double *__restrict__ Nt_out,
+                                                                    int *__restrict__ it_out) {
+  __shared__ ChemShared sh;
+  extern __shared__ double region_s[];
+  const int tid = threadIdx.x;
+  load_shared(sh, tb, dv, tid);
+  const ChemTeam cx = make_team(tb, dv, sh, region_s, tid);
+  const long col = ((long)blockIdx.x * blockDim.x + tid) / kTeam;
+  if (col >= ncol) return;  // the whole team leaves together; no barrier follows
+  const double *T = Tin + col * nd, *lt = logtau + col * nd;
+  double Ntop = 0.0, Nprev = 0.0, Tprev = 0.0, tprev = 0.0, op_prev = 0.0, rho_prev = 0.0;
+  for (int z = 0; z < nd; ++z) {
+    // every value below is the same on all lanes of the team
+    const double Tz = T[z];
+    const ChemTemp ct = set_temperature(tb, cx, Tz);
+    const double tau = pow(10.0, lt[z]);
+    const double dtau = z ? tau - tprev : 0.0;
+    double Nt = z ? (Nprev * kK * Tprev + rho_prev * kGrav / op_prev * dtau) / kK / Tz : 0.3 / kK / Tz;
+    double op = 0.0, rho = 0.0;
+    int it = 0;
+    while (it < kHeMaxIter) {
+      const ChemSol sol = solve_point(tb, dv, sh, cx, ct, Nt);
+      op = referent_opacity(tb, cx, ct, sol);
+      rho = mass_density(tshfl(sol.Nown, 1));
+      double dN;
+      if (z == 0) {
+        dN = (tau * rho * kGrav / op) / kK / Tz - Nt;
+      } else {
+        const double kappa_mean = sqrt(op * op_prev / rho / rho_prev);
+        const double delta_p = kGrav / kappa_mean * dtau;
+        dN = (Nprev * kK * Tprev + delta_p) / kK / Tz - Nt;
       }
+      Nt += dN;
+      ++it;
+      if (fabs(dN / (z ? Ntop : Nt)) < 1E-2) break;
     }
-    double op = 6.65E-25 * N0;
-#pragma unroll
-    for (int a = 1; a < kTeam; ++a) {
-      const double pa = tshfl(part, a);
-      if (a <= na) op += pa;
+    if (z == 0) Ntop = Nt;
+    if (cx.lane == 0) {
+      Nt_out[col * nd + z] = Nt;
+      if (it_out) it_out[col * nd + z] = it;
     }
-    const double NH = tshfl(Nown * rf, 1);  // neutral hydrogen: element 1, stage 0
-    if (lane == 0) {
-      if (tb->hminus) op += hminus_opacity(T, N0, NH, lam);
-      opref_out[pt] = op;
-    }
+    Nprev = Nt;
+    Tprev = Tz;
+    tprev = tau;
+    op_prev = op;
+    rho_prev = rho;
   }
 }
 
@@ -710,7 +888,7 @@ int derive(const snapi_chem_table *tb, ChemDerived &d) {
     }
     d.roff[a] = slot;
     d.uoff[a] = 2 * el.nstage - 1 + nt;
-    slot += 5 * el.nstage - 1 + nt;
+    slot += 6 * el.nstage - 2 + nt;
   }
   d.nslot = slot;
   // Traving stages to lanes, longest first onto the least-loaded lane
@@ -764,16 +942,32 @@ int derive(const snapi_chem_table *tb, ChemDerived &d) {
   return SNAPI_OK;
 }
 
+// Workgroups for n teams and the dynamic LDS of their regions; false when
+// either exceeds what a launch takes.
+bool team_grid(long n, int nslot, long &blocks, size_t &lds) {
+  blocks = (n * kTeam + kChemThreads - 1) / kChemThreads;
+  lds = (size_t)(kChemThreads / kTeam) * nslot * sizeof(double);
+  return blocks <= 0x7fffffffL && lds <= 44 * 1024;  // with the static tables, within 64 KiB
+}
+
 int launch_chemeq(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot, long npt, int nd, const double *T,
-                  const double *Nt, double *ne, double *stage, double *mol, double *lev, double *opref, int *it,
-                  hipStream_t st) {
-  const long threads = npt * kTeam;
-  const long blocks = (threads + kChemThreads - 1) / kChemThreads;
-  if (blocks > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
-  const size_t lds = (size_t)(kChemThreads / kTeam) * nslot * sizeof(double);
-  if (lds > 44 * 1024) return SNAPI_ERR_UNSUPPORTED;  // with the static tables, within 64 KiB
+                  const double *Nt, double *ne, double *stage, double *mol, double *lev, double *opref, double *rho,
+                  int *it, hipStream_t st) {
+  long blocks;
+  size_t lds;
+  if (!team_grid(npt, nslot, blocks, lds)) return SNAPI_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(chemeq_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, npt, T, Nt, nd,
-                     ne, stage, mol, lev, opref, it);
+                     ne, stage, mol, lev, opref, rho, it);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+int launch_hequilibrium(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot, long ncol, int nd,
+                        const double *T, const double *logtau, double *Nt, int *it, hipStream_t st) {
+  long blocks;
+  size_t lds;
+  if (!team_grid(ncol, nslot, blocks, lds)) return SNAPI_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(hequilibrium_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, ncol, nd, T,
+                     logtau, Nt, it);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
@@ -817,7 +1011,7 @@ extern "C" int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd,
   rc = launch_chemeq((const snapi_chem_table *)buf,
                      (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), d.nslot, (long)ncol * nd, nd, T_dev,
                      Nt_dev, ne_dev, stage_dev, table->nmolecule ? molecule_dev : nullptr,
-                     table->nlevel ? level_dev : nullptr, opref_dev, iterations_dev, st);
+                     table->nlevel ? level_dev : nullptr, opref_dev, nullptr, iterations_dev, st);
   // the table buffer is freed once the kernel that reads it has finished
   if (hipStreamSynchronize(st) != hipSuccess && rc == SNAPI_OK) rc = SNAPI_ERR_DEVICE;
   hipFree(buf);
@@ -852,7 +1046,7 @@ extern "C" int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, con
   if (rc) goto done;
   rc = launch_chemeq((const snapi_chem_table *)buf,
                      (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), d.nslot, (long)np, nd, dT, dNt, dne,
-                     dst, dmo, dlv, dkr, dit, nullptr);
+                     dst, dmo, dlv, dkr, nullptr, dit, nullptr);
   if (rc) goto done;
   CCHK(hipDeviceSynchronize());
   CCHK(hipMemcpy(ne, dne, np * 8, hipMemcpyDeviceToHost));
@@ -863,5 +1057,72 @@ extern "C" int snapi_chemeq(const snapi_chem_table *table, int ncol, int nd, con
   if (dit) CCHK(hipMemcpy(iterations, dit, np * sizeof(int), hipMemcpyDeviceToHost));
 done:
   hipFree(dT); hipFree(dNt); hipFree(dne); hipFree(dst); hipFree(dmo); hipFree(dlv); hipFree(dkr); hipFree(dit); hipFree(buf);
+  return rc;
+}
+
+extern "C" int snapi_hequilibrium_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,
+                                      const double *logtau_dev, double *Nt_dev, double *ne_dev, double *opref_dev,
+                                      double *rho_dev, double *stage_dev, double *molecule_dev, double *level_dev,
+                                      int *iterations_dev, void *stream) {
+  if (!table || ncol <= 0 || nd <= 0 || !T_dev || !logtau_dev || !Nt_dev) return SNAPI_ERR_ARG;
+  if (!(table->lambda_ref > 0.0)) return SNAPI_ERR_ARG;
+  ChemDerived d;
+  int rc = derive(table, d);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  void *buf = nullptr;
+  rc = upload_table(table, d, &buf, st);
+  if (rc) return rc;
+  const snapi_chem_table *dtb = (const snapi_chem_table *)buf;
+  const ChemDerived *ddv = (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table));
+  rc = launch_hequilibrium(dtb, ddv, d.nslot, ncol, nd, T_dev, logtau_dev, Nt_dev, iterations_dev, st);
+  // chemical equilibrium, LTE populations, opacity and density at the final Nt (atmos_pp.cc:449-454)
+  if (rc == SNAPI_OK && (ne_dev || opref_dev || rho_dev || stage_dev || molecule_dev || level_dev))
+    rc = launch_chemeq(dtb, ddv, d.nslot, (long)ncol * nd, nd, T_dev, Nt_dev, ne_dev, stage_dev,
+                       table->nmolecule ? molecule_dev : nullptr, table->nlevel ? level_dev : nullptr, opref_dev,
+                       rho_dev, nullptr, st);
+  if (hipStreamSynchronize(st) != hipSuccess && rc == SNAPI_OK) rc = SNAPI_ERR_DEVICE;
+  hipFree(buf);
+  return rc;
+}
+
+extern "C" int snapi_hequilibrium(const snapi_chem_table *table, int ncol, int nd, const double *T,
+                                  const double *logtau, double *Nt, double *ne, double *opref, double *rho,
+                                  double *stage, double *molecule, double *level, int *iterations) {
+  if (!table || ncol <= 0 || nd <= 0 || !T || !logtau || !Nt) return SNAPI_ERR_ARG;
+  if (!(table->lambda_ref > 0.0)) return SNAPI_ERR_ARG;
+  ChemDerived d;
+  int rc = derive(table, d);
+  if (rc) return rc;
+  const size_t np = (size_t)ncol * nd;
+  const size_t nst = (size_t)d.nst, nm = (size_t)table->nmolecule, nl = (size_t)table->nlevel;
+  double *dT = nullptr, *dlt = nullptr, *dNt = nullptr, *dne = nullptr, *dkr = nullptr, *drho = nullptr,
+         *dst = nullptr, *dmo = nullptr, *dlv = nullptr;
+  int *dit = nullptr;
+  CCHK(hipMalloc((void **)&dT, np * 8));
+  CCHK(hipMalloc((void **)&dlt, np * 8));
+  CCHK(hipMalloc((void **)&dNt, np * 8));
+  if (ne) CCHK(hipMalloc((void **)&dne, np * 8));
+  if (opref) CCHK(hipMalloc((void **)&dkr, np * 8));
+  if (rho) CCHK(hipMalloc((void **)&drho, np * 8));
+  if (stage) CCHK(hipMalloc((void **)&dst, np * nst * 8));
+  if (molecule && nm) CCHK(hipMalloc((void **)&dmo, np * nm * 8));
+  if (level && nl) CCHK(hipMalloc((void **)&dlv, np * nl * 8));
+  if (iterations) CCHK(hipMalloc((void **)&dit, np * sizeof(int)));
+  CCHK(hipMemcpy(dT, T, np * 8, hipMemcpyHostToDevice));
+  CCHK(hipMemcpy(dlt, logtau, np * 8, hipMemcpyHostToDevice));
+  rc = snapi_hequilibrium_dev(table, ncol, nd, dT, dlt, dNt, dne, dkr, drho, dst, dmo, dlv, dit, nullptr);
+  if (rc) goto done;
+  CCHK(hipMemcpy(Nt, dNt, np * 8, hipMemcpyDeviceToHost));
+  if (dne) CCHK(hipMemcpy(ne, dne, np * 8, hipMemcpyDeviceToHost));
+  if (dkr) CCHK(hipMemcpy(opref, dkr, np * 8, hipMemcpyDeviceToHost));
+  if (drho) CCHK(hipMemcpy(rho, drho, np * 8, hipMemcpyDeviceToHost));
+  if (dst) CCHK(hipMemcpy(stage, dst, np * nst * 8, hipMemcpyDeviceToHost));
+  if (dmo) CCHK(hipMemcpy(molecule, dmo, np * nm * 8, hipMemcpyDeviceToHost));
+  if (dlv) CCHK(hipMemcpy(level, dlv, np * nl * 8, hipMemcpyDeviceToHost));
+  if (dit) CCHK(hipMemcpy(iterations, dit, np * sizeof(int), hipMemcpyDeviceToHost));
+done:
+  hipFree(dT); hipFree(dlt); hipFree(dNt); hipFree(dne); hipFree(dkr); hipFree(drho);
+  hipFree(dst); hipFree(dmo); hipFree(dlv); hipFree(dit);
   return rc;
 }

@@ -356,3 +356,21 @@ def chemeq_problem(nx, ny, seed=5):
     T = np.ascontiguousarray(T0[None, :] + dT[:, None])
     Nt = np.ascontiguousarray(Nt0[None, :] * 10.0 ** dn[:, None])
     return dict(table=table, T=T, Nt=Nt)
+
+
+def hequilibrium_problem(nx, ny, seed=6):
+    """Columns for hydrostatic equilibrium: the species table recorded from the
+    reference with the opacity taken at 5000 A as enforce_hequilibrium does,
+    and nx*ny columns of the cfg0 column's temperatures on its 61-point
+    log tau grid (-6 .. 1), with a Gaussian random field of 150 K rms per
+    column plus 30 K rms independent noise per depth.  Returns dict(table,
+    T [ncol][nd], logtau [ncol][nd])."""
+    d = np.load(os.path.join(GOLDEN, "chemeq.npz"))
+    table = abi.make_chem_table(d, referent=1)
+    he = d["he"]
+    rng = np.random.default_rng(seed)
+    ncol, nd = nx * ny, he.shape[0]
+    dT = gaussian_random_field(nx, ny, 150.0, 8.0, rng).reshape(ncol)
+    T = np.ascontiguousarray(he[None, :, 0] + dT[:, None] + rng.normal(0.0, 30.0, (ncol, nd)))
+    logtau = np.ascontiguousarray(np.broadcast_to(he[None, :, 1], (ncol, nd)))
+    return dict(table=table, T=T, logtau=logtau)

@@ -75,6 +75,7 @@ struct ChemDerived {
   int nsched[kTeam];                  // Traving stages evaluated by each lane of a team
   int sched[kTeam][kMaxSched];        // element * kMaxS + stage
   double travf;                       // highest-level constant of trav::U (partf.cc:222)
+  double hm[4];                       // H- at lambda_ref: Gray's alpha and the free-free f_0, f_1, f_2
   StageDesc sd[kMaxE][kMaxS];
   double xl[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS];  // first quantum number of each term
   double rxl[kMaxE][kMaxS * SNAPI_CHEM_MAX_TERMS]; // its reciprocal
@@ -245,22 +246,28 @@ __device__ double dissoc(const snapi_chem_molecule &m, double T) {
   return 0.0;
 }
 
-// H- bound-free (Gray's polynomial cross-section times the Saha ratio) and
-// free-free opacity per neutral hydrogen atom at wavelength lam [cm]
-// (h_minus_mol::opacity_continuum, mol.cc:669-695).
-__device__ double hminus_opacity(double T, double ne, double NH, double lam) {
+// The wavelength-only parts of the H- opacity (h_minus_mol::opacity_continuum,
+// mol.cc:669-695): Gray's bound-free cross-section polynomial alpha(lambda)
+// and the coefficients f_0, f_1, f_2 of the free-free fit, lam in cm.
+void hminus_lambda_terms(double lam, double hm[4]) {
   const double a[7] = {1.99654, -1.18267E-5, 2.64243E-6, -4.40524E-10, 3.23992E-14, -1.39568E-18, 2.78701E-23};
   double alpha = 0.0;
   for (int i = 0; i < 7; ++i) alpha += a[i] * pow(lam * 1E8, (double)i);
-  const double T_P = 5040. / T;
-  const double op_bf = 4.158E-28 * alpha * ne * kK * T * pow(T_P, 2.5) * pow(10.0, 0.754 * T_P) * NH *
-                       (1.0 - exp(-kH * kC / lam / kK / T));
   const double l = log10(lam * 1E8);
-  const double f_0 = -2.2763 - 1.6850 * l + 0.76661 * l * l - 0.053346 * l * l * l;
-  const double f_1 = 15.2827 - 9.2846 * l + 1.99381 * l * l - 0.142631 * l * l * l;
-  const double f_2 = -197.789 + 190.266 * l - 67.9775 * l * l + 10.6913 * l * l * l - 0.625151 * l * l * l * l;
+  hm[0] = alpha;
+  hm[1] = -2.2763 - 1.6850 * l + 0.76661 * l * l - 0.053346 * l * l * l;
+  hm[2] = 15.2827 - 9.2846 * l + 1.99381 * l * l - 0.142631 * l * l * l;
+  hm[3] = -197.789 + 190.266 * l - 67.9775 * l * l + 10.6913 * l * l * l - 0.625151 * l * l * l * l;
+}
+
+// H- bound-free (the cross-section times the Saha ratio, with stimulated
+// emission factor stim_1 = 1 - exp(-hc/lambda kT)) and free-free opacity on
+// NH neutral hydrogen atoms (mol.cc:669-695), from the wavelength terms hm.
+__device__ double hminus_opacity(double T, double ne, double NH, const double *hm, double stim_1) {
+  const double T_P = 5040. / T;
+  const double op_bf = 4.158E-28 * hm[0] * ne * kK * T * pow(T_P, 2.5) * pow(10.0, 0.754 * T_P) * NH * stim_1;
   const double lt = log10(T_P);
-  const double op_ff = 1E-26 * ne * kK * T * pow(10.0, f_0 + f_1 * lt + f_2 * lt * lt) * NH;
+  const double op_ff = 1E-26 * ne * kK * T * pow(10.0, hm[1] + hm[2] * lt + hm[3] * lt * lt) * NH;
   return op_bf + op_ff;
 }
 
@@ -653,15 +660,17 @@ __device__ __forceinline__ ChemSol solve_point(const snapi_chem_table *__restric
 // (atom::opacity_continuum, atom_rad.cc:151-187; its free-free part is
 // discarded there), then H- on element 1's neutral stage.  Every lane of the
 // team returns the sum.
-__device__ __forceinline__ double referent_opacity(const snapi_chem_table *__restrict__ tb, const ChemTeam &cx,
+__device__ __forceinline__ double referent_opacity(const snapi_chem_table *__restrict__ tb,
+                                                   const ChemDerived *__restrict__ dv, const ChemTeam &cx,
                                                    const ChemTemp &ct, const ChemSol &sol) {
   const double lam = tb->lambda_ref, T = ct.T, kT = ct.kT, N0 = sol.N0, Nown = sol.Nown, rf = sol.rf;
   const int lane = cx.lane, ns = cx.ns;
   const double *R = cx.R, *frac = cx.frac;
   double part = 0.0;
+  const double stim = exp(-kH * kC / lam / kK / T);
   if (cx.is_el) {
     const NePow w = ne_powers(N0);
-    const double stim = exp(-kH * kC / lam / kK / T), tp = pow(T, -1.5);
+    const double tp = pow(T, -1.5);
     const snapi_chem_element &el = tb->element[cx.ea];
     for (int j = 0; j < tb->nbf; ++j) {
       const snapi_chem_bf &b = tb->bf[j];
@@ -683,7 +692,7 @@ __device__ __forceinline__ double referent_opacity(const snapi_chem_table *__res
     if (a <= cx.na) op += pa;
   }
   const double NH = tshfl(Nown * rf, 1);  // neutral hydrogen: element 1, stage 0
-  if (tb->hminus) op += hminus_opacity(T, N0, NH, lam);
+  if (tb->hminus) op += hminus_opacity(T, N0, NH, dv->hm, 1.0 - stim);
   return op;
 }
 
@@ -762,7 +771,7 @@ __global__ __launch_bounds__(kChemThreads) void chemeq_kernel(const snapi_chem_t
     }
   }
   if (opref_out) {
-    const double op = referent_opacity(tb, cx, ct, sol);
+    const double op = referent_opacity(tb, dv, cx, ct, sol);
     if (lane == 0) opref_out[pt] = op;
   }
   if (rho_out && lane == 1) rho_out[pt] = mass_density(Nown);
@@ -784,7 +793,7 @@ constexpr int kHeMaxIter = 20;      // iterations per depth (atmos_pp.cc:385)
 // in LDS through that depth's iterations.  Writes the final Nt and the
 // iterations per depth; chemical equilibrium at the final Nt follows in
 // chemeq_kernel.
-__global__ __launch_bounds__(kChemThreads) void hequilibrium_kernel(const snapi_chem_table *__restrict__ tb,
+__global__ __launch_bounds__(kChemThreads, 2) void hequilibrium_kernel(const snapi_chem_table *__restrict__ tb,
                                                                     const ChemDerived *__restrict__ dv, long ncol,
                                                                     int nd, const double *__restrict__ Tin,
                                                                     const double *__restrict__ logtau,
@@ -810,7 +819,7 @@ __global__ __launch_bounds__(kChemThreads) void hequilibrium_kernel(const snapi_
     int it = 0;
     while (it < kHeMaxIter) {
       const ChemSol sol = solve_point(tb, dv, sh, cx, ct, Nt);
-      op = referent_opacity(tb, cx, ct, sol);
+      op = referent_opacity(tb, dv, cx, ct, sol);
       rho = mass_density(tshfl(sol.Nown, 1));
       double dN;
       if (z == 0) {
@@ -866,6 +875,7 @@ int derive(const snapi_chem_table *tb, ChemDerived &d) {
   d.nst = nst;
   const double er = kE * kRyd * kRyd;
   d.travf = 1.405169521381833474E+05 * ((4.0 * kPi * kE0) / (er * er));
+  if (tb->lambda_ref > 0.0) hminus_lambda_terms(tb->lambda_ref, d.hm);
   int slot = 0;
   for (int a = 0; a < tb->nelement; ++a) {
     const snapi_chem_element &el = tb->element[a];

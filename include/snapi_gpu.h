@@ -456,6 +456,17 @@ typedef struct {
   snapi_chem_bf bf[SNAPI_CHEM_MAX_BF];
 } snapi_chem_table;
 
+/* Where the chemical-equilibrium results go in a column state (see
+ * snapi_eos_state). */
+typedef struct {
+  int nline;                           /* lines of the line table: nfield = SNAPI_NF_BASE + 2 nline */
+  int line_level[SNAPI_MAX_LINES][2];  /* table level (0-based) of each line's lower, upper level */
+  int nhi_element;                     /* element (1-based) whose neutral-stage density is NHI */
+  double lambda_mid;                   /* Thomson emissivity wavelength [cm] (vacuum) */
+  double lambda_bg;                    /* wavelength of the bound-free background [cm] (vacuum) */
+  double sigma_bg[SNAPI_CHEM_MAX_BF];  /* cross-section of table bf transition j at lambda_bg, 0: none */
+} snapi_state_map;
+
 /*
  * Chemical equilibrium (atmosphere::chemeq, atmos/atmos_chm.cc:28-363) at
  * every depth point of ncol columns, then LTE level populations (atom::lte,

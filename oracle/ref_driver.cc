@@ -210,6 +210,13 @@ static void dump_species_opem(atmosphere *a, fp_t ***Vr, fp_t ****Bv, fp_t th, f
       for (int l = 1; l <= nlambda; l += nlambda - 1)
         for (int z = a->x3l; z <= a->x3h; ++z)
           printf("bfop %d %d %d %.17g %.17g\n", s, l, z, op[l][x1][x2][z][1][1], em[l][x1][x2][z][1]);
+      // each transition's edge and cross-section at the last wavelength, where
+      // the background of a window below every edge is evaluated
+      for (int zi = 0; zi < at->Z; ++zi)
+        for (int l = 0; l < at->nl[zi]; ++l)
+          if (at->bf[zi][l])
+            printf("bfsig %d %d %d %.17g %.17g %.17g\n", s, zi, l, at->bf[zi][l]->getlambda_crit(),
+                   lam_vac[nlambda], at->bf[zi][l]->U(lam_vac[nlambda]));
     }
     del_ft6dim(op, 1, nlambda, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h, 1, 4, 1, 4);
     del_ft5dim(em, 1, nlambda, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h, 1, 4);

@@ -92,6 +92,10 @@ double orc_chem_dissoc(const snapi_chem_molecule *m, double T);
 int orc_chemeq_point(const snapi_chem_table *tb, double T, double Nt, double *ne, double *stage, double *mol,
                      double *lev, double *opref);
 double orc_chem_opref(const snapi_chem_table *tb, double T, double ne, const double *stage_pop);
+int orc_eos_state(const snapi_chem_table *tb, const snapi_state_map *map, int nd, double *state, double *resp,
+                  double *dstate);
+int orc_eos_state_batch(const snapi_chem_table *tb, const snapi_state_map *map, int ncol, int nd, double *state,
+                        double *resp, double *dstate, int nthreads);
 int orc_hequilibrium(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, double *Nt,
                      double *ne, double *opref, double *rho, int *iters);
 int orc_hequilibrium_batch(const snapi_chem_table *tb, int ncol, int nd, const double *T, const double *logtau,

@@ -515,3 +515,184 @@ int orc_hequilibrium_batch(const snapi_chem_table *tb, int ncol, int nd, const d
   }
   return bad ? -1 : 0;
 }
+
+/* ---- the per-depth state of the synthesis and response paths ---- */
+
+#define C_DELTA_T 5.0     /* delta_T, cmlib/const.h:27 */
+#define C_DELTA_NTF 1E-3  /* delta_Nt_frac, cmlib/const.h:29 */
+
+/* Planck_f (cmlib/mathtools.cc:1150-1152) with the reference's constants. */
+static double planck_f(double lam, double T) {
+  return 2.0 * 6.62606957 * 8.9875517E-7 / lam / lam / lam / lam / lam / (exp(1.4387769 / lam / T) - 1.0);
+}
+
+static void stage_offsets(const snapi_chem_table *tb, int *soff) {
+  soff[1] = 0;
+  for (int a = 1; a < tb->nelement; ++a) soff[a + 1] = soff[a] + tb->element[a - 1].nstage;
+}
+
+/* Thomson scattering at ne, T (atmosphere::op_em_vector, atmos_rad_synthesis.cc:
+ * 20-50: emissivity at lambda_mid) plus the bound-free background of the
+ * map's transitions at lambda_bg (atom::boundfree_op_em_vector,
+ * atom_rad_synthesis.cc:226-262), whose level populations come from the stage
+ * populations st at (Tl, nel), as atom::lte left them. */
+static void background(const snapi_chem_table *tb, const snapi_state_map *map, const int *soff, double T, double ne,
+                       const double *st, double Tl, double nel, double *op, double *em) {
+  double o = ne * 6.65E-25;
+  double e = o * planck_f(map->lambda_mid, T);
+  const double lam = map->lambda_bg;
+  for (int j = 0; j < tb->nbf; ++j) {
+    const double sig = map->sigma_bg[j];
+    if (sig == 0.0) continue;
+    const snapi_chem_bf *b = &tb->bf[j];
+    const snapi_chem_element *el = &tb->element[b->element - 1];
+    const int z = b->stage;
+    const double nl = lte_level(el, z, b->g, b->energy, Tl, nel, st[soff[b->element] + z]);
+    const double nu = lte_level(el, z + 1, b->g_up, b->energy_up, Tl, nel, st[soff[b->element] + z + 1]);
+    const double pop_mod = nu * ne * C_SAHA * pow(T, -1.5) * b->g / b->g_up * exp((el->eion[z] - b->energy) / C_K / T);
+    const double stim = exp(-C_H * C_C / lam / C_K / T);
+    o += sig * (nl - pop_mod * stim);
+    e += sig * pop_mod * (1.0 - stim) * planck_f(lam, T);
+  }
+  *op = o;
+  *em = e;
+}
+
+static double pf_U(const snapi_partition *pf, double T, double ne) {
+  double U, dU;
+  orc_chem_partition(pf, T, ne, &U, &dU);
+  return U;
+}
+
+/* One column's state as the reference's synthesis path holds it after popsetup
+ * and compute_op_referent, and -- with resp/dstate -- as its response path
+ * holds it (obs_stokes_responses, atmos_rad_synthesis.cc:173-190):
+ *   compute_op_referent_derivative (atmos_ppbez_rts.cc:283-325): central
+ *     differences of the referent opacity, T +- delta_T/2 and Nt +- delta_Nt/2;
+ *   ne_lte_derivatives (atmos_responses.cc:364-423): Ne and stage populations
+ *     by a forward difference in T (T + delta_T against T) and a central one
+ *     in Nt at T + delta_T/2, after which T stays raised by delta_T/2 with Ne
+ *     and the stage populations re-solved there, while the level populations
+ *     and the referent opacity keep their values at T;
+ *   atom::compute_lte_population_responses_analytical (atom_responses.cc:
+ *     453-501): level responses from the stage responses and the Boltzmann
+ *     factor, with U differenced at +- delta_T/2 and +- 1e-3 Ne / 2.
+ * state [nfield][nd]: T and NT in; NE, NHI, OPREF, BGOP, BGEM and the line
+ * level populations out.  resp [nfield][nd] (may be NULL): the whole state as
+ * the response path sees it.  dstate [2][nfield][nd] (may be NULL): d/dT and
+ * d/dNt of NE, NHI, OPREF and the line level populations, zero elsewhere. */
+int orc_eos_state(const snapi_chem_table *tb, const snapi_state_map *map, int nd, double *state, double *resp,
+                  double *dstate) {
+  const int nf = SNAPI_NF_BASE + 2 * map->nline;
+  int soff[MAXE + 2];
+  stage_offsets(tb, soff);
+  const int ih = soff[map->nhi_element];
+  if (dstate) memset(dstate, 0, sizeof(double) * 2 * (size_t)nf * nd);
+  for (int zd = 0; zd < nd; ++zd) {
+    const double T0 = state[SNAPI_F_T * nd + zd], Nt0 = state[SNAPI_F_NT * nd + zd];
+    double st0[MAXE * MAXS], ne0, op0;
+    chemeq_point(tb, T0, Nt0, &ne0, st0, NULL, NULL, &op0, NULL);
+    double bo, be;
+    background(tb, map, soff, T0, ne0, st0, T0, ne0, &bo, &be);
+    state[SNAPI_F_NE * nd + zd] = ne0;
+    state[SNAPI_F_NHI * nd + zd] = st0[ih];
+    state[SNAPI_F_OPREF * nd + zd] = op0;
+    state[SNAPI_F_BGOP * nd + zd] = bo;
+    state[SNAPI_F_BGEM * nd + zd] = be;
+    for (int i = 0; i < map->nline; ++i)
+      for (int u = 0; u < 2; ++u) {
+        const snapi_chem_level *L = &tb->level[map->line_level[i][u]];
+        state[(SNAPI_NF_BASE + 2 * i + u) * nd + zd] =
+            lte_level(&tb->element[L->element - 1], L->stage, L->g, L->energy, T0, ne0, st0[soff[L->element] + L->stage]);
+      }
+    if (resp)
+      for (int f = 0; f < nf; ++f) resp[f * nd + zd] = state[f * nd + zd];
+    if (!resp && !dstate) continue;
+    double T = T0, Nt = Nt0, e, op_a, op_b;
+    /* compute_op_referent_derivative */
+    T += C_DELTA_T * 0.5;
+    chemeq_point(tb, T, Nt, &e, NULL, NULL, NULL, &op_a, NULL);
+    T -= C_DELTA_T;
+    chemeq_point(tb, T, Nt, &e, NULL, NULL, NULL, &op_b, NULL);
+    const double dop_dT = (op_a - op_b) / C_DELTA_T;
+    T += C_DELTA_T * 0.5;
+    const double dNt0 = Nt * C_DELTA_NTF;
+    Nt += dNt0 * 0.5;
+    chemeq_point(tb, T, Nt, &e, NULL, NULL, NULL, &op_a, NULL);
+    Nt -= dNt0;
+    chemeq_point(tb, T, Nt, &e, NULL, NULL, NULL, &op_b, NULL);
+    const double dop_dNt = (op_a - op_b) / dNt0;
+    Nt += dNt0 * 0.5;
+    /* ne_lte_derivatives */
+    double sa[MAXE * MAXS], sb[MAXE * MAXS], dsT[MAXE * MAXS], dsN[MAXE * MAXS], ne_a, ne_b;
+    const int nst = soff[tb->nelement] + tb->element[tb->nelement - 1].nstage;
+    T += C_DELTA_T;
+    chemeq_point(tb, T, Nt, &ne_a, sa, NULL, NULL, NULL, NULL);
+    T -= C_DELTA_T;
+    chemeq_point(tb, T, Nt, &ne_b, sb, NULL, NULL, NULL, NULL);
+    const double dne_dT = (ne_a - ne_b) / C_DELTA_T;
+    for (int s = 0; s < nst; ++s) dsT[s] = (sa[s] - sb[s]) / C_DELTA_T;
+    T += C_DELTA_T * 0.5;
+    const double dNt1 = C_DELTA_NTF * Nt;
+    Nt += dNt1 * 0.5;
+    chemeq_point(tb, T, Nt, &ne_a, sa, NULL, NULL, NULL, NULL);
+    Nt -= dNt1;
+    chemeq_point(tb, T, Nt, &ne_b, sb, NULL, NULL, NULL, NULL);
+    const double dne_dNt = (ne_a - ne_b) / dNt1;
+    for (int s = 0; s < nst; ++s) dsN[s] = (sa[s] - sb[s]) / dNt1;
+    Nt += dNt1 * 0.5;
+    double ste[MAXE * MAXS], ne_e;
+    chemeq_point(tb, T, Nt, &ne_e, ste, NULL, NULL, NULL, NULL);
+    if (resp) {
+      background(tb, map, soff, T, ne_e, st0, T0, ne0, &bo, &be);
+      resp[SNAPI_F_T * nd + zd] = T;
+      resp[SNAPI_F_NT * nd + zd] = Nt;
+      resp[SNAPI_F_NE * nd + zd] = ne_e;
+      resp[SNAPI_F_NHI * nd + zd] = ste[ih];
+      resp[SNAPI_F_BGOP * nd + zd] = bo;
+      resp[SNAPI_F_BGEM * nd + zd] = be;
+    }
+    if (!dstate) continue;
+    double *dT = dstate, *dN = dstate + (size_t)nf * nd;
+    dT[SNAPI_F_NE * nd + zd] = dne_dT;
+    dN[SNAPI_F_NE * nd + zd] = dne_dNt;
+    dT[SNAPI_F_NHI * nd + zd] = dsT[ih];
+    dN[SNAPI_F_NHI * nd + zd] = dsN[ih];
+    dT[SNAPI_F_OPREF * nd + zd] = dop_dT;
+    dN[SNAPI_F_OPREF * nd + zd] = dop_dNt;
+    /* compute_lte_population_responses_analytical at the raised T */
+    for (int i = 0; i < map->nline; ++i)
+      for (int u = 0; u < 2; ++u) {
+        const snapi_chem_level *L = &tb->level[map->line_level[i][u]];
+        const snapi_partition *pf = &tb->element[L->element - 1].pf[L->stage];
+        const int s = soff[L->element] + L->stage;
+        const double U = pf_U(pf, T, ne_e);
+        double dU_dT = pf_U(pf, T + 0.5 * C_DELTA_T, ne_e);
+        dU_dT -= pf_U(pf, T - 0.5 * C_DELTA_T, ne_e);
+        dU_dT /= C_DELTA_T;
+        const double dne = C_DELTA_NTF * ne_e;
+        double dU_dne = pf_U(pf, T, ne_e + dne * 0.5);
+        dU_dne -= pf_U(pf, T, ne_e - dne * 0.5);
+        dU_dne /= dne;
+        const double bz = L->g * exp(-L->energy / (C_K * T));
+        double rT = dsT[s] * bz / U;
+        rT -= ste[s] * bz / U / U * (dU_dT + dU_dne * dne_dT);
+        rT += ste[s] * bz / U * L->energy / C_K / T / T;
+        double rN = dsN[s] * bz / U;
+        rN -= ste[s] * bz / U / U * (dU_dne * dne_dNt);
+        dT[(SNAPI_NF_BASE + 2 * i + u) * nd + zd] = rT;
+        dN[(SNAPI_NF_BASE + 2 * i + u) * nd + zd] = rN;
+      }
+  }
+  return 0;
+}
+
+int orc_eos_state_batch(const snapi_chem_table *tb, const snapi_state_map *map, int ncol, int nd, double *state,
+                        double *resp, double *dstate, int nthreads) {
+  const size_t nf = SNAPI_NF_BASE + 2 * (size_t)map->nline;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int c = 0; c < ncol; ++c)
+    orc_eos_state(tb, map, nd, state + c * nf * nd, resp ? resp + c * nf * nd : NULL,
+                  dstate ? dstate + c * 2 * nf * nd : NULL);
+  return 0;
+}

@@ -226,6 +226,51 @@ def make_chem_table(d, levels=True, referent=0):
     return t
 
 
+class SnapiStateMap(C.Structure):
+    _fields_ = [("nline", C.c_int), ("line_level", (C.c_int * 2) * MAX_LINES), ("nhi_element", C.c_int),
+                ("lambda_mid", C.c_double), ("lambda_bg", C.c_double), ("sigma_bg", C.c_double * CHEM_MAX_BF)]
+
+
+def make_state_map(table, d, lines, bfsig, lam_mid, nhi_element=1):
+    """Where chemical equilibrium fills a column state: each line's lower and
+    upper level as the table's level index, found from (species, stage,
+    level) of lines [nline][>=4] in the table arrays d (levels: species
+    0-based, stage, level, g); the bound-free background from bfsig rows
+    (species, stage, level, edge, wavelength, cross-section) as the reference
+    records them at the window's last wavelength, matched to the table's bf
+    transitions (those with no table entry must have zero cross-section)."""
+    m = SnapiStateMap()
+    lv = np.asarray(d["levels"])
+
+    def level_index(s, z, l):
+        hit = np.nonzero((lv[:, 0] == s) & (lv[:, 1] == z) & (lv[:, 2] == l))[0]
+        if len(hit) != 1:
+            raise ValueError("level (%d, %d, %d) not in the table" % (s, z, l))
+        return int(hit[0])
+
+    m.nline = len(lines)
+    for i, row in enumerate(lines):
+        s, z, lo, up = (int(x) for x in row[:4])
+        m.line_level[i][0] = level_index(s, z, lo)
+        m.line_level[i][1] = level_index(s, z, up)
+    m.nhi_element = nhi_element
+    m.lambda_mid = lam_mid
+    lam_bg = [r[4] for r in bfsig]
+    m.lambda_bg = float(lam_bg[0]) if len(lam_bg) else 0.0
+    if len(lam_bg) and not np.all(np.asarray(lam_bg) == lam_bg[0]):
+        raise ValueError("one background wavelength expected")
+    bi = np.asarray(d.get("bf_int", np.zeros((0, 5))))
+    for r in bfsig:
+        s, z, l, sig = int(r[0]), int(r[1]), int(r[2]), float(r[5])
+        hit = [j for j in range(len(bi)) if bi[j, 0] == s and bi[j, 1] == z and bi[j, 2] == l]
+        if not hit:
+            if sig != 0.0:
+                raise ValueError("bound-free transition (%d, %d, %d) is missing from the table" % (s, z, l))
+            continue
+        m.sigma_bg[hit[0]] = sig
+    return m
+
+
 def chem_stage_count(table):
     return sum(table.element[e].nstage for e in range(table.nelement))
 

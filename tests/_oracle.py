@@ -289,6 +289,7 @@ def _chem_lib():
         L.orc_hequilibrium_batch.argtypes = [TB, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_int),
                                              C.c_int]
         L.orc_hequilibrium_batch.restype = C.c_int
+        L.orc_eos_state_batch.argtypes = [TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, dp, dp, dp, C.c_int]
         L._chem_ready = True
     return L
 
@@ -374,3 +375,17 @@ def hequilibrium(table, T, logtau, nthreads=0):
                                   it.ctypes.data_as(C.POINTER(C.c_int)), nthreads if nthreads else (os.cpu_count() or 1))
     assert rc == 0
     return (*out, it)
+
+
+def eos_state(table, smap, state, nthreads=0):
+    """state [ncol][nfield][nd] with T and NT set -> (state with the
+    equation-of-state fields filled, the response path's state, dstate
+    [ncol][2][nfield][nd])."""
+    L = _chem_lib()
+    st = np.array(state, dtype=np.float64, order="C", copy=True)
+    ncol, nf, nd = st.shape
+    resp = np.zeros_like(st)
+    ds = np.zeros((ncol, 2, nf, nd))
+    L.orc_eos_state_batch(C.byref(table), C.byref(smap), ncol, nd, ptr(st), ptr(resp), ptr(ds),
+                          nthreads if nthreads else (os.cpu_count() or 1))
+    return st, resp, ds

@@ -168,7 +168,7 @@ def run(d, args, he=True):
 
 def parse_synth(lines):
     rec = {"depth": [], "lambda": [], "stokes": [], "stokes_steps": [], "opem": [], "spop": [], "line": [],
-           "zc": [], "lpop": [], "level": [], "species": [], "sdepth": [], "bfop": []}
+           "zc": [], "lpop": [], "level": [], "species": [], "sdepth": [], "bfop": [], "bfsig": []}
     for ln in lines:
         t = ln.split()
         if not t:
@@ -199,6 +199,8 @@ def save_synth(rec, rows, name, extra=None):
     spop = np.array([[float(x) for x in r] for r in rec["spop"]])
     sdepth = np.array([[float(x) for x in r] for r in rec["sdepth"]])
     bfop = np.array([[float(x) for x in r] for r in rec["bfop"]])
+    # species, stage, level, edge wavelength, last (vacuum) wavelength, cross-section there
+    bfsig = np.array([[float(x) for x in r] for r in rec["bfsig"]]).reshape(-1, 6)
     lines = []
     for r in rec["line"]:
         s, z, lo, up = (int(x) for x in r[:4])
@@ -215,6 +217,7 @@ def save_synth(rec, rows, name, extra=None):
     np.savez_compressed(
         os.path.join(HERE, name),
         depth=dep, lam=lamv, stokes=stokes, stokes_steps=steps, opem=opem, spop=spop, sdepth=sdepth, bfop=bfop,
+        bfsig=bfsig,
         lines=np.array([l[:4] + [l[4]] + [0] * 0 for l in lines], dtype=np.int64),
         line_data=np.array([l[5:12] for l in lines]), line_split=np.array([l[12] for l in lines]),
         zc=np.array(zc, dtype=np.float64), lpop=lpop,

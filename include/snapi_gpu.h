@@ -37,6 +37,10 @@
  *                                                                   atmos/atmos_chm.cc:28-363,
  *                                                                   atmos/atmol/atom/atom.cc:779-787, 892-924
  *   snapi_hequilibrium*      atmos_pp::enforce_hequilibrium         atmos/plpar/atmos_pp.cc:372-455
+ *   snapi_eos_state*         popsetup + compute_op_referent, and the response path's
+ *                            compute_op_referent_derivative, ne_lte_derivatives and
+ *                            compute_lte_population_responses_analytical
+ *                                                                   atmos/atmos_rad_synthesis.cc:173-190
  */
 #ifndef SNAPI_GPU_H
 #define SNAPI_GPU_H
@@ -558,6 +562,35 @@ int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd, const doub
 int snapi_hequilibrium(const snapi_chem_table *table, int ncol, int nd, const double *T, const double *logtau,
                        double *Nt, double *ne, double *opref, double *rho, double *stage, double *molecule,
                        double *level, int *iterations);
+/*
+ * The equation-of-state fields of a column state (double), as the reference's
+ * synthesis path holds them after popsetup and compute_op_referent, and as
+ * its response path holds them (obs_stokes_responses,
+ * atmos/atmos_rad_synthesis.cc:173-190) with the thermodynamic derivatives
+ * that snapi_stokes_responses and snapi_stokes_adjoint read:
+ *   state [ncol][nfield][nd] (nfield = SNAPI_NF_BASE + 2 map->nline): T and
+ *     NT in; NE, NHI (the neutral-stage density of map->nhi_element), OPREF
+ *     (at the table's lambda_ref: airtovac(5000 A) for the reference's
+ *     response and synthesis paths), BGOP and BGEM (Thomson with its
+ *     emissivity at lambda_mid plus the map's bound-free transitions at
+ *     lambda_bg, atom::boundfree_op_em_vector, atom_rad_synthesis.cc:226-262)
+ *     and the line level populations out; other fields untouched;
+ *   resp [ncol][nfield][nd] out (may be NULL): the response path's state:
+ *     ne_lte_derivatives (atmos/atmos_responses.cc:364-423) leaves T raised
+ *     by delta_T/2 = 2.5 K with Ne and the stage populations re-solved there
+ *     and the level populations and referent opacity of T;
+ *   dstate [ncol][2][nfield][nd] out (may be NULL): d/dT and d/dNt of NE, NHI,
+ *     OPREF (compute_op_referent_derivative, plpar/atmos_ppbez_rts.cc:283-325),
+ *     and of the line level populations
+ *     (atom::compute_lte_population_responses_analytical,
+ *     atmol/atom/atom_responses.cc:453-501); zero elsewhere.
+ * Ten chemical equilibria per depth point with resp or dstate, one without.
+ */
+int snapi_eos_state(const snapi_chem_table *table, const snapi_state_map *map, int ncol, int nd, double *state,
+                    double *resp, double *dstate);
+int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *map, int ncol, int nd,
+                        double *state_dev, double *resp_dev, double *dstate_dev, void *stream);
+
 /* snapi_hequilibrium on device-resident arrays, all work on `stream`. */
 int snapi_hequilibrium_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,
                            const double *logtau_dev, double *Nt_dev, double *ne_dev, double *opref_dev,

@@ -14,6 +14,11 @@ from per-species arrays).  Arrays (double, C order):
                                     (atmosphere::opacity_continuum, atmos/atmos_rad.cc:286-294)
   iterations  [ncol][nd]            Newton steps taken (-1: singular system)
 
+Equation-of-state fields of a column state (snapi_eos_state[_dev]): from T
+and NT, the synthesis path's Ne, neutral hydrogen, referent opacity,
+background and line level populations, and the response path's state and
+thermodynamic derivatives (atmos/atmos_rad_synthesis.cc:173-190).
+
 Hydrostatic equilibrium (snapi_hequilibrium, snapi_hequilibrium_dev):
 atmos_pp::enforce_hequilibrium (atmos/plpar/atmos_pp.cc:372-455) on columns
 of temperature and log referent optical depth, top first; Nt, ne, opref and
@@ -42,6 +47,11 @@ def _declare(L):
     L.snapi_hequilibrium.restype = C.c_int
     L.snapi_hequilibrium_dev.argtypes = [TB, C.c_int, C.c_int] + [C.c_void_p] * 11
     L.snapi_hequilibrium_dev.restype = C.c_int
+    SM = C.POINTER(abi.SnapiStateMap)
+    L.snapi_eos_state.argtypes = [TB, SM, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.snapi_eos_state.restype = C.c_int
+    L.snapi_eos_state_dev.argtypes = [TB, SM, C.c_int, C.c_int] + [C.c_void_p] * 4
+    L.snapi_eos_state_dev.restype = C.c_int
     L._chem_declared = True
 
 
@@ -152,3 +162,42 @@ def hequilibrium_dev(table, T_t, logtau_t, Nt_t, ne_t=None, opref_t=None, rho_t=
     sp = None if stream is None else C.c_void_p(stream.cuda_stream)
     rc = L.snapi_hequilibrium_dev(C.byref(table), ncol, nd, *ptrs, sp)
     check("snapi_hequilibrium_dev", rc)
+
+
+def eos_state(table, smap, state, derivatives=True):
+    """state [ncol][nfield][nd] (T, NT set) -> (state with the equation-of-state
+    fields filled, the response path's state, dstate [ncol][2][nfield][nd]);
+    the last two are None without derivatives."""
+    L = lib()
+    _declare(L)
+    st = np.array(state, dtype=np.float64, order="C", copy=True)
+    if st.ndim != 3 or st.shape[1] != abi.nfield(smap.nline):
+        raise ValueError("state must be [ncol][%d][nd]" % abi.nfield(smap.nline))
+    ncol, nf, nd = st.shape
+    resp = np.zeros_like(st) if derivatives else None
+    ds = np.zeros((ncol, 2, nf, nd)) if derivatives else None
+    rc = L.snapi_eos_state(C.byref(table), C.byref(smap), ncol, nd, abi.ptr(st),
+                           None if resp is None else abi.ptr(resp), None if ds is None else abi.ptr(ds))
+    check("snapi_eos_state", rc)
+    return st, resp, ds
+
+
+def eos_state_dev(table, smap, state_t, resp_t=None, dstate_t=None, stream=None):
+    """snapi_eos_state_dev on float64 device tensors: state_t [ncol][nfield][nd]
+    updated in place, resp_t like it, dstate_t [ncol][2][nfield][nd]."""
+    L = lib()
+    _declare(L)
+    ncol, nf, nd = state_t.shape
+    for t, shp in ((state_t, (ncol, nf, nd)), (resp_t, (ncol, nf, nd)), (dstate_t, (ncol, 2, nf, nd))):
+        if t is None:
+            continue
+        import torch
+        if t.dtype != torch.float64 or tuple(t.shape) != shp or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("eos_state_dev: expected a contiguous float64 device tensor of shape %s" % (shp,))
+    if nf != abi.nfield(smap.nline):
+        raise ValueError("state must have %d fields" % abi.nfield(smap.nline))
+    sp = None if stream is None else C.c_void_p(stream.cuda_stream)
+    rc = L.snapi_eos_state_dev(C.byref(table), C.byref(smap), ncol, nd, state_t.data_ptr(),
+                               None if resp_t is None else resp_t.data_ptr(),
+                               None if dstate_t is None else dstate_t.data_ptr(), sp)
+    check("snapi_eos_state_dev", rc)

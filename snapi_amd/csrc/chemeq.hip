@@ -846,6 +846,239 @@ __global__ __launch_bounds__(kChemThreads, 2) void hequilibrium_kernel(const sna
   }
 }
 
+// ---- the per-depth state of the synthesis and response paths ----
+
+constexpr double kDeltaT = 5.0;     // delta_T, cmlib/const.h:27
+constexpr double kDeltaNtf = 1E-3;  // delta_Nt_frac, cmlib/const.h:29
+constexpr int kEosVariants = 10;    // chemical equilibria per depth point with derivatives
+constexpr int kMaxWant = 64;        // stage populations recorded per equilibrium
+
+// Stage populations an equilibrium records: idx[a][z] >= 0 is the slot of
+// stage z of element a (0-based) in a variant's record.
+struct EosWant {
+  int nwant;
+  signed char idx[kMaxE][kMaxS];
+};
+
+// (T, Nt) of variant v at a point, with the reference's arithmetic order:
+// v0 the point itself (popsetup); v1..v4 compute_op_referent_derivative
+// (atmos_ppbez_rts.cc:283-325, T +- delta_T/2 then Nt +- delta_Nt/2); v5..v9
+// ne_lte_derivatives (atmos_responses.cc:364-423: T + delta_T and T, then at
+// T + delta_T/2: Nt +- delta_Nt/2 and Nt).  Also the step sizes.
+struct EosSeq {
+  double T[kEosVariants], Nt[kEosVariants], dNt0, dNt1;
+};
+
+__host__ __device__ inline EosSeq eos_sequence(double T0, double Nt0) {
+  EosSeq q;
+  double T = T0, Nt = Nt0;
+  q.T[0] = T, q.Nt[0] = Nt;
+  T += kDeltaT * 0.5;
+  q.T[1] = T, q.Nt[1] = Nt;
+  T -= kDeltaT;
+  q.T[2] = T, q.Nt[2] = Nt;
+  T += kDeltaT * 0.5;
+  q.dNt0 = Nt * kDeltaNtf;
+  Nt += q.dNt0 * 0.5;
+  q.T[3] = T, q.Nt[3] = Nt;
+  Nt -= q.dNt0;
+  q.T[4] = T, q.Nt[4] = Nt;
+  Nt += q.dNt0 * 0.5;
+  T += kDeltaT;
+  q.T[5] = T, q.Nt[5] = Nt;
+  T -= kDeltaT;
+  q.T[6] = T, q.Nt[6] = Nt;
+  T += kDeltaT * 0.5;
+  q.dNt1 = kDeltaNtf * Nt;
+  Nt += q.dNt1 * 0.5;
+  q.T[7] = T, q.Nt[7] = Nt;
+  Nt -= q.dNt1;
+  q.T[8] = T, q.Nt[8] = Nt;
+  Nt += q.dNt1 * 0.5;
+  q.T[9] = T, q.Nt[9] = Nt;
+  return q;
+}
+
+// One team per (depth point, variant): chemical equilibrium at the variant's
+// (T, Nt); the record holds Ne, the referent opacity and the wanted stage
+// populations.  state [ncol][nf][nd] supplies T and NT.
+__global__ __launch_bounds__(kChemThreads) void eos_variants_kernel(const snapi_chem_table *__restrict__ tb,
+                                                                    const ChemDerived *__restrict__ dv, long npt,
+                                                                    int nd, int nf, int nvar,
+                                                                    const double *__restrict__ state, EosWant want,
+                                                                    double *__restrict__ rec) {
+  __shared__ ChemShared sh;
+  extern __shared__ double region_s[];
+  const int tid = threadIdx.x;
+  load_shared(sh, tb, dv, tid);
+  const ChemTeam cx = make_team(tb, dv, sh, region_s, tid);
+  const long t = ((long)blockIdx.x * blockDim.x + tid) / kTeam;
+  if (t >= npt * nvar) return;  // the whole team leaves together; no barrier follows
+  const long pt = t / nvar;
+  const int v = (int)(t % nvar);
+  const long c = pt / nd, zd = pt % nd;
+  const double *st = state + c * nf * nd + zd;
+  const EosSeq q = eos_sequence(st[SNAPI_F_T * nd], st[SNAPI_F_NT * nd]);
+  double Tv = q.T[0], Ntv = q.Nt[0];
+#pragma unroll
+  for (int k = 1; k < kEosVariants; ++k)  // selects in registers, no indexed stack array
+    if (k == v) Tv = q.T[k], Ntv = q.Nt[k];
+  const ChemTemp ct = set_temperature(tb, cx, Tv);
+  const ChemSol sol = solve_point(tb, dv, sh, cx, ct, Ntv);
+  const int nrec = 2 + want.nwant;
+  double *r = rec + t * nrec;
+  const double op = (v <= 4) ? referent_opacity(tb, dv, cx, ct, sol) : 0.0;
+  if (cx.lane == 0) {
+    r[0] = sol.N0;
+    r[1] = op;
+  }
+  if (cx.is_el)
+    for (int z = 0; z < cx.ns; ++z) {
+      const int k = want.idx[cx.ea][z];
+      if (k >= 0) r[2 + k] = sol.Nown * ((z ? cx.frac[z - 1] : 1.0) * sol.rf);
+    }
+}
+
+// U of stage z of element a (0-based) at (T, ne), as the team evaluation
+// forms it from the temperature-only part and the Traving asymptotic terms.
+__device__ double pf_direct(const snapi_chem_table *__restrict__ tb, const ChemDerived *__restrict__ dv, int a,
+                            int z, double T, double ne) {
+  const snapi_partition &pf = tb->element[a].pf[z];
+  const double x = 5040.0 / T;
+  double U = pf_base(pf, T, x);
+  const StageDesc &d = dv->sd[a][z];
+  if (d.nterm < 0) return U;
+  const NePow w = ne_powers(ne);
+  const double p = d.zf * w.rr6, rp = w.r6 * d.rzf, alpha = d.z2 * x;
+  for (int i = 0; i < d.nterm; ++i) {
+    const double ci = pf.g2[i] * exp(-2.302585 * pf.xion[i] * x);
+    U += ci * qas_f(dv->xl[a][d.toff + i], dv->rxl[a][d.toff + i], p, rp, alpha);
+  }
+  return U;
+}
+
+// LTE population of a level (atom::lte, atom.cc:779-787).
+__device__ double lte_pop(const snapi_chem_table *__restrict__ tb, const ChemDerived *__restrict__ dv, int a, int z,
+                          double g, double energy, double T, double ne, double Nz) {
+  return Nz * ((g != 0.0) ? g * exp(-energy / (kK * T)) / pf_direct(tb, dv, a, z, T, ne) : 1.0);
+}
+
+// Planck_f (cmlib/mathtools.cc:1150-1152) with the reference's constants.
+__device__ double planck_f(double lam, double T) {
+  return 2.0 * 6.62606957 * 8.9875517E-7 / lam / lam / lam / lam / lam / (exp(1.4387769 / lam / T) - 1.0);
+}
+
+// Thomson scattering at (T, ne) with its emissivity at lambda_mid
+// (atmosphere::op_em_vector, atmos_rad_synthesis.cc:20-50) plus the map's
+// bound-free background at lambda_bg (atom::boundfree_op_em_vector,
+// atom_rad_synthesis.cc:226-262) with level populations from the stage
+// populations stp (record slots) at (Tl, nel).
+__device__ void eos_background(const snapi_chem_table *__restrict__ tb, const ChemDerived *__restrict__ dv,
+                               const snapi_state_map *__restrict__ map, const EosWant &want, double T, double ne,
+                               const double *stp, double Tl, double nel, double &op, double &em) {
+  double o = ne * 6.65E-25;
+  double e = o * planck_f(map->lambda_mid, T);
+  const double lam = map->lambda_bg;
+  for (int j = 0; j < tb->nbf; ++j) {
+    const double sig = map->sigma_bg[j];
+    if (sig == 0.0) continue;
+    const snapi_chem_bf &b = tb->bf[j];
+    const int a = b.element - 1, z = b.stage;
+    const double nl = lte_pop(tb, dv, a, z, b.g, b.energy, Tl, nel, stp[want.idx[a][z]]);
+    const double nu = lte_pop(tb, dv, a, z + 1, b.g_up, b.energy_up, Tl, nel, stp[want.idx[a][z + 1]]);
+    const double pop_mod = nu * ne * kSaha * pow(T, -1.5) * b.g / b.g_up * exp((tb->element[a].eion[z] - b.energy) / kK / T);
+    const double stim = exp(-kH * kC / lam / kK / T);
+    o += sig * (nl - pop_mod * stim);
+    e += sig * pop_mod * (1.0 - stim) * planck_f(lam, T);
+  }
+  op = o;
+  em = e;
+}
+
+// One thread per depth point: the synthesis path's state from variant 0 and,
+// with resp/dstate, the response path's state and the thermodynamic
+// derivatives from the other variants: referent opacity by central
+// differences (v1 - v2, v3 - v4), Ne and the stage populations by a forward
+// difference in T (v5 - v6) and a central one in Nt (v7 - v8), the response
+// state at v9 (T raised by delta_T/2, level populations and referent opacity
+// kept from v0), level responses by
+// atom::compute_lte_population_responses_analytical (atom_responses.cc:453-501).
+__global__ void eos_combine_kernel(const snapi_chem_table *__restrict__ tb, const ChemDerived *__restrict__ dv,
+                                   const snapi_state_map *__restrict__ map, long npt, int nd, int nf, int nvar,
+                                   EosWant want, const double *__restrict__ rec, double *__restrict__ state,
+                                   double *__restrict__ resp, double *__restrict__ dstate) {
+  const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= npt) return;
+  const long c = pt / nd, zd = pt % nd;
+  double *st = state + c * nf * nd + zd;
+  const double T0 = st[SNAPI_F_T * nd];
+  const int nrec = 2 + want.nwant;
+  const double *r0 = rec + pt * nvar * nrec;
+  const double ne0 = r0[0], *s0 = r0 + 2;
+  const int ih = want.idx[map->nhi_element - 1][0];
+  double bo, be;
+  eos_background(tb, dv, map, want, T0, ne0, s0, T0, ne0, bo, be);
+  st[SNAPI_F_NE * nd] = ne0;
+  st[SNAPI_F_NHI * nd] = s0[ih];
+  st[SNAPI_F_OPREF * nd] = r0[1];
+  st[SNAPI_F_BGOP * nd] = bo;
+  st[SNAPI_F_BGEM * nd] = be;
+  for (int i = 0; i < map->nline; ++i)
+    for (int u = 0; u < 2; ++u) {
+      const snapi_chem_level &L = tb->level[map->line_level[i][u]];
+      const int a = L.element - 1;
+      st[(SNAPI_NF_BASE + 2 * i + u) * nd] = lte_pop(tb, dv, a, L.stage, L.g, L.energy, T0, ne0, s0[want.idx[a][L.stage]]);
+    }
+  if (!resp && !dstate) return;
+  const EosSeq q = eos_sequence(T0, st[SNAPI_F_NT * nd]);
+  const double *rv[kEosVariants];
+  for (int v = 0; v < kEosVariants; ++v) rv[v] = r0 + v * nrec;
+  const double T = q.T[9], ne_e = rv[9][0], *se = rv[9] + 2;
+  if (resp) {
+    double *rs = resp + c * nf * nd + zd;
+    for (int f = 0; f < nf; ++f) rs[f * nd] = st[f * nd];
+    eos_background(tb, dv, map, want, T, ne_e, s0, T0, ne0, bo, be);
+    rs[SNAPI_F_T * nd] = T;
+    rs[SNAPI_F_NT * nd] = q.Nt[9];
+    rs[SNAPI_F_NE * nd] = ne_e;
+    rs[SNAPI_F_NHI * nd] = se[ih];
+    rs[SNAPI_F_BGOP * nd] = bo;
+    rs[SNAPI_F_BGEM * nd] = be;
+  }
+  if (!dstate) return;
+  double *dT = dstate + c * 2 * nf * nd + zd, *dN = dT + (size_t)nf * nd;
+  for (int f = 0; f < nf; ++f) dT[f * nd] = dN[f * nd] = 0.0;
+  const double dne_dT = (rv[5][0] - rv[6][0]) / kDeltaT, dne_dNt = (rv[7][0] - rv[8][0]) / q.dNt1;
+  dT[SNAPI_F_NE * nd] = dne_dT;
+  dN[SNAPI_F_NE * nd] = dne_dNt;
+  dT[SNAPI_F_NHI * nd] = (rv[5][2 + ih] - rv[6][2 + ih]) / kDeltaT;
+  dN[SNAPI_F_NHI * nd] = (rv[7][2 + ih] - rv[8][2 + ih]) / q.dNt1;
+  dT[SNAPI_F_OPREF * nd] = (rv[1][1] - rv[2][1]) / kDeltaT;
+  dN[SNAPI_F_OPREF * nd] = (rv[3][1] - rv[4][1]) / q.dNt0;
+  for (int i = 0; i < map->nline; ++i)
+    for (int u = 0; u < 2; ++u) {
+      const snapi_chem_level &L = tb->level[map->line_level[i][u]];
+      const int a = L.element - 1, z = L.stage, k = 2 + want.idx[a][z];
+      const double dsT = (rv[5][k] - rv[6][k]) / kDeltaT, dsN = (rv[7][k] - rv[8][k]) / q.dNt1;
+      const double U = pf_direct(tb, dv, a, z, T, ne_e);
+      double dU_dT = pf_direct(tb, dv, a, z, T + 0.5 * kDeltaT, ne_e);
+      dU_dT -= pf_direct(tb, dv, a, z, T - 0.5 * kDeltaT, ne_e);
+      dU_dT /= kDeltaT;
+      const double dne = kDeltaNtf * ne_e;
+      double dU_dne = pf_direct(tb, dv, a, z, T, ne_e + dne * 0.5);
+      dU_dne -= pf_direct(tb, dv, a, z, T, ne_e - dne * 0.5);
+      dU_dne /= dne;
+      const double bz = L.g * exp(-L.energy / (kK * T)), Nz = rv[9][k];
+      double rT = dsT * bz / U;
+      rT -= Nz * bz / U / U * (dU_dT + dU_dne * dne_dT);
+      rT += Nz * bz / U * L.energy / kK / T / T;
+      double rN = dsN * bz / U;
+      rN -= Nz * bz / U / U * (dU_dne * dne_dNt);
+      dT[(SNAPI_NF_BASE + 2 * i + u) * nd] = rT;
+      dN[(SNAPI_NF_BASE + 2 * i + u) * nd] = rN;
+    }
+}
+
 // Checks the table against the kernel's limits and derives its constants.
 int derive(const snapi_chem_table *tb, ChemDerived &d) {
   if (tb->nelement < 1 || tb->nelement > kMaxE || tb->nmolecule < 0 || tb->nmolecule > kMaxM ||
@@ -1004,6 +1237,53 @@ int upload_table(const snapi_chem_table *tb, const ChemDerived &d, void **buf, h
   return SNAPI_OK;
 }
 
+// Checks the state map against the table and lists the stage populations
+// the equilibria must record.
+int eos_want(const snapi_chem_table *tb, const snapi_state_map *map, EosWant &w) {
+  if (map->nline < 0 || map->nline > SNAPI_MAX_LINES || map->nhi_element < 1 || map->nhi_element > tb->nelement ||
+      !(map->lambda_mid > 0.0))
+    return SNAPI_ERR_ARG;
+  memset(&w, -1, sizeof(w));
+  w.nwant = 0;
+  auto add = [&](int a, int z) {
+    if (w.idx[a][z] < 0) {
+      if (w.nwant >= kMaxWant) return false;
+      w.idx[a][z] = (signed char)w.nwant++;
+    }
+    return true;
+  };
+  if (!add(map->nhi_element - 1, 0)) return SNAPI_ERR_UNSUPPORTED;
+  for (int i = 0; i < map->nline; ++i)
+    for (int u = 0; u < 2; ++u) {
+      const int l = map->line_level[i][u];
+      if (l < 0 || l >= tb->nlevel) return SNAPI_ERR_ARG;
+      if (!add(tb->level[l].element - 1, tb->level[l].stage)) return SNAPI_ERR_UNSUPPORTED;
+    }
+  for (int j = 0; j < tb->nbf; ++j) {
+    if (map->sigma_bg[j] == 0.0) continue;
+    if (!(map->lambda_bg > 0.0)) return SNAPI_ERR_ARG;
+    const snapi_chem_bf &b = tb->bf[j];
+    if (!add(b.element - 1, b.stage) || !add(b.element - 1, b.stage + 1)) return SNAPI_ERR_UNSUPPORTED;
+  }
+  return SNAPI_OK;
+}
+
+int launch_eos(const snapi_chem_table *dtb, const ChemDerived *ddv, const snapi_state_map *dmap, int nslot, long npt,
+               int nd, int nf, int nvar, const EosWant &want, double *rec, double *state, double *resp,
+               double *dstate, hipStream_t st) {
+  long blocks;
+  size_t lds;
+  if (!team_grid(npt * nvar, nslot, blocks, lds)) return SNAPI_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(eos_variants_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, npt, nd, nf,
+                     nvar, state, want, rec);
+  if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;
+  const long b2 = (npt + 255) / 256;
+  if (b2 > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(eos_combine_kernel, dim3((unsigned)b2), dim3(256), 0, st, dtb, ddv, dmap, npt, nd, nf, nvar, want,
+                     rec, state, resp, dstate);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
 }  // namespace
 
 extern "C" int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,
@@ -1134,5 +1414,58 @@ extern "C" int snapi_hequilibrium(const snapi_chem_table *table, int ncol, int n
 done:
   hipFree(dT); hipFree(dlt); hipFree(dNt); hipFree(dne); hipFree(dkr); hipFree(drho);
   hipFree(dst); hipFree(dmo); hipFree(dlv); hipFree(dit);
+  return rc;
+}
+
+extern "C" int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *map, int ncol, int nd,
+                                   double *state_dev, double *resp_dev, double *dstate_dev, void *stream) {
+  if (!table || !map || ncol <= 0 || nd <= 0 || !state_dev) return SNAPI_ERR_ARG;
+  if (!(table->lambda_ref > 0.0)) return SNAPI_ERR_ARG;
+  ChemDerived d;
+  int rc = derive(table, d);
+  if (rc) return rc;
+  EosWant want;
+  rc = eos_want(table, map, want);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int nf = SNAPI_NF_BASE + 2 * map->nline;
+  const long npt = (long)ncol * nd;
+  const int nvar = (resp_dev || dstate_dev) ? kEosVariants : 1;
+  void *buf = nullptr, *mbuf = nullptr, *rec = nullptr;
+  rc = upload_table(table, d, &buf, st);
+  if (rc) return rc;
+  if (hipMalloc(&mbuf, sizeof(snapi_state_map)) != hipSuccess ||
+      hipMalloc(&rec, (size_t)npt * nvar * (2 + want.nwant) * sizeof(double)) != hipSuccess ||
+      hipMemcpyAsync(mbuf, map, sizeof(snapi_state_map), hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = SNAPI_ERR_DEVICE;
+  if (rc == SNAPI_OK)
+    rc = launch_eos((const snapi_chem_table *)buf, (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)),
+                    (const snapi_state_map *)mbuf, d.nslot, npt, nd, nf, nvar, want, (double *)rec, state_dev,
+                    resp_dev, dstate_dev, st);
+  if (hipStreamSynchronize(st) != hipSuccess && rc == SNAPI_OK) rc = SNAPI_ERR_DEVICE;
+  hipFree(buf);
+  hipFree(mbuf);
+  hipFree(rec);
+  return rc;
+}
+
+extern "C" int snapi_eos_state(const snapi_chem_table *table, const snapi_state_map *map, int ncol, int nd,
+                               double *state, double *resp, double *dstate) {
+  if (!table || !map || ncol <= 0 || nd <= 0 || !state) return SNAPI_ERR_ARG;
+  if (map->nline < 0 || map->nline > SNAPI_MAX_LINES) return SNAPI_ERR_ARG;
+  const size_t n = (size_t)ncol * (SNAPI_NF_BASE + 2 * map->nline) * nd;
+  double *ds = nullptr, *dr = nullptr, *dd = nullptr;
+  int rc = SNAPI_OK;
+  CCHK(hipMalloc((void **)&ds, n * 8));
+  if (resp) CCHK(hipMalloc((void **)&dr, n * 8));
+  if (dstate) CCHK(hipMalloc((void **)&dd, 2 * n * 8));
+  CCHK(hipMemcpy(ds, state, n * 8, hipMemcpyHostToDevice));
+  rc = snapi_eos_state_dev(table, map, ncol, nd, ds, dr, dd, nullptr);
+  if (rc) goto done;
+  CCHK(hipMemcpy(state, ds, n * 8, hipMemcpyDeviceToHost));
+  if (dr) CCHK(hipMemcpy(resp, dr, n * 8, hipMemcpyDeviceToHost));
+  if (dd) CCHK(hipMemcpy(dstate, dd, 2 * n * 8, hipMemcpyDeviceToHost));
+done:
+  hipFree(ds); hipFree(dr); hipFree(dd);
   return rc;
 }

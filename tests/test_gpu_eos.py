@@ -1,0 +1,110 @@
+"""The equation-of-state pass on the GPU (snapi_eos_state) against the
+reference's records of the cfg0 column (synthesis and response paths) and
+against the oracle on perturbed columns; the device entry point against the
+host one.
+
+Tolerance: double precision; the GPU's chemical equilibria agree with the
+oracle's to 1e-9 where both take the same Newton steps, which the state
+fields inherit.  The derivatives are differences over 5 K or 1e-3 Nt of
+equilibria converged to 1e-4: a different Newton step count on one side of a
+difference moves it by up to that convergence tolerance over the step, so the
+derivatives are held to 1e-9 of their column's largest value on at least 99 %
+of the points and to 1e-2 of it everywhere."""
+import os
+
+import numpy as np
+import pytest
+
+import _golden
+import _oracle
+from snapi_amd import abi, chemeq, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(name):
+    d = _golden.chemeq()
+    dd = {k: v for k, v in d.items() if k != "table"}
+    tb = abi.make_chem_table(dd, referent=0)
+    g = _golden.load(name + ".npz")
+    col = workloads.column_from_dump(os.path.join(_golden.HERE, name + ".npz"))
+    smap = abi.make_state_map(tb, dd, g["lines"], g["bfsig"], col["ray"].lam_mid)
+    return tb, smap, g, col
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+FIELDS = [abi.F_NE, abi.F_NHI, abi.F_OPREF, abi.F_BGOP, abi.F_BGEM]
+
+
+def test_eos_state_matches_reference():
+    # the two records differ in their wavelength grids, hence in lambda_mid
+    # and lambda_bg: each is checked with its own map
+    tb, smap0, _, c0 = setup("cfg0_synth")
+    tb, smap, g, col = setup("cfg0_resp")
+    inp = np.zeros_like(col["state"])
+    inp[abi.F_T], inp[abi.F_NT] = c0["state"][abi.F_T], c0["state"][abi.F_NT]
+    print("eos_state vs reference: cfg0 column, %d lines" % smap.nline, flush=True)
+    st, _, _ = chemeq.eos_state(tb, smap0, inp[None], derivatives=False)
+    _, resp, ds = chemeq.eos_state(tb, smap, inp[None])
+    fwd = c0["state"]
+    e_f = max(rel(st[0, f], fwd[f]) for f in FIELDS + list(range(abi.NF_BASE, fwd.shape[0])))
+    ref = col["state"]
+    e_r = max(rel(resp[0, f], ref[f]) for f in FIELDS + [abi.F_T, abi.F_NT] + list(range(abi.NF_BASE, ref.shape[0])))
+    th = g["thermo"]
+    e_d = max(rel(ds[0, p, f], th[:, 2 + 3 * p + k]) for p in range(2)
+              for k, f in enumerate((abi.F_NE, abi.F_NHI, abi.F_OPREF)))
+    print("  synthesis state %.2e  response state %.2e  derivatives %.2e" % (e_f, e_r, e_d), flush=True)
+    assert e_f < 1e-9 and e_r < 1e-9 and e_d < 1e-7
+
+
+def test_eos_state_perturbed_columns_match_oracle():
+    tb, smap, g, col = setup("cfg0_synth")
+    p = workloads.chemeq_problem(8, 4, seed=12)
+    ncol, nd = p["T"].shape
+    nf = abi.nfield(smap.nline)
+    inp = np.zeros((ncol, nf, nd))
+    inp[:, abi.F_T], inp[:, abi.F_NT] = p["T"], p["Nt"]
+    print("eos_state vs oracle: %d columns x %d depths" % (ncol, nd), flush=True)
+    st, resp, ds = chemeq.eos_state(tb, smap, inp)
+    ost, oresp, ods = _oracle.eos_state(tb, smap, inp)
+    fields = FIELDS + list(range(abi.NF_BASE, nf))
+    e_f = max(rel(st[:, f], ost[:, f]) for f in fields)
+    e_r = max(rel(resp[:, f], oresp[:, f]) for f in fields + [abi.F_T, abi.F_NT])
+    print("  state %.2e  response state %.2e" % (e_f, e_r), flush=True)
+    assert e_f < 1e-9 and e_r < 1e-9
+    for pp in range(2):
+        for f in [abi.F_NE, abi.F_NHI, abi.F_OPREF] + list(range(abi.NF_BASE, nf)):
+            a, b = ds[:, pp, f], ods[:, pp, f]
+            scale = np.max(np.abs(b), axis=1, keepdims=True)
+            e = np.abs(a - b) / scale
+            print("  d/d%s field %d: worst %.2e, 99th pct %.2e" % ("TN"[pp], f, e.max(), np.quantile(e, 0.99)))
+            assert np.quantile(e, 0.99) < 1e-9 and e.max() < 1e-2
+
+
+def test_eos_state_dev_equals_host():
+    import torch
+    tb, smap, g, col = setup("cfg0_synth")
+    p = workloads.chemeq_problem(4, 2, seed=3)
+    ncol, nd = p["T"].shape
+    nf = abi.nfield(smap.nline)
+    inp = np.zeros((ncol, nf, nd))
+    inp[:, abi.F_T], inp[:, abi.F_NT] = p["T"], p["Nt"]
+    st, resp, ds = chemeq.eos_state(tb, smap, inp)
+    dev = torch.device("cuda")
+    s_t = torch.from_numpy(inp).to(dev)
+    r_t = torch.zeros_like(s_t)
+    d_t = torch.zeros((ncol, 2, nf, nd), dtype=torch.float64, device=dev)
+    print("eos_state_dev vs eos_state: %d x %d" % (ncol, nd), flush=True)
+    chemeq.eos_state_dev(tb, smap, s_t, r_t, d_t)
+    torch.cuda.synchronize()
+    assert np.array_equal(s_t.cpu().numpy(), st)
+    assert np.array_equal(r_t.cpu().numpy(), resp)
+    assert np.array_equal(d_t.cpu().numpy(), ds)
+    # the state alone: one equilibrium per point, same fields
+    s2 = torch.from_numpy(inp).to(dev)
+    chemeq.eos_state_dev(tb, smap, s2)
+    torch.cuda.synchronize()
+    assert np.array_equal(s2.cpu().numpy(), st)

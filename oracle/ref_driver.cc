@@ -656,6 +656,9 @@ static int mode_interp(int argc, char **argv, io_class &io) {
     nt[i] = pos;
     nv[i] = is_temp ? 4000.0 + 3000.0 * urand() : 2.0 * urand() - 1.0;
   }
+  // optional: the deepest node's position (the temperature's bottom slope
+  // comes from a table in that position)
+  if (argc > 5) nt[nn] = atof(argv[5]);
   for (int z = 1; z <= nd; ++z) grid[z] = -6.0 + 7.0 * (z - 1) / (nd - 1);
   atmospheric_interpolation(nt, nv, nn, grid, q, 1, nd, is_temp);
   printf("nodes");

@@ -161,6 +161,90 @@ def _interp_param(t, y, x):
     return out
 
 
+_FALC_DT = [1694.91525424, 2164.5021645, 2342.50815695, 2289.18827635, 2155.92183045, 2094.90240415,
+            2080.69183003, 2066.44925898, 1960.27182436, 1826.11342193, 1729.74468968]
+_FALC_TAU = [-0.01232, 0.08798, 0.2002, 0.32704, 0.46667, 0.61462, 0.7674, 0.92221, 1.07711, 1.22829, 1.37282]
+
+
+def _interpol_1d(f, x, xi):
+    """interpol_1d (cmlib/mathtools.cc:1057-1117): quadratic Bezier between
+    the bracketing points with Fritsch-Butland derivatives (one-sided at the
+    ends), constant outside."""
+    N = len(x)
+    if not (xi < x[N - 1] and xi > x[0]):
+        return f[N - 1] if xi >= x[N - 1] else f[0]
+    i = 0
+    while i <= N - 2 and not xi < x[i + 1]:
+        i += 1
+    h1 = h2 = 0.0
+    if i > 0:
+        h1, h2 = x[i] - x[i - 1], x[i + 1] - x[i]
+        d1, d2 = (f[i] - f[i - 1]) / h1, (f[i + 1] - f[i]) / h2
+        al = 1.0 / 3.0 * (1.0 + h2 / (h1 + h2))
+        fd1 = d1 * d2 / (al * d2 + (1.0 - al) * d1) if d1 * d2 > 0 else 0.0
+    else:
+        fd1 = (f[1] - f[0]) / (x[1] - x[0])
+    C0 = f[i] + h2 * 0.5 * fd1
+    if i + 1 < N - 1:
+        h1, h2 = x[i + 1] - x[i], x[i + 2] - x[i + 1]
+        d1, d2 = (f[i + 1] - f[i]) / h1, (f[i + 2] - f[i + 1]) / h2
+        al = 1.0 / 3.0 * (1.0 + h2 / (h1 + h2))
+        fd2 = d1 * d2 / (al * d2 + (1.0 - al) * d1) if d1 * d2 > 0 else 0.0
+    else:
+        fd2 = (f[N - 1] - f[N - 2]) / (x[N - 1] - x[N - 2])
+    C1 = f[i + 1] - h1 * 0.5 * fd2
+    C = (C0 + C1) * 0.5
+    u = (xi - x[i]) / (x[i + 1] - x[i])
+    return (1.0 - u) * (1.0 - u) * f[i] + u * u * f[i + 1] + u * 2.0 * (1.0 - u) * C
+
+
+def _interp_temp(t, y, x):
+    """atmospheric_interpolation for the temperature (cmlib/mathtools.cc:
+    1208-1315, is_temp): top derivative 0, bottom derivative from the FAL-C
+    table at the deepest node (interpol_1d, its end values outside the
+    table), the same Bezier segments as the other parameters, then the
+    result clamped to 3400..20000 K; one node gives a constant."""
+    n = len(t)
+    if n == 1:
+        return np.full_like(x, y[0])
+    if t[-1] < _FALC_TAU[0]:
+        db = _FALC_DT[0]
+    elif t[-1] > _FALC_TAU[-1]:
+        db = _FALC_DT[-1]
+    else:
+        db = _interpol_1d(_FALC_DT, _FALC_TAU, t[-1])
+    d = np.zeros(n)
+    d[-1] = db
+    for i in range(1, n - 1):
+        h1, h2 = t[i] - t[i - 1], t[i + 1] - t[i]
+        d1, d2 = (y[i] - y[i - 1]) / h1, (y[i + 1] - y[i]) / h2
+        if d1 * d2 > 0:
+            al = 0.3333333 * (1.0 + h2 / (h1 + h2))
+            d[i] = d1 * d2 / (al * d2 + (1.0 - al) * d1)
+    out = _bezier(t, y, d, x)
+    return np.clip(out, 3400.0, 20000.0)
+
+
+def _bezier(t, y, d, x):
+    n = len(t)
+    out = np.empty_like(x)
+    for j, xx in enumerate(x):
+        if xx < t[0]:
+            out[j] = y[0] + d[0] * (xx - t[0])
+        elif xx > t[-1]:
+            out[j] = y[-1] + d[-1] * (xx - t[-1])
+        else:
+            ii = 0
+            while ii < n - 1 and not xx <= t[ii + 1]:
+                ii += 1
+            hh = t[ii + 1] - t[ii]
+            E = y[ii] + hh / 3.0 * d[ii]
+            F = y[ii + 1] - hh / 3.0 * d[ii + 1]
+            u = (xx - t[ii]) / hh
+            out[j] = (1 - u) ** 3 * y[ii] + u ** 3 * y[ii + 1] + 3 * u * (1 - u) ** 2 * E + 3 * u * u * (1 - u) * F
+    return out
+
+
 def nodes_to_state(model_nodes, values, state):
     """Writes vt, vLOS, |B|, inclination, azimuth of state [nfield][nd] from
     node values, the mu = 1 projection of atmosphere::transform

@@ -322,24 +322,32 @@ def build_lmstep(d):
 
 
 def build_interp(d):
-    """Non-temperature node interpolation (is_temp 0) for 1..5 nodes."""
+    """Node interpolation: non-temperature (is_temp 0) for 1..5 nodes, keys
+    n<k>_*; temperature (is_temp 1: flat top, bottom slope from the FAL-C
+    table at the deepest node, clamped to 3400..20000 K) for 1..7 nodes with
+    the deepest node below, inside and above the table's range, keys t<k>_*."""
     recs = {}
-    for nn, seed in ((1, 11), (2, 12), (3, 13), (4, 14), (5, 15)):
-        out = run(d, ["interp", str(nn), "0", str(seed)], he=False)
+    cases = [("n%d" % nn, [str(nn), "0", str(seed)]) for nn, seed in ((1, 11), (2, 12), (3, 13), (4, 14), (5, 15))]
+    cases += [("t1", ["1", "1", "21"]), ("t2", ["2", "1", "22", "-1.0"]), ("t3", ["3", "1", "23", "0.1"]),
+              ("t4", ["4", "1", "24", "0.55"]), ("t5", ["5", "1", "25", "1.0"]), ("t6", ["6", "1", "26", "1.5"]),
+              ("t7", ["7", "1", "27", "0.9"])]
+    for key, args in cases:
+        nn = key
+        out = run(d, ["interp"] + args, he=False)
         q = []
         for ln in out:
             t = ln.split()
             if t[0] == "nodes":
                 v = [float(x) for x in t[1:]]
-                recs["n%d_tau" % nn] = np.array(v[0::2])
-                recs["n%d_val" % nn] = np.array(v[1::2])
+                recs["%s_tau" % nn] = np.array(v[0::2])
+                recs["%s_val" % nn] = np.array(v[1::2])
             elif t[0] == "q":
                 q.append([float(t[1]), float(t[2])])
         q = np.array(q)
-        recs["n%d_grid" % nn] = q[:, 0]
-        recs["n%d_q" % nn] = q[:, 1]
+        recs["%s_grid" % nn] = q[:, 0]
+        recs["%s_q" % nn] = q[:, 1]
     np.savez_compressed(os.path.join(HERE, "interp.npz"), **recs)
-    print("interp: 5 node sets")
+    print("interp: %d node sets" % len(cases))
 
 
 NLTE_ATOM = "calcium_ir_rh.cfg"

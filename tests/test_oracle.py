@@ -210,6 +210,18 @@ def test_node_interpolation_matches_reference_golden():
         assert np.max(np.abs(q - g["n%d_q" % n])) < 1e-14, n
 
 
+def test_temperature_interpolation_matches_reference_golden():
+    """Temperature node interpolation (atmospheric_interpolation with
+    is_temp): 1..7 seeded nodes, the deepest node below, inside and above the
+    FAL-C slope table (interpol_1d inside it)."""
+    import _golden
+    g = _golden.load("interp.npz")
+    for n in range(1, 8):
+        q = _oracle._interp_temp(g["t%d_tau" % n], g["t%d_val" % n], g["t%d_grid" % n])
+        err = np.max(np.abs(q - g["t%d_q" % n]) / g["t%d_q" % n])
+        assert err < 1e-14, (n, err)
+
+
 def test_ali_fixture_replicates_reference_nltepops():
     """The golden loop (driven with the reference's functions) ends where the
     reference's own nltepops ends."""

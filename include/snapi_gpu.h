@@ -24,7 +24,7 @@
  *                            (multiply_with_transpose, solve_svd    cmlib/mathtools.cc:615-624, 344-387)
  *   snapi_formal_scalar      atmos_ppbez::formal (scalar, with Lambda*)
  *                                                                   atmos/plpar/atmos_ppbez_rts.cc:21-219
- *   snapi_node_jacobian      atmosphere::obs_stokes_responses_to_nodes
+ *   snapi_node_jacobian[_eos] atmosphere::obs_stokes_responses_to_nodes
  *                                                                   atmos/atmos_fit.cc:1205-1352
  *   snapi_invert*            atmosphere::stokes_lm_fit              atmos/atmos_fit.cc:18-227
  *   snapi_ali_accumulate     atom::add (Jbar, Lambda*, norm sums)   atmos/atmol/atom/atom.cc:1865-1927
@@ -590,6 +590,27 @@ int snapi_eos_state(const snapi_chem_table *table, const snapi_state_map *map, i
                     double *resp, double *dstate);
 int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *map, int ncol, int nd,
                         double *state_dev, double *resp_dev, double *dstate_dev, void *stream);
+
+/*
+ * snapi_node_jacobian with temperature nodes (code 1), as
+ * obs_stokes_responses_to_nodes (atmos_fit.cc:1205-1352) runs them: the
+ * state's thermodynamics are rebuilt from the nodes (build_from_nodes,
+ * atmos/plpar/atmos_pp.cc:58-180: the temperature interpolation on the
+ * state's tau grid, hydrostatic equilibrium with chem_he, whose lambda_ref
+ * is the 5000 A the march uses), the response path's state and
+ * thermodynamic derivatives follow with chem and map (snapi_eos_state;
+ * chem's lambda_ref airtovac(5000 A)), and the per-depth responses to T and
+ * Nt are contracted with the temperature nodes' responses dT/dnode and
+ * dNt/dnode, differences over delta_T = 5 K of two rebuilds each.  The
+ * Stokes vector returned is the response path's, as the reference returns
+ * it.  At least one temperature node; adjoint Jacobian only
+ * (SNAPI_ERR_UNSUPPORTED otherwise).  J per K for temperature nodes.
+ */
+int snapi_node_jacobian_eos(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                            const snapi_node_model *model, const snapi_chem_table *chem,
+                            const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
+                            const void *state, const double *nodes, int nlambda, const double *lam, void *stokes,
+                            void *J);
 
 /* snapi_hequilibrium on device-resident arrays, all work on `stream`. */
 int snapi_hequilibrium_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,

@@ -89,6 +89,95 @@ __device__ double interp_param(const NodeArgs &na, const double *v, int code, do
          3.0 * u * u * (1.0 - u) * F;
 }
 
+// Temperature gradient of the FAL-C model against log tau near the bottom
+// (atmospheric_interpolation, cmlib/mathtools.cc:1227-1232): the bottom
+// derivative of the temperature interpolation.
+__constant__ double kFalcDT[11] = {1694.91525424, 2164.5021645, 2342.50815695, 2289.18827635, 2155.92183045,
+                                   2094.90240415, 2080.69183003, 2066.44925898, 1960.27182436, 1826.11342193,
+                                   1729.74468968};
+__constant__ double kFalcTau[11] = {-0.01232, 0.08798, 0.2002, 0.32704, 0.46667, 0.61462,
+                                    0.7674,   0.92221, 1.07711, 1.22829, 1.37282};
+
+// interpol_1d (cmlib/mathtools.cc:1057-1117): quadratic Bezier between the
+// bracketing points, Fritsch-Butland derivatives (one-sided at the ends).
+// Called only strictly inside the table.
+__device__ double interpol_1d(const double *f, const double *x, int N, double xi) {
+  int i;
+  for (i = 0; i <= N - 2; ++i)
+    if (xi < x[i + 1]) break;
+  double h1 = 0.0, h2 = 0.0, fd1, fd2;
+  if (i > 0) {
+    h1 = x[i] - x[i - 1];
+    h2 = x[i + 1] - x[i];
+    const double d1 = (f[i] - f[i - 1]) / h1, d2 = (f[i + 1] - f[i]) / h2;
+    const double al = 1.0 / 3.0 * (1.0 + h2 / (h1 + h2));
+    fd1 = (d1 * d2 > 0) ? d1 * d2 / (al * d2 + (1.0 - al) * d1) : 0.0;
+  } else {
+    fd1 = (f[1] - f[0]) / (x[1] - x[0]);
+  }
+  const double C0 = f[i] + h2 * 0.5 * fd1;
+  if (i + 1 < N - 1) {
+    h1 = x[i + 1] - x[i];
+    h2 = x[i + 2] - x[i + 1];
+    const double d1 = (f[i + 1] - f[i]) / h1, d2 = (f[i + 2] - f[i + 1]) / h2;
+    const double al = 1.0 / 3.0 * (1.0 + h2 / (h1 + h2));
+    fd2 = (d1 * d2 > 0) ? d1 * d2 / (al * d2 + (1.0 - al) * d1) : 0.0;
+  } else {
+    fd2 = (f[N - 1] - f[N - 2]) / (x[N - 1] - x[N - 2]);
+  }
+  const double C1 = f[i + 1] - h1 * 0.5 * fd2;
+  const double C = (C0 + C1) * 0.5;
+  const double u = (xi - x[i]) / (x[i + 1] - x[i]);
+  return (1.0 - u) * (1.0 - u) * f[i] + u * u * f[i + 1] + u * 2.0 * (1.0 - u) * C;
+}
+
+// atmospheric_interpolation of the temperature: flat top, bottom derivative
+// from the FAL-C table at the deepest node (its end values beyond the
+// table), the Bezier segments of interp_param, clamped to 3400..20000 K; one
+// node gives a constant.
+__device__ double interp_temp(const NodeArgs &na, const double *v, double x) {
+  double t[SNAPI_MAX_NODES], y[SNAPI_MAX_NODES], d[SNAPI_MAX_NODES];
+  int n = 0;
+  for (int i = 0; i < na.nnode; ++i)
+    if (na.code[i] == NP_T) {
+      t[n] = na.ltau[i];
+      y[n] = v[i];
+      ++n;
+    }
+  if (n == 0) return 0.0;
+  if (n == 1) return y[0];
+  d[0] = 0.0;
+  const double tb = t[n - 1];
+  d[n - 1] = tb < kFalcTau[0] ? kFalcDT[0] : tb > kFalcTau[10] ? kFalcDT[10] : interpol_1d(kFalcDT, kFalcTau, 11, tb);
+  for (int i = 1; i < n - 1; ++i) {
+    const double h1 = t[i] - t[i - 1], h2 = t[i + 1] - t[i];
+    const double d1 = (y[i] - y[i - 1]) / h1, d2 = (y[i + 1] - y[i]) / h2;
+    d[i] = 0.0;
+    if (d1 * d2 > 0) {
+      const double al = 0.3333333 * (1.0 + h2 / (h1 + h2));
+      d[i] = d1 * d2 / (al * d2 + (1.0 - al) * d1);
+    }
+  }
+  double q;
+  if (x < t[0]) {
+    q = y[0] + d[0] * (x - t[0]);
+  } else if (x > t[n - 1]) {
+    q = y[n - 1] + d[n - 1] * (x - t[n - 1]);
+  } else {
+    int ii = 0;
+    for (ii = 0; ii < n - 1; ++ii)
+      if (x <= t[ii + 1]) break;
+    const double hh = t[ii + 1] - t[ii];
+    const double E = y[ii] + hh / 3.0 * d[ii], F = y[ii + 1] - hh / 3.0 * d[ii + 1];
+    const double u = (x - t[ii]) / hh;
+    q = (1.0 - u) * (1.0 - u) * (1.0 - u) * y[ii] + u * u * u * y[ii + 1] + 3.0 * u * (1.0 - u) * (1.0 - u) * E +
+        3.0 * u * u * (1.0 - u) * F;
+  }
+  if (q < 3400.0) q = 3400.0;
+  if (q > 20000.0) q = 20000.0;
+  return q;
+}
+
 // Field values of one depth from the node values, as the atmosphere stores
 // and projects them for the mu = 1 ray: B = |B| (sin th cos ph, sin th sin
 // ph, cos th), inclination acos(Bz/|B|), azimuth atan(By/Bx) (atmos/
@@ -629,12 +718,187 @@ done:
   return rc;
 }
 
+// ---- temperature nodes: the thermodynamic state rebuilt from the nodes ----
+
+constexpr double kNodeStepT = 5.0;  // delta_T, the temperature-node step (cmlib/const.h:27)
+
+// Temperature profiles of one pixel for the node responses
+// (obs_stokes_responses_to_nodes, atmos_fit.cc:1225-1248): variant 0 the
+// nodes as given, then per temperature node j the node raised by delta_T/2
+// (variant 1+2j) and lowered from there by delta_T (2+2j).  Writes
+// Tv[pix][nvar][nd] and the log tau of each depth (from the state's grid)
+// for the hydrostatic march, ltv[pix][nvar][nd].
+template <typename R>
+__global__ void tprofiles_kernel(NodeArgs na, int npix, int nd, size_t state_stride, const R *__restrict__ state,
+                                 const double *__restrict__ nodes, int nvar, double *__restrict__ Tv,
+                                 double *__restrict__ ltv) {
+  const int pix = blockIdx.x;
+  const double *v0 = nodes + (size_t)pix * na.nnode;
+  const R *st = state + (size_t)pix * state_stride;
+  double v[SNAPI_MAX_NODES];
+  for (int var = 0; var < nvar; ++var) {
+    for (int i = 0; i < na.nnode; ++i) v[i] = v0[i];
+    if (var > 0) {
+      int j = (var - 1) / 2, n = 0;
+      for (int i = 0; i < na.nnode; ++i)
+        if (na.code[i] == NP_T && j-- == 0) n = i;
+      v[n] += kNodeStepT * 0.5;
+      if ((var - 1) % 2) v[n] -= kNodeStepT;
+    }
+    for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+      const double x = log10(-(double)st[SNAPI_F_GRID * nd + z]);
+      Tv[((size_t)pix * nvar + var) * nd + z] = interp_temp(na, v, x);
+      ltv[((size_t)pix * nvar + var) * nd + z] = x;
+    }
+  }
+}
+
+// Node responses of the temperature nodes, dT(z)/dnode and dNt(z)/dnode by
+// the difference of the raised and lowered profiles over delta_T, into the
+// node maps' rows of those nodes; the double state gets the given nodes' T
+// and Nt and the other fields of state.
+template <typename R>
+__global__ void tmaps_kernel(NodeArgs na, int npix, int nd, int nf, int nvar, const double *__restrict__ Tv,
+                             const double *__restrict__ Ntv, const R *__restrict__ state, R *__restrict__ nmap,
+                             R *__restrict__ nmap2, double *__restrict__ state_d) {
+  const int pix = blockIdx.x;
+  for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+    const double *T0 = Tv + (size_t)pix * nvar * nd, *N0 = Ntv + (size_t)pix * nvar * nd;
+    for (int f = 0; f < nf; ++f)
+      state_d[((size_t)pix * nf + f) * nd + z] = (double)state[((size_t)pix * nf + f) * nd + z];
+    state_d[((size_t)pix * nf + SNAPI_F_T) * nd + z] = T0[z];
+    state_d[((size_t)pix * nf + SNAPI_F_NT) * nd + z] = N0[z];
+    int j = 0;
+    for (int n = 0; n < na.nnode; ++n) {
+      if (na.code[n] != NP_T) {
+        nmap2[((size_t)pix * na.nnode + n) * nd + z] = R(0);
+        continue;
+      }
+      const size_t a = (size_t)(1 + 2 * j) * nd + z, b = (size_t)(2 + 2 * j) * nd + z;
+      double dT = T0[a];
+      dT -= T0[b];
+      dT /= kNodeStepT;
+      double dN = N0[a];
+      dN -= N0[b];
+      dN /= kNodeStepT;
+      nmap[((size_t)pix * na.nnode + n) * nd + z] = (R)dT;
+      nmap2[((size_t)pix * na.nnode + n) * nd + z] = (R)dN;
+      ++j;
+    }
+  }
+}
+
+template <typename A, typename B>
+__global__ void convert_kernel(size_t n, const A *__restrict__ a, B *__restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (B)a[i];
+}
+
+// obs_stokes_responses_to_nodes (atmos_fit.cc:1205-1352) with temperature
+// nodes: build_from_nodes (temperature interpolation, hydrostatic
+// equilibrium on the state's tau grid, atmos_pp.cc:58-180, 372-455), the
+// response path's state and thermodynamic derivatives (obs_stokes_responses'
+// preamble), the adjoint per-depth responses contracted with the node
+// responses: T and Nt rows for the temperature nodes (2 x delta_T/2 rebuilds
+// each), the interpolation's differences for the others.
+template <typename R>
+static int node_jacobian_eos_impl(int precision, const snapi_line_table *lines, const snapi_ray *ray, NodeArgs na,
+                                  const snapi_chem_table *chem, const snapi_chem_table *chem_he,
+                                  const snapi_state_map *map, int npix, int nd, const void *state_h,
+                                  const double *nodes_h, int nlambda, const double *lam_h, void *stokes_h, void *J_h) {
+  KLines kl;
+  SynthArgs a;
+  int rc = prepare_synth(lines, ray, 1, nd, nlambda, kl, a);
+  if (rc) return rc;
+  const int nf = a.nfield, nn = na.nnode, nl = nlambda;
+  if (nf != SNAPI_NF_BASE + 2 * map->nline) return SNAPI_ERR_ARG;
+  int ntn = 0;
+  for (int n = 0; n < nn; ++n) ntn += na.code[n] == NP_T;
+  const int nvar = 1 + 2 * ntn;
+  const size_t es = sizeof(R), ns = (size_t)nf * nd;
+  for (int n = 0; n < nn; ++n) na.scale[n] = 1.0;  // unscaled derivatives
+  R *dstate = nullptr, *rstate = nullptr, *rdst = nullptr, *S = nullptr, *J = nullptr, *nmap = nullptr,
+    *nmap2 = nullptr, *scr = nullptr;
+  double *dlam = nullptr, *dnodes = nullptr, *Tv = nullptr, *ltv = nullptr, *Ntv = nullptr, *sd = nullptr,
+         *rsd = nullptr, *dsd = nullptr;
+  RespArgs ra = {};
+  HIPCHK(hipMalloc((void **)&dstate, (size_t)npix * ns * es));
+  HIPCHK(hipMalloc((void **)&rstate, (size_t)npix * ns * es));
+  HIPCHK(hipMalloc((void **)&rdst, (size_t)npix * 2 * ns * es));
+  HIPCHK(hipMalloc((void **)&S, (size_t)npix * nl * 4 * es));
+  HIPCHK(hipMalloc((void **)&J, (size_t)npix * nl * 4 * nn * es));
+  HIPCHK(hipMalloc((void **)&nmap, (size_t)npix * nn * nd * es));
+  HIPCHK(hipMalloc((void **)&nmap2, (size_t)npix * nn * nd * es));
+  HIPCHK(hipMalloc((void **)&dlam, (size_t)nl * 8));
+  HIPCHK(hipMalloc((void **)&dnodes, (size_t)npix * nn * 8));
+  HIPCHK(hipMalloc((void **)&Tv, (size_t)npix * nvar * nd * 8));
+  HIPCHK(hipMalloc((void **)&ltv, (size_t)npix * nvar * nd * 8));
+  HIPCHK(hipMalloc((void **)&Ntv, (size_t)npix * nvar * nd * 8));
+  HIPCHK(hipMalloc((void **)&sd, (size_t)npix * ns * 8));
+  HIPCHK(hipMalloc((void **)&rsd, (size_t)npix * ns * 8));
+  HIPCHK(hipMalloc((void **)&dsd, (size_t)npix * 2 * ns * 8));
+  HIPCHK(hipMemcpy(dstate, state_h, (size_t)npix * ns * es, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dlam, lam_h, (size_t)nl * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dnodes, nodes_h, (size_t)npix * nn * 8, hipMemcpyHostToDevice));
+  reset_status();
+  hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, nf, dnodes, (size_t)nn,
+                     dstate, ns, nmap, (const int *)nullptr);
+  hipLaunchKernelGGL(tprofiles_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, ns, dstate, dnodes, nvar, Tv,
+                     ltv);
+  if (hipGetLastError() != hipSuccess) {
+    rc = SNAPI_ERR_DEVICE;
+    goto done;
+  }
+  rc = snapi_hequilibrium_dev(chem_he, npix * nvar, nd, Tv, ltv, Ntv, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, nullptr, nullptr);
+  if (rc) goto done;
+  hipLaunchKernelGGL(tmaps_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, nf, nvar, Tv, Ntv, dstate, nmap,
+                     nmap2, sd);
+  rc = snapi_eos_state_dev(chem, map, npix, nd, sd, rsd, dsd, nullptr);
+  if (rc) goto done;
+  hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, 0, (size_t)npix * ns, rsd, rstate);
+  hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, 0, (size_t)npix * 2 * ns, dsd, rdst);
+  ra.nseed = 4;
+  ra.sink = SINK_NODE;
+  for (int s = 0; s < 4; ++s) ra.ws[s] = 1.0;
+  ra.nnode = nn;
+  for (int n = 0; n < nn; ++n) {
+    const int c = na.code[n];
+    ra.pidx[n] = c == NP_T ? 0 : c == NP_VT ? 2 : c == NP_VS ? 3 : c == NP_B ? 4 : c == NP_THETA ? 5 : 6;
+    ra.pidx2[n] = c == NP_T ? 1 : -1;
+    ra.nscale[n] = 1.0;
+  }
+  ra.nmap = nmap;
+  ra.nmap2 = nmap2;
+  {
+    const size_t per_col = resp_scratch_per_col(nl, nd, es);
+    size_t c = ((size_t)16 << 30) / (per_col ? per_col : 1);
+    if (c < 1) c = 1;
+    if (c > (size_t)npix) c = npix;
+    HIPCHK(hipMalloc((void **)&scr, per_col * c));
+    rc = launch_resp<R>(kl, a, ra, npix, rstate, rdst, dlam, S, J, scr, (int)c, 0);
+    if (rc) goto done;
+  }
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    rc = SNAPI_ERR_DEVICE;
+    goto done;
+  }
+  HIPCHK(hipMemcpy(stokes_h, S, (size_t)npix * nl * 4 * es, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(J_h, J, (size_t)npix * nl * 4 * nn * es, hipMemcpyDeviceToHost));
+  rc = read_status();
+done:
+  hipFree(dstate); hipFree(rstate); hipFree(rdst); hipFree(S); hipFree(J); hipFree(nmap); hipFree(nmap2);
+  hipFree(scr); hipFree(dlam); hipFree(dnodes); hipFree(Tv); hipFree(ltv); hipFree(Ntv); hipFree(sd);
+  hipFree(rsd); hipFree(dsd);
+  return rc;
+}
+
 }  // namespace snapi
 
 using namespace snapi;
 
 static int setup_nodes(const snapi_line_table *lines, const snapi_ray *ray, const snapi_node_model *model, int npix,
-                       int nd, int nlambda, int jacobian, int max_iter, NodeArgs &na) {
+                       int nd, int nlambda, int jacobian, int max_iter, NodeArgs &na, bool temperature = false) {
   if (!lines || !ray || !model) return SNAPI_ERR_ARG;
   if (npix <= 0 || nd < 3 || nlambda <= 0 || max_iter <= 0) return SNAPI_ERR_ARG;
   if (model->nnode <= 0 || model->nnode > SNAPI_MAX_NODES - 1) return SNAPI_ERR_ARG;
@@ -645,11 +909,12 @@ static int setup_nodes(const snapi_line_table *lines, const snapi_ray *ray, cons
   na.nnode = model->nnode;
   for (int n = 0; n < na.nnode; ++n) {
     const int c = model->param[n];
-    if (c < NP_VT || c > NP_PHI) return SNAPI_ERR_UNSUPPORTED;  // temperature nodes need the equation of state
+    // temperature nodes need the equation of state (the _eos entry points)
+    if (c < (temperature ? NP_T : NP_VT) || c > NP_PHI) return SNAPI_ERR_UNSUPPORTED;
     na.code[n] = c;
     na.ltau[n] = model->logtau[n];
-    na.scale[n] = c == NP_VT ? 1E5 : c == NP_VS ? 1E5 : c == NP_B ? 1000.0 : 1.0;
-    na.step[n] = (c == NP_VT || c == NP_VS) ? 20.0 : c == NP_B ? 0.01 : 0.001;
+    na.scale[n] = c == NP_T ? 5000.0 : c == NP_VT ? 1E5 : c == NP_VS ? 1E5 : c == NP_B ? 1000.0 : 1.0;
+    na.step[n] = c == NP_T ? kNodeStepT : (c == NP_VT || c == NP_VS) ? 20.0 : c == NP_B ? 0.01 : 0.001;
   }
   return SNAPI_OK;
 }
@@ -710,5 +975,27 @@ extern "C" int snapi_node_jacobian(int precision, int jacobian, const snapi_line
   if (precision == SNAPI_MIXED)
     return node_jacobian_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state, nodes, nlambda, lam,
                                      stokes, J);
+  return SNAPI_ERR_ARG;
+}
+
+extern "C" int snapi_node_jacobian_eos(int precision, int jacobian, const snapi_line_table *lines,
+                                       const snapi_ray *ray, const snapi_node_model *model,
+                                       const snapi_chem_table *chem, const snapi_chem_table *chem_he,
+                                       const snapi_state_map *map, int npix, int nd, const void *state,
+                                       const double *nodes, int nlambda, const double *lam, void *stokes, void *J) {
+  if (!state || !nodes || !lam || !stokes || !J || !chem || !chem_he || !map) return SNAPI_ERR_ARG;
+  NodeArgs na;
+  int rc = setup_nodes(lines, ray, model, npix, nd, nlambda, jacobian, 1, na, true);
+  if (rc) return rc;
+  int ntn = 0;
+  for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
+  if (ntn < 1) return SNAPI_ERR_ARG;  // build_from_nodes needs a temperature node
+  if (jacobian != SNAPI_JAC_ADJOINT) return SNAPI_ERR_UNSUPPORTED;
+  if (precision == SNAPI_FP64)
+    return node_jacobian_eos_impl<double>(precision, lines, ray, na, chem, chem_he, map, npix, nd, state, nodes,
+                                          nlambda, lam, stokes, J);
+  if (precision == SNAPI_MIXED)
+    return node_jacobian_eos_impl<float>(precision, lines, ray, na, chem, chem_he, map, npix, nd, state, nodes,
+                                         nlambda, lam, stokes, J);
   return SNAPI_ERR_ARG;
 }

@@ -24,6 +24,11 @@ struct RespArgs {
   int pidx[SNAPI_MAX_NODES];
   double nscale[SNAPI_MAX_NODES];
   const void *nmap;   // [ncol][nnode][nd]
+  // a second parameter row per node (temperature nodes move T and, through
+  // hydrostatic equilibrium, Nt): with nmap2 set, node n with pidx2[n] >= 0
+  // adds sum_z dS_s/dq_{pidx2[n]}(z) nmap2[col][n][z]
+  int pidx2[SNAPI_MAX_NODES];
+  const void *nmap2;  // [ncol][nnode][nd] or null
   const void *mask;   // [ncol][nlambda] or null
   const int *active;  // [ncol] or null: columns with 0 are skipped
 };

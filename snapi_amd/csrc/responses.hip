@@ -536,6 +536,11 @@ __global__ __launch_bounds__(kRespThreads) void resp_kernel(const KLines lines, 
           const int p = ra.pidx[n];
           R acc = R(0);
           for (int k = 0; k < nd; ++k) acc += scr.adj(k, p, s, w) * nm[(size_t)n * nd + zof(k)];
+          if (ra.nmap2 && ra.pidx2[n] >= 0) {
+            const R *nm2 = static_cast<const R *>(ra.nmap2) + (size_t)col * ra.nnode * nd;
+            const int p2 = ra.pidx2[n];
+            for (int k = 0; k < nd; ++k) acc += scr.adj(k, p2, s, w) * nm2[(size_t)n * nd + zof(k)];
+          }
           out[((size_t)col * nl * 4 + (size_t)l * 4 + s) * ra.nnode + n] = acc * (R)(ra.nscale[n] * ra.ws[s]) * mk;
         }
     }

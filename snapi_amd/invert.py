@@ -116,3 +116,31 @@ def node_jacobian(table, ray, model, state, nodes, lam_vac, precision=abi.SNAPI_
            len(lam), abi.ptr(lam), abi.vptr(stokes), abi.vptr(J))
     check("snapi_node_jacobian", rc)
     return stokes, J
+
+
+def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_vac, precision=abi.SNAPI_FP64):
+    """(stokes, J) of snapi_node_jacobian_eos: temperature nodes allowed, the
+    thermodynamic fields of state rebuilt from the nodes (hydrostatic
+    equilibrium with chem_he, equation of state with chem and smap); adjoint
+    Jacobian; the Stokes vector of the response path."""
+    dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
+    st = np.ascontiguousarray(state, dtype=dt)
+    if st.ndim == 2:
+        st = st[None]
+    npix, nf, nd = st.shape
+    if nf != abi.nfield(table.nline):
+        raise ValueError("state has %d fields, line table needs %d" % (nf, abi.nfield(table.nline)))
+    nv = np.ascontiguousarray(nodes, dtype=np.float64).reshape(npix, model.nnode)
+    lam = np.ascontiguousarray(lam_vac, dtype=np.float64)
+    stokes = np.zeros((npix, len(lam), 4), dtype=dt)
+    J = np.zeros((npix, len(lam) * 4, model.nnode), dtype=dt)
+    f = lib().snapi_node_jacobian_eos
+    f.restype = C.c_int
+    TB = C.POINTER(abi.SnapiChemTable)
+    f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay),
+                  C.POINTER(SnapiNodeModel), TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p,
+                  C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+    rc = f(precision, 0, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he),
+           C.byref(smap), npix, nd, abi.vptr(st), abi.ptr(nv), len(lam), abi.ptr(lam), abi.vptr(stokes), abi.vptr(J))
+    check("snapi_node_jacobian_eos", rc)
+    return stokes, J

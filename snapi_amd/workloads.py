@@ -374,3 +374,17 @@ def hequilibrium_problem(nx, ny, seed=6):
     T = np.ascontiguousarray(he[None, :, 0] + dT[:, None] + rng.normal(0.0, 30.0, (ncol, nd)))
     logtau = np.ascontiguousarray(np.broadcast_to(he[None, :, 1], (ncol, nd)))
     return dict(table=table, T=T, logtau=logtau)
+
+
+def eos_tables(dump_name, lam_mid):
+    """The species tables and state map that rebuild the thermodynamics of a
+    column recorded in tests/golden/<dump_name>.npz: the table at
+    airtovac(5000 A) (synthesis and response paths), the one at 5000 A
+    (hydrostatic equilibrium), and the state map of the recorded lines and
+    bound-free background.  Returns (chem, chem_he, smap)."""
+    d = np.load(os.path.join(GOLDEN, "chemeq.npz"))
+    g = np.load(os.path.join(GOLDEN, dump_name + ".npz"))
+    chem = abi.make_chem_table(d, referent=0)
+    chem_he = abi.make_chem_table(d, referent=1)
+    smap = abi.make_state_map(chem, d, g["lines"], g["bfsig"], lam_mid)
+    return chem, chem_he, smap

@@ -305,6 +305,50 @@ def node_jacobian(table, ray, model_nodes, values, state, lam, fd_scale=1.0):
     return S[0], J
 
 
+def node_jacobian_eos(table, ray, model_nodes, values, state, lam, chem, chem_he, smap, fd_scale=1.0):
+    """Stokes and dS/dnode of one pixel with temperature nodes the
+    reference's way (obs_stokes_responses_to_nodes, atmos_fit.cc:1205-1352):
+    build_from_nodes (temperature interpolation, hydrostatic equilibrium),
+    the response path's state and derivatives, per-depth responses, and the
+    node responses: dT/dnode and dNt/dnode from rebuilds at the node
+    +- delta_T/2, the interpolation's differences for the other nodes."""
+    from snapi_amd import abi
+    st = nodes_to_state(model_nodes, values, state)
+    x = np.log10(-st[abi.F_GRID])
+    tn = [i for i, (p, _) in enumerate(model_nodes) if p == "T"]
+    t_pos = np.array([model_nodes[i][1] for i in tn])
+
+    def build(vals):
+        T = _interp_temp(t_pos, np.array([vals[i] for i in tn]), x)
+        Nt, _, _, _, _ = hequilibrium(chem_he, T, x)
+        return T, Nt[0]
+
+    T0, N0 = build(values)
+    st[abi.F_T], st[abi.F_NT] = T0, N0
+    _, resp, ds = eos_state(chem, smap, st[None])
+    S, R, rc = responses(table, ray, resp, ds, lam, fd_scale=fd_scale)
+    nd = st.shape[1]
+    others = [(p, lt) for p, lt in model_nodes if p != "T"]
+    ov = [v for (p, _), v in zip(model_nodes, values) if p != "T"]
+    nm_o = node_map(others, ov, state) if others else np.zeros((0, nd))
+    pidx = {"vt": 2, "vlos": 3, "B": 4, "theta": 5, "phi": 6}
+    J = np.zeros((len(lam) * 4, len(model_nodes)))
+    k = 0
+    for n, (p, _) in enumerate(model_nodes):
+        if p == "T":
+            vp = np.array(values, dtype=np.float64)
+            vp[n] += 2.5
+            Tp, Np = build(vp)
+            vp[n] -= 5.0
+            Tm, Nm = build(vp)
+            dT, dN = (Tp - Tm) / 5.0, (Np - Nm) / 5.0
+            J[:, n] = (np.einsum("zls,z->ls", R[0, 0], dT) + np.einsum("zls,z->ls", R[0, 1], dN)).reshape(-1)
+        else:
+            J[:, n] = np.einsum("zls,z->ls", R[0, pidx[p]], nm_o[k]).reshape(-1)
+            k += 1
+    return S[0], J
+
+
 def _ali_lib():
     from snapi_amd import abi
     L = lib()

@@ -1,0 +1,65 @@
+"""Node Jacobians with temperature nodes on the GPU (snapi_node_jacobian_eos:
+the 14-node model of the cfg2 workload, 7 of them temperature nodes) against
+the oracle's reference-algorithm Jacobian: build_from_nodes with the
+temperature interpolation and hydrostatic equilibrium, the response path's
+state and thermodynamic derivatives, per-depth responses contracted with the
+node responses (temperature nodes: dT/dnode and dNt/dnode from rebuilds at
++- 2.5 K).
+
+Tolerance: the GPU sweeps the exact derivative through the solver while the
+oracle differences it (10x smaller steps than the reference's), as for the
+other node Jacobians: 1e-4 of each (pixel, node) column's largest value on
+99 % of the rows, 3e-3 everywhere.  The Stokes vectors (both at the response
+path's state) agree to 1e-10 of the continuum."""
+import numpy as np
+import pytest
+
+import _oracle
+from snapi_amd import abi, invert, workloads
+
+pytestmark = pytest.mark.gpu
+
+NODES = [("T", -5.0), ("T", -4.0), ("T", -3.0), ("T", -2.0), ("T", -1.0), ("T", 0.0), ("T", 1.0),
+         ("vt", -2.0), ("vlos", -4.0), ("vlos", 0.0), ("B", -4.0), ("B", 0.0), ("theta", -2.0), ("phi", -2.0)]
+
+
+@pytest.fixture(scope="module")
+def pix():
+    t = workloads.template_cfg1()
+    chem, chem_he, smap = workloads.eos_tables("cfg1_template", t["ray"].lam_mid)
+    rng = np.random.default_rng(33)
+    npix = 3
+    lt = np.log10(-t["state"][abi.F_GRID])
+    Tn = np.interp([n[1] for n in NODES[:7]], lt, t["state"][abi.F_T])
+    v = np.zeros((npix, len(NODES)))
+    v[:, :7] = Tn[None] + rng.normal(0.0, 100.0, (npix, 7))
+    v[:, 7] = rng.uniform(0.3e5, 1.8e5, npix)
+    v[:, 8] = rng.normal(0, 1.5e5, npix)
+    v[:, 9] = v[:, 8] + rng.normal(0, 0.5e5, npix)
+    v[:, 10] = rng.uniform(200, 2500, npix)
+    v[:, 11] = v[:, 10] * rng.uniform(0.7, 1.3, npix)
+    v[:, 12] = np.arccos(rng.uniform(-0.9, 0.9, npix))
+    v[:, 13] = rng.uniform(0.1, 1.4, npix)
+    st = np.repeat(t["state"][None], npix, axis=0)
+    return dict(t=t, v=v, st=st, lam=t["lam_vac"][::6], model=invert.node_model(NODES), chem=chem,
+                chem_he=chem_he, smap=smap)
+
+
+def test_temperature_node_jacobian_against_oracle(pix):
+    p = pix
+    t = p["t"]
+    print("node Jacobian with temperature nodes: %d pixels, %d wavelengths, 14 nodes, adjoint FP64"
+          % (len(p["v"]), len(p["lam"])), flush=True)
+    s1, J1 = invert.node_jacobian_eos(t["table"], t["ray"], p["model"], p["chem"], p["chem_he"], p["smap"], p["st"],
+                                      p["v"], p["lam"])
+    for i in range(len(p["v"])):
+        so, Jo = _oracle.node_jacobian_eos(t["table"], t["ray"], NODES, p["v"][i], p["st"][i], p["lam"], p["chem"],
+                                           p["chem_he"], p["smap"], fd_scale=0.1)
+        es = np.max(np.abs(s1[i] - so)) / np.max(so[:, 0])
+        scale = np.maximum(np.abs(Jo).max(axis=0, keepdims=True), 1e-300)
+        e = np.abs(J1[i] - Jo) / scale
+        print("  pixel %d: Stokes %.2e; J max %.2e, p99 %.2e; T nodes max %.2e" %
+              (i, es, e.max(), np.quantile(e, 0.99), e[:, :7].max()), flush=True)
+        assert es < 1e-10
+        assert np.quantile(e, 0.99) < 1e-4 and e.max() < 3e-3, i
+    assert np.all(np.abs(J1[:, :, :7]).max(axis=1) > 0)

@@ -522,6 +522,7 @@ def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):
             % (label, total, sec, total / sec, it.mean(), np.median(red)))
         return out
 
+    tnodes = tnode_inversion(torch, dev, tmpl, lam_d, stream, ws)
     mixed = run(npix, abi.SNAPI_MIXED, 2, "mixed tangent")
     fd = run(min(nfd, npix), abi.SNAPI_FP64, 1, "fp64 differences")
     adj = run(min(nfd, npix), abi.SNAPI_MIXED, 0, "mixed adjoint")
@@ -540,7 +541,66 @@ def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):
             "value": mixed["pixel_iter_per_s"], "unit": "LM pixel-iterations/s",
             "mixed_tangent": mixed, "fp64_differences": fd, "mixed_adjoint": adj,
             "speedup_vs_fp64_differences": mixed["pixel_iter_per_s"] / fd["pixel_iter_per_s"],
-            "jacobian_rel_err_mixed_vs_fp64": jerr}
+            "jacobian_rel_err_mixed_vs_fp64": jerr, "with_temperature_nodes": tnodes}
+
+
+def tnode_inversion(torch, dev, tmpl, lam_d, stream, ws, side=48):
+    """The 14-node model (7 temperature nodes): every iteration rebuilds the
+    thermodynamics (hydrostatic equilibrium, equation of state) and the
+    temperature-node responses (two rebuilds per node), adjoint Jacobian in
+    mixed precision."""
+    from snapi_amd import abi, api, invert, workloads
+    table, ray = tmpl["table"], tmpl["ray"]
+    nlam = lam_d.shape[0]
+    t0 = time.time()
+    prob = workloads.inversion_problem_tnodes(tmpl, side, side, seed=5)
+    chem, chem_he, smap = workloads.eos_tables("cfg1_template", ray.lam_mid)
+    npix = prob["base"].shape[0]
+    nn = len(prob["nodes"])
+    model = invert.node_model(prob["nodes"])
+    st64 = torch.from_numpy(prob["base"]).to(dev)
+    truth = torch.from_numpy(prob["truth"]).to(dev)
+    invert.build_from_nodes_dev(model, chem, chem_he, smap, st64, truth, stream)
+    obs64 = torch.empty((npix, nlam, 4), dtype=torch.float64, device=dev)
+    api.synth_stokes_dev(table, ray, st64, lam_d, obs64, abi.SNAPI_FP64, stream)
+    torch.cuda.synchronize()
+    ic = obs64[:, :, 0].amax(dim=1)
+    g = torch.Generator(device=dev)
+    g.manual_seed(9)
+    obs64 += torch.randn(obs64.shape, dtype=torch.float64, device=dev, generator=g) * (1e-3 * ic)[:, None, None]
+    mask = torch.from_numpy(prob["mask"]).to(dev)
+    start = torch.from_numpy(prob["start"]).to(dev)
+    wsq = sum(w * w for w in ws)
+    chi2_noise = mask.sum(dim=1) * wsq * (1e-3 * ic) ** 2
+    log("inversion with temperature nodes: %d pixels, %d nodes, built in %.1fs" % (npix, nn, time.time() - t0))
+    st_t = st64.to(torch.float32).contiguous()
+    ob_t = obs64.to(torch.float32).contiguous()
+    mk_t = mask.to(torch.float32).contiguous()
+    nodes = torch.empty((npix, nn), dtype=torch.float64, device=dev)
+    chi2 = torch.empty(npix, dtype=torch.float64, device=dev)
+    iters = torch.empty(npix, dtype=torch.int32, device=dev)
+    invert.invert_eos_dev(table, ray, model, chem, chem_he, smap, st_t[:16], lam_d, ob_t[:16], mk_t[:16],
+                          start[:16], nodes[:16], chi2[:16], iters[:16], wstokes=ws, max_iter=2,
+                          precision=abi.SNAPI_MIXED, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    total = invert.invert_eos_dev(table, ray, model, chem, chem_he, smap, st_t, lam_d, ob_t, mk_t, start, nodes,
+                                  chi2, iters, wstokes=ws, max_iter=30, precision=abi.SNAPI_MIXED, stream=stream)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t1
+    it = iters.cpu().numpy()
+    red = (chi2 / chi2_noise).cpu().numpy()
+    dT = np.abs(nodes.cpu().numpy()[:, 2:6] - prob["truth"][:, 2:6])
+    out = {"workload": "batched LM inversion with the 14-node model (7 T, 2 vLOS, 1 vmicro, 2 B, 1 incl, 1 azi), "
+                       "%d pixels, 71 depths, %d wavelengths masked to 120-%d, noise 1e-3 Ic, start +/-20 %%, "
+                       "<=30 iterations, mixed adjoint" % (npix, nlam, nlam),
+           "pixels": npix, "pixel_iterations": int(total), "seconds": round(sec, 3),
+           "pixel_iter_per_s": total / sec, "mean_iterations": float(it.mean()),
+           "at_iteration_limit": float(np.mean(it >= 30)), "median_chi2_over_noise": float(np.median(red)),
+           "median_abs_T_err_logtau_-3_to_0_K": float(np.median(dT))}
+    log("14-node inversion: %d pixel-iterations in %.2fs = %.4g /s; mean iterations %.1f; median chi2/noise %.3g; "
+        "median |dT| (log tau -3..0) %.0f K" % (total, sec, total / sec, it.mean(), np.median(red), np.median(dT)))
+    return out
 
 
 if __name__ == "__main__":

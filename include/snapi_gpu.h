@@ -26,7 +26,8 @@
  *                                                                   atmos/plpar/atmos_ppbez_rts.cc:21-219
  *   snapi_node_jacobian[_eos] atmosphere::obs_stokes_responses_to_nodes
  *                                                                   atmos/atmos_fit.cc:1205-1352
- *   snapi_invert*            atmosphere::stokes_lm_fit              atmos/atmos_fit.cc:18-227
+ *   snapi_invert*[_eos]      atmosphere::stokes_lm_fit              atmos/atmos_fit.cc:18-227
+ *   snapi_build_from_nodes*  atmos_pp::build_from_nodes + popsetup  atmos/plpar/atmos_pp.cc:58-180
  *   snapi_ali_accumulate     atom::add (Jbar, Lambda*, norm sums)   atmos/atmol/atom/atom.cc:1865-1927
  *   snapi_ali_newpops        atom::newpops                          atmos/atmol/atom/atom_pop.cc:27-135
  *   snapi_ali_solve*         atmosphere::nltepops (one NLTE atom)   atmos/atmos_pop.cc:67-240
@@ -606,6 +607,46 @@ int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *ma
  * it.  At least one temperature node; adjoint Jacobian only
  * (SNAPI_ERR_UNSUPPORTED otherwise).  J per K for temperature nodes.
  */
+/*
+ * snapi_invert[_dev] with temperature nodes: per pixel and iteration the
+ * thermodynamics are rebuilt from the nodes as snapi_node_jacobian_eos
+ * does; the Jacobian and the current chi^2 come from the response path, the
+ * trial's chi^2 from the synthesis path's state (forward_evaluate), as
+ * stokes_lm_fit takes them (atmos_fit.cc:18-227).  The state's
+ * thermodynamic fields are rebuilt; its grid and, for parameters without
+ * nodes, the fields stay as given.  At least one temperature node; adjoint
+ * Jacobian only.
+ */
+int snapi_invert_eos(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                     const snapi_node_model *model, const snapi_chem_table *chem, const snapi_chem_table *chem_he,
+                     const snapi_state_map *map, int npix, int nd, const void *state, int nlambda,
+                     const double *lam, const void *obs, const void *mask, const double *wstokes, double lm_start,
+                     int max_iter, int nslot, double *nodes, double *chi2, int *iterations,
+                     long long *pixel_iterations);
+int snapi_invert_eos_dev(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                         const snapi_node_model *model, const snapi_chem_table *chem,
+                         const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
+                         const void *state_dev, int nlambda, const double *lam_dev, const void *obs_dev,
+                         const void *mask_dev, const double *wstokes, double lm_start, int max_iter, int nslot,
+                         const double *nodes0_dev, double *nodes_dev, double *chi2_dev, int *iterations_dev,
+                         long long *pixel_iterations, void *stream);
+
+/*
+ * atmos_pp::build_from_nodes (atmos/plpar/atmos_pp.cc:58-180) followed by
+ * popsetup and compute_op_referent, for npix pixels: the node-controlled
+ * fields of state (double [npix][nfield][nd], nfield from map) from the
+ * node values (nodes [npix][nnode]), T by the temperature interpolation on
+ * the state's tau grid (GRID in), NT by hydrostatic equilibrium with
+ * chem_he, and the equation-of-state fields of the synthesis path with
+ * chem and map (snapi_eos_state).  At least one temperature node.
+ */
+int snapi_build_from_nodes(const snapi_node_model *model, const snapi_chem_table *chem,
+                           const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
+                           double *state, const double *nodes);
+int snapi_build_from_nodes_dev(const snapi_node_model *model, const snapi_chem_table *chem,
+                               const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
+                               double *state_dev, const double *nodes_dev, void *stream);
+
 int snapi_node_jacobian_eos(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
                             const snapi_node_model *model, const snapi_chem_table *chem,
                             const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,

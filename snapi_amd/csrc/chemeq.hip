@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "rt_common.h"
+#include "chem_device.h"
 
 namespace {
 
@@ -798,7 +799,8 @@ __global__ __launch_bounds__(kChemThreads, 2) void hequilibrium_kernel(const sna
                                                                     int nd, const double *__restrict__ Tin,
                                                                     const double *__restrict__ logtau,
                                                                     double *__restrict__ Nt_out,
-                                                                    int *__restrict__ it_out) {
+                                                                    int *__restrict__ it_out,
+                                                                    const int *__restrict__ active, int group) {
   __shared__ ChemShared sh;
   extern __shared__ double region_s[];
   const int tid = threadIdx.x;
@@ -806,6 +808,7 @@ __global__ __launch_bounds__(kChemThreads, 2) void hequilibrium_kernel(const sna
   const ChemTeam cx = make_team(tb, dv, sh, region_s, tid);
   const long col = ((long)blockIdx.x * blockDim.x + tid) / kTeam;
   if (col >= ncol) return;  // the whole team leaves together; no barrier follows
+  if (active && !active[col / group]) return;
   const double *T = Tin + col * nd, *lt = logtau + col * nd;
   double Ntop = 0.0, Nprev = 0.0, Tprev = 0.0, tprev = 0.0, op_prev = 0.0, rho_prev = 0.0;
   for (int z = 0; z < nd; ++z) {
@@ -906,7 +909,8 @@ __global__ __launch_bounds__(kChemThreads) void eos_variants_kernel(const snapi_
                                                                     const ChemDerived *__restrict__ dv, long npt,
                                                                     int nd, int nf, int nvar,
                                                                     const double *__restrict__ state, EosWant want,
-                                                                    double *__restrict__ rec) {
+                                                                    double *__restrict__ rec,
+                                                                    const int *__restrict__ active) {
   __shared__ ChemShared sh;
   extern __shared__ double region_s[];
   const int tid = threadIdx.x;
@@ -917,6 +921,7 @@ __global__ __launch_bounds__(kChemThreads) void eos_variants_kernel(const snapi_
   const long pt = t / nvar;
   const int v = (int)(t % nvar);
   const long c = pt / nd, zd = pt % nd;
+  if (active && !active[c]) return;
   const double *st = state + c * nf * nd + zd;
   const EosSeq q = eos_sequence(st[SNAPI_F_T * nd], st[SNAPI_F_NT * nd]);
   double Tv = q.T[0], Ntv = q.Nt[0];
@@ -1006,10 +1011,12 @@ __device__ void eos_background(const snapi_chem_table *__restrict__ tb, const Ch
 __global__ void eos_combine_kernel(const snapi_chem_table *__restrict__ tb, const ChemDerived *__restrict__ dv,
                                    const snapi_state_map *__restrict__ map, long npt, int nd, int nf, int nvar,
                                    EosWant want, const double *__restrict__ rec, double *__restrict__ state,
-                                   double *__restrict__ resp, double *__restrict__ dstate) {
+                                   double *__restrict__ resp, double *__restrict__ dstate,
+                                   const int *__restrict__ active) {
   const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (pt >= npt) return;
   const long c = pt / nd, zd = pt % nd;
+  if (active && !active[c]) return;
   double *st = state + c * nf * nd + zd;
   const double T0 = st[SNAPI_F_T * nd];
   const int nrec = 2 + want.nwant;
@@ -1205,12 +1212,13 @@ int launch_chemeq(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot
 }
 
 int launch_hequilibrium(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot, long ncol, int nd,
-                        const double *T, const double *logtau, double *Nt, int *it, hipStream_t st) {
+                        const double *T, const double *logtau, double *Nt, int *it, hipStream_t st,
+                        const int *active = nullptr, int group = 1) {
   long blocks;
   size_t lds;
   if (!team_grid(ncol, nslot, blocks, lds)) return SNAPI_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(hequilibrium_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, ncol, nd, T,
-                     logtau, Nt, it);
+                     logtau, Nt, it, active, group);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
@@ -1270,21 +1278,82 @@ int eos_want(const snapi_chem_table *tb, const snapi_state_map *map, EosWant &w)
 
 int launch_eos(const snapi_chem_table *dtb, const ChemDerived *ddv, const snapi_state_map *dmap, int nslot, long npt,
                int nd, int nf, int nvar, const EosWant &want, double *rec, double *state, double *resp,
-               double *dstate, hipStream_t st) {
+               double *dstate, hipStream_t st, const int *active = nullptr) {
   long blocks;
   size_t lds;
   if (!team_grid(npt * nvar, nslot, blocks, lds)) return SNAPI_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(eos_variants_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, npt, nd, nf,
-                     nvar, state, want, rec);
+                     nvar, state, want, rec, active);
   if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;
   const long b2 = (npt + 255) / 256;
   if (b2 > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(eos_combine_kernel, dim3((unsigned)b2), dim3(256), 0, st, dtb, ddv, dmap, npt, nd, nf, nvar, want,
-                     rec, state, resp, dstate);
+                     rec, state, resp, dstate, active);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
 }  // namespace
+
+namespace snapi {
+
+// Device copies of a species table (with its derived constants) and,
+// optionally, a state map, for callers that run the hydrostatic march and
+// the equation-of-state pass many times (the inversion).
+int chem_upload(const snapi_chem_table *tb, const snapi_state_map *map, ChemDevice &cd, hipStream_t st) {
+  cd = ChemDevice{};
+  if (!(tb->lambda_ref > 0.0)) return SNAPI_ERR_ARG;
+  ChemDerived d;
+  int rc = derive(tb, d);
+  if (rc) return rc;
+  cd.nslot = d.nslot;
+  if (map) {
+    EosWant w;
+    rc = eos_want(tb, map, w);
+    if (rc) return rc;
+    cd.nrec = 2 + w.nwant;
+    cd.nf = SNAPI_NF_BASE + 2 * map->nline;
+    static_assert(sizeof(EosWant) <= sizeof(cd.want), "want table fits");
+    memcpy(cd.want, &w, sizeof(w));
+    if (hipMalloc(&cd.map, sizeof(snapi_state_map)) != hipSuccess ||
+        hipMemcpyAsync(cd.map, map, sizeof(snapi_state_map), hipMemcpyHostToDevice, st) != hipSuccess) {
+      chem_free(cd);
+      return SNAPI_ERR_DEVICE;
+    }
+  }
+  rc = upload_table(tb, d, &cd.buf, st);
+  if (rc) chem_free(cd);
+  return rc;
+}
+
+void chem_free(ChemDevice &cd) {
+  hipFree(cd.buf);
+  hipFree(cd.map);
+  cd.buf = cd.map = nullptr;
+}
+
+size_t chem_eos_scratch(const ChemDevice &cd, long npt, bool derivatives) {
+  return (size_t)npt * (derivatives ? kEosVariants : 1) * cd.nrec * sizeof(double);
+}
+
+int chem_hequilibrium(const ChemDevice &cd, long ncol, int nd, const double *T, const double *logtau, double *Nt,
+                      const int *active, int group, hipStream_t st) {
+  return launch_hequilibrium((const snapi_chem_table *)cd.buf,
+                             (const ChemDerived *)((const char *)cd.buf + sizeof(snapi_chem_table)), cd.nslot, ncol,
+                             nd, T, logtau, Nt, nullptr, st, active, group);
+}
+
+int chem_eos(const ChemDevice &cd, long ncol, int nd, double *state, double *resp, double *dstate, double *scratch,
+             const int *active, hipStream_t st) {
+  EosWant w;
+  memcpy(&w, cd.want, sizeof(w));
+  const int nvar = (resp || dstate) ? kEosVariants : 1;
+  return launch_eos((const snapi_chem_table *)cd.buf,
+                    (const ChemDerived *)((const char *)cd.buf + sizeof(snapi_chem_table)),
+                    (const snapi_state_map *)cd.map, cd.nslot, ncol * nd, nd, cd.nf, nvar, w, scratch, state, resp,
+                    dstate, st, active);
+}
+
+}  // namespace snapi
 
 extern "C" int snapi_chemeq_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,
                                 const double *Nt_dev, double *ne_dev, double *stage_dev, double *molecule_dev,

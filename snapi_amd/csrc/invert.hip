@@ -33,6 +33,7 @@
 #include "resp_kernel.h"
 #include "rt_common.h"
 #include "synth_kernel.h"
+#include "chem_device.h"
 
 extern "C" int snapi_synth_stokes_dev(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol,
                                       int nd, const void *state_dev, int nlambda, const double *lam_dev,
@@ -418,6 +419,95 @@ __global__ void count_active_kernel(int nslot, const int *__restrict__ active, i
   if (slot < nslot && active[slot]) atomicAdd(count, 1);
 }
 
+// ---- temperature nodes: the thermodynamic state rebuilt from the nodes ----
+
+constexpr double kNodeStepT = 5.0;  // delta_T, the temperature-node step (cmlib/const.h:27)
+
+// Temperature profiles of one pixel for the node responses
+// (obs_stokes_responses_to_nodes, atmos_fit.cc:1225-1248): variant 0 the
+// nodes as given, then per temperature node j the node raised by delta_T/2
+// (variant 1+2j) and lowered from there by delta_T (2+2j).  Writes
+// Tv[pix][nvar][nd] and the log tau of each depth (from the state's grid)
+// for the hydrostatic march, ltv[pix][nvar][nd].
+template <typename R>
+__global__ void tprofiles_kernel(NodeArgs na, int npix, int nd, size_t state_stride, const R *__restrict__ state,
+                                 const double *__restrict__ nodes, int nvar, double *__restrict__ Tv,
+                                 double *__restrict__ ltv, const int *__restrict__ active) {
+  const int pix = blockIdx.x;
+  if (active && !active[pix]) return;
+  const double *v0 = nodes + (size_t)pix * na.nnode;
+  const R *st = state + (size_t)pix * state_stride;
+  double v[SNAPI_MAX_NODES];
+  for (int var = 0; var < nvar; ++var) {
+    for (int i = 0; i < na.nnode; ++i) v[i] = v0[i];
+    if (var > 0) {
+      int j = (var - 1) / 2, n = 0;
+      for (int i = 0; i < na.nnode; ++i)
+        if (na.code[i] == NP_T && j-- == 0) n = i;
+      v[n] += kNodeStepT * 0.5;
+      if ((var - 1) % 2) v[n] -= kNodeStepT;
+    }
+    for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+      const double x = log10(-(double)st[SNAPI_F_GRID * nd + z]);
+      Tv[((size_t)pix * nvar + var) * nd + z] = interp_temp(na, v, x);
+      ltv[((size_t)pix * nvar + var) * nd + z] = x;
+    }
+  }
+}
+
+// Node responses of the temperature nodes, dT(z)/dnode and dNt(z)/dnode by
+// the difference of the raised and lowered profiles over delta_T, into the
+// node maps' rows of those nodes; the double state gets the given nodes' T
+// and Nt and the other fields of state.
+template <typename R>
+__global__ void tmaps_kernel(NodeArgs na, int npix, int nd, int nf, int nvar, const double *__restrict__ Tv,
+                             const double *__restrict__ Ntv, const R *__restrict__ state, R *__restrict__ nmap,
+                             R *__restrict__ nmap2, double *__restrict__ state_d, const int *__restrict__ active) {
+  const int pix = blockIdx.x;
+  if (active && !active[pix]) return;
+  for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+    const double *T0 = Tv + (size_t)pix * nvar * nd, *N0 = Ntv + (size_t)pix * nvar * nd;
+    for (int f = 0; f < nf; ++f)
+      state_d[((size_t)pix * nf + f) * nd + z] = (double)state[((size_t)pix * nf + f) * nd + z];
+    state_d[((size_t)pix * nf + SNAPI_F_T) * nd + z] = T0[z];
+    state_d[((size_t)pix * nf + SNAPI_F_NT) * nd + z] = N0[z];
+    if (!nmap) continue;  // the state alone
+    int j = 0;
+    for (int n = 0; n < na.nnode; ++n) {
+      if (na.code[n] != NP_T) {
+        nmap2[((size_t)pix * na.nnode + n) * nd + z] = R(0);
+        continue;
+      }
+      const size_t a = (size_t)(1 + 2 * j) * nd + z, b = (size_t)(2 + 2 * j) * nd + z;
+      double dT = T0[a];
+      dT -= T0[b];
+      dT /= kNodeStepT;
+      double dN = N0[a];
+      dN -= N0[b];
+      dN /= kNodeStepT;
+      nmap[((size_t)pix * na.nnode + n) * nd + z] = (R)dT;
+      nmap2[((size_t)pix * na.nnode + n) * nd + z] = (R)dN;
+      ++j;
+    }
+  }
+}
+
+// The temperature and density columns of the given nodes into a double state.
+__global__ void set_tn_kernel(int nd, int nf, const double *__restrict__ Tv, const double *__restrict__ Ntv,
+                              double *__restrict__ state) {
+  const int pix = blockIdx.x;
+  for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+    state[((size_t)pix * nf + SNAPI_F_T) * nd + z] = Tv[(size_t)pix * nd + z];
+    state[((size_t)pix * nf + SNAPI_F_NT) * nd + z] = Ntv[(size_t)pix * nd + z];
+  }
+}
+
+template <typename A, typename B>
+__global__ void convert_kernel(size_t n, const A *__restrict__ a, B *__restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (B)a[i];
+}
+
 #define HIPCHK(x)                          \
   do {                                     \
     if ((x) != hipSuccess) {               \
@@ -426,12 +516,18 @@ __global__ void count_active_kernel(int nslot, const int *__restrict__ active, i
     }                                      \
   } while (0)
 
+// Species tables and state map of an inversion with temperature nodes.
+struct ThermoArgs {
+  const snapi_chem_table *chem, *chem_he;
+  const snapi_state_map *map;
+};
+
 template <typename R>
 static int invert_impl(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
                        const NodeArgs &na, int npix, int nd, const void *pstate_h, int nlambda, const double *lam_h,
                        const void *obs_h, const void *mask_h, const double *ws, double lm0, int max_iter, int nslot,
                        const double *nodes0, double *nodes_out, double *chi2_out, int *iter_out, long long *stats,
-                       bool dev, hipStream_t st) {
+                       bool dev, hipStream_t st, const ThermoArgs *th = nullptr) {
   KLines kl;
   SynthArgs a;
   int rc = prepare_synth(lines, ray, 1, nd, nlambda, kl, a);
@@ -445,6 +541,15 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   double *dlam = nullptr, *pnodes0 = nullptr, *snodes = nullptr, *tnodes = nullptr, *JTJ = nullptr, *rhs = nullptr;
   double *corr = nullptr, *onodes = nullptr, *ochi2 = nullptr, *fdn = nullptr, *hv = nullptr, *dbl = nullptr;
   int *ints = nullptr, *queue = nullptr, *oiter = nullptr, *count = nullptr;
+  // temperature nodes: profiles and densities of the rebuilds, double states
+  // for the equation of state, the response path's state and derivatives
+  ChemDevice cdev, cdev_he;
+  double *Tv = nullptr, *ltv = nullptr, *Ntv = nullptr, *sd = nullptr, *rsd = nullptr, *dsd = nullptr,
+         *tsd = nullptr, *escr = nullptr;
+  R *rstate = nullptr, *rdst = nullptr, *nmap2 = nullptr;
+  int ntn = 0;
+  for (int n = 0; n < nn; ++n) ntn += na.code[n] == NP_T;
+  const int nvar = 1 + 2 * ntn;
   // caller's buffers (device variant) or copies of the host arrays
   void *own[8] = {};
   SlotState ss;
@@ -506,6 +611,27 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   ss.chi2 = dbl + nslot;
   ss.chi2t = dbl + 2 * nslot;
   ss.track = dbl + 3 * nslot;
+  if (th) {
+    if (jacobian != SNAPI_JAC_ADJOINT || nf != SNAPI_NF_BASE + 2 * th->map->nline) {
+      rc = jacobian != SNAPI_JAC_ADJOINT ? SNAPI_ERR_UNSUPPORTED : SNAPI_ERR_ARG;
+      goto done;
+    }
+    rc = chem_upload(th->chem, th->map, cdev, st);
+    if (rc) goto done;
+    rc = chem_upload(th->chem_he, nullptr, cdev_he, st);
+    if (rc) goto done;
+    HIPCHK(hipMalloc((void **)&Tv, (size_t)nslot * nvar * nd * 8));
+    HIPCHK(hipMalloc((void **)&ltv, (size_t)nslot * nvar * nd * 8));
+    HIPCHK(hipMalloc((void **)&Ntv, (size_t)nslot * nvar * nd * 8));
+    HIPCHK(hipMalloc((void **)&sd, (size_t)nslot * ns * 8));
+    HIPCHK(hipMalloc((void **)&rsd, (size_t)nslot * ns * 8));
+    HIPCHK(hipMalloc((void **)&dsd, (size_t)nslot * 2 * ns * 8));
+    HIPCHK(hipMalloc((void **)&tsd, (size_t)nslot * ns * 8));
+    HIPCHK(hipMalloc((void **)&escr, chem_eos_scratch(cdev, (long)nslot * nd, true)));
+    HIPCHK(hipMalloc((void **)&rstate, (size_t)nslot * ns * es));
+    HIPCHK(hipMalloc((void **)&rdst, (size_t)nslot * 2 * ns * es));
+    HIPCHK(hipMalloc((void **)&nmap2, (size_t)nslot * nn * nd * es));
+  }
   if (jacobian == SNAPI_JAC_ADJOINT) {
     per_col = resp_scratch_per_col(nl, nd, es);
     size_t c = ((size_t)16 << 30) / (per_col ? per_col : 1);
@@ -543,11 +669,14 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     for (int s = 0; s < 4; ++s) ra.ws[s] = ws[s];
     ra.nnode = nn;
     for (int n = 0; n < nn; ++n) {
-      // node parameter -> per-depth response index (vt 2, vLOS 3, |B| 4, incl 5, azi 6)
-      ra.pidx[n] = na.code[n] == NP_VT ? 2 : na.code[n] == NP_VS ? 3 : na.code[n] == NP_B ? 4 : na.code[n] == NP_THETA ? 5 : 6;
+      // node parameter -> per-depth response index (T 0 and Nt 1, vt 2, vLOS 3, |B| 4, incl 5, azi 6)
+      const int c = na.code[n];
+      ra.pidx[n] = c == NP_T ? 0 : c == NP_VT ? 2 : c == NP_VS ? 3 : c == NP_B ? 4 : c == NP_THETA ? 5 : 6;
+      ra.pidx2[n] = c == NP_T ? 1 : -1;
       ra.nscale[n] = na.scale[n];
     }
     ra.nmap = nmap;
+    ra.nmap2 = th ? nmap2 : nullptr;
     ra.mask = smask;
     ra.active = ss.needj;
     long long pixel_iters = 0;
@@ -565,7 +694,24 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
       // model -> state and Jacobian where the model changed
       hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, snodes,
                          (size_t)nn, sstate, ns, nmap, ss.needj);
-      if (jacobian == SNAPI_JAC_ADJOINT) {
+      if (th) {
+        // build_from_nodes at the nodes and at each temperature node +- 2.5 K,
+        // then the response path's state and derivatives (the reference's
+        // Jacobian and the current chi^2 come from that path)
+        hipLaunchKernelGGL(tprofiles_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ns, sstate, snodes, nvar,
+                           Tv, ltv, ss.needj);
+        rc = chem_hequilibrium(cdev_he, (long)nslot * nvar, nd, Tv, ltv, Ntv, ss.needj, nvar, st);
+        if (rc) goto done;
+        hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, nvar, Tv, Ntv, sstate,
+                           nmap, nmap2, sd, ss.needj);
+        rc = chem_eos(cdev, nslot, nd, sd, rsd, dsd, escr, ss.needj, st);
+        if (rc) goto done;
+        hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, rsd, rstate);
+        hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * 2 * ns, dsd,
+                           rdst);
+        rc = launch_resp<R>(kl, a, ra, nslot, rstate, rdst, dlam, S, J, scr, chunk, st);
+        if (rc) goto done;
+      } else if (jacobian == SNAPI_JAC_ADJOINT) {
         rc = launch_resp<R>(kl, a, ra, nslot, sstate, dzero, dlam, S, J, scr, chunk, st);
         if (rc) goto done;
       } else if (jacobian == SNAPI_JAC_TANGENT) {
@@ -599,6 +745,19 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
                          ss.active);
       hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, tnodes,
                          (size_t)nn, tstate, ns, (R *)nullptr, ss.active);
+      if (th) {
+        // the trial's thermodynamics: temperature, hydrostatic equilibrium,
+        // the synthesis path's equation of state (forward_evaluate)
+        hipLaunchKernelGGL(tprofiles_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ns, tstate, tnodes, 1,
+                           Tv, ltv, ss.active);
+        rc = chem_hequilibrium(cdev_he, nslot, nd, Tv, ltv, Ntv, ss.active, 1, st);
+        if (rc) goto done;
+        hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, 1, Tv, Ntv, tstate,
+                           (R *)nullptr, (R *)nullptr, tsd, ss.active);
+        rc = chem_eos(cdev, nslot, nd, tsd, nullptr, nullptr, escr, ss.active, st);
+        if (rc) goto done;
+        hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, tsd, tstate);
+      }
       rc = synth_dev_masked(precision, lines, ray, nslot, nd, tstate, nl, dlam, St, ss.active, 1, st);
       if (rc) goto done;
       hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, St, sobs, smask, ws[0], ws[1],
@@ -629,6 +788,9 @@ done:
   hipFree(fdstate); hipFree(Sall); hipFree(scr); hipFree(snodes);
   hipFree(tnodes); hipFree(JTJ); hipFree(rhs); hipFree(corr); hipFree(fdn);
   hipFree(hv); hipFree(dbl); hipFree(ints); hipFree(queue);
+  chem_free(cdev); chem_free(cdev_he);
+  hipFree(Tv); hipFree(ltv); hipFree(Ntv); hipFree(sd); hipFree(rsd); hipFree(dsd); hipFree(tsd); hipFree(escr);
+  hipFree(rstate); hipFree(rdst); hipFree(nmap2);
   return rc;
 }
 
@@ -718,82 +880,6 @@ done:
   return rc;
 }
 
-// ---- temperature nodes: the thermodynamic state rebuilt from the nodes ----
-
-constexpr double kNodeStepT = 5.0;  // delta_T, the temperature-node step (cmlib/const.h:27)
-
-// Temperature profiles of one pixel for the node responses
-// (obs_stokes_responses_to_nodes, atmos_fit.cc:1225-1248): variant 0 the
-// nodes as given, then per temperature node j the node raised by delta_T/2
-// (variant 1+2j) and lowered from there by delta_T (2+2j).  Writes
-// Tv[pix][nvar][nd] and the log tau of each depth (from the state's grid)
-// for the hydrostatic march, ltv[pix][nvar][nd].
-template <typename R>
-__global__ void tprofiles_kernel(NodeArgs na, int npix, int nd, size_t state_stride, const R *__restrict__ state,
-                                 const double *__restrict__ nodes, int nvar, double *__restrict__ Tv,
-                                 double *__restrict__ ltv) {
-  const int pix = blockIdx.x;
-  const double *v0 = nodes + (size_t)pix * na.nnode;
-  const R *st = state + (size_t)pix * state_stride;
-  double v[SNAPI_MAX_NODES];
-  for (int var = 0; var < nvar; ++var) {
-    for (int i = 0; i < na.nnode; ++i) v[i] = v0[i];
-    if (var > 0) {
-      int j = (var - 1) / 2, n = 0;
-      for (int i = 0; i < na.nnode; ++i)
-        if (na.code[i] == NP_T && j-- == 0) n = i;
-      v[n] += kNodeStepT * 0.5;
-      if ((var - 1) % 2) v[n] -= kNodeStepT;
-    }
-    for (int z = threadIdx.x; z < nd; z += blockDim.x) {
-      const double x = log10(-(double)st[SNAPI_F_GRID * nd + z]);
-      Tv[((size_t)pix * nvar + var) * nd + z] = interp_temp(na, v, x);
-      ltv[((size_t)pix * nvar + var) * nd + z] = x;
-    }
-  }
-}
-
-// Node responses of the temperature nodes, dT(z)/dnode and dNt(z)/dnode by
-// the difference of the raised and lowered profiles over delta_T, into the
-// node maps' rows of those nodes; the double state gets the given nodes' T
-// and Nt and the other fields of state.
-template <typename R>
-__global__ void tmaps_kernel(NodeArgs na, int npix, int nd, int nf, int nvar, const double *__restrict__ Tv,
-                             const double *__restrict__ Ntv, const R *__restrict__ state, R *__restrict__ nmap,
-                             R *__restrict__ nmap2, double *__restrict__ state_d) {
-  const int pix = blockIdx.x;
-  for (int z = threadIdx.x; z < nd; z += blockDim.x) {
-    const double *T0 = Tv + (size_t)pix * nvar * nd, *N0 = Ntv + (size_t)pix * nvar * nd;
-    for (int f = 0; f < nf; ++f)
-      state_d[((size_t)pix * nf + f) * nd + z] = (double)state[((size_t)pix * nf + f) * nd + z];
-    state_d[((size_t)pix * nf + SNAPI_F_T) * nd + z] = T0[z];
-    state_d[((size_t)pix * nf + SNAPI_F_NT) * nd + z] = N0[z];
-    int j = 0;
-    for (int n = 0; n < na.nnode; ++n) {
-      if (na.code[n] != NP_T) {
-        nmap2[((size_t)pix * na.nnode + n) * nd + z] = R(0);
-        continue;
-      }
-      const size_t a = (size_t)(1 + 2 * j) * nd + z, b = (size_t)(2 + 2 * j) * nd + z;
-      double dT = T0[a];
-      dT -= T0[b];
-      dT /= kNodeStepT;
-      double dN = N0[a];
-      dN -= N0[b];
-      dN /= kNodeStepT;
-      nmap[((size_t)pix * na.nnode + n) * nd + z] = (R)dT;
-      nmap2[((size_t)pix * na.nnode + n) * nd + z] = (R)dN;
-      ++j;
-    }
-  }
-}
-
-template <typename A, typename B>
-__global__ void convert_kernel(size_t n, const A *__restrict__ a, B *__restrict__ b) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    b[i] = (B)a[i];
-}
-
 // obs_stokes_responses_to_nodes (atmos_fit.cc:1205-1352) with temperature
 // nodes: build_from_nodes (temperature interpolation, hydrostatic
 // equilibrium on the state's tau grid, atmos_pp.cc:58-180, 372-455), the
@@ -844,7 +930,7 @@ static int node_jacobian_eos_impl(int precision, const snapi_line_table *lines, 
   hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, nf, dnodes, (size_t)nn,
                      dstate, ns, nmap, (const int *)nullptr);
   hipLaunchKernelGGL(tprofiles_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, ns, dstate, dnodes, nvar, Tv,
-                     ltv);
+                     ltv, (const int *)nullptr);
   if (hipGetLastError() != hipSuccess) {
     rc = SNAPI_ERR_DEVICE;
     goto done;
@@ -853,7 +939,7 @@ static int node_jacobian_eos_impl(int precision, const snapi_line_table *lines, 
                               nullptr, nullptr, nullptr);
   if (rc) goto done;
   hipLaunchKernelGGL(tmaps_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, nf, nvar, Tv, Ntv, dstate, nmap,
-                     nmap2, sd);
+                     nmap2, sd, (const int *)nullptr);
   rc = snapi_eos_state_dev(chem, map, npix, nd, sd, rsd, dsd, nullptr);
   if (rc) goto done;
   hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, 0, (size_t)npix * ns, rsd, rstate);
@@ -998,4 +1084,132 @@ extern "C" int snapi_node_jacobian_eos(int precision, int jacobian, const snapi_
     return node_jacobian_eos_impl<float>(precision, lines, ray, na, chem, chem_he, map, npix, nd, state, nodes,
                                          nlambda, lam, stokes, J);
   return SNAPI_ERR_ARG;
+}
+
+extern "C" int snapi_invert_eos_dev(int precision, int jacobian, const snapi_line_table *lines,
+                                    const snapi_ray *ray, const snapi_node_model *model,
+                                    const snapi_chem_table *chem, const snapi_chem_table *chem_he,
+                                    const snapi_state_map *map, int npix, int nd, const void *state_dev, int nlambda,
+                                    const double *lam_dev, const void *obs_dev, const void *mask_dev,
+                                    const double *wstokes, double lm_start, int max_iter, int nslot,
+                                    const double *nodes0_dev, double *nodes_dev, double *chi2_dev,
+                                    int *iterations_dev, long long *pixel_iterations, void *stream) {
+  if (!state_dev || !lam_dev || !obs_dev || !wstokes || !nodes0_dev || !nodes_dev || !chi2_dev || !iterations_dev ||
+      !chem || !chem_he || !map)
+    return SNAPI_ERR_ARG;
+  NodeArgs na;
+  int rc = setup_nodes(lines, ray, model, npix, nd, nlambda, jacobian, max_iter, na, true);
+  if (rc) return rc;
+  int ntn = 0;
+  for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
+  if (ntn < 1) return SNAPI_ERR_ARG;
+  if (nslot <= 0 || nslot > npix) nslot = npix;
+  const ThermoArgs th{chem, chem_he, map};
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == SNAPI_FP64)
+    return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
+                               mask_dev, wstokes, lm_start, max_iter, nslot, nodes0_dev, nodes_dev, chi2_dev,
+                               iterations_dev, pixel_iterations, true, st, &th);
+  if (precision == SNAPI_MIXED)
+    return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
+                              mask_dev, wstokes, lm_start, max_iter, nslot, nodes0_dev, nodes_dev, chi2_dev,
+                              iterations_dev, pixel_iterations, true, st, &th);
+  return SNAPI_ERR_ARG;
+}
+
+extern "C" int snapi_invert_eos(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                                const snapi_node_model *model, const snapi_chem_table *chem,
+                                const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
+                                const void *state, int nlambda, const double *lam, const void *obs, const void *mask,
+                                const double *wstokes, double lm_start, int max_iter, int nslot, double *nodes,
+                                double *chi2, int *iterations, long long *pixel_iterations) {
+  if (!state || !lam || !obs || !wstokes || !nodes || !chi2 || !iterations || !chem || !chem_he || !map)
+    return SNAPI_ERR_ARG;
+  NodeArgs na;
+  int rc = setup_nodes(lines, ray, model, npix, nd, nlambda, jacobian, max_iter, na, true);
+  if (rc) return rc;
+  int ntn = 0;
+  for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
+  if (ntn < 1) return SNAPI_ERR_ARG;
+  if (nslot <= 0 || nslot > npix) nslot = npix;
+  const ThermoArgs th{chem, chem_he, map};
+  if (precision == SNAPI_FP64)
+    return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
+                               lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0,
+                               &th);
+  if (precision == SNAPI_MIXED)
+    return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
+                              lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0,
+                              &th);
+  return SNAPI_ERR_ARG;
+}
+
+// build_from_nodes (atmos_pp.cc:58-180) with popsetup and compute_op_referent:
+// the node-controlled fields, the temperature, hydrostatic equilibrium and
+// the synthesis path's equation-of-state fields of each pixel's state.
+extern "C" int snapi_build_from_nodes_dev(const snapi_node_model *model, const snapi_chem_table *chem,
+                                          const snapi_chem_table *chem_he, const snapi_state_map *map, int npix,
+                                          int nd, double *state_dev, const double *nodes_dev, void *stream) {
+  if (!model || !chem || !chem_he || !map || !state_dev || !nodes_dev || npix <= 0 || nd < 3 ||
+      nd > SNAPI_MAX_ND || model->nnode <= 0 || model->nnode > SNAPI_MAX_NODES - 1)
+    return SNAPI_ERR_ARG;
+  NodeArgs na{};
+  na.nnode = model->nnode;
+  int ntn = 0;
+  for (int n = 0; n < na.nnode; ++n) {
+    const int c = model->param[n];
+    if (c < NP_T || c > NP_PHI) return SNAPI_ERR_UNSUPPORTED;
+    na.code[n] = c;
+    na.ltau[n] = model->logtau[n];
+    ntn += c == NP_T;
+  }
+  if (ntn < 1) return SNAPI_ERR_ARG;
+  const int nf = SNAPI_NF_BASE + 2 * map->nline;
+  const size_t ns = (size_t)nf * nd;
+  hipStream_t st = (hipStream_t)stream;
+  ChemDevice cdev, cdev_he;
+  double *Tv = nullptr, *ltv = nullptr, *Ntv = nullptr, *escr = nullptr;
+  int rc = chem_upload(chem, map, cdev, st);
+  if (!rc) rc = chem_upload(chem_he, nullptr, cdev_he, st);
+  if (!rc && (hipMalloc((void **)&Tv, (size_t)npix * nd * 8) != hipSuccess ||
+              hipMalloc((void **)&ltv, (size_t)npix * nd * 8) != hipSuccess ||
+              hipMalloc((void **)&Ntv, (size_t)npix * nd * 8) != hipSuccess ||
+              hipMalloc((void **)&escr, chem_eos_scratch(cdev, (long)npix * nd, false)) != hipSuccess))
+    rc = SNAPI_ERR_DEVICE;
+  if (!rc) {
+    hipLaunchKernelGGL(nodes_to_state_kernel<double>, dim3(npix), dim3(128), 0, st, na, npix, nd, nf, nodes_dev,
+                       (size_t)na.nnode, state_dev, ns, (double *)nullptr, (const int *)nullptr);
+    hipLaunchKernelGGL(tprofiles_kernel<double>, dim3(npix), dim3(128), 0, st, na, npix, nd, ns, state_dev,
+                       nodes_dev, 1, Tv, ltv, (const int *)nullptr);
+    rc = hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+  }
+  if (!rc) rc = chem_hequilibrium(cdev_he, npix, nd, Tv, ltv, Ntv, nullptr, 1, st);
+  if (!rc) {
+    hipLaunchKernelGGL(set_tn_kernel, dim3(npix), dim3(128), 0, st, nd, nf, Tv, Ntv, state_dev);
+    rc = hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+  }
+  if (!rc) rc = chem_eos(cdev, npix, nd, state_dev, nullptr, nullptr, escr, nullptr, st);
+  if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = SNAPI_ERR_DEVICE;
+  chem_free(cdev);
+  chem_free(cdev_he);
+  hipFree(Tv); hipFree(ltv); hipFree(Ntv); hipFree(escr);
+  return rc;
+}
+
+extern "C" int snapi_build_from_nodes(const snapi_node_model *model, const snapi_chem_table *chem,
+                                      const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
+                                      double *state, const double *nodes) {
+  if (!model || !map || !state || !nodes || npix <= 0 || nd <= 0) return SNAPI_ERR_ARG;
+  const size_t n = (size_t)npix * (SNAPI_NF_BASE + 2 * map->nline) * nd;
+  double *ds = nullptr, *dn = nullptr;
+  int rc = SNAPI_OK;
+  if (hipMalloc((void **)&ds, n * 8) != hipSuccess || hipMalloc((void **)&dn, (size_t)npix * model->nnode * 8) != hipSuccess ||
+      hipMemcpy(ds, state, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dn, nodes, (size_t)npix * model->nnode * 8, hipMemcpyHostToDevice) != hipSuccess)
+    rc = SNAPI_ERR_DEVICE;
+  if (!rc) rc = snapi_build_from_nodes_dev(model, chem, chem_he, map, npix, nd, ds, dn, nullptr);
+  if (!rc && hipMemcpy(state, ds, n * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = SNAPI_ERR_DEVICE;
+  hipFree(ds);
+  hipFree(dn);
+  return rc;
 }

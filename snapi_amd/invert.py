@@ -144,3 +144,89 @@ def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_
            C.byref(smap), npix, nd, abi.vptr(st), abi.ptr(nv), len(lam), abi.ptr(lam), abi.vptr(stokes), abi.vptr(J))
     check("snapi_node_jacobian_eos", rc)
     return stokes, J
+
+
+def invert_eos(table, ray, model, chem, chem_he, smap, state, lam_vac, obs, nodes0, mask=None,
+               wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0, precision=abi.SNAPI_FP64):
+    """snapi_invert_eos: the inversion with temperature nodes (adjoint
+    Jacobian). Returns (nodes, chi2, iterations, total pixel-iterations)."""
+    dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
+    st = np.ascontiguousarray(state, dtype=dt)
+    npix, nf, nd = st.shape
+    lam = np.ascontiguousarray(lam_vac, dtype=np.float64)
+    ob = np.ascontiguousarray(obs, dtype=dt).reshape(npix, len(lam), 4)
+    mk = None if mask is None else np.ascontiguousarray(mask, dtype=dt).reshape(npix, len(lam))
+    nodes = np.ascontiguousarray(nodes0, dtype=np.float64).reshape(npix, model.nnode).copy()
+    chi2 = np.zeros(npix)
+    iters = np.zeros(npix, dtype=np.int32)
+    total = C.c_longlong(0)
+    ws = np.ascontiguousarray(wstokes, dtype=np.float64)
+    f = lib().snapi_invert_eos
+    f.restype = C.c_int
+    TB = C.POINTER(abi.SnapiChemTable)
+    f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay), C.POINTER(SnapiNodeModel),
+                  TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_int, C.c_int,
+                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+    rc = f(precision, 0, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
+           npix, nd, abi.vptr(st), len(lam), abi.ptr(lam), abi.vptr(ob), None if mk is None else abi.vptr(mk),
+           abi.ptr(ws), lm_start, max_iter, nslot, abi.ptr(nodes), abi.ptr(chi2),
+           iters.ctypes.data_as(C.POINTER(C.c_int)), C.byref(total))
+    check("snapi_invert_eos", rc)
+    return nodes, chi2, iters, total.value
+
+
+def build_from_nodes(model, chem, chem_he, smap, state, nodes):
+    """snapi_build_from_nodes: state [npix][nfield][nd] (double, GRID set)
+    with the node fields, T, NT and the equation-of-state fields filled."""
+    st = np.array(state, dtype=np.float64, order="C", copy=True)
+    if st.ndim == 2:
+        st = st[None]
+    npix, nf, nd = st.shape
+    nv = np.ascontiguousarray(nodes, dtype=np.float64).reshape(npix, model.nnode)
+    f = lib().snapi_build_from_nodes
+    f.restype = C.c_int
+    TB = C.POINTER(abi.SnapiChemTable)
+    f.argtypes = [C.POINTER(SnapiNodeModel), TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int,
+                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    check("snapi_build_from_nodes", f(C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap), npix, nd,
+                                      abi.ptr(st), abi.ptr(nv)))
+    return st
+
+
+def build_from_nodes_dev(model, chem, chem_he, smap, state_t, nodes_t, stream=None):
+    """snapi_build_from_nodes_dev on float64 device tensors (state in place)."""
+    npix, nf, nd = state_t.shape
+    f = lib().snapi_build_from_nodes_dev
+    f.restype = C.c_int
+    TB = C.POINTER(abi.SnapiChemTable)
+    f.argtypes = [C.POINTER(SnapiNodeModel), TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p,
+                  C.c_void_p, C.c_void_p]
+    sp = None if stream is None else C.c_void_p(stream.cuda_stream)
+    check("snapi_build_from_nodes_dev", f(C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap), npix, nd,
+                                          state_t.data_ptr(), nodes_t.data_ptr(), sp))
+
+
+def invert_eos_dev(table, ray, model, chem, chem_he, smap, state_t, lam_t, obs_t, mask_t, nodes0_t, nodes_t, chi2_t,
+                   iters_t, wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0,
+                   precision=abi.SNAPI_MIXED, stream=None):
+    """snapi_invert_eos_dev on torch device tensors; returns the total
+    pixel-iterations."""
+    npix, nf, nd = state_t.shape
+    nl = lam_t.shape[0]
+    ws = np.ascontiguousarray(wstokes, dtype=np.float64)
+    total = C.c_longlong(0)
+    f = lib().snapi_invert_eos_dev
+    f.restype = C.c_int
+    TB = C.POINTER(abi.SnapiChemTable)
+    f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay), C.POINTER(SnapiNodeModel),
+                  TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_int, C.c_int, C.c_void_p,
+                  C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]
+    sp = None if stream is None else C.c_void_p(stream.cuda_stream)
+    rc = f(precision, 0, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
+           npix, nd, state_t.data_ptr(), nl, lam_t.data_ptr(), obs_t.data_ptr(),
+           None if mask_t is None else mask_t.data_ptr(), abi.ptr(ws), lm_start, max_iter, nslot,
+           nodes0_t.data_ptr(), nodes_t.data_ptr(), chi2_t.data_ptr(), iters_t.data_ptr(), C.byref(total), sp)
+    check("snapi_invert_eos_dev", rc)
+    return total.value

@@ -186,6 +186,36 @@ def inversion_problem(template, nx, ny, seed=3):
     return dict(state=st, truth=truth, start=start, mask=mask, nodes=INVERSION_NODES)
 
 
+# the deepest temperature node at log tau 0.5, where the cube's columns stay
+# below the reference's 12000 K bracket (atmos_model.cc:12-19)
+TNODE_INVERSION_NODES = [("T", -5.0), ("T", -4.0), ("T", -3.0), ("T", -2.0), ("T", -1.0), ("T", 0.0), ("T", 0.5)] \
+    + INVERSION_NODES
+
+
+def inversion_problem_tnodes(template, nx, ny, seed=3):
+    """The inversion workload with the full 14-node model (7 temperature
+    nodes at log tau -5..0.5, then the 7 nodes of inversion_problem): truth
+    temperatures are the cube columns' temperatures at the node positions,
+    the other nodes as inversion_problem draws them; starting values
+    perturbed by up to +/-20 % (temperatures included); the same wavelength
+    masks.  The pixels' states are built from the truth nodes by the caller
+    (snapi_build_from_nodes).  Returns dict(base [npix][nfield][nd], truth
+    [npix][14], start, mask, nodes)."""
+    p = inversion_problem(template, nx, ny, seed=seed)
+    st = p["state"]
+    npix = st.shape[0]
+    logtau = template["logtau"]
+    rng = np.random.default_rng(seed + 2000)
+    tt = np.empty((npix, 7))
+    for k, (_, lt) in enumerate(TNODE_INVERSION_NODES[:7]):
+        z = int(np.argmin(np.abs(logtau - lt)))
+        tt[:, k] = st[:, abi.F_T, z]
+    truth = np.concatenate([tt, p["truth"]], axis=1)
+    start = np.concatenate([tt * rng.uniform(0.8, 1.2, tt.shape), p["start"]], axis=1)
+    start[:, :7] = np.clip(start[:, :7], 3400.0, 12000.0)
+    return dict(base=st, truth=truth, start=start, mask=p["mask"], nodes=TNODE_INVERSION_NODES)
+
+
 def ali_template():
     """The NLTE column recorded from the reference (tests/golden/nlte.npz):
     a Ca II 5-level + continuum atom (the reference's calcium_ir_rh.cfg) in

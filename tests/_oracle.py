@@ -349,6 +349,76 @@ def node_jacobian_eos(table, ray, model_nodes, values, state, lam, chem, chem_he
     return S[0], J
 
 
+def forward_state_eos(model_nodes, values, state, chem, chem_he, smap):
+    """The synthesis path's state of a pixel built from nodes with
+    temperature nodes: build_from_nodes (interpolation, hydrostatic
+    equilibrium) then popsetup and compute_op_referent."""
+    from snapi_amd import abi
+    st = nodes_to_state(model_nodes, values, state)
+    x = np.log10(-st[abi.F_GRID])
+    tn = [i for i, (p, _) in enumerate(model_nodes) if p == "T"]
+    st[abi.F_T] = _interp_temp(np.array([model_nodes[i][1] for i in tn]), np.array([values[i] for i in tn]), x)
+    Nt, _, _, _, _ = hequilibrium(chem_he, st[abi.F_T], x)
+    st[abi.F_NT] = Nt[0]
+    out, _, _ = eos_state(chem, smap, st[None])
+    return out[0]
+
+
+_SCALE = {"T": 5000.0, "vt": 1e5, "vlos": 1e5, "B": 1000.0, "theta": 1.0, "phi": 1.0}
+_BRACKET = {"T": (3400.0, 12000.0), "vt": (1e2, 15e5), "vlos": (-15e5, 15e5), "B": (1.0, 1e4)}
+
+
+def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, smap, wstokes, max_iter=30,
+               lm_start=1.0):
+    """stokes_lm_fit (atmos_fit.cc:18-227) of one pixel with temperature
+    nodes, in numpy over the oracle's chain: the Jacobian and the current
+    chi^2 from the response path (node_jacobian_eos), the trial's chi^2 from
+    the synthesis path, responses scaled by the parameter scales and Stokes
+    weights (scale_rf), (1 + lambda) damping and the truncated-SVD solve,
+    corrections scaled back and bracketed, lambda / sqrt(10) (floor 1e-5) on
+    success and * sqrt(10) on failure, stop after three accepted steps whose
+    chi^2 changes by less than 1e-3 or at max_iter.  Returns (nodes, chi2,
+    iterations)."""
+    ws = np.asarray(wstokes, dtype=np.float64)
+    sc = np.array([_SCALE[p] for p, _ in model_nodes])
+    v = np.array(start, dtype=np.float64)
+    lm = lm_start
+    need, track, chi = True, [], 0.0
+    it = 0
+
+    def forward(vals):
+        st = forward_state_eos(model_nodes, vals, state, chem, chem_he, smap)
+        s, _ = synth(table, ray, st[None], lam)
+        return s[0]
+
+    while it < max_iter:
+        if need:
+            S, J = node_jacobian_eos(table, ray, model_nodes, v, state, lam, chem, chem_he, smap)
+            r = ((obs - S) * ws).reshape(-1)
+            chi = float((r ** 2).sum())
+            Js = J * sc[None, :] * np.tile(ws, len(lam))[:, None]
+        _, _, corr = lm_step(Js, r, lm)
+        vt = v + corr * sc
+        for i, (p, _) in enumerate(model_nodes):
+            if p in _BRACKET:
+                vt[i] = min(max(vt[i], _BRACKET[p][0]), _BRACKET[p][1])
+        chit = float((((forward(vt) - obs) * ws) ** 2).sum())
+        it += 1
+        stop = False
+        if chit < chi:
+            v = vt
+            lm = max(lm / np.sqrt(10.0), 1e-5)
+            need = True
+            track = (track + [chi])[-3:]
+            stop = len(track) == 3 and abs((track[1] - track[2]) / track[2]) < 1e-3
+        else:
+            lm *= np.sqrt(10.0)
+            need = False
+        if stop:
+            break
+    return v, chi, it
+
+
 def _ali_lib():
     from snapi_amd import abi
     L = lib()

@@ -1,0 +1,125 @@
+"""Batched LM inversion with temperature nodes on the GPU
+(snapi_invert_eos): the 14-node model (7 temperature nodes), synthetic
+observations from known nodes (the oracle's build_from_nodes, hydrostatic
+equilibrium and equation of state, then the GPU synthesis), inverted from
+perturbed starts with the adjoint Jacobian in FP64 and mixed precision.
+
+The reference's fit takes its Jacobian and the current chi^2 from the
+response path (T raised by 2.5 K, thermodynamic derivatives by differences)
+and the trial's chi^2 from the synthesis path.  With seven temperature nodes,
+two of them above the lines' formation region, that fit reduces chi^2 by one
+to two orders of magnitude in 30 iterations and then creeps (the oracle's
+numpy restatement of the same fit shows the same behaviour), so the
+inversion is checked against that restatement: on two pixels and ten
+iterations the GPU's chi^2, iteration count and nodes follow the oracle's
+(chi^2 to 5 %, the temperature nodes where the lines form to 5 K; the GPU's
+Jacobian is the exact derivative through the solver where the oracle
+differences it, which moves the trajectory slightly); on twelve pixels and
+30 iterations chi^2 on the synthesis path falls by 10x at the median (the
+fit's own pace from 150 K starting errors on every temperature node; its
+acceptance test compares the trial with the response path's chi^2, so a
+pixel's synthesis-path chi^2 is not bound to fall at every step)."""
+import numpy as np
+import pytest
+
+import _oracle
+from snapi_amd import abi, api, invert, workloads
+
+pytestmark = pytest.mark.gpu
+
+# the deepest temperature node at log tau 0.5, where the columns stay below
+# the fit's 12000 K bracket
+NODES = [("T", -5.0), ("T", -4.0), ("T", -3.0), ("T", -2.0), ("T", -1.0), ("T", 0.0), ("T", 0.5),
+         ("vt", -2.0), ("vlos", -4.0), ("vlos", 0.0), ("B", -4.0), ("B", 0.0), ("theta", -2.0), ("phi", -2.0)]
+
+
+@pytest.fixture(scope="module")
+def problem():
+    t = workloads.template_cfg1()
+    chem, chem_he, smap = workloads.eos_tables("cfg1_template", t["ray"].lam_mid)
+    rng = np.random.default_rng(17)
+    npix = 12
+    lt = np.log10(-t["state"][abi.F_GRID])
+    Tn = np.interp([n[1] for n in NODES[:7]], lt, t["state"][abi.F_T])
+    truth = np.zeros((npix, len(NODES)))
+    truth[:, :7] = Tn[None] + rng.normal(0.0, 80.0, (npix, 7))
+    truth[:, 7] = rng.uniform(0.3e5, 1.8e5, npix)
+    truth[:, 8] = rng.normal(0, 1.5e5, npix)
+    truth[:, 9] = truth[:, 8] + rng.normal(0, 0.5e5, npix)
+    truth[:, 10] = rng.uniform(200, 2500, npix)
+    truth[:, 11] = truth[:, 10] * rng.uniform(0.7, 1.3, npix)
+    truth[:, 12] = np.arccos(rng.uniform(-0.9, 0.9, npix))
+    truth[:, 13] = rng.uniform(0.1, 1.4, npix)
+    lam = t["lam_vac"][::2]
+    states = np.stack([_oracle.forward_state_eos(NODES, truth[p], t["state"], chem, chem_he, smap)
+                       for p in range(npix)])
+    obs = api.synth_stokes(t["table"], t["ray"], states, lam, abi.SNAPI_FP64)
+    start = truth.copy()
+    start[:, :7] += rng.normal(0.0, 150.0, (npix, 7))
+    start[:, 7:] *= rng.uniform(0.8, 1.2, (npix, 7))
+    return dict(t=t, truth=truth, states=states, lam=lam, obs=obs, start=start, chem=chem, chem_he=chem_he,
+                smap=smap)
+
+
+@pytest.mark.parametrize("precision", [abi.SNAPI_FP64, abi.SNAPI_MIXED])
+def test_inversion_with_temperature_nodes(problem, precision):
+    p = problem
+    t = p["t"]
+    print("inversion with temperature nodes: %d pixels, %d nodes, %d wavelengths, precision %d" % (
+        len(p["truth"]), len(NODES), len(p["lam"]), precision), flush=True)
+    model = invert.node_model(NODES)
+    ws = np.array([1.0, 5.0, 5.0, 3.0])
+    st0 = np.stack([_oracle.forward_state_eos(NODES, p["start"][i], t["state"], p["chem"], p["chem_he"], p["smap"])
+                    for i in range(len(p["start"]))])
+    s0 = api.synth_stokes(t["table"], t["ray"], st0, p["lam"], abi.SNAPI_FP64)
+    chi0 = (((s0 - p["obs"]) * ws) ** 2).sum(axis=(1, 2))
+    nodes, chi2, iters, total = invert.invert_eos(t["table"], t["ray"], model, p["chem"], p["chem_he"], p["smap"],
+                                                  p["states"], p["lam"], p["obs"], p["start"], wstokes=ws,
+                                                  max_iter=30, nslot=5, precision=precision)
+    dT = nodes[:, :7] - p["truth"][:, :7]
+    # the fit reports the response path's chi^2; both ends on the synthesis path
+    st1 = invert.build_from_nodes(model, p["chem"], p["chem_he"], p["smap"], np.repeat(t["state"][None], 12, axis=0),
+                                  nodes)
+    s1 = api.synth_stokes(t["table"], t["ray"], st1, p["lam"], abi.SNAPI_FP64)
+    chi1 = (((s1 - p["obs"]) * ws) ** 2).sum(axis=(1, 2))
+    print("  chi2 start median %.3e, end median %.3e (reported %.3e); iterations %s; total %d" % (
+        np.median(chi0), np.median(chi1), np.median(chi2), np.bincount(iters), total), flush=True)
+    print("  |T - truth| median per node:", np.round(np.median(np.abs(dT), axis=0), 1), flush=True)
+    assert total == iters.sum()
+    assert np.all(iters >= 1) and np.all(iters <= 30)
+    assert np.median(chi1 / chi0) < 0.1 and np.mean(chi1 < chi0) >= 0.75
+
+
+def test_inversion_with_temperature_nodes_follows_oracle_fit(problem):
+    p = problem
+    t = p["t"]
+    model = invert.node_model(NODES)
+    ws = np.array([1.0, 5.0, 5.0, 3.0])
+    sel = [0, 1]
+    print("inversion with temperature nodes vs the oracle's fit: pixels %s, 10 iterations" % sel, flush=True)
+    nodes, chi2, iters, total = invert.invert_eos(t["table"], t["ray"], model, p["chem"], p["chem_he"], p["smap"],
+                                                  p["states"][sel], p["lam"], p["obs"][sel], p["start"][sel],
+                                                  wstokes=ws, max_iter=10, precision=abi.SNAPI_FP64)
+    for k, i in enumerate(sel):
+        vo, co, io = _oracle.lm_fit_eos(t["table"], t["ray"], NODES, p["start"][i], t["state"], p["lam"], p["obs"][i],
+                                        p["chem"], p["chem_he"], p["smap"], ws, max_iter=10)
+        print("  pixel %d: chi2 GPU %.4e oracle %.4e; iterations %d / %d; T(-3..0) GPU %s oracle %s" % (
+            i, chi2[k], co, iters[k], io, np.round(nodes[k, 2:6], 1), np.round(vo[2:6], 1)), flush=True)
+        assert abs(chi2[k] / co - 1.0) < 5e-2
+        assert iters[k] == io
+        assert np.max(np.abs(nodes[k, 2:6] - vo[2:6])) < 5.0
+
+
+def test_build_from_nodes_matches_oracle(problem):
+    """snapi_build_from_nodes against the oracle's chain on the twelve pixels:
+    node fields, temperature, hydrostatic densities and equation-of-state
+    fields to 1e-9 (hydrostatic iteration counts equal)."""
+    p = problem
+    t = p["t"]
+    model = invert.node_model(NODES)
+    print("build_from_nodes: %d pixels" % len(p["truth"]), flush=True)
+    st = invert.build_from_nodes(model, p["chem"], p["chem_he"], p["smap"], np.repeat(t["state"][None], 12, axis=0),
+                                 p["truth"])
+    err = np.max(np.abs(st - p["states"]) / np.maximum(np.abs(p["states"]), 1e-300))
+    print("  max rel diff %.2e" % err, flush=True)
+    assert err < 1e-9

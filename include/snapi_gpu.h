@@ -28,6 +28,8 @@
  *                                                                   atmos/atmos_fit.cc:1205-1352
  *   snapi_invert*[_eos]      atmosphere::stokes_lm_fit              atmos/atmos_fit.cc:18-227
  *   snapi_build_from_nodes*  atmos_pp::build_from_nodes + popsetup  atmos/plpar/atmos_pp.cc:58-180
+ *   snapi_spectral_convolve* convolve_spectra_with_gauss, convolve_response_with_gauss
+ *                                                                   cmlib/mathtools.cc:1350-1427, 1473-1498
  *   snapi_ali_accumulate     atom::add (Jbar, Lambda*, norm sums)   atmos/atmol/atom/atom.cc:1865-1927
  *   snapi_ali_newpops        atom::newpops                          atmos/atmol/atom/atom_pop.cc:27-135
  *   snapi_ali_solve*         atmosphere::nltepops (one NLTE atom)   atmos/atmos_pop.cc:67-240
@@ -531,6 +533,27 @@ int snapi_formal_responses_dev(int ncol, int nprob, int nd, const double *grid_d
  *   M [nsys][n][n]; B, X [nsys][nrhs][n] (one right-hand side per row).
  */
 int snapi_response_solve(int nsys, int n, int nrhs, const double *M, const double *B, double *X);
+
+/*
+ * Gaussian spectral convolution (convolve_spectra_with_gauss,
+ * cmlib/mathtools.cc:1350-1427), as the fit applies it to the spectrum and,
+ * component by component, to the node Jacobian (convolve_response_with_gauss,
+ * :1473-1498; atmos_fit.cc:104-110): each series is resampled with
+ * interpol_1d onto an even grid of step width/2 padded by 10 widths,
+ * convolved with exp(-d^2/width^2)/(sqrt(pi) width) by the trapezoidal rule
+ * and resampled back, in double.
+ *   data [nseries][nlambda][ncomp] in place (double for SNAPI_FP64, float for
+ *     SNAPI_MIXED): Stokes [npix][nlambda][4] (ncomp 4) or J
+ *     [npix][nlambda*4][nnode] (ncomp 4 nnode);
+ *   lam [nlambda] ascending, host memory (the grid is planned on the host);
+ *   width [cm] > 0.  SNAPI_ERR_UNSUPPORTED when even one component's fine grid
+ *   does not fit the workgroup's 60 KiB of LDS (about 2,500 fine points).
+ */
+int snapi_spectral_convolve(int precision, int nseries, int nlambda, int ncomp, const double *lam, double width,
+                            void *data);
+/* The same on a device array; lam in host memory, lam_dev its device copy. */
+int snapi_spectral_convolve_dev(int precision, int nseries, int nlambda, int ncomp, const double *lam,
+                                const double *lam_dev, double width, void *data_dev, void *stream);
 
 /* snapi_chemeq on device-resident arrays (T, Nt and the outputs: device
  * pointers; the table stays a host struct and is copied), all work on

@@ -668,6 +668,42 @@ static int mode_interp(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// Gaussian spectral convolution (convolve_spectra_with_gauss, cmlib/
+// mathtools.cc:1350-1427) of a seeded four-component spectrum:
+//   conv <nlambda> <width [A]> <seed> <uneven>
+// wavelengths around 6302 A, evenly spaced 10 mA apart or (uneven = 1) with
+// seeded gaps of 5..15 mA; the components are two absorption profiles and
+// their derivatives.  Prints the grid, the input and the convolved spectra.
+static int mode_conv(int argc, char **argv, io_class &io) {
+  if (argc < 6) return 2;
+  int nl = atoi(argv[2]);
+  fp_t width = atof(argv[3]) * 1E-8;
+  g_state = strtoull(argv[4], 0, 10) * 2654435761ULL + 88172645463325252ULL;
+  int uneven = atoi(argv[5]);
+  fp_t *lam = new fp_t[nl] - 1;
+  fp_t **S = ft2dim(1, 4, 1, nl);
+  fp_t x = 6300.5;
+  for (int l = 1; l <= nl; ++l) {
+    lam[l] = x * 1E-8;
+    x += uneven ? 0.005 + 0.01 * urand() : 0.01;
+  }
+  const fp_t c1 = 6301.5, c2 = 6302.5, d = 0.08 + 0.04 * urand();
+  for (int l = 1; l <= nl; ++l) {
+    const fp_t a = lam[l] * 1E8;
+    const fp_t g1 = exp(-(a - c1) * (a - c1) / (d * d)), g2 = exp(-(a - c2) * (a - c2) / (d * d));
+    S[1][l] = 1.0 - 0.6 * g1 - 0.4 * g2;
+    S[2][l] = 0.05 * g1 * (a - c1) / d;
+    S[3][l] = -0.03 * g2;
+    S[4][l] = 0.2 * (a - c1) / d * g1 - 0.1 * (a - c2) / d * g2;
+  }
+  for (int l = 1; l <= nl; ++l)
+    printf("in %.17g %.17g %.17g %.17g %.17g\n", lam[l], S[1][l], S[2][l], S[3][l], S[4][l]);
+  convolve_spectra_with_gauss(S, lam, nl, width);
+  for (int l = 1; l <= nl; ++l)
+    printf("out %.17g %.17g %.17g %.17g %.17g\n", lam[l], S[1][l], S[2][l], S[3][l], S[4][l]);
+  return 0;
+}
+
 // Chemical equilibrium (atmos/atmos_chm.cc:28-363) and LTE level populations
 // (atmos/atmol/atom/atom.cc:763-787) on every depth point of the column:
 //   chemeq <atmos.cfg>
@@ -900,6 +936,7 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "lmstep")) return mode_lmstep(argc, argv, io);
   if (!strcmp(argv[1], "formal_scalar")) return mode_formal_scalar(argc, argv, io);
   if (!strcmp(argv[1], "interp")) return mode_interp(argc, argv, io);
+  if (!strcmp(argv[1], "conv")) return mode_conv(argc, argv, io);
   if (!strcmp(argv[1], "nlte")) return mode_nlte(argc, argv, io);
   if (!strcmp(argv[1], "chemeq")) return mode_chemeq(argc, argv, io);
   if (!strcmp(argv[1], "nlteresp")) return mode_nlteresp(argc, argv, io);

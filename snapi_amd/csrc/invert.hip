@@ -190,8 +190,18 @@ __device__ DepthFields fields_at(const NodeArgs &na, const double *v, double x) 
   DepthFields f;
   f.vt = interp_param(na, v, NP_VT, x);
   f.vlos = interp_param(na, v, NP_VS, x);
-  const double B = interp_param(na, v, NP_B, x);
-  const double th = interp_param(na, v, NP_THETA, x), ph = interp_param(na, v, NP_PHI, x);
+  // build_from_nodes (atmos_pp.cc:121-167): a field without nodes is 1e-3 G
+  // along 1e-3 rad, and the inclination stays at least 1e-3 rad
+  bool hb = false, ht = false, hp = false;
+  for (int i = 0; i < na.nnode; ++i) {
+    hb |= na.code[i] == NP_B;
+    ht |= na.code[i] == NP_THETA;
+    hp |= na.code[i] == NP_PHI;
+  }
+  const double B = hb ? interp_param(na, v, NP_B, x) : 1E-3;
+  double th = ht ? interp_param(na, v, NP_THETA, x) : 1E-3;
+  if (fabs(th) < 1E-3) th = 1E-3;
+  const double ph = hp ? interp_param(na, v, NP_PHI, x) : 1E-3;
   const double bx = B * sin(th) * cos(ph), by = B * sin(th) * sin(ph), bz = B * cos(th);
   f.bm = sqrt(bx * bx + by * by + bz * bz);
   f.inc = acos(bz / f.bm) * 180.0 / kPi;

@@ -198,6 +198,33 @@ def _interpol_1d(f, x, xi):
     return (1.0 - u) * (1.0 - u) * f[i] + u * u * f[i + 1] + u * 2.0 * (1.0 - u) * C
 
 
+def convolve_gauss(S, lam, width):
+    """convolve_spectra_with_gauss (cmlib/mathtools.cc:1350-1427): the
+    spectra S [nlambda][ncomp] on wavelengths lam resampled (interpol_1d,
+    constant beyond the ends) onto an even grid of step width/2 padded by
+    10 widths, convolved with exp(-d^2/width^2)/(sqrt(pi) width) by the
+    trapezoidal rule, and resampled back."""
+    S = np.asarray(S, dtype=np.float64)
+    lam = np.asarray(lam, dtype=np.float64)
+    N = len(lam)
+    l_min = lam[0] - 10.0 * width
+    l_max = lam[N - 1] + 10.0 * width
+    Nf = int((l_max - l_min) / (0.5 * width)) + 1
+    step = (l_max - l_min) / (Nf - 1)
+    lf = np.array([l_min + l * step for l in range(Nf)])
+    nc = S.shape[1]
+    Si = np.array([[_interpol_1d(S[:, c], lam, lf[l]) for c in range(nc)] for l in range(Nf)])
+    w = np.full(Nf, step)
+    w[0] = w[-1] = step * 0.5
+    Sc = np.zeros((Nf, nc))
+    for l in range(Nf):
+        ker = 0.56418958354 / width * np.exp(-(lf - lf[l]) * (lf - lf[l]) / width / width)
+        for c in range(nc):
+            # cumsum adds in order, as the reference's loop does
+            Sc[l, c] = np.cumsum((Si[:, c] * ker) * w)[-1]
+    return np.array([[_interpol_1d(Sc[:, c], lf, lam[l]) for c in range(nc)] for l in range(N)])
+
+
 def _interp_temp(t, y, x):
     """atmospheric_interpolation for the temperature (cmlib/mathtools.cc:
     1208-1315, is_temp): top derivative 0, bottom derivative from the FAL-C
@@ -256,6 +283,11 @@ def nodes_to_state(model_nodes, values, state):
     for name in ("vt", "vlos", "B", "theta", "phi"):
         sel = [(lt, v) for (p, lt), v in zip(model_nodes, values) if p == name]
         q[name] = _interp_param(np.array([s[0] for s in sel]), np.array([s[1] for s in sel]), x)
+        # build_from_nodes (atmos/plpar/atmos_pp.cc:121-167): a field without
+        # nodes is 1e-3 G along 1e-3 rad; the inclination stays >= 1e-3 rad
+        if not sel and name in ("B", "theta", "phi"):
+            q[name] = np.full_like(x, 1e-3)
+    q["theta"] = np.where(np.abs(q["theta"]) < 1e-3, 1e-3, q["theta"])
     bx = q["B"] * np.sin(q["theta"]) * np.cos(q["phi"])
     by = q["B"] * np.sin(q["theta"]) * np.sin(q["phi"])
     bz = q["B"] * np.cos(q["theta"])

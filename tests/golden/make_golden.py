@@ -321,6 +321,23 @@ def build_lmstep(d):
     print("lmstep: %d cases" % 3)
 
 
+def build_conv(d):
+    """Gaussian spectral convolution of the reference (convolve_spectra_with_gauss)
+    on seeded spectra: even and uneven grids, widths of 10..60 mA."""
+    recs = {}
+    for key, args in (("c0", ["201", "0.025", "31", "0"]), ("c1", ["151", "0.04", "32", "1"]),
+                      ("c2", ["301", "0.01", "33", "0"]), ("c3", ["97", "0.06", "34", "1"])):
+        out = run(d, ["conv"] + args, he=False)
+        rin = np.array([[float(x) for x in ln.split()[1:]] for ln in out if ln.startswith("in ")])
+        rout = np.array([[float(x) for x in ln.split()[1:]] for ln in out if ln.startswith("out ")])
+        recs[key + "_lam"] = rin[:, 0]
+        recs[key + "_in"] = rin[:, 1:]
+        recs[key + "_out"] = rout[:, 1:]
+        recs[key + "_width"] = np.array(float(args[1]) * 1e-8)
+    np.savez_compressed(os.path.join(HERE, "conv.npz"), **recs)
+    print("conv: 4 cases")
+
+
 def build_interp(d):
     """Node interpolation: non-temperature (is_temp 0) for 1..5 nodes, keys
     n<k>_*; temperature (is_temp 1: flat top, bottom slope from the FAL-C
@@ -683,6 +700,7 @@ def main():
         build_formal_scalar(d)
         build_lmstep(d)
         build_interp(d)
+        build_conv(d)
         build_nlte(d)
         build_chemeq(d)
         build_nlteresp(d)

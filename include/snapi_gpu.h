@@ -638,21 +638,24 @@ int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *ma
  * stokes_lm_fit takes them (atmos_fit.cc:18-227).  The state's
  * thermodynamic fields are rebuilt; its grid and, for parameters without
  * nodes, the fields stay as given.  At least one temperature node; adjoint
- * Jacobian only.
+ * Jacobian only.  broadening > 0: the Gaussian spectral broadening [cm] the
+ * fit applies to the current spectrum and Jacobian (before the Stokes weights
+ * and the mask) and to every trial spectrum (snapi_spectral_convolve;
+ * atmos_fit.cc:104-110, 717-737); 0: none.
  */
 int snapi_invert_eos(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
                      const snapi_node_model *model, const snapi_chem_table *chem, const snapi_chem_table *chem_he,
                      const snapi_state_map *map, int npix, int nd, const void *state, int nlambda,
-                     const double *lam, const void *obs, const void *mask, const double *wstokes, double lm_start,
-                     int max_iter, int nslot, double *nodes, double *chi2, int *iterations,
-                     long long *pixel_iterations);
+                     const double *lam, const void *obs, const void *mask, const double *wstokes,
+                     double broadening, double lm_start, int max_iter, int nslot, double *nodes, double *chi2,
+                     int *iterations, long long *pixel_iterations);
 int snapi_invert_eos_dev(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
                          const snapi_node_model *model, const snapi_chem_table *chem,
                          const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
                          const void *state_dev, int nlambda, const double *lam_dev, const void *obs_dev,
-                         const void *mask_dev, const double *wstokes, double lm_start, int max_iter, int nslot,
-                         const double *nodes0_dev, double *nodes_dev, double *chi2_dev, int *iterations_dev,
-                         long long *pixel_iterations, void *stream);
+                         const void *mask_dev, const double *wstokes, double broadening, double lm_start,
+                         int max_iter, int nslot, const double *nodes0_dev, double *nodes_dev, double *chi2_dev,
+                         int *iterations_dev, long long *pixel_iterations, void *stream);
 
 /*
  * atmos_pp::build_from_nodes (atmos/plpar/atmos_pp.cc:58-180) followed by

@@ -29,4 +29,10 @@ int chem_hequilibrium(const ChemDevice &cd, long ncol, int nd, const double *T, 
 int chem_eos(const ChemDevice &cd, long ncol, int nd, double *state, double *resp, double *dstate, double *scratch,
              const int *active, hipStream_t st);
 
+// Gaussian spectral convolution of nseries [nl][ncomp] series in place
+// (instrument.hip); lam_h the host copy of lam_d; series with active == 0
+// are skipped.
+int spectral_convolve_launch(int precision, int nseries, int nl, int ncomp, const double *lam_h,
+                             const double *lam_d, double width, void *data, const int *active, hipStream_t st);
+
 }  // namespace snapi

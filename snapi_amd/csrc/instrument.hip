@@ -64,7 +64,9 @@ template <typename R>
 __global__ __launch_bounds__(kConvThreads) void gauss_convolve_kernel(int nl, int ncomp, int kc,
                                                                       const double *__restrict__ lam, double width,
                                                                       int Nf, double l_min, double step,
-                                                                      R *__restrict__ data) {
+                                                                      R *__restrict__ data,
+                                                                      const int *__restrict__ active) {
+  if (active && !active[blockIdx.x]) return;  // uniform over the workgroup
   extern __shared__ double sm[];
   double *lf = sm;                   // [Nf] fine wavelengths
   double *lm = lf + Nf;              // [nl] the series' wavelengths
@@ -121,14 +123,14 @@ bool conv_plan(const double *lam_h, int nl, double width, int &Nf, double &l_min
 
 template <typename R>
 int launch_conv(int nseries, int nl, int ncomp, const double *lam_h, const double *lam_d, double width, R *data,
-                hipStream_t st) {
+                hipStream_t st, const int *active = nullptr) {
   int Nf, kc;
   double l_min, step;
   size_t lds;
   if (!conv_plan(lam_h, nl, width, Nf, l_min, step, kc, lds)) return SNAPI_ERR_UNSUPPORTED;
   const int ng = (ncomp + kc - 1) / kc;
   hipLaunchKernelGGL(gauss_convolve_kernel<R>, dim3(nseries, ng), dim3(kConvThreads), lds, st, nl, ncomp, kc, lam_d,
-                     width, Nf, l_min, step, data);
+                     width, Nf, l_min, step, data, active);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
@@ -142,6 +144,19 @@ int check_args(int precision, int nseries, int nl, int ncomp, const double *lam,
 }
 
 }  // namespace
+
+namespace snapi {
+
+// The convolution for callers that hold the data already (the inversion):
+// series with active[series] == 0 are left alone.
+int spectral_convolve_launch(int precision, int nseries, int nl, int ncomp, const double *lam_h,
+                             const double *lam_d, double width, void *data, const int *active, hipStream_t st) {
+  return precision == SNAPI_FP64
+             ? launch_conv<double>(nseries, nl, ncomp, lam_h, lam_d, width, (double *)data, st, active)
+             : launch_conv<float>(nseries, nl, ncomp, lam_h, lam_d, width, (float *)data, st, active);
+}
+
+}  // namespace snapi
 
 extern "C" int snapi_spectral_convolve_dev(int precision, int nseries, int nlambda, int ncomp, const double *lam,
                                            const double *lam_dev, double width, void *data_dev, void *stream) {

@@ -530,7 +530,18 @@ __global__ void convert_kernel(size_t n, const A *__restrict__ a, B *__restrict_
 struct ThermoArgs {
   const snapi_chem_table *chem, *chem_he;
   const snapi_state_map *map;
+  double broadening;  // Gaussian spectral broadening width [cm], 0: none
 };
+
+// mask[slot][l] applied to J[slot][l*4+s][n] after the convolution.
+template <typename R>
+__global__ void mask_jacobian_kernel(int nl, int nc, const R *__restrict__ mask, R *__restrict__ J,
+                                     const int *__restrict__ active) {
+  const int slot = blockIdx.x;
+  if (!active[slot]) return;
+  for (int i = threadIdx.x; i < nl * nc; i += blockDim.x)
+    J[(size_t)slot * nl * nc + i] *= mask[(size_t)slot * nl + i / nc];
+}
 
 template <typename R>
 static int invert_impl(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
@@ -557,6 +568,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   double *Tv = nullptr, *ltv = nullptr, *Ntv = nullptr, *sd = nullptr, *rsd = nullptr, *dsd = nullptr,
          *tsd = nullptr, *escr = nullptr;
   R *rstate = nullptr, *rdst = nullptr, *nmap2 = nullptr;
+  std::vector<double> lam_host;  // the convolution plans its grid on the host
   int ntn = 0;
   for (int n = 0; n < nn; ++n) ntn += na.code[n] == NP_T;
   const int nvar = 1 + 2 * ntn;
@@ -641,6 +653,11 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     HIPCHK(hipMalloc((void **)&rstate, (size_t)nslot * ns * es));
     HIPCHK(hipMalloc((void **)&rdst, (size_t)nslot * 2 * ns * es));
     HIPCHK(hipMalloc((void **)&nmap2, (size_t)nslot * nn * nd * es));
+    if (th->broadening > 0.0) {
+      lam_host.resize(nl);
+      HIPCHK(hipMemcpy(lam_host.data(), dev ? lam_h : (const double *)lam_h, (size_t)nl * 8,
+                       dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+    }
   }
   if (jacobian == SNAPI_JAC_ADJOINT) {
     per_col = resp_scratch_per_col(nl, nd, es);
@@ -687,7 +704,9 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     }
     ra.nmap = nmap;
     ra.nmap2 = th ? nmap2 : nullptr;
-    ra.mask = smask;
+    // with spectral broadening the Jacobian is convolved before the mask is
+    // applied (convolve_response_with_gauss, then scale_rf)
+    ra.mask = (th && th->broadening > 0.0) ? nullptr : smask;
     ra.active = ss.needj;
     long long pixel_iters = 0;
     for (int it = 0;; ++it) {
@@ -721,6 +740,15 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
                            rdst);
         rc = launch_resp<R>(kl, a, ra, nslot, rstate, rdst, dlam, S, J, scr, chunk, st);
         if (rc) goto done;
+        if (th->broadening > 0.0) {
+          rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, S, ss.needj,
+                                        st);
+          if (!rc)
+            rc = spectral_convolve_launch(precision, nslot, nl, 4 * nn, lam_host.data(), dlam, th->broadening, J,
+                                          ss.needj, st);
+          if (rc) goto done;
+          hipLaunchKernelGGL(mask_jacobian_kernel<R>, dim3(nslot), dim3(256), 0, st, nl, 4 * nn, smask, J, ss.needj);
+        }
       } else if (jacobian == SNAPI_JAC_ADJOINT) {
         rc = launch_resp<R>(kl, a, ra, nslot, sstate, dzero, dlam, S, J, scr, chunk, st);
         if (rc) goto done;
@@ -770,6 +798,11 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
       }
       rc = synth_dev_masked(precision, lines, ray, nslot, nd, tstate, nl, dlam, St, ss.active, 1, st);
       if (rc) goto done;
+      if (th && th->broadening > 0.0) {  // forward_evaluate (atmos_fit.cc:717-737)
+        rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, St, ss.active,
+                                      st);
+        if (rc) goto done;
+      }
       hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, St, sobs, smask, ws[0], ws[1],
                          ws[2], ws[3], (R *)nullptr, ss.chi2t, ss.active);
       hipLaunchKernelGGL(update_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, max_iter, ss, snodes,
@@ -1101,8 +1134,8 @@ extern "C" int snapi_invert_eos_dev(int precision, int jacobian, const snapi_lin
                                     const snapi_chem_table *chem, const snapi_chem_table *chem_he,
                                     const snapi_state_map *map, int npix, int nd, const void *state_dev, int nlambda,
                                     const double *lam_dev, const void *obs_dev, const void *mask_dev,
-                                    const double *wstokes, double lm_start, int max_iter, int nslot,
-                                    const double *nodes0_dev, double *nodes_dev, double *chi2_dev,
+                                    const double *wstokes, double broadening, double lm_start, int max_iter,
+                                    int nslot, const double *nodes0_dev, double *nodes_dev, double *chi2_dev,
                                     int *iterations_dev, long long *pixel_iterations, void *stream) {
   if (!state_dev || !lam_dev || !obs_dev || !wstokes || !nodes0_dev || !nodes_dev || !chi2_dev || !iterations_dev ||
       !chem || !chem_he || !map)
@@ -1114,7 +1147,8 @@ extern "C" int snapi_invert_eos_dev(int precision, int jacobian, const snapi_lin
   for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
   if (ntn < 1) return SNAPI_ERR_ARG;
   if (nslot <= 0 || nslot > npix) nslot = npix;
-  const ThermoArgs th{chem, chem_he, map};
+  if (!(broadening >= 0.0)) return SNAPI_ERR_ARG;
+  const ThermoArgs th{chem, chem_he, map, broadening};
   hipStream_t st = (hipStream_t)stream;
   if (precision == SNAPI_FP64)
     return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
@@ -1131,8 +1165,8 @@ extern "C" int snapi_invert_eos(int precision, int jacobian, const snapi_line_ta
                                 const snapi_node_model *model, const snapi_chem_table *chem,
                                 const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
                                 const void *state, int nlambda, const double *lam, const void *obs, const void *mask,
-                                const double *wstokes, double lm_start, int max_iter, int nslot, double *nodes,
-                                double *chi2, int *iterations, long long *pixel_iterations) {
+                                const double *wstokes, double broadening, double lm_start, int max_iter, int nslot,
+                                double *nodes, double *chi2, int *iterations, long long *pixel_iterations) {
   if (!state || !lam || !obs || !wstokes || !nodes || !chi2 || !iterations || !chem || !chem_he || !map)
     return SNAPI_ERR_ARG;
   NodeArgs na;
@@ -1142,7 +1176,8 @@ extern "C" int snapi_invert_eos(int precision, int jacobian, const snapi_line_ta
   for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
   if (ntn < 1) return SNAPI_ERR_ARG;
   if (nslot <= 0 || nslot > npix) nslot = npix;
-  const ThermoArgs th{chem, chem_he, map};
+  if (!(broadening >= 0.0)) return SNAPI_ERR_ARG;
+  const ThermoArgs th{chem, chem_he, map, broadening};
   if (precision == SNAPI_FP64)
     return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
                                lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0,

@@ -147,7 +147,8 @@ def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_
 
 
 def invert_eos(table, ray, model, chem, chem_he, smap, state, lam_vac, obs, nodes0, mask=None,
-               wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0, precision=abi.SNAPI_FP64):
+               wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0, precision=abi.SNAPI_FP64,
+               broadening=0.0):
     """snapi_invert_eos: the inversion with temperature nodes (adjoint
     Jacobian). Returns (nodes, chi2, iterations, total pixel-iterations)."""
     dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
@@ -166,11 +167,11 @@ def invert_eos(table, ray, model, chem, chem_he, smap, state, lam_vac, obs, node
     TB = C.POINTER(abi.SnapiChemTable)
     f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay), C.POINTER(SnapiNodeModel),
                   TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double),
-                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_int, C.c_int,
+                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
     rc = f(precision, 0, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
            npix, nd, abi.vptr(st), len(lam), abi.ptr(lam), abi.vptr(ob), None if mk is None else abi.vptr(mk),
-           abi.ptr(ws), lm_start, max_iter, nslot, abi.ptr(nodes), abi.ptr(chi2),
+           abi.ptr(ws), float(broadening), lm_start, max_iter, nslot, abi.ptr(nodes), abi.ptr(chi2),
            iters.ctypes.data_as(C.POINTER(C.c_int)), C.byref(total))
     check("snapi_invert_eos", rc)
     return nodes, chi2, iters, total.value
@@ -209,7 +210,7 @@ def build_from_nodes_dev(model, chem, chem_he, smap, state_t, nodes_t, stream=No
 
 def invert_eos_dev(table, ray, model, chem, chem_he, smap, state_t, lam_t, obs_t, mask_t, nodes0_t, nodes_t, chi2_t,
                    iters_t, wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0,
-                   precision=abi.SNAPI_MIXED, stream=None):
+                   precision=abi.SNAPI_MIXED, stream=None, broadening=0.0):
     """snapi_invert_eos_dev on torch device tensors; returns the total
     pixel-iterations."""
     npix, nf, nd = state_t.shape
@@ -221,12 +222,12 @@ def invert_eos_dev(table, ray, model, chem, chem_he, smap, state_t, lam_t, obs_t
     TB = C.POINTER(abi.SnapiChemTable)
     f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay), C.POINTER(SnapiNodeModel),
                   TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
-                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_int, C.c_int, C.c_void_p,
-                  C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]
+                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
+                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]
     sp = None if stream is None else C.c_void_p(stream.cuda_stream)
     rc = f(precision, 0, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
            npix, nd, state_t.data_ptr(), nl, lam_t.data_ptr(), obs_t.data_ptr(),
-           None if mask_t is None else mask_t.data_ptr(), abi.ptr(ws), lm_start, max_iter, nslot,
+           None if mask_t is None else mask_t.data_ptr(), abi.ptr(ws), float(broadening), lm_start, max_iter, nslot,
            nodes0_t.data_ptr(), nodes_t.data_ptr(), chi2_t.data_ptr(), iters_t.data_ptr(), C.byref(total), sp)
     check("snapi_invert_eos_dev", rc)
     return total.value

@@ -401,7 +401,7 @@ _BRACKET = {"T": (3400.0, 12000.0), "vt": (1e2, 15e5), "vlos": (-15e5, 15e5), "B
 
 
 def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, smap, wstokes, max_iter=30,
-               lm_start=1.0):
+               lm_start=1.0, broadening=0.0):
     """stokes_lm_fit (atmos_fit.cc:18-227) of one pixel with temperature
     nodes, in numpy over the oracle's chain: the Jacobian and the current
     chi^2 from the response path (node_jacobian_eos), the trial's chi^2 from
@@ -409,8 +409,10 @@ def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, s
     weights (scale_rf), (1 + lambda) damping and the truncated-SVD solve,
     corrections scaled back and bracketed, lambda / sqrt(10) (floor 1e-5) on
     success and * sqrt(10) on failure, stop after three accepted steps whose
-    chi^2 changes by less than 1e-3 or at max_iter.  Returns (nodes, chi2,
-    iterations)."""
+    chi^2 changes by less than 1e-3 or at max_iter.  With broadening [cm] the
+    current spectrum and every column of the Jacobian (before the scaling) and
+    every trial spectrum are convolved with the Gaussian (atmos_fit.cc:104-110,
+    717-737).  Returns (nodes, chi2, iterations)."""
     ws = np.asarray(wstokes, dtype=np.float64)
     sc = np.array([_SCALE[p] for p, _ in model_nodes])
     v = np.array(start, dtype=np.float64)
@@ -418,14 +420,19 @@ def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, s
     need, track, chi = True, [], 0.0
     it = 0
 
+    def conv(a):
+        return convolve_gauss(a, lam, broadening) if broadening > 0.0 else a
+
     def forward(vals):
         st = forward_state_eos(model_nodes, vals, state, chem, chem_he, smap)
         s, _ = synth(table, ray, st[None], lam)
-        return s[0]
+        return conv(s[0])
 
     while it < max_iter:
         if need:
             S, J = node_jacobian_eos(table, ray, model_nodes, v, state, lam, chem, chem_he, smap)
+            S = conv(S)
+            J = conv(J.reshape(len(lam), 4 * len(model_nodes))).reshape(len(lam) * 4, len(model_nodes))
             r = ((obs - S) * ws).reshape(-1)
             chi = float((r ** 2).sum())
             Js = J * sc[None, :] * np.tile(ws, len(lam))[:, None]

@@ -123,3 +123,27 @@ def test_build_from_nodes_matches_oracle(problem):
     err = np.max(np.abs(st - p["states"]) / np.maximum(np.abs(p["states"]), 1e-300))
     print("  max rel diff %.2e" % err, flush=True)
     assert err < 1e-9
+
+
+def test_inversion_with_spectral_broadening_follows_oracle_fit(problem):
+    """With a 30 mA Gaussian instrument profile the fit convolves the current
+    spectrum, the Jacobian and every trial (atmos_fit.cc:104-110, 717-737):
+    one pixel, six iterations, against the oracle's fit with the same
+    convolution; observations convolved likewise."""
+    from snapi_amd import instrument
+    p = problem
+    t = p["t"]
+    model = invert.node_model(NODES)
+    ws = np.array([1.0, 5.0, 5.0, 3.0])
+    w = 3.0e-10
+    obs = instrument.spectral_convolve(p["obs"][:1], p["lam"], w)
+    print("inversion with spectral broadening vs the oracle's fit: 1 pixel, 6 iterations", flush=True)
+    nodes, chi2, iters, total = invert.invert_eos(t["table"], t["ray"], model, p["chem"], p["chem_he"], p["smap"],
+                                                  p["states"][:1], p["lam"], obs, p["start"][:1], wstokes=ws,
+                                                  max_iter=6, precision=abi.SNAPI_FP64, broadening=w)
+    vo, co, io = _oracle.lm_fit_eos(t["table"], t["ray"], NODES, p["start"][0], t["state"], p["lam"], obs[0],
+                                    p["chem"], p["chem_he"], p["smap"], ws, max_iter=6, broadening=w)
+    print("  chi2 GPU %.4e oracle %.4e; iterations %d / %d; T(-3..0) GPU %s oracle %s" % (
+        chi2[0], co, iters[0], io, np.round(nodes[0, 2:6], 1), np.round(vo[2:6], 1)), flush=True)
+    assert abs(chi2[0] / co - 1.0) < 5e-2 and iters[0] == io
+    assert np.max(np.abs(nodes[0, 2:6] - vo[2:6])) < 5.0

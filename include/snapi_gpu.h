@@ -28,6 +28,9 @@
  *                                                                   atmos/atmos_fit.cc:1205-1352
  *   snapi_invert*[_eos]      atmosphere::stokes_lm_fit              atmos/atmos_fit.cc:18-227
  *   snapi_build_from_nodes*  atmos_pp::build_from_nodes + popsetup  atmos/plpar/atmos_pp.cc:58-180
+ *   snapi_ana_*              ana_fzread / ana_fzwrite               cmlib/ana_read.cc, ana_write.cc
+ *   snapi_cube_to_state_dev  atmosphere::read_spinor3d (+ project)  atmos/atmos_fio.cc:79-127
+ *   snapi_stokes_to_cube_dev the job's spectra cube                 master/job.cc:590-597
  *   snapi_spectral_convolve* convolve_spectra_with_gauss, convolve_response_with_gauss
  *                                                                   cmlib/mathtools.cc:1350-1427, 1473-1498
  *   snapi_ali_accumulate     atom::add (Jbar, Lambda*, norm sums)   atmos/atmol/atom/atom.cc:1865-1927
@@ -47,6 +50,8 @@
  */
 #ifndef SNAPI_GPU_H
 #define SNAPI_GPU_H
+
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -554,6 +559,40 @@ int snapi_spectral_convolve(int precision, int nseries, int nlambda, int ncomp, 
 /* The same on a device array; lam in host memory, lam_dev its device copy. */
 int snapi_spectral_convolve_dev(int precision, int nseries, int nlambda, int ncomp, const double *lam,
                                 const double *lam_dev, double width, void *data_dev, void *stream);
+
+/*
+ * Cube I/O.  ANA f0 files, uncompressed (ana_fzread / ana_fzwrite,
+ * cmlib/ana_read.cc:74-160, ana_write.cc:120-171): header blocks of 512
+ * bytes, dims[0] fastest; data of type 0 int8, 1 int16, 2 int32, 3 float32,
+ * 4 float64, 5 int64, converted to this (little-endian) host's byte order on
+ * reading; written little-endian with one header block per 255 characters of
+ * header text.  Crunch-compressed files give SNAPI_ERR_UNSUPPORTED.
+ *   snapi_ana_info: type, ndim, dims[16] and whether compressed;
+ *   snapi_ana_read: the data into a buffer of exactly the data's bytes.
+ */
+int snapi_ana_info(const char *path, int *type, int *ndim, int *dims, int *compressed);
+int snapi_ana_read(const char *path, void *data, size_t bytes);
+int snapi_ana_write(const char *path, const void *data, int type, int ndim, const int *dims, const char *header);
+
+/*
+ * The 12 x NX x NY x NZ model cube (atmosphere::read_spinor3d,
+ * atmos/atmos_fio.cc:79-127; NZ fastest, as the ANA file (nz, ny, nx, 12)
+ * holds it) to column states (col = x * ny + y): per point log tau, height,
+ * T, Pg, Pe, rho, kappa_ref, |B|, vmicro, vz, inclination, azimuth [rad] give
+ * GRID = -10^log tau, T (at least 3200 K), NT = Pg / kT, NE = Pe / kT, OPREF,
+ * VT, and the ray projections of the velocity and field for the ray (theta,
+ * phi) (atmosphere::project / transform, atmos/atmos_rad.cc:15-51); the other
+ * fields are zero, for snapi_eos_state to fill as popsetup and
+ * compute_op_referent would.  cube_dev double; state in the precision's type
+ * [nx*ny][nfield][nz].
+ */
+int snapi_cube_to_state_dev(int precision, const double *cube_dev, int nx, int ny, int nz, int nfield,
+                            double theta, double phi, void *state_dev, void *stream);
+/* Column Stokes spectra [nx*ny][nlambda][4] to the job's output cube
+ * [nx][ny][4][nlambda] in double (master/job.cc:590-597; written as an ANA
+ * FLOAT64 file with dims (nlambda, 4, ny, nx)). */
+int snapi_stokes_to_cube_dev(int precision, const void *stokes_dev, int nx, int ny, int nlambda, double *cube_dev,
+                             void *stream);
 
 /* snapi_chemeq on device-resident arrays (T, Nt and the outputs: device
  * pointers; the table stays a host struct and is copied), all work on

@@ -65,6 +65,7 @@
 #include "atmol/atom/atom.h"
 #include "atmol/mol/mol.h"
 #include "plpar/atmos_ppbez.h"
+#include "ana_io.h"
 #include "const.h"  // last: it defines one-letter macros (c, h, k, R, ...)
 #undef protected
 #undef private
@@ -704,6 +705,56 @@ static int mode_conv(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// ANA f0 files as the reference writes them (ana_fzwrite, cmlib/
+// ana_write.cc:120-171, through write_file, cmlib/fileio.cc:273-330):
+//   ana <out_prefix>
+// writes <out_prefix>_3d.f0 (write_file of a 3-D array, FLOAT32, the
+// nodes-cube layout), <out_prefix>_4d.f0 (write_file of a 4-D array,
+// FLOAT64, the spectra-cube layout) with seeded values, and prints the values
+// in the reference's index order.
+static int mode_ana(int argc, char **argv, io_class &io) {
+  if (argc < 3) return 2;
+  g_state = 12345ULL * 2654435761ULL + 88172645463325252ULL;
+  const int n1 = 3, n2 = 4, n3 = 5;
+  fp_t ***a = ft3dim(1, n1, 1, n2, 1, n3);
+  for (int i = 1; i <= n1; ++i)
+    for (int j = 1; j <= n2; ++j)
+      for (int kk = 1; kk <= n3; ++kk) {
+        a[i][j][kk] = 1000.0 * urand() - 500.0;
+        printf("a3 %d %d %d %.9g\n", i, j, kk, (double)(float)a[i][j][kk]);
+      }
+  // write_file's 3-D layout (fileio.cc:273-286; fileio.cc itself needs
+  // cfitsio and is not built): float, dims (n1, n2, n3), n1 fastest
+  char name[1024];
+  snprintf(name, sizeof(name), "%s_3d.f0", argv[2]);
+  {
+    int ds[4] = {0, n1, n2, n3};
+    float *q = new float[n1 * n2 * n3];
+    for (int i = 1; i <= n1; ++i)
+      for (int j = 1; j <= n2; ++j)
+        for (int kk = 1; kk <= n3; ++kk) q[((kk - 1) * n2 + (j - 1)) * n1 + (i - 1)] = (float)a[i][j][kk];
+    ana_fzwrite((byte *)q, name, ds, 3, 0, FLOAT32, io);
+    delete[] q;
+  }
+  const int m1 = 2, m2 = 3, m3 = 4, m4 = 6;
+  fp_t ****b = ft4dim(1, m1, 1, m2, 1, m3, 1, m4);
+  for (int i = 1; i <= m1; ++i)
+    for (int j = 1; j <= m2; ++j)
+      for (int kk = 1; kk <= m3; ++kk)
+        for (int l = 1; l <= m4; ++l) {
+          b[i][j][kk][l] = urand() * 1e14;
+          printf("a4 %d %d %d %d %.17g\n", i, j, kk, l, b[i][j][kk][l]);
+        }
+  // write_file's 4-D double layout (fileio.cc:320-330): dims (m4, m3, m2, m1),
+  // the array as it lies in memory
+  snprintf(name, sizeof(name), "%s_4d.f0", argv[2]);
+  {
+    int ds[5] = {0, m4, m3, m2, m1};
+    ana_fzwrite((byte *)(b[1][1][1] + 1), name, ds, 4, 0, FLOAT64, io);
+  }
+  return 0;
+}
+
 // Chemical equilibrium (atmos/atmos_chm.cc:28-363) and LTE level populations
 // (atmos/atmol/atom/atom.cc:763-787) on every depth point of the column:
 //   chemeq <atmos.cfg>
@@ -937,6 +988,7 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "formal_scalar")) return mode_formal_scalar(argc, argv, io);
   if (!strcmp(argv[1], "interp")) return mode_interp(argc, argv, io);
   if (!strcmp(argv[1], "conv")) return mode_conv(argc, argv, io);
+  if (!strcmp(argv[1], "ana")) return mode_ana(argc, argv, io);
   if (!strcmp(argv[1], "nlte")) return mode_nlte(argc, argv, io);
   if (!strcmp(argv[1], "chemeq")) return mode_chemeq(argc, argv, io);
   if (!strcmp(argv[1], "nlteresp")) return mode_nlteresp(argc, argv, io);

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsnapi_gpu.so")
 SOURCES = ["runtime.hip", "synth.hip", "responses.hip", "tangent.hip", "lm.hip", "formal_scalar.hip", "invert.hip",
            "ali.hip", "chemeq.hip",
-           "nlte_resp.hip", "instrument.hip"]
+           "nlte_resp.hip", "instrument.hip", "cube.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # packed-FP32 instructions stay off: hipcc would otherwise form

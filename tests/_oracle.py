@@ -225,6 +225,36 @@ def convolve_gauss(S, lam, width):
     return np.array([[_interpol_1d(Sc[:, c], lf, lam[l]) for c in range(nc)] for l in range(N)])
 
 
+def cube_to_state(cube, nfield, theta=np.pi, phi=0.0):
+    """read_spinor3d's mapping of the 12-parameter cube (atmos/atmos_fio.cc:
+    104-122) and the ray projection (atmos/atmos_rad.cc:15-51): state
+    [nx*ny][nfield][nz] with the other fields zero."""
+    from snapi_amd import abi
+    _, nx, ny, nz = cube.shape
+    q = cube.reshape(12, nx * ny, nz)
+    kB = 1.3806503E-16
+    T = np.where(q[2] > 3200.0, q[2], 3200.0)
+    B, el, az, Vz = q[7], q[10], q[11], q[9]
+    Bx, By, Bz = B * np.sin(el) * np.cos(az), B * np.sin(el) * np.sin(az), B * np.cos(el)
+    cp, sp, ct, st_ = np.cos(phi), np.sin(phi), np.cos(np.pi - theta), np.sin(np.pi - theta)
+    mg = np.sqrt(Bx * Bx + By * By + Bz * Bz)
+    nBx = ct * cp * Bx + sp * By - st_ * cp * Bz
+    nBy = -ct * sp * Bx + cp * By + st_ * sp * Bz
+    nBz = st_ * Bx + ct * Bz
+    out = np.zeros((nx * ny, nfield, nz))
+    out[:, abi.F_GRID] = -10.0 ** q[0]
+    out[:, abi.F_T] = T
+    out[:, abi.F_NT] = q[3] / (kB * T)
+    out[:, abi.F_NE] = q[4] / (kB * T)
+    out[:, abi.F_OPREF] = q[6]
+    out[:, abi.F_VT] = q[8]
+    out[:, abi.F_VLOS] = np.cos(np.pi - theta) * Vz
+    out[:, abi.F_BMAG] = mg
+    out[:, abi.F_BINC] = np.arccos(nBz / mg) * 180.0 / np.pi
+    out[:, abi.F_BAZI] = np.arctan(nBy / nBx) * 180.0 / np.pi
+    return out
+
+
 def _interp_temp(t, y, x):
     """atmospheric_interpolation for the temperature (cmlib/mathtools.cc:
     1208-1315, is_temp): top derivative 0, bottom derivative from the FAL-C

@@ -338,6 +338,25 @@ def build_conv(d):
     print("conv: 4 cases")
 
 
+def build_ana(d):
+    """ANA f0 files written by the reference's ana_fzwrite in write_file's
+    3-D float and 4-D double layouts (the nodes and spectra cubes), and the
+    values in the reference's index order."""
+    out = run(d, ["ana", os.path.join(d, "ref")], he=False)
+    a3 = np.zeros((3, 4, 5))
+    a4 = np.zeros((2, 3, 4, 6))
+    for ln in out:
+        t = ln.split()
+        if t[0] == "a3":
+            a3[int(t[1]) - 1, int(t[2]) - 1, int(t[3]) - 1] = float(t[4])
+        elif t[0] == "a4":
+            a4[int(t[1]) - 1, int(t[2]) - 1, int(t[3]) - 1, int(t[4]) - 1] = float(t[5])
+    for n in ("3d", "4d"):
+        shutil.copyfile(os.path.join(d, "ref_%s.f0" % n), os.path.join(HERE, "ana_ref_%s.f0" % n))
+    np.savez_compressed(os.path.join(HERE, "ana.npz"), a3=a3, a4=a4)
+    print("ana: 2 files")
+
+
 def build_interp(d):
     """Node interpolation: non-temperature (is_temp 0) for 1..5 nodes, keys
     n<k>_*; temperature (is_temp 1: flat top, bottom slope from the FAL-C
@@ -701,6 +720,7 @@ def main():
         build_lmstep(d)
         build_interp(d)
         build_conv(d)
+        build_ana(d)
         build_nlte(d)
         build_chemeq(d)
         build_nlteresp(d)

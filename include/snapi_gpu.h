@@ -216,7 +216,14 @@ typedef struct {
 enum snapi_jacobian {
   SNAPI_JAC_ADJOINT = 0,     /* reverse sweep contracted with the node responses */
   SNAPI_JAC_DIFFERENCES = 1, /* central differences of the node values through the synthesis (2n+1 syntheses) */
-  SNAPI_JAC_TANGENT = 2      /* forward-mode sweep carrying every node direction (no per-depth storage) */
+  SNAPI_JAC_TANGENT = 2,     /* forward-mode sweep carrying every node direction (no per-depth storage) */
+  /* flag, ORed with one of the above: the node-to-depth maps as the exact
+   * derivatives of the node interpolation and field projection, and, for
+   * temperature nodes, dNt/dnode by the hydrostatic tangent
+   * (snapi_hequilibrium_tangent), instead of the reference's central
+   * differences of the interpolation and its two hydrostatic rebuilds per
+   * temperature node (atmos/atmos_fit.cc:1225-1305) */
+  SNAPI_JAC_ANALYTIC_MAPS = 16
 };
 
 /*
@@ -723,6 +730,24 @@ int snapi_hequilibrium_dev(const snapi_chem_table *table, int ncol, int nd, cons
                            const double *logtau_dev, double *Nt_dev, double *ne_dev, double *opref_dev,
                            double *rho_dev, double *stage_dev, double *molecule_dev, double *level_dev,
                            int *iterations_dev, void *stream);
+
+/*
+ * Tangent of the hydrostatic equilibrium: for ncol columns already in
+ * equilibrium (T, logtau, Nt [ncol][nd], Nt as snapi_hequilibrium returns
+ * it), dNt [ncol][ndir][nd] along the temperature perturbations dT
+ * [ncol][ndir][nd].  The linearised equations of the converged march
+ * (P_0 = tau_0 g rho_0 / kappa_0; P_z - P_(z-1) = g dtau / kappa_mean), with
+ * the density and opacity derivatives by central differences of the chemical
+ * equilibrium (delta_T/2, delta_Nt_frac/2).  No counterpart in the
+ * reference, whose obs_stokes_responses_to_nodes (atmos/atmos_fit.cc:
+ * 1225-1248) differences two hydrostatic rebuilds per temperature node; this
+ * replaces those rebuilds (SNAPI_JAC_ANALYTIC_MAPS).
+ */
+int snapi_hequilibrium_tangent(const snapi_chem_table *table, int ncol, int nd, const double *T,
+                               const double *logtau, const double *Nt, int ndir, const double *dT, double *dNt);
+int snapi_hequilibrium_tangent_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,
+                                   const double *logtau_dev, const double *Nt_dev, int ndir, const double *dT_dev,
+                                   double *dNt_dev, void *stream);
 
 #ifdef __cplusplus
 }

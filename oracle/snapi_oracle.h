@@ -100,6 +100,10 @@ int orc_hequilibrium(const snapi_chem_table *tb, int nd, const double *T, const 
                      double *ne, double *opref, double *rho, int *iters);
 int orc_hequilibrium_batch(const snapi_chem_table *tb, int ncol, int nd, const double *T, const double *logtau,
                            double *Nt, double *ne, double *opref, double *rho, int *iters, int nthreads);
+int orc_hequilibrium_tol(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, double *Nt,
+                         double tol, int max_iter);
+int orc_he_tangent(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, const double *Nt,
+                   int ndir, const double *dT, double *dNt);
 int orc_chemeq(const snapi_chem_table *tb, int ncol, int nd, const double *T, const double *Nt, double *ne,
                double *stage, double *mol, double *lev, double *opref, int *iterations, int nthreads);
 

@@ -10,6 +10,8 @@
  *   level populations     atom::lte, atmos/atmol/atom/atom.cc:779-787
  *   referent opacity      atmosphere::opacity_continuum, atmos/atmos_rad.cc:286-294
  *   hydrostatic eq.       atmos_pp::enforce_hequilibrium, atmos/plpar/atmos_pp.cc:372-455
+ *   hydrostatic tangent   no reference counterpart (the reference differences
+ *                         two rebuilds, atmos_fit.cc:1225-1248)
  *
  * The reference solves each Newton system by SVD (cmlib svdcmp) and exits
  * when a singular value falls below 1e-30 of the largest; for every system it
@@ -24,9 +26,12 @@
  * against the reference's compute_op_referent on the same points and the LTE
  * detailed-balance form of its bound-free term; the hydrostatic march against
  * the reference's enforce_hequilibrium on the cfg0 column (Nt, Ne, opacity and
- * density at every depth) and against the pressure balance it integrates.
+ * density at every depth) and against the pressure balance it integrates;
+ * the hydrostatic tangent against central differences of the march solved to
+ * rounding (orc_hequilibrium_tol).
  */
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "snapi_oracle.h"
@@ -451,11 +456,9 @@ int orc_chemeq(const snapi_chem_table *tb, int ncol, int nd, const double *T, co
  * chemical equilibrium at the final Nt.  rho = 1.4 * 1.67e-24 times the total
  * (free) density of element 1.  iters[z]: iterations run at depth z. */
 #define C_GRAV 274.88E2
-int orc_hequilibrium(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, double *Nt,
-                     double *ne, double *opref, double *rho, int *iters) {
+static int hequilibrium_march(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, double *Nt,
+                              double *ne, double *opref, double *rho, int *iters, double break_me, int max_iter) {
   if (nd < 1 || !(tb->lambda_ref > 0.0)) return -1;
-  const int max_iter = 20;
-  const double break_me = 1E-2;
   double e, op_prev = 0.0, rho_prev = 0.0;
   {
     Nt[0] = 0.3 / C_K / T[0];
@@ -501,6 +504,69 @@ int orc_hequilibrium(const snapi_chem_table *tb, int nd, const double *T, const 
     if (opref) opref[z] = op;
     if (rho) rho[z] = nh * 1.4 * 1.67E-24;
   }
+  return 0;
+}
+
+int orc_hequilibrium(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, double *Nt,
+                     double *ne, double *opref, double *rho, int *iters) {
+  return hequilibrium_march(tb, nd, T, logtau, Nt, ne, opref, rho, iters, 1E-2, 20);
+}
+
+/* The same march with another stopping rule (correction below tol of the
+ * top point's Nt, at most max_iter iterations per depth): with a tight tol
+ * it solves the hydrostatic equations to rounding, the reference for the
+ * tangent below. */
+int orc_hequilibrium_tol(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, double *Nt,
+                         double tol, int max_iter) {
+  return hequilibrium_march(tb, nd, T, logtau, Nt, NULL, NULL, NULL, NULL, tol, max_iter);
+}
+
+/* Tangent of the hydrostatic equilibrium (no counterpart in the reference,
+ * which differences two rebuilds): with P = Nt k T and r = rho / kappa the
+ * mass per unit opacity, the converged march satisfies
+ *   P_0 = tau_0 g r_0,   P_z = P_(z-1) + g (tau_z - tau_(z-1)) sqrt(r_z r_(z-1)).
+ * Differentiated along a temperature perturbation dT[z], with d ln r / dT and
+ * d ln r / dNt at each point by central differences of the chemical
+ * equilibrium (T +- delta_T/2, Nt +- delta_Nt_frac Nt / 2, cmlib/const.h:
+ * 27-29), every dNt[z] follows from the point above it:
+ *   dNt_0 (k T_0 - Q_0 a_N0) = (Q_0 a_T0 - k Nt_0) dT_0,  Q_0 = tau_0 g r_0;
+ *   dNt_z (k T_z - Q_z a_Nz / 2) = -k Nt_z dT_z + k (T_(z-1) dNt_(z-1) + Nt_(z-1) dT_(z-1))
+ *        + Q_z / 2 (a_Tz dT_z + a_T(z-1) dT_(z-1) + a_N(z-1) dNt_(z-1)),
+ *   Q_z = g (tau_z - tau_(z-1)) sqrt(r_z r_(z-1)).
+ * T, logtau, Nt [nd]: a converged column; dT, dNt [ndir][nd]. */
+int orc_he_tangent(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, const double *Nt,
+                   int ndir, const double *dT, double *dNt) {
+  if (nd < 1 || ndir < 0 || !(tb->lambda_ref > 0.0)) return -1;
+  double *r = malloc(sizeof(double) * 3 * nd), *aT = r + nd, *aN = r + 2 * nd;
+  for (int z = 0; z < nd; ++z) {
+    double e, op, nh, rv[5];
+    const double dNs = 1E-3 * Nt[z];
+    const double Tv[5] = {T[z], T[z] + 2.5, T[z] - 2.5, T[z], T[z]};
+    const double Nv[5] = {Nt[z], Nt[z], Nt[z], Nt[z] + 0.5 * dNs, Nt[z] - 0.5 * dNs};
+    for (int v = 0; v < 5; ++v) {
+      chemeq_point(tb, Tv[v], Nv[v], &e, NULL, NULL, NULL, &op, &nh);
+      rv[v] = nh * 1.4 * 1.67E-24 / op;
+    }
+    r[z] = rv[0];
+    aT[z] = (rv[1] - rv[2]) / 5.0 / rv[0];
+    aN[z] = (rv[3] - rv[4]) / dNs / rv[0];
+  }
+  for (int k = 0; k < ndir; ++k) {
+    const double *dt = dT + (size_t)k * nd;
+    double *dn = dNt + (size_t)k * nd;
+    double tau_prev = pow(10.0, logtau[0]);
+    const double Q0 = tau_prev * C_GRAV * r[0];
+    dn[0] = (Q0 * aT[0] - C_K * Nt[0]) * dt[0] / (C_K * T[0] - Q0 * aN[0]);
+    for (int z = 1; z < nd; ++z) {
+      const double tau = pow(10.0, logtau[z]);
+      const double Q = C_GRAV * (tau - tau_prev) * sqrt(r[z] * r[z - 1]);
+      const double rhs = -C_K * Nt[z] * dt[z] + C_K * (T[z - 1] * dn[z - 1] + Nt[z - 1] * dt[z - 1]) +
+                         0.5 * Q * (aT[z] * dt[z] + aT[z - 1] * dt[z - 1] + aN[z - 1] * dn[z - 1]);
+      dn[z] = rhs / (C_K * T[z] - 0.5 * Q * aN[z]);
+      tau_prev = tau;
+    }
+  }
+  free(r);
   return 0;
 }
 

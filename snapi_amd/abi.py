@@ -16,6 +16,10 @@ SNAPI_ERR_NONFINITE = -4
 SNAPI_FP64 = 0
 SNAPI_MIXED = 1
 
+# jacobian argument (include/snapi_gpu.h enum snapi_jacobian)
+SNAPI_JAC_ADJOINT, SNAPI_JAC_DIFFERENCES, SNAPI_JAC_TANGENT = 0, 1, 2
+SNAPI_JAC_ANALYTIC_MAPS = 16
+
 F_T, F_VT, F_VLOS, F_BMAG, F_BINC, F_BAZI, F_NE, F_NHI, F_OPREF, F_GRID, F_BGOP, F_BGEM, F_NT = range(13)
 NF_BASE = 13
 

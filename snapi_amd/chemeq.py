@@ -47,6 +47,8 @@ def _declare(L):
     L.snapi_hequilibrium.restype = C.c_int
     L.snapi_hequilibrium_dev.argtypes = [TB, C.c_int, C.c_int] + [C.c_void_p] * 11
     L.snapi_hequilibrium_dev.restype = C.c_int
+    L.snapi_hequilibrium_tangent.argtypes = [TB, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp]
+    L.snapi_hequilibrium_tangent.restype = C.c_int
     SM = C.POINTER(abi.SnapiStateMap)
     L.snapi_eos_state.argtypes = [TB, SM, C.c_int, C.c_int, _dp, _dp, _dp]
     L.snapi_eos_state.restype = C.c_int
@@ -134,6 +136,23 @@ def hequilibrium(table, T, logtau, populations=False):
                               out["iterations"].ctypes.data_as(C.POINTER(C.c_int)))
     check("snapi_hequilibrium", rc)
     return out
+
+
+def hequilibrium_tangent(table, T, logtau, Nt, dT):
+    """snapi_hequilibrium_tangent: dNt [ncol][ndir][nd] of columns in
+    hydrostatic equilibrium (T, logtau, Nt [ncol][nd]) along the temperature
+    perturbations dT [ncol][ndir][nd]."""
+    L = lib()
+    _declare(L)
+    T, logtau = _shape(T, logtau)
+    ncol, nd = T.shape
+    Nt = np.ascontiguousarray(Nt, dtype=np.float64).reshape(ncol, nd)
+    dT = np.ascontiguousarray(dT, dtype=np.float64).reshape(ncol, -1, nd)
+    dN = np.zeros_like(dT)
+    rc = L.snapi_hequilibrium_tangent(C.byref(table), ncol, nd, abi.ptr(T), abi.ptr(logtau), abi.ptr(Nt),
+                                      dT.shape[1], abi.ptr(dT), abi.ptr(dN))
+    check("snapi_hequilibrium_tangent", rc)
+    return dN
 
 
 def hequilibrium_dev(table, T_t, logtau_t, Nt_t, ne_t=None, opref_t=None, rho_t=None, stage_t=None,

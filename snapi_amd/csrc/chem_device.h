@@ -24,6 +24,13 @@ size_t chem_eos_scratch(const ChemDevice &cd, long npt, bool derivatives);
 // active[c / group] is set
 int chem_hequilibrium(const ChemDevice &cd, long ncol, int nd, const double *T, const double *logtau, double *Nt,
                       const int *active, int group, hipStream_t st);
+// tangent of the hydrostatic equilibrium of ncol converged columns along
+// ndir temperature perturbations each (snapi_hequilibrium_tangent): dT, dN
+// [ncol][ndir][nd]; scratch of chem_he_tangent_scratch bytes; columns with
+// active[c] == 0 are skipped
+size_t chem_he_tangent_scratch(long ncol, int nd);
+int chem_he_tangent(const ChemDevice &cd, long ncol, int nd, const double *T, const double *logtau, const double *Nt,
+                    int ndir, const double *dT, double *dN, double *scratch, const int *active, hipStream_t st);
 // the equation-of-state pass (snapi_eos_state) on ncol columns of a double
 // state; resp/dstate may be null; columns with active[c] == 0 are skipped
 int chem_eos(const ChemDevice &cd, long ncol, int nd, double *state, double *resp, double *dstate, double *scratch,

@@ -944,6 +944,81 @@ __global__ __launch_bounds__(kChemThreads) void eos_variants_kernel(const snapi_
     }
 }
 
+// Tangent of the hydrostatic equilibrium.  With P = Nt k T and r = rho /
+// kappa the mass per unit opacity, a converged march satisfies
+//   P_0 = tau_0 g r_0,   P_z = P_(z-1) + g (tau_z - tau_(z-1)) sqrt(r_z r_(z-1)),
+// so a temperature perturbation dT moves Nt by the solution of the
+// linearised equations, from the top down.  d ln r / dT and d ln r / dNt at
+// each point come from central differences of the chemical equilibrium with
+// the reference's steps (T +- delta_T/2, Nt +- delta_Nt_frac Nt / 2).  The
+// reference instead differences two hydrostatic rebuilds per temperature
+// node (atmos_fit.cc:1225-1248); this needs five equilibria per point for
+// all nodes together.
+
+// One team per (point, variant): r at v0 (T, Nt), v1/v2 T +- delta_T/2,
+// v3/v4 Nt +- delta_Nt_frac Nt / 2.  rec [npt][5].
+__global__ __launch_bounds__(kChemThreads) void he_partials_kernel(const snapi_chem_table *__restrict__ tb,
+                                                                   const ChemDerived *__restrict__ dv, long npt,
+                                                                   int nd, const double *__restrict__ Tin,
+                                                                   const double *__restrict__ Ntin,
+                                                                   double *__restrict__ rec,
+                                                                   const int *__restrict__ active) {
+  __shared__ ChemShared sh;
+  extern __shared__ double region_s[];
+  const int tid = threadIdx.x;
+  load_shared(sh, tb, dv, tid);
+  const ChemTeam cx = make_team(tb, dv, sh, region_s, tid);
+  const long t = ((long)blockIdx.x * blockDim.x + tid) / kTeam;
+  if (t >= npt * 5) return;  // the whole team leaves together; no barrier follows
+  const long pt = t / 5;
+  const int v = (int)(t % 5);
+  if (active && !active[pt / nd]) return;
+  const double T = Tin[pt], Nt = Ntin[pt], dNs = kDeltaNtf * Nt;
+  const double Tv = v == 1 ? T + 0.5 * kDeltaT : v == 2 ? T - 0.5 * kDeltaT : T;
+  const double Nv = v == 3 ? Nt + 0.5 * dNs : v == 4 ? Nt - 0.5 * dNs : Nt;
+  const ChemTemp ct = set_temperature(tb, cx, Tv);
+  const ChemSol sol = solve_point(tb, dv, sh, cx, ct, Nv);
+  const double op = referent_opacity(tb, dv, cx, ct, sol);
+  const double rho = mass_density(tshfl(sol.Nown, 1));
+  if (cx.lane == 0) rec[t] = rho / op;
+}
+
+// One thread per (column, direction): the linearised march.  T, logtau, Nt
+// [ncol][nd]; dT, dN [ncol][ndir][nd].
+__global__ void he_tangent_kernel(long ncol, int nd, int ndir, const double *__restrict__ T,
+                                  const double *__restrict__ logtau, const double *__restrict__ Nt,
+                                  const double *__restrict__ rec, const double *__restrict__ dT,
+                                  double *__restrict__ dN, const int *__restrict__ active) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncol * ndir) return;
+  const long c = i / ndir;
+  if (active && !active[c]) return;
+  const double *Tc = T + c * nd, *lt = logtau + c * nd, *Nc = Nt + c * nd, *rc = rec + c * nd * 5;
+  const double *dt = dT + i * nd;
+  double *dn = dN + i * nd;
+  double r_prev = 0.0, aT_prev = 0.0, aN_prev = 0.0, tau_prev = 0.0;
+  for (int z = 0; z < nd; ++z) {
+    const double *q = rc + z * 5;
+    const double r = q[0], aT = (q[1] - q[2]) / kDeltaT / r, aN = (q[3] - q[4]) / (kDeltaNtf * Nc[z]) / r;
+    const double tau = pow(10.0, lt[z]);
+    double d;
+    if (z == 0) {
+      const double Q = tau * kGrav * r;
+      d = (Q * aT - kK * Nc[0]) * dt[0] / (kK * Tc[0] - Q * aN);
+    } else {
+      const double Q = kGrav * (tau - tau_prev) * sqrt(r * r_prev);
+      const double rhs = -kK * Nc[z] * dt[z] + kK * (Tc[z - 1] * dn[z - 1] + Nc[z - 1] * dt[z - 1]) +
+                         0.5 * Q * (aT * dt[z] + aT_prev * dt[z - 1] + aN_prev * dn[z - 1]);
+      d = rhs / (kK * Tc[z] - 0.5 * Q * aN);
+    }
+    dn[z] = d;
+    r_prev = r;
+    aT_prev = aT;
+    aN_prev = aN;
+    tau_prev = tau;
+  }
+}
+
 // U of stage z of element a (0-based) at (T, ne), as the team evaluation
 // forms it from the temperature-only part and the Traving asymptotic terms.
 __device__ double pf_direct(const snapi_chem_table *__restrict__ tb, const ChemDerived *__restrict__ dv, int a,
@@ -1211,6 +1286,22 @@ int launch_chemeq(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
+int launch_he_tangent(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot, long ncol, int nd,
+                      const double *T, const double *logtau, const double *Nt, int ndir, const double *dT, double *dN,
+                      double *rec, hipStream_t st, const int *active = nullptr) {
+  long blocks;
+  size_t lds;
+  if (!team_grid(ncol * nd * 5, nslot, blocks, lds)) return SNAPI_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(he_partials_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, ncol * nd, nd,
+                     T, Nt, rec, active);
+  const long b2 = (ncol * ndir + 127) / 128;
+  if (b2 > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
+  if (b2 > 0)
+    hipLaunchKernelGGL(he_tangent_kernel, dim3((unsigned)b2), dim3(128), 0, st, ncol, nd, ndir, T, logtau, Nt, rec, dT,
+                       dN, active);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
 int launch_hequilibrium(const snapi_chem_table *dtb, const ChemDerived *ddv, int nslot, long ncol, int nd,
                         const double *T, const double *logtau, double *Nt, int *it, hipStream_t st,
                         const int *active = nullptr, int group = 1) {
@@ -1340,6 +1431,15 @@ int chem_hequilibrium(const ChemDevice &cd, long ncol, int nd, const double *T, 
   return launch_hequilibrium((const snapi_chem_table *)cd.buf,
                              (const ChemDerived *)((const char *)cd.buf + sizeof(snapi_chem_table)), cd.nslot, ncol,
                              nd, T, logtau, Nt, nullptr, st, active, group);
+}
+
+size_t chem_he_tangent_scratch(long ncol, int nd) { return (size_t)ncol * nd * 5 * sizeof(double); }
+
+int chem_he_tangent(const ChemDevice &cd, long ncol, int nd, const double *T, const double *logtau, const double *Nt,
+                    int ndir, const double *dT, double *dN, double *scratch, const int *active, hipStream_t st) {
+  return launch_he_tangent((const snapi_chem_table *)cd.buf,
+                           (const ChemDerived *)((const char *)cd.buf + sizeof(snapi_chem_table)), cd.nslot, ncol, nd,
+                           T, logtau, Nt, ndir, dT, dN, scratch, st, active);
 }
 
 int chem_eos(const ChemDevice &cd, long ncol, int nd, double *state, double *resp, double *dstate, double *scratch,
@@ -1483,6 +1583,60 @@ extern "C" int snapi_hequilibrium(const snapi_chem_table *table, int ncol, int n
 done:
   hipFree(dT); hipFree(dlt); hipFree(dNt); hipFree(dne); hipFree(dkr); hipFree(drho);
   hipFree(dst); hipFree(dmo); hipFree(dlv); hipFree(dit);
+  return rc;
+}
+
+extern "C" int snapi_hequilibrium_tangent_dev(const snapi_chem_table *table, int ncol, int nd, const double *T_dev,
+                                              const double *logtau_dev, const double *Nt_dev, int ndir,
+                                              const double *dT_dev, double *dNt_dev, void *stream) {
+  if (!table || ncol <= 0 || nd <= 0 || ndir < 0 || !T_dev || !logtau_dev || !Nt_dev) return SNAPI_ERR_ARG;
+  if (ndir > 0 && (!dT_dev || !dNt_dev)) return SNAPI_ERR_ARG;
+  if (!(table->lambda_ref > 0.0)) return SNAPI_ERR_ARG;
+  ChemDerived d;
+  int rc = derive(table, d);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  void *buf = nullptr;
+  double *rec = nullptr;
+  rc = upload_table(table, d, &buf, st);
+  if (rc) return rc;
+  if (hipMalloc((void **)&rec, (size_t)ncol * nd * 5 * sizeof(double)) != hipSuccess) {
+    hipFree(buf);
+    return SNAPI_ERR_DEVICE;
+  }
+  rc = launch_he_tangent((const snapi_chem_table *)buf,
+                         (const ChemDerived *)((const char *)buf + sizeof(snapi_chem_table)), d.nslot, ncol, nd, T_dev,
+                         logtau_dev, Nt_dev, ndir, dT_dev, dNt_dev, rec, st);
+  if (hipStreamSynchronize(st) != hipSuccess && rc == SNAPI_OK) rc = SNAPI_ERR_DEVICE;
+  hipFree(rec);
+  hipFree(buf);
+  return rc;
+}
+
+extern "C" int snapi_hequilibrium_tangent(const snapi_chem_table *table, int ncol, int nd, const double *T,
+                                          const double *logtau, const double *Nt, int ndir, const double *dT,
+                                          double *dNt) {
+  if (!table || ncol <= 0 || nd <= 0 || ndir < 0 || !T || !logtau || !Nt) return SNAPI_ERR_ARG;
+  if (ndir > 0 && (!dT || !dNt)) return SNAPI_ERR_ARG;
+  const size_t np = (size_t)ncol * nd, nq = np * ndir;
+  double *dT_ = nullptr, *dlt = nullptr, *dN_ = nullptr, *ddT = nullptr, *ddN = nullptr;
+  int rc = SNAPI_OK;
+  CCHK(hipMalloc((void **)&dT_, np * 8));
+  CCHK(hipMalloc((void **)&dlt, np * 8));
+  CCHK(hipMalloc((void **)&dN_, np * 8));
+  if (nq) {
+    CCHK(hipMalloc((void **)&ddT, nq * 8));
+    CCHK(hipMalloc((void **)&ddN, nq * 8));
+    CCHK(hipMemcpy(ddT, dT, nq * 8, hipMemcpyHostToDevice));
+  }
+  CCHK(hipMemcpy(dT_, T, np * 8, hipMemcpyHostToDevice));
+  CCHK(hipMemcpy(dlt, logtau, np * 8, hipMemcpyHostToDevice));
+  CCHK(hipMemcpy(dN_, Nt, np * 8, hipMemcpyHostToDevice));
+  rc = snapi_hequilibrium_tangent_dev(table, ncol, nd, dT_, dlt, dN_, ndir, ddT, ddN, nullptr);
+  if (rc) goto done;
+  if (nq) CCHK(hipMemcpy(dNt, ddN, nq * 8, hipMemcpyDeviceToHost));
+done:
+  hipFree(dT_); hipFree(dlt); hipFree(dN_); hipFree(ddT); hipFree(ddN);
   return rc;
 }
 

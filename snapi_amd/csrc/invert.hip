@@ -50,30 +50,62 @@ struct NodeArgs {
   double ltau[SNAPI_MAX_NODES];   // node positions, log tau_500
   double scale[SNAPI_MAX_NODES];  // T 5000, vt 1e5, vs 1e5, B 1000, angles 1 (const.h:35-38)
   double step[SNAPI_MAX_NODES];   // node-response steps: vt/vs 20 cm/s, B 0.01 G, angles 1e-3 rad (const.h:27-32)
+  int analytic;                   // node responses by exact derivatives instead of the reference's differences
 };
 
-// atmospheric_interpolation for a non-temperature parameter at log tau x:
-// cubic Bezier between nodes, zero end derivatives, interior derivatives by
-// the weighted harmonic mean of the neighbouring slopes, linear (here:
-// constant) extrapolation outside the node range.
-__device__ double interp_param(const NodeArgs &na, const double *v, int code, double x) {
-  double t[SNAPI_MAX_NODES], y[SNAPI_MAX_NODES], d[SNAPI_MAX_NODES];
-  int n = 0;
-  for (int i = 0; i < na.nnode; ++i)
-    if (na.code[i] == code) {
-      t[n] = na.ltau[i];
-      y[n] = v[i];
-      ++n;
-    }
-  if (n == 0) return 0.0;
-  if (n == 1) return y[0];
-  d[0] = 0.0;
-  d[n - 1] = 0.0;
+// Forward-mode dual number: a value and its derivative along one node
+// direction.  The node interpolation and the field projection are written
+// once over V = double or Dual, so the analytic node responses follow the
+// same arithmetic as the values.
+struct Dual {
+  double v, d;
+};
+__device__ inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+__device__ inline Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+__device__ inline Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+__device__ inline Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+__device__ inline Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+__device__ inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ inline Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+// the overloads below would hide the double ones inside this namespace
+using ::acos;
+using ::atan;
+using ::cos;
+using ::sin;
+using ::sqrt;
+__device__ inline double val(double a) { return a; }
+__device__ inline double val(Dual a) { return a.v; }
+__device__ inline Dual sin(Dual a) { return {sin(a.v), cos(a.v) * a.d}; }
+__device__ inline Dual cos(Dual a) { return {cos(a.v), -sin(a.v) * a.d}; }
+__device__ inline Dual sqrt(Dual a) {
+  const double r = sqrt(a.v);
+  return {r, 0.5 * a.d / r};
+}
+__device__ inline Dual acos(Dual a) { return {acos(a.v), -a.d / sqrt(1.0 - a.v * a.v)}; }
+__device__ inline Dual atan(Dual a) { return {atan(a.v), a.d / (1.0 + a.v * a.v)}; }
+template <typename V>
+__device__ inline V constant(double c) {
+  return V(c);
+}
+template <>
+__device__ inline Dual constant<Dual>(double c) {
+  return {c, 0.0};
+}
+
+// Monotone interior node derivatives (weighted harmonic mean of the
+// neighbouring slopes, zero where they change sign) and the cubic Bezier
+// segment through them, with linear extrapolation outside the nodes.
+template <typename V>
+__device__ V bezier_nodes(const double *t, const V *y, V *d, int n, double x) {
   for (int i = 1; i < n - 1; ++i) {
     const double h1 = t[i] - t[i - 1], h2 = t[i + 1] - t[i];
-    const double d1 = (y[i] - y[i - 1]) / h1, d2 = (y[i + 1] - y[i]) / h2;
-    d[i] = 0.0;
-    if (d1 * d2 > 0) {
+    const V d1 = (y[i] - y[i - 1]) / h1, d2 = (y[i + 1] - y[i]) / h2;
+    d[i] = constant<V>(0.0);
+    if (val(d1 * d2) > 0) {
       const double al = 0.3333333 * (1.0 + h2 / (h1 + h2));
       d[i] = d1 * d2 / (al * d2 + (1.0 - al) * d1);
     }
@@ -84,10 +116,32 @@ __device__ double interp_param(const NodeArgs &na, const double *v, int code, do
   for (ii = 0; ii < n - 1; ++ii)
     if (x <= t[ii + 1]) break;
   const double hh = t[ii + 1] - t[ii];
-  const double E = y[ii] + hh / 3.0 * d[ii], F = y[ii + 1] - hh / 3.0 * d[ii + 1];
+  const V E = y[ii] + hh / 3.0 * d[ii], F = y[ii + 1] - hh / 3.0 * d[ii + 1];
   const double u = (x - t[ii]) / hh;
   return (1.0 - u) * (1.0 - u) * (1.0 - u) * y[ii] + u * u * u * y[ii + 1] + 3.0 * u * (1.0 - u) * (1.0 - u) * E +
          3.0 * u * u * (1.0 - u) * F;
+}
+
+// atmospheric_interpolation for a non-temperature parameter at log tau x:
+// cubic Bezier between nodes, zero end derivatives, interior derivatives by
+// the weighted harmonic mean of the neighbouring slopes, linear (here:
+// constant) extrapolation outside the node range.
+template <typename V>
+__device__ V interp_param(const NodeArgs &na, const V *v, int code, double x) {
+  double t[SNAPI_MAX_NODES];
+  V y[SNAPI_MAX_NODES], d[SNAPI_MAX_NODES];
+  int n = 0;
+  for (int i = 0; i < na.nnode; ++i)
+    if (na.code[i] == code) {
+      t[n] = na.ltau[i];
+      y[n] = v[i];
+      ++n;
+    }
+  if (n == 0) return constant<V>(0.0);
+  if (n == 1) return y[0];
+  d[0] = constant<V>(0.0);
+  d[n - 1] = constant<V>(0.0);
+  return bezier_nodes(t, y, d, n, x);
 }
 
 // Temperature gradient of the FAL-C model against log tau near the bottom
@@ -136,8 +190,10 @@ __device__ double interpol_1d(const double *f, const double *x, int N, double xi
 // from the FAL-C table at the deepest node (its end values beyond the
 // table), the Bezier segments of interp_param, clamped to 3400..20000 K; one
 // node gives a constant.
-__device__ double interp_temp(const NodeArgs &na, const double *v, double x) {
-  double t[SNAPI_MAX_NODES], y[SNAPI_MAX_NODES], d[SNAPI_MAX_NODES];
+template <typename V>
+__device__ V interp_temp(const NodeArgs &na, const V *v, double x) {
+  double t[SNAPI_MAX_NODES];
+  V y[SNAPI_MAX_NODES], d[SNAPI_MAX_NODES];
   int n = 0;
   for (int i = 0; i < na.nnode; ++i)
     if (na.code[i] == NP_T) {
@@ -145,37 +201,16 @@ __device__ double interp_temp(const NodeArgs &na, const double *v, double x) {
       y[n] = v[i];
       ++n;
     }
-  if (n == 0) return 0.0;
+  if (n == 0) return constant<V>(0.0);
   if (n == 1) return y[0];
-  d[0] = 0.0;
+  d[0] = constant<V>(0.0);
   const double tb = t[n - 1];
-  d[n - 1] = tb < kFalcTau[0] ? kFalcDT[0] : tb > kFalcTau[10] ? kFalcDT[10] : interpol_1d(kFalcDT, kFalcTau, 11, tb);
-  for (int i = 1; i < n - 1; ++i) {
-    const double h1 = t[i] - t[i - 1], h2 = t[i + 1] - t[i];
-    const double d1 = (y[i] - y[i - 1]) / h1, d2 = (y[i + 1] - y[i]) / h2;
-    d[i] = 0.0;
-    if (d1 * d2 > 0) {
-      const double al = 0.3333333 * (1.0 + h2 / (h1 + h2));
-      d[i] = d1 * d2 / (al * d2 + (1.0 - al) * d1);
-    }
-  }
-  double q;
-  if (x < t[0]) {
-    q = y[0] + d[0] * (x - t[0]);
-  } else if (x > t[n - 1]) {
-    q = y[n - 1] + d[n - 1] * (x - t[n - 1]);
-  } else {
-    int ii = 0;
-    for (ii = 0; ii < n - 1; ++ii)
-      if (x <= t[ii + 1]) break;
-    const double hh = t[ii + 1] - t[ii];
-    const double E = y[ii] + hh / 3.0 * d[ii], F = y[ii + 1] - hh / 3.0 * d[ii + 1];
-    const double u = (x - t[ii]) / hh;
-    q = (1.0 - u) * (1.0 - u) * (1.0 - u) * y[ii] + u * u * u * y[ii + 1] + 3.0 * u * (1.0 - u) * (1.0 - u) * E +
-        3.0 * u * u * (1.0 - u) * F;
-  }
-  if (q < 3400.0) q = 3400.0;
-  if (q > 20000.0) q = 20000.0;
+  d[n - 1] = constant<V>(tb < kFalcTau[0]    ? kFalcDT[0]
+                         : tb > kFalcTau[10] ? kFalcDT[10]
+                                             : interpol_1d(kFalcDT, kFalcTau, 11, tb));
+  V q = bezier_nodes(t, y, d, n, x);
+  if (val(q) < 3400.0) q = constant<V>(3400.0);
+  if (val(q) > 20000.0) q = constant<V>(20000.0);
   return q;
 }
 
@@ -183,11 +218,14 @@ __device__ double interp_temp(const NodeArgs &na, const double *v, double x) {
 // and projects them for the mu = 1 ray: B = |B| (sin th cos ph, sin th sin
 // ph, cos th), inclination acos(Bz/|B|), azimuth atan(By/Bx) (atmos/
 // atmos_rad.cc:25-51).  out: vt, vlos, |B|, inclination [deg], azimuth [deg].
-struct DepthFields {
-  double vt, vlos, bm, inc, azi;
+template <typename V>
+struct DepthFieldsT {
+  V vt, vlos, bm, inc, azi;
 };
-__device__ DepthFields fields_at(const NodeArgs &na, const double *v, double x) {
-  DepthFields f;
+using DepthFields = DepthFieldsT<double>;
+template <typename V>
+__device__ DepthFieldsT<V> fields_at(const NodeArgs &na, const V *v, double x) {
+  DepthFieldsT<V> f;
   f.vt = interp_param(na, v, NP_VT, x);
   f.vlos = interp_param(na, v, NP_VS, x);
   // build_from_nodes (atmos_pp.cc:121-167): a field without nodes is 1e-3 G
@@ -198,11 +236,11 @@ __device__ DepthFields fields_at(const NodeArgs &na, const double *v, double x) 
     ht |= na.code[i] == NP_THETA;
     hp |= na.code[i] == NP_PHI;
   }
-  const double B = hb ? interp_param(na, v, NP_B, x) : 1E-3;
-  double th = ht ? interp_param(na, v, NP_THETA, x) : 1E-3;
-  if (fabs(th) < 1E-3) th = 1E-3;
-  const double ph = hp ? interp_param(na, v, NP_PHI, x) : 1E-3;
-  const double bx = B * sin(th) * cos(ph), by = B * sin(th) * sin(ph), bz = B * cos(th);
+  const V B = hb ? interp_param(na, v, NP_B, x) : constant<V>(1E-3);
+  V th = ht ? interp_param(na, v, NP_THETA, x) : constant<V>(1E-3);
+  if (fabs(val(th)) < 1E-3) th = constant<V>(1E-3);
+  const V ph = hp ? interp_param(na, v, NP_PHI, x) : constant<V>(1E-3);
+  const V bx = B * sin(th) * cos(ph), by = B * sin(th) * sin(ph), bz = B * cos(th);
   f.bm = sqrt(bx * bx + by * by + bz * bz);
   f.inc = acos(bz / f.bm) * 180.0 / kPi;
   f.azi = atan(by / bx) * 180.0 / kPi;
@@ -227,7 +265,18 @@ __global__ void nodes_to_state_kernel(NodeArgs na, int nslot, int nd, int nf, co
     st[SNAPI_F_BMAG * nd + z] = (R)f.bm;
     st[SNAPI_F_BINC * nd + z] = (R)f.inc;
     st[SNAPI_F_BAZI * nd + z] = (R)f.azi;
-    if (nmap) {
+    if (nmap && na.analytic) {
+      // d(vt + vz + |B| + inc + azi [rad]) / dnode, one dual pass per node
+      Dual vd[SNAPI_MAX_NODES];
+      for (int i = 0; i < na.nnode; ++i) vd[i] = Dual{v[i], 0.0};
+      for (int n = 0; n < na.nnode; ++n) {
+        vd[n].d = 1.0;
+        const DepthFieldsT<Dual> a = fields_at(na, (const Dual *)vd, x);
+        vd[n].d = 0.0;
+        const double dq = a.vt.d + a.vlos.d + a.bm.d + (a.inc.d + a.azi.d) * kPi / 180.0;
+        nmap[((size_t)slot * na.nnode + n) * nd + z] = (R)dq;
+      }
+    } else if (nmap) {
       double vv[SNAPI_MAX_NODES];
       for (int i = 0; i < na.nnode; ++i) vv[i] = v[i];
       for (int n = 0; n < na.nnode; ++n) {
@@ -502,6 +551,48 @@ __global__ void tmaps_kernel(NodeArgs na, int npix, int nd, int nf, int nvar, co
   }
 }
 
+// Analytic node responses of the temperature nodes: dT(z)/dnode, the
+// interpolation's exact derivative at the given nodes, into dT
+// [pix][ntn][nd] (for the hydrostatic tangent) and the node maps' rows.
+// ltv [pix][nd]: log tau of each depth.
+template <typename R>
+__global__ void tderiv_kernel(NodeArgs na, int npix, int nd, int ntn, const double *__restrict__ nodes,
+                              const double *__restrict__ ltv, double *__restrict__ dT, R *__restrict__ nmap,
+                              const int *__restrict__ active) {
+  const int pix = blockIdx.x;
+  if (active && !active[pix]) return;
+  Dual vd[SNAPI_MAX_NODES];
+  for (int i = 0; i < na.nnode; ++i) vd[i] = Dual{nodes[(size_t)pix * na.nnode + i], 0.0};
+  for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+    const double x = ltv[(size_t)pix * nd + z];
+    int j = 0;
+    for (int n = 0; n < na.nnode; ++n) {
+      if (na.code[n] != NP_T) continue;
+      vd[n].d = 1.0;
+      const Dual T = interp_temp(na, (const Dual *)vd, x);
+      vd[n].d = 0.0;
+      dT[((size_t)pix * ntn + j) * nd + z] = T.d;
+      nmap[((size_t)pix * na.nnode + n) * nd + z] = (R)T.d;
+      ++j;
+    }
+  }
+}
+
+// dNt/dnode rows of the node maps from the hydrostatic tangent dN
+// [pix][ntn][nd]; zero for the other nodes.
+template <typename R>
+__global__ void tmap2_kernel(NodeArgs na, int npix, int nd, int ntn, const double *__restrict__ dN,
+                             R *__restrict__ nmap2, const int *__restrict__ active) {
+  const int pix = blockIdx.x;
+  if (active && !active[pix]) return;
+  for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+    int j = 0;
+    for (int n = 0; n < na.nnode; ++n)
+      nmap2[((size_t)pix * na.nnode + n) * nd + z] =
+          na.code[n] == NP_T ? (R)dN[((size_t)pix * ntn + j++) * nd + z] : R(0);
+  }
+}
+
 // The temperature and density columns of the given nodes into a double state.
 __global__ void set_tn_kernel(int nd, int nf, const double *__restrict__ Tv, const double *__restrict__ Ntv,
                               double *__restrict__ state) {
@@ -571,7 +662,10 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   std::vector<double> lam_host;  // the convolution plans its grid on the host
   int ntn = 0;
   for (int n = 0; n < nn; ++n) ntn += na.code[n] == NP_T;
-  const int nvar = 1 + 2 * ntn;
+  // hydrostatic builds per pixel for the Jacobian: the nodes and each
+  // temperature node +- delta_T/2, or the nodes alone with the tangent
+  const int nvar = na.analytic ? 1 : 1 + 2 * ntn;
+  double *dTd = nullptr, *dNd = nullptr, *hscr = nullptr;
   // caller's buffers (device variant) or copies of the host arrays
   void *own[8] = {};
   SlotState ss;
@@ -653,6 +747,11 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     HIPCHK(hipMalloc((void **)&rstate, (size_t)nslot * ns * es));
     HIPCHK(hipMalloc((void **)&rdst, (size_t)nslot * 2 * ns * es));
     HIPCHK(hipMalloc((void **)&nmap2, (size_t)nslot * nn * nd * es));
+    if (na.analytic && ntn) {
+      HIPCHK(hipMalloc((void **)&dTd, (size_t)nslot * ntn * nd * 8));
+      HIPCHK(hipMalloc((void **)&dNd, (size_t)nslot * ntn * nd * 8));
+      HIPCHK(hipMalloc((void **)&hscr, chem_he_tangent_scratch(nslot, nd)));
+    }
     if (th->broadening > 0.0) {
       lam_host.resize(nl);
       HIPCHK(hipMemcpy(lam_host.data(), dev ? lam_h : (const double *)lam_h, (size_t)nl * 8,
@@ -731,8 +830,20 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
                            Tv, ltv, ss.needj);
         rc = chem_hequilibrium(cdev_he, (long)nslot * nvar, nd, Tv, ltv, Ntv, ss.needj, nvar, st);
         if (rc) goto done;
-        hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, nvar, Tv, Ntv, sstate,
-                           nmap, nmap2, sd, ss.needj);
+        if (na.analytic) {
+          if (ntn) {
+            hipLaunchKernelGGL(tderiv_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, snodes, ltv, dTd,
+                               nmap, ss.needj);
+            rc = chem_he_tangent(cdev_he, nslot, nd, Tv, ltv, Ntv, ntn, dTd, dNd, hscr, ss.needj, st);
+            if (rc) goto done;
+          }
+          hipLaunchKernelGGL(tmap2_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, dNd, nmap2, ss.needj);
+          hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, 1, Tv, Ntv, sstate,
+                             (R *)nullptr, (R *)nullptr, sd, ss.needj);
+        } else {
+          hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, nvar, Tv, Ntv, sstate,
+                             nmap, nmap2, sd, ss.needj);
+        }
         rc = chem_eos(cdev, nslot, nd, sd, rsd, dsd, escr, ss.needj, st);
         if (rc) goto done;
         hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, rsd, rstate);
@@ -833,6 +944,7 @@ done:
   hipFree(hv); hipFree(dbl); hipFree(ints); hipFree(queue);
   chem_free(cdev); chem_free(cdev_he);
   hipFree(Tv); hipFree(ltv); hipFree(Ntv); hipFree(sd); hipFree(rsd); hipFree(dsd); hipFree(tsd); hipFree(escr);
+  hipFree(dTd); hipFree(dNd); hipFree(hscr);
   hipFree(rstate); hipFree(rdst); hipFree(nmap2);
   return rc;
 }
@@ -943,9 +1055,11 @@ static int node_jacobian_eos_impl(int precision, const snapi_line_table *lines, 
   if (nf != SNAPI_NF_BASE + 2 * map->nline) return SNAPI_ERR_ARG;
   int ntn = 0;
   for (int n = 0; n < nn; ++n) ntn += na.code[n] == NP_T;
-  const int nvar = 1 + 2 * ntn;
+  const int nvar = na.analytic ? 1 : 1 + 2 * ntn;
   const size_t es = sizeof(R), ns = (size_t)nf * nd;
   for (int n = 0; n < nn; ++n) na.scale[n] = 1.0;  // unscaled derivatives
+  ChemDevice cdev_he;
+  double *dTd = nullptr, *dNd = nullptr, *hscr = nullptr;
   R *dstate = nullptr, *rstate = nullptr, *rdst = nullptr, *S = nullptr, *J = nullptr, *nmap = nullptr,
     *nmap2 = nullptr, *scr = nullptr;
   double *dlam = nullptr, *dnodes = nullptr, *Tv = nullptr, *ltv = nullptr, *Ntv = nullptr, *sd = nullptr,
@@ -981,8 +1095,26 @@ static int node_jacobian_eos_impl(int precision, const snapi_line_table *lines, 
   rc = snapi_hequilibrium_dev(chem_he, npix * nvar, nd, Tv, ltv, Ntv, nullptr, nullptr, nullptr, nullptr, nullptr,
                               nullptr, nullptr, nullptr);
   if (rc) goto done;
-  hipLaunchKernelGGL(tmaps_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, nf, nvar, Tv, Ntv, dstate, nmap,
-                     nmap2, sd, (const int *)nullptr);
+  if (na.analytic) {
+    if (ntn) {
+      HIPCHK(hipMalloc((void **)&dTd, (size_t)npix * ntn * nd * 8));
+      HIPCHK(hipMalloc((void **)&dNd, (size_t)npix * ntn * nd * 8));
+      HIPCHK(hipMalloc((void **)&hscr, chem_he_tangent_scratch(npix, nd)));
+      rc = chem_upload(chem_he, nullptr, cdev_he, 0);
+      if (rc) goto done;
+      hipLaunchKernelGGL(tderiv_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, ntn, dnodes, ltv, dTd, nmap,
+                         (const int *)nullptr);
+      rc = chem_he_tangent(cdev_he, npix, nd, Tv, ltv, Ntv, ntn, dTd, dNd, hscr, nullptr, 0);
+      if (rc) goto done;
+    }
+    hipLaunchKernelGGL(tmap2_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, ntn, dNd, nmap2,
+                       (const int *)nullptr);
+    hipLaunchKernelGGL(tmaps_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, nf, 1, Tv, Ntv, dstate,
+                       (R *)nullptr, (R *)nullptr, sd, (const int *)nullptr);
+  } else {
+    hipLaunchKernelGGL(tmaps_kernel<R>, dim3(npix), dim3(128), 0, 0, na, npix, nd, nf, nvar, Tv, Ntv, dstate, nmap,
+                       nmap2, sd, (const int *)nullptr);
+  }
   rc = snapi_eos_state_dev(chem, map, npix, nd, sd, rsd, dsd, nullptr);
   if (rc) goto done;
   hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, 0, (size_t)npix * ns, rsd, rstate);
@@ -1018,7 +1150,8 @@ static int node_jacobian_eos_impl(int precision, const snapi_line_table *lines, 
 done:
   hipFree(dstate); hipFree(rstate); hipFree(rdst); hipFree(S); hipFree(J); hipFree(nmap); hipFree(nmap2);
   hipFree(scr); hipFree(dlam); hipFree(dnodes); hipFree(Tv); hipFree(ltv); hipFree(Ntv); hipFree(sd);
-  hipFree(rsd); hipFree(dsd);
+  hipFree(rsd); hipFree(dsd); hipFree(dTd); hipFree(dNd); hipFree(hscr);
+  chem_free(cdev_he);
   return rc;
 }
 
@@ -1026,8 +1159,12 @@ done:
 
 using namespace snapi;
 
+// Checks the node model and fills na; strips SNAPI_JAC_ANALYTIC_MAPS from
+// jacobian into na.analytic.
 static int setup_nodes(const snapi_line_table *lines, const snapi_ray *ray, const snapi_node_model *model, int npix,
-                       int nd, int nlambda, int jacobian, int max_iter, NodeArgs &na, bool temperature = false) {
+                       int nd, int nlambda, int &jacobian, int max_iter, NodeArgs &na, bool temperature = false) {
+  const int analytic = (jacobian & SNAPI_JAC_ANALYTIC_MAPS) != 0;
+  jacobian &= ~SNAPI_JAC_ANALYTIC_MAPS;
   if (!lines || !ray || !model) return SNAPI_ERR_ARG;
   if (npix <= 0 || nd < 3 || nlambda <= 0 || max_iter <= 0) return SNAPI_ERR_ARG;
   if (model->nnode <= 0 || model->nnode > SNAPI_MAX_NODES - 1) return SNAPI_ERR_ARG;
@@ -1036,6 +1173,7 @@ static int setup_nodes(const snapi_line_table *lines, const snapi_ray *ray, cons
   if (cos(ray->theta) > -0.999999) return SNAPI_ERR_UNSUPPORTED;  // node responses assume the mu = 1 ray
   na = NodeArgs{};
   na.nnode = model->nnode;
+  na.analytic = analytic;
   for (int n = 0; n < na.nnode; ++n) {
     const int c = model->param[n];
     // temperature nodes need the equation of state (the _eos entry points)

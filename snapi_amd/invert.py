@@ -118,11 +118,14 @@ def node_jacobian(table, ray, model, state, nodes, lam_vac, precision=abi.SNAPI_
     return stokes, J
 
 
-def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_vac, precision=abi.SNAPI_FP64):
+def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_vac, precision=abi.SNAPI_FP64,
+                      analytic_maps=False):
     """(stokes, J) of snapi_node_jacobian_eos: temperature nodes allowed, the
     thermodynamic fields of state rebuilt from the nodes (hydrostatic
     equilibrium with chem_he, equation of state with chem and smap); adjoint
-    Jacobian; the Stokes vector of the response path."""
+    Jacobian; the Stokes vector of the response path.  analytic_maps: node
+    responses by exact derivatives and the hydrostatic tangent
+    (SNAPI_JAC_ANALYTIC_MAPS) instead of the reference's differences."""
     dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
     st = np.ascontiguousarray(state, dtype=dt)
     if st.ndim == 2:
@@ -140,7 +143,8 @@ def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_
     f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay),
                   C.POINTER(SnapiNodeModel), TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p,
                   C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
-    rc = f(precision, 0, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he),
+    jac = abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0
+    rc = f(precision, jac, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he),
            C.byref(smap), npix, nd, abi.vptr(st), abi.ptr(nv), len(lam), abi.ptr(lam), abi.vptr(stokes), abi.vptr(J))
     check("snapi_node_jacobian_eos", rc)
     return stokes, J
@@ -148,9 +152,10 @@ def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_
 
 def invert_eos(table, ray, model, chem, chem_he, smap, state, lam_vac, obs, nodes0, mask=None,
                wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0, precision=abi.SNAPI_FP64,
-               broadening=0.0):
+               broadening=0.0, analytic_maps=False):
     """snapi_invert_eos: the inversion with temperature nodes (adjoint
-    Jacobian). Returns (nodes, chi2, iterations, total pixel-iterations)."""
+    Jacobian; analytic_maps as node_jacobian_eos). Returns (nodes, chi2,
+    iterations, total pixel-iterations)."""
     dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
     st = np.ascontiguousarray(state, dtype=dt)
     npix, nf, nd = st.shape
@@ -169,7 +174,8 @@ def invert_eos(table, ray, model, chem, chem_he, smap, state, lam_vac, obs, node
                   TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double),
                   C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
-    rc = f(precision, 0, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
+    jac = abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0
+    rc = f(precision, jac, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
            npix, nd, abi.vptr(st), len(lam), abi.ptr(lam), abi.vptr(ob), None if mk is None else abi.vptr(mk),
            abi.ptr(ws), float(broadening), lm_start, max_iter, nslot, abi.ptr(nodes), abi.ptr(chi2),
            iters.ctypes.data_as(C.POINTER(C.c_int)), C.byref(total))
@@ -210,7 +216,7 @@ def build_from_nodes_dev(model, chem, chem_he, smap, state_t, nodes_t, stream=No
 
 def invert_eos_dev(table, ray, model, chem, chem_he, smap, state_t, lam_t, obs_t, mask_t, nodes0_t, nodes_t, chi2_t,
                    iters_t, wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0,
-                   precision=abi.SNAPI_MIXED, stream=None, broadening=0.0):
+                   precision=abi.SNAPI_MIXED, stream=None, broadening=0.0, analytic_maps=False):
     """snapi_invert_eos_dev on torch device tensors; returns the total
     pixel-iterations."""
     npix, nf, nd = state_t.shape
@@ -225,7 +231,8 @@ def invert_eos_dev(table, ray, model, chem, chem_he, smap, state_t, lam_t, obs_t
                   C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]
     sp = None if stream is None else C.c_void_p(stream.cuda_stream)
-    rc = f(precision, 0, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
+    jac = abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0
+    rc = f(precision, jac, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
            npix, nd, state_t.data_ptr(), nl, lam_t.data_ptr(), obs_t.data_ptr(),
            None if mask_t is None else mask_t.data_ptr(), abi.ptr(ws), float(broadening), lm_start, max_iter, nslot,
            nodes0_t.data_ptr(), nodes_t.data_ptr(), chi2_t.data_ptr(), iters_t.data_ptr(), C.byref(total), sp)

@@ -132,33 +132,19 @@ def _interp_param(t, y, x):
     non-temperature parameter: zero end derivatives, weighted harmonic-mean
     interior derivatives, cubic Bezier segments."""
     n = len(t)
+    dt = np.result_type(np.asarray(y), np.float64)  # complex node values: complex-step derivatives
     if n == 0:
-        return np.zeros_like(x)
+        return np.zeros(x.shape, dt)
     if n == 1:
-        return np.full_like(x, y[0])
-    d = np.zeros(n)
+        return np.full(x.shape, y[0], dt)
+    d = np.zeros(n, dt)
     for i in range(1, n - 1):
         h1, h2 = t[i] - t[i - 1], t[i + 1] - t[i]
         d1, d2 = (y[i] - y[i - 1]) / h1, (y[i + 1] - y[i]) / h2
-        if d1 * d2 > 0:
+        if (d1 * d2).real > 0:
             al = 0.3333333 * (1.0 + h2 / (h1 + h2))
             d[i] = d1 * d2 / (al * d2 + (1.0 - al) * d1)
-    out = np.empty_like(x)
-    for j, xx in enumerate(x):
-        if xx < t[0]:
-            out[j] = y[0] + d[0] * (xx - t[0])
-        elif xx > t[-1]:
-            out[j] = y[-1] + d[-1] * (xx - t[-1])
-        else:
-            ii = 0
-            while ii < n - 1 and not xx <= t[ii + 1]:
-                ii += 1
-            hh = t[ii + 1] - t[ii]
-            E = y[ii] + hh / 3.0 * d[ii]
-            F = y[ii + 1] - hh / 3.0 * d[ii + 1]
-            u = (xx - t[ii]) / hh
-            out[j] = (1 - u) ** 3 * y[ii] + u ** 3 * y[ii + 1] + 3 * u * (1 - u) ** 2 * E + 3 * u * u * (1 - u) * F
-    return out
+    return _bezier(t, y, d, x)
 
 
 _FALC_DT = [1694.91525424, 2164.5021645, 2342.50815695, 2289.18827635, 2155.92183045, 2094.90240415,
@@ -262,29 +248,31 @@ def _interp_temp(t, y, x):
     table), the same Bezier segments as the other parameters, then the
     result clamped to 3400..20000 K; one node gives a constant."""
     n = len(t)
+    dt = np.result_type(np.asarray(y), np.float64)
     if n == 1:
-        return np.full_like(x, y[0])
+        return np.full(x.shape, y[0], dt)
     if t[-1] < _FALC_TAU[0]:
         db = _FALC_DT[0]
     elif t[-1] > _FALC_TAU[-1]:
         db = _FALC_DT[-1]
     else:
         db = _interpol_1d(_FALC_DT, _FALC_TAU, t[-1])
-    d = np.zeros(n)
+    d = np.zeros(n, dt)
     d[-1] = db
     for i in range(1, n - 1):
         h1, h2 = t[i] - t[i - 1], t[i + 1] - t[i]
         d1, d2 = (y[i] - y[i - 1]) / h1, (y[i + 1] - y[i]) / h2
-        if d1 * d2 > 0:
+        if (d1 * d2).real > 0:
             al = 0.3333333 * (1.0 + h2 / (h1 + h2))
             d[i] = d1 * d2 / (al * d2 + (1.0 - al) * d1)
     out = _bezier(t, y, d, x)
-    return np.clip(out, 3400.0, 20000.0)
+    out = np.where(out.real < 3400.0, 3400.0, out)
+    return np.where(out.real > 20000.0, 20000.0, out)
 
 
 def _bezier(t, y, d, x):
     n = len(t)
-    out = np.empty_like(x)
+    out = np.empty(x.shape, np.result_type(np.asarray(y), np.asarray(d), np.float64))
     for j, xx in enumerate(x):
         if xx < t[0]:
             out[j] = y[0] + d[0] * (xx - t[0])
@@ -309,6 +297,18 @@ def nodes_to_state(model_nodes, values, state):
     from snapi_amd import abi
     st = state.copy()
     x = np.log10(-st[abi.F_GRID])
+    vt, vlos, bm, inc, azi = _node_fields(model_nodes, values, x)
+    st[abi.F_VT] = vt
+    st[abi.F_VLOS] = vlos
+    st[abi.F_BMAG] = bm
+    st[abi.F_BINC] = inc
+    st[abi.F_BAZI] = azi
+    return st
+
+
+def _node_fields(model_nodes, values, x):
+    """vt, vLOS, |B|, inclination [deg], azimuth [deg] at log tau x from the
+    node values (complex values give complex-step derivatives)."""
     q = {}
     for name in ("vt", "vlos", "B", "theta", "phi"):
         sel = [(lt, v) for (p, lt), v in zip(model_nodes, values) if p == name]
@@ -317,17 +317,12 @@ def nodes_to_state(model_nodes, values, state):
         # nodes is 1e-3 G along 1e-3 rad; the inclination stays >= 1e-3 rad
         if not sel and name in ("B", "theta", "phi"):
             q[name] = np.full_like(x, 1e-3)
-    q["theta"] = np.where(np.abs(q["theta"]) < 1e-3, 1e-3, q["theta"])
+    q["theta"] = np.where(np.abs(q["theta"].real) < 1e-3, 1e-3, q["theta"])
     bx = q["B"] * np.sin(q["theta"]) * np.cos(q["phi"])
     by = q["B"] * np.sin(q["theta"]) * np.sin(q["phi"])
     bz = q["B"] * np.cos(q["theta"])
     bm = np.sqrt(bx * bx + by * by + bz * bz)
-    st[abi.F_VT] = q["vt"]
-    st[abi.F_VLOS] = q["vlos"]
-    st[abi.F_BMAG] = bm
-    st[abi.F_BINC] = np.degrees(np.arccos(bz / bm))
-    st[abi.F_BAZI] = np.degrees(np.arctan(by / bx))
-    return st
+    return q["vt"], q["vlos"], bm, np.arccos(bz / bm) * (180.0 / np.pi), np.arctan(by / bx) * (180.0 / np.pi)
 
 
 _NODE_FIELD = {"vt": (1, 1.0), "vlos": (2, 1.0), "B": (3, 1.0), "theta": (4, np.pi / 180.0), "phi": (5, np.pi / 180.0)}
@@ -350,6 +345,30 @@ def node_map(model_nodes, values, state):
     return out
 
 
+def node_map_analytic(model_nodes, values, state):
+    """d q_p(z) / d node_n of the node interpolation by complex-step
+    differentiation (Im f(v + i h e_n) / h, exact to rounding where the
+    interpolation is smooth), angles in radians, temperature nodes' rows
+    dT(z) / dnode.  [nnode][nd]."""
+    from snapi_amd import abi
+    x = np.log10(-state[abi.F_GRID])
+    tn = [i for i, (p, _) in enumerate(model_nodes) if p == "T"]
+    others = [(i, m) for i, m in enumerate(model_nodes) if m[0] != "T"]
+    h = 1e-30
+    out = np.zeros((len(model_nodes), state.shape[1]))
+    for n, (p, _) in enumerate(model_nodes):
+        v = np.array(values, dtype=np.complex128)
+        v[n] += 1j * h
+        if p == "T":
+            T = _interp_temp(np.array([model_nodes[i][1] for i in tn]), np.array([v[i] for i in tn]), x)
+            out[n] = T.imag / h
+        else:
+            f = _node_fields([m for _, m in others], [v[i] for i, _ in others], x)
+            k = ("vt", "vlos", "B", "theta", "phi").index(p)
+            out[n] = f[k].imag / h * (np.pi / 180.0 if k >= 3 else 1.0)
+    return out
+
+
 def node_jacobian(table, ray, model_nodes, values, state, lam, fd_scale=1.0):
     """Stokes and dS/dnode [nlambda*4][nnode] of one pixel the reference's way:
     per-depth responses (local analytic derivatives propagated by central
@@ -367,13 +386,16 @@ def node_jacobian(table, ray, model_nodes, values, state, lam, fd_scale=1.0):
     return S[0], J
 
 
-def node_jacobian_eos(table, ray, model_nodes, values, state, lam, chem, chem_he, smap, fd_scale=1.0):
+def node_jacobian_eos(table, ray, model_nodes, values, state, lam, chem, chem_he, smap, fd_scale=1.0,
+                      analytic=False):
     """Stokes and dS/dnode of one pixel with temperature nodes the
     reference's way (obs_stokes_responses_to_nodes, atmos_fit.cc:1205-1352):
     build_from_nodes (temperature interpolation, hydrostatic equilibrium),
     the response path's state and derivatives, per-depth responses, and the
     node responses: dT/dnode and dNt/dnode from rebuilds at the node
-    +- delta_T/2, the interpolation's differences for the other nodes."""
+    +- delta_T/2, the interpolation's differences for the other nodes.  With
+    analytic, the node responses are the interpolation's exact derivatives
+    (node_map_analytic) and dNt/dnode the hydrostatic tangent (he_tangent)."""
     from snapi_amd import abi
     st = nodes_to_state(model_nodes, values, state)
     x = np.log10(-st[abi.F_GRID])
@@ -395,6 +417,16 @@ def node_jacobian_eos(table, ray, model_nodes, values, state, lam, chem, chem_he
     nm_o = node_map(others, ov, state) if others else np.zeros((0, nd))
     pidx = {"vt": 2, "vlos": 3, "B": 4, "theta": 5, "phi": 6}
     J = np.zeros((len(lam) * 4, len(model_nodes)))
+    if analytic:
+        nm = node_map_analytic(model_nodes, values, state)
+        dN = he_tangent(chem_he, T0, x, N0, nm[tn]) if tn else None
+        for n, (p, _) in enumerate(model_nodes):
+            if p == "T":
+                J[:, n] = (np.einsum("zls,z->ls", R[0, 0], nm[n]) +
+                           np.einsum("zls,z->ls", R[0, 1], dN[tn.index(n)])).reshape(-1)
+            else:
+                J[:, n] = np.einsum("zls,z->ls", R[0, pidx[p]], nm[n]).reshape(-1)
+        return S[0], J
     k = 0
     for n, (p, _) in enumerate(model_nodes):
         if p == "T":
@@ -556,6 +588,8 @@ def _chem_lib():
         L.orc_hequilibrium_batch.argtypes = [TB, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_int),
                                              C.c_int]
         L.orc_hequilibrium_batch.restype = C.c_int
+        L.orc_hequilibrium_tol.argtypes = [TB, C.c_int, dp, dp, dp, C.c_double, C.c_int]
+        L.orc_he_tangent.argtypes = [TB, C.c_int, dp, dp, dp, C.c_int, dp, dp]
         L.orc_eos_state_batch.argtypes = [TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, dp, dp, dp, C.c_int]
         L._chem_ready = True
     return L
@@ -642,6 +676,27 @@ def hequilibrium(table, T, logtau, nthreads=0):
                                   it.ctypes.data_as(C.POINTER(C.c_int)), nthreads if nthreads else (os.cpu_count() or 1))
     assert rc == 0
     return (*out, it)
+
+
+def hequilibrium_tol(table, T, logtau, tol, max_iter=200):
+    """The hydrostatic march of one column with another stopping rule
+    (correction below tol of the top Nt) -> Nt [nd]."""
+    L = _chem_lib()
+    T, logtau = _c(T), _c(logtau)
+    Nt = np.zeros_like(T)
+    assert L.orc_hequilibrium_tol(C.byref(table), T.size, ptr(T), ptr(logtau), ptr(Nt), tol, max_iter) == 0
+    return Nt
+
+
+def he_tangent(table, T, logtau, Nt, dT):
+    """dNt [ndir][nd] of a converged column along temperature perturbations
+    dT [ndir][nd] (the linearised hydrostatic equations)."""
+    L = _chem_lib()
+    T, logtau, Nt = _c(T), _c(logtau), _c(Nt)
+    dT = np.ascontiguousarray(np.atleast_2d(dT), dtype=np.float64)
+    dN = np.zeros_like(dT)
+    assert L.orc_he_tangent(C.byref(table), T.size, ptr(T), ptr(logtau), ptr(Nt), dT.shape[0], ptr(dT), ptr(dN)) == 0
+    return dN
 
 
 def eos_state(table, smap, state, nthreads=0):

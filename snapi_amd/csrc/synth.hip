@@ -33,9 +33,18 @@ __global__ __launch_bounds__(kSynthThreads) void synth_kernel(const KLines lines
   if (active && !active[col / active_div]) return;  // uniform over the workgroup
   const R *st = state + (size_t)col * args.nfield * nd;
   ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
+  // the grouped wavelength order (lambda_order) in the float path; the double
+  // path, at one workgroup per CU, measured faster in index order
+  int *order = nullptr;
+  if (sizeof(R) == 4 && args.nlambda <= kOrderMax) {
+    order = reinterpret_cast<int *>(smem_raw + ColumnTile<R>::bytes(nd, lines.nline));
+    lambda_order(lines, lam, args.nlambda, reinterpret_cast<float *>(order + args.nlambda), order + 2 * args.nlambda,
+                 order);
+  }
   stage_column<R, R>(lines, args, st, tile);
   __syncthreads();
-  for (int l = threadIdx.x; l < args.nlambda; l += blockDim.x) {
+  for (int i = threadIdx.x; i < args.nlambda; i += blockDim.x) {
+    const int l = order ? order[i] : i;
     R out[4];
     int rc = solve_lambda<R>(lines, args, tile, lam[l], out);
     R *o = stokes + ((size_t)col * args.nlambda + l) * 4;
@@ -52,7 +61,7 @@ static int launch_synth(const KLines &kl, const SynthArgs &a, const void *state,
                         int *status, hipStream_t stream, const int *active = nullptr, int active_div = 1) {
   int threads = ((a.nlambda + 63) / 64) * 64;
   if (threads > kSynthThreads) threads = kSynthThreads;
-  size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline);
+  size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + (sizeof(R) == 4 && a.nlambda <= kOrderMax ? (size_t)a.nlambda * 12 : 0);
   hipLaunchKernelGGL(synth_kernel<R>, dim3(a.ncol), dim3(threads), smem, stream, kl, a,
                      static_cast<const R *>(state), lam, static_cast<R *>(stokes), status, active,
                      active_div > 0 ? active_div : 1);

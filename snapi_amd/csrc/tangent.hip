@@ -324,6 +324,13 @@ __global__ __launch_bounds__(kTanThreads) void tangent_kernel(const KLines lines
   const R *st = state + (size_t)col * nf * nd;
   ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
   DerivTile<R> dt = DerivTile<R>::carve(sm + ColumnTile<R>::bytes(nd, lines.nline) / sizeof(R), nd, lines.nline);
+  // the synthesis kernel's grouped wavelength order (float path)
+  int *order = nullptr;
+  if (sizeof(R) == 4 && nl <= kOrderMax) {
+    order = reinterpret_cast<int *>(smem_raw + ColumnTile<R>::bytes(nd, lines.nline) +
+                                    DerivTile<R>::bytes(nd, lines.nline, ng));
+    lambda_order(lines, lam, nl, reinterpret_cast<float *>(order + nl), order + 2 * nl, order);
+  }
   stage_column<R, R>(lines, args, st, tile);
   stage_deriv<R, R>(lines, args, st, dt);
   const R *nmg = static_cast<const R *>(ra.nmap) + ((size_t)col * ra.nnode + n0) * nd;
@@ -342,7 +349,8 @@ __global__ __launch_bounds__(kTanThreads) void tangent_kernel(const KLines lines
   const R mct = (R)(-args.ct);
   auto zof = [&](int k) { return up ? nd - 1 - k : k; };
   const R *mk_all = static_cast<const R *>(ra.mask);
-  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+  for (int i = threadIdx.x; i < nl; i += blockDim.x) {
+    const int l = order ? order[i] : i;
     LamConst<R> c;
     lambda_consts<R>(lines, args, lam[l], c);
     R G[NQ][11], u[NQ][4], ds0[NQ];
@@ -480,7 +488,8 @@ int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, 
   if (threads > kTanThreads) threads = kTanThreads;
   const int npass = (ra.nnode + kTanMaxNodes - 1) / kTanMaxNodes;
   const int ng = ra.nnode < kTanMaxNodes ? ra.nnode : kTanMaxNodes;
-  const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + DerivTile<R>::bytes(a.nd, kl.nline, ng);
+  const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + DerivTile<R>::bytes(a.nd, kl.nline, ng) +
+                      (sizeof(R) == 4 && a.nlambda <= kOrderMax ? (size_t)a.nlambda * 12 : 0);
   if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;
   int *status = device_status_slot();
   if (!status) return SNAPI_ERR_DEVICE;

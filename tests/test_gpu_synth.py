@@ -66,6 +66,25 @@ def test_cube_fp64_and_mixed_against_oracle(cube):
     assert err.max() < 1e-4
 
 
+def test_wavelength_order_leaves_results_unchanged(cube):
+    """The float kernels take a workgroup's wavelengths in order of their
+    nearest line and distance from it (up to 2048 wavelengths; beyond that
+    in index order).  Each wavelength is independent, so a wavelength's
+    Stokes vector is the same bits whatever else the launch holds: the cube
+    grid reversed, and the grid followed by 1,800 far points (2,101
+    wavelengths, index order)."""
+    t, st = cube
+    st32 = st[:8].astype(np.float32)
+    lam = t["lam_vac"]
+    _say("wavelength order: 8 columns, %d / reversed / %d wavelengths, mixed" % (len(lam), len(lam) + 1800))
+    base = api.synth_stokes(t["table"], t["ray"], st32, lam, abi.SNAPI_MIXED)
+    rev = api.synth_stokes(t["table"], t["ray"], st32, lam[::-1].copy(), abi.SNAPI_MIXED)
+    assert np.array_equal(base, rev[:, ::-1])
+    far = np.concatenate([lam, lam[-1] + 1e-9 * np.arange(1, 1801)])
+    big = api.synth_stokes(t["table"], t["ray"], st32, far, abi.SNAPI_MIXED)
+    assert np.array_equal(base, big[:, :len(lam)])
+
+
 def test_edge_cases_against_oracle(cfg0):
     """Two depth points, zero field, one wavelength, more wavelengths than
     threads in a block, an inclined ray, no H-, and a height grid."""

@@ -136,7 +136,7 @@ def main():
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
             "algorithmic_bytes": int(ncol * nf * nd * 4 + ncol * nlam * 16),
-            "kernel": "synth_kernel<float>", "flops_per_lambda_point": round(fpl, 1),
+            "kernel": "synth_kernel_f", "flops_per_lambda_point": round(fpl, 1),
             "ms_per_launch": round(ms, 4)}
 
     fp64 = None

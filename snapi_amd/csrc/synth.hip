@@ -21,11 +21,9 @@
 namespace snapi {
 
 template <typename R>
-__global__ __launch_bounds__(kSynthThreads) void synth_kernel(const KLines lines, const SynthArgs args,
-                                                             const R *__restrict__ state,
-                                                             const double *__restrict__ lam,
-                                                             R *__restrict__ stokes, int *__restrict__ status,
-                                                             const int *__restrict__ active, int active_div) {
+__device__ __forceinline__ void synth_body(const KLines &lines, const SynthArgs &args, const R *__restrict__ state,
+                                           const double *__restrict__ lam, R *__restrict__ stokes,
+                                           int *__restrict__ status, const int *__restrict__ active, int active_div) {
   extern __shared__ unsigned char smem_raw[];
   R *sm = reinterpret_cast<R *>(smem_raw);
   const int nd = args.nd;
@@ -56,13 +54,42 @@ __global__ __launch_bounds__(kSynthThreads) void synth_kernel(const KLines lines
   }
 }
 
+// The float kernel is held to eight wavefronts per SIMD (64 VGPRs, 85
+// spilled to scratch), which hides the latency of its LDS reads,
+// transcendentals and branches far better than four at 119 VGPRs; on the
+// 512 x 512 x 71 cube: 4 waves 301 ms, 5 waves 242 ms, 6 waves 210 ms,
+// 8 waves 185 ms.  The double kernel keeps its registers (253 VGPRs; five
+// wavefronts would spill 263 and measured 32 % slower).
+template <typename R> struct SynthKernel;
+__global__ __launch_bounds__(kSynthThreads, 8) void synth_kernel_f(const KLines lines, const SynthArgs args,
+                                                                  const float *__restrict__ state,
+                                                                  const double *__restrict__ lam,
+                                                                  float *__restrict__ stokes,
+                                                                  int *__restrict__ status,
+                                                                  const int *__restrict__ active, int active_div) {
+  synth_body<float>(lines, args, state, lam, stokes, status, active, active_div);
+}
+__global__ __launch_bounds__(kSynthThreads) void synth_kernel_d(const KLines lines, const SynthArgs args,
+                                                               const double *__restrict__ state,
+                                                               const double *__restrict__ lam,
+                                                               double *__restrict__ stokes, int *__restrict__ status,
+                                                               const int *__restrict__ active, int active_div) {
+  synth_body<double>(lines, args, state, lam, stokes, status, active, active_div);
+}
+template <> struct SynthKernel<float> {
+  static constexpr auto fn = synth_kernel_f;
+};
+template <> struct SynthKernel<double> {
+  static constexpr auto fn = synth_kernel_d;
+};
+
 template <typename R>
 static int launch_synth(const KLines &kl, const SynthArgs &a, const void *state, const double *lam, void *stokes,
                         int *status, hipStream_t stream, const int *active = nullptr, int active_div = 1) {
   int threads = ((a.nlambda + 63) / 64) * 64;
   if (threads > kSynthThreads) threads = kSynthThreads;
   size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + (sizeof(R) == 4 && a.nlambda <= kOrderMax ? (size_t)a.nlambda * 12 : 0);
-  hipLaunchKernelGGL(synth_kernel<R>, dim3(a.ncol), dim3(threads), smem, stream, kl, a,
+  hipLaunchKernelGGL(SynthKernel<R>::fn, dim3(a.ncol), dim3(threads), smem, stream, kl, a,
                      static_cast<const R *>(state), lam, static_cast<R *>(stokes), status, active,
                      active_div > 0 ? active_div : 1);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;

@@ -162,8 +162,14 @@ template <> struct AliNorm<float> {
 // last ray; the additions happen in the same order either way.  kL and kB
 // bound the atom's lines and bound-free transitions the work-items carry in
 // registers, kS (32 or 64) the reduced slots per depth.
+// Wavefronts per SIMD the float kernel is held to (128 VGPRs, 49 spilled
+// in the 32-slot variant); the double ones keep their registers.  On 4096
+// columns x 81 depths x 220 wavelengths x 10 rays: 2 waves (188 VGPRs)
+// 5.99 s, 4 waves 5.12 s, 8 waves 8.98 s.
+template <typename R> constexpr int kAliWaves = 1;
+template <> constexpr int kAliWaves<float> = 4;
 template <typename R, bool kLds, int kL, int kB, int kS>
-__global__ __launch_bounds__(kAliThreads) void ali_acc_kernel(const AliAtom A, const AliGridArgs G, int nd, AliCol P) {
+__global__ __launch_bounds__(kAliThreads, kAliWaves<R>) void ali_acc_kernel(const AliAtom A, const AliGridArgs G, int nd, AliCol P) {
   extern __shared__ unsigned char smem_raw[];
   const int col = blockIdx.x;
   if (P.active && !P.active[col]) return;  // uniform over the workgroup

@@ -479,7 +479,8 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
 // The float kernel is held to two wavefronts per SIMD (256 registers, some
 // spilled): its 392 registers would otherwise leave one wavefront per SIMD,
 // and the sweep waits on memory and LDS; measured 1.34x faster on the
-// inversion workload.  The double kernel keeps its registers.
+// inversion workload (four wavefronts, 128 VGPRs with 393 spilled, measured
+// 0.7x).  The double kernel keeps its registers.
 __global__ __launch_bounds__(kTanThreads, 2) void tangent_kernel_f(const KLines lines, const SynthArgs args,
                                                                   const RespArgs ra, const float *__restrict__ state,
                                                                   const double *__restrict__ lam,

@@ -162,8 +162,9 @@ template <> struct AliNorm<float> {
 // last ray; the additions happen in the same order either way.  kL and kB
 // bound the atom's lines and bound-free transitions the work-items carry in
 // registers, kS (32 or 64) the reduced slots per depth.
-// Wavefronts per SIMD the float kernel is held to (128 VGPRs, 49 spilled
-// in the 32-slot variant); the double ones keep their registers.  On 4096
+// Wavefronts per SIMD the float kernel is held to (128 VGPRs: 49 spilled in
+// the 32-slot variant, 159 in the 64-slot one); the double ones keep their
+// registers.  On 4096
 // columns x 81 depths x 220 wavelengths x 10 rays: 2 waves (188 VGPRs)
 // 5.99 s, 4 waves 5.12 s, 8 waves 8.98 s.
 template <typename R> constexpr int kAliWaves = 1;

@@ -166,7 +166,7 @@ template <> struct AliNorm<float> {
 // the 32-slot variant, 159 in the 64-slot one); the double ones keep their
 // registers.  On 4096
 // columns x 81 depths x 220 wavelengths x 10 rays: 2 waves (188 VGPRs)
-// 5.99 s, 4 waves 5.12 s, 8 waves 8.98 s.
+// 5.99 s, 4 waves 5.12 s, 5 waves 7.17 s, 8 waves 8.98 s.
 template <typename R> constexpr int kAliWaves = 1;
 template <> constexpr int kAliWaves<float> = 4;
 template <typename R, bool kLds, int kL, int kB, int kS>

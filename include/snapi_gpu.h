@@ -62,7 +62,8 @@ enum snapi_status {
   SNAPI_ERR_ARG = -1,         /* bad sizes or null pointers */
   SNAPI_ERR_DEVICE = -2,      /* HIP runtime error */
   SNAPI_ERR_UNSUPPORTED = -3, /* input outside what the kernels implement */
-  SNAPI_ERR_NONFINITE = -4    /* a result was NaN/Inf */
+  SNAPI_ERR_NONFINITE = -4,   /* a result was NaN/Inf */
+  SNAPI_ERR_CAPACITY = -5     /* the problem needs more than a fixed-size work area holds */
 };
 
 /* Precision of the compute path.
@@ -149,6 +150,35 @@ int snapi_version(void);
 int snapi_device_count(void);
 
 /*
+ * Kernel status.  The kernels report what they cannot compute (a regime the
+ * build does not implement, bit 1 -> SNAPI_ERR_UNSUPPORTED) and non-finite
+ * results (bit 2 -> SNAPI_ERR_NONFINITE) by OR-ing bits into a status word on
+ * the device.  The host-pointer entry points clear the word before their
+ * launches and return what it holds after them.  The *_dev entry points only
+ * launch, so their callers read the word when their stream's work is done.
+ *
+ * A caller owns a word by creating a context on the current device and
+ * binding it to the calling thread: every launch that thread makes on that
+ * device then reports into the context's word, and no other caller's calls
+ * touch it.  Without a bound context the launches report into a word of the
+ * device, which snapi_get_status reads.  No counterpart in the reference,
+ * whose functions log and return an int (or exit) at the point of failure.
+ *   snapi_context_create: a context (and its zeroed word) on the current device;
+ *   snapi_context_bind: ctx for the calling thread's later calls (NULL: the
+ *     device's word again);
+ *   snapi_context_status / snapi_get_status: synchronise `stream` (a
+ *     hipStream_t, null = the default stream), return SNAPI_OK,
+ *     SNAPI_ERR_UNSUPPORTED or SNAPI_ERR_NONFINITE for the bits the word
+ *     holds, and clear them when reset is nonzero.
+ */
+typedef struct snapi_context snapi_context;
+int snapi_context_create(snapi_context **ctx);
+int snapi_context_destroy(snapi_context *ctx);
+int snapi_context_bind(snapi_context *ctx);
+int snapi_context_status(snapi_context *ctx, void *stream, int reset);
+int snapi_get_status(void *stream, int reset);
+
+/*
  * Polarized synthesis of a batch of columns on one wavelength grid.
  *   state   [ncol][nfield][nd] (double for SNAPI_FP64, float for SNAPI_MIXED)
  *   lam     [nlambda] vacuum wavelengths [cm] (double)
@@ -174,9 +204,10 @@ int snapi_synth_stokes_dev(int precision, const snapi_line_table *lines, const s
  * path set ray->hminus_mid = 1.
  *   stokes [ncol][nlambda][4], resp [ncol][7][nd][nlambda][4]
  * All arrays in the precision's type (double for SNAPI_FP64, float for
- * SNAPI_MIXED), lam in double.  Valid for the mu = 1 ray, where the ray and
- * atmosphere frames coincide (atmosphere::transform_responses is the
- * identity there).
+ * SNAPI_MIXED), lam in double.  Valid for the mu = 1 ray (theta = pi), where
+ * the ray and atmosphere frames coincide (atmosphere::transform_responses,
+ * atmos/atmos_rad.cc:53-101, is the identity there); any other ray returns
+ * SNAPI_ERR_UNSUPPORTED, as does snapi_stokes_adjoint.
  */
 int snapi_stokes_responses(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
                            const void *state, const void *dstate, int nlambda, const double *lam, void *stokes,

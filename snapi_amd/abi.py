@@ -12,6 +12,7 @@ SNAPI_ERR_ARG = -1
 SNAPI_ERR_DEVICE = -2
 SNAPI_ERR_UNSUPPORTED = -3
 SNAPI_ERR_NONFINITE = -4
+SNAPI_ERR_CAPACITY = -5
 
 SNAPI_FP64 = 0
 SNAPI_MIXED = 1
@@ -34,6 +35,7 @@ STATUS_TEXT = {
     SNAPI_ERR_DEVICE: "HIP runtime error",
     SNAPI_ERR_UNSUPPORTED: "input outside what the kernels implement",
     SNAPI_ERR_NONFINITE: "non-finite result",
+    SNAPI_ERR_CAPACITY: "problem larger than the fixed work area",
 }
 
 

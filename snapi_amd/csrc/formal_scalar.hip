@@ -86,17 +86,19 @@ extern "C" int snapi_formal_scalar(int nprob, int nd, const double *grid, const 
       hipMalloc((void **)&dI, n) != hipSuccess || hipMalloc((void **)&dL, n) != hipSuccess) {
     rc = SNAPI_ERR_DEVICE;
   } else {
-    hipMemcpy(dg, grid, n, hipMemcpyHostToDevice);
-    hipMemcpy(dop, op, n, hipMemcpyHostToDevice);
-    hipMemcpy(dem, em, n, hipMemcpyHostToDevice);
-    hipMemcpy(dct, ct, nprob * sizeof(double), hipMemcpyHostToDevice);
+    if (hipMemcpy(dg, grid, n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dop, op, n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dem, em, n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dct, ct, nprob * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      rc = SNAPI_ERR_DEVICE;
+  }
+  if (rc == SNAPI_OK) {
     hipLaunchKernelGGL(formal_scalar_kernel, dim3((nprob + 255) / 256), dim3(256), 0, 0, nprob, nd, dg, dop, dem,
                        dct, boundary, dI, dL);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = SNAPI_ERR_DEVICE;
-    else {
-      hipMemcpy(I, dI, n, hipMemcpyDeviceToHost);
-      if (lstar) hipMemcpy(lstar, dL, n, hipMemcpyDeviceToHost);
-    }
+    else if (hipMemcpy(I, dI, n, hipMemcpyDeviceToHost) != hipSuccess ||
+             (lstar && hipMemcpy(lstar, dL, n, hipMemcpyDeviceToHost) != hipSuccess))
+      rc = SNAPI_ERR_DEVICE;
   }
   hipFree(dg);
   hipFree(dop);

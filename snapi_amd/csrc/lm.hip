@@ -206,9 +206,12 @@ extern "C" int snapi_lm_step(int precision, int npix, int nrow, int npar, const 
       hipMalloc((void **)&dcorr, (size_t)npix * npar * 8) != hipSuccess) {
     rc = SNAPI_ERR_DEVICE;
   } else {
-    hipMemcpy(dJ, J, nJ * es, hipMemcpyHostToDevice);
-    hipMemcpy(dr, r, nr * es, hipMemcpyHostToDevice);
-    hipMemcpy(dlm, lm, (size_t)npix * 8, hipMemcpyHostToDevice);
+    if (hipMemcpy(dJ, J, nJ * es, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dr, r, nr * es, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dlm, lm, (size_t)npix * 8, hipMemcpyHostToDevice) != hipSuccess)
+      rc = SNAPI_ERR_DEVICE;
+  }
+  if (rc == SNAPI_OK) {
     if (precision == SNAPI_FP64)
       launch_lm_normal<double>(npix, nrow, npar, (const double *)dJ, (const double *)dr, dG, drhs, nullptr, nullptr,
                                0);
@@ -216,11 +219,10 @@ extern "C" int snapi_lm_step(int precision, int npix, int nrow, int npar, const 
       launch_lm_normal<float>(npix, nrow, npar, (const float *)dJ, (const float *)dr, dG, drhs, nullptr, nullptr, 0);
     launch_lm_solve(npix, npar, dG, drhs, dlm, dcorr, nullptr, 0);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = SNAPI_ERR_DEVICE;
-    else {
-      hipMemcpy(JTJ, dG, nG * 8, hipMemcpyDeviceToHost);
-      hipMemcpy(rhs, drhs, (size_t)npix * npar * 8, hipMemcpyDeviceToHost);
-      hipMemcpy(corr, dcorr, (size_t)npix * npar * 8, hipMemcpyDeviceToHost);
-    }
+    else if (hipMemcpy(JTJ, dG, nG * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(rhs, drhs, (size_t)npix * npar * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(corr, dcorr, (size_t)npix * npar * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = SNAPI_ERR_DEVICE;
   }
   hipFree(dJ);
   hipFree(dr);

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <vector>
 
 #include "rt_common.h"
 #include "synth_kernel.h"
@@ -587,6 +588,10 @@ static int run_resp(int precision, const snapi_line_table *lines, const snapi_ra
   if (rc) return rc;
   if (!state || !dstate || !lam || !stokes || !out_host) return SNAPI_ERR_ARG;
   if (sink == SINK_GRAD && (!obs || !ws)) return SNAPI_ERR_ARG;
+  // per-depth responses are formed in the atmosphere's frame only on the
+  // mu = 1 ray (theta = pi), where atmosphere::transform_responses
+  // (atmos/atmos_rad.cc:53-101) is the identity
+  if (!(cos(ray->theta) < -0.999999)) return SNAPI_ERR_UNSUPPORTED;
   const size_t es = precision == SNAPI_FP64 ? 8 : 4;
   const size_t nf = a.nfield;
   const size_t sb = (size_t)ncol * nf * nd * es, db = 2 * sb, lb = (size_t)nlambda * 8;
@@ -606,12 +611,15 @@ static int run_resp(int precision, const snapi_line_table *lines, const snapi_ra
       (sink == SINK_GRAD && hipMalloc(&dobs, stb) != hipSuccess)) {
     rc = SNAPI_ERR_DEVICE;
   } else {
-    hipMemcpy(ds, state, sb, hipMemcpyHostToDevice);
-    hipMemcpy(dd, dstate, db, hipMemcpyHostToDevice);
-    hipMemcpy(dl, lam, lb, hipMemcpyHostToDevice);
-    hipMemset(dout, 0, ob);
-    if (dobs) hipMemcpy(dobs, obs, stb, hipMemcpyHostToDevice);
-    reset_status();
+    if (hipMemcpy(ds, state, sb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dd, dstate, db, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dl, lam, lb, hipMemcpyHostToDevice) != hipSuccess || hipMemset(dout, 0, ob) != hipSuccess ||
+        (dobs && hipMemcpy(dobs, obs, stb, hipMemcpyHostToDevice) != hipSuccess))
+      rc = SNAPI_ERR_DEVICE;
+    else
+      rc = reset_status();
+  }
+  if (rc == SNAPI_OK) {
     RespArgs ra = {};
     ra.nseed = sink == SINK_GRAD ? 1 : 4;
     ra.sink = sink;
@@ -623,9 +631,11 @@ static int run_resp(int precision, const snapi_line_table *lines, const snapi_ra
       rc = launch_resp<float>(kl, a, ra, ncol, ds, dd, dl, dstk, dout, dscr, chunk, 0);
     if (rc == SNAPI_OK && hipDeviceSynchronize() != hipSuccess) rc = SNAPI_ERR_DEVICE;
     if (rc == SNAPI_OK) {
-      hipMemcpy(stokes, dstk, stb, hipMemcpyDeviceToHost);
-      hipMemcpy(out_host, dout, ob, hipMemcpyDeviceToHost);
-      rc = read_status();
+      if (hipMemcpy(stokes, dstk, stb, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(out_host, dout, ob, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = SNAPI_ERR_DEVICE;
+      else
+        rc = read_status();
     }
   }
   hipFree(ds);
@@ -654,10 +664,14 @@ extern "C" int snapi_stokes_adjoint(int precision, const snapi_line_table *lines
                                     int nd, const void *state, const void *dstate, int nlambda, const double *lam,
                                     const void *obs, const double *wstokes, void *grad) {
   if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
-  if (!grad) return SNAPI_ERR_ARG;
-  const size_t stb = (size_t)ncol * nlambda * 4;
-  double *tmp = new double[stb > 0 ? stb : 1];
-  int rc = run_resp(precision, lines, ray, ncol, nd, state, dstate, nlambda, lam, tmp, grad, SINK_GRAD, obs, wstokes);
-  delete[] tmp;
-  return rc;
+  if (!grad || ncol <= 0 || nlambda <= 0 || nd < 2 || nd > SNAPI_MAX_ND) return SNAPI_ERR_ARG;
+  // the emergent spectrum is computed on the way and not returned
+  std::vector<double> tmp;
+  try {
+    tmp.resize((size_t)ncol * nlambda * 4);
+  } catch (...) {
+    return SNAPI_ERR_ARG;
+  }
+  return run_resp(precision, lines, ray, ncol, nd, state, dstate, nlambda, lam, tmp.data(), grad, SINK_GRAD, obs,
+                  wstokes);
 }

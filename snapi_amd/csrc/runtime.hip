@@ -14,21 +14,67 @@ static constexpr int kMaxDevices = 64;
 static int *g_status[kMaxDevices];
 static std::mutex g_mu;
 
+}  // namespace snapi
+
+// A caller-owned status word on one device (snapi_context_create).  The
+// calling thread's bound context, when it lives on the current device,
+// receives the status bits of every launch that thread makes.
+struct snapi_context {
+  int device;
+  int *word;
+};
+
+namespace snapi {
+
+static thread_local snapi_context *t_ctx = nullptr;
+
+static int current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -1;
+  return dev;
+}
+
 int *device_status_slot() {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  const int dev = current_device();
+  if (dev < 0) return nullptr;
+  if (t_ctx && t_ctx->device == dev) return t_ctx->word;
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_status[dev]) {
-    if (hipMalloc((void **)&g_status[dev], sizeof(int)) != hipSuccess) return nullptr;
-    if (hipMemset(g_status[dev], 0, sizeof(int)) != hipSuccess) return nullptr;
+    int *w = nullptr;
+    if (hipMalloc((void **)&w, sizeof(int)) != hipSuccess) return nullptr;
+    if (hipMemset(w, 0, sizeof(int)) != hipSuccess) {
+      (void)hipFree(w);
+      return nullptr;
+    }
+    g_status[dev] = w;
   }
   return g_status[dev];
 }
 
-void reset_status() {
+static int status_of(int h) {
+  if (h & 2) return SNAPI_ERR_NONFINITE;
+  if (h & 1) return SNAPI_ERR_UNSUPPORTED;
+  return SNAPI_OK;
+}
+
+// Reads (and, with reset, clears) a status word after the work on `stream`.
+static int take_status(int *w, hipStream_t stream, int reset) {
+  if (!w) return SNAPI_ERR_DEVICE;
+  if (hipStreamSynchronize(stream) != hipSuccess) return SNAPI_ERR_DEVICE;
+  int h = 0;
+  if (hipMemcpy(&h, w, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return SNAPI_ERR_DEVICE;
+  if (reset && (hipMemset(w, 0, sizeof(int)) != hipSuccess || hipDeviceSynchronize() != hipSuccess))
+    return SNAPI_ERR_DEVICE;
+  return status_of(h);
+}
+
+int reset_status() {
   int *s = device_status_slot();
+  if (!s) return SNAPI_ERR_DEVICE;
   // completed before returning, so work on any stream sees the cleared flag
-  if (s && hipMemset(s, 0, sizeof(int)) == hipSuccess) (void)hipStreamSynchronize(nullptr);
+  if (hipMemset(s, 0, sizeof(int)) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess)
+    return SNAPI_ERR_DEVICE;
+  return SNAPI_OK;
 }
 
 int read_status() {
@@ -36,9 +82,7 @@ int read_status() {
   if (!s) return SNAPI_ERR_DEVICE;
   int h = 0;
   if (hipMemcpy(&h, s, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return SNAPI_ERR_DEVICE;
-  if (h & 2) return SNAPI_ERR_NONFINITE;
-  if (h & 1) return SNAPI_ERR_UNSUPPORTED;
-  return SNAPI_OK;
+  return status_of(h);
 }
 
 // Line constants: the factors of atom_rad_synthesis.cc:403-433 and the
@@ -107,7 +151,51 @@ int prepare_synth(const snapi_line_table *lines, const snapi_ray *ray, int ncol,
 
 }  // namespace snapi
 
-extern "C" int snapi_version(void) { return 100; }
+using namespace snapi;
+
+extern "C" int snapi_version(void) { return 101; }
+
+extern "C" int snapi_context_create(snapi_context **ctx) {
+  if (!ctx) return SNAPI_ERR_ARG;
+  *ctx = nullptr;
+  const int dev = current_device();
+  if (dev < 0) return SNAPI_ERR_DEVICE;
+  int *w = nullptr;
+  if (hipMalloc((void **)&w, sizeof(int)) != hipSuccess) return SNAPI_ERR_DEVICE;
+  if (hipMemset(w, 0, sizeof(int)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    (void)hipFree(w);
+    return SNAPI_ERR_DEVICE;
+  }
+  *ctx = new snapi_context{dev, w};
+  return SNAPI_OK;
+}
+
+extern "C" int snapi_context_destroy(snapi_context *ctx) {
+  if (!ctx) return SNAPI_ERR_ARG;
+  if (t_ctx == ctx) t_ctx = nullptr;
+  const hipError_t e = hipFree(ctx->word);
+  delete ctx;
+  return e == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+extern "C" int snapi_context_bind(snapi_context *ctx) {
+  t_ctx = ctx;
+  return SNAPI_OK;
+}
+
+extern "C" int snapi_context_status(snapi_context *ctx, void *stream, int reset) {
+  if (!ctx) return SNAPI_ERR_ARG;
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) return SNAPI_ERR_DEVICE;
+  if (prev != ctx->device && hipSetDevice(ctx->device) != hipSuccess) return SNAPI_ERR_DEVICE;
+  const int rc = take_status(ctx->word, static_cast<hipStream_t>(stream), reset);
+  if (prev != ctx->device && hipSetDevice(prev) != hipSuccess) return SNAPI_ERR_DEVICE;
+  return rc;
+}
+
+extern "C" int snapi_get_status(void *stream, int reset) {
+  return take_status(device_status_slot(), static_cast<hipStream_t>(stream), reset);
+}
 
 extern "C" int snapi_device_count(void) {
   int n = 0;

@@ -152,10 +152,13 @@ extern "C" int snapi_synth_stokes(int precision, const snapi_line_table *lines, 
       hipMalloc((void **)&dlam, nlambda * sizeof(double)) != hipSuccess) {
     rc = SNAPI_ERR_DEVICE;
   } else {
-    hipMemcpy(ds, state, sbytes, hipMemcpyHostToDevice);
-    hipMemcpy(dlam, lam, nlambda * sizeof(double), hipMemcpyHostToDevice);
-    reset_status();
-    rc = snapi_synth_stokes_dev(precision, lines, ray, ncol, nd, ds, nlambda, dlam, dout, nullptr);
+    if (hipMemcpy(ds, state, sbytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dlam, lam, nlambda * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      rc = SNAPI_ERR_DEVICE;
+    else
+      rc = reset_status();
+    if (rc == SNAPI_OK)
+      rc = snapi_synth_stokes_dev(precision, lines, ray, ncol, nd, ds, nlambda, dlam, dout, nullptr);
     if (rc == SNAPI_OK) {
       if (hipDeviceSynchronize() != hipSuccess) rc = SNAPI_ERR_DEVICE;
       else if (hipMemcpy(stokes, dout, obytes, hipMemcpyDeviceToHost) != hipSuccess) rc = SNAPI_ERR_DEVICE;

@@ -328,9 +328,10 @@ int prepare_synth(const snapi_line_table *lines, const snapi_ray *ray, int ncol,
 int synth_dev_masked(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
                      const void *state_dev, int nlambda, const double *lam_dev, void *stokes_dev, const int *active,
                      int active_div, hipStream_t stream);
-// A per-device int the kernels OR their status bits into.
+// The int the kernels OR their status bits into: the calling thread's bound
+// context (snapi_context_bind) on the current device, else the device's own.
 int *device_status_slot();
-void reset_status();
+int reset_status();
 int read_status();
 
 }  // namespace snapi

@@ -72,3 +72,64 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_native, "_lib", None)
     with pytest.raises(RuntimeError, match="missing"):
         _native.lib()
+
+
+def _cfg1_inputs(ncol=1):
+    import numpy as np
+    from snapi_amd import workloads
+    t = workloads.template_cfg1()
+    st = workloads.cube_state(t, 1, ncol, seed=3, dtype=np.float64)
+    lam = np.ascontiguousarray(t["lam_vac"][::50])
+    return t, st, lam
+
+
+def test_responses_refuse_an_inclined_ray(libpath):
+    """Per-depth responses and the adjoint exist only on the mu = 1 ray; any
+    other ray is refused before a device is touched."""
+    import math
+    import numpy as np
+    from snapi_amd import abi
+    lib = C.CDLL(libpath)
+    t, st, lam = _cfg1_inputs()
+    ds = np.zeros((st.shape[0], 2) + st.shape[1:])
+    ray = abi.SnapiRay(theta=math.pi - math.acos(0.6), lam_mid=t["ray"].lam_mid, tau_grid=1, hminus=1, hminus_mid=1)
+    stokes = np.zeros((st.shape[0], lam.size, 4))
+    resp = np.zeros((st.shape[0], 7, st.shape[2], lam.size, 4))
+    vp = C.c_void_p
+    rc = lib.snapi_stokes_responses(abi.SNAPI_FP64, C.byref(t["table"]), C.byref(ray), st.shape[0], st.shape[2],
+                                    vp(st.ctypes.data), vp(ds.ctypes.data), lam.size, vp(lam.ctypes.data),
+                                    vp(stokes.ctypes.data), vp(resp.ctypes.data))
+    assert rc == abi.SNAPI_ERR_UNSUPPORTED
+    ws = (C.c_double * 4)(1, 1, 1, 1)
+    grad = np.zeros((st.shape[0], 7, st.shape[2]))
+    rc = lib.snapi_stokes_adjoint(abi.SNAPI_FP64, C.byref(t["table"]), C.byref(ray), st.shape[0], st.shape[2],
+                                  vp(st.ctypes.data), vp(ds.ctypes.data), lam.size, vp(lam.ctypes.data),
+                                  vp(stokes.ctypes.data), ws, vp(grad.ctypes.data))
+    assert rc == abi.SNAPI_ERR_UNSUPPORTED
+
+
+def test_adjoint_rejects_negative_sizes_before_allocating(libpath):
+    import math
+    import numpy as np
+    from snapi_amd import abi
+    lib = C.CDLL(libpath)
+    t, st, lam = _cfg1_inputs()
+    ds = np.zeros((st.shape[0], 2) + st.shape[1:])
+    ray = abi.SnapiRay(theta=math.pi, lam_mid=t["ray"].lam_mid, tau_grid=1, hminus=1, hminus_mid=1)
+    vp = C.c_void_p
+    ws = (C.c_double * 4)(1, 1, 1, 1)
+    grad = np.zeros(16)
+    for ncol, nl in ((-1, lam.size), (1, -5), (0, lam.size)):
+        rc = lib.snapi_stokes_adjoint(abi.SNAPI_FP64, C.byref(t["table"]), C.byref(ray), ncol, st.shape[2],
+                                      vp(st.ctypes.data), vp(ds.ctypes.data), nl, vp(lam.ctypes.data),
+                                      vp(st.ctypes.data), ws, vp(grad.ctypes.data))
+        assert rc == abi.SNAPI_ERR_ARG, (ncol, nl, rc)
+
+
+def test_status_api_checks_its_arguments(libpath):
+    from snapi_amd import abi
+    lib = C.CDLL(libpath)
+    assert lib.snapi_context_create(None) == abi.SNAPI_ERR_ARG
+    assert lib.snapi_context_status(None, None, 0) == abi.SNAPI_ERR_ARG
+    assert lib.snapi_context_destroy(None) == abi.SNAPI_ERR_ARG
+    assert lib.snapi_context_bind(None) == abi.SNAPI_OK

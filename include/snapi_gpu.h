@@ -422,6 +422,28 @@ int snapi_ali_solve_dev(int precision, const snapi_ali_atom *atom, const snapi_a
                         int max_iter, double *pops_dev, int *iterations_dev, double *relchange_dev, void *stream);
 
 /*
+ * snapi_ali_solve_dev without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * semantics and kernels as snapi_ali_solve_dev; the grid's arrays, the sums
+ * and the iteration's work areas live in a fixed device area of
+ * snapi_ali_noalloc_capacity() bytes (64 MiB), one per device, held until the
+ * process ends.  A problem needing more returns SNAPI_ERR_CAPACITY before any
+ * device work (about 2,400 columns of 81 depths for an atom of 10
+ * transitions and 220 wavelengths).  Every call on a device shares that
+ * device's area and writes every value it reads from it, so calls that
+ * overlap on one device (on different streams or threads) race; calls in
+ * sequence do not.  It calls no hipMalloc, hipFree or allocating library
+ * routine.
+ */
+size_t snapi_ali_noalloc_capacity(void);
+int snapi_ali_solve_dev_noalloc(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol,
+                                int nd, const double *colstate_dev, const double *vproj_dev,
+                                const double *gamma_dev, const double *bgop_dev, const double *bgem_dev,
+                                const double *coll_dev, double tol, int max_iter, double *pops_dev,
+                                int *iterations_dev, double *relchange_dev, void *stream);
+
+/*
  * Chemical equilibrium and LTE populations, per depth point, in double.
  *
  * The species table mirrors what the reference's atom and molecule objects

@@ -24,6 +24,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "rt_common.h"
@@ -558,9 +559,33 @@ __global__ void ali_zero_kernel(size_t n_per_col, int ncol, const int *__restric
     if (!mask || mask[i / n_per_col]) p[i] = 0.0;
 }
 
-__global__ void ali_count_kernel(int ncol, const int *__restrict__ active, int *__restrict__ count) {
+// Number of active columns, written (not added) by one workgroup, so the
+// counter needs no clearing between iterations.
+__global__ __launch_bounds__(256) void ali_count_kernel(int ncol, const int *__restrict__ active,
+                                                        int *__restrict__ count) {
+  __shared__ int part[256];
+  int n = 0;
+  for (int col = threadIdx.x; col < ncol; col += 256) n += active[col] ? 1 : 0;
+  part[threadIdx.x] = n;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = part[0];
+}
+
+// Start of the iteration: every column active, nothing to redo, no
+// iterations yet (kernels rather than memsets, so a captured stream records
+// no memset nodes).
+__global__ void ali_init_kernel(int ncol, int *__restrict__ active, int *__restrict__ retry, int *__restrict__ iters,
+                                double *__restrict__ rc_out) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col < ncol && active[col]) atomicAdd(count, 1);
+  if (col >= ncol) return;
+  active[col] = 1;
+  retry[col] = 0;
+  iters[col] = 0;
+  rc_out[col] = 0.0;
 }
 
 // ---------------------------------------------------------------- host side
@@ -819,37 +844,33 @@ done:
   return rc;
 }
 
+// Device work areas of the iteration beyond its inputs and outputs.
+struct AliWork {
+  double *newp = nullptr;  // [ncol][nd][nlevel] populations of the current solve
+  double *rc = nullptr;    // [ncol] relative change of the current solve
+  int *ints = nullptr;     // [3 ncol + 1]: active, retry, negative flags, active count
+};
+
+static size_t ali_work_doubles(int ncol, int nd, int nlevel) { return (size_t)ncol * nd * nlevel + ncol; }
+static size_t ali_work_ints(int ncol) { return (size_t)ncol * 3 + 1; }
+
 // The iteration on device-resident inputs (P: heavy arrays and the grid's
-// arrays on the device, P.acc a scratch of ncol * nd * ntrans * 4 doubles).
+// arrays on the device, P.acc a scratch of ncol * nd * ntrans * 4 doubles;
+// W: the work areas).  The only host traffic is one int per iteration.
 static int ali_solve_core(int precision, const AliAtom &A, const AliGridArgs &G, int ncol, int nd, AliCol P,
-                          const double *dcoll, double *dpops, int *diters, double *drco, double tol, int max_iter,
-                          hipStream_t st) {
+                          const AliWork &W, const double *dcoll, double *dpops, int *diters, double *drco, double tol,
+                          int max_iter, hipStream_t st) {
   const int n = A.nlevel, nt = A.ntrans;
   int rc = SNAPI_OK;
-  double *dnew = nullptr, *drc = nullptr;
-  int *dint = nullptr;
-  int *active, *retry, *neg, *count;
-  ACHK(hipMalloc((void **)&dnew, (size_t)ncol * nd * n * 8));
-  ACHK(hipMalloc((void **)&drc, (size_t)ncol * 8));
-  ACHK(hipMalloc((void **)&dint, ((size_t)ncol * 3 + 1) * sizeof(int)));
-  active = dint;
-  retry = dint + ncol;
-  neg = dint + 2 * ncol;
-  count = dint + 3 * ncol;
-  {
-    std::vector<int> ones(ncol, 1);
-    ACHK(hipMemcpyAsync(active, ones.data(), (size_t)ncol * sizeof(int), hipMemcpyHostToDevice, st));
-    ACHK(hipMemsetAsync(retry, 0, (size_t)ncol * sizeof(int), st));
-    ACHK(hipMemsetAsync(diters, 0, (size_t)ncol * sizeof(int), st));
-    ACHK(hipMemsetAsync(drco, 0, (size_t)ncol * 8, st));
-    ACHK(hipStreamSynchronize(st));
-  }
+  double *dnew = W.newp, *drc = W.rc;
+  int *active = W.ints, *retry = W.ints + ncol, *neg = W.ints + 2 * ncol, *count = W.ints + 3 * ncol;
+  hipLaunchKernelGGL(ali_init_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, ncol, active, retry, diters, drco);
+  ACHK(hipGetLastError());
   P.active = active;
   P.pops = dpops;
   for (int it = 0; it < max_iter; ++it) {
     int nact = 0;
-    ACHK(hipMemsetAsync(count, 0, sizeof(int), st));
-    hipLaunchKernelGGL(ali_count_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, ncol, active, count);
+    hipLaunchKernelGGL(ali_count_kernel, dim3(1), dim3(256), 0, st, ncol, active, count);
     ACHK(hipMemcpyAsync(&nact, count, sizeof(int), hipMemcpyDeviceToHost, st));
     ACHK(hipStreamSynchronize(st));
     if (nact == 0) break;
@@ -871,8 +892,21 @@ static int ali_solve_core(int precision, const AliAtom &A, const AliGridArgs &G,
   }
   ACHK(hipStreamSynchronize(st));
 done:
-  hipFree(dnew); hipFree(drc); hipFree(dint);
   return rc;
+}
+
+// The work areas from the device heap (freed by release_work).
+static int alloc_work(int ncol, int nd, int nlevel, AliWork &W) {
+  W = AliWork{};
+  if (hipMalloc((void **)&W.newp, ali_work_doubles(ncol, nd, nlevel) * 8) != hipSuccess) return SNAPI_ERR_DEVICE;
+  W.rc = W.newp + (size_t)ncol * nd * nlevel;
+  if (hipMalloc((void **)&W.ints, ali_work_ints(ncol) * sizeof(int)) != hipSuccess) return SNAPI_ERR_DEVICE;
+  return SNAPI_OK;
+}
+static void release_work(AliWork &W) {
+  hipFree(W.newp);
+  hipFree(W.ints);
+  W = AliWork{};
 }
 
 extern "C" int snapi_ali_solve(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid, int ncol, int nd,
@@ -892,6 +926,7 @@ extern "C" int snapi_ali_solve(int precision, const snapi_ali_atom *atom, const 
   const int n = atom->nlevel;
   AliBuffers B;
   AliCol P;
+  AliWork W;
   double *dcoll = nullptr, *drco = nullptr;
   int *diters = nullptr;
   rc = upload_inputs(atom, grid, ncol, nd, colstate, vproj, gamma, bgop, bgem, pops, B, P);
@@ -900,13 +935,16 @@ extern "C" int snapi_ali_solve(int precision, const snapi_ali_atom *atom, const 
   ACHK(hipMemcpy(dcoll, coll, (size_t)ncol * nd * n * n * 8, hipMemcpyHostToDevice));
   ACHK(hipMalloc((void **)&drco, (size_t)ncol * 8));
   ACHK(hipMalloc((void **)&diters, (size_t)ncol * sizeof(int)));
-  rc = ali_solve_core(precision, A, G, ncol, nd, P, dcoll, B.pops, diters, drco, tol, max_iter, 0);
+  rc = alloc_work(ncol, nd, n, W);
+  if (rc) goto done;
+  rc = ali_solve_core(precision, A, G, ncol, nd, P, W, dcoll, B.pops, diters, drco, tol, max_iter, 0);
   if (rc) goto done;
   ACHK(hipMemcpy(pops, B.pops, (size_t)ncol * nd * n * 8, hipMemcpyDeviceToHost));
   ACHK(hipMemcpy(iterations, diters, (size_t)ncol * sizeof(int), hipMemcpyDeviceToHost));
   ACHK(hipMemcpy(relchange, drco, (size_t)ncol * 8, hipMemcpyDeviceToHost));
 done:
   B.release();
+  release_work(W);
   hipFree(dcoll); hipFree(drco); hipFree(diters);
   return rc;
 }
@@ -930,6 +968,8 @@ extern "C" int snapi_ali_solve_dev(int precision, const snapi_ali_atom *atom, co
   const size_t nlam = grid->nlam, nt = atom->ntrans;
   double *g = nullptr, *acc = nullptr;
   AliCol P;
+  AliWork W;
+  ACHK(alloc_work(ncol, nd, atom->nlevel, W) == SNAPI_OK ? hipSuccess : hipErrorOutOfMemory);
   ACHK(hipMalloc((void **)&g, (3 * nlam + nt * nlam) * 8));
   ACHK(hipMalloc((void **)&acc, (size_t)ncol * nd * nt * 4 * 8));
   ACHK(hipMemcpyAsync(g, grid->lam, nlam * 8, hipMemcpyHostToDevice, st));
@@ -948,10 +988,107 @@ extern "C" int snapi_ali_solve_dev(int precision, const snapi_ali_atom *atom, co
   P.sigma = g + 3 * nlam;
   P.acc = acc;
   P.active = nullptr;
-  rc = ali_solve_core(precision, A, G, ncol, nd, P, coll_dev, pops_dev, iterations_dev, relchange_dev, tol, max_iter,
-                      st);
+  rc = ali_solve_core(precision, A, G, ncol, nd, P, W, coll_dev, pops_dev, iterations_dev, relchange_dev, tol,
+                      max_iter, st);
 done:
+  release_work(W);
   hipFree(g);
   hipFree(acc);
   return rc;
 }
+
+// ------------------------------------------- allocation-free ALI iteration
+// The work areas of snapi_ali_solve_dev_noalloc live in one fixed device
+// array.  Each device that runs it holds its own copy (at its own address,
+// looked up once per device) until the process ends.
+namespace snapi {
+
+static constexpr size_t kAliScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kAliScratchBytes];
+
+static constexpr int kAliScratchDevices = 64;
+static unsigned char *g_scratch_dev[kAliScratchDevices];
+static std::mutex g_scratch_mu;
+
+// Device address of this device's copy of `scratch` (null on failure).
+static unsigned char *scratch_address() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kAliScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of the fixed area a problem needs: the grid's arrays, the sums, the
+// work areas.
+static size_t noalloc_bytes(int ncol, int nd, int nlevel, int ntrans, int nlam) {
+  return round256((size_t)(3 + ntrans) * nlam * 8) + round256((size_t)ncol * nd * ntrans * 4 * 8) +
+         round256(ali_work_doubles(ncol, nd, nlevel) * 8) + round256(ali_work_ints(ncol) * sizeof(int));
+}
+
+}  // namespace snapi
+
+extern "C" size_t snapi_ali_noalloc_capacity(void) { return kAliScratchBytes; }
+
+extern "C" int snapi_ali_solve_dev_noalloc(int precision, const snapi_ali_atom *atom, const snapi_ali_grid *grid,
+                                           int ncol, int nd, const double *colstate_dev, const double *vproj_dev,
+                                           const double *gamma_dev, const double *bgop_dev, const double *bgem_dev,
+                                           const double *coll_dev, double tol, int max_iter, double *pops_dev,
+                                           int *iterations_dev, double *relchange_dev, void *stream) {
+  if (bad_inputs(atom, grid, ncol, nd, colstate_dev, vproj_dev, gamma_dev, bgop_dev, bgem_dev) || !coll_dev ||
+      !pops_dev || !iterations_dev || !relchange_dev || max_iter < 1)
+    return SNAPI_ERR_ARG;
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
+  AliAtom A;
+  AliGridArgs G;
+  int rc = convert_atom(atom, A);
+  if (rc) return rc;
+  rc = convert_grid(grid, G);
+  if (rc) return rc;
+  const size_t nlam = grid->nlam, nt = atom->ntrans;
+  if (noalloc_bytes(ncol, nd, atom->nlevel, atom->ntrans, grid->nlam) > kAliScratchBytes) return SNAPI_ERR_CAPACITY;
+  unsigned char *base = scratch_address();
+  if (!base) return SNAPI_ERR_DEVICE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // carve the fixed area: grid arrays, sums, work areas (256-byte aligned)
+  unsigned char *q = base;
+  double *g = reinterpret_cast<double *>(q);
+  q += round256((3 + nt) * nlam * 8);
+  double *acc = reinterpret_cast<double *>(q);
+  q += round256((size_t)ncol * nd * nt * 4 * 8);
+  AliWork W;
+  W.newp = reinterpret_cast<double *>(q);
+  W.rc = W.newp + (size_t)ncol * nd * atom->nlevel;
+  q += round256(ali_work_doubles(ncol, nd, atom->nlevel) * 8);
+  W.ints = reinterpret_cast<int *>(q);
+  AliCol P;
+  ACHK(hipMemcpyAsync(g, grid->lam, nlam * 8, hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(g + nlam, grid->lamw, nlam * 8, hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(g + 2 * nlam, grid->fudge, nlam * 8, hipMemcpyHostToDevice, st));
+  ACHK(hipMemcpyAsync(g + 3 * nlam, grid->sigma, nt * nlam * 8, hipMemcpyHostToDevice, st));
+  P.cs = colstate_dev;
+  P.vproj = vproj_dev;
+  P.gamma = gamma_dev;
+  P.bgop = bgop_dev;
+  P.bgem = bgem_dev;
+  P.pops = pops_dev;
+  P.lam = g;
+  P.lamw = g + nlam;
+  P.fudge = g + 2 * nlam;
+  P.sigma = g + 3 * nlam;
+  P.acc = acc;
+  P.active = nullptr;
+  // every value the iteration reads from the area is written first in this
+  // call (init kernel, count kernel, per-iteration zeroing of the sums)
+  rc = ali_solve_core(precision, A, G, ncol, nd, P, W, coll_dev, pops_dev, iterations_dev, relchange_dev, tol,
+                      max_iter, st);
+done:
+  return rc;
+}
+

@@ -12,7 +12,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     txt = open(os.path.join(REPO, "include", "snapi_gpu.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\bint\s+(snapi_\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(?:int|size_t)\s+(snapi_\w+)\s*\(", txt)))
 
 
 @pytest.fixture(scope="module")

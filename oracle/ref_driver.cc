@@ -35,6 +35,20 @@
 //          NLTE population responses of the active atom: its linear system,
 //          right-hand sides and solution, and formal_with_responses inputs
 //          and outputs at selected wavelengths.
+//   nodes  <atmos.cfg> <lambda.txt> <el> <az> <model.txt>
+//          atmos_pp::build_from_nodes (plpar/atmos_pp.cc:58-248) at the node
+//          values of model.txt, the ray projection of the field
+//          (atmos/atmos_rad.cc:15-51), the synthesis-path Stokes vector of
+//          forward_evaluate (atmos/atmos_fit.cc:717-737), and
+//          obs_stokes_responses_to_nodes (atmos/atmos_fit.cc:1205-1352): the
+//          response-path Stokes vector, the node Jacobian and the model's
+//          node responses dq_p(z)/dnode.
+//   fit    <atmos.cfg> <lambda.txt> <el> <az> <fit.txt> [last]
+//          atmosphere::stokes_lm_fit (atmos/atmos_fit.cc:18-227) from the start
+//          model of fit.txt on its observation, run with 1, 2, ..., kmax
+//          iterations: the nodes each run returns (after polish_angles) and
+//          the chi^2 of the spectrum it returns, which is the fit's
+//          trajectory, the algorithm being deterministic.
 //
 // This file is test infrastructure: it is compiled only against the reference
 // sources (oracle/ref.mk) and never linked into the product.
@@ -975,6 +989,183 @@ static int mode_nlteresp(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// Node model from a text stream: "tau_min tau_max", the node count, then one
+// "code logtau value" line per node (code 1 T, 2 vt, 3 vLOS, 4 |B|,
+// 5 inclination [rad], 6 azimuth [rad]), grouped in that order as the
+// reference's model holds them (atmos/atmos.h:377-531).  Regularisation off.
+static model *read_model(FILE *f) {
+  double tmin, tmax;
+  int n;
+  if (fscanf(f, "%lf %lf %d", &tmin, &tmax, &n) != 3 || n < 1) { fprintf(stderr, "bad model header\n"); exit(2); }
+  int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+  int *code = new int[n];
+  double *lt = new double[n], *val = new double[n];
+  for (int i = 0; i < n; ++i) {
+    if (fscanf(f, "%d %lf %lf", &code[i], &lt[i], &val[i]) != 3 || code[i] < 1 || code[i] > 6) {
+      fprintf(stderr, "bad model node\n");
+      exit(2);
+    }
+    if (i && code[i] < code[i - 1]) { fprintf(stderr, "model nodes must be grouped by parameter\n"); exit(2); }
+    cnt[code[i]]++;
+  }
+  model *m = model_new(cnt[1], cnt[2], cnt[3], cnt[4], cnt[5], cnt[6]);
+  m->set_tau_min(tmin);
+  m->set_tau_max(tmax);
+  int idx = 0;
+  for (int pc = 1; pc <= 6; ++pc) {
+    if (!cnt[pc]) continue;
+    fp_t *t = new fp_t[cnt[pc]] - 1, *v = new fp_t[cnt[pc]] - 1;
+    for (int j = 1; j <= cnt[pc]; ++j, ++idx) {
+      t[j] = lt[idx];
+      v[j] = val[idx];
+    }
+    switch (pc) {
+      case 1: m->set_temp_nodes(t, v); break;
+      case 2: m->set_vt_nodes(t, v); break;
+      case 3: m->set_vs_nodes(t, v); break;
+      case 4: m->set_B_nodes(t, v); break;
+      case 5: m->set_theta_nodes(t, v); break;
+      case 6: m->set_phi_nodes(t, v); break;
+    }
+    delete[](t + 1);
+    delete[](v + 1);
+  }
+  m->set_temp_reg(0, 0.0);
+  m->set_vt_reg(0, 0.0);
+  m->set_vs_reg(0, 0.0);
+  m->set_B_reg(0, 0.0);
+  m->set_theta_reg(0, 0.0);
+  m->set_phi_reg(0, 0.0);
+  delete[] code;
+  delete[] lt;
+  delete[] val;
+  return m;
+}
+
+static void print_nodes(const char *tag, int step, model *m) {
+  printf("%s %d", tag, step);
+  for (int i = 1; i <= m->get_N_nodes_total(); ++i) printf(" %.17g", m->get_parameter(i));
+  printf("\n");
+}
+
+static int mode_nodes(int argc, char **argv, io_class &io) {
+  if (argc < 7) return 2;
+  atmosphere *a = make_atmos(argv[2], io);
+  int nlambda;
+  fp_t *lambda = read_lambda(argv[3], nlambda);
+  fp_t el = atof(argv[4]), az = atof(argv[5]);
+  FILE *f = fopen(argv[6], "r");
+  if (!f) { fprintf(stderr, "cannot open %s\n", argv[6]); return 2; }
+  model *m = read_model(f);
+  fclose(f);
+  int x1 = a->x1l, x2 = a->x2l;
+  const int np = m->get_N_nodes_total();
+  a->set_grid(1);
+  a->build_from_nodes(m);
+  printf("nd %d np %d nlambda %d\n", a->x3h - a->x3l + 1, np, nlambda);
+  fp_t ***Vr = a->project(a->Vx, a->Vy, a->Vz, el, az, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  fp_t ****Bv = a->transform(a->Bx, a->By, a->Bz, el, az, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  for (int z = a->x3l; z <= a->x3h; ++z) {
+    printf("bfn %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", z - a->x3l,
+           a->tau_referent[x1][x2][z], a->T[x1][x2][z], a->Nt[x1][x2][z], a->Ne[x1][x2][z],
+           a->op_referent[x1][x2][z], a->rho[x1][x2][z], a->Vt[x1][x2][z], a->Vz[x1][x2][z], a->Bx[x1][x2][z],
+           a->By[x1][x2][z], a->Bz[x1][x2][z]);
+    printf("proj %d %.17g %.17g %.17g %.17g\n", z - a->x3l, Vr[x1][x2][z], Bv[1][x1][x2][z], Bv[2][x1][x2][z],
+           Bv[3][x1][x2][z]);
+  }
+  del_ft4dim(Bv, 1, 3, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  del_ft3dim(Vr, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  observable *fw = a->forward_evaluate(el, az, lambda, nlambda, 0.0, 1.0, 0.0, 0, 0);
+  fp_t **Sf = fw->get_S(1, 1);
+  for (int l = 1; l <= nlambda; ++l) printf("fwd %d %.17g %.17g %.17g %.17g\n", l, Sf[1][l], Sf[2][l], Sf[3][l], Sf[4][l]);
+  del_ft2dim(Sf, 1, 4, 1, nlambda);
+  delete fw;
+  fp_t ***J = ft3dim(1, np, 1, nlambda, 1, 4);
+  observable *o = a->obs_stokes_responses_to_nodes(m, el, az, lambda, nlambda, J, 0);
+  fp_t **Sv = o->get_S(1, 1);
+  for (int l = 1; l <= nlambda; ++l)
+    printf("stokes %d %.17g %.17g %.17g %.17g\n", l, Sv[1][l], Sv[2][l], Sv[3][l], Sv[4][l]);
+  for (int i = 1; i <= np; ++i)
+    for (int l = 1; l <= nlambda; ++l)
+      printf("jac %d %d %.17g %.17g %.17g %.17g\n", i, l, J[i][l][1], J[i][l][2], J[i][l][3], J[i][l][4]);
+  // the state the response path left behind (T raised by delta_T/2 by
+  // ne_lte_derivatives, atmos/atmos_responses.cc:364-423)
+  for (int z = a->x3l; z <= a->x3h; ++z)
+    printf("post %d %.17g %.17g %.17g %.17g\n", z - a->x3l, a->T[x1][x2][z], a->Nt[x1][x2][z], a->Ne[x1][x2][z],
+           a->op_referent[x1][x2][z]);
+  fp_t ***NR = m->get_response_to_parameters();
+  const int nd = a->x3h - a->x3l + 1;
+  for (int i = 1; i <= np; ++i)
+    for (int q = 1; q <= 7; ++q)
+      for (int z = 1; z <= nd; ++z)
+        if (NR[i][q][z] != 0.0) printf("nmap %d %d %d %.17g\n", i, q, z - 1, NR[i][q][z]);
+  print_nodes("model", 0, m);
+  return 0;
+}
+
+static int mode_fit(int argc, char **argv, io_class &io) {
+  if (argc < 7) return 2;
+  atmosphere *a = make_atmos(argv[2], io);
+  int nlambda;
+  fp_t *lambda = read_lambda(argv[3], nlambda);
+  fp_t el = atof(argv[4]), az = atof(argv[5]);
+  FILE *f = fopen(argv[6], "r");
+  if (!f) { fprintf(stderr, "cannot open %s\n", argv[6]); return 2; }
+  model *start = read_model(f);
+  fp_t ws[4], lm_start;
+  int kmax;
+  if (fscanf(f, "%lf %lf %lf %lf %lf %d", &ws[0], &ws[1], &ws[2], &ws[3], &lm_start, &kmax) != 6) return 2;
+  fp_t *mask = new fp_t[nlambda] - 1;
+  for (int l = 1; l <= nlambda; ++l)
+    if (fscanf(f, "%lf", &mask[l]) != 1) return 2;
+  fp_t ****S = ft4dim(1, 1, 1, 1, 1, 4, 1, nlambda);
+  for (int l = 1; l <= nlambda; ++l)
+    for (int s = 1; s <= 4; ++s)
+      if (fscanf(f, "%lf", &S[1][1][s][l]) != 1) return 2;
+  fclose(f);
+  const int np = start->get_N_nodes_total();
+  printf("np %d nlambda %d kmax %d\n", np, nlambda, kmax);
+  print_nodes("start", 0, start);
+  // a seventh argument "last": only the run with kmax iterations
+  const int first = (argc > 7 && !strcmp(argv[7], "last")) ? kmax : 1;
+  for (int it = first; it <= kmax; ++it) {
+    model *m = clone(start);
+    observable *o = new observable(1, 1, 4, nlambda);
+    o->set(S);
+    o->set_lambda(lambda);
+    o->set_mask(mask);
+    o->set_w_stokes(ws);
+    o->set_inv_parameters(0.0, 0.0, 1.0, 1.0);
+    o->set_no_iterations(it);
+    o->set_start_lambda(lm_start);
+    o->set_stopping_chisq(0.0);
+    // the fit reports the current chi^2 of every iteration on stderr; the
+    // marker tells the runs apart
+    fprintf(stderr, "fitrun %d\n", it);
+    fflush(stderr);
+    observable *r = a->stokes_lm_fit(o, el, az, m);
+    fflush(stderr);
+    fp_t **Sr = r->get_S(1, 1);
+    fp_t chi = 0.0;
+    for (int l = 1; l <= nlambda; ++l)
+      for (int s = 1; s <= 4; ++s) {
+        fp_t d = (S[1][1][s][l] - Sr[s][l]) * ws[s - 1] * mask[l];
+        chi += d * d;
+      }
+    print_nodes("fitk", it, m);
+    printf("fitchi %d %.17g\n", it, chi);
+    if (it == kmax)
+      for (int l = 1; l <= nlambda; ++l)
+        printf("fitS %d %.17g %.17g %.17g %.17g\n", l, Sr[1][l], Sr[2][l], Sr[3][l], Sr[4][l]);
+    fflush(stdout);
+    del_ft2dim(Sr, 1, 4, 1, nlambda);
+    delete r;
+    delete o;
+    delete m;
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr, "usage: ref_driver synth|resp|lmstep|formal_scalar ...\n");
@@ -992,6 +1183,8 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "nlte")) return mode_nlte(argc, argv, io);
   if (!strcmp(argv[1], "chemeq")) return mode_chemeq(argc, argv, io);
   if (!strcmp(argv[1], "nlteresp")) return mode_nlteresp(argc, argv, io);
+  if (!strcmp(argv[1], "nodes")) return mode_nodes(argc, argv, io);
+  if (!strcmp(argv[1], "fit")) return mode_fit(argc, argv, io);
   fprintf(stderr, "unknown mode %s\n", argv[1]);
   return 2;
 }

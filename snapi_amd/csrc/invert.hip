@@ -437,7 +437,14 @@ __global__ void refill_kernel(NodeArgs na, int nslot, int npix, int nf, int nd, 
   if (threadIdx.x == 0) {
     const int old = ss.pix[slot];
     if (old >= 0) {
-      for (int n = 0; n < na.nnode; ++n) out_nodes[(size_t)old * na.nnode + n] = snodes[(size_t)slot * na.nnode + n];
+      // the result with its angles polished (model::polish_angles,
+      // atmos_model.cc:992-1001): inclination acos(cos), azimuth atan(tan)
+      for (int n = 0; n < na.nnode; ++n) {
+        double v = snodes[(size_t)slot * na.nnode + n];
+        if (na.code[n] == NP_THETA) v = acos(cos(v));
+        else if (na.code[n] == NP_PHI) v = atan(tan(v));
+        out_nodes[(size_t)old * na.nnode + n] = v;
+      }
       out_chi2[old] = ss.chi2[slot];
       out_iter[old] = ss.iter[slot];
     }
@@ -451,7 +458,10 @@ __global__ void refill_kernel(NodeArgs na, int nslot, int npix, int nf, int nd, 
       ss.lm[slot] = lm0;
       ss.active[slot] = 1;
       ss.needj[slot] = 1;
-      for (int n = 0; n < na.nnode; ++n) snodes[(size_t)slot * na.nnode + n] = pnodes0[(size_t)next * na.nnode + n];
+      // the start put in the parameter limits, as stokes_lm_fit does before
+      // its first iteration (atmos_fit.cc:91)
+      for (int n = 0; n < na.nnode; ++n)
+        snodes[(size_t)slot * na.nnode + n] = bracket(na.code[n], pnodes0[(size_t)next * na.nnode + n]);
     } else {
       ss.needj[slot] = 0;
     }
@@ -789,6 +799,16 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   }
   reset_status();
   {
+    // With temperature nodes the fit is the reference's: its Jacobian and
+    // current chi^2 come from obs_stokes_responses, whose H- opacity is taken
+    // at the window's middle wavelength (mol.cc:414-469), its trials from
+    // obs_stokes with H- at every wavelength (mol.cc:379-412).
+    SynthArgs a_resp = a;
+    snapi_ray ray_syn = *ray;
+    if (th) {
+      a_resp.hminus_mid = 1;
+      ray_syn.hminus_mid = 0;
+    }
     RespArgs ra = {};
     ra.nseed = 4;
     ra.sink = SINK_NODE;
@@ -849,7 +869,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
         hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, rsd, rstate);
         hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * 2 * ns, dsd,
                            rdst);
-        rc = launch_resp<R>(kl, a, ra, nslot, rstate, rdst, dlam, S, J, scr, chunk, st);
+        rc = launch_resp<R>(kl, a_resp, ra, nslot, rstate, rdst, dlam, S, J, scr, chunk, st);
         if (rc) goto done;
         if (th->broadening > 0.0) {
           rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, S, ss.needj,
@@ -907,7 +927,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
         if (rc) goto done;
         hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, tsd, tstate);
       }
-      rc = synth_dev_masked(precision, lines, ray, nslot, nd, tstate, nl, dlam, St, ss.active, 1, st);
+      rc = synth_dev_masked(precision, lines, th ? &ray_syn : ray, nslot, nd, tstate, nl, dlam, St, ss.active, 1, st);
       if (rc) goto done;
       if (th && th->broadening > 0.0) {  // forward_evaluate (atmos_fit.cc:717-737)
         rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, St, ss.active,
@@ -1137,7 +1157,11 @@ static int node_jacobian_eos_impl(int precision, const snapi_line_table *lines, 
     if (c < 1) c = 1;
     if (c > (size_t)npix) c = npix;
     HIPCHK(hipMalloc((void **)&scr, per_col * c));
-    rc = launch_resp<R>(kl, a, ra, npix, rstate, rdst, dlam, S, J, scr, (int)c, 0);
+    // the response path's H- at the window's middle wavelength, as
+    // obs_stokes_responses_to_nodes evaluates it (mol.cc:414-469)
+    SynthArgs a_resp = a;
+    a_resp.hminus_mid = 1;
+    rc = launch_resp<R>(kl, a_resp, ra, npix, rstate, rdst, dlam, S, J, scr, (int)c, 0);
     if (rc) goto done;
   }
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {

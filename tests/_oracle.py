@@ -410,7 +410,8 @@ def node_jacobian_eos(table, ray, model_nodes, values, state, lam, chem, chem_he
     T0, N0 = build(values)
     st[abi.F_T], st[abi.F_NT] = T0, N0
     _, resp, ds = eos_state(chem, smap, st[None])
-    S, R, rc = responses(table, ray, resp, ds, lam, fd_scale=fd_scale)
+    # the response path evaluates H- at the middle wavelength, whatever the ray says
+    S, R, rc = responses(table, response_ray(ray), resp, ds, lam, fd_scale=fd_scale)
     nd = st.shape[1]
     others = [(p, lt) for p, lt in model_nodes if p != "T"]
     ov = [v for (p, _), v in zip(model_nodes, values) if p != "T"]
@@ -427,6 +428,34 @@ def node_jacobian_eos(table, ray, model_nodes, values, state, lam, chem, chem_he
             else:
                 J[:, n] = np.einsum("zls,z->ls", R[0, pidx[p]], nm[n]).reshape(-1)
         return S[0], J
+    maps = node_maps_eos(model_nodes, values, state, chem_he)
+    for n in range(len(model_nodes)):
+        J[:, n] = np.einsum("qzls,qz->ls", R[0], maps[n]).reshape(-1)
+    return S[0], J
+
+
+def node_maps_eos(model_nodes, values, state, chem_he):
+    """The node responses obs_stokes_responses_to_nodes stores in the model
+    (atmos_fit.cc:1225-1305), [nnode][7][nd] with q = 0 T, 1 Nt, 2 vt,
+    3 vLOS, 4 |B|, 5 inclination, 6 azimuth (angles per radian): for
+    temperature nodes dT/dnode and dNt/dnode from the rebuilds at the node
+    +- delta_T/2, for the others the interpolation's central differences."""
+    from snapi_amd import abi
+    x = np.log10(-state[abi.F_GRID])
+    tn = [i for i, (p, _) in enumerate(model_nodes) if p == "T"]
+    t_pos = np.array([model_nodes[i][1] for i in tn])
+    nd = state.shape[1]
+    out = np.zeros((len(model_nodes), 7, nd))
+
+    def build(vals):
+        T = _interp_temp(t_pos, np.array([vals[i] for i in tn]), x)
+        Nt, _, _, _, _ = hequilibrium(chem_he, T, x)
+        return T, Nt[0]
+
+    others = [(p, lt) for p, lt in model_nodes if p != "T"]
+    ov = [v for (p, _), v in zip(model_nodes, values) if p != "T"]
+    nm_o = node_map(others, ov, state) if others else np.zeros((0, nd))
+    pidx = {"vt": 2, "vlos": 3, "B": 4, "theta": 5, "phi": 6}
     k = 0
     for n, (p, _) in enumerate(model_nodes):
         if p == "T":
@@ -435,12 +464,11 @@ def node_jacobian_eos(table, ray, model_nodes, values, state, lam, chem, chem_he
             Tp, Np = build(vp)
             vp[n] -= 5.0
             Tm, Nm = build(vp)
-            dT, dN = (Tp - Tm) / 5.0, (Np - Nm) / 5.0
-            J[:, n] = (np.einsum("zls,z->ls", R[0, 0], dT) + np.einsum("zls,z->ls", R[0, 1], dN)).reshape(-1)
+            out[n, 0], out[n, 1] = (Tp - Tm) / 5.0, (Np - Nm) / 5.0
         else:
-            J[:, n] = np.einsum("zls,z->ls", R[0, pidx[p]], nm_o[k]).reshape(-1)
+            out[n, pidx[p]] = nm_o[k]
             k += 1
-    return S[0], J
+    return out
 
 
 def forward_state_eos(model_nodes, values, state, chem, chem_he, smap):
@@ -462,48 +490,88 @@ _SCALE = {"T": 5000.0, "vt": 1e5, "vlos": 1e5, "B": 1000.0, "theta": 1.0, "phi":
 _BRACKET = {"T": (3400.0, 12000.0), "vt": (1e2, 15e5), "vlos": (-15e5, 15e5), "B": (1.0, 1e4)}
 
 
+def response_ray(ray):
+    """The ray of the reference's response path: H- at the window's middle
+    wavelength (h_minus_mol::op_em_vector_plus_pert, mol.cc:414-469), where
+    the synthesis path evaluates it at every wavelength (mol.cc:379-412)."""
+    from snapi_amd import abi
+    return abi.SnapiRay(theta=ray.theta, lam_mid=ray.lam_mid, tau_grid=ray.tau_grid, hminus=ray.hminus, hminus_mid=1)
+
+
+def synthesis_ray(ray):
+    from snapi_amd import abi
+    return abi.SnapiRay(theta=ray.theta, lam_mid=ray.lam_mid, tau_grid=ray.tau_grid, hminus=ray.hminus, hminus_mid=0)
+
+
+def bracket(model_nodes, v):
+    """model::bracket_parameter_values (atmos_model.cc:960-977)."""
+    out = np.array(v, dtype=np.float64)
+    for i, (p, _) in enumerate(model_nodes):
+        if p in _BRACKET:
+            out[i] = min(max(out[i], _BRACKET[p][0]), _BRACKET[p][1])
+    return out
+
+
+def polish_angles(model_nodes, v):
+    """model::polish_angles (atmos_model.cc:992-1001): inclination to
+    acos(cos theta), azimuth to atan(tan phi)."""
+    out = np.array(v, dtype=np.float64)
+    for i, (p, _) in enumerate(model_nodes):
+        if p == "theta":
+            out[i] = np.arccos(np.cos(out[i]))
+        elif p == "phi":
+            out[i] = np.arctan(np.tan(out[i]))
+    return out
+
+
 def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, smap, wstokes, max_iter=30,
-               lm_start=1.0, broadening=0.0):
+               lm_start=1.0, broadening=0.0, mask=None, history=False):
     """stokes_lm_fit (atmos_fit.cc:18-227) of one pixel with temperature
-    nodes, in numpy over the oracle's chain: the Jacobian and the current
-    chi^2 from the response path (node_jacobian_eos), the trial's chi^2 from
-    the synthesis path, responses scaled by the parameter scales and Stokes
-    weights (scale_rf), (1 + lambda) damping and the truncated-SVD solve,
-    corrections scaled back and bracketed, lambda / sqrt(10) (floor 1e-5) on
-    success and * sqrt(10) on failure, stop after three accepted steps whose
-    chi^2 changes by less than 1e-3 or at max_iter.  With broadening [cm] the
-    current spectrum and every column of the Jacobian (before the scaling) and
-    every trial spectrum are convolved with the Gaussian (atmos_fit.cc:104-110,
-    717-737).  Returns (nodes, chi2, iterations)."""
+    nodes, in numpy over the oracle's chain: the start bracketed into the
+    parameter limits (:91); the Jacobian and the current chi^2 from the
+    response path (node_jacobian_eos, H- at the middle wavelength), the
+    trial's chi^2 from the synthesis path (forward_evaluate, :717-737);
+    responses scaled by the parameter scales, Stokes weights and wavelength
+    mask (scale_rf, :250-260), residuals by the weights and mask
+    (calc_residual, :229-236), (1 + lambda) damping and the truncated-SVD
+    solve, corrections scaled back and bracketed, lambda / sqrt(10) (floor
+    1e-5) on success and * sqrt(10) on failure, stop after three accepted
+    steps whose chi^2 changes by less than 1e-3 or at max_iter; the angles
+    polished at the end (:217).  With broadening [cm] the current spectrum
+    and every column of the Jacobian (before the scaling) and every trial
+    spectrum are convolved with the Gaussian (:104-110).  Returns (nodes,
+    chi2, iterations), chi2 the response path's of the last Jacobian, and with
+    history also the nodes after every iteration."""
     ws = np.asarray(wstokes, dtype=np.float64)
+    wl = np.ones(len(lam)) if mask is None else np.asarray(mask, dtype=np.float64)
+    w = (wl[:, None] * ws[None, :]).reshape(-1)
     sc = np.array([_SCALE[p] for p, _ in model_nodes])
-    v = np.array(start, dtype=np.float64)
+    rray, sray = response_ray(ray), synthesis_ray(ray)
+    v = bracket(model_nodes, start)
     lm = lm_start
     need, track, chi = True, [], 0.0
     it = 0
+    hist = []
 
     def conv(a):
         return convolve_gauss(a, lam, broadening) if broadening > 0.0 else a
 
     def forward(vals):
         st = forward_state_eos(model_nodes, vals, state, chem, chem_he, smap)
-        s, _ = synth(table, ray, st[None], lam)
+        s, _ = synth(table, sray, st[None], lam)
         return conv(s[0])
 
     while it < max_iter:
         if need:
-            S, J = node_jacobian_eos(table, ray, model_nodes, v, state, lam, chem, chem_he, smap)
+            S, J = node_jacobian_eos(table, rray, model_nodes, v, state, lam, chem, chem_he, smap)
             S = conv(S)
             J = conv(J.reshape(len(lam), 4 * len(model_nodes))).reshape(len(lam) * 4, len(model_nodes))
-            r = ((obs - S) * ws).reshape(-1)
+            r = (obs - S).reshape(-1) * w
             chi = float((r ** 2).sum())
-            Js = J * sc[None, :] * np.tile(ws, len(lam))[:, None]
+            Js = J * sc[None, :] * w[:, None]
         _, _, corr = lm_step(Js, r, lm)
-        vt = v + corr * sc
-        for i, (p, _) in enumerate(model_nodes):
-            if p in _BRACKET:
-                vt[i] = min(max(vt[i], _BRACKET[p][0]), _BRACKET[p][1])
-        chit = float((((forward(vt) - obs) * ws) ** 2).sum())
+        vt = bracket(model_nodes, v + corr * sc)
+        chit = float((((forward(vt) - obs).reshape(-1) * w) ** 2).sum())
         it += 1
         stop = False
         if chit < chi:
@@ -515,8 +583,12 @@ def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, s
         else:
             lm *= np.sqrt(10.0)
             need = False
+        hist.append(polish_angles(model_nodes, v))
         if stop:
             break
+    v = polish_angles(model_nodes, v)
+    if history:
+        return v, chi, it, np.array(hist)
     return v, chi, it
 
 

@@ -3,7 +3,7 @@ code (built from /root/reference by oracle/ref.mk into oracle/_ref/).
 
 Run here only (the reference is not present where the GPU tests run):
 
-    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py
+    make -f oracle/ref.mk -j8 && python tests/golden/make_golden.py [fixture ...]
 
 Inputs are written into a scratch directory in the reference's own config
 formats (atmos{} block, atom{} files, SPINOR 1-D atmosphere): the atom files
@@ -708,22 +708,169 @@ def build_nlteresp(d):
     print("nlteresp: system %d x %d, %d formal problems" % (n, n, npr))
 
 
+# The 14-node model of the inversion workload (7 T, 1 vt, 2 vLOS, 2 B, 1
+# inclination, 1 azimuth; codes as the reference's model orders them) and two
+# node sets: a field inclined towards the observer and one pointing away with
+# a negative azimuth, so that the acos/atan folds of the field projection are
+# both exercised.
+FIT_NODES = [(1, -5.0), (1, -4.0), (1, -3.0), (1, -2.0), (1, -1.0), (1, 0.0), (1, 0.5), (2, -2.0), (3, -4.0),
+             (3, 0.0), (4, -4.0), (4, 0.0), (5, -2.0), (6, -2.0)]
+NODE_SETS = [
+    [4600.0, 4450.0, 4500.0, 4900.0, 5300.0, 6300.0, 7600.0, 1.1e5, 0.8e5, -0.5e5, 900.0, 1300.0, 0.7, 0.4],
+    [4700.0, 4400.0, 4550.0, 4800.0, 5400.0, 6200.0, 7400.0, 0.6e5, -1.2e5, 0.3e5, 2200.0, 1500.0, 2.6, -0.9],
+]
+
+
+def fit_lambda():
+    """Every third point of the 301-point grid: 101 points over both lines."""
+    return cfg1_lambda()[::3]
+
+
+def write_model(path, values, tau_min=-6.0, tau_max=1.0):
+    with open(path, "w") as f:
+        f.write("%.17g %.17g\n%d\n" % (tau_min, tau_max, len(FIT_NODES)))
+        for (code, lt), v in zip(FIT_NODES, values):
+            f.write("%d %.17g %.17g\n" % (code, lt, v))
+
+
+def run_nodes(d, values, tag):
+    write_model(os.path.join(d, "model_%s.txt" % tag), values)
+    out = run(d, ["nodes", "atmos.cfg", "lambda.txt", repr(math.pi), "0.0", "model_%s.txt" % tag], he=False)
+    rec = {k: [] for k in ("bfn", "proj", "fwd", "stokes", "jac", "nmap", "model", "post")}
+    for ln in out:
+        t = ln.split()
+        if t and t[0] in rec:
+            rec[t[0]].append([float(x) for x in t[1:]])
+    nl = len(rec["fwd"])
+    npar = len(FIT_NODES)
+    nd = len(rec["bfn"])
+    jac = np.zeros((npar, nl, 4))
+    for r in rec["jac"]:
+        jac[int(r[0]) - 1, int(r[1]) - 1] = r[2:]
+    nmap = np.zeros((npar, 7, nd))
+    for r in rec["nmap"]:
+        nmap[int(r[0]) - 1, int(r[1]) - 1, int(r[2])] = r[3]
+    return dict(bfn=np.array(rec["bfn"])[:, 1:], proj=np.array(rec["proj"])[:, 1:], fwd=np.array(rec["fwd"])[:, 1:],
+                stokes=np.array(rec["stokes"])[:, 1:], jac=jac, nmap=nmap, model_after=np.array(rec["model"][0][1:]),
+                post=np.array(rec["post"])[:, 1:])
+
+
+def build_nodes(d):
+    """obs_stokes_responses_to_nodes with build_from_nodes for the 14-node
+    model on the 71-depth grid (log tau -6..1) at two node sets: the built
+    state (tau, T, Nt, Ne, op_ref, rho, vt, vz, Bx, By, Bz), the ray
+    projection (vLOS, |B|, inclination and azimuth in degrees), the
+    synthesis-path Stokes (forward_evaluate), the response-path Stokes, the
+    Jacobian [node][lambda][4] and the node responses [node][7][depth]."""
+    rows = cfg0_atmosphere(71)
+    lam = fit_lambda()
+    setup_dir(d, rows, lam)
+    recs = dict(codes=np.array([c for c, _ in FIT_NODES]), logtau=np.array([t for _, t in FIT_NODES]), lam_air=lam)
+    for i, vals in enumerate(NODE_SETS):
+        r = run_nodes(d, vals, "n%d" % i)
+        recs["values%d" % i] = np.array(vals)
+        for k, v in r.items():
+            recs["%s%d" % (k, i)] = v
+    np.savez_compressed(os.path.join(HERE, "nodes.npz"), **recs)
+    print("nodes: %d node sets, %d wavelengths" % (len(NODE_SETS), len(lam)))
+
+
+def _fit_cases():
+    """Three pixels: observations are the reference's synthesis-path spectrum
+    at node set 0 or 1 plus Gaussian noise of 1e-3 of the first wavelength's
+    intensity; starts perturbed by 120 K rms in temperature and +-20 % in the
+    other nodes; pixel 1 with 30 % of the wavelengths masked; pixel 2 starts
+    with B = 0.5 G and vt = 50 cm/s, below the fit's bracket."""
+    rng = np.random.default_rng(2024)
+    cases = []
+    for k, base in enumerate((0, 1, 0)):
+        truth = np.array(NODE_SETS[base])
+        start = truth.copy()
+        start[:7] += rng.normal(0.0, 120.0, 7)
+        start[7:] *= rng.uniform(0.8, 1.2, 7)
+        if k == 2:
+            start[7] = 50.0
+            start[10] = 0.5
+        cases.append(dict(base=base, truth=truth, start=start, seed=100 + k, masked=(k == 1)))
+    return cases
+
+
+def build_fit(d, kmax=8):
+    """atmosphere::stokes_lm_fit on three pixels of the 14-node model, run with
+    1..kmax iterations (the nodes and synthesis-path chi^2 after each, i.e.
+    the trajectory), weights (1, 5, 5, 3), start lambda 1."""
+    rows = cfg0_atmosphere(71)
+    lam = fit_lambda()
+    setup_dir(d, rows, lam)
+    ws = np.array([1.0, 5.0, 5.0, 3.0])
+    recs = dict(codes=np.array([c for c, _ in FIT_NODES]), logtau=np.array([t for _, t in FIT_NODES]), lam_air=lam,
+                ws=ws, kmax=np.array(kmax), lm_start=np.array(1.0))
+    procs = []
+    for k, cs in enumerate(_fit_cases()):
+        fwd = run_nodes(d, cs["truth"], "t%d" % k)["fwd"]
+        rng = np.random.default_rng(cs["seed"])
+        obs = fwd + rng.normal(0.0, 1e-3 * fwd[0, 0], fwd.shape)
+        mask = np.ones(len(lam))
+        if cs["masked"]:
+            mask[rng.permutation(len(lam))[: int(0.3 * len(lam))]] = 0.0
+        path = os.path.join(d, "fit_%d.txt" % k)
+        write_model(path, cs["start"])
+        with open(path, "a") as f:
+            f.write("%.17g %.17g %.17g %.17g %.17g %d\n" % (ws[0], ws[1], ws[2], ws[3], 1.0, kmax))
+            f.write(" ".join("%.17g" % m for m in mask) + "\n")
+            for row in obs:
+                f.write(" ".join("%.17g" % v for v in row) + "\n")
+        recs.update({"truth%d" % k: cs["truth"], "start%d" % k: cs["start"], "obs%d" % k: obs, "mask%d" % k: mask})
+        env = dict(os.environ)
+        env["SNAPI_REF_HE"] = "0"
+        procs.append(subprocess.Popen([DRIVER, "fit", "atmos.cfg", "lambda.txt", repr(math.pi), "0.0", path], cwd=d,
+                                      env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+    for k, pr in enumerate(procs):
+        out, err = pr.communicate()
+        if pr.returncode:
+            raise RuntimeError("fit driver failed on pixel %d" % k)
+        # stderr of the longest run: one "test chi-squared" line per iteration
+        # (the current chi^2 / (4 nlambda 1e-6), atmos_fit.cc:157-158)
+        runs, cur = {}, None
+        for ln in err.splitlines():
+            if ln.startswith("fitrun "):
+                cur = int(ln.split()[1])
+                runs[cur] = []
+            elif "test chi-squared" in ln and cur is not None:
+                runs[cur].append(float(ln.split("=")[1].split(",")[0]) * 4.0 * len(lam) * 1e-6)
+        recs["iters%d" % k] = np.array([len(runs.get(i, [])) for i in range(1, kmax + 1)])
+        recs["chicur%d" % k] = np.array(runs.get(kmax, []))
+        nodes, chi, S = [], [], []
+        for ln in out.splitlines():
+            t = ln.split()
+            if not t:
+                continue
+            if t[0] == "fitk":
+                nodes.append([float(x) for x in t[2:]])
+            elif t[0] == "fitchi":
+                chi.append(float(t[2]))
+            elif t[0] == "fitS":
+                S.append([float(x) for x in t[2:]])
+        recs["nodes%d" % k] = np.array(nodes)
+        recs["chi%d" % k] = np.array(chi)
+        recs["S%d" % k] = np.array(S)
+        print("fit pixel %d: chi2 %s" % (k, np.array2string(np.array(chi), precision=3)))
+    np.savez_compressed(os.path.join(HERE, "fit.npz"), **recs)
+    print("fit: %d pixels, %d iterations" % (len(procs), kmax))
+
+
+BUILDERS = ["cfg0", "cfg1_template", "resp", "formal_scalar", "lmstep", "interp", "conv", "ana", "nlte", "chemeq",
+            "nlteresp", "nodes", "fit"]
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
+    want = sys.argv[1:] or BUILDERS
     d = tempfile.mkdtemp(prefix="snapi_golden_")
     try:
-        build_cfg0(d)
-        build_cfg1_template(d)
-        build_resp(d)
-        build_formal_scalar(d)
-        build_lmstep(d)
-        build_interp(d)
-        build_conv(d)
-        build_ana(d)
-        build_nlte(d)
-        build_chemeq(d)
-        build_nlteresp(d)
+        for name in want:
+            globals()["build_" + name](d)
     finally:
         shutil.rmtree(d)
 

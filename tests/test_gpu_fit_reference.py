@@ -1,0 +1,110 @@
+"""The GPU fit path against the reference itself (tests/golden/nodes.npz and
+fit.npz, written by the compiled reference): snapi_build_from_nodes against
+atmos_pp::build_from_nodes (plpar/atmos_pp.cc:58-248) and the ray projection
+(atmos/atmos_rad.cc:15-51); snapi_node_jacobian_eos against
+obs_stokes_responses_to_nodes (atmos/atmos_fit.cc:1205-1352); snapi_invert_eos
+against the trajectory of stokes_lm_fit (atmos/atmos_fit.cc:18-227), eight
+iterations on three pixels (one masked, one starting below the parameter
+limits).
+
+Tolerances: the GPU's Jacobian is the exact derivative through the solver,
+the reference's a difference once per node direction; in FP64 each node's
+column agrees to 3e-4 of its largest entry, and the trajectory's nodes to
+2e-3 of their values.  Mixed precision: Stokes to 1e-4 of Ic, Jacobian
+columns to 1e-3."""
+import numpy as np
+import pytest
+
+import _golden
+from snapi_amd import abi, invert, workloads
+
+pytestmark = pytest.mark.gpu
+
+NAMES = {1: "T", 2: "vt", 3: "vlos", 4: "B", 5: "theta", 6: "phi"}
+
+
+def airtovac(lam_cm):
+    s = 1e4 / (lam_cm * 1e8)
+    n = 1.0 + 0.00008336624212083 + 0.02408926869968 / (130.1065924522 - s * s) \
+        + 0.0001599740894897 / (38.92568793293 - s * s)
+    return lam_cm * n
+
+
+@pytest.fixture(scope="module")
+def setup():
+    g = _golden.load("nodes.npz")
+    f = _golden.load("fit.npz")
+    t = workloads.template_cfg1()
+    chem, chem_he, smap = workloads.eos_tables("cfg1_template", t["ray"].lam_mid)
+    nodes = [(NAMES[int(c)], float(l)) for c, l in zip(g["codes"], g["logtau"])]
+    st = t["state"].copy()
+    st[abi.F_GRID] = g["bfn0"][:, 0]
+    lam = np.array([airtovac(x * 1e-8) for x in g["lam_air"]])
+    return dict(g=g, f=f, t=t, chem=chem, chem_he=chem_he, smap=smap, nodes=nodes, model=invert.node_model(nodes),
+                state=st, lam=lam)
+
+
+def test_build_from_nodes_matches_reference(setup):
+    s = setup
+    g = s["g"]
+    vals = np.stack([g["values0"], g["values1"]])
+    print("build_from_nodes vs reference: 2 node sets, %d depths" % s["state"].shape[1], flush=True)
+    st = invert.build_from_nodes(s["model"], s["chem"], s["chem_he"], s["smap"],
+                                 np.repeat(s["state"][None], 2, axis=0), vals)
+    for k in range(2):
+        bfn, proj = g["bfn%d" % k], g["proj%d" % k]
+        errs = dict(T=np.abs(st[k, abi.F_T] / bfn[:, 1] - 1).max(), Nt=np.abs(st[k, abi.F_NT] / bfn[:, 2] - 1).max(),
+                    Ne=np.abs(st[k, abi.F_NE] / bfn[:, 3] - 1).max(),
+                    vlos=np.abs(st[k, abi.F_VLOS] - proj[:, 0]).max() / np.abs(proj[:, 0]).max(),
+                    B=np.abs(st[k, abi.F_BMAG] / proj[:, 1] - 1).max(),
+                    inc=np.abs(st[k, abi.F_BINC] - proj[:, 2]).max(), azi=np.abs(st[k, abi.F_BAZI] - proj[:, 3]).max())
+        print("  set %d: " % k + " ".join("%s %.1e" % kv for kv in errs.items()), flush=True)
+        assert max(errs.values()) < 1e-9
+
+
+@pytest.mark.parametrize("precision", [abi.SNAPI_FP64, abi.SNAPI_MIXED])
+def test_node_jacobian_matches_reference(setup, precision):
+    s = setup
+    g = s["g"]
+    vals = np.stack([g["values0"], g["values1"]])
+    st = np.repeat(s["state"][None], 2, axis=0)
+    print("node Jacobian vs obs_stokes_responses_to_nodes: 2 node sets, %d wavelengths, precision %d"
+          % (len(s["lam"]), precision), flush=True)
+    S, J = invert.node_jacobian_eos(s["t"]["table"], s["t"]["ray"], s["model"], s["chem"], s["chem_he"], s["smap"],
+                                    st, vals, s["lam"], precision=precision)
+    bar_s, bar_j = (1e-10, 3e-4) if precision == abi.SNAPI_FP64 else (1e-4, 1e-3)
+    for k in range(2):
+        ref = g["stokes%d" % k]
+        es = np.abs(S[k].astype(np.float64) - ref).max() / ref[:, 0].max()
+        jr = np.transpose(g["jac%d" % k], (1, 2, 0)).reshape(-1, len(s["nodes"]))
+        ej = np.abs(J[k].astype(np.float64) - jr).max(axis=0) / np.abs(jr).max(axis=0)
+        print("  set %d: Stokes %.2e Ic; J per node max %.2e (node %d)" % (k, es, ej.max(), ej.argmax()), flush=True)
+        assert es < bar_s
+        assert ej.max() < bar_j
+
+
+def test_inversion_follows_reference_trajectory(setup):
+    """snapi_invert_eos run with 1..8 iterations on the three pixels at once:
+    after each, the nodes within 2e-3 of the reference's and the same
+    iteration counts (the reference runs to its cap here)."""
+    s = setup
+    f = s["f"]
+    K = int(f["kmax"])
+    obs = np.stack([f["obs%d" % p] for p in range(3)])
+    mask = np.stack([f["mask%d" % p] for p in range(3)])
+    start = np.stack([f["start%d" % p] for p in range(3)])
+    st = np.repeat(s["state"][None], 3, axis=0)
+    print("inversion vs stokes_lm_fit: 3 pixels, 1..%d iterations, FP64" % K, flush=True)
+    worst = 0.0
+    for k in range(1, K + 1):
+        nodes, chi2, iters, total = invert.invert_eos(s["t"]["table"], s["t"]["ray"], s["model"], s["chem"],
+                                                      s["chem_he"], s["smap"], st, s["lam"], obs, start, mask=mask,
+                                                      wstokes=f["ws"], lm_start=float(f["lm_start"]), max_iter=k,
+                                                      nslot=3, precision=abi.SNAPI_FP64)
+        for p in range(3):
+            ref = f["nodes%d" % p][k - 1]
+            rel = np.abs(nodes[p] - ref) / np.abs(ref)
+            worst = max(worst, rel.max())
+            assert rel.max() < 2e-3, (k, p, rel.max(), rel.argmax())
+            assert iters[p] == f["iters%d" % p][k - 1]
+    print("  worst node difference %.2e" % worst, flush=True)

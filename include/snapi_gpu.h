@@ -726,8 +726,10 @@ int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *ma
  * Nt are contracted with the temperature nodes' responses dT/dnode and
  * dNt/dnode, differences over delta_T = 5 K of two rebuilds each.  The
  * Stokes vector returned is the response path's, as the reference returns
- * it.  At least one temperature node; adjoint Jacobian only
- * (SNAPI_ERR_UNSUPPORTED otherwise).  J per K for temperature nodes.
+ * it, with H- at ray->lam_mid as that path evaluates it
+ * (atmos/atmol/mol/mol.cc:414-469) whatever ray->hminus_mid says.  At least
+ * one temperature node; adjoint Jacobian only (SNAPI_ERR_UNSUPPORTED
+ * otherwise).  J per K for temperature nodes.
  */
 /*
  * snapi_invert[_dev] with temperature nodes: per pixel and iteration the
@@ -736,8 +738,13 @@ int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *ma
  * trial's chi^2 from the synthesis path's state (forward_evaluate), as
  * stokes_lm_fit takes them (atmos_fit.cc:18-227).  The state's
  * thermodynamic fields are rebuilt; its grid and, for parameters without
- * nodes, the fields stay as given.  At least one temperature node; adjoint
- * Jacobian only.  broadening > 0: the Gaussian spectral broadening [cm] the
+ * nodes, the fields stay as given.  As the reference: H- at ray->lam_mid on
+ * the response path and at every wavelength on the trials (mol.cc:414-469,
+ * 379-412) whatever ray->hminus_mid says; the start nodes bracketed into
+ * the parameter limits before the first iteration (atmos_fit.cc:91); the
+ * returned inclination and azimuth polished to acos(cos) and atan(tan)
+ * (model::polish_angles, atmos/atmos_model.cc:992-1001).  At least one
+ * temperature node; adjoint Jacobian only.  broadening > 0: the Gaussian spectral broadening [cm] the
  * fit applies to the current spectrum and Jacobian (before the Stokes weights
  * and the mask) and to every trial spectrum (snapi_spectral_convolve;
  * atmos_fit.cc:104-110, 717-737); 0: none.

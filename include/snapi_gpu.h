@@ -12,6 +12,10 @@
  *
  * Reference interfaces replaced (ivanzmilic/snapi):
  *   snapi_synth_stokes*      atmosphere::obs_stokes                 atmos/atmos_obs.cc:16-87
+ *   snapi_line_add, snapi_zeeman_pattern, snapi_vdw_damping, snapi_radiative_damping
+ *                            the atom class's line data: atom::atom, damp_rad,
+ *                            compute_damp_col, zeeman_setup  atmos/atmol/atom/atom.cc:233-253,
+ *                                                                   926-935, 987-1308, 1620-1773
  *                            (atmosphere::op_em_vector              atmos/atmos_rad_synthesis.cc:20-72
  *                             + atmos_ppbez::formal (polarized)     atmos/plpar/atmos_ppbez_rts.cc:918-1218)
  *   snapi_stokes_responses*  atmosphere::obs_stokes_responses       atmos/atmos_rad_synthesis.cc:166-399
@@ -134,6 +138,51 @@ typedef struct {
   snapi_line line[SNAPI_MAX_LINES];
   snapi_component comp[SNAPI_MAX_COMPONENTS];
 } snapi_line_table;
+
+/*
+ * Building the line table from level data, as the reference's atom class
+ * does (host functions; no device is touched).
+ *
+ * snapi_line_add appends one line to t (and its Zeeman components to
+ * t->comp): lam0 = h c / (E_u - E_l); B_ul = lam0^3 / (2 h c) A_ul and
+ * B_lu = g_u / g_l B_ul, both times lam0^2 / c (atom::atom,
+ * atmos/atmol/atom/atom.cc:233-253); the van der Waals cross-section and
+ * exponent of snapi_vdw_damping; the Zeeman pattern of
+ * snapi_zeeman_pattern.  stage is the ionisation stage of both levels (0
+ * neutral), e_ion its ionisation energy [erg], energies [erg] from the
+ * stage's ground level, grad the radiative damping (snapi_radiative_damping).
+ * For hydrogen the reference sets l = level index - 1 (atom.cc:999-1004);
+ * pass l that way.
+ *
+ * snapi_vdw_damping: atom::compute_damp_col (atom.cc:987-1308): effective
+ * quantum numbers n* = (stage + 1) sqrt(13.6 eV / (e_ion - E)); for neutral
+ * s-p, p-d and d-f transitions (l_lower + l_upper = 1, 3, 5) the
+ * Anstee-Barklem-O'Mara tables interpolated at the two n* (interpol_2d,
+ * cmlib/mathtools.cc:1006-1041), otherwise Unsold's C6 with alpha = 0.4.
+ *
+ * snapi_zeeman_pattern: atom::zeeman_setup (atom.cc:1620-1773) for levels of
+ * 2J = j2_lower, j2_upper and Lande factors g_lower, g_upper: split = 1 for
+ * Delta J = +-1 and for Delta J = 0 with J != 0; then ncomp[0] pi, ncomp[1]
+ * sigma_b and ncomp[2] sigma_r components written to comp in that order,
+ * shifts in units of lambda_B and strengths normalised per group.
+ *
+ * snapi_radiative_damping: atom::damp_rad (atom.cc:926-935), the sum of the
+ * A of every transition out of both levels; A [nlevel][nlevel], A[u][l].
+ */
+typedef struct {
+  double energy;  /* excitation energy [erg] */
+  double g;       /* statistical weight */
+  int j2;         /* 2 J */
+  int l;          /* orbital quantum number of the level's configuration */
+  double g_lande; /* Lande factor */
+} snapi_level;
+int snapi_zeeman_pattern(int j2_lower, int j2_upper, double g_lower, double g_upper, int *split, int *ncomp,
+                         snapi_component *comp, int max_comp);
+int snapi_vdw_damping(int stage, double e_ion, double e_lower, double e_upper, int l_lower, int l_upper, double *sigma,
+                      double *alpha);
+int snapi_radiative_damping(int nlevel, const double *A, int lower, int upper, double *grad);
+int snapi_line_add(snapi_line_table *t, double mass, int stage, double e_ion, const snapi_level *lower,
+                   const snapi_level *upper, double aul, double grad);
 
 /* Ray and grid options shared by the synthesis and response entry points. */
 typedef struct {

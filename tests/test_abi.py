@@ -48,7 +48,8 @@ def test_struct_layout_matches_header():
              ("snapi_ali_grid", abi.SnapiAliGrid), ("snapi_partition", abi.SnapiPartition),
              ("snapi_chem_element", abi.SnapiChemElement), ("snapi_chem_molecule", abi.SnapiChemMolecule),
              ("snapi_chem_level", abi.SnapiChemLevel), ("snapi_chem_bf", abi.SnapiChemBf),
-             ("snapi_chem_table", abi.SnapiChemTable), ("snapi_state_map", abi.SnapiStateMap)]
+             ("snapi_chem_table", abi.SnapiChemTable), ("snapi_state_map", abi.SnapiStateMap),
+             ("snapi_level", __import__("snapi_amd.lines", fromlist=["SnapiLevel"]).SnapiLevel)]
     lines = []
     want = []
     for cname, cls in pairs:

@@ -452,16 +452,61 @@ def ali_leg(torch, dev, stream, side, with_cpu):
         q = {k: np.ascontiguousarray(p[k][:nc]) for k in ("colstate", "vproj", "gamma", "bgop", "bgem", "coll")}
         q.update(atom=p["atom"], grid=p["grid"], nd=nd, ntrans=p["atom"].ntrans, nlevel=p["atom"].nlevel)
         t1 = time.time()
-        _, itc, _ = _oracle.ali_solve(q, p["pops"][:nc], 1e-4, 300, ncol=nc, nthreads=nthreads)
+        opops, itc, _ = _oracle.ali_solve(q, p["pops"][:nc], 1e-4, 300, ncol=nc, nthreads=nthreads)
         dt = time.time() - t1
         cpu = {"value": float(itc.sum()) * nray * nlam / dt, "unit": "ALI formal solves/s", "cores": nthreads,
                "kind": "port", "sample": "%d columns of the same workload to 1e-4 (FP64 oracle, OpenMP)" % nc,
                "seconds": round(dt, 2)}
-        log("ALI cpu oracle: %.4g formal solves/s on %d threads" % (cpu["value"], nthreads))
+        cpu["fp64_gpu_populations_rel_err_vs_oracle"] = float(np.max(np.abs(p64[:nc] / opops - 1.0)))
+        cpu["mixed_populations_rel_err_vs_oracle"] = float(np.max(np.abs(pm[:nc] / opops - 1.0)))
+        cpu["iterations_equal_fp64_gpu"] = bool(np.array_equal(it64[:nc], itc))
+        log("ALI cpu oracle: %.4g formal solves/s on %d threads; populations vs oracle: fp64 GPU %.2e, mixed %.2e"
+            % (cpu["value"], nthreads, cpu["fp64_gpu_populations_rel_err_vs_oracle"],
+               cpu["mixed_populations_rel_err_vs_oracle"]))
+    syn = ca8542_leg(torch, dev, stream, p, pm, with_cpu)
     return {"workload": "NLTE Ca II 5 levels + continuum by ALI, %d columns x %d depths, %d wavelengths, %d rays, "
                         "to relative change 1e-4" % (ncol, nd, nlam, nray),
             "value": mixed["formal_solves_per_s"], "unit": "ALI formal solves/s (column x ray x wavelength)",
-            "mixed": mixed, "fp64_gpu": fp64, "populations_rel_err_mixed_vs_fp64": err, "cpu_baseline": cpu}
+            "mixed": mixed, "fp64_gpu": fp64, "populations_rel_err_mixed_vs_fp64": err, "cpu_baseline": cpu,
+            "synthesis_8542": syn}
+
+
+def ca8542_leg(torch, dev, stream, p, pops, with_cpu):
+    """Polarized Ca II 8542 A synthesis (|B| <= 500 G) from the ALI leg's
+    converged mixed-precision populations on every column, mixed precision,
+    HIP events on the launch stream; the FP64 oracle on 16 columns checks it."""
+    from snapi_amd import abi, api, workloads
+    tb, ray, st, lam = workloads.ca8542_synthesis(p, pops)
+    ncol, nf, nd = st.shape
+    st_t = torch.from_numpy(st.astype(np.float32)).to(dev)
+    lam_t = torch.from_numpy(lam).to(dev)
+    out = torch.zeros((ncol, len(lam), 4), dtype=torch.float32, device=dev)
+    api.synth_stokes_dev(tb, ray, st_t, lam_t, out, abi.SNAPI_MIXED, stream=stream)  # kernel loading, not timed
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+        for _ in range(5):
+            api.synth_stokes_dev(tb, ray, st_t, lam_t, out, abi.SNAPI_MIXED, stream=stream)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    res = {"workload": "Ca II 8542 A polarized synthesis from the converged populations, %d columns x %d depths, "
+                       "%d wavelengths, |B| <= 500 G, mixed" % (ncol, nd, len(lam)),
+           "ms_per_launch": round(ms, 3), "lambda_point_solves_per_s": ncol * len(lam) / (ms * 1e-3)}
+    log("8542 synthesis: %d columns x %d wavelengths in %.3f ms = %.4g lambda-point solves/s"
+        % (ncol, len(lam), ms, res["lambda_point_solves_per_s"]))
+    if with_cpu:
+        import _oracle
+        nc = 16
+        ref, _ = _oracle.synth(tb, ray, st[:nc], lam)
+        g = out[:nc].cpu().numpy().astype(np.float64)
+        res["max_err_ic_vs_oracle"] = float(np.abs(g - ref).max() / ref[..., 0].max())
+        res["max_abs_V_over_Ic"] = float(np.abs(ref[..., 3]).max() / ref[..., 0].max())
+        log("8542 synthesis vs oracle on %d columns: %.2e Ic (max |V|/Ic %.2e)"
+            % (nc, res["max_err_ic_vs_oracle"], res["max_abs_V_over_Ic"]))
+    return res
 
 
 def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):

@@ -318,12 +318,123 @@ def ali_problem(nx, ny, nd=81, nlam=220, nmu=5, seed=9):
     grid = abi.make_ali_grid(lam, lamw, np.ones_like(lam), sig, theta, wray, 0, t["boundary"], 1)
     vproj = np.zeros((ncol, len(theta), nd))
     gamma = np.repeat(rs(t["gamma"])[None], ncol, 0)
-    coll = np.repeat(rs(np.moveaxis(t["coll"], 0, -1), log=False)[None], ncol, 0)
-    coll = np.ascontiguousarray(np.moveaxis(coll, -1, 1))
-    pops = np.ascontiguousarray(np.repeat(rs(t["pop0"].T).T[None], ncol, 0))
+    # collision rates at the template's temperature and electron density,
+    # carried to each column's: excitation and ionisation i -> j scale as
+    # Ne T^-1/2 exp(-dE/kT) (van Regemorter, Seaton), de-excitation as
+    # Ne T^-1/2, three-body recombination as Ne^2 T^-2 (detailed balance with
+    # the Saha-Boltzmann ratio), so every pair keeps detailed balance at the
+    # column's temperature
+    coll0 = np.moveaxis(rs(np.moveaxis(t["coll"], 0, -1), log=False), -1, 0)  # [nd][n][n]
+    T0, N0 = cs0[abi.A_T], cs0[abi.A_NE]
+    Tc, Nc = cs[:, abi.A_T], cs[:, abi.A_NE]  # [ncol][nd]
+    n = t["nlevel"]
+    de = np.zeros(n)  # binding energy of each level below the continuum [erg]
+    cont = np.zeros(n, dtype=bool)
+    for tr in trans:
+        if tr["kind"] == abi.TR_BF:
+            de[tr["lower"]] = tr["de"]
+            cont[tr["upper"]] = True
+    kb = 1.3806503e-16
+    fT = np.sqrt(T0[None] / Tc)
+    fN = Nc / N0[None]
+    coll = np.empty((ncol, nd, n, n))
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                coll[:, :, i, j] = 0.0
+                continue
+            if cont[j] and not cont[i]:  # ionisation
+                f = fN * fT * np.exp(-de[i] / kb * (1.0 / Tc - 1.0 / T0[None]))
+            elif cont[i] and not cont[j]:  # three-body recombination
+                f = fN * fN * fT ** 4
+            elif j > i:  # excitation over de[i] - de[j]
+                f = fN * fT * np.exp(-(de[i] - de[j]) / kb * (1.0 / Tc - 1.0 / T0[None]))
+            else:  # de-excitation
+                f = fN * fT
+            coll[:, :, i, j] = coll0[None, :, i, j] * f
+    coll = np.ascontiguousarray(coll)
+    # start populations: LTE (Saha-Boltzmann) at each column's temperature and
+    # electron density with the atom's total population, as nltepops starts
+    # from ltepops
+    ratio = np.zeros((ncol, nd, n))
+    for tr in trans:
+        if tr["kind"] == abi.TR_BF:
+            ratio[:, :, tr["lower"]] = Nc * tr["gratio"] * 2.0706615382e-16 * Tc ** -1.5 * np.exp(tr["de"] / (kb * Tc))
+    ratio[:, :, cont] = 1.0
+    pops = np.ascontiguousarray(ratio * (cs[:, abi.A_NA] / ratio.sum(axis=2))[:, :, None])
     return dict(atom=atom, grid=grid, colstate=np.ascontiguousarray(cs), vproj=vproj,
                 gamma=np.ascontiguousarray(gamma), bgop=np.ascontiguousarray(bgop), bgem=bgem, coll=coll,
                 pops=pops, nlam=len(lam), nray=len(theta))
+
+
+CA_ABUNDANCE = 2.19e-6  # Ca / H by number (log eps = 6.34)
+
+
+def ca8542_synthesis(p, pops, seed=12, nlam=201, half_width=1.0e-8):
+    """Polarized synthesis inputs of the Ca II 8542 A line (3d 2D5/2 - 4p
+    2P3/2) in the ALI workload's columns with their converged populations
+    pops [ncol][nd][nlevel]: the line built from the atom's levels
+    (energies from the bound-free thresholds, g and 2J from the statistical
+    weights, LS Lande factors 1.2 and 4/3, the atom's A values;
+    snapi_line_add), the columns' temperature, microturbulence, electron
+    density, referent opacity and grid, the population-independent
+    background of the ALI grid point nearest the line centre (it holds H-,
+    so the synthesis adds none), a seeded field with |B| uniform on
+    0-500 G, cos(inclination) uniform on [-1, 1] and azimuth on [0, 180)
+    deg, no velocity, neutral hydrogen taken as the total hydrogen density
+    NA / A(Ca), and nlam vacuum wavelengths over +-half_width [cm].
+    Returns (table, ray, state [ncol][nfield][nd], lam_vac)."""
+    from . import lines as L
+    atom_tr = []
+    de = {}
+    for k in range(p["atom"].ntrans):
+        tr = p["atom"].tr[k]
+        atom_tr.append(tr)
+        if tr.kind == abi.TR_BF:
+            de[tr.lower] = (tr.de, tr.gratio)
+    n = p["atom"].nlevel
+    e0 = de[0][0]
+    energy = {i: e0 - de[i][0] for i in de}
+    A = np.zeros((n, n))
+    line = None
+    for tr in atom_tr:
+        if tr.kind == abi.TR_LINE:
+            A[tr.upper, tr.lower] = tr.aul
+            if abs(tr.lam0 - 8544.4e-8) < 2e-8:
+                line = tr
+    if line is None:
+        raise ValueError("the atom has no line at 8542 A")
+    lo, up = line.lower, line.upper
+    glo, gup = de[lo][1], de[up][1]
+    lower = L.level(energy[lo], glo, int(round(glo)) - 1, 2, 1.2)
+    upper = L.level(energy[up], gup, int(round(gup)) - 1, 1, 4.0 / 3.0)
+    t = L.empty_table()
+    L.add_line(t, p["atom"].mass, 1, e0, lower, upper, line.aul, L.radiative_damping(A, lo, up))
+    lam0 = t.line[0].lam0
+    lam = lam0 + np.linspace(-half_width, half_width, nlam)
+    cs = p["colstate"]
+    ncol, _, nd = cs.shape
+    rng = np.random.default_rng(seed)
+    nf = abi.nfield(1)
+    st = np.zeros((ncol, nf, nd))
+    st[:, abi.F_T] = cs[:, abi.A_T]
+    st[:, abi.F_VT] = cs[:, abi.A_VT]
+    st[:, abi.F_NE] = cs[:, abi.A_NE]
+    st[:, abi.F_OPREF] = cs[:, abi.A_OPREF]
+    st[:, abi.F_GRID] = cs[:, abi.A_GRID]
+    st[:, abi.F_NT] = cs[:, abi.A_NA] / CA_ABUNDANCE
+    st[:, abi.F_NHI] = cs[:, abi.A_NA] / CA_ABUNDANCE
+    st[:, abi.F_BMAG] = rng.uniform(0.0, 500.0, (ncol, 1))
+    st[:, abi.F_BINC] = np.degrees(np.arccos(rng.uniform(-1.0, 1.0, (ncol, 1))))
+    st[:, abi.F_BAZI] = rng.uniform(0.0, 180.0, (ncol, 1))
+    glam = np.array([p["grid"].lam[i] for i in range(p["grid"].nlam)])
+    il = int(np.argmin(np.abs(glam - lam0)))
+    st[:, abi.F_BGOP] = p["bgop"][:, 0, :, il]
+    st[:, abi.F_BGEM] = p["bgem"][:, 0, :, il]
+    st[:, abi.NF_BASE] = pops[:, :, lo]
+    st[:, abi.NF_BASE + 1] = pops[:, :, up]
+    ray = abi.SnapiRay(theta=np.pi, lam_mid=0.5 * (lam[0] + lam[-1]), tau_grid=1, hminus=0, hminus_mid=0)
+    return t, ray, np.ascontiguousarray(st), lam
 
 
 def response_column_from_dump(path):

@@ -34,6 +34,8 @@
  *   snapi_build_from_nodes*  atmos_pp::build_from_nodes + popsetup  atmos/plpar/atmos_pp.cc:58-180
  *   snapi_ana_*              ana_fzread / ana_fzwrite               cmlib/ana_read.cc, ana_write.cc
  *   snapi_cube_to_state_dev  atmosphere::read_spinor3d (+ project)  atmos/atmos_fio.cc:79-127
+ *   snapi_optical_depth_scale* atmos_ppbez::optical_depth_scale, compute_tau_referent
+ *                                                                   atmos/plpar/atmos_ppbez_rts.cc:221-266, 328-338
  *   snapi_stokes_to_cube_dev the job's spectra cube                 master/job.cc:590-597
  *   snapi_spectral_convolve* convolve_spectra_with_gauss, convolve_response_with_gauss
  *                                                                   cmlib/mathtools.cc:1350-1427, 1473-1498
@@ -702,6 +704,21 @@ int snapi_cube_to_state_dev(int precision, const double *cube_dev, int nx, int n
  * FLOAT64 file with dims (nlambda, 4, ny, nx)). */
 int snapi_stokes_to_cube_dev(int precision, const void *stokes_dev, int nx, int ny, int nlambda, double *cube_dev,
                              void *stream);
+
+/*
+ * Optical-depth scale on a height grid (atmos_ppbez::optical_depth_scale,
+ * atmos/plpar/atmos_ppbez_rts.cc:221-266): per column the Bezier integral of
+ * the opacity along the ray theta (reference convention: theta = 0 runs from
+ * the top down) with Fritsch-Butland derivatives, one-sided at the ends, and
+ * tau = 1e-8 at the first point.  With referent = 1 and theta = 0 this is
+ * compute_tau_referent (:328-338): tau_referent = -tau from the referent
+ * opacity, the grid the synthesis takes with tau_grid = 1.
+ *   height [ncol][nd] (x3, cm; top first), op [ncol][nd] (cm^-1), tau [ncol][nd] out.
+ */
+int snapi_optical_depth_scale(int ncol, int nd, const double *height, const double *op, double theta, int referent,
+                              double *tau);
+int snapi_optical_depth_scale_dev(int ncol, int nd, const double *height_dev, const double *op_dev, double theta,
+                                  int referent, double *tau_dev, void *stream);
 
 /* snapi_chemeq on device-resident arrays (T, Nt and the outputs: device
  * pointers; the table stays a host struct and is copied), all work on

@@ -78,3 +78,23 @@ def stokes_adjoint(table, ray, state, dstate, lam_vac, obs, wstokes=(1.0, 1.0, 1
                                     len(lam), abi.ptr(lam), abi.vptr(ob), abi.ptr(ws), abi.vptr(grad))
     check("snapi_stokes_adjoint", rc)
     return grad
+
+
+def optical_depth_scale(height, op, theta=0.0, referent=False):
+    """tau [ncol][nd] of snapi_optical_depth_scale (atmos_ppbez::optical_depth_scale,
+    plpar/atmos_ppbez_rts.cc:221-266); referent: -tau, compute_tau_referent."""
+    h = np.ascontiguousarray(height, dtype=np.float64)
+    o = np.ascontiguousarray(op, dtype=np.float64)
+    if h.ndim == 1:
+        h, o = h[None], o[None]
+    if h.shape != o.shape:
+        raise ValueError("height and opacity shapes differ: %s %s" % (h.shape, o.shape))
+    tau = np.zeros_like(h)
+    f = lib().snapi_optical_depth_scale
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double, C.c_int,
+                  C.POINTER(C.c_double)]
+    check("snapi_optical_depth_scale", f(h.shape[0], h.shape[1], abi.ptr(h), abi.ptr(o), float(theta),
+                                         1 if referent else 0, abi.ptr(tau)))
+    return tau
+

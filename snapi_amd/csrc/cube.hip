@@ -104,7 +104,74 @@ __global__ void stokes_to_cube_kernel(const R *__restrict__ stokes, long ncol, i
   cube[i] = (double)stokes[(col * nl + l) * 4 + s];
 }
 
+// Optical-depth scale of one column along a ray (atmos_ppbez::
+// optical_depth_scale, atmos/plpar/atmos_ppbez_rts.cc:221-266): the Bezier
+// integral of the opacity over the height grid with Fritsch-Butland
+// derivatives (one-sided at the ends), tau = 1e-8 at the first point, the
+// integration running downward for cos(theta) > 0 and upward otherwise.
+// One work-item per column; the depth loop is serial, and each depth reads
+// its neighbours' opacity and height once (HBM-bound, about 24 B per point).
+__global__ void tau_scale_kernel(int ncol, int nd, const double *__restrict__ x3, const double *__restrict__ op,
+                                 double ct, double sign, double *__restrict__ tau) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const double *x = x3 + (size_t)col * nd, *o = op + (size_t)col * nd;
+  double *t = tau + (size_t)col * nd;
+  const int dz = ct > 0.0 ? 1 : -1, zl = ct > 0.0 ? 0 : nd - 1, zh = ct > 0.0 ? nd - 1 : 0;
+  const double m = -ct;
+  // Fritsch-Butland derivative of the opacity along the ray at point z
+  auto deriv = [&](int z) {
+    if (z == zl) return (o[zl + dz] - o[zl]) / (x[zl + dz] - x[zl]) * m;
+    if (z == zh) return (o[zh] - o[zh - dz]) / (x[zh] - x[zh - dz]) * m;
+    const double h2 = (x[z + dz] - x[z]) / m, h1 = (x[z] - x[z - dz]) / m;
+    const double d2 = (o[z + dz] - o[z]) / h2, d1 = (o[z] - o[z - dz]) / h1;
+    if (!(d1 * d2 > 0.0)) return 0.0;
+    const double al = (1.0 + h1 / (h1 + h2)) / 3.0;
+    return d1 * d2 / (al * d1 + (1.0 - al) * d2);
+  };
+  double tt = 1E-8, Dp = deriv(zl);
+  t[zl] = sign * tt;
+  for (int z = zl + dz; (z - zh - dz) * dz < 0; z += dz) {
+    const double D = deriv(z);
+    const double h = (x[z] - x[z - dz]) / m;
+    const double C = 0.5 * ((o[z] - h / 2.0 * D) + (o[z - dz] + h / 2.0 * Dp));
+    tt += h / 3.0 * (o[z - dz] + o[z] + C);
+    t[z] = sign * tt;
+    Dp = D;
+  }
+}
+
 }  // namespace
+
+extern "C" int snapi_optical_depth_scale_dev(int ncol, int nd, const double *height_dev, const double *op_dev,
+                                             double theta, int referent, double *tau_dev, void *stream) {
+  if (ncol <= 0 || nd < 2 || !height_dev || !op_dev || !tau_dev) return SNAPI_ERR_ARG;
+  const double ct = cos(theta);
+  if (ct == 0.0) return SNAPI_ERR_ARG;
+  hipLaunchKernelGGL(tau_scale_kernel, dim3((ncol + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream), ncol,
+                     nd, height_dev, op_dev, ct, referent ? -1.0 : 1.0, tau_dev);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+extern "C" int snapi_optical_depth_scale(int ncol, int nd, const double *height, const double *op, double theta,
+                                         int referent, double *tau) {
+  if (ncol <= 0 || nd < 2 || !height || !op || !tau) return SNAPI_ERR_ARG;
+  const size_t n = (size_t)ncol * nd * sizeof(double);
+  double *dh = nullptr, *dop = nullptr, *dt = nullptr;
+  int rc = SNAPI_OK;
+  if (hipMalloc((void **)&dh, n) != hipSuccess || hipMalloc((void **)&dop, n) != hipSuccess ||
+      hipMalloc((void **)&dt, n) != hipSuccess || hipMemcpy(dh, height, n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dop, op, n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = SNAPI_ERR_DEVICE;
+  if (rc == SNAPI_OK) rc = snapi_optical_depth_scale_dev(ncol, nd, dh, dop, theta, referent, dt, nullptr);
+  if (rc == SNAPI_OK &&
+      (hipDeviceSynchronize() != hipSuccess || hipMemcpy(tau, dt, n, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = SNAPI_ERR_DEVICE;
+  hipFree(dh);
+  hipFree(dop);
+  hipFree(dt);
+  return rc;
+}
 
 extern "C" int snapi_ana_info(const char *path, int *type, int *ndim, int *dims, int *compressed) {
   if (!path || !type || !ndim || !dims) return SNAPI_ERR_ARG;

@@ -241,6 +241,34 @@ def cube_to_state(cube, nfield, theta=np.pi, phi=0.0):
     return out
 
 
+def optical_depth_scale(x3, op, theta=0.0):
+    """atmos_ppbez::optical_depth_scale (plpar/atmos_ppbez_rts.cc:221-266)
+    of one column: Fritsch-Butland derivatives of the opacity along the ray
+    (one-sided at both ends), tau = 1e-8 at the first point, Bezier steps
+    h/3 (o_(z-1) + o_z + (C0 + C1)/2)."""
+    ct = np.cos(theta)
+    nd = len(x3)
+    dz = 1 if ct > 0 else -1
+    zl, zh = (0, nd - 1) if ct > 0 else (nd - 1, 0)
+    m = -ct
+    D = np.zeros(nd)
+    D[zl] = (op[zl + dz] - op[zl]) / (x3[zl + dz] - x3[zl]) * m
+    D[zh] = (op[zh] - op[zh - dz]) / (x3[zh] - x3[zh - dz]) * m
+    for z in range(zl + dz, zh, dz):
+        h2, h1 = (x3[z + dz] - x3[z]) / m, (x3[z] - x3[z - dz]) / m
+        d2, d1 = (op[z + dz] - op[z]) / h2, (op[z] - op[z - dz]) / h1
+        if d1 * d2 > 0:
+            al = (1.0 + h1 / (h1 + h2)) / 3.0
+            D[z] = d1 * d2 / (al * d1 + (1.0 - al) * d2)
+    tt = np.zeros(nd)
+    tt[zl] = 1e-8
+    for z in range(zl + dz, zh + dz, dz):
+        h = (x3[z] - x3[z - dz]) / m
+        C = 0.5 * ((op[z] - h / 2.0 * D[z]) + (op[z - dz] + h / 2.0 * D[z - dz]))
+        tt[z] = tt[z - dz] + h / 3.0 * (op[z - dz] + op[z] + C)
+    return tt
+
+
 def _interp_temp(t, y, x):
     """atmospheric_interpolation for the temperature (cmlib/mathtools.cc:
     1208-1315, is_temp): top derivative 0, bottom derivative from the FAL-C

@@ -859,8 +859,20 @@ def build_fit(d, kmax=8):
     print("fit: %d pixels, %d iterations" % (len(procs), kmax))
 
 
+def build_tau(d):
+    """The reference's own test atmosphere test/inverted_atmos.f0 (the
+    falc_1000 column on its height grid as the reference wrote it back): its
+    log tau column is compute_tau_referent of its height and referent-opacity
+    columns (tau = 1e-8 at the top).  Stored: height, opacity, log tau."""
+    sys.path.insert(0, REPO)
+    from snapi_amd import cube
+    a = np.asarray(cube.ana_read(os.path.join(REF, "test", "inverted_atmos.f0"))).reshape(12, -1)
+    np.savez_compressed(os.path.join(HERE, "tau_scale.npz"), logtau=a[0], height=a[1], opref=a[6], T=a[2])
+    print("tau_scale: %d depths" % a.shape[1])
+
+
 BUILDERS = ["cfg0", "cfg1_template", "resp", "formal_scalar", "lmstep", "interp", "conv", "ana", "nlte", "chemeq",
-            "nlteresp", "nodes", "fit"]
+            "nlteresp", "nodes", "fit", "tau"]
 
 
 def main():

@@ -300,3 +300,19 @@ def test_gaussian_convolution_matches_reference_golden():
     line = 1.0 - g["c0_in"][:, :1]
     conv = _oracle.convolve_gauss(line, lam, 2.5e-10)
     assert abs(np.trapezoid(conv[:, 0], lam) / np.trapezoid(line[:, 0], lam) - 1.0) < 1e-3
+
+
+def test_optical_depth_scale_matches_reference_file():
+    """The log tau column of the reference's own test/inverted_atmos.f0 is
+    compute_tau_referent of its height and referent-opacity columns
+    (tests/golden/tau_scale.npz); the restatement gives it to rounding.  The
+    mu scaling is exact: tau(mu) - 1e-8 = (tau(1) - 1e-8) / mu."""
+    import _golden
+    g = _golden.load("tau_scale.npz")
+    tt = _oracle.optical_depth_scale(g["height"], g["opref"])
+    assert np.max(np.abs(np.log10(tt) - g["logtau"])) < 1e-14
+    t6 = _oracle.optical_depth_scale(g["height"], g["opref"], math.acos(0.6))
+    assert np.max(np.abs((t6 - 1e-8) / ((tt - 1e-8) / 0.6) - 1.0)[1:]) < 1e-12
+    # the upward ray starts at the bottom
+    tu = _oracle.optical_depth_scale(g["height"], g["opref"], math.pi)
+    assert tu[-1] == 1e-8 and np.all(np.diff(tu) < 0)

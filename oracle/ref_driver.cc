@@ -43,6 +43,8 @@
 //          obs_stokes_responses_to_nodes (atmos/atmos_fit.cc:1205-1352): the
 //          response-path Stokes vector, the node Jacobian and the model's
 //          node responses dq_p(z)/dnode.
+//   time_synth <atmos.cfg> <lambda.txt> <el> <az> <reps>
+//          wall time of obs_stokes per call on the column.
 //   fit    <atmos.cfg> <lambda.txt> <el> <az> <fit.txt> [last]
 //          atmosphere::stokes_lm_fit (atmos/atmos_fit.cc:18-227) from the start
 //          model of fit.txt on its observation, run with 1, 2, ..., kmax
@@ -56,6 +58,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <cmath>
+#include <ctime>
 #include <unistd.h>
 
 // The driver reads the reference's protected state; it is compiled on its own
@@ -1166,6 +1169,27 @@ static int mode_fit(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// time_synth <atmos.cfg> <lambda.txt> <el> <az> <reps>: wall time of
+// atmosphere::obs_stokes (atmos/atmos_obs.cc:16-87) on the column, one call
+// untimed, then reps timed calls.
+static int mode_time_synth(int argc, char **argv, io_class &io) {
+  if (argc < 7) return 2;
+  atmosphere *a = make_atmos(argv[2], io);
+  int nlambda;
+  fp_t *lambda = read_lambda(argv[3], nlambda);
+  fp_t el = atof(argv[4]), az = atof(argv[5]);
+  const int reps = atoi(argv[6]);
+  delete a->obs_stokes(el, az, lambda, nlambda);
+  timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (int r = 0; r < reps; ++r) delete a->obs_stokes(el, az, lambda, nlambda);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  const double sec = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+  printf("time_synth nd %d nlambda %d reps %d seconds_per_call %.6f lambda_points_per_s %.6g\n",
+         a->x3h - a->x3l + 1, nlambda, reps, sec / reps, reps * (double)nlambda / sec);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr, "usage: ref_driver synth|resp|lmstep|formal_scalar ...\n");
@@ -1185,6 +1209,7 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "nlteresp")) return mode_nlteresp(argc, argv, io);
   if (!strcmp(argv[1], "nodes")) return mode_nodes(argc, argv, io);
   if (!strcmp(argv[1], "fit")) return mode_fit(argc, argv, io);
+  if (!strcmp(argv[1], "time_synth")) return mode_time_synth(argc, argv, io);
   fprintf(stderr, "unknown mode %s\n", argv[1]);
   return 2;
 }

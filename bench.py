@@ -61,6 +61,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--lm-side", type=int, default=256, help="inversion leg: side of the pixel square")
     ap.add_argument("--lm-fd-pixels", type=int, default=8192, help="pixels of the FP64 difference-Jacobian run")
+    ap.add_argument("--tnode-side", type=int, default=48, help="14-node inversion: side of the pixel square")
     ap.add_argument("--no-lm", action="store_true")
     ap.add_argument("--ali-side", type=int, default=64, help="ALI leg: side of the column square")
     ap.add_argument("--no-ali", action="store_true")
@@ -175,7 +176,7 @@ def main():
 
     lm = None
     if not args.no_lm:
-        lm = lm_leg(torch, dev, tmpl, lam_d, stream, args.lm_side, args.lm_fd_pixels)
+        lm = lm_leg(torch, dev, tmpl, lam_d, stream, args.lm_side, args.lm_fd_pixels, args.tnode_side)
 
     alires = None
     if not args.no_ali:
@@ -509,7 +510,7 @@ def ca8542_leg(torch, dev, stream, p, pops, with_cpu):
     return res
 
 
-def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):
+def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd, tnode_side=48):
     from snapi_amd import abi, api, invert, workloads
     table, ray = tmpl["table"], tmpl["ray"]
     nlam = lam_d.shape[0]
@@ -567,7 +568,7 @@ def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd):
             % (label, total, sec, total / sec, it.mean(), np.median(red)))
         return out
 
-    tnodes = tnode_inversion(torch, dev, tmpl, lam_d, stream, ws)
+    tnodes = tnode_inversion(torch, dev, tmpl, lam_d, stream, ws, side=tnode_side)
     mixed = run(npix, abi.SNAPI_MIXED, 2, "mixed tangent")
     fd = run(min(nfd, npix), abi.SNAPI_FP64, 1, "fp64 differences")
     adj = run(min(nfd, npix), abi.SNAPI_MIXED, 0, "mixed adjoint")

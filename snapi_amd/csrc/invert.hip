@@ -738,8 +738,9 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   ss.chi2t = dbl + 2 * nslot;
   ss.track = dbl + 3 * nslot;
   if (th) {
-    if (jacobian != SNAPI_JAC_ADJOINT || nf != SNAPI_NF_BASE + 2 * th->map->nline) {
-      rc = jacobian != SNAPI_JAC_ADJOINT ? SNAPI_ERR_UNSUPPORTED : SNAPI_ERR_ARG;
+    const bool jac_ok = jacobian == SNAPI_JAC_ADJOINT || jacobian == SNAPI_JAC_TANGENT;
+    if (!jac_ok || nf != SNAPI_NF_BASE + 2 * th->map->nline) {
+      rc = !jac_ok ? SNAPI_ERR_UNSUPPORTED : SNAPI_ERR_ARG;
       goto done;
     }
     rc = chem_upload(th->chem, th->map, cdev, st);
@@ -869,7 +870,10 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
         hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, rsd, rstate);
         hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * 2 * ns, dsd,
                            rdst);
-        rc = launch_resp<R>(kl, a_resp, ra, nslot, rstate, rdst, dlam, S, J, scr, chunk, st);
+        if (jacobian == SNAPI_JAC_TANGENT)
+          rc = launch_tangent<R>(kl, a_resp, ra, nslot, rstate, rdst, dlam, S, J, st);
+        else
+          rc = launch_resp<R>(kl, a_resp, ra, nslot, rstate, rdst, dlam, S, J, scr, chunk, st);
         if (rc) goto done;
         if (th->broadening > 0.0) {
           rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, S, ss.needj,
@@ -884,7 +888,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
         rc = launch_resp<R>(kl, a, ra, nslot, sstate, dzero, dlam, S, J, scr, chunk, st);
         if (rc) goto done;
       } else if (jacobian == SNAPI_JAC_TANGENT) {
-        rc = launch_tangent<R>(kl, a, ra, nslot, sstate, dlam, S, J, st);
+        rc = launch_tangent<R>(kl, a, ra, nslot, sstate, nullptr, dlam, S, J, st);
         if (rc) goto done;
       } else {
         hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, fdn, hv,
@@ -1009,7 +1013,7 @@ static int node_jacobian_impl(int precision, int jacobian, const snapi_line_tabl
   }
   ra.nmap = nmap;
   if (jacobian == SNAPI_JAC_TANGENT) {
-    rc = launch_tangent<R>(kl, a, ra, npix, dstate, dlam, S, J, 0);
+    rc = launch_tangent<R>(kl, a, ra, npix, dstate, nullptr, dlam, S, J, 0);
     if (rc) goto done;
   } else if (jacobian == SNAPI_JAC_ADJOINT) {
     const size_t per_col = resp_scratch_per_col(nl, nd, es);
@@ -1063,7 +1067,8 @@ done:
 // responses: T and Nt rows for the temperature nodes (2 x delta_T/2 rebuilds
 // each), the interpolation's differences for the others.
 template <typename R>
-static int node_jacobian_eos_impl(int precision, const snapi_line_table *lines, const snapi_ray *ray, NodeArgs na,
+static int node_jacobian_eos_impl(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
+                                  NodeArgs na,
                                   const snapi_chem_table *chem, const snapi_chem_table *chem_he,
                                   const snapi_state_map *map, int npix, int nd, const void *state_h,
                                   const double *nodes_h, int nlambda, const double *lam_h, void *stokes_h, void *J_h) {
@@ -1152,16 +1157,20 @@ static int node_jacobian_eos_impl(int precision, const snapi_line_table *lines, 
   ra.nmap = nmap;
   ra.nmap2 = nmap2;
   {
-    const size_t per_col = resp_scratch_per_col(nl, nd, es);
-    size_t c = ((size_t)16 << 30) / (per_col ? per_col : 1);
-    if (c < 1) c = 1;
-    if (c > (size_t)npix) c = npix;
-    HIPCHK(hipMalloc((void **)&scr, per_col * c));
     // the response path's H- at the window's middle wavelength, as
     // obs_stokes_responses_to_nodes evaluates it (mol.cc:414-469)
     SynthArgs a_resp = a;
     a_resp.hminus_mid = 1;
-    rc = launch_resp<R>(kl, a_resp, ra, npix, rstate, rdst, dlam, S, J, scr, (int)c, 0);
+    if (jacobian == SNAPI_JAC_TANGENT) {
+      rc = launch_tangent<R>(kl, a_resp, ra, npix, rstate, rdst, dlam, S, J, 0);
+    } else {
+      const size_t per_col = resp_scratch_per_col(nl, nd, es);
+      size_t c = ((size_t)16 << 30) / (per_col ? per_col : 1);
+      if (c < 1) c = 1;
+      if (c > (size_t)npix) c = npix;
+      HIPCHK(hipMalloc((void **)&scr, per_col * c));
+      rc = launch_resp<R>(kl, a_resp, ra, npix, rstate, rdst, dlam, S, J, scr, (int)c, 0);
+    }
     if (rc) goto done;
   }
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
@@ -1281,13 +1290,13 @@ extern "C" int snapi_node_jacobian_eos(int precision, int jacobian, const snapi_
   int ntn = 0;
   for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
   if (ntn < 1) return SNAPI_ERR_ARG;  // build_from_nodes needs a temperature node
-  if (jacobian != SNAPI_JAC_ADJOINT) return SNAPI_ERR_UNSUPPORTED;
+  if (jacobian != SNAPI_JAC_ADJOINT && jacobian != SNAPI_JAC_TANGENT) return SNAPI_ERR_UNSUPPORTED;
   if (precision == SNAPI_FP64)
-    return node_jacobian_eos_impl<double>(precision, lines, ray, na, chem, chem_he, map, npix, nd, state, nodes,
-                                          nlambda, lam, stokes, J);
+    return node_jacobian_eos_impl<double>(precision, jacobian, lines, ray, na, chem, chem_he, map, npix, nd, state,
+                                          nodes, nlambda, lam, stokes, J);
   if (precision == SNAPI_MIXED)
-    return node_jacobian_eos_impl<float>(precision, lines, ray, na, chem, chem_he, map, npix, nd, state, nodes,
-                                         nlambda, lam, stokes, J);
+    return node_jacobian_eos_impl<float>(precision, jacobian, lines, ray, na, chem, chem_he, map, npix, nd, state,
+                                         nodes, nlambda, lam, stokes, J);
   return SNAPI_ERR_ARG;
 }
 

@@ -40,11 +40,12 @@ int launch_resp(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, con
 size_t resp_scratch_per_col(int nlambda, int nd, size_t esz);
 
 // Node Jacobian by the forward-mode sweep (tangent.hip): uses ra.nnode,
-// pidx (2..6), nscale, ws, nmap, mask and active; writes stokes
-// [ncol][nlambda][4] and J[ncol][nlambda*4][nnode].
+// pidx (0..6), pidx2 and nmap2 (temperature nodes), nscale, ws, nmap, mask
+// and active; writes stokes [ncol][nlambda][4] and J[ncol][nlambda*4][nnode].
+// Temperature and density rows read dd_ (dstate [ncol][2][nfield][nd]).
 constexpr int kTanMaxNodes = 8;  // node directions carried per workgroup
 template <typename R>
-int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, const void *ds_, const double *dl,
-                   void *dstk_, void *dJ_, hipStream_t stream);
+int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, const void *ds_, const void *dd_,
+                   const double *dl, void *dstk_, void *dJ_, hipStream_t stream);
 
 }  // namespace snapi

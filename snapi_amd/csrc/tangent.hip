@@ -32,17 +32,24 @@
 namespace snapi {
 
 constexpr int kTanThreads = 64;
-enum { Q_VT = 0, Q_VL = 1, Q_B = 2, Q_TH = 3, Q_PH = 4, NQ = 5 };
+// parameter rows of the local derivatives: the kinematic and magnetic ones,
+// then (kernels with thermodynamic directions) temperature and density
+enum { Q_VT = 0, Q_VL = 1, Q_B = 2, Q_TH = 3, Q_PH = 4, Q_T = 5, Q_N = 6, NQ = 5, NQ_TH = 7 };
+
+// parameter row of a per-depth response index (0 T, 1 Nt, 2 vt, 3 vLOS,
+// 4 |B|, 5 inclination, 6 azimuth)
+__host__ __device__ inline int qrow(int pidx) { return pidx >= 2 ? pidx - 2 : (pidx == 0 ? Q_T : Q_N); }
 
 // LDS tile of the derivative factors of one column, after the ColumnTile.
 template <typename R> struct DerivTile {
   R *sct, *dqt, *dut, *mhs;  // per depth: sin(th)cos(th), d qf/d th, d uf/d th, -sin(th)/2
   R *gv;                     // per line and depth: d ln(Doppler width) / d vt
   R *nm;                     // [node of the pass][z]: node map d q(z) / d node
+  R *nm2;                    // [node of the pass][z]: the node's second row (Nt of a temperature node)
   static __host__ __device__ size_t bytes(int nd, int nline, int nn) {
-    return sizeof(R) * (size_t)nd * (4 + nline + nn);
+    return sizeof(R) * (size_t)nd * (4 + nline + 2 * nn);
   }
-  static __device__ DerivTile carve(R *p, int nd, int nline) {
+  static __device__ DerivTile carve(R *p, int nd, int nline, int nn) {
     DerivTile t;
     t.sct = p;
     t.dqt = p + nd;
@@ -50,9 +57,152 @@ template <typename R> struct DerivTile {
     t.mhs = p + 3 * nd;
     t.gv = p + 4 * nd;
     t.nm = p + (size_t)(4 + nline) * nd;
+    t.nm2 = t.nm + (size_t)nn * nd;
     return t;
   }
 };
+
+// LDS tile of the temperature and density derivatives of one column (the
+// response path's: the per-depth derivative fields of dstate), after the
+// DerivTile.
+template <typename R> struct ThermoTile {
+  // per line and depth: d ln(Doppler width)/dT; the damping's T derivative
+  // beyond its Doppler-width part; da/dNt; the line's normalised opacity and
+  // emissivity factors differentiated through the level populations (d/dT,
+  // d/dNt)
+  R *gT, *adT, *aN, *pT, *eT, *pN, *eN;
+  // per depth: d/dT, d/dNt of the background eta_I and eps_I (Thomson and H-
+  // at the middle wavelength; the bound-free part the reference adds as 0),
+  // normalised; d ln(kappa_ref)/dT, d ln(kappa_ref)/dNt
+  R *bKT, *bET, *bKN, *bEN, *rT, *rN;
+  static __host__ __device__ size_t bytes(int nd, int nline) { return sizeof(R) * (size_t)nd * (6 + 7 * nline); }
+  static __device__ ThermoTile carve(R *p, int nd, int nline) {
+    ThermoTile t;
+    R **f[6] = {&t.bKT, &t.bET, &t.bKN, &t.bEN, &t.rT, &t.rN};
+    for (int i = 0; i < 6; ++i) {
+      *f[i] = p;
+      p += nd;
+    }
+    R **g[7] = {&t.gT, &t.adT, &t.aN, &t.pT, &t.eT, &t.pN, &t.eN};
+    for (int i = 0; i < 7; ++i) {
+      *g[i] = p;
+      p += (size_t)nd * nline;
+    }
+    return t;
+  }
+};
+
+// The wavelength-independent parts of the T and Nt derivatives of the
+// opacity of op_em_vector_plus_pert (atom_rad_synthesis.cc:771-1026, mol.cc:
+// 414-469, Thomson and the referent-opacity normalisation of
+// atmos_rad.cc:187-201), in double from the state and dstate
+// [2][nfield][nd] (d/dT, d/dNt of NE, NHI, OPREF and the level populations).
+// H- is taken at the middle wavelength, as that path does.
+template <typename R, typename S>
+__device__ void stage_thermo(const KLines &L, const SynthArgs &a, const S *__restrict__ st, const S *__restrict__ dst,
+                             ThermoTile<R> &t) {
+  const int nd = a.nd, nf = a.nfield;
+  const double lm = a.lam_mid;
+  auto planck = [](double l, double T) {
+    return 2.0 * 6.62606957 * 8.9875517E-7 / l / l / l / l / l / (exp(1.4387769 / l / T) - 1.0);
+  };
+  auto planck_dT = [&](double l, double T) {
+    const double x = 1.4387769 / l / T;
+    const double tmp = (x > 5.0) ? 1.0 : exp(x) / (exp(x) - 1.0);
+    return planck(l, T) * tmp * 1.4387769 / l / T / T;
+  };
+  for (int z = threadIdx.x; z < nd; z += blockDim.x) {
+    const double T = (double)st[SNAPI_F_T * nd + z], Ne = (double)st[SNAPI_F_NE * nd + z];
+    const double nhi = (double)st[SNAPI_F_NHI * nd + z];
+    double dNe[2], dNH[2], dr[2];
+    for (int p = 0; p < 2; ++p) {
+      dNe[p] = (double)dst[((size_t)p * nf + SNAPI_F_NE) * nd + z];
+      dNH[p] = (double)dst[((size_t)p * nf + SNAPI_F_NHI) * nd + z];
+      dr[p] = (double)dst[((size_t)p * nf + SNAPI_F_OPREF) * nd + z];
+    }
+    const double r = a.tau_grid ? (double)st[SNAPI_F_OPREF * nd + z] : 1.0;
+    const double iref = 1.0 / r;
+    double dK[2], dE[2];
+    const double opth = Ne * kThomson;
+    for (int p = 0; p < 2; ++p) {
+      const double o = dNe[p] * kThomson;
+      dK[p] = o;
+      dE[p] = planck(lm, T) * o + (p == 0 ? opth * planck_dT(lm, T) : 0.0);
+    }
+    if (a.hminus) {
+      const double ac[7] = {1.99654, -1.18267E-5, 2.64243E-6, -4.40524E-10, 3.23992E-14, -1.39568E-18, 2.78701E-23};
+      double alpha = 0.0, pw = 1.0;
+      for (int i = 0; i < 7; ++i) {
+        alpha += ac[i] * pw;
+        pw *= lm * 1E8;
+      }
+      const double TP = 5040. / T;
+      const double stim = exp(-kH * kC / lm / kK / T);
+      const double obf = 4.158E-28 * alpha * Ne * kK * T * pow(TP, 2.5) * pow(10.0, 0.754 * TP) * nhi * (1.0 - stim);
+      const double l = log10(lm * 1E8);
+      const double f0 = -2.2763 - 1.6850 * l + 0.76661 * l * l - 0.053346 * l * l * l;
+      const double f1 = 15.2827 - 9.2846 * l + 1.99381 * l * l - 0.142631 * l * l * l;
+      const double f2 = -197.789 + 190.266 * l - 67.9775 * l * l + 10.6913 * l * l * l - 0.625151 * l * l * l * l;
+      const double ltp = log10(TP);
+      const double ef = pow(10.0, f0 + f1 * ltp + f2 * ltp * ltp);
+      const double off = 1E-26 * Ne * kK * T * ef * nhi;
+      const double o = obf + off;
+      const double B = planck(lm, T);
+      double d[2];
+      d[0] = obf * (1.0 / T + 2.5 * 1.0 / TP * (-5040.0 / T / T)) + obf * dNe[0] / Ne + obf * dNH[0] / nhi +
+             obf * (-2.302585 * 0.754 * 5040.0 / T / T) + obf * (-stim * kH * kC / lm / kK / T / T) / (1.0 - stim);
+      d[1] = obf * dNe[1] / Ne + obf * dNH[1] / nhi;
+      const double dltp = 1.0 / 2.302585 / TP * (-TP / T);
+      const double def = ef * 2.302585 * (f1 * dltp + 2.0 * f2 * ltp * dltp);
+      const double cf = 1E-26 * kK;
+      d[0] += cf * Ne * ef * nhi + cf * T * Ne * nhi * def + cf * T * ef * (Ne * dNH[0] + nhi * dNe[0]);
+      d[1] += cf * T * ef * (Ne * dNH[1] + nhi * dNe[1]);
+      for (int p = 0; p < 2; ++p) {
+        dK[p] += d[p];
+        dE[p] += B * d[p] + (p == 0 ? o * planck_dT(lm, T) : 0.0);
+      }
+    }
+    t.bKT[z] = (R)(dK[0] * iref);
+    t.bET[z] = (R)(dE[0] * iref);
+    t.bKN[z] = (R)(dK[1] * iref);
+    t.bEN[z] = (R)(dE[1] * iref);
+    t.rT[z] = (R)(a.tau_grid ? dr[0] / r : 0.0);
+    t.rN[z] = (R)(a.tau_grid ? dr[1] / r : 0.0);
+  }
+  for (int i = threadIdx.x; i < nd * L.nline; i += blockDim.x) {
+    const int il = i / nd, z = i - il * nd;
+    const KLine &ln = L.l[il];
+    const double T = (double)st[SNAPI_F_T * nd + z], vt = (double)st[SNAPI_F_VT * nd + z];
+    const double nhi = (double)st[SNAPI_F_NHI * nd + z], Nt = (double)st[SNAPI_F_NT * nd + z];
+    const double iref = a.tau_grid ? 1.0 / (double)st[SNAPI_F_OPREF * nd + z] : 1.0;
+    const double dNH0 = (double)dst[((size_t)0 * nf + SNAPI_F_NHI) * nd + z];
+    const double dvd2 = ln.twok_m * T + vt * vt;
+    const double dld = sqrt(dvd2) * ln.lam0 / kC;
+    const double vm = sqrt(ln.vmfac * T);
+    const double vpow = pow(vm / 1E6, -ln.alpha);
+    const double gc = nhi * vm * vpow * ln.sigma;
+    const double vmd = 0.5 / vm * ln.vmfac;
+    double dgc_T = nhi * ln.sigma * (vmd * vpow - vm * ln.alpha * pow(vm / 1E6, -ln.alpha - 1.0) * vmd / 1E6);
+    dgc_T += dNH0 * vm * vpow * ln.sigma;
+    t.gT[i] = (R)(0.5 * ln.twok_m / dvd2);
+    t.adT[i] = (R)(ln.sf * dgc_T / dld);
+    // the reference's density derivative of the damping (a_der for Nt)
+    t.aN[i] = (R)((gc * 1E-3 / dld * ln.sf) / (1E-3 * Nt));
+    const int fl = SNAPI_NF_BASE + 2 * il, fu = fl + 1;
+    for (int p = 0; p < 2; ++p) {
+      const double dlo = (double)dst[((size_t)p * nf + fl) * nd + z], dup = (double)dst[((size_t)p * nf + fu) * nd + z];
+      const R po = (R)((dlo * ln.opf_l - dup * ln.opf_u) / dld * iref);
+      const R pe = (R)(dup * ln.emf / dld * iref);
+      if (p == 0) {
+        t.pT[i] = po;
+        t.eT[i] = pe;
+      } else {
+        t.pN[i] = po;
+        t.eN[i] = pe;
+      }
+    }
+  }
+}
 
 template <typename R, typename S>
 __device__ void stage_deriv(const KLines &L, const SynthArgs &a, const S *__restrict__ st, DerivTile<R> &t) {
@@ -79,10 +229,10 @@ __device__ void stage_deriv(const KLines &L, const SynthArgs &a, const S *__rest
 // Opacity (K, E) of depth z and its derivatives G[q] = d(K, E)/dq along the
 // parameters in qmask, entries 0..6 = K, 7..10 = E.  FULL = false computes
 // only the eta_I entry G[q][0] (the look-ahead of the optical-depth step).
-template <typename R, bool FULL>
+template <typename R, bool FULL, int NQR>
 __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, const ColumnTile<R> &t,
-                                         const DerivTile<R> &d, const LamConst<R> &c, int z, unsigned qmask,
-                                         R K[7], R E[4], R G[NQ][11]) {
+                                         const DerivTile<R> &d, const ThermoTile<R> &th, const LamConst<R> &c, int z,
+                                         unsigned qmask, R K[7], R E[4], R G[NQR][11]) {
   R KI = t.bgop[z], EI = t.bgem[z];
   if (a.hminus) {
     const R tinv = t.tinv[z];
@@ -94,11 +244,13 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
     EI += o * B;
   }
 #pragma unroll
-  for (int q = 0; q < NQ; ++q)
+  for (int q = 0; q < NQR; ++q)
 #pragma unroll
     for (int i = 0; i < 11; ++i) G[q][i] = R(0);
   const bool wvt = qmask & (1u << Q_VT), wvl = qmask & (1u << Q_VL), wb = qmask & (1u << Q_B);
   const bool wth = qmask & (1u << Q_TH), wph = qmask & (1u << Q_PH);
+  constexpr bool kTh = NQR > Q_N;
+  const bool wT = kTh && (qmask & (1u << Q_T)), wN = kTh && (qmask & (1u << Q_N));
   R KQ = 0, KU = 0, KV = 0, RQ = 0, RU = 0, RV = 0, EQ = 0, EU = 0, EV = 0;
   const int nd = a.nd;
   const R s2h = t.s2h[z], c2q = t.c2q[z], qf = t.qf[z], uf = t.uf[z], vf = t.vf[z];
@@ -125,6 +277,19 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
         G[Q_VL][0] += opl * gl;
         if (FULL) G[Q_VL][7] += eml * gl;
       }
+      if constexpr (kTh) {
+        if (wT) {
+          const R gT = th.gT[i], adT = th.adT[i];
+          const R gtt = -gT * (H + x0 * Hx - av * Fx) - adT * Fx;
+          G[Q_T][0] += opl * gtt + th.pT[i] * H;
+          if (FULL) G[Q_T][7] += eml * gtt + th.eT[i] * H;
+        }
+        if (wN) {
+          const R gnn = -th.aN[i] * Fx;
+          G[Q_N][0] += opl * gnn + th.pN[i] * H;
+          if (FULL) G[Q_N][7] += eml * gnn + th.eN[i] * H;
+        }
+      }
       continue;
     }
     const R lbd = t.lbd[i];
@@ -147,24 +312,25 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
       }
     }
     // pattern geometry (atom_rad_synthesis.cc:486-497) of a profile triplet
-    auto add = [&](const R h[3], const R f[3], R *Kq, bool full) {
+    auto addf = [&](const R h[3], const R f[3], R *Kq, bool full, R fo, R fe) {
       const R hbr = h[2] + h[1], fbr = f[2] + f[1];
       const R eI = s2h * h[0] + c2q * hbr;
-      Kq[0] += eI * opl;
+      Kq[0] += eI * fo;
       if (!full) return;
       const R hl = h[0] - R(0.5) * hbr, fl = f[0] - R(0.5) * fbr;
       const R eQ = qf * hl, eU = uf * hl, eV = vf * (h[2] - h[1]);
-      Kq[1] += eQ * opl;
-      Kq[2] += eU * opl;
-      Kq[3] += eV * opl;
-      Kq[4] += qf * fl * opl;
-      Kq[5] += uf * fl * opl;
-      Kq[6] += vf * (f[2] - f[1]) * opl;
-      Kq[7] += eI * eml;
-      Kq[8] += eQ * eml;
-      Kq[9] += eU * eml;
-      Kq[10] += eV * eml;
+      Kq[1] += eQ * fo;
+      Kq[2] += eU * fo;
+      Kq[3] += eV * fo;
+      Kq[4] += qf * fl * fo;
+      Kq[5] += uf * fl * fo;
+      Kq[6] += vf * (f[2] - f[1]) * fo;
+      Kq[7] += eI * fe;
+      Kq[8] += eQ * fe;
+      Kq[9] += eU * fe;
+      Kq[10] += eV * fe;
     };
+    auto add = [&](const R h[3], const R f[3], R *Kq, bool full) { addf(h, f, Kq, full, opl, eml); };
     {
       R kk[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       add(HS, FS, kk, true);
@@ -199,6 +365,30 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
       }
       add(h, f, G[Q_B], FULL);
     }
+    if constexpr (kTh) {
+      if (wT) {
+        const R gT = th.gT[i], adT = th.adT[i];
+        R h[3], f[3];
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+          h[g] = -gT * (HS[g] + x0 * HX[g] + lbd * HXS[g] - av * FX[g]) - adT * FX[g];
+          f[g] = -gT * (FS[g] + x0 * FX[g] + lbd * FXS[g] + av * HX[g]) + adT * HX[g];
+        }
+        add(h, f, G[Q_T], FULL);
+        addf(HS, FS, G[Q_T], FULL, th.pT[i], th.eT[i]);
+      }
+      if (wN) {
+        const R aN = th.aN[i];
+        R h[3], f[3];
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+          h[g] = -aN * FX[g];
+          f[g] = aN * HX[g];
+        }
+        add(h, f, G[Q_N], FULL);
+        addf(HS, FS, G[Q_N], FULL, th.pN[i], th.eN[i]);
+      }
+    }
     const R hl = HS[0] - R(0.5) * (HS[2] + HS[1]), fl = FS[0] - R(0.5) * (FS[2] + FS[1]);
     if (wth) {
       // d/d inclination; rho_V as the reference writes it (cos, :999)
@@ -231,16 +421,44 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
   }
   K[0] = KI; K[1] = KQ; K[2] = KU; K[3] = KV; K[4] = RQ; K[5] = RU; K[6] = RV;
   E[0] = EI; E[1] = EQ; E[2] = EU; E[3] = EV;
+  if constexpr (kTh) {
+    // the background's T and Nt terms, then the referent-opacity
+    // normalisation: d(K / r) = dK / r - (K / r) d ln r
+    if (wT) {
+      G[Q_T][0] += th.bKT[z];
+      if (FULL) G[Q_T][7] += th.bET[z];
+      const R rt = th.rT[z];
+      G[Q_T][0] -= rt * KI;
+      if (FULL) {
+#pragma unroll
+        for (int k = 1; k < 7; ++k) G[Q_T][k] -= rt * K[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) G[Q_T][7 + k] -= rt * E[k];
+      }
+    }
+    if (wN) {
+      G[Q_N][0] += th.bKN[z];
+      if (FULL) G[Q_N][7] += th.bEN[z];
+      const R rn = th.rN[z];
+      G[Q_N][0] -= rn * KI;
+      if (FULL) {
+#pragma unroll
+        for (int k = 1; k < 7; ++k) G[Q_N][k] -= rn * K[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) G[Q_N][7 + k] -= rn * E[k];
+      }
+    }
+  }
 }
 
 // u_q = dS_q - dK~_q I of one depth for every parameter in qmask: the
 // derivative of the reduced source terms (K~ = K/eta_I - 1, S = E/eta_I).
-template <typename R>
-__device__ __forceinline__ void reduced_u(const R G[NQ][11], R eta, const Kred<R> &k, const R S[4], const R I[4],
-                                          unsigned qmask, R u[NQ][4], R ds0[NQ]) {
+template <typename R, int NQR>
+__device__ __forceinline__ void reduced_u(const R G[NQR][11], R eta, const Kred<R> &k, const R S[4], const R I[4],
+                                          unsigned qmask, R u[NQR][4], R ds0[NQR]) {
   const R ie = rrcp<R>(eta);
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
+  for (int q = 0; q < NQR; ++q) {
     if (!(qmask & (1u << q))) {
       u[q][0] = u[q][1] = u[q][2] = u[q][3] = R(0);
       ds0[q] = R(0);
@@ -266,10 +484,11 @@ __device__ __forceinline__ void reduced_u(const R G[NQ][11], R eta, const Kred<R
 // one-hot row oh[q] (uniform over the workgroup): a select chain on q would
 // be folded into a dynamic index, which places the private array in scratch
 // memory.
-template <typename R, int N> __device__ __forceinline__ R pick(const R (&A)[NQ][N], const R oh[NQ], int i) {
+template <typename R, int NQR, int N>
+__device__ __forceinline__ R pick(const R (&A)[NQR][N], const R oh[NQR], int i) {
   R r = oh[0] * A[0][i];
 #pragma unroll
-  for (int k = 1; k < NQ; ++k) r += oh[k] * A[k][i];
+  for (int k = 1; k < NQR; ++k) r += oh[k] * A[k][i];
   return r;
 }
 
@@ -305,44 +524,76 @@ __device__ __forceinline__ R fb_deriv_p(R om, R o0, R op, R h1, R h2, R &gm, R &
   return R(0);
 }
 
-// One workgroup per (column, pass of up to kTanMaxNodes nodes), work-items
-// over wavelengths.  Writes the emergent Stokes vector (pass 0) and
-// J[col][l*4+s][n] = dI_s/dnode_n * nscale[n] * ws[s] * mask[col][l].
-template <typename R>
+// One workgroup per column of one pass of up to kTanMaxNodes nodes,
+// work-items over wavelengths.  Writes the emergent Stokes vector (pass 0)
+// and J[col][l*4+s][n] = dI_s/dnode_n * nscale[n] * ws[s] * mask[col][l].
+// NQR = NQ_TH: the pass carries temperature or density rows, whose local
+// derivatives read dstate; each node's direction is
+// nmap[n] d/dq_pidx[n] + nmap2[n] d/dq_pidx2[n].
+template <typename R, int NQR>
 __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArgs &args, const RespArgs &ra,
-                                             const R *__restrict__ state, const double *__restrict__ lam,
-                                             R *__restrict__ stokes, R *__restrict__ J, int *__restrict__ status) {
+                                             const R *__restrict__ state, const R *__restrict__ dstate,
+                                             const double *__restrict__ lam, R *__restrict__ stokes,
+                                             R *__restrict__ J, int *__restrict__ status, int pass) {
   extern __shared__ unsigned char smem_raw[];
   R *sm = reinterpret_cast<R *>(smem_raw);
   const int nd = args.nd, nl = args.nlambda, nf = args.nfield;
   const int col = ra.col0 + blockIdx.x;
   if (ra.active && !ra.active[col]) return;  // uniform over the workgroup
-  const int n0 = blockIdx.y * kTanMaxNodes;
+  const int n0 = pass * kTanMaxNodes;
+  constexpr bool kTh = NQR > Q_N;
   const int ng = min(ra.nnode - n0, kTanMaxNodes);
   const R *st = state + (size_t)col * nf * nd;
   ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
-  DerivTile<R> dt = DerivTile<R>::carve(sm + ColumnTile<R>::bytes(nd, lines.nline) / sizeof(R), nd, lines.nline);
+  DerivTile<R> dt =
+      DerivTile<R>::carve(sm + ColumnTile<R>::bytes(nd, lines.nline) / sizeof(R), nd, lines.nline, kTanMaxNodes);
+  ThermoTile<R> tht = ThermoTile<R>::carve(
+      sm + (ColumnTile<R>::bytes(nd, lines.nline) + DerivTile<R>::bytes(nd, lines.nline, kTanMaxNodes)) / sizeof(R),
+      nd, lines.nline);
   // the synthesis kernel's grouped wavelength order (float path)
   int *order = nullptr;
   if (sizeof(R) == 4 && nl <= kOrderMax) {
     order = reinterpret_cast<int *>(smem_raw + ColumnTile<R>::bytes(nd, lines.nline) +
-                                    DerivTile<R>::bytes(nd, lines.nline, ng));
+                                    DerivTile<R>::bytes(nd, lines.nline, kTanMaxNodes) +
+                                    (kTh ? ThermoTile<R>::bytes(nd, lines.nline) : 0));
     lambda_order(lines, lam, nl, reinterpret_cast<float *>(order + nl), order + 2 * nl, order);
   }
   stage_column<R, R>(lines, args, st, tile);
   stage_deriv<R, R>(lines, args, st, dt);
+  if constexpr (kTh) stage_thermo<R, R>(lines, args, st, dstate + (size_t)col * 2 * nf * nd, tht);
   const R *nmg = static_cast<const R *>(ra.nmap) + ((size_t)col * ra.nnode + n0) * nd;
   for (int i = threadIdx.x; i < ng * nd; i += blockDim.x) dt.nm[i] = nmg[i];
+  if constexpr (kTh) {
+    const R *nmg2 = ra.nmap2 ? static_cast<const R *>(ra.nmap2) + ((size_t)col * ra.nnode + n0) * nd : nullptr;
+    for (int i = threadIdx.x; i < ng * nd; i += blockDim.x) {
+      const int j = i / nd;
+      dt.nm2[i] = (nmg2 && ra.pidx2[n0 + j] >= 0) ? nmg2[i] : R(0);
+    }
+  }
   __syncthreads();
-  R oh[kTanMaxNodes][NQ];  // one-hot parameter row of each node of the pass
+  // one-hot parameter rows of each node of the pass (the second: the
+  // density row of a temperature node)
+  R oh[kTanMaxNodes][NQR], oh2[kTh ? kTanMaxNodes : 1][NQR];
   unsigned qmask = 0;
 #pragma unroll
   for (int j = 0; j < kTanMaxNodes; ++j) {
-    const int q = j < ng ? ra.pidx[n0 + j] - 2 : -1;
+    const int q = j < ng ? qrow(ra.pidx[n0 + j]) : -1;
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) oh[j][k] = (q == k) ? R(1) : R(0);
+    for (int k = 0; k < NQR; ++k) oh[j][k] = (q == k) ? R(1) : R(0);
     if (q >= 0) qmask |= 1u << q;
+    if constexpr (kTh) {
+      const int q2 = (j < ng && ra.nmap2 && ra.pidx2[n0 + j] >= 0) ? qrow(ra.pidx2[n0 + j]) : -1;
+#pragma unroll
+      for (int k = 0; k < NQR; ++k) oh2[j][k] = (q2 == k) ? R(1) : R(0);
+      if (q2 >= 0) qmask |= 1u << q2;
+    }
   }
+  // node j's derivative of entry i of the rows A at depth z
+  auto ndir = [&](int j, int z, const auto &A, int i) {
+    R r = dt.nm[j * nd + z] * pick<R>(A, oh[j], i);
+    if constexpr (kTh) r += dt.nm2[j * nd + z] * pick<R>(A, oh2[j], i);
+    return r;
+  };
   const bool up = args.ct < 0.0;
   const R mct = (R)(-args.ct);
   auto zof = [&](int k) { return up ? nd - 1 - k : k; };
@@ -351,32 +602,38 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
     const int l = order ? order[i] : i;
     LamConst<R> c;
     lambda_consts<R>(lines, args, lam[l], c);
-    R G[NQ][11], u[NQ][4], ds0[NQ];
+    R G[NQR][11], u[NQR][4], ds0[NQR];
     R K0[7], E0[4], K1[7], E1[4];
     // bottom point: full derivatives, boundary I = (S_I, 0, 0, 0)
     const int z0 = zof(0), z1 = zof(1);
-    deriv_at<R, true>(lines, args, tile, dt, c, z0, qmask, K0, E0, G);
+    deriv_at<R, true, NQR>(lines, args, tile, dt, tht, c, z0, qmask, K0, E0, G);
     Kred<R> kp, kc;
     R Sp[4], Sc[4];
     reduce_ks<R>(K0, E0, kp, Sp);
     R Ip[4] = {up ? E0[0] / K0[0] : R(0), R(0), R(0), R(0)};
-    reduced_u<R>(G, K0[0], kp, Sp, Ip, qmask, u, ds0);
+    reduced_u<R, NQR>(G, K0[0], kp, Sp, Ip, qmask, u, ds0);
     R dI[kTanMaxNodes][4], v[kTanMaxNodes][4], dem[kTanMaxNodes], de0[kTanMaxNodes], dDm[kTanMaxNodes];
+    R ds0r[NQR][1];
+#pragma unroll
+    for (int k = 0; k < NQR; ++k) ds0r[k][0] = ds0[k];
 #pragma unroll
     for (int j = 0; j < kTanMaxNodes; ++j) {
-      const R m = j < ng ? dt.nm[j * nd + z0] : R(0);
-      dem[j] = m * pick<R, 11>(G, oh[j], 0);
+      const bool on = j < ng;
+      dem[j] = on ? ndir(j, z0, G, 0) : R(0);
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        v[j][s] = m * pick<R, 4>(u, oh[j], s);
+        v[j][s] = on ? ndir(j, z0, u, s) : R(0);
         dI[j][s] = R(0);
       }
+      // the bottom boundary I = S_I: its derivative is that of S_I
       R d0 = R(0);
-#pragma unroll
-      for (int k = 0; k < NQ; ++k) d0 += oh[j][k] * ds0[k];
-      dI[j][0] = up ? m * d0 : R(0);
+      if (on) {
+        d0 = dt.nm[j * nd + z0] * pick<R>(ds0r, oh[j], 0);
+        if constexpr (kTh) d0 += dt.nm2[j * nd + z0] * pick<R>(ds0r, oh2[j], 0);
+      }
+      dI[j][0] = up ? d0 : R(0);
     }
-    deriv_at<R, false>(lines, args, tile, dt, c, z1, qmask, K1, E1, G);
+    deriv_at<R, false, NQR>(lines, args, tile, dt, tht, c, z1, qmask, K1, E1, G);
     reduce_ks<R>(K1, E1, kc, Sc);
     R oom = K0[0], oo0 = K1[0];
     R g0 = tile.grid[z1];
@@ -384,18 +641,17 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
     R Dm = (oo0 - oom) / h0;
 #pragma unroll
     for (int j = 0; j < kTanMaxNodes; ++j) {
-      const R m = j < ng ? dt.nm[j * nd + z1] : R(0);
-      de0[j] = m * pick<R, 11>(G, oh[j], 0);
+      de0[j] = j < ng ? ndir(j, z1, G, 0) : R(0);
       dDm[j] = (de0[j] - dem[j]) / h0;
     }
     int rc = 0;
     for (int k = 1; k <= nd - 2; ++k) {
       const int zk = zof(k), zn = zof(k + 1);
       R Kn[7], En[4];
-      deriv_at<R, false>(lines, args, tile, dt, c, zn, qmask, Kn, En, G);
+      deriv_at<R, false, NQR>(lines, args, tile, dt, tht, c, zn, qmask, Kn, En, G);
       R dep[kTanMaxNodes];
 #pragma unroll
-      for (int j = 0; j < kTanMaxNodes; ++j) dep[j] = (j < ng ? dt.nm[j * nd + zn] : R(0)) * pick<R, 11>(G, oh[j], 0);
+      for (int j = 0; j < kTanMaxNodes; ++j) dep[j] = j < ng ? ndir(j, zn, G, 0) : R(0);
       const R oop = Kn[0];
       const R gp = tile.grid[zn];
       const R hp = (gp - g0) / mct;
@@ -424,21 +680,20 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
       // full derivatives of depth k, reduced with the new intensity
       {
         R Kd[7], Ed[4];
-        deriv_at<R, true>(lines, args, tile, dt, c, zk, qmask, Kd, Ed, G);
-        reduced_u<R>(G, oo0, kc, Sc, In, qmask, u, ds0);
+        deriv_at<R, true, NQR>(lines, args, tile, dt, tht, c, zk, qmask, Kd, Ed, G);
+        reduced_u<R, NQR>(G, oo0, kc, Sc, In, qmask, u, ds0);
       }
       const R hh = h0 * h0 * R(1.0 / 12.0);
 #pragma unroll
       for (int j = 0; j < kTanMaxNodes; ++j) {
         if (j >= ng) continue;
-        const R m = dt.nm[j * nd + zk];
-          const R dD0 = fgm * dem[j] + fg0 * de0[j] + fgp * dep[j];
+        const R dD0 = fgm * dem[j] + fg0 * de0[j] + fgp * dep[j];
         const R dtb = h0 * R(0.5) * (dem[j] + de0[j]) + hh * (dDm[j] - dD0);
         R kvd[4], rhs[4];
         kmul<R>(kp, dI[j], kvd);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          const R us = m * pick<R, 4>(u, oh[j], s);
+          const R us = ndir(j, zk, u, s);
           rhs[s] = dtb * (dw0 * Pp[s] + dw1 * Pc[s] + detb * Ip[s]) + w0 * v[j][s] + w1 * us + etb * dI[j][s] -
                    w0 * kvd[s];
           v[j][s] = us;
@@ -462,7 +717,7 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
       g0 = gp;
     }
     const size_t o = (size_t)col * nl + l;
-    if (blockIdx.y == 0 && stokes)
+    if (pass == 0 && stokes)
       for (int s = 0; s < 4; ++s) stokes[o * 4 + s] = Ip[s];
     if (!(fabs((double)Ip[0]) < 1e300)) rc |= 2;
     if (rc) atomicOr(status, rc);
@@ -476,57 +731,71 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
   }
 }
 
-// The float kernel is held to two wavefronts per SIMD (256 registers, some
-// spilled): its 392 registers would otherwise leave one wavefront per SIMD,
+// The float kernels are held to two wavefronts per SIMD (256 registers, some
+// spilled): their 392 registers would otherwise leave one wavefront per SIMD,
 // and the sweep waits on memory and LDS; measured 1.34x faster on the
 // inversion workload (four wavefronts, 128 VGPRs with 393 spilled, measured
-// 0.7x).  The double kernel keeps its registers.
-__global__ __launch_bounds__(kTanThreads, 2) void tangent_kernel_f(const KLines lines, const SynthArgs args,
-                                                                  const RespArgs ra, const float *__restrict__ state,
-                                                                  const double *__restrict__ lam,
-                                                                  float *__restrict__ stokes, float *__restrict__ J,
-                                                                  int *__restrict__ status) {
-  tangent_body<float>(lines, args, ra, state, lam, stokes, J, status);
-}
-__global__ __launch_bounds__(kTanThreads) void tangent_kernel_d(const KLines lines, const SynthArgs args,
-                                                               const RespArgs ra, const double *__restrict__ state,
-                                                               const double *__restrict__ lam,
-                                                               double *__restrict__ stokes, double *__restrict__ J,
-                                                               int *__restrict__ status) {
-  tangent_body<double>(lines, args, ra, state, lam, stokes, J, status);
-}
+// 0.7x).  The double kernels keep their registers.  The _th variants carry the
+// temperature and density rows.
+#define SNAPI_TANGENT_KERNEL(NAME, R, NQR, BOUNDS)                                                                 \
+  __global__ BOUNDS void NAME(const KLines lines, const SynthArgs args, const RespArgs ra,                        \
+                              const R *__restrict__ state, const R *__restrict__ dstate,                          \
+                              const double *__restrict__ lam, R *__restrict__ stokes, R *__restrict__ J,         \
+                              int *__restrict__ status, int pass) {                                               \
+    tangent_body<R, NQR>(lines, args, ra, state, dstate, lam, stokes, J, status, pass);                           \
+  }
+SNAPI_TANGENT_KERNEL(tangent_kernel_f, float, NQ, __launch_bounds__(kTanThreads, 2))
+SNAPI_TANGENT_KERNEL(tangent_kernel_th_f, float, NQ_TH, __launch_bounds__(kTanThreads, 2))
+SNAPI_TANGENT_KERNEL(tangent_kernel_d, double, NQ, __launch_bounds__(kTanThreads))
+SNAPI_TANGENT_KERNEL(tangent_kernel_th_d, double, NQ_TH, __launch_bounds__(kTanThreads))
+#undef SNAPI_TANGENT_KERNEL
 
 template <typename R>
-int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, const void *ds_, const double *dl,
-                   void *dstk_, void *dJ_, hipStream_t stream) {
+int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, const void *ds_, const void *dd_,
+                   const double *dl, void *dstk_, void *dJ_, hipStream_t stream) {
   if (ra.nnode <= 0 || ra.nnode > SNAPI_MAX_NODES || !ra.nmap) return SNAPI_ERR_ARG;
-  for (int n = 0; n < ra.nnode; ++n)
-    if (ra.pidx[n] < 2 || ra.pidx[n] > 6) return SNAPI_ERR_UNSUPPORTED;  // T and Nt need the equation of state
+  bool thermo = false;
+  for (int n = 0; n < ra.nnode; ++n) {
+    if (ra.pidx[n] < 0 || ra.pidx[n] > 6) return SNAPI_ERR_ARG;
+    thermo |= ra.pidx[n] < 2 || (ra.nmap2 && ra.pidx2[n] >= 0);
+  }
+  // temperature and density directions need the thermodynamic derivatives,
+  // and H- at the middle wavelength (the response path's) to keep the
+  // background's derivatives wavelength-independent
+  if (thermo && (!dd_ || (a.hminus && !a.hminus_mid))) return SNAPI_ERR_UNSUPPORTED;
   int threads = ((a.nlambda + 63) / 64) * 64;
   if (threads > kTanThreads) threads = kTanThreads;
   const int npass = (ra.nnode + kTanMaxNodes - 1) / kTanMaxNodes;
-  const int ng = ra.nnode < kTanMaxNodes ? ra.nnode : kTanMaxNodes;
-  const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + DerivTile<R>::bytes(a.nd, kl.nline, ng) +
-                      (sizeof(R) == 4 && a.nlambda <= kOrderMax ? (size_t)a.nlambda * 12 : 0);
-  if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;
   int *status = device_status_slot();
   if (!status) return SNAPI_ERR_DEVICE;
   ra.col0 = 0;
-  if (sizeof(R) == 4)
-    hipLaunchKernelGGL(tangent_kernel_f, dim3(ncol, npass), dim3(threads), smem, stream, kl, a, ra,
-                       static_cast<const float *>(ds_), dl, static_cast<float *>(dstk_), static_cast<float *>(dJ_),
-                       status);
-  else
-    hipLaunchKernelGGL(tangent_kernel_d, dim3(ncol, npass), dim3(threads), smem, stream, kl, a, ra,
-                       static_cast<const double *>(ds_), dl, static_cast<double *>(dstk_), static_cast<double *>(dJ_),
-                       status);
-  if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;
+  for (int pass = 0; pass < npass; ++pass) {
+    bool th = false;
+    for (int n = pass * kTanMaxNodes; n < ra.nnode && n < (pass + 1) * kTanMaxNodes; ++n)
+      th |= ra.pidx[n] < 2 || (ra.nmap2 && ra.pidx2[n] >= 0);
+    const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + DerivTile<R>::bytes(a.nd, kl.nline, kTanMaxNodes) +
+                        (th ? ThermoTile<R>::bytes(a.nd, kl.nline) : 0) +
+                        (sizeof(R) == 4 && a.nlambda <= kOrderMax ? (size_t)a.nlambda * 12 : 0);
+    if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;
+    const R *st = static_cast<const R *>(ds_), *dd = static_cast<const R *>(dd_);
+    R *so = static_cast<R *>(dstk_), *Jo = static_cast<R *>(dJ_);
+    if (sizeof(R) == 4) {
+      auto k = th ? tangent_kernel_th_f : tangent_kernel_f;
+      hipLaunchKernelGGL(k, dim3(ncol), dim3(threads), smem, stream, kl, a, ra, (const float *)st, (const float *)dd,
+                         dl, (float *)so, (float *)Jo, status, pass);
+    } else {
+      auto k = th ? tangent_kernel_th_d : tangent_kernel_d;
+      hipLaunchKernelGGL(k, dim3(ncol), dim3(threads), smem, stream, kl, a, ra, (const double *)st,
+                         (const double *)dd, dl, (double *)so, (double *)Jo, status, pass);
+    }
+    if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;
+  }
   return SNAPI_OK;
 }
 
-template int launch_tangent<double>(const KLines &, const SynthArgs &, RespArgs, int, const void *, const double *,
-                                    void *, void *, hipStream_t);
-template int launch_tangent<float>(const KLines &, const SynthArgs &, RespArgs, int, const void *, const double *,
-                                   void *, void *, hipStream_t);
+template int launch_tangent<double>(const KLines &, const SynthArgs &, RespArgs, int, const void *, const void *,
+                                    const double *, void *, void *, hipStream_t);
+template int launch_tangent<float>(const KLines &, const SynthArgs &, RespArgs, int, const void *, const void *,
+                                   const double *, void *, void *, hipStream_t);
 
 }  // namespace snapi

@@ -119,13 +119,14 @@ def node_jacobian(table, ray, model, state, nodes, lam_vac, precision=abi.SNAPI_
 
 
 def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_vac, precision=abi.SNAPI_FP64,
-                      analytic_maps=False):
+                      analytic_maps=False, jacobian=abi.SNAPI_JAC_ADJOINT):
     """(stokes, J) of snapi_node_jacobian_eos: temperature nodes allowed, the
     thermodynamic fields of state rebuilt from the nodes (hydrostatic
     equilibrium with chem_he, equation of state with chem and smap); adjoint
-    Jacobian; the Stokes vector of the response path.  analytic_maps: node
-    responses by exact derivatives and the hydrostatic tangent
-    (SNAPI_JAC_ANALYTIC_MAPS) instead of the reference's differences."""
+    or (jacobian = SNAPI_JAC_TANGENT) forward-mode Jacobian; the Stokes
+    vector of the response path.  analytic_maps: node responses by exact
+    derivatives and the hydrostatic tangent (SNAPI_JAC_ANALYTIC_MAPS) instead
+    of the reference's differences."""
     dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
     st = np.ascontiguousarray(state, dtype=dt)
     if st.ndim == 2:
@@ -143,7 +144,7 @@ def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_
     f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay),
                   C.POINTER(SnapiNodeModel), TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p,
                   C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
-    jac = abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0
+    jac = (abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0) | int(jacobian)
     rc = f(precision, jac, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he),
            C.byref(smap), npix, nd, abi.vptr(st), abi.ptr(nv), len(lam), abi.ptr(lam), abi.vptr(stokes), abi.vptr(J))
     check("snapi_node_jacobian_eos", rc)
@@ -152,7 +153,7 @@ def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_
 
 def invert_eos(table, ray, model, chem, chem_he, smap, state, lam_vac, obs, nodes0, mask=None,
                wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0, precision=abi.SNAPI_FP64,
-               broadening=0.0, analytic_maps=False):
+               broadening=0.0, analytic_maps=False, jacobian=abi.SNAPI_JAC_ADJOINT):
     """snapi_invert_eos: the inversion with temperature nodes (adjoint
     Jacobian; analytic_maps as node_jacobian_eos). Returns (nodes, chi2,
     iterations, total pixel-iterations)."""
@@ -174,7 +175,7 @@ def invert_eos(table, ray, model, chem, chem_he, smap, state, lam_vac, obs, node
                   TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double),
                   C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
-    jac = abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0
+    jac = (abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0) | int(jacobian)
     rc = f(precision, jac, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
            npix, nd, abi.vptr(st), len(lam), abi.ptr(lam), abi.vptr(ob), None if mk is None else abi.vptr(mk),
            abi.ptr(ws), float(broadening), lm_start, max_iter, nslot, abi.ptr(nodes), abi.ptr(chi2),
@@ -216,7 +217,8 @@ def build_from_nodes_dev(model, chem, chem_he, smap, state_t, nodes_t, stream=No
 
 def invert_eos_dev(table, ray, model, chem, chem_he, smap, state_t, lam_t, obs_t, mask_t, nodes0_t, nodes_t, chi2_t,
                    iters_t, wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0,
-                   precision=abi.SNAPI_MIXED, stream=None, broadening=0.0, analytic_maps=False):
+                   precision=abi.SNAPI_MIXED, stream=None, broadening=0.0, analytic_maps=False,
+                   jacobian=abi.SNAPI_JAC_ADJOINT):
     """snapi_invert_eos_dev on torch device tensors; returns the total
     pixel-iterations."""
     npix, nf, nd = state_t.shape
@@ -231,7 +233,7 @@ def invert_eos_dev(table, ray, model, chem, chem_he, smap, state_t, lam_t, obs_t
                   C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]
     sp = None if stream is None else C.c_void_p(stream.cuda_stream)
-    jac = abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0
+    jac = (abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0) | int(jacobian)
     rc = f(precision, jac, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
            npix, nd, state_t.data_ptr(), nl, lam_t.data_ptr(), obs_t.data_ptr(),
            None if mask_t is None else mask_t.data_ptr(), abi.ptr(ws), float(broadening), lm_start, max_iter, nslot,

@@ -108,3 +108,55 @@ def test_inversion_follows_reference_trajectory(setup):
             assert rel.max() < 2e-3, (k, p, rel.max(), rel.argmax())
             assert iters[p] == f["iters%d" % p][k - 1]
     print("  worst node difference %.2e" % worst, flush=True)
+
+
+@pytest.mark.parametrize("precision", [abi.SNAPI_FP64, abi.SNAPI_MIXED])
+def test_tangent_node_jacobian_with_temperature_nodes(setup, precision):
+    """The forward-mode sweep with the temperature and density rows against
+    the reference's Jacobian (3e-4 per node in FP64, 1e-3 mixed) and, in
+    FP64, against the adjoint sweep (both exact derivatives of the same
+    discrete model: 1e-9)."""
+    s = setup
+    g = s["g"]
+    vals = np.stack([g["values0"], g["values1"]])
+    st = np.repeat(s["state"][None], 2, axis=0)
+    print("tangent node Jacobian with temperature nodes: 2 node sets, %d wavelengths, precision %d"
+          % (len(s["lam"]), precision), flush=True)
+    S, J = invert.node_jacobian_eos(s["t"]["table"], s["t"]["ray"], s["model"], s["chem"], s["chem_he"], s["smap"],
+                                    st, vals, s["lam"], precision=precision, jacobian=abi.SNAPI_JAC_TANGENT)
+    if precision == abi.SNAPI_FP64:
+        Sa, Ja = invert.node_jacobian_eos(s["t"]["table"], s["t"]["ray"], s["model"], s["chem"], s["chem_he"],
+                                          s["smap"], st, vals, s["lam"], precision=precision)
+        ea = float((np.abs(J - Ja).max(axis=1) / np.abs(Ja).max(axis=1)).max())
+        print("  tangent vs adjoint: max %.2e of each node's largest entry" % ea, flush=True)
+        assert ea < 1e-9
+        assert np.abs(S - Sa).max() / Sa[..., 0].max() < 1e-12
+    bar = 3e-4 if precision == abi.SNAPI_FP64 else 1e-3
+    for k in range(2):
+        jr = np.transpose(g["jac%d" % k], (1, 2, 0)).reshape(-1, len(s["nodes"]))
+        ej = np.abs(J[k].astype(np.float64) - jr).max(axis=0) / np.abs(jr).max(axis=0)
+        print("  set %d vs reference: J per node max %.2e (node %d)" % (k, ej.max(), ej.argmax()), flush=True)
+        assert ej.max() < bar
+
+
+def test_tangent_inversion_follows_reference_trajectory(setup):
+    """The inversion with temperature nodes on the tangent Jacobian: the same
+    trajectory as the reference over eight iterations (2e-3)."""
+    s = setup
+    f = s["f"]
+    K = int(f["kmax"])
+    obs = np.stack([f["obs%d" % p] for p in range(3)])
+    mask = np.stack([f["mask%d" % p] for p in range(3)])
+    start = np.stack([f["start%d" % p] for p in range(3)])
+    st = np.repeat(s["state"][None], 3, axis=0)
+    print("inversion (tangent Jacobian) vs stokes_lm_fit: 3 pixels, %d iterations, FP64" % K, flush=True)
+    nodes, chi2, iters, total = invert.invert_eos(s["t"]["table"], s["t"]["ray"], s["model"], s["chem"],
+                                                  s["chem_he"], s["smap"], st, s["lam"], obs, start, mask=mask,
+                                                  wstokes=f["ws"], lm_start=float(f["lm_start"]), max_iter=K,
+                                                  nslot=3, precision=abi.SNAPI_FP64,
+                                                  jacobian=abi.SNAPI_JAC_TANGENT)
+    for p in range(3):
+        rel = np.abs(nodes[p] - f["nodes%d" % p][K - 1]) / np.abs(f["nodes%d" % p][K - 1])
+        print("  pixel %d: iterations %d, worst node %.2e" % (p, iters[p], rel.max()), flush=True)
+        assert rel.max() < 2e-3 and iters[p] == K
+

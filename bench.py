@@ -61,7 +61,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--lm-side", type=int, default=256, help="inversion leg: side of the pixel square")
     ap.add_argument("--lm-fd-pixels", type=int, default=8192, help="pixels of the FP64 difference-Jacobian run")
-    ap.add_argument("--tnode-side", type=int, default=48, help="14-node inversion: side of the pixel square")
+    ap.add_argument("--tnode-side", type=int, default=256, help="14-node inversion: side of the pixel square")
+    ap.add_argument("--tnode-legs", default="mixed_tangent_analytic,mixed_tangent_analytic_first,fp64_differences",
+                    help="14-node inversion runs, comma-separated: mixed_tangent_analytic, mixed_adjoint, "
+                         "mixed_adjoint_analytic, fp64_differences")
+    ap.add_argument("--tnode-fd-iters", type=int, default=2, help="iterations of the FP64 difference-Jacobian run")
     ap.add_argument("--no-lm", action="store_true")
     ap.add_argument("--ali-side", type=int, default=64, help="ALI leg: side of the column square")
     ap.add_argument("--no-ali", action="store_true")
@@ -176,7 +180,8 @@ def main():
 
     lm = None
     if not args.no_lm:
-        lm = lm_leg(torch, dev, tmpl, lam_d, stream, args.lm_side, args.lm_fd_pixels, args.tnode_side)
+        lm = lm_leg(torch, dev, tmpl, lam_d, stream, args.lm_side, args.lm_fd_pixels, args.tnode_side,
+                    tuple(x for x in args.tnode_legs.split(",") if x), args.tnode_fd_iters)
 
     alires = None
     if not args.no_ali:
@@ -510,7 +515,8 @@ def ca8542_leg(torch, dev, stream, p, pops, with_cpu):
     return res
 
 
-def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd, tnode_side=48):
+def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd, tnode_side=48, tnode_legs=("mixed_tangent_analytic",),
+           tnode_fd_iters=2):
     from snapi_amd import abi, api, invert, workloads
     table, ray = tmpl["table"], tmpl["ray"]
     nlam = lam_d.shape[0]
@@ -568,7 +574,8 @@ def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd, tnode_side=48):
             % (label, total, sec, total / sec, it.mean(), np.median(red)))
         return out
 
-    tnodes = tnode_inversion(torch, dev, tmpl, lam_d, stream, ws, side=tnode_side)
+    tnodes = tnode_inversion(torch, dev, tmpl, lam_d, stream, ws, side=tnode_side, legs=tnode_legs,
+                             fd_iters=tnode_fd_iters)
     mixed = run(npix, abi.SNAPI_MIXED, 2, "mixed tangent")
     fd = run(min(nfd, npix), abi.SNAPI_FP64, 1, "fp64 differences")
     adj = run(min(nfd, npix), abi.SNAPI_MIXED, 0, "mixed adjoint")
@@ -581,16 +588,20 @@ def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd, tnode_side=48):
     _, j32 = invert.node_jacobian(table, ray, model, st8.astype(np.float32), v8, lam_h, abi.SNAPI_MIXED, 2)
     sc = np.maximum(np.abs(j64).max(axis=1, keepdims=True), 1e-300)
     jerr = float(np.max(np.abs(j32.astype(np.float64) - j64) / sc))
-    return {"workload": "batched LM inversion, %d pixels of the cube, 71 depths, %d wavelengths masked to "
-                        "120-%d per pixel, %d nodes (2 vLOS, 1 vmicro, 2 B, 1 incl, 1 azi), noise 1e-3 Ic, "
-                        "start +/-20 %%, <=30 iterations" % (npix, nlam, nlam, nn),
-            "value": mixed["pixel_iter_per_s"], "unit": "LM pixel-iterations/s",
-            "mixed_tangent": mixed, "fp64_differences": fd, "mixed_adjoint": adj,
-            "speedup_vs_fp64_differences": mixed["pixel_iter_per_s"] / fd["pixel_iter_per_s"],
-            "jacobian_rel_err_mixed_vs_fp64": jerr, "with_temperature_nodes": tnodes}
+    kin = {"workload": "batched LM inversion, %d pixels of the cube, 71 depths, %d wavelengths masked to "
+                       "120-%d per pixel, %d nodes (2 vLOS, 1 vmicro, 2 B, 1 incl, 1 azi) with the thermodynamic "
+                       "state held fixed, noise 1e-3 Ic, start +/-20 %%, <=30 iterations" % (npix, nlam, nlam, nn),
+           "value": mixed["pixel_iter_per_s"], "unit": "LM pixel-iterations/s",
+           "mixed_tangent": mixed, "fp64_differences": fd, "mixed_adjoint": adj,
+           "speedup_vs_fp64_differences": mixed["pixel_iter_per_s"] / fd["pixel_iter_per_s"],
+           "jacobian_rel_err_mixed_vs_fp64": jerr}
+    head = tnodes.get("mixed_tangent_analytic") or next(v for v in tnodes.values() if isinstance(v, dict))
+    return {"workload": tnodes["workload"] + "; mixed tangent Jacobian, analytic node maps",
+            "value": head["pixel_iter_per_s"], "unit": "LM pixel-iterations/s", "nodes": 14,
+            "pixels": head["pixels"], "with_temperature_nodes": tnodes, "kinematic_7_nodes": kin}
 
 
-def tnode_inversion(torch, dev, tmpl, lam_d, stream, ws, side=48):
+def tnode_inversion(torch, dev, tmpl, lam_d, stream, ws, side=48, legs=("mixed_tangent_analytic",), fd_iters=2):
     """The 14-node model (7 temperature nodes): every iteration rebuilds the
     thermodynamics (hydrostatic equilibrium, equation of state) and the
     temperature-node responses (two rebuilds per node), adjoint Jacobian in
@@ -619,42 +630,80 @@ def tnode_inversion(torch, dev, tmpl, lam_d, stream, ws, side=48):
     wsq = sum(w * w for w in ws)
     chi2_noise = mask.sum(dim=1) * wsq * (1e-3 * ic) ** 2
     log("inversion with temperature nodes: %d pixels, %d nodes, built in %.1fs" % (npix, nn, time.time() - t0))
-    st_t = st64.to(torch.float32).contiguous()
-    ob_t = obs64.to(torch.float32).contiguous()
-    mk_t = mask.to(torch.float32).contiguous()
     nodes = torch.empty((npix, nn), dtype=torch.float64, device=dev)
     chi2 = torch.empty(npix, dtype=torch.float64, device=dev)
     iters = torch.empty(npix, dtype=torch.int32, device=dev)
 
-    def run(analytic):
-        invert.invert_eos_dev(table, ray, model, chem, chem_he, smap, st_t[:16], lam_d, ob_t[:16], mk_t[:16],
+    def run(analytic, jacobian, precision=abi.SNAPI_MIXED, max_iter=30, label="", nslot=0):
+        dt_ = torch.float32 if precision == abi.SNAPI_MIXED else torch.float64
+        st_p = st64.to(dt_).contiguous()
+        ob_p = obs64.to(dt_).contiguous()
+        mk_p = mask.to(dt_).contiguous()
+        invert.invert_eos_dev(table, ray, model, chem, chem_he, smap, st_p[:16], lam_d, ob_p[:16], mk_p[:16],
                               start[:16], nodes[:16], chi2[:16], iters[:16], wstokes=ws, max_iter=2,
-                              precision=abi.SNAPI_MIXED, stream=stream, analytic_maps=analytic)
+                              precision=precision, stream=stream, analytic_maps=analytic, jacobian=jacobian)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        total = invert.invert_eos_dev(table, ray, model, chem, chem_he, smap, st_t, lam_d, ob_t, mk_t, start, nodes,
-                                      chi2, iters, wstokes=ws, max_iter=30, precision=abi.SNAPI_MIXED, stream=stream,
-                                      analytic_maps=analytic)
+        total = invert.invert_eos_dev(table, ray, model, chem, chem_he, smap, st_p, lam_d, ob_p, mk_p, start, nodes,
+                                      chi2, iters, wstokes=ws, max_iter=max_iter, nslot=nslot, precision=precision,
+                                      stream=stream, analytic_maps=analytic, jacobian=jacobian)
         torch.cuda.synchronize()
         sec = time.perf_counter() - t1
         it = iters.cpu().numpy()
         red = (chi2 / chi2_noise).cpu().numpy()
         dT = np.abs(nodes.cpu().numpy()[:, 2:6] - prob["truth"][:, 2:6])
-        log("14-node inversion%s: %d pixel-iterations in %.2fs = %.4g /s; mean iterations %.1f; median chi2/noise "
-            "%.3g; median |dT| (log tau -3..0) %.0f K" % (" (analytic maps)" if analytic else "", total, sec,
-                                                          total / sec, it.mean(), np.median(red), np.median(dT)))
+        log("14-node inversion (%s): %d pixel-iterations in %.2fs = %.4g /s; mean iterations %.1f; median chi2/noise "
+            "%.3g; median |dT| (log tau -3..0) %.0f K" % (label, total, sec, total / sec, it.mean(), np.median(red),
+                                                          np.median(dT)))
         return {"pixels": npix, "pixel_iterations": int(total), "seconds": round(sec, 3),
                 "pixel_iter_per_s": total / sec, "mean_iterations": float(it.mean()),
-                "at_iteration_limit": float(np.mean(it >= 30)), "median_chi2_over_noise": float(np.median(red)),
+                "at_iteration_limit": float(np.mean(it >= max_iter)), "iteration_cap": max_iter,
+                "median_chi2_over_noise": float(np.median(red)),
                 "median_abs_T_err_logtau_-3_to_0_K": float(np.median(dT))}
 
     out = {"workload": "batched LM inversion with the 14-node model (7 T, 2 vLOS, 1 vmicro, 2 B, 1 incl, 1 azi), "
                        "%d pixels, 71 depths, %d wavelengths masked to 120-%d, noise 1e-3 Ic, start +/-20 %%, "
-                       "<=30 iterations, mixed adjoint" % (npix, nlam, nlam)}
-    out.update(run(False))
-    # node responses by exact derivatives and the hydrostatic tangent instead
-    # of two hydrostatic rebuilds per temperature node
-    out["analytic_maps"] = run(True)
+                       "<=30 iterations" % (npix, nlam, nlam)}
+    for key in legs:
+        if key == "mixed_tangent_analytic":
+            # the fast path: forward-mode Jacobian, node responses by exact
+            # derivatives and the hydrostatic tangent
+            out[key] = run(True, abi.SNAPI_JAC_TANGENT, label="mixed tangent, analytic maps")
+        elif key == "mixed_adjoint":
+            # the reference's node responses (two hydrostatic rebuilds per
+            # temperature node) with the adjoint Jacobian
+            out[key] = run(False, abi.SNAPI_JAC_ADJOINT, label="mixed adjoint, reference node responses")
+        elif key == "mixed_adjoint_analytic":
+            out[key] = run(True, abi.SNAPI_JAC_ADJOINT, label="mixed adjoint, analytic maps")
+        elif key == "mixed_tangent_analytic_first":
+            out[key] = run(True, abi.SNAPI_JAC_TANGENT, max_iter=fd_iters,
+                           label="mixed tangent, analytic maps, first %d iterations" % fd_iters)
+        elif key == "fp64_differences":
+            # 2n + 1 node sets per slot, each with its own hydrostatic march
+            # and equation of state: 8192 slots at a time through the queue
+            out[key] = run(False, abi.SNAPI_JAC_DIFFERENCES, abi.SNAPI_FP64, fd_iters,
+                           label="fp64 differences, first %d iterations" % fd_iters, nslot=min(8192, npix))
+    if "mixed_tangent_analytic_first" in out and "fp64_differences" in out:
+        # the same iterations (every one with a Jacobian) of the same pixels
+        out["speedup_vs_fp64_differences_same_iterations"] = (
+            out["mixed_tangent_analytic_first"]["pixel_iter_per_s"] / out["fp64_differences"]["pixel_iter_per_s"])
+    if "mixed_tangent_analytic" in out and "fp64_differences" in out:
+        out["speedup_vs_fp64_differences"] = (out["mixed_tangent_analytic"]["pixel_iter_per_s"] /
+                                              out["fp64_differences"]["pixel_iter_per_s"])
+    # node-Jacobian error of the mixed tangent sweep against the FP64 one at
+    # the truth of 8 pixels (every 10th wavelength), relative to the largest
+    # |dS/dnode| of each pixel and node
+    lam_h = lam_d.cpu().numpy()[::10]
+    st8 = invert.build_from_nodes(model, chem, chem_he, smap, prob["base"][:8], prob["truth"][:8])
+    v8 = prob["truth"][:8]
+    _, j64 = invert.node_jacobian_eos(table, ray, model, chem, chem_he, smap, st8, v8, lam_h, abi.SNAPI_FP64,
+                                      jacobian=abi.SNAPI_JAC_TANGENT)
+    _, j32 = invert.node_jacobian_eos(table, ray, model, chem, chem_he, smap, st8, v8, lam_h, abi.SNAPI_MIXED,
+                                      jacobian=abi.SNAPI_JAC_TANGENT)
+    sc = np.maximum(np.abs(j64).max(axis=1, keepdims=True), 1e-300)
+    out["jacobian_rel_err_mixed_vs_fp64"] = float(np.max(np.abs(j32.astype(np.float64) - j64) / sc))
+    log("14-node tangent Jacobian, mixed vs FP64: %.2e of each node's largest entry"
+        % out["jacobian_rel_err_mixed_vs_fp64"])
     return out
 
 

@@ -483,6 +483,12 @@ template <typename R> __global__ void fill_kernel(size_t n, R v, R *__restrict__
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// per-(slot, node set) activity from the slots' flags
+__global__ void expand_active_kernel(long n, int group, const int *__restrict__ active, int *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = active[i / group];
+}
+
 __global__ void count_active_kernel(int nslot, const int *__restrict__ active, int *__restrict__ count) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot < nslot && active[slot]) atomicAdd(count, 1);
@@ -676,6 +682,9 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   // temperature node +- delta_T/2, or the nodes alone with the tangent
   const int nvar = na.analytic ? 1 : 1 + 2 * ntn;
   double *dTd = nullptr, *dNd = nullptr, *hscr = nullptr;
+  // difference Jacobian with temperature nodes: per (slot, node set)
+  int *fd_active = nullptr;
+  double *fdTv = nullptr, *fdltv = nullptr, *fdNtv = nullptr, *fdsd = nullptr, *fdescr = nullptr;
   // caller's buffers (device variant) or copies of the host arrays
   void *own[8] = {};
   SlotState ss;
@@ -738,7 +747,11 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   ss.chi2t = dbl + 2 * nslot;
   ss.track = dbl + 3 * nslot;
   if (th) {
-    const bool jac_ok = jacobian == SNAPI_JAC_ADJOINT || jacobian == SNAPI_JAC_TANGENT;
+    // the difference Jacobian differences the synthesis path's spectrum of
+    // every node set; with spectral broadening only the response-path
+    // Jacobians are built
+    const bool jac_ok = jacobian == SNAPI_JAC_ADJOINT || jacobian == SNAPI_JAC_TANGENT ||
+                        (jacobian == SNAPI_JAC_DIFFERENCES && !(th->broadening > 0.0));
     if (!jac_ok || nf != SNAPI_NF_BASE + 2 * th->map->nline) {
       rc = !jac_ok ? SNAPI_ERR_UNSUPPORTED : SNAPI_ERR_ARG;
       goto done;
@@ -781,6 +794,17 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     HIPCHK(hipMalloc((void **)&Sall, (size_t)nslot * nv * nl * 4 * es));
     HIPCHK(hipMalloc((void **)&fdn, (size_t)nslot * nv * nn * 8));
     HIPCHK(hipMalloc((void **)&hv, (size_t)nslot * nn * 8));
+    if (th) {
+      // every node set's own temperature, hydrostatic densities and
+      // equation of state
+      const size_t nsv = (size_t)nslot * nv;
+      HIPCHK(hipMalloc((void **)&fd_active, nsv * sizeof(int)));
+      HIPCHK(hipMalloc((void **)&fdTv, nsv * nd * 8));
+      HIPCHK(hipMalloc((void **)&fdltv, nsv * nd * 8));
+      HIPCHK(hipMalloc((void **)&fdNtv, nsv * nd * 8));
+      HIPCHK(hipMalloc((void **)&fdsd, nsv * ns * 8));
+      HIPCHK(hipMalloc((void **)&fdescr, chem_eos_scratch(cdev, (long)nsv * nd, false)));
+    }
   }
   if (!dev) {
     HIPCHK(hipMemcpy(pstate, pstate_h, (size_t)npix * ns * es, hipMemcpyHostToDevice));
@@ -843,7 +867,37 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
       // model -> state and Jacobian where the model changed
       hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, snodes,
                          (size_t)nn, sstate, ns, nmap, ss.needj);
-      if (th) {
+      if (th && jacobian == SNAPI_JAC_DIFFERENCES) {
+        // central differences of the synthesis path over the 2n + 1 node
+        // sets, every set with its own temperature, hydrostatic equilibrium
+        // and equation of state (forward_evaluate per set)
+        const long nsv = (long)nslot * nv;
+        hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, fdn, hv,
+                           ss.needj);
+        for (int k = 0; k < nv; ++k) {
+          HIPCHK(hipMemcpy2DAsync(fdstate + (size_t)k * ns, nv * ns * es, sstate, ns * es, ns * es, nslot,
+                                  hipMemcpyDeviceToDevice, st));
+          hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf,
+                             fdn + (size_t)k * nn, (size_t)nv * nn, fdstate + (size_t)k * ns, (size_t)nv * ns,
+                             (R *)nullptr, ss.needj);
+        }
+        hipLaunchKernelGGL(expand_active_kernel, dim3((unsigned)((nsv + 255) / 256)), dim3(256), 0, st, nsv, nv,
+                           ss.needj, fd_active);
+        hipLaunchKernelGGL(tprofiles_kernel<R>, dim3((unsigned)nsv), dim3(128), 0, st, na, (int)nsv, nd, ns, fdstate,
+                           fdn, 1, fdTv, fdltv, fd_active);
+        rc = chem_hequilibrium(cdev_he, nsv, nd, fdTv, fdltv, fdNtv, fd_active, 1, st);
+        if (rc) goto done;
+        hipLaunchKernelGGL(tmaps_kernel<R>, dim3((unsigned)nsv), dim3(128), 0, st, na, (int)nsv, nd, nf, 1, fdTv,
+                           fdNtv, fdstate, (R *)nullptr, (R *)nullptr, fdsd, fd_active);
+        rc = chem_eos(cdev, nsv, nd, fdsd, nullptr, nullptr, fdescr, fd_active, st);
+        if (rc) goto done;
+        hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nsv * ns, fdsd,
+                           fdstate);
+        rc = synth_dev_masked(precision, lines, &ray_syn, (int)nsv, nd, fdstate, nl, dlam, Sall, ss.needj, nv, st);
+        if (rc) goto done;
+        hipLaunchKernelGGL(fdjac_kernel<R>, dim3(nslot), dim3(256), 0, st, na, nslot, nl, Sall, hv, smask, ws[0], ws[1],
+                           ws[2], ws[3], S, J, ss.needj);
+      } else if (th) {
         // build_from_nodes at the nodes and at each temperature node +- 2.5 K,
         // then the response path's state and derivatives (the reference's
         // Jacobian and the current chi^2 come from that path)
@@ -964,6 +1018,7 @@ done:
   hipFree(sstate); hipFree(tstate); hipFree(dzero);
   hipFree(sobs); hipFree(smask); hipFree(S); hipFree(St); hipFree(J); hipFree(r); hipFree(nmap);
   hipFree(fdstate); hipFree(Sall); hipFree(scr); hipFree(snodes);
+  hipFree(fd_active); hipFree(fdTv); hipFree(fdltv); hipFree(fdNtv); hipFree(fdsd); hipFree(fdescr);
   hipFree(tnodes); hipFree(JTJ); hipFree(rhs); hipFree(corr); hipFree(fdn);
   hipFree(hv); hipFree(dbl); hipFree(ints); hipFree(queue);
   chem_free(cdev); chem_free(cdev_he);

@@ -312,7 +312,7 @@ def test_optical_depth_scale_matches_reference_file():
     tt = _oracle.optical_depth_scale(g["height"], g["opref"])
     assert np.max(np.abs(np.log10(tt) - g["logtau"])) < 1e-14
     t6 = _oracle.optical_depth_scale(g["height"], g["opref"], math.acos(0.6))
-    assert np.max(np.abs((t6 - 1e-8) / ((tt - 1e-8) / 0.6) - 1.0)[1:]) < 1e-12
+    assert np.max(np.abs((t6[1:] - 1e-8) / ((tt[1:] - 1e-8) / 0.6) - 1.0)) < 1e-12
     # the upward ray starts at the bottom
     tu = _oracle.optical_depth_scale(g["height"], g["opref"], math.pi)
     assert tu[-1] == 1e-8 and np.all(np.diff(tu) < 0)

@@ -46,8 +46,10 @@ template <typename R> struct DerivTile {
   R *gv;                     // per line and depth: d ln(Doppler width) / d vt
   R *nm;                     // [node of the pass][z]: node map d q(z) / d node
   R *nm2;                    // [node of the pass][z]: the node's second row (Nt of a temperature node)
-  static __host__ __device__ size_t bytes(int nd, int nline, int nn) {
-    return sizeof(R) * (size_t)nd * (4 + nline + 2 * nn);
+  // nn nodes in the pass; two: the second node map is held (passes with
+  // temperature or density rows)
+  static __host__ __device__ size_t bytes(int nd, int nline, int nn, bool two) {
+    return sizeof(R) * (size_t)nd * (4 + nline + (two ? 2 : 1) * nn);
   }
   static __device__ DerivTile carve(R *p, int nd, int nline, int nn) {
     DerivTile t;
@@ -545,16 +547,15 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
   const int ng = min(ra.nnode - n0, kTanMaxNodes);
   const R *st = state + (size_t)col * nf * nd;
   ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
-  DerivTile<R> dt =
-      DerivTile<R>::carve(sm + ColumnTile<R>::bytes(nd, lines.nline) / sizeof(R), nd, lines.nline, kTanMaxNodes);
+  DerivTile<R> dt = DerivTile<R>::carve(sm + ColumnTile<R>::bytes(nd, lines.nline) / sizeof(R), nd, lines.nline, ng);
   ThermoTile<R> tht = ThermoTile<R>::carve(
-      sm + (ColumnTile<R>::bytes(nd, lines.nline) + DerivTile<R>::bytes(nd, lines.nline, kTanMaxNodes)) / sizeof(R),
-      nd, lines.nline);
+      sm + (ColumnTile<R>::bytes(nd, lines.nline) + DerivTile<R>::bytes(nd, lines.nline, ng, kTh)) / sizeof(R), nd,
+      lines.nline);
   // the synthesis kernel's grouped wavelength order (float path)
   int *order = nullptr;
   if (sizeof(R) == 4 && nl <= kOrderMax) {
     order = reinterpret_cast<int *>(smem_raw + ColumnTile<R>::bytes(nd, lines.nline) +
-                                    DerivTile<R>::bytes(nd, lines.nline, kTanMaxNodes) +
+                                    DerivTile<R>::bytes(nd, lines.nline, ng, kTh) +
                                     (kTh ? ThermoTile<R>::bytes(nd, lines.nline) : 0));
     lambda_order(lines, lam, nl, reinterpret_cast<float *>(order + nl), order + 2 * nl, order);
   }
@@ -773,7 +774,8 @@ int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, 
     bool th = false;
     for (int n = pass * kTanMaxNodes; n < ra.nnode && n < (pass + 1) * kTanMaxNodes; ++n)
       th |= ra.pidx[n] < 2 || (ra.nmap2 && ra.pidx2[n] >= 0);
-    const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + DerivTile<R>::bytes(a.nd, kl.nline, kTanMaxNodes) +
+    const int ng = ra.nnode - pass * kTanMaxNodes < kTanMaxNodes ? ra.nnode - pass * kTanMaxNodes : kTanMaxNodes;
+    const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + DerivTile<R>::bytes(a.nd, kl.nline, ng, th) +
                         (th ? ThermoTile<R>::bytes(a.nd, kl.nline) : 0) +
                         (sizeof(R) == 4 && a.nlambda <= kOrderMax ? (size_t)a.nlambda * 12 : 0);
     if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;

@@ -794,8 +794,11 @@ int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *ma
  * Stokes vector returned is the response path's, as the reference returns
  * it, with H- at ray->lam_mid as that path evaluates it
  * (atmos/atmol/mol/mol.cc:414-469) whatever ray->hminus_mid says.  At least
- * one temperature node; adjoint Jacobian only (SNAPI_ERR_UNSUPPORTED
- * otherwise).  J per K for temperature nodes.
+ * one temperature node.  SNAPI_JAC_ADJOINT contracts the per-depth T and Nt
+ * responses with the node maps; SNAPI_JAC_TANGENT carries each temperature
+ * node's direction dT/dnode d/dT + dNt/dnode d/dNt through the forward
+ * sweep; SNAPI_JAC_DIFFERENCES is SNAPI_ERR_UNSUPPORTED here (the fits take
+ * it, below).  J per K for temperature nodes.
  */
 /*
  * snapi_invert[_dev] with temperature nodes: per pixel and iteration the
@@ -810,7 +813,11 @@ int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *ma
  * the parameter limits before the first iteration (atmos_fit.cc:91); the
  * returned inclination and azimuth polished to acos(cos) and atan(tan)
  * (model::polish_angles, atmos/atmos_model.cc:992-1001).  At least one
- * temperature node; adjoint Jacobian only.  broadening > 0: the Gaussian spectral broadening [cm] the
+ * temperature node.  Jacobians: SNAPI_JAC_ADJOINT and SNAPI_JAC_TANGENT as
+ * snapi_node_jacobian_eos; SNAPI_JAC_DIFFERENCES differences the synthesis
+ * path's spectrum over node sets each rebuilt through hydrostatic
+ * equilibrium and the equation of state (the FP64 baseline; refused with
+ * broadening > 0).  broadening > 0: the Gaussian spectral broadening [cm] the
  * fit applies to the current spectrum and Jacobian (before the Stokes weights
  * and the mask) and to every trial spectrum (snapi_spectral_convolve;
  * atmos_fit.cc:104-110, 717-737); 0: none.

@@ -384,6 +384,10 @@ __global__ void trial_kernel(NodeArgs na, int nslot, const double *__restrict__ 
 
 struct SlotState {
   int *pix, *iter, *corrected, *ntrack, *active, *needj;
+  // 1 where the Jacobian must march the hydrostatic equilibrium itself: a
+  // slot just refilled; 0 after an accepted trial, whose temperature and
+  // equilibrium profiles (at the now current nodes) are still in place
+  int *march;
   double *lm, *chi2, *chi2t, *track;
 };
 
@@ -420,6 +424,7 @@ __global__ void update_kernel(NodeArgs na, int nslot, int max_iter, SlotState ss
   }
   if (stop || ss.iter[slot] >= max_iter) ss.active[slot] = 0;
   ss.needj[slot] = ss.active[slot] && ss.corrected[slot];
+  ss.march[slot] = 0;
 }
 
 // Finished slots write their pixel's result and take the next pixel of the
@@ -458,12 +463,14 @@ __global__ void refill_kernel(NodeArgs na, int nslot, int npix, int nf, int nd, 
       ss.lm[slot] = lm0;
       ss.active[slot] = 1;
       ss.needj[slot] = 1;
+      ss.march[slot] = 1;
       // the start put in the parameter limits, as stokes_lm_fit does before
       // its first iteration (atmos_fit.cc:91)
       for (int n = 0; n < na.nnode; ++n)
         snodes[(size_t)slot * na.nnode + n] = bracket(na.code[n], pnodes0[(size_t)next * na.nnode + n]);
     } else {
       ss.needj[slot] = 0;
+      ss.march[slot] = 0;
     }
   }
   __syncthreads();
@@ -733,7 +740,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   HIPCHK(hipMalloc((void **)&rhs, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&corr, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&dbl, (size_t)nslot * 6 * 8));
-  HIPCHK(hipMalloc((void **)&ints, (size_t)nslot * 6 * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&ints, (size_t)nslot * 7 * sizeof(int)));
   HIPCHK(hipMalloc((void **)&queue, 2 * sizeof(int)));
   count = queue + 1;
   ss.pix = ints;
@@ -742,6 +749,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   ss.ntrack = ints + 3 * nslot;
   ss.active = ints + 4 * nslot;
   ss.needj = ints + 5 * nslot;
+  ss.march = ints + 6 * nslot;
   ss.lm = dbl;
   ss.chi2 = dbl + nslot;
   ss.chi2t = dbl + 2 * nslot;
@@ -814,7 +822,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     HIPCHK(hipMemcpy(pnodes0, nodes0, (size_t)npix * nn * 8, hipMemcpyHostToDevice));
   }
   HIPCHK(hipMemsetAsync(dzero, 0, (size_t)nslot * 2 * ns * es, st));
-  HIPCHK(hipMemsetAsync(ints, 0, (size_t)nslot * 6 * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(ints, 0, (size_t)nslot * 7 * sizeof(int), st));
   HIPCHK(hipMemsetAsync(dbl, 0, (size_t)nslot * 6 * 8, st));
   HIPCHK(hipMemsetAsync(queue, 0, 2 * sizeof(int), st));
   {
@@ -901,9 +909,13 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
         // build_from_nodes at the nodes and at each temperature node +- 2.5 K,
         // then the response path's state and derivatives (the reference's
         // Jacobian and the current chi^2 come from that path)
+        // with the analytic maps (one profile per slot) an accepted trial
+        // leaves the temperature and equilibrium of the current nodes in
+        // Tv, ltv and Ntv: only refilled slots march
+        const int *hmask = na.analytic ? ss.march : ss.needj;
         hipLaunchKernelGGL(tprofiles_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ns, sstate, snodes, nvar,
-                           Tv, ltv, ss.needj);
-        rc = chem_hequilibrium(cdev_he, (long)nslot * nvar, nd, Tv, ltv, Ntv, ss.needj, nvar, st);
+                           Tv, ltv, hmask);
+        rc = chem_hequilibrium(cdev_he, (long)nslot * nvar, nd, Tv, ltv, Ntv, hmask, nvar, st);
         if (rc) goto done;
         if (na.analytic) {
           if (ntn) {

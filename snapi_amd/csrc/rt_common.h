@@ -95,7 +95,14 @@ template <> __device__ __forceinline__ float rsqrt_unit<float>(float x) { return
 // regions; H = Re w / sqrt(pi), F = Im w / sqrt(pi).  Region 4 takes
 // cos(Im u) as +sqrt(1 - sin^2(Im u)), the form the reference evaluates
 // (cmlib/profiles.cc:7-37).  The region tests are uniform per lane.
+template <typename R> __device__ __forceinline__ void voigt_raw(R v, R a, R &H, R &F);
 template <typename R> __device__ __forceinline__ void voigt(R v, R a, R &H, R &F) {
+  voigt_raw<R>(v, a, H, F);
+  H *= R(1.0 / kSqrtPi);
+  F *= R(1.0 / kSqrtPi);
+}
+// w(z) itself: H = Re w, F = Im w without the 1/sqrt(pi).
+template <typename R> __device__ __forceinline__ void voigt_raw(R v, R a, R &H, R &F) {
   const R av = v < R(0) ? -v : v;
   const R s = av + a;
   const cplx<R> t = cm<R>(a, -v);
@@ -138,8 +145,8 @@ template <typename R> __device__ __forceinline__ void voigt(R v, R a, R &H, R &F
     cplx<R> r = cdiv(w4, q);
     w = cm<R>(e * rsqrt_unit<R>(R(1) - su * su) - r.re, e * su - r.im);
   }
-  H = w.re * R(1.0 / kSqrtPi);
-  F = w.im * R(1.0 / kSqrtPi);
+  H = w.re;
+  F = w.im;
 }
 
 // Linear short-characteristics weights with the Taylor branch below

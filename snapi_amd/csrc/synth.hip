@@ -31,20 +31,16 @@ __device__ __forceinline__ void synth_body(const KLines &lines, const SynthArgs 
   if (active && !active[col / active_div]) return;  // uniform over the workgroup
   const R *st = state + (size_t)col * args.nfield * nd;
   ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
-  // the grouped wavelength order (lambda_order) in the float path; the double
-  // path, at one workgroup per CU, measured faster in index order
-  int *order = nullptr;
-  if (sizeof(R) == 4 && args.nlambda <= kOrderMax) {
-    order = reinterpret_cast<int *>(smem_raw + ColumnTile<R>::bytes(nd, lines.nline));
-    lambda_order(lines, lam, args.nlambda, reinterpret_cast<float *>(order + args.nlambda), order + 2 * args.nlambda,
-                 order);
-  }
-  stage_column<R, R>(lines, args, st, tile);
+  RayGeo<R> geo = RayGeo<R>::carve(sm + ColumnTile<R>::bytes(nd, lines.nline) / sizeof(R), nd);
+  stage_column<R, R, true>(lines, args, st, tile);
+  stage_ray<R, R>(args, st, geo);
   __syncthreads();
-  for (int i = threadIdx.x; i < args.nlambda; i += blockDim.x) {
-    const int l = order ? order[i] : i;
+  // wavelengths in index order: 64 neighbouring wavelengths of a wavefront
+  // lie in the same W4 regions of most Zeeman components, so the wavefront
+  // executes fewer of the four branches than any regrouping tried
+  for (int l = threadIdx.x; l < args.nlambda; l += blockDim.x) {
     R out[4];
-    int rc = solve_lambda<R>(lines, args, tile, lam[l], out);
+    int rc = solve_lambda<R>(lines, args, tile, geo, lam[l], out);
     R *o = stokes + ((size_t)col * args.nlambda + l) * 4;
     o[0] = out[0];
     o[1] = out[1];
@@ -54,12 +50,13 @@ __device__ __forceinline__ void synth_body(const KLines &lines, const SynthArgs 
   }
 }
 
-// The float kernel is held to eight wavefronts per SIMD (64 VGPRs, 85
-// spilled to scratch), which hides the latency of its LDS reads,
-// transcendentals and branches far better than four at 119 VGPRs; on the
-// 512 x 512 x 71 cube: 4 waves 301 ms, 5 waves 242 ms, 6 waves 210 ms,
-// 8 waves 185 ms.  The double kernel keeps its registers (253 VGPRs; five
-// wavefronts would spill 263 and measured 32 % slower).
+// The float kernel is held to eight wavefronts per SIMD (64 VGPRs, 228
+// bytes per work-item spilled to scratch), which hides the latency of its
+// LDS and kernel-argument reads and of its transcendentals far better than
+// four at 113 VGPRs without spills; on the 512 x 512 x 71 cube: 4 waves
+// 310 ms, 6 waves 214 ms, 7 waves 172 ms, 8 waves 160-166 ms.  The double
+// kernel keeps its registers (239 VGPRs; five wavefronts would spill and
+// measured 32 % slower).
 template <typename R> struct SynthKernel;
 __global__ __launch_bounds__(kSynthThreads, 8) void synth_kernel_f(const KLines lines, const SynthArgs args,
                                                                   const float *__restrict__ state,
@@ -88,7 +85,7 @@ static int launch_synth(const KLines &kl, const SynthArgs &a, const void *state,
                         int *status, hipStream_t stream, const int *active = nullptr, int active_div = 1) {
   int threads = ((a.nlambda + 63) / 64) * 64;
   if (threads > kSynthThreads) threads = kSynthThreads;
-  size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + (sizeof(R) == 4 && a.nlambda <= kOrderMax ? (size_t)a.nlambda * 12 : 0);
+  const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + RayGeo<R>::bytes(a.nd);
   hipLaunchKernelGGL(SynthKernel<R>::fn, dim3(a.ncol), dim3(threads), smem, stream, kl, a,
                      static_cast<const R *>(state), lam, static_cast<R *>(stokes), status, active,
                      active_div > 0 ? active_div : 1);

@@ -53,7 +53,9 @@ template <typename R> struct ColumnTile {
 // the line-strength factors of atom_rad_synthesis.cc:432-433, the
 // magnetic-field geometry of :416-440, H- depth factors of mol.cc:527-541 and
 // 621-627), evaluated in double from the column state.
-template <typename R, typename S>
+// kRaw: the line factors opl and eml carry the 1/sqrt(pi) of H and F, for
+// the opacity build on the unscaled W4 value (voigt_raw).
+template <typename R, typename S, bool kRaw = false>
 __device__ void stage_column(const KLines &L, const SynthArgs &a, const S *__restrict__ st, ColumnTile<R> &t) {
   const int nd = a.nd;
   for (int z = threadIdx.x; z < nd; z += blockDim.x) {
@@ -100,8 +102,42 @@ __device__ void stage_column(const KLines &L, const SynthArgs &a, const S *__res
     t.vcd[i] = (R)(vl / kC * idld);
     t.lbd[i] = (R)(K.lbfac * bm * idld);
     t.a[i] = (R)(K.sf * (K.grad + gc) * idld);
-    t.opl[i] = (R)((nlo * K.opf_l - nup * K.opf_u) * idld * iref);
-    t.eml[i] = (R)(nup * K.emf * idld * iref);
+    const double ws = kRaw ? 1.0 / kSqrtPi : 1.0;
+    t.opl[i] = (R)((nlo * K.opf_l - nup * K.opf_u) * idld * iref * ws);
+    t.eml[i] = (R)(nup * K.emf * idld * iref * ws);
+  }
+}
+
+// Optical-path steps along the ray, index k = position along the ray:
+// h_k = (grid_k - grid_{k-1}) / (-cos theta) for k >= 1, its reciprocal, and
+// the Fritsch-Butland weight al_k = (1 + h_k / (h_k + h_{k+1})) / 3 at the
+// interior points (plpar/atmos_ppbez_rts.cc:961-988).  They depend on the
+// column's grid only, so every wavelength shares them.
+template <typename R> struct RayGeo {
+  R *h, *ih, *al;
+  static __host__ __device__ size_t bytes(int nd) { return sizeof(R) * 3 * (size_t)nd; }
+  static __device__ RayGeo carve(R *p, int nd) { return {p, p + nd, p + 2 * nd}; }
+};
+template <typename R, typename S>
+__device__ void stage_ray(const SynthArgs &a, const S *__restrict__ st, RayGeo<R> &g) {
+  const int nd = a.nd;
+  const bool up = a.ct < 0.0;
+  const double mct = -a.ct;
+  const S *grid = st + SNAPI_F_GRID * nd;
+  auto gr = [&](int k) { return (double)grid[up ? nd - 1 - k : k]; };
+  for (int k = threadIdx.x; k < nd; k += blockDim.x) {
+    if (k == 0) {
+      g.h[0] = g.ih[0] = g.al[0] = R(0);
+      continue;
+    }
+    const double h = (gr(k) - gr(k - 1)) / mct;
+    g.h[k] = (R)h;
+    g.ih[k] = (R)(1.0 / h);
+    g.al[k] = R(0);
+    if (k <= nd - 2) {
+      const double h2 = (gr(k + 1) - gr(k)) / mct;
+      g.al[k] = (R)((1.0 + h / (h + h2)) / 3.0);
+    }
   }
 }
 
@@ -180,7 +216,9 @@ __device__ __forceinline__ void lambda_consts(const KLines &L, const SynthArgs &
 // Opacity (7 independent matrix elements) and emissivity of one depth point,
 // normalised by the referent opacity when the tau grid is used.
 // K = {eta_I, eta_Q, eta_U, eta_V, rho_Q, rho_U, rho_V}.
-template <typename R>
+// kRaw: the tile was staged with kRaw (1/sqrt(pi) in opl and eml) and the
+// unscaled W4 value is accumulated.
+template <typename R, bool kRaw = false>
 __device__ __forceinline__ void opacity_at(const KLines &L, const SynthArgs &a, const ColumnTile<R> &t,
                                            const LamConst<R> &c, int z, R K[7], R E[4]) {
   R KI = t.bgop[z], EI = t.bgem[z];
@@ -205,7 +243,8 @@ __device__ __forceinline__ void opacity_at(const KLines &L, const SynthArgs &a, 
     const R opl = t.opl[i], eml = t.eml[i];
     if (!ln.split) {
       R H, F;
-      voigt<R>(x0, av, H, F);
+      if (kRaw) voigt_raw<R>(x0, av, H, F);
+      else voigt<R>(x0, av, H, F);
       KI += opl * H;
       EI += eml * H;
       continue;
@@ -217,7 +256,8 @@ __device__ __forceinline__ void opacity_at(const KLines &L, const SynthArgs &a, 
     for (int g = 0; g < 3; ++g) {
       for (int m = 0; m < ln.nc[g]; ++m, ++cidx) {
         R H, F;
-        voigt<R>(x0 + (R)L.shift[cidx] * lbd, av, H, F);
+        if (kRaw) voigt_raw<R>(x0 + (R)L.shift[cidx] * lbd, av, H, F);
+        else voigt<R>(x0 + (R)L.shift[cidx] * lbd, av, H, F);
         const R sw = (R)L.str[cidx];
         Hs[g] += H * sw;
         Fs[g] += F * sw;
@@ -254,63 +294,71 @@ template <typename R> __device__ __forceinline__ void reduce_ks(const R K[7], co
 // One wavelength: opacity build streamed along the ray with one point of
 // look-ahead (the Fritsch-Butland derivative at k needs eta_I at k+1), and
 // the linear DELO-type recurrence
-//   (1 + w1 K~_k) I_k = e^{-dtau} I_{k-1} - w0 K~_{k-1} I_{k-1} + w0 S_{k-1} + w1 S_k
-// of plpar/atmos_ppbez_rts.cc:1055-1197.  The last point along the ray is
-// the copy of the one before it (:1201-1205), so the emergent vector is
-// I_{nd-2}.  Returns a snapi_status bit pattern (0 = fine).
+//   (1 + w1 K~_k) I_k = e^{-dtau} I_{k-1} + w0 (S_{k-1} - K~_{k-1} I_{k-1}) + w1 S_k
+// of plpar/atmos_ppbez_rts.cc:1055-1197.  X = S_{k-1} - K~_{k-1} I_{k-1} is
+// formed as soon as I_{k-1} is known, so that the opacity build of the next
+// point holds 4 values of the previous point instead of 14.  The last point
+// along the ray is the copy of the one before it (:1201-1205), so the
+// emergent vector is I_{nd-2}.  The tile is staged with kRaw.  Returns a
+// snapi_status bit pattern (0 = fine).
 template <typename R>
-__device__ int solve_lambda(const KLines &L, const SynthArgs &a, const ColumnTile<R> &t, double lam, R out[4]) {
+__device__ int solve_lambda(const KLines &L, const SynthArgs &a, const ColumnTile<R> &t, const RayGeo<R> &g,
+                            double lam, R out[4]) {
   LamConst<R> c;
   lambda_consts<R>(L, a, lam, c);
   const int nd = a.nd;
   const bool up = a.ct < 0.0;  // outgoing ray: from the bottom (z = nd-1) to the top
-  const R mct = (R)(-a.ct);
   auto zof = [&](int k) { return up ? nd - 1 - k : k; };
-  R K0[7], E0[4], K1[7], E1[4];
-  opacity_at<R>(L, a, t, c, zof(0), K0, E0);
-  opacity_at<R>(L, a, t, c, zof(1), K1, E1);
-  R oom = K0[0], oo0 = K1[0];
-  R gm = t.grid[zof(0)], g0 = t.grid[zof(1)];
-  R h0 = (g0 - gm) / mct;
-  R Dm = (oo0 - oom) / h0;
-  R Ip[4];
-  Ip[0] = up ? E0[0] / K0[0] : R(0);
-  Ip[1] = Ip[2] = Ip[3] = R(0);
-  Kred<R> kp, kc;
-  R Sp[4], Sc[4];
-  reduce_ks<R>(K0, E0, kp, Sp);
-  reduce_ks<R>(K1, E1, kc, Sc);
+  Kred<R> kc;
+  R Sc[4], Ip[4], X[4], oom, oo0;
+  {
+    R K0[7], E0[4], Sp[4], kv[4];
+    Kred<R> kp;
+    opacity_at<R, true>(L, a, t, c, zof(0), K0, E0);
+    oom = K0[0];
+    Ip[0] = up ? E0[0] / K0[0] : R(0);
+    Ip[1] = Ip[2] = Ip[3] = R(0);
+    reduce_ks<R>(K0, E0, kp, Sp);
+    kmul<R>(kp, Ip, kv);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) X[s] = Sp[s] - kv[s];
+  }
+  {
+    R K1[7], E1[4];
+    opacity_at<R, true>(L, a, t, c, zof(1), K1, E1);
+    oo0 = K1[0];
+    reduce_ks<R>(K1, E1, kc, Sc);
+  }
+  R h0 = g.h[1];
+  R Dm = (oo0 - oom) * g.ih[1];  // one-sided at the first point
   int rc = 0;
   for (int k = 1; k <= nd - 2; ++k) {
     R Kn[7], En[4];
-    const int zn = zof(k + 1);
-    opacity_at<R>(L, a, t, c, zn, Kn, En);
+    opacity_at<R, true>(L, a, t, c, zof(k + 1), Kn, En);
     const R oop = Kn[0];
-    const R gp = t.grid[zn];
-    const R hp = (gp - g0) / mct;
-    const R D0 = fb_deriv<R>(oom, oo0, oop, h0, hp);
+    const R d1 = (oo0 - oom) * g.ih[k], d2 = (oop - oo0) * g.ih[k + 1];
+    const R al = g.al[k];
+    const R D0 = d1 * d2 > R(0) ? d1 * d2 * rrcp<R>(al * d1 + (R(1) - al) * d2) : R(0);
     const R C = R(0.5) * ((oo0 - h0 * R(0.5) * D0) + (oom + h0 * R(0.5) * Dm));
     const R tb = h0 * R(1.0 / 3.0) * (oom + oo0 + C);
     if (!(tb < R(1E10))) rc |= 1;
     R etb, w0, w1;
     sc_weights<R>(tb, etb, w0, w1);
-    R kv[4], B[4], In[4];
-    kmul<R>(kp, Ip, kv);
+    R B[4], In[4], kv[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) B[s] = -w0 * kv[s] + w0 * Sp[s] + w1 * Sc[s] + Ip[s] * etb;
+    for (int s = 0; s < 4; ++s) B[s] = w0 * X[s] + w1 * Sc[s] + Ip[s] * etb;
     solve_1pk<R>(kc, w1, B, In);
+    kmul<R>(kc, In, kv);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       Ip[s] = In[s];
-      Sp[s] = Sc[s];
+      X[s] = Sc[s] - kv[s];
     }
-    kp = kc;
     reduce_ks<R>(Kn, En, kc, Sc);
     oom = oo0;
     oo0 = oop;
     Dm = D0;
-    h0 = hp;
-    g0 = gp;
+    h0 = g.h[k + 1];
   }
   out[0] = Ip[0];
   out[1] = Ip[1];

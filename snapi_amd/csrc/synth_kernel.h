@@ -141,44 +141,6 @@ __device__ void stage_ray(const SynthArgs &a, const S *__restrict__ st, RayGeo<R
   }
 }
 
-// Work-item order of the wavelengths: grouped by their nearest line (by
-// distance relative to the line centre), and within a group by that
-// distance (ties by index).  The Voigt function takes one of four branches
-// by the distance from each Zeeman component in Doppler widths, and a
-// wavefront executes every branch one of its work-items needs: consecutive
-// wavelengths straddle a line core and take all of them, while a wavefront
-// of one line's nearest wavelengths meets the other lines only in their far
-// wings, the cheapest branch.  The results do not depend on the order.  key
-// holds nlambda floats and near nlambda ints of LDS, order nlambda ints;
-// ends with a barrier.
-constexpr int kOrderMax = 2048;
-__device__ inline void lambda_order(const KLines &L, const double *__restrict__ lam, int nl, float *key, int *near,
-                                    int *order) {
-  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-    double k = 1e30;
-    int n = 0;
-    for (int i = 0; i < L.nline; ++i) {
-      const double d = fabs(lam[l] - L.l[i].lam0) / L.l[i].lam0;
-      if (d < k) k = d, n = i;
-    }
-    key[l] = (float)k;
-    near[l] = n;
-  }
-  __syncthreads();
-  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-    const float k = key[l];
-    const int n = near[l];
-    int r = 0;
-    for (int j = 0; j < nl; ++j) {
-      const float kj = key[j];
-      const int nj = near[j];
-      r += (nj < n) || (nj == n && (kj < k || (kj == k && j < l)));
-    }
-    order[r] = l;
-  }
-  __syncthreads();
-}
-
 // Wavelength-dependent constants of one work-item.
 template <typename R> struct LamConst {
   double lam;           // lambda

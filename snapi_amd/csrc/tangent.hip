@@ -551,14 +551,6 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
   ThermoTile<R> tht = ThermoTile<R>::carve(
       sm + (ColumnTile<R>::bytes(nd, lines.nline) + DerivTile<R>::bytes(nd, lines.nline, ng, kTh)) / sizeof(R), nd,
       lines.nline);
-  // the synthesis kernel's grouped wavelength order (float path)
-  int *order = nullptr;
-  if (sizeof(R) == 4 && nl <= kOrderMax) {
-    order = reinterpret_cast<int *>(smem_raw + ColumnTile<R>::bytes(nd, lines.nline) +
-                                    DerivTile<R>::bytes(nd, lines.nline, ng, kTh) +
-                                    (kTh ? ThermoTile<R>::bytes(nd, lines.nline) : 0));
-    lambda_order(lines, lam, nl, reinterpret_cast<float *>(order + nl), order + 2 * nl, order);
-  }
   stage_column<R, R>(lines, args, st, tile);
   stage_deriv<R, R>(lines, args, st, dt);
   if constexpr (kTh) stage_thermo<R, R>(lines, args, st, dstate + (size_t)col * 2 * nf * nd, tht);
@@ -599,8 +591,8 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
   const R mct = (R)(-args.ct);
   auto zof = [&](int k) { return up ? nd - 1 - k : k; };
   const R *mk_all = static_cast<const R *>(ra.mask);
-  for (int i = threadIdx.x; i < nl; i += blockDim.x) {
-    const int l = order ? order[i] : i;
+  // wavelengths in index order, as the synthesis kernel runs them
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
     LamConst<R> c;
     lambda_consts<R>(lines, args, lam[l], c);
     R G[NQR][11], u[NQR][4], ds0[NQR];
@@ -776,8 +768,7 @@ int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, 
       th |= ra.pidx[n] < 2 || (ra.nmap2 && ra.pidx2[n] >= 0);
     const int ng = ra.nnode - pass * kTanMaxNodes < kTanMaxNodes ? ra.nnode - pass * kTanMaxNodes : kTanMaxNodes;
     const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + DerivTile<R>::bytes(a.nd, kl.nline, ng, th) +
-                        (th ? ThermoTile<R>::bytes(a.nd, kl.nline) : 0) +
-                        (sizeof(R) == 4 && a.nlambda <= kOrderMax ? (size_t)a.nlambda * 12 : 0);
+                        (th ? ThermoTile<R>::bytes(a.nd, kl.nline) : 0);
     if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;
     const R *st = static_cast<const R *>(ds_), *dd = static_cast<const R *>(dd_);
     R *so = static_cast<R *>(dstk_), *Jo = static_cast<R *>(dJ_);

@@ -124,10 +124,19 @@ def main():
     value = ncol * nlam / (ms * 1e-3)
     log("mixed: %.3f ms/step, %.4g lambda-point solves/s" % (ms, value))
 
-    # algorithmic flops per lambda point on a sample of the cube
+    # algorithmic flops per lambda point: the frozen operation count of this
+    # workload (profiles/synth_flops.json, counted through the FP64
+    # restatement on a 64-column sample of this cube); the operation model of
+    # the kernel (snapi_amd/flops.py) on the same sample is reported beside it
     rng = np.random.default_rng(5)
     samp = rng.choice(ncol, 64, replace=False)
-    fpl = flops.flops_per_lambda_point(table, ray, state32[samp].astype(np.float64), lam)
+    fpl_model = flops.flops_per_lambda_point(table, ray, state32[samp].astype(np.float64), lam)
+    fpl, fpl_src = fpl_model, "model"
+    ff = os.path.join(REPO, "profiles", "synth_flops.json")
+    if os.path.exists(ff):
+        fc = json.load(open(ff)).get("cube512")
+        if fc and (nx, ny, nd, nlam) == (512, 512, fc["depths"], fc["wavelengths"]):
+            fpl, fpl_src = fc["flops_per_lambda_point"], "counted"
     launch_flops = fpl * ncol * nlam
     achieved = launch_flops / (ms * 1e-3) / 1e12
     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs of this
@@ -141,7 +150,8 @@ def main():
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
             "algorithmic_bytes": int(ncol * nf * nd * 4 + ncol * nlam * 16),
-            "kernel": "synth_kernel_f", "flops_per_lambda_point": round(fpl, 1),
+            "kernel": "synth_kernel_f", "flops_per_lambda_point": round(fpl, 1), "flops_source": fpl_src,
+            "flops_per_lambda_point_model": round(fpl_model, 1),
             "ms_per_launch": round(ms, 4)}
 
     fp64 = None

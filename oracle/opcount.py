@@ -1,0 +1,116 @@
+"""Algorithmic operation count of one lambda-point of the polarized
+synthesis -- MEASUREMENT INFRASTRUCTURE ONLY.
+
+Builds oracle/_ref/opcount (oracle/opcount.cc: the oracle's C synthesis
+compiled with an op-counting scalar), runs it on a sample of columns of a
+synthesis workload and writes the per-lambda-point count.  The count is the
+roofline's algorithmic work: the synthesis kernel is timed, the count is
+not re-derived at bench time but read from the table this script writes.
+
+    python oracle/opcount.py            # writes profiles/synth_flops.json
+
+Workloads (the ones the synthesis roofline is quoted on):
+  cube512: the 512 x 512 x 71 cube of bench.py (seed 1, float32 state), 301
+           wavelengths; bench.py's 64-column sample (numpy
+           default_rng(5).choice(262144, 64, replace=False)).
+  cfg0:    the reference's one-column case (its depths and wavelength grid).
+"""
+import json
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+BIN = os.path.join(HERE, "_ref", "opcount")
+
+
+def build():
+    os.makedirs(os.path.dirname(BIN), exist_ok=True)
+    src = os.path.join(HERE, "opcount.cc")
+    deps = [src, os.path.join(HERE, "snapi_oracle.c"), os.path.join(HERE, "snapi_oracle.h")]
+    if os.path.exists(BIN) and all(os.path.getmtime(BIN) >= os.path.getmtime(d) for d in deps):
+        return BIN
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas", "-o", BIN, src],
+                   check=True)
+    return BIN
+
+
+def write_input(path, table, ray, states, lam):
+    states = np.ascontiguousarray(states, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    ncol, nf, nd = states.shape
+    with open(path, "wb") as f:
+        f.write(struct.pack("<9i", table.nline, table.ncomp_total, ncol, nd, nf, lam.size, ray.tau_grid, ray.hminus,
+                            ray.hminus_mid))
+        f.write(struct.pack("<2d", ray.theta, ray.lam_mid))
+        for i in range(table.nline):
+            L = table.line[i]
+            f.write(struct.pack("<8d", L.lam0, L.aul, L.blu, L.bul, L.grad, L.sigma, L.alpha, L.mass))
+            f.write(struct.pack("<5i", L.split, L.ncomp[0], L.ncomp[1], L.ncomp[2], L.comp0))
+        for c in range(table.ncomp_total):
+            f.write(struct.pack("<2d", table.comp[c].shift, table.comp[c].strength))
+        f.write(states.tobytes())
+        f.write(lam.tobytes())
+
+
+def count(table, ray, states, lam):
+    """Per column: (counted operations, raw operations, stokes [nlambda][4])."""
+    exe = build()
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "in.bin")
+        write_input(p, table, ray, states, lam)
+        out = subprocess.run([exe, p], capture_output=True, text=True, check=True).stdout.split("\n")
+    res = []
+    for line in out:
+        if not line.strip():
+            continue
+        w = line.split()
+        st = np.array([float.fromhex(x) for x in w[3:]]).reshape(-1, 4)
+        res.append((int(w[1]), int(w[2]), st))
+    return res
+
+
+def cube512_sample():
+    sys.path.insert(0, REPO)
+    from snapi_amd import workloads
+    t = workloads.template_cfg1()
+    st = workloads.cube_state(t, 512, 512, seed=1, dtype=np.float32)
+    samp = np.random.default_rng(5).choice(st.shape[0], 64, replace=False)
+    return t["table"], t["ray"], st[samp].astype(np.float64), t["lam_vac"]
+
+
+def cfg0_case():
+    sys.path.insert(0, REPO)
+    from snapi_amd import workloads
+    t = workloads.cfg0_column()
+    return t["table"], t["ray"], t["state"][None].astype(np.float64), t["lam_vac"]
+
+
+def main():
+    table = {}
+    for name, make in (("cfg0", cfg0_case), ("cube512", cube512_sample)):
+        tb, ray, st, lam = make()
+        res = count(tb, ray, st, lam)
+        c = np.array([r[0] for r in res], dtype=np.float64)
+        raw = np.array([r[1] for r in res], dtype=np.float64)
+        table[name] = {"flops_per_lambda_point": float(c.mean() / lam.size),
+                       "raw_per_lambda_point": float(raw.mean() / lam.size),
+                       "columns": int(st.shape[0]), "depths": int(st.shape[2]), "wavelengths": int(lam.size),
+                       "per_column_min": float(c.min() / lam.size), "per_column_max": float(c.max() / lam.size),
+                       "counted_per_column": [int(x) for x in c]}
+        print(name, table[name], flush=True)
+    table["method"] = ("oracle/opcount.cc: snapi_oracle.c's orc_synth with an op-counting double; add, sub, mul, "
+                       "div, sqrt, exp, log, log10, sin, cos = 1, pow = 3, FMA = 2; each (operation, operands) "
+                       "counted once per column")
+    table["command"] = "python oracle/opcount.py"
+    with open(os.path.join(REPO, "profiles", "synth_flops.json"), "w") as f:
+        json.dump(table, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -39,6 +39,9 @@
  *   snapi_stokes_to_cube_dev the job's spectra cube                 master/job.cc:590-597
  *   snapi_spectral_convolve* convolve_spectra_with_gauss, convolve_response_with_gauss
  *                                                                   cmlib/mathtools.cc:1350-1427, 1473-1498
+ *   snapi_psf_convolve*      convolve_spectra_with_psf, convolve_response_with_psf
+ *                                                                   cmlib/mathtools.cc:1429-1465, 1500-1522
+ *   snapi_add_scattered_light* observable::add_scattered_light       cmlib/obs.cc:412-428
  *   snapi_ali_accumulate     atom::add (Jbar, Lambda*, norm sums)   atmos/atmol/atom/atom.cc:1865-1927
  *   snapi_ali_newpops        atom::newpops                          atmos/atmol/atom/atom_pop.cc:27-135
  *   snapi_ali_solve*         atmosphere::nltepops (one NLTE atom)   atmos/atmos_pop.cc:67-240
@@ -288,12 +291,41 @@ int snapi_lm_step(int precision, int npix, int nrow, int npar, const void *J, co
 /* Node model of the inversion (the reference's model class, atmos/atmos.h:
  * 377-531): one entry per node, parameter code as the reference orders them
  * (1 T, 2 vt, 3 vLOS, 4 |B|, 5 inclination [rad], 6 azimuth [rad]) and the
- * node's position in log tau_500. */
+ * node's position in log tau_500.
+ * reg_type[p], reg_alpha[p] (p = parameter code, index 0 unused): the
+ * model's regularisation of that parameter (model::temp_reg_type ... and
+ * their alphas, set from REG_TYPE / REG_ALPHA of the parameter's block,
+ * jsub/cfg.cc:374-381): 0 none, 1 departure from zero, 2 Tikhonov
+ * (differences of neighbouring nodes).  The fits add alpha^2 R^T R of each
+ * regularised parameter to J^T J before the damping
+ * (atmosphere::regularize_hessian, atmos/atmos_fit.cc:1371-1480), the
+ * right-hand side unchanged as there.  R covers the parameter's first run
+ * of consecutive nodes; as in the reference, a run that ends the node list
+ * contributes nothing to departure-from-zero and only its first difference
+ * to Tikhonov, and a single Tikhonov node couples to the node after it.  A
+ * parameter without nodes is not regularised; an entry of R beyond the
+ * last node is dropped. */
 typedef struct {
   int nnode;
   int param[SNAPI_MAX_NODES];
   double logtau[SNAPI_MAX_NODES];
+  int reg_type[7];
+  double reg_alpha[7];
 } snapi_node_model;
+
+/* Instrument effects the fits apply (atmosphere::stokes_lm_fit,
+ * atmos/atmos_fit.cc:104-115, and forward_evaluate, :717-737); a null
+ * pointer means none.  broadening [cm] > 0: the Gaussian convolution of
+ * snapi_spectral_convolve; npsf > 0: the sampled PSF of snapi_psf_convolve
+ * (psf in host memory, npsf <= 1024).  As the reference: the current
+ * spectrum and the Jacobian take the Gaussian when broadening > 0 and
+ * otherwise the PSF; every trial spectrum takes the Gaussian, then the
+ * PSF. */
+typedef struct {
+  double broadening;
+  int npsf;
+  const double *psf;
+} snapi_instrument;
 
 enum snapi_jacobian {
   SNAPI_JAC_ADJOINT = 0,     /* reverse sweep contracted with the node responses */
@@ -672,6 +704,38 @@ int snapi_spectral_convolve_dev(int precision, int nseries, int nlambda, int nco
                                 const double *lam_dev, double width, void *data_dev, void *stream);
 
 /*
+ * Convolution with a sampled spectral PSF (convolve_spectra_with_psf,
+ * cmlib/mathtools.cc:1429-1465), as the fit applies it to the spectrum and
+ * to the node Jacobian (convolve_response_with_psf, :1500-1522;
+ * atmos/atmos_fit.cc:111-115, 730-734): the npsf samples psf[0..npsf-1],
+ * on the wavelength grid's step, centred on each point,
+ *   out[l] = sum_j in[l - (npsf-1)/2 + j] psf[j] / sum(psf),
+ * over the samples that fall on the grid (no renormalisation at the ends),
+ * in double.  Deviation: the reference reads its 1-based PSF array from
+ * index 0 while normalising over 1..n, and near the first wavelength it
+ * anchors the PSF at the clipped start instead of centring it; here every
+ * point takes the n given samples centred on it.  In the interior the two
+ * agree when the reference's samples are the given ones shifted by one.
+ *   data [nseries][nlambda][ncomp] in place (double for SNAPI_FP64, float
+ *   for SNAPI_MIXED), as for snapi_spectral_convolve; npsf 1..1024.
+ */
+int snapi_psf_convolve(int precision, int nseries, int nlambda, int ncomp, int npsf, const double *psf, void *data);
+/* The same on device arrays (psf_dev in device memory). */
+int snapi_psf_convolve_dev(int precision, int nseries, int nlambda, int ncomp, int npsf, const double *psf_dev,
+                           void *data_dev, void *stream);
+
+/*
+ * Scattered light of observable::add_scattered_light (cmlib/obs.cc:412-428):
+ * Stokes I of every wavelength raised by fraction * continuum_level.
+ *   stokes [nseries][nlambda][4] in place (double for SNAPI_FP64, float for
+ *   SNAPI_MIXED).
+ */
+int snapi_add_scattered_light(int precision, int nseries, int nlambda, double fraction, double continuum_level,
+                              void *stokes);
+int snapi_add_scattered_light_dev(int precision, int nseries, int nlambda, double fraction, double continuum_level,
+                                  void *stokes_dev, void *stream);
+
+/*
  * Cube I/O.  ANA f0 files, uncompressed (ana_fzread / ana_fzwrite,
  * cmlib/ana_read.cc:74-160, ana_write.cc:120-171): header blocks of 512
  * bytes, dims[0] fastest; data of type 0 int8, 1 int16, 2 int32, 3 float32,
@@ -817,24 +881,23 @@ int snapi_eos_state_dev(const snapi_chem_table *table, const snapi_state_map *ma
  * snapi_node_jacobian_eos; SNAPI_JAC_DIFFERENCES differences the synthesis
  * path's spectrum over node sets each rebuilt through hydrostatic
  * equilibrium and the equation of state (the FP64 baseline; refused with
- * broadening > 0).  broadening > 0: the Gaussian spectral broadening [cm] the
- * fit applies to the current spectrum and Jacobian (before the Stokes weights
- * and the mask) and to every trial spectrum (snapi_spectral_convolve;
- * atmos_fit.cc:104-110, 717-737); 0: none.
+ * instrument effects).  inst (or null): the instrument effects of
+ * snapi_instrument, applied to the current spectrum and Jacobian before the
+ * Stokes weights and the mask, and to every trial spectrum.
  */
 int snapi_invert_eos(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
                      const snapi_node_model *model, const snapi_chem_table *chem, const snapi_chem_table *chem_he,
                      const snapi_state_map *map, int npix, int nd, const void *state, int nlambda,
                      const double *lam, const void *obs, const void *mask, const double *wstokes,
-                     double broadening, double lm_start, int max_iter, int nslot, double *nodes, double *chi2,
-                     int *iterations, long long *pixel_iterations);
+                     const snapi_instrument *inst, double lm_start, int max_iter, int nslot, double *nodes,
+                     double *chi2, int *iterations, long long *pixel_iterations);
 int snapi_invert_eos_dev(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
                          const snapi_node_model *model, const snapi_chem_table *chem,
                          const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
                          const void *state_dev, int nlambda, const double *lam_dev, const void *obs_dev,
-                         const void *mask_dev, const double *wstokes, double broadening, double lm_start,
-                         int max_iter, int nslot, const double *nodes0_dev, double *nodes_dev, double *chi2_dev,
-                         int *iterations_dev, long long *pixel_iterations, void *stream);
+                         const void *mask_dev, const double *wstokes, const snapi_instrument *inst,
+                         double lm_start, int max_iter, int nslot, const double *nodes0_dev, double *nodes_dev,
+                         double *chi2_dev, int *iterations_dev, long long *pixel_iterations, void *stream);
 
 /*
  * atmos_pp::build_from_nodes (atmos/plpar/atmos_pp.cc:58-180) followed by

@@ -45,6 +45,15 @@
 //          node responses dq_p(z)/dnode.
 //   time_synth <atmos.cfg> <lambda.txt> <el> <az> <reps>
 //          wall time of obs_stokes per call on the column.
+//   psf    <nl> <npsf> <seed>
+//          convolve_spectra_with_psf (cmlib/mathtools.cc:1429-1465) on seeded
+//          spectra and a seeded PSF of npsf samples p[0..npsf-1], handed to
+//          the routine as the array it indexes: element 0 = p[0], elements
+//          1..npsf-1 = p[1..npsf-1] and element npsf = p[0], so that its sum
+//          over 1..npsf is the sum of the samples.
+//   reg    <atmos.cfg> <model.txt>
+//          atmosphere::regularize_hessian (atmos/atmos_fit.cc:1371-1480) on a
+//          zero J^T J for the model's regularisation: the matrix it adds.
 //   fit    <atmos.cfg> <lambda.txt> <el> <az> <fit.txt> [last]
 //          atmosphere::stokes_lm_fit (atmos/atmos_fit.cc:18-227) from the start
 //          model of fit.txt on its observation, run with 1, 2, ..., kmax
@@ -722,6 +731,53 @@ static int mode_conv(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+static int mode_psf(int argc, char **argv, io_class &io) {
+  if (argc < 5) return 2;
+  const int nl = atoi(argv[2]), n = atoi(argv[3]);
+  if (nl < 1 || n < 1) return 2;
+  g_state = strtoull(argv[4], 0, 10) * 2654435761ULL + 88172645463325252ULL;
+  fp_t **S = ft2dim(1, 4, 1, nl);
+  for (int l = 1; l <= nl; ++l)
+    for (int s = 1; s <= 4; ++s) S[s][l] = (s == 1 ? 1.0 : 0.0) + 0.3 * (2.0 * urand() - 1.0);
+  fp_t *p = new fp_t[n];
+  for (int j = 0; j < n; ++j) p[j] = 0.1 + urand();
+  fp_t *arr = new fp_t[n + 1];
+  arr[0] = p[0];
+  for (int j = 1; j < n; ++j) arr[j] = p[j];
+  arr[n] = p[0];
+  printf("psf");
+  for (int j = 0; j < n; ++j) printf(" %.17g", p[j]);
+  printf("\n");
+  for (int l = 1; l <= nl; ++l) printf("in %.17g %.17g %.17g %.17g\n", S[1][l], S[2][l], S[3][l], S[4][l]);
+  convolve_spectra_with_psf(S, 0, nl, n, arr);
+  for (int l = 1; l <= nl; ++l) printf("out %.17g %.17g %.17g %.17g\n", S[1][l], S[2][l], S[3][l], S[4][l]);
+  return 0;
+}
+
+static model *read_model(FILE *f);
+
+static int mode_reg(int argc, char **argv, io_class &io) {
+  if (argc < 4) return 2;
+  atmosphere *a = make_atmos(argv[2], io);
+  FILE *f = fopen(argv[3], "r");
+  if (!f) { fprintf(stderr, "cannot open %s\n", argv[3]); return 2; }
+  model *m = read_model(f);
+  fclose(f);
+  const int np = m->get_N_nodes_total();
+  fp_t **JTJ = ft2dim(1, np, 1, np);
+  memset(JTJ[1] + 1, 0, np * np * sizeof(fp_t));
+  fp_t *rhs = new fp_t[np] - 1;
+  memset(rhs + 1, 0, np * sizeof(fp_t));
+  a->regularize_hessian(JTJ, rhs, m);
+  printf("np %d\n", np);
+  for (int i = 1; i <= np; ++i) {
+    printf("G %d", i);
+    for (int j = 1; j <= np; ++j) printf(" %.17g", JTJ[i][j]);
+    printf(" rhs %.17g\n", rhs[i]);
+  }
+  return 0;
+}
+
 // ANA f0 files as the reference writes them (ana_fzwrite, cmlib/
 // ana_write.cc:120-171, through write_file, cmlib/fileio.cc:273-330):
 //   ana <out_prefix>
@@ -995,7 +1051,9 @@ static int mode_nlteresp(int argc, char **argv, io_class &io) {
 // Node model from a text stream: "tau_min tau_max", the node count, then one
 // "code logtau value" line per node (code 1 T, 2 vt, 3 vLOS, 4 |B|,
 // 5 inclination [rad], 6 azimuth [rad]), grouped in that order as the
-// reference's model holds them (atmos/atmos.h:377-531).  Regularisation off.
+// reference's model holds them (atmos/atmos.h:377-531), then the count of
+// regularisation entries and one "code type alpha" line each (the model's
+// *_reg_type and *_reg_alpha; the rest off).
 static model *read_model(FILE *f) {
   double tmin, tmax;
   int n;
@@ -1033,12 +1091,26 @@ static model *read_model(FILE *f) {
     delete[](t + 1);
     delete[](v + 1);
   }
-  m->set_temp_reg(0, 0.0);
-  m->set_vt_reg(0, 0.0);
-  m->set_vs_reg(0, 0.0);
-  m->set_B_reg(0, 0.0);
-  m->set_theta_reg(0, 0.0);
-  m->set_phi_reg(0, 0.0);
+  int rtype[7] = {0, 0, 0, 0, 0, 0, 0};
+  double ralpha[7] = {0, 0, 0, 0, 0, 0, 0};
+  int nreg = 0;
+  if (fscanf(f, "%d", &nreg) != 1 || nreg < 0 || nreg > 6) { fprintf(stderr, "bad regularisation count\n"); exit(2); }
+  for (int i = 0; i < nreg; ++i) {
+    int pc, t;
+    double al;
+    if (fscanf(f, "%d %d %lf", &pc, &t, &al) != 3 || pc < 1 || pc > 6) {
+      fprintf(stderr, "bad regularisation entry\n");
+      exit(2);
+    }
+    rtype[pc] = t;
+    ralpha[pc] = al;
+  }
+  m->set_temp_reg(rtype[1], ralpha[1]);
+  m->set_vt_reg(rtype[2], ralpha[2]);
+  m->set_vs_reg(rtype[3], ralpha[3]);
+  m->set_B_reg(rtype[4], ralpha[4]);
+  m->set_theta_reg(rtype[5], ralpha[5]);
+  m->set_phi_reg(rtype[6], ralpha[6]);
   delete[] code;
   delete[] lt;
   delete[] val;
@@ -1209,6 +1281,8 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "nlteresp")) return mode_nlteresp(argc, argv, io);
   if (!strcmp(argv[1], "nodes")) return mode_nodes(argc, argv, io);
   if (!strcmp(argv[1], "fit")) return mode_fit(argc, argv, io);
+  if (!strcmp(argv[1], "psf")) return mode_psf(argc, argv, io);
+  if (!strcmp(argv[1], "reg")) return mode_reg(argc, argv, io);
   if (!strcmp(argv[1], "time_synth")) return mode_time_synth(argc, argv, io);
   fprintf(stderr, "unknown mode %s\n", argv[1]);
   return 2;

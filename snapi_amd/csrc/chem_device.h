@@ -41,5 +41,9 @@ int chem_eos(const ChemDevice &cd, long ncol, int nd, double *state, double *res
 // are skipped.
 int spectral_convolve_launch(int precision, int nseries, int nl, int ncomp, const double *lam_h,
                              const double *lam_d, double width, void *data, const int *active, hipStream_t st);
+// Sampled-PSF convolution of the same series (instrument.hip); psf_d the
+// npsf samples on the device.
+int psf_convolve_launch(int precision, int nseries, int nl, int ncomp, int npsf, const double *psf_d, void *data,
+                        const int *active, hipStream_t st);
 
 }  // namespace snapi

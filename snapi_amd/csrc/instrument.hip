@@ -2,7 +2,11 @@
 // Gaussian spectral convolution the fit applies to both
 // (convolve_spectra_with_gauss and convolve_response_with_gauss,
 // cmlib/mathtools.cc:1350-1427, 1473-1498; atmosphere::stokes_lm_fit,
-// atmos/atmos_fit.cc:104-110).
+// atmos/atmos_fit.cc:104-110), the convolution with a sampled spectral PSF
+// (convolve_spectra_with_psf and convolve_response_with_psf,
+// mathtools.cc:1429-1465, 1500-1522; atmos_fit.cc:111-115, 730-734) and the
+// scattered-light offset of observable::add_scattered_light
+// (cmlib/obs.cc:412-428).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -143,9 +147,88 @@ int check_args(int precision, int nseries, int nl, int ncomp, const double *lam,
   return SNAPI_OK;
 }
 
+// Sampled-PSF convolution.  One workgroup per (series, group of up to kc
+// components): the series' components are staged in LDS in double, then
+//   out[l] = sum_j in[l - h + j] psf[j] / norm,  h = (npsf - 1) / 2,
+// over the samples that fall on the grid, norm = sum_j psf[j] (the full
+// sum, also at the ends).  Terms are summed in j order as (in psf) / norm,
+// the order of the reference's loop.  data [nseries][nl][ncomp].
+constexpr int kPsfMax = 1024;
+template <typename R>
+__global__ __launch_bounds__(kConvThreads) void psf_convolve_kernel(int nl, int ncomp, int kc, int npsf,
+                                                                    const double *__restrict__ psf,
+                                                                    R *__restrict__ data,
+                                                                    const int *__restrict__ active) {
+  if (active && !active[blockIdx.x]) return;  // uniform over the workgroup
+  extern __shared__ double sm[];
+  double *p = sm;                // [npsf]
+  double *src = p + npsf;        // [nl][kc]
+  __shared__ double norm;
+  const int c0 = blockIdx.y * kc, nc = min(kc, ncomp - c0);
+  R *d = data + (size_t)blockIdx.x * nl * ncomp;
+  for (int j = threadIdx.x; j < npsf; j += blockDim.x) p[j] = psf[j];
+  for (int i = threadIdx.x; i < nl * kc; i += blockDim.x) {
+    const int l = i / kc, c = i % kc;
+    src[i] = c < nc ? (double)d[(size_t)l * ncomp + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int j = 0; j < npsf; ++j) s += p[j];
+    norm = s;
+  }
+  __syncthreads();
+  const int h = (npsf - 1) / 2;
+  const double nrm = norm;
+  for (int i = threadIdx.x; i < nl * kc; i += blockDim.x) {
+    const int l = i / kc, c = i % kc;
+    if (c >= nc) continue;
+    const int j0 = max(0, h - l), j1 = min(npsf - 1, nl - 1 - l + h);
+    double acc = 0.0;
+    for (int j = j0; j <= j1; ++j) acc += src[(l - h + j) * kc + c] * p[j] / nrm;
+    d[(size_t)l * ncomp + c0 + c] = (R)acc;
+  }
+}
+
+template <typename R>
+int launch_psf(int nseries, int nl, int ncomp, int npsf, const double *psf_d, R *data, hipStream_t st,
+               const int *active = nullptr) {
+  int kc = 16;
+  while (kc > 1 && ((size_t)npsf + (size_t)nl * kc) * sizeof(double) > 48 * 1024) kc /= 2;
+  const size_t lds = ((size_t)npsf + (size_t)nl * kc) * sizeof(double);
+  if (lds > 48 * 1024) return SNAPI_ERR_UNSUPPORTED;
+  const int ng = (ncomp + kc - 1) / kc;
+  hipLaunchKernelGGL(psf_convolve_kernel<R>, dim3(nseries, ng), dim3(kConvThreads), lds, st, nl, ncomp, kc, npsf,
+                     psf_d, data, active);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+int check_psf_args(int precision, int nseries, int nl, int ncomp, int npsf, const double *psf) {
+  if ((precision != SNAPI_FP64 && precision != SNAPI_MIXED) || nseries <= 0 || nl <= 0 || ncomp <= 0 || npsf <= 0 ||
+      npsf > kPsfMax || !psf)
+    return SNAPI_ERR_ARG;
+  return SNAPI_OK;
+}
+
+// S_I += fraction * continuum_level on every wavelength of every series.
+template <typename R>
+__global__ void scattered_light_kernel(size_t n, R add, R *__restrict__ stokes) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    stokes[i * 4] += add;
+}
+
 }  // namespace
 
 namespace snapi {
+
+// The PSF convolution for callers that hold the data and the PSF on the
+// device already (the inversion); series with active[series] == 0 are left
+// alone.  npsf in 1..1024.
+int psf_convolve_launch(int precision, int nseries, int nl, int ncomp, int npsf, const double *psf_d, void *data,
+                        const int *active, hipStream_t st) {
+  return precision == SNAPI_FP64 ? launch_psf<double>(nseries, nl, ncomp, npsf, psf_d, (double *)data, st, active)
+                                 : launch_psf<float>(nseries, nl, ncomp, npsf, psf_d, (float *)data, st, active);
+}
 
 // The convolution for callers that hold the data already (the inversion):
 // series with active[series] == 0 are left alone.
@@ -186,6 +269,67 @@ extern "C" int snapi_spectral_convolve(int precision, int nseries, int nlambda, 
   if (!rc && (hipDeviceSynchronize() != hipSuccess || hipMemcpy(data, dd, n, hipMemcpyDeviceToHost) != hipSuccess))
     rc = SNAPI_ERR_DEVICE;
   hipFree(dl);
-  hipFree(dd);
+  (void)hipFree(dd);
+  return rc;
+}
+
+extern "C" int snapi_psf_convolve_dev(int precision, int nseries, int nlambda, int ncomp, int npsf,
+                                      const double *psf_dev, void *data_dev, void *stream) {
+  int rc = check_psf_args(precision, nseries, nlambda, ncomp, npsf, psf_dev);
+  if (rc) return rc;
+  if (!data_dev) return SNAPI_ERR_ARG;
+  return snapi::psf_convolve_launch(precision, nseries, nlambda, ncomp, npsf, psf_dev, data_dev, nullptr,
+                                    (hipStream_t)stream);
+}
+
+extern "C" int snapi_psf_convolve(int precision, int nseries, int nlambda, int ncomp, int npsf, const double *psf,
+                                  void *data) {
+  int rc = check_psf_args(precision, nseries, nlambda, ncomp, npsf, psf);
+  if (rc) return rc;
+  if (!data) return SNAPI_ERR_ARG;
+  const size_t es = precision == SNAPI_FP64 ? 8 : 4, n = (size_t)nseries * nlambda * ncomp * es;
+  double *dp = nullptr;
+  void *dd = nullptr;
+  if (hipMalloc((void **)&dp, (size_t)npsf * 8) != hipSuccess || hipMalloc(&dd, n) != hipSuccess ||
+      hipMemcpy(dp, psf, (size_t)npsf * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dd, data, n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = SNAPI_ERR_DEVICE;
+  if (!rc) rc = snapi_psf_convolve_dev(precision, nseries, nlambda, ncomp, npsf, dp, dd, nullptr);
+  if (!rc && (hipDeviceSynchronize() != hipSuccess || hipMemcpy(data, dd, n, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = SNAPI_ERR_DEVICE;
+  (void)hipFree(dp);
+  (void)hipFree(dd);
+  return rc;
+}
+
+extern "C" int snapi_add_scattered_light_dev(int precision, int nseries, int nlambda, double fraction,
+                                             double continuum_level, void *stokes_dev, void *stream) {
+  if ((precision != SNAPI_FP64 && precision != SNAPI_MIXED) || nseries <= 0 || nlambda <= 0 || !stokes_dev)
+    return SNAPI_ERR_ARG;
+  const size_t n = (size_t)nseries * nlambda;
+  const double add = fraction * continuum_level;
+  const unsigned nb = (unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+  if (precision == SNAPI_FP64)
+    hipLaunchKernelGGL(scattered_light_kernel<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, n, add,
+                       (double *)stokes_dev);
+  else
+    hipLaunchKernelGGL(scattered_light_kernel<float>, dim3(nb), dim3(256), 0, (hipStream_t)stream, n, (float)add,
+                       (float *)stokes_dev);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
+extern "C" int snapi_add_scattered_light(int precision, int nseries, int nlambda, double fraction,
+                                         double continuum_level, void *stokes) {
+  if ((precision != SNAPI_FP64 && precision != SNAPI_MIXED) || nseries <= 0 || nlambda <= 0 || !stokes)
+    return SNAPI_ERR_ARG;
+  const size_t n = (size_t)nseries * nlambda * 4 * (precision == SNAPI_FP64 ? 8 : 4);
+  void *dd = nullptr;
+  int rc = SNAPI_OK;
+  if (hipMalloc(&dd, n) != hipSuccess || hipMemcpy(dd, stokes, n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = SNAPI_ERR_DEVICE;
+  if (!rc) rc = snapi_add_scattered_light_dev(precision, nseries, nlambda, fraction, continuum_level, dd, nullptr);
+  if (!rc && (hipDeviceSynchronize() != hipSuccess || hipMemcpy(stokes, dd, n, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = SNAPI_ERR_DEVICE;
+  (void)hipFree(dd);
   return rc;
 }

@@ -644,7 +644,9 @@ __global__ void convert_kernel(size_t n, const A *__restrict__ a, B *__restrict_
 struct ThermoArgs {
   const snapi_chem_table *chem, *chem_he;
   const snapi_state_map *map;
-  double broadening;  // Gaussian spectral broadening width [cm], 0: none
+  double broadening;   // Gaussian spectral broadening width [cm], 0: none
+  int npsf;            // spectral PSF samples, 0: none
+  const double *psf;   // [npsf], host memory
 };
 
 // mask[slot][l] applied to J[slot][l*4+s][n] after the convolution.
@@ -662,7 +664,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
                        const NodeArgs &na, int npix, int nd, const void *pstate_h, int nlambda, const double *lam_h,
                        const void *obs_h, const void *mask_h, const double *ws, double lm0, int max_iter, int nslot,
                        const double *nodes0, double *nodes_out, double *chi2_out, int *iter_out, long long *stats,
-                       bool dev, hipStream_t st, const ThermoArgs *th = nullptr) {
+                       bool dev, hipStream_t st, const ThermoArgs *th = nullptr, const double *reg_h = nullptr) {
   KLines kl;
   SynthArgs a;
   int rc = prepare_synth(lines, ray, 1, nd, nlambda, kl, a);
@@ -676,6 +678,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   double *dlam = nullptr, *pnodes0 = nullptr, *snodes = nullptr, *tnodes = nullptr, *JTJ = nullptr, *rhs = nullptr;
   double *corr = nullptr, *onodes = nullptr, *ochi2 = nullptr, *fdn = nullptr, *hv = nullptr, *dbl = nullptr;
   int *ints = nullptr, *queue = nullptr, *oiter = nullptr, *count = nullptr;
+  double *dreg = nullptr, *dpsf = nullptr;  // regularisation [nn][nn], spectral PSF
   // temperature nodes: profiles and densities of the rebuilds, double states
   // for the equation of state, the response path's state and derivatives
   ChemDevice cdev, cdev_he;
@@ -737,6 +740,10 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   HIPCHK(hipMalloc((void **)&snodes, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&tnodes, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&JTJ, (size_t)nslot * nn * nn * 8));
+  if (reg_h) {
+    HIPCHK(hipMalloc((void **)&dreg, (size_t)nn * nn * 8));
+    HIPCHK(hipMemcpy(dreg, reg_h, (size_t)nn * nn * 8, hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMalloc((void **)&rhs, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&corr, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&dbl, (size_t)nslot * 6 * 8));
@@ -759,7 +766,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     // every node set; with spectral broadening only the response-path
     // Jacobians are built
     const bool jac_ok = jacobian == SNAPI_JAC_ADJOINT || jacobian == SNAPI_JAC_TANGENT ||
-                        (jacobian == SNAPI_JAC_DIFFERENCES && !(th->broadening > 0.0));
+                        (jacobian == SNAPI_JAC_DIFFERENCES && !(th->broadening > 0.0) && th->npsf == 0);
     if (!jac_ok || nf != SNAPI_NF_BASE + 2 * th->map->nline) {
       rc = !jac_ok ? SNAPI_ERR_UNSUPPORTED : SNAPI_ERR_ARG;
       goto done;
@@ -783,6 +790,10 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
       HIPCHK(hipMalloc((void **)&dTd, (size_t)nslot * ntn * nd * 8));
       HIPCHK(hipMalloc((void **)&dNd, (size_t)nslot * ntn * nd * 8));
       HIPCHK(hipMalloc((void **)&hscr, chem_he_tangent_scratch(nslot, nd)));
+    }
+    if (th->npsf > 0) {
+      HIPCHK(hipMalloc((void **)&dpsf, (size_t)th->npsf * 8));
+      HIPCHK(hipMemcpy(dpsf, th->psf, (size_t)th->npsf * 8, hipMemcpyHostToDevice));
     }
     if (th->broadening > 0.0) {
       lam_host.resize(nl);
@@ -856,9 +867,9 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     }
     ra.nmap = nmap;
     ra.nmap2 = th ? nmap2 : nullptr;
-    // with spectral broadening the Jacobian is convolved before the mask is
-    // applied (convolve_response_with_gauss, then scale_rf)
-    ra.mask = (th && th->broadening > 0.0) ? nullptr : smask;
+    // with instrument effects the Jacobian is convolved before the mask is
+    // applied (convolve_response_with_gauss / _psf, then scale_rf)
+    ra.mask = (th && (th->broadening > 0.0 || th->npsf > 0)) ? nullptr : smask;
     ra.active = ss.needj;
     long long pixel_iters = 0;
     for (int it = 0;; ++it) {
@@ -941,12 +952,19 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
         else
           rc = launch_resp<R>(kl, a_resp, ra, nslot, rstate, rdst, dlam, S, J, scr, chunk, st);
         if (rc) goto done;
+        // the Gaussian, or else the PSF, on the current spectrum and the
+        // Jacobian (atmos_fit.cc:104-115)
         if (th->broadening > 0.0) {
           rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, S, ss.needj,
                                         st);
           if (!rc)
             rc = spectral_convolve_launch(precision, nslot, nl, 4 * nn, lam_host.data(), dlam, th->broadening, J,
                                           ss.needj, st);
+          if (rc) goto done;
+          hipLaunchKernelGGL(mask_jacobian_kernel<R>, dim3(nslot), dim3(256), 0, st, nl, 4 * nn, smask, J, ss.needj);
+        } else if (th->npsf > 0) {
+          rc = psf_convolve_launch(precision, nslot, nl, 4, th->npsf, dpsf, S, ss.needj, st);
+          if (!rc) rc = psf_convolve_launch(precision, nslot, nl, 4 * nn, th->npsf, dpsf, J, ss.needj, st);
           if (rc) goto done;
           hipLaunchKernelGGL(mask_jacobian_kernel<R>, dim3(nslot), dim3(256), 0, st, nl, 4 * nn, smask, J, ss.needj);
         }
@@ -978,7 +996,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
                          ws[2], ws[3], r, ss.chi2, ss.needj);
       // normal equations and damped solve
       launch_lm_normal<R>(nslot, nl * 4, nn, J, r, JTJ, rhs, nullptr, ss.active, st);
-      launch_lm_solve(nslot, nn, JTJ, rhs, ss.lm, corr, ss.active, st);
+      launch_lm_solve(nslot, nn, JTJ, rhs, ss.lm, corr, ss.active, st, dreg);
       // trial model and its synthesis
       hipLaunchKernelGGL(trial_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, corr, tnodes,
                          ss.active);
@@ -999,9 +1017,13 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
       }
       rc = synth_dev_masked(precision, lines, th ? &ray_syn : ray, nslot, nd, tstate, nl, dlam, St, ss.active, 1, st);
       if (rc) goto done;
-      if (th && th->broadening > 0.0) {  // forward_evaluate (atmos_fit.cc:717-737)
+      if (th && th->broadening > 0.0) {  // forward_evaluate (atmos_fit.cc:717-737): Gaussian, then PSF
         rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, St, ss.active,
                                       st);
+        if (rc) goto done;
+      }
+      if (th && th->npsf > 0) {
+        rc = psf_convolve_launch(precision, nslot, nl, 4, th->npsf, dpsf, St, ss.active, st);
         if (rc) goto done;
       }
       hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, St, sobs, smask, ws[0], ws[1],
@@ -1037,6 +1059,7 @@ done:
   hipFree(Tv); hipFree(ltv); hipFree(Ntv); hipFree(sd); hipFree(rsd); hipFree(dsd); hipFree(tsd); hipFree(escr);
   hipFree(dTd); hipFree(dNd); hipFree(hscr);
   hipFree(rstate); hipFree(rdst); hipFree(nmap2);
+  hipFree(dreg); hipFree(dpsf);
   return rc;
 }
 
@@ -1283,6 +1306,71 @@ static int setup_nodes(const snapi_line_table *lines, const snapi_ray *ray, cons
     na.scale[n] = c == NP_T ? 5000.0 : c == NP_VT ? 1E5 : c == NP_VS ? 1E5 : c == NP_B ? 1000.0 : 1.0;
     na.step[n] = c == NP_T ? kNodeStepT : (c == NP_VT || c == NP_VS) ? 20.0 : c == NP_B ? 0.01 : 0.001;
   }
+  for (int p = NP_T; p <= NP_PHI; ++p)
+    if (model->reg_type[p] < 0 || model->reg_type[p] > 2 || !std::isfinite(model->reg_alpha[p])) return SNAPI_ERR_ARG;
+  return SNAPI_OK;
+}
+
+// The regularisation of atmosphere::regularize_hessian / regularize_parameter
+// (atmos/atmos_fit.cc:1371-1480): for each parameter code p with a
+// regularisation type, R over the parameter's first run of consecutive
+// nodes [from, to] (to is the node before the first node of another
+// parameter after from; none when the run reaches the end of the list, as
+// the reference's to stays 0 there), departure from zero R[i][i] = 1 on
+// the run, Tikhonov R[i][i] = 1, R[i][i-1] = -1 for i in (from, to] and
+// R[from][from] = -1, R[from][from+1] = 1; G += alpha^2 R^T R.  G [nn][nn]
+// row-major; returns false when no parameter is regularised.
+static bool reg_matrix(const snapi_node_model *model, const NodeArgs &na, double *G) {
+  const int n = na.nnode;
+  bool on = false;
+  for (int i = 0; i < n * n; ++i) G[i] = 0.0;
+  for (int p = NP_T; p <= NP_PHI; ++p) {
+    const int type = model->reg_type[p];
+    if (!type) continue;
+    int from = -1;
+    for (int i = 0; i < n; ++i)
+      if (na.code[i] == p) {
+        from = i;
+        break;
+      }
+    if (from < 0) continue;  // no node of this parameter
+    int to = -1;
+    for (int i = from; i < n; ++i)
+      if (na.code[i] != p) {
+        to = i - 1;
+        break;
+      }
+    double R[SNAPI_MAX_NODES][SNAPI_MAX_NODES] = {};
+    if (type == 2) {
+      for (int i = from + 1; i <= to; ++i) {
+        R[i][i] = 1.0;
+        R[i][i - 1] = -1.0;
+      }
+      R[from][from] = -1.0;
+      if (from + 1 < n) R[from][from + 1] = 1.0;
+    } else {
+      for (int i = from; i <= to; ++i) R[i][i] = 1.0;
+    }
+    const double a2 = model->reg_alpha[p] * model->reg_alpha[p];
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        double rr = 0.0;
+        for (int k = 0; k < n; ++k) rr += R[k][i] * R[k][j];
+        G[i * n + j] += rr * a2;
+      }
+    on = true;
+  }
+  return on;
+}
+
+// Checks the fit's instrument effects (null: none) into th.
+static int instrument_args(const snapi_instrument *inst, ThermoArgs &th) {
+  if (!inst) return SNAPI_OK;
+  if (!(inst->broadening >= 0.0) || inst->npsf < 0 || inst->npsf > 1024 || (inst->npsf > 0 && !inst->psf))
+    return SNAPI_ERR_ARG;
+  th.broadening = inst->broadening;
+  th.npsf = inst->npsf;
+  th.psf = inst->psf;
   return SNAPI_OK;
 }
 
@@ -1296,12 +1384,16 @@ extern "C" int snapi_invert(int precision, int jacobian, const snapi_line_table 
   int rc = setup_nodes(lines, ray, model, npix, nd, nlambda, jacobian, max_iter, na);
   if (rc) return rc;
   if (nslot <= 0 || nslot > npix) nslot = npix;
+  double G[SNAPI_MAX_NODES * SNAPI_MAX_NODES];
+  const double *reg = reg_matrix(model, na, G) ? G : nullptr;
   if (precision == SNAPI_FP64)
     return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
-                               lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0);
+                               lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0,
+                               nullptr, reg);
   if (precision == SNAPI_MIXED)
     return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
-                              lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0);
+                              lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0,
+                              nullptr, reg);
   return SNAPI_ERR_ARG;
 }
 
@@ -1318,14 +1410,16 @@ extern "C" int snapi_invert_dev(int precision, int jacobian, const snapi_line_ta
   if (rc) return rc;
   if (nslot <= 0 || nslot > npix) nslot = npix;
   hipStream_t st = (hipStream_t)stream;
+  double G[SNAPI_MAX_NODES * SNAPI_MAX_NODES];
+  const double *reg = reg_matrix(model, na, G) ? G : nullptr;
   if (precision == SNAPI_FP64)
     return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
                                mask_dev, wstokes, lm_start, max_iter, nslot, nodes0_dev, nodes_dev, chi2_dev,
-                               iterations_dev, pixel_iterations, true, st);
+                               iterations_dev, pixel_iterations, true, st, nullptr, reg);
   if (precision == SNAPI_MIXED)
     return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
                               mask_dev, wstokes, lm_start, max_iter, nslot, nodes0_dev, nodes_dev, chi2_dev,
-                              iterations_dev, pixel_iterations, true, st);
+                              iterations_dev, pixel_iterations, true, st, nullptr, reg);
   return SNAPI_ERR_ARG;
 }
 
@@ -1372,9 +1466,10 @@ extern "C" int snapi_invert_eos_dev(int precision, int jacobian, const snapi_lin
                                     const snapi_chem_table *chem, const snapi_chem_table *chem_he,
                                     const snapi_state_map *map, int npix, int nd, const void *state_dev, int nlambda,
                                     const double *lam_dev, const void *obs_dev, const void *mask_dev,
-                                    const double *wstokes, double broadening, double lm_start, int max_iter,
-                                    int nslot, const double *nodes0_dev, double *nodes_dev, double *chi2_dev,
-                                    int *iterations_dev, long long *pixel_iterations, void *stream) {
+                                    const double *wstokes, const snapi_instrument *inst, double lm_start,
+                                    int max_iter, int nslot, const double *nodes0_dev, double *nodes_dev,
+                                    double *chi2_dev, int *iterations_dev, long long *pixel_iterations,
+                                    void *stream) {
   if (!state_dev || !lam_dev || !obs_dev || !wstokes || !nodes0_dev || !nodes_dev || !chi2_dev || !iterations_dev ||
       !chem || !chem_he || !map)
     return SNAPI_ERR_ARG;
@@ -1385,17 +1480,20 @@ extern "C" int snapi_invert_eos_dev(int precision, int jacobian, const snapi_lin
   for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
   if (ntn < 1) return SNAPI_ERR_ARG;
   if (nslot <= 0 || nslot > npix) nslot = npix;
-  if (!(broadening >= 0.0)) return SNAPI_ERR_ARG;
-  const ThermoArgs th{chem, chem_he, map, broadening};
+  ThermoArgs th{chem, chem_he, map, 0.0, 0, nullptr};
+  double G[SNAPI_MAX_NODES * SNAPI_MAX_NODES];
+  const double *reg = reg_matrix(model, na, G) ? G : nullptr;
+  rc = instrument_args(inst, th);
+  if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   if (precision == SNAPI_FP64)
     return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
                                mask_dev, wstokes, lm_start, max_iter, nslot, nodes0_dev, nodes_dev, chi2_dev,
-                               iterations_dev, pixel_iterations, true, st, &th);
+                               iterations_dev, pixel_iterations, true, st, &th, reg);
   if (precision == SNAPI_MIXED)
     return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state_dev, nlambda, lam_dev, obs_dev,
                               mask_dev, wstokes, lm_start, max_iter, nslot, nodes0_dev, nodes_dev, chi2_dev,
-                              iterations_dev, pixel_iterations, true, st, &th);
+                              iterations_dev, pixel_iterations, true, st, &th, reg);
   return SNAPI_ERR_ARG;
 }
 
@@ -1403,8 +1501,9 @@ extern "C" int snapi_invert_eos(int precision, int jacobian, const snapi_line_ta
                                 const snapi_node_model *model, const snapi_chem_table *chem,
                                 const snapi_chem_table *chem_he, const snapi_state_map *map, int npix, int nd,
                                 const void *state, int nlambda, const double *lam, const void *obs, const void *mask,
-                                const double *wstokes, double broadening, double lm_start, int max_iter, int nslot,
-                                double *nodes, double *chi2, int *iterations, long long *pixel_iterations) {
+                                const double *wstokes, const snapi_instrument *inst, double lm_start, int max_iter,
+                                int nslot, double *nodes, double *chi2, int *iterations,
+                                long long *pixel_iterations) {
   if (!state || !lam || !obs || !wstokes || !nodes || !chi2 || !iterations || !chem || !chem_he || !map)
     return SNAPI_ERR_ARG;
   NodeArgs na;
@@ -1414,16 +1513,19 @@ extern "C" int snapi_invert_eos(int precision, int jacobian, const snapi_line_ta
   for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
   if (ntn < 1) return SNAPI_ERR_ARG;
   if (nslot <= 0 || nslot > npix) nslot = npix;
-  if (!(broadening >= 0.0)) return SNAPI_ERR_ARG;
-  const ThermoArgs th{chem, chem_he, map, broadening};
+  ThermoArgs th{chem, chem_he, map, 0.0, 0, nullptr};
+  double G[SNAPI_MAX_NODES * SNAPI_MAX_NODES];
+  const double *reg = reg_matrix(model, na, G) ? G : nullptr;
+  rc = instrument_args(inst, th);
+  if (rc) return rc;
   if (precision == SNAPI_FP64)
     return invert_impl<double>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
                                lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0,
-                               &th);
+                               &th, reg);
   if (precision == SNAPI_MIXED)
     return invert_impl<float>(precision, jacobian, lines, ray, na, npix, nd, state, nlambda, lam, obs, mask, wstokes,
                               lm_start, max_iter, nslot, nodes, nodes, chi2, iterations, pixel_iterations, false, 0,
-                              &th);
+                              &th, reg);
   return SNAPI_ERR_ARG;
 }
 

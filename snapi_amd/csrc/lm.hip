@@ -82,9 +82,12 @@ __global__ __launch_bounds__(256) void lm_normal_kernel(int npix, int nrow, int 
 
 // One wavefront per pixel: damped matrix in LDS, cyclic Jacobi, truncated
 // pseudo-inverse applied to rhs.
+// reg (or null): [npar][npar] added to every pixel's J^T J before the
+// damping, the regularisation of atmos_fit.cc:1371-1480.
 __global__ __launch_bounds__(64) void lm_solve_kernel(int npix, int npar, const double *__restrict__ JTJ,
                                                      const double *__restrict__ rhs, const double *__restrict__ lm,
-                                                     double *__restrict__ corr, const int *__restrict__ active) {
+                                                     double *__restrict__ corr, const int *__restrict__ active,
+                                                     const double *__restrict__ reg) {
   __shared__ double M[kNP][kNP + 1];
   __shared__ double Q[kNP][kNP + 1];
   __shared__ double rot[2];
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(64) void lm_solve_kernel(int npix, int npar, const 
   const double damp = 1.0 + lm[pix];
   for (int i = j; i < n * n; i += 64) {
     const int a = i / n, b = i - a * n;
-    M[a][b] = G[i] * (a == b ? damp : 1.0);
+    M[a][b] = (reg ? G[i] + reg[i] : G[i]) * (a == b ? damp : 1.0);
     Q[a][b] = (a == b) ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -182,8 +185,8 @@ template void launch_lm_normal<double>(int, int, int, const double *, const doub
                                        const int *, hipStream_t);
 
 void launch_lm_solve(int npix, int npar, const double *JTJ, const double *rhs, const double *lm, double *corr,
-                     const int *active, hipStream_t s) {
-  hipLaunchKernelGGL(lm_solve_kernel, dim3(npix), dim3(64), 0, s, npix, npar, JTJ, rhs, lm, corr, active);
+                     const int *active, hipStream_t s, const double *reg) {
+  hipLaunchKernelGGL(lm_solve_kernel, dim3(npix), dim3(64), 0, s, npix, npar, JTJ, rhs, lm, corr, active, reg);
 }
 
 }  // namespace snapi

@@ -12,6 +12,6 @@ void launch_lm_normal(int npix, int nrow, int npar, const T *J, const T *r, doub
                       const int *active, hipStream_t s);
 // damped FP64 pseudo-inverse solve per pixel
 void launch_lm_solve(int npix, int npar, const double *JTJ, const double *rhs, const double *lm, double *corr,
-                     const int *active, hipStream_t s);
+                     const int *active, hipStream_t s, const double *reg = nullptr);
 
 }  // namespace snapi

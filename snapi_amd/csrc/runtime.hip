@@ -153,7 +153,7 @@ int prepare_synth(const snapi_line_table *lines, const snapi_ray *ray, int ncol,
 
 using namespace snapi;
 
-extern "C" int snapi_version(void) { return 101; }
+extern "C" int snapi_version(void) { return 102; }
 
 extern "C" int snapi_context_create(snapi_context **ctx) {
   if (!ctx) return SNAPI_ERR_ARG;

@@ -12,17 +12,49 @@ PARAM = {"T": 1, "vt": 2, "vlos": 3, "B": 4, "theta": 5, "phi": 6}
 
 
 class SnapiNodeModel(C.Structure):
-    _fields_ = [("nnode", C.c_int), ("param", C.c_int * abi.MAX_NODES), ("logtau", C.c_double * abi.MAX_NODES)]
+    _fields_ = [("nnode", C.c_int), ("param", C.c_int * abi.MAX_NODES), ("logtau", C.c_double * abi.MAX_NODES),
+                ("reg_type", C.c_int * 7), ("reg_alpha", C.c_double * 7)]
 
 
-def node_model(nodes):
-    """nodes: list of (parameter name, log tau) in the reference's order."""
+class SnapiInstrument(C.Structure):
+    _fields_ = [("broadening", C.c_double), ("npsf", C.c_int), ("psf", C.POINTER(C.c_double))]
+
+
+REG_NONE, REG_ZERO, REG_TIKHONOV = 0, 1, 2
+
+
+def node_model(nodes, reg=None):
+    """nodes: list of (parameter name, log tau) in the reference's order;
+    reg: {parameter name: (type, alpha)} with type REG_ZERO (departure from
+    zero) or REG_TIKHONOV (model::*_reg_type / *_reg_alpha)."""
     m = SnapiNodeModel()
     m.nnode = len(nodes)
     for i, (p, lt) in enumerate(nodes):
         m.param[i] = PARAM[p]
         m.logtau[i] = lt
+    for p, (t, a) in (reg or {}).items():
+        m.reg_type[PARAM[p]] = int(t)
+        m.reg_alpha[PARAM[p]] = float(a)
     return m
+
+
+def instrument(broadening=0.0, psf=None):
+    """snapi_instrument of a fit (None when there is nothing to apply); the
+    PSF samples are kept alive on the returned structure."""
+    if not broadening and psf is None:
+        return None
+    inst = SnapiInstrument()
+    inst.broadening = float(broadening)
+    if psf is not None:
+        p = np.ascontiguousarray(psf, dtype=np.float64)
+        inst.npsf = p.size
+        inst.psf = abi.ptr(p)
+        inst._keep = p
+    return inst
+
+
+def _inst_ptr(inst):
+    return None if inst is None else C.byref(inst)
 
 
 def invert(table, ray, model, state, lam_vac, obs, nodes0, mask=None, wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0,
@@ -153,9 +185,10 @@ def node_jacobian_eos(table, ray, model, chem, chem_he, smap, state, nodes, lam_
 
 def invert_eos(table, ray, model, chem, chem_he, smap, state, lam_vac, obs, nodes0, mask=None,
                wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0, precision=abi.SNAPI_FP64,
-               broadening=0.0, analytic_maps=False, jacobian=abi.SNAPI_JAC_ADJOINT):
+               broadening=0.0, analytic_maps=False, jacobian=abi.SNAPI_JAC_ADJOINT, psf=None):
     """snapi_invert_eos: the inversion with temperature nodes (adjoint
-    Jacobian; analytic_maps as node_jacobian_eos). Returns (nodes, chi2,
+    Jacobian; analytic_maps as node_jacobian_eos); broadening [cm] and the
+    sampled spectral psf as snapi_instrument. Returns (nodes, chi2,
     iterations, total pixel-iterations)."""
     dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
     st = np.ascontiguousarray(state, dtype=dt)
@@ -168,17 +201,18 @@ def invert_eos(table, ray, model, chem, chem_he, smap, state, lam_vac, obs, node
     iters = np.zeros(npix, dtype=np.int32)
     total = C.c_longlong(0)
     ws = np.ascontiguousarray(wstokes, dtype=np.float64)
+    inst = instrument(broadening, psf)
     f = lib().snapi_invert_eos
     f.restype = C.c_int
     TB = C.POINTER(abi.SnapiChemTable)
     f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay), C.POINTER(SnapiNodeModel),
                   TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double),
-                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
-                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(SnapiInstrument), C.c_double, C.c_int,
+                  C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
     jac = (abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0) | int(jacobian)
     rc = f(precision, jac, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
            npix, nd, abi.vptr(st), len(lam), abi.ptr(lam), abi.vptr(ob), None if mk is None else abi.vptr(mk),
-           abi.ptr(ws), float(broadening), lm_start, max_iter, nslot, abi.ptr(nodes), abi.ptr(chi2),
+           abi.ptr(ws), _inst_ptr(inst), lm_start, max_iter, nslot, abi.ptr(nodes), abi.ptr(chi2),
            iters.ctypes.data_as(C.POINTER(C.c_int)), C.byref(total))
     check("snapi_invert_eos", rc)
     return nodes, chi2, iters, total.value
@@ -218,25 +252,26 @@ def build_from_nodes_dev(model, chem, chem_he, smap, state_t, nodes_t, stream=No
 def invert_eos_dev(table, ray, model, chem, chem_he, smap, state_t, lam_t, obs_t, mask_t, nodes0_t, nodes_t, chi2_t,
                    iters_t, wstokes=(1.0, 1.0, 1.0, 1.0), lm_start=1.0, max_iter=30, nslot=0,
                    precision=abi.SNAPI_MIXED, stream=None, broadening=0.0, analytic_maps=False,
-                   jacobian=abi.SNAPI_JAC_ADJOINT):
+                   jacobian=abi.SNAPI_JAC_ADJOINT, psf=None):
     """snapi_invert_eos_dev on torch device tensors; returns the total
     pixel-iterations."""
     npix, nf, nd = state_t.shape
     nl = lam_t.shape[0]
     ws = np.ascontiguousarray(wstokes, dtype=np.float64)
     total = C.c_longlong(0)
+    inst = instrument(broadening, psf)
     f = lib().snapi_invert_eos_dev
     f.restype = C.c_int
     TB = C.POINTER(abi.SnapiChemTable)
     f.argtypes = [C.c_int, C.c_int, C.POINTER(abi.SnapiLineTable), C.POINTER(abi.SnapiRay), C.POINTER(SnapiNodeModel),
                   TB, TB, C.POINTER(abi.SnapiStateMap), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
-                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
-                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]
+                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(SnapiInstrument), C.c_double, C.c_int,
+                  C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]
     sp = None if stream is None else C.c_void_p(stream.cuda_stream)
     jac = (abi.SNAPI_JAC_ANALYTIC_MAPS if analytic_maps else 0) | int(jacobian)
     rc = f(precision, jac, C.byref(table), C.byref(ray), C.byref(model), C.byref(chem), C.byref(chem_he), C.byref(smap),
            npix, nd, state_t.data_ptr(), nl, lam_t.data_ptr(), obs_t.data_ptr(),
-           None if mask_t is None else mask_t.data_ptr(), abi.ptr(ws), float(broadening), lm_start, max_iter, nslot,
+           None if mask_t is None else mask_t.data_ptr(), abi.ptr(ws), _inst_ptr(inst), lm_start, max_iter, nslot,
            nodes0_t.data_ptr(), nodes_t.data_ptr(), chi2_t.data_ptr(), iters_t.data_ptr(), C.byref(total), sp)
     check("snapi_invert_eos_dev", rc)
     return total.value

@@ -108,16 +108,17 @@ def responses(table, ray, state, dstate, lam, nthreads=0, fd_scale=1.0):
     return out, resp, rc
 
 
-def lm_step(J, r, lm):
+def lm_step(J, r, lm, reg=None):
     """One damped normal-equation solve as atmos/atmos_fit.cc:129-160 does it:
-    J^T J, J^T r, JTJ_ii *= (1 + lm), then solve_svd (cmlib/mathtools.cc:
-    344-387): pseudo-inverse dropping singular values with w_max/w > 1e6.
-    numpy's SVD is the textbook routine the reference's svdcmp computes."""
+    J^T J, J^T r, the regularisation reg (reg_matrix) added to J^T J, JTJ_ii
+    *= (1 + lm), then solve_svd (cmlib/mathtools.cc: 344-387):
+    pseudo-inverse dropping singular values with w_max/w > 1e6.  numpy's SVD
+    is the textbook routine the reference's svdcmp computes."""
     J = np.asarray(J, dtype=np.float64)
     r = np.asarray(r, dtype=np.float64)
     JTJ = J.T @ J
     rhs = J.T @ r
-    M = JTJ.copy()
+    M = JTJ.copy() if reg is None else JTJ + reg
     M[np.diag_indices_from(M)] *= (1.0 + lm)
     U, w, Vt = np.linalg.svd(M)
     wmax = w.max()
@@ -540,6 +541,72 @@ def bracket(model_nodes, v):
     return out
 
 
+def reg_matrix(model_nodes, reg):
+    """atmosphere::regularize_hessian / regularize_parameter
+    (atmos/atmos_fit.cc:1371-1480) as the matrix it adds to J^T J: for each
+    parameter p with reg[p] = (type, alpha), R over its first run of
+    consecutive nodes, from its first node to the node before the next
+    parameter's (none when the run ends the list: the reference's `to`
+    stays 0), type 1 R_ii = 1 on the run, type 2 R_ii = 1, R_i,i-1 = -1
+    after the first node and R_ff = -1, R_f,f+1 = 1 at the first (dropped
+    beyond the last node); the sum of alpha^2 R^T R.  The right-hand side is
+    not changed (its update is commented out at :1466-1468)."""
+    n = len(model_nodes)
+    G = np.zeros((n, n))
+    codes = [p for p, _ in model_nodes]
+    for p, (typ, alpha) in (reg or {}).items():
+        if not typ or p not in codes:
+            continue
+        f = codes.index(p)
+        t = -1
+        for i in range(f, n):
+            if codes[i] != p:
+                t = i - 1
+                break
+        R = np.zeros((n, n))
+        if typ == 2:
+            for i in range(f + 1, t + 1):
+                R[i, i] = 1.0
+                R[i, i - 1] = -1.0
+            R[f, f] = -1.0
+            if f + 1 < n:
+                R[f, f + 1] = 1.0
+        else:
+            for i in range(f, t + 1):
+                R[i, i] = 1.0
+        G += alpha * alpha * (R.T @ R)
+    return G
+
+
+def psf_convolve(S, psf):
+    """convolve_spectra_with_psf (cmlib/mathtools.cc:1429-1465) in its
+    defined form, along axis 0 of S [nlambda][...]: the n samples centred on
+    every point, out[l] = sum_j S[l - (n-1)/2 + j] psf[j] / sum(psf), samples
+    off the grid dropped, the sum in j order as (S psf) / norm."""
+    S = np.asarray(S, dtype=np.float64)
+    p = np.asarray(psf, dtype=np.float64)
+    n, nl = p.size, S.shape[0]
+    norm = 0.0
+    for x in p:
+        norm += x
+    h = (n - 1) // 2
+    out = np.zeros_like(S)
+    for l in range(nl):
+        acc = np.zeros(S.shape[1:])
+        for j in range(max(0, h - l), min(n - 1, nl - 1 - l + h) + 1):
+            acc = acc + S[l - h + j] * p[j] / norm
+        out[l] = acc
+    return out
+
+
+def add_scattered_light(S, fraction, continuum_level):
+    """observable::add_scattered_light (cmlib/obs.cc:412-428): Stokes I of
+    every wavelength raised by fraction * continuum_level; S [...][nlambda][4]."""
+    out = np.array(S, dtype=np.float64)
+    out[..., 0] += fraction * continuum_level
+    return out
+
+
 def polish_angles(model_nodes, v):
     """model::polish_angles (atmos_model.cc:992-1001): inclination to
     acos(cos theta), azimuth to atan(tan phi)."""
@@ -553,7 +620,7 @@ def polish_angles(model_nodes, v):
 
 
 def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, smap, wstokes, max_iter=30,
-               lm_start=1.0, broadening=0.0, mask=None, history=False):
+               lm_start=1.0, broadening=0.0, mask=None, history=False, psf=None, reg=None):
     """stokes_lm_fit (atmos_fit.cc:18-227) of one pixel with temperature
     nodes, in numpy over the oracle's chain: the start bracketed into the
     parameter limits (:91); the Jacobian and the current chi^2 from the
@@ -567,7 +634,11 @@ def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, s
     steps whose chi^2 changes by less than 1e-3 or at max_iter; the angles
     polished at the end (:217).  With broadening [cm] the current spectrum
     and every column of the Jacobian (before the scaling) and every trial
-    spectrum are convolved with the Gaussian (:104-110).  Returns (nodes,
+    spectrum are convolved with the Gaussian (:104-110).  With a sampled psf
+    the current spectrum and the Jacobian take the PSF instead when there is
+    no broadening (:111-115), and every trial the Gaussian, then the PSF
+    (forward_evaluate, :726-734).  reg {parameter: (type, alpha)}: the
+    model's regularisation (reg_matrix) in every solve.  Returns (nodes,
     chi2, iterations), chi2 the response path's of the last Jacobian, and with
     history also the nodes after every iteration."""
     ws = np.asarray(wstokes, dtype=np.float64)
@@ -581,13 +652,18 @@ def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, s
     it = 0
     hist = []
 
+    G = reg_matrix(model_nodes, reg) if reg else None
+
     def conv(a):
-        return convolve_gauss(a, lam, broadening) if broadening > 0.0 else a
+        if broadening > 0.0:
+            return convolve_gauss(a, lam, broadening)
+        return psf_convolve(a, psf) if psf is not None else a
 
     def forward(vals):
         st = forward_state_eos(model_nodes, vals, state, chem, chem_he, smap)
         s, _ = synth(table, sray, st[None], lam)
-        return conv(s[0])
+        s = convolve_gauss(s[0], lam, broadening) if broadening > 0.0 else s[0]
+        return psf_convolve(s, psf) if psf is not None else s
 
     while it < max_iter:
         if need:
@@ -597,7 +673,7 @@ def lm_fit_eos(table, ray, model_nodes, start, state, lam, obs, chem, chem_he, s
             r = (obs - S).reshape(-1) * w
             chi = float((r ** 2).sum())
             Js = J * sc[None, :] * w[:, None]
-        _, _, corr = lm_step(Js, r, lm)
+        _, _, corr = lm_step(Js, r, lm, G)
         vt = bracket(model_nodes, v + corr * sc)
         chit = float((((forward(vt) - obs).reshape(-1) * w) ** 2).sum())
         it += 1

@@ -726,11 +726,16 @@ def fit_lambda():
     return cfg1_lambda()[::3]
 
 
-def write_model(path, values, tau_min=-6.0, tau_max=1.0):
+def write_model(path, values, tau_min=-6.0, tau_max=1.0, nodes=None, reg=()):
+    """The driver's model file; reg: (code, type, alpha) entries."""
+    nodes = FIT_NODES if nodes is None else nodes
     with open(path, "w") as f:
-        f.write("%.17g %.17g\n%d\n" % (tau_min, tau_max, len(FIT_NODES)))
-        for (code, lt), v in zip(FIT_NODES, values):
+        f.write("%.17g %.17g\n%d\n" % (tau_min, tau_max, len(nodes)))
+        for (code, lt), v in zip(nodes, values):
             f.write("%d %.17g %.17g\n" % (code, lt, v))
+        f.write("%d\n" % len(reg))
+        for code, typ, alpha in reg:
+            f.write("%d %d %.17g\n" % (code, typ, alpha))
 
 
 def run_nodes(d, values, tag):
@@ -795,18 +800,33 @@ def _fit_cases():
     return cases
 
 
-def build_fit(d, kmax=8):
+# the model's regularisation in the regularised fit: temperature Tikhonov,
+# vLOS Tikhonov, |B| departure from zero, alphas of the order of the square
+# roots of the scaled J^T J diagonal of these parameters
+REG_FIT = [(1, 2, 3e15), (3, 2, 1e14), (4, 1, 2e14)]
+
+
+def build_fitreg(d):
+    """The fit of build_fit with the model regularised (REG_FIT), pixels 0
+    and 1, 1..6 iterations."""
+    build_fit(d, kmax=6, reg=REG_FIT, name="fitreg", pixels=(0, 1))
+
+
+def build_fit(d, kmax=8, reg=(), name="fit", pixels=(0, 1, 2)):
     """atmosphere::stokes_lm_fit on three pixels of the 14-node model, run with
     1..kmax iterations (the nodes and synthesis-path chi^2 after each, i.e.
-    the trajectory), weights (1, 5, 5, 3), start lambda 1."""
+    the trajectory), weights (1, 5, 5, 3), start lambda 1; reg the model's
+    regularisation entries."""
     rows = cfg0_atmosphere(71)
     lam = fit_lambda()
     setup_dir(d, rows, lam)
     ws = np.array([1.0, 5.0, 5.0, 3.0])
     recs = dict(codes=np.array([c for c, _ in FIT_NODES]), logtau=np.array([t for _, t in FIT_NODES]), lam_air=lam,
-                ws=ws, kmax=np.array(kmax), lm_start=np.array(1.0))
+                ws=ws, kmax=np.array(kmax), lm_start=np.array(1.0), reg=np.array(reg, dtype=np.float64).reshape(-1, 3))
     procs = []
     for k, cs in enumerate(_fit_cases()):
+        if k not in pixels:
+            continue
         fwd = run_nodes(d, cs["truth"], "t%d" % k)["fwd"]
         rng = np.random.default_rng(cs["seed"])
         obs = fwd + rng.normal(0.0, 1e-3 * fwd[0, 0], fwd.shape)
@@ -814,7 +834,7 @@ def build_fit(d, kmax=8):
         if cs["masked"]:
             mask[rng.permutation(len(lam))[: int(0.3 * len(lam))]] = 0.0
         path = os.path.join(d, "fit_%d.txt" % k)
-        write_model(path, cs["start"])
+        write_model(path, cs["start"], reg=reg)
         with open(path, "a") as f:
             f.write("%.17g %.17g %.17g %.17g %.17g %d\n" % (ws[0], ws[1], ws[2], ws[3], 1.0, kmax))
             f.write(" ".join("%.17g" % m for m in mask) + "\n")
@@ -825,7 +845,7 @@ def build_fit(d, kmax=8):
         env["SNAPI_REF_HE"] = "0"
         procs.append(subprocess.Popen([DRIVER, "fit", "atmos.cfg", "lambda.txt", repr(math.pi), "0.0", path], cwd=d,
                                       env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
-    for k, pr in enumerate(procs):
+    for k, pr in zip(pixels, procs):
         out, err = pr.communicate()
         if pr.returncode:
             raise RuntimeError("fit driver failed on pixel %d" % k)
@@ -855,8 +875,56 @@ def build_fit(d, kmax=8):
         recs["chi%d" % k] = np.array(chi)
         recs["S%d" % k] = np.array(S)
         print("fit pixel %d: chi2 %s" % (k, np.array2string(np.array(chi), precision=3)))
-    np.savez_compressed(os.path.join(HERE, "fit.npz"), **recs)
-    print("fit: %d pixels, %d iterations" % (len(procs), kmax))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **recs)
+    print("%s: %d pixels, %d iterations" % (name, len(procs), kmax))
+
+
+def build_psf(d):
+    """convolve_spectra_with_psf of the reference on seeded spectra and
+    PSFs (the driver hands the samples over so that the reference's window,
+    read from index 0 of its array, holds them in order): odd sample
+    counts, whose window is the defined centred one from the (npsf-1)/2-th
+    wavelength on."""
+    recs = {}
+    for key, args in (("p0", ["101", "7", "41"]), ("p1", ["60", "13", "42"]), ("p2", ["31", "1", "43"]),
+                      ("p3", ["40", "31", "44"])):
+        out = run(d, ["psf"] + args, he=False)
+        recs[key + "_psf"] = np.array([float(x) for x in out[0].split()[1:]])
+        recs[key + "_in"] = np.array([[float(x) for x in ln.split()[1:]] for ln in out if ln.startswith("in ")])
+        recs[key + "_out"] = np.array([[float(x) for x in ln.split()[1:]] for ln in out if ln.startswith("out ")])
+    np.savez_compressed(os.path.join(HERE, "psf.npz"), **recs)
+    print("psf: 4 cases")
+
+
+REG_CASES = [
+    # (nodes, [(code, type, alpha)])
+    (FIT_NODES, [(1, 2, 0.5), (3, 1, 2.0)]),            # T Tikhonov (7 nodes), vLOS zero (2)
+    (FIT_NODES, [(4, 2, 1.3), (2, 1, 0.7)]),            # |B| Tikhonov (2), vt zero (1)
+    (FIT_NODES, [(5, 2, 0.9), (1, 1, 0.1)]),            # one inclination node: couples to the azimuth node
+    (FIT_NODES, [(6, 1, 3.0), (4, 1, 0.2)]),            # the last parameter's run ends the list: nothing
+    ([(1, -4.0), (1, -2.0), (1, 0.0), (4, -3.0), (4, -1.0)], [(4, 2, 1.1), (1, 2, 0.4)]),  # last run, Tikhonov
+]
+
+
+def build_reg(d):
+    """atmosphere::regularize_hessian of the reference on a zero J^T J: the
+    matrix it adds, for node models with Tikhonov and departure-from-zero
+    regularisation, including the runs that end the node list."""
+    rows = cfg0_atmosphere(71)
+    setup_dir(d, rows, fit_lambda())
+    recs = {}
+    for k, (nodes, reg) in enumerate(REG_CASES):
+        path = os.path.join(d, "reg_%d.txt" % k)
+        write_model(path, [5000.0 if c == 1 else 1.0 for c, _ in nodes], nodes=nodes, reg=reg)
+        out = run(d, ["reg", "atmos.cfg", path], he=False)
+        G = np.array([[float(x) for x in ln.split()[2:-2]] for ln in out if ln.startswith("G ")])
+        rhs = np.array([float(ln.split()[-1]) for ln in out if ln.startswith("G ")])
+        recs["codes%d" % k] = np.array([c for c, _ in nodes])
+        recs["reg%d" % k] = np.array(reg, dtype=np.float64)
+        recs["G%d" % k] = G
+        recs["rhs%d" % k] = rhs
+    np.savez_compressed(os.path.join(HERE, "reg.npz"), **recs)
+    print("reg: %d models" % len(REG_CASES))
 
 
 def build_tau(d):
@@ -872,7 +940,7 @@ def build_tau(d):
 
 
 BUILDERS = ["cfg0", "cfg1_template", "resp", "formal_scalar", "lmstep", "interp", "conv", "ana", "nlte", "chemeq",
-            "nlteresp", "nodes", "fit", "tau"]
+            "nlteresp", "nodes", "fit", "tau", "psf", "reg", "fitreg"]
 
 
 def main():

@@ -49,7 +49,9 @@ def test_struct_layout_matches_header():
              ("snapi_chem_element", abi.SnapiChemElement), ("snapi_chem_molecule", abi.SnapiChemMolecule),
              ("snapi_chem_level", abi.SnapiChemLevel), ("snapi_chem_bf", abi.SnapiChemBf),
              ("snapi_chem_table", abi.SnapiChemTable), ("snapi_state_map", abi.SnapiStateMap),
-             ("snapi_level", __import__("snapi_amd.lines", fromlist=["SnapiLevel"]).SnapiLevel)]
+             ("snapi_level", __import__("snapi_amd.lines", fromlist=["SnapiLevel"]).SnapiLevel),
+             ("snapi_node_model", __import__("snapi_amd.invert", fromlist=["SnapiNodeModel"]).SnapiNodeModel),
+             ("snapi_instrument", __import__("snapi_amd.invert", fromlist=["SnapiInstrument"]).SnapiInstrument)]
     lines = []
     want = []
     for cname, cls in pairs:

@@ -5,7 +5,9 @@ atmos_pp::build_from_nodes (plpar/atmos_pp.cc:58-248) and the ray projection
 obs_stokes_responses_to_nodes (atmos/atmos_fit.cc:1205-1352); snapi_invert_eos
 against the trajectory of stokes_lm_fit (atmos/atmos_fit.cc:18-227), eight
 iterations on three pixels (one masked, one starting below the parameter
-limits).
+limits), and six iterations on two pixels with the model regularised
+(tests/golden/fitreg.npz: temperature and vLOS Tikhonov, |B| departure from
+zero; regularize_hessian, atmos_fit.cc:1371-1480).
 
 Tolerances: the GPU's Jacobian is the exact derivative through the solver,
 the reference's a difference once per node direction; in FP64 each node's
@@ -193,3 +195,36 @@ def test_difference_jacobian_inversion_with_temperature_nodes(setup):
         assert chif < 1.2 * f["chi%d" % p][-1]
         assert rel[2:6].max() < 1e-2
 
+
+
+def test_regularised_inversion_follows_reference_trajectory(setup):
+    """snapi_invert_eos with the regularisation in the node model, 1..6
+    iterations on two pixels: the nodes within 2e-3 of the reference's
+    regularised fit after each, the same iteration counts, and a trajectory
+    that differs from the unregularised one."""
+    s = setup
+    f = _golden.load("fitreg.npz")
+    reg = {NAMES[int(c)]: (int(t), a) for c, t, a in f["reg"]}
+    model = invert.node_model(s["nodes"], reg=reg)
+    K = int(f["kmax"])
+    pix = (0, 1)
+    obs = np.stack([f["obs%d" % p] for p in pix])
+    mask = np.stack([f["mask%d" % p] for p in pix])
+    start = np.stack([f["start%d" % p] for p in pix])
+    st = np.repeat(s["state"][None], 2, axis=0)
+    print("regularised inversion vs stokes_lm_fit: 2 pixels, 1..%d iterations, FP64" % K, flush=True)
+    worst = 0.0
+    for k in range(1, K + 1):
+        nodes, chi2, iters, total = invert.invert_eos(s["t"]["table"], s["t"]["ray"], model, s["chem"],
+                                                      s["chem_he"], s["smap"], st, s["lam"], obs, start, mask=mask,
+                                                      wstokes=f["ws"], lm_start=float(f["lm_start"]), max_iter=k,
+                                                      nslot=2, precision=abi.SNAPI_FP64)
+        for i, p in enumerate(pix):
+            ref = f["nodes%d" % p][k - 1]
+            rel = np.abs(nodes[i] - ref) / np.abs(ref)
+            worst = max(worst, rel.max())
+            assert rel.max() < 2e-3, (k, p, rel.max(), rel.argmax())
+            assert iters[i] == f["iters%d" % p][k - 1]
+    print("  worst node difference %.2e" % worst, flush=True)
+    free = s["f"]
+    assert np.max(np.abs(f["nodes0"][K - 1] / free["nodes0"][K - 1] - 1.0)) > 1e-2

@@ -77,3 +77,54 @@ def test_long_grid_uses_smaller_groups():
     e = np.max(np.abs(out - o)) / np.max(np.abs(o))
     print("  vs oracle %.2e" % e, flush=True)
     assert e < 1e-10
+
+
+def test_psf_convolution_matches_reference_and_oracle():
+    """snapi_psf_convolve against the reference's convolve_spectra_with_psf
+    where its window is the centred one (tests/golden/psf.npz) and against
+    the oracle's defined version everywhere (1e-14 of a component's largest
+    value in FP64, 1e-6 mixed)."""
+    g = _golden.load("psf.npz")
+    for k in range(4):
+        p, S, out = g["p%d_psf" % k], g["p%d_in" % k], g["p%d_out" % k]
+        h = (p.size - 1) // 2
+        print("psf convolution p%d: %d samples, %d wavelengths" % (k, p.size, S.shape[0]), flush=True)
+        o = instrument.psf_convolve(S[None], p)[0]
+        scale = np.max(np.abs(out), axis=0)
+        e_ref = np.max(np.abs(o[h:] - out[h:]) / scale)
+        e_or = np.max(np.abs(o - _oracle.psf_convolve(S, p)) / scale)
+        m = instrument.psf_convolve(S[None], p, abi.SNAPI_MIXED)[0]
+        e_m = np.max(np.abs(m - _oracle.psf_convolve(S, p)) / scale)
+        print("  vs reference %.2e, vs oracle %.2e, mixed %.2e" % (e_ref, e_or, e_m), flush=True)
+        assert e_ref < 1e-14 and e_or < 1e-14 and e_m < 1e-6
+
+
+def test_psf_jacobian_columns_and_device_entry_point():
+    """A [npix][nlambda][4 nnode] Jacobian convolves column by column (the
+    workgroups take up to 16 columns at a time), through the device entry
+    point on a stream, in both precisions."""
+    import torch
+    rng = np.random.default_rng(8)
+    J = rng.normal(size=(3, 57, 4 * 14))
+    p = rng.uniform(0.1, 1.0, 11)
+    ref = np.stack([_oracle.psf_convolve(J[i], p) for i in range(3)])
+    for dt, tol in ((torch.float64, 1e-13), (torch.float32, 1e-5)):
+        print("psf on a Jacobian, device entry point, %s" % dt, flush=True)
+        jt = torch.from_numpy(J).to("cuda", dt)
+        pt = torch.from_numpy(p).to("cuda")
+        s = torch.cuda.Stream()
+        instrument.psf_convolve_dev(jt, pt, s)
+        s.synchronize()
+        assert np.max(np.abs(jt.cpu().numpy() - ref)) < tol * np.max(np.abs(ref))
+
+
+def test_scattered_light():
+    S = np.random.default_rng(2).normal(size=(4, 23, 4))
+    print("scattered light: host and device entry points", flush=True)
+    o = instrument.add_scattered_light(S, 0.07, 3.0)
+    assert np.allclose(o, _oracle.add_scattered_light(S, 0.07, 3.0), rtol=0, atol=1e-15)
+    import torch
+    st = torch.from_numpy(S).to("cuda", torch.float32)
+    instrument.add_scattered_light_dev(st, 0.07, 3.0)
+    torch.cuda.synchronize()
+    assert np.allclose(st.cpu().numpy(), _oracle.add_scattered_light(S, 0.07, 3.0), rtol=0, atol=1e-6)

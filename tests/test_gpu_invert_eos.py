@@ -147,3 +147,38 @@ def test_inversion_with_spectral_broadening_follows_oracle_fit(problem):
         chi2[0], co, iters[0], io, np.round(nodes[0, 2:6], 1), np.round(vo[2:6], 1)), flush=True)
     assert abs(chi2[0] / co - 1.0) < 5e-2 and iters[0] == io
     assert np.max(np.abs(nodes[0, 2:6] - vo[2:6])) < 5.0
+
+
+def test_inversion_with_psf_follows_oracle_fit(problem):
+    """With a sampled 9-point spectral PSF and no Gaussian the fit convolves
+    the current spectrum, the Jacobian and every trial with the PSF
+    (atmos_fit.cc:111-115, 730-734): one pixel, six iterations, against the
+    oracle's fit with the same convolution; observations convolved
+    likewise."""
+    from snapi_amd import instrument
+    p = problem
+    t = p["t"]
+    model = invert.node_model(NODES)
+    ws = np.array([1.0, 5.0, 5.0, 3.0])
+    psf = np.exp(-0.5 * ((np.arange(9) - 4.0) / 1.7) ** 2) * (1.0 + 0.05 * np.arange(9))
+    obs = instrument.psf_convolve(p["obs"][:1], psf)
+    print("inversion with a spectral PSF vs the oracle's fit: 1 pixel, 6 iterations", flush=True)
+    nodes, chi2, iters, total = invert.invert_eos(t["table"], t["ray"], model, p["chem"], p["chem_he"], p["smap"],
+                                                  p["states"][:1], p["lam"], obs, p["start"][:1], wstokes=ws,
+                                                  max_iter=6, precision=abi.SNAPI_FP64, psf=psf)
+    vo, co, io = _oracle.lm_fit_eos(t["table"], t["ray"], NODES, p["start"][0], t["state"], p["lam"], obs[0],
+                                    p["chem"], p["chem_he"], p["smap"], ws, max_iter=6, psf=psf)
+    print("  chi2 GPU %.4e oracle %.4e; iterations %d / %d; T(-3..0) GPU %s oracle %s" % (
+        chi2[0], co, iters[0], io, np.round(nodes[0, 2:6], 1), np.round(vo[2:6], 1)), flush=True)
+    assert abs(chi2[0] / co - 1.0) < 5e-2 and iters[0] == io
+    assert np.max(np.abs(nodes[0, 2:6] - vo[2:6])) < 5.0
+
+
+def test_instrument_arguments_are_checked(problem):
+    p = problem
+    t = p["t"]
+    model = invert.node_model(NODES)
+    for kw in (dict(broadening=-1e-10), dict(psf=np.ones(1025))):
+        with pytest.raises(RuntimeError):
+            invert.invert_eos(t["table"], t["ray"], model, p["chem"], p["chem_he"], p["smap"], p["states"][:1],
+                              p["lam"], p["obs"][:1], p["start"][:1], max_iter=1, precision=abi.SNAPI_FP64, **kw)

@@ -130,6 +130,27 @@ def test_lm_fit_follows_reference_trajectory(setup):
         assert abs(chif / f["chi%d" % p][-1] - 1.0) < 1e-3, (p, chif, f["chi%d" % p][-1])
 
 
+def test_regularised_lm_fit_follows_reference_trajectory(setup):
+    """The fit with the model regularised (tests/golden/fitreg.npz:
+    temperature and vLOS Tikhonov, |B| departure from zero): the oracle's
+    nodes after each of the reference's six iterations within 5e-4, the same
+    iteration counts, on both pixels."""
+    s = setup
+    f = _golden.load("fitreg.npz")
+    names = {1: "T", 2: "vt", 3: "vlos", 4: "B", 5: "theta", 6: "phi"}
+    reg = {names[int(c)]: (int(t), a) for c, t, a in f["reg"]}
+    K = int(f["kmax"])
+    for p in (0, 1):
+        v, chi, it, hist = _oracle.lm_fit_eos(s["t"]["table"], s["t"]["ray"], s["nodes"], f["start%d" % p], s["state"],
+                                              s["lam"], f["obs%d" % p], s["chem"], s["chem_he"], s["smap"], f["ws"],
+                                              max_iter=K, lm_start=float(f["lm_start"]), mask=f["mask%d" % p],
+                                              history=True, reg=reg)
+        assert it == f["iters%d" % p][-1]
+        for j in range(it):
+            rel = np.abs(hist[j] - f["nodes%d" % p][j]) / np.abs(f["nodes%d" % p][j])
+            assert rel.max() < 5e-4, (p, j, rel.max(), rel.argmax())
+
+
 def test_bracket_and_polish_are_the_reference_rules():
     nodes = [("T", 0.0), ("vt", 0.0), ("vlos", 0.0), ("B", 0.0), ("theta", 0.0), ("phi", 0.0)]
     b = _oracle.bracket(nodes, [3000.0, 50.0, -2e6, 0.5, 3.5, 2.0])

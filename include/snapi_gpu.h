@@ -349,8 +349,14 @@ enum snapi_jacobian {
  *   null: all 1) in the precision's type; nodes [npix][nnode] in: start
  *   values, out: solution; chi2 [npix], iterations [npix] out.
  * nslot pixels are iterated at once; a finished pixel's slot is refilled from
- * the queue of the remaining ones.  pixel_iterations (optional) returns the
- * total number of LM iterations over all pixels.
+ * the queue of the remaining ones; nslot <= 0: one slot per pixel, capped at
+ * what half of the free device memory holds.  The host reads the active
+ * count only every k iterations (k doubling while a batch of iterations
+ * takes under 2 ms, up to 32); the _dev variants on a stream other than the
+ * null stream replay the iterations after the first from one captured HIP
+ * graph (SNAPI_LAUNCH=eager in the environment: eager launches; the results
+ * are the same bit for bit).  pixel_iterations (optional) returns the total
+ * number of LM iterations over all pixels, counted on the device.
  */
 int snapi_invert(int precision, int jacobian, const snapi_line_table *lines, const snapi_ray *ray,
                  const snapi_node_model *model, int npix, int nd, const void *state, int nlambda, const double *lam,

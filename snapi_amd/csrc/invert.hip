@@ -26,7 +26,10 @@
 // the launches full.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "lm_kernel.h"
@@ -496,9 +499,79 @@ __global__ void expand_active_kernel(long n, int group, const int *__restrict__ 
   if (i < n) out[i] = active[i / group];
 }
 
-__global__ void count_active_kernel(int nslot, const int *__restrict__ active, int *__restrict__ count) {
-  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot < nslot && active[slot]) atomicAdd(count, 1);
+// One workgroup of 1024: count[0] = the active slots, total[0] += count[0].
+__global__ __launch_bounds__(1024) void active_count_kernel(int nslot, const int *__restrict__ active,
+                                                            int *__restrict__ count, long long *__restrict__ total) {
+  __shared__ int part[1024];
+  int n = 0;
+  for (int s = threadIdx.x; s < nslot; s += blockDim.x) n += active[s] != 0;
+  part[threadIdx.x] = n;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    count[0] = part[0];
+    total[0] += part[0];
+  }
+}
+
+// Runs iteration() until an iteration finds no active slot.  Iterations go
+// in batches of k between host reads of the active count (k doubles while a
+// batch takes under 2 ms, up to 32; the extra iterations of the last batch
+// find nothing to do).  On a stream other than the null stream the first
+// iteration runs eagerly and the second is captured into a HIP graph that
+// every later iteration replays; SNAPI_LAUNCH=eager in the environment
+// keeps every iteration eager.  count: the device word the iteration
+// writes; pinned: one int of pinned host memory.
+template <typename F>
+static int run_iterations(F &iteration, hipStream_t st, const int *count, int *pinned) {
+  const char *mode = getenv("SNAPI_LAUNCH");
+  bool want_graph = st != nullptr && !(mode && !strcmp(mode, "eager"));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int rc = SNAPI_OK, done_iters = 0, k = 1;
+  for (;;) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int b = 0; b < k && !rc; ++b, ++done_iters) {
+      if (exec) {
+        if (hipGraphLaunch(exec, st) != hipSuccess) rc = SNAPI_ERR_DEVICE;
+        continue;
+      }
+      if (want_graph && done_iters >= 1) {
+        // capture one iteration; on any failure stay eager
+        want_graph = false;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+          const int r = iteration();
+          const hipError_t e = hipStreamEndCapture(st, &graph);
+          if (r == SNAPI_OK && e == hipSuccess && graph &&
+              hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+            if (hipGraphLaunch(exec, st) != hipSuccess) rc = SNAPI_ERR_DEVICE;
+            continue;
+          }
+          (void)hipGetLastError();
+          if (exec) (void)hipGraphExecDestroy(exec);
+          if (graph) (void)hipGraphDestroy(graph);
+          exec = nullptr;
+          graph = nullptr;
+        }
+      }
+      rc = iteration();
+    }
+    if (rc) break;
+    if (hipMemcpyAsync(pinned, count, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      rc = SNAPI_ERR_DEVICE;
+      break;
+    }
+    if (*pinned == 0) break;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (ms < 2.0 && k < 32) k *= 2;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  return rc;
 }
 
 // ---- temperature nodes: the thermodynamic state rebuilt from the nodes ----
@@ -677,7 +750,8 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   R *fdstate = nullptr, *Sall = nullptr, *scr = nullptr;
   double *dlam = nullptr, *pnodes0 = nullptr, *snodes = nullptr, *tnodes = nullptr, *JTJ = nullptr, *rhs = nullptr;
   double *corr = nullptr, *onodes = nullptr, *ochi2 = nullptr, *fdn = nullptr, *hv = nullptr, *dbl = nullptr;
-  int *ints = nullptr, *queue = nullptr, *oiter = nullptr, *count = nullptr;
+  int *ints = nullptr, *queue = nullptr, *oiter = nullptr, *count = nullptr, *pinned = nullptr;
+  long long *total = nullptr;
   double *dreg = nullptr, *dpsf = nullptr;  // regularisation [nn][nn], spectral PSF
   // temperature nodes: profiles and densities of the rebuilds, double states
   // for the equation of state, the response path's state and derivatives
@@ -698,10 +772,47 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   // caller's buffers (device variant) or copies of the host arrays
   void *own[8] = {};
   SlotState ss;
-  int chunk = nslot;
   size_t per_col = 0;
+  int chunk = 0;
   int *status = device_status_slot();
   if (!status) return SNAPI_ERR_DEVICE;
+  if (th) {
+    // the difference Jacobian differences the synthesis path's spectrum of
+    // every node set; with instrument effects only the response-path
+    // Jacobians are built
+    const bool jac_ok = jacobian == SNAPI_JAC_ADJOINT || jacobian == SNAPI_JAC_TANGENT ||
+                        (jacobian == SNAPI_JAC_DIFFERENCES && !(th->broadening > 0.0) && th->npsf == 0);
+    if (!jac_ok || nf != SNAPI_NF_BASE + 2 * th->map->nline) {
+      rc = !jac_ok ? SNAPI_ERR_UNSUPPORTED : SNAPI_ERR_ARG;
+      goto done;
+    }
+    rc = chem_upload(th->chem, th->map, cdev, st);
+    if (rc) goto done;
+    rc = chem_upload(th->chem_he, nullptr, cdev_he, st);
+    if (rc) goto done;
+  }
+  if (nslot <= 0) {
+    // as many slots as the pixels, or as fit in half of the free device
+    // memory: the bytes one slot holds in the buffers allocated below
+    size_t b = es * ((size_t)ns * 4 + (size_t)nl * 17 + (size_t)nl * 4 * nn + (size_t)nn * nd) +
+               8 * ((size_t)nn * 4 + (size_t)nn * nn + 6) + 4 * 7;
+    if (th)
+      b += 8 * ((size_t)3 * nvar * nd + (size_t)ns * 5) + chem_eos_scratch(cdev, nd, true) +
+           es * ((size_t)ns * 3 + (size_t)nn * nd) +
+           (na.analytic && ntn ? 8 * (size_t)2 * ntn * nd + chem_he_tangent_scratch(1, nd) : 0);
+    if (jacobian == SNAPI_JAC_ADJOINT) b += resp_scratch_per_col(nl, nd, es);
+    if (jacobian == SNAPI_JAC_DIFFERENCES) {
+      b += es * ((size_t)nv * ns + (size_t)nv * nl * 4) + 8 * ((size_t)nv * nn + nn);
+      if (th) b += (size_t)nv * (4 + 8 * (size_t)nd * 3 + 8 * (size_t)ns) + chem_eos_scratch(cdev, (long)nv * nd, false);
+    }
+    size_t fr = 0, tot = 0;
+    nslot = npix;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > 0) {
+      const size_t fit = (fr / 2) / b;
+      if (fit < (size_t)nslot) nslot = fit > 0 ? (int)fit : 1;
+    }
+  }
+  chunk = nslot;
   if (dev) {
     pstate = (R *)pstate_h;
     pobs = (R *)obs_h;
@@ -748,8 +859,10 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   HIPCHK(hipMalloc((void **)&corr, (size_t)nslot * nn * 8));
   HIPCHK(hipMalloc((void **)&dbl, (size_t)nslot * 6 * 8));
   HIPCHK(hipMalloc((void **)&ints, (size_t)nslot * 7 * sizeof(int)));
-  HIPCHK(hipMalloc((void **)&queue, 2 * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&queue, 4 * sizeof(int)));  // next pixel, active count, total (long long)
   count = queue + 1;
+  total = reinterpret_cast<long long *>(queue + 2);
+  HIPCHK(hipHostMalloc((void **)&pinned, sizeof(int), hipHostMallocDefault));
   ss.pix = ints;
   ss.iter = ints + nslot;
   ss.corrected = ints + 2 * nslot;
@@ -762,19 +875,6 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   ss.chi2t = dbl + 2 * nslot;
   ss.track = dbl + 3 * nslot;
   if (th) {
-    // the difference Jacobian differences the synthesis path's spectrum of
-    // every node set; with spectral broadening only the response-path
-    // Jacobians are built
-    const bool jac_ok = jacobian == SNAPI_JAC_ADJOINT || jacobian == SNAPI_JAC_TANGENT ||
-                        (jacobian == SNAPI_JAC_DIFFERENCES && !(th->broadening > 0.0) && th->npsf == 0);
-    if (!jac_ok || nf != SNAPI_NF_BASE + 2 * th->map->nline) {
-      rc = !jac_ok ? SNAPI_ERR_UNSUPPORTED : SNAPI_ERR_ARG;
-      goto done;
-    }
-    rc = chem_upload(th->chem, th->map, cdev, st);
-    if (rc) goto done;
-    rc = chem_upload(th->chem_he, nullptr, cdev_he, st);
-    if (rc) goto done;
     HIPCHK(hipMalloc((void **)&Tv, (size_t)nslot * nvar * nd * 8));
     HIPCHK(hipMalloc((void **)&ltv, (size_t)nslot * nvar * nd * 8));
     HIPCHK(hipMalloc((void **)&Ntv, (size_t)nslot * nvar * nd * 8));
@@ -835,7 +935,7 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
   HIPCHK(hipMemsetAsync(dzero, 0, (size_t)nslot * 2 * ns * es, st));
   HIPCHK(hipMemsetAsync(ints, 0, (size_t)nslot * 7 * sizeof(int), st));
   HIPCHK(hipMemsetAsync(dbl, 0, (size_t)nslot * 6 * 8, st));
-  HIPCHK(hipMemsetAsync(queue, 0, 2 * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(queue, 0, 4 * sizeof(int), st));
   {
     std::vector<int> neg(nslot, -1);
     HIPCHK(hipMemcpyAsync(ss.pix, neg.data(), nslot * sizeof(int), hipMemcpyHostToDevice, st));
@@ -871,170 +971,169 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
     // applied (convolve_response_with_gauss / _psf, then scale_rf)
     ra.mask = (th && (th->broadening > 0.0 || th->npsf > 0)) ? nullptr : smask;
     ra.active = ss.needj;
-    long long pixel_iters = 0;
-    for (int it = 0;; ++it) {
+    // One LM iteration over all slots, enqueued on st: refill finished
+    // slots, count the active ones (and add them to the device total), then
+    // Jacobian, normal equations, trial and accept/reject.  Every kernel
+    // skips inactive slots, so an iteration after the last pixel finished
+    // changes nothing.
+    auto iteration = [&]() -> int {
       hipLaunchKernelGGL(refill_kernel<R>, dim3(nslot), dim3(256), 0, st, na, nslot, npix, nf, nd, nl, queue, ss, pstate,
                          pobs, mask_h ? pmask : (R *)nullptr, pnodes0, lm0, sstate, tstate, sobs, smask, snodes, onodes,
                          ochi2, oiter);
-      HIPCHK(hipMemsetAsync(count, 0, sizeof(int), st));
-      hipLaunchKernelGGL(count_active_kernel, dim3((nslot + 255) / 256), dim3(256), 0, st, nslot, ss.active, count);
-      int nact = 0;
-      HIPCHK(hipMemcpyAsync(&nact, count, sizeof(int), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      if (nact == 0) break;
-      pixel_iters += nact;
-      // model -> state and Jacobian where the model changed
-      hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, snodes,
-                         (size_t)nn, sstate, ns, nmap, ss.needj);
-      if (th && jacobian == SNAPI_JAC_DIFFERENCES) {
-        // central differences of the synthesis path over the 2n + 1 node
-        // sets, every set with its own temperature, hydrostatic equilibrium
-        // and equation of state (forward_evaluate per set)
-        const long nsv = (long)nslot * nv;
-        hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, fdn, hv,
-                           ss.needj);
-        for (int k = 0; k < nv; ++k) {
-          HIPCHK(hipMemcpy2DAsync(fdstate + (size_t)k * ns, nv * ns * es, sstate, ns * es, ns * es, nslot,
-                                  hipMemcpyDeviceToDevice, st));
-          hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf,
-                             fdn + (size_t)k * nn, (size_t)nv * nn, fdstate + (size_t)k * ns, (size_t)nv * ns,
-                             (R *)nullptr, ss.needj);
-        }
-        hipLaunchKernelGGL(expand_active_kernel, dim3((unsigned)((nsv + 255) / 256)), dim3(256), 0, st, nsv, nv,
-                           ss.needj, fd_active);
-        hipLaunchKernelGGL(tprofiles_kernel<R>, dim3((unsigned)nsv), dim3(128), 0, st, na, (int)nsv, nd, ns, fdstate,
-                           fdn, 1, fdTv, fdltv, fd_active);
-        rc = chem_hequilibrium(cdev_he, nsv, nd, fdTv, fdltv, fdNtv, fd_active, 1, st);
-        if (rc) goto done;
-        hipLaunchKernelGGL(tmaps_kernel<R>, dim3((unsigned)nsv), dim3(128), 0, st, na, (int)nsv, nd, nf, 1, fdTv,
-                           fdNtv, fdstate, (R *)nullptr, (R *)nullptr, fdsd, fd_active);
-        rc = chem_eos(cdev, nsv, nd, fdsd, nullptr, nullptr, fdescr, fd_active, st);
-        if (rc) goto done;
-        hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nsv * ns, fdsd,
-                           fdstate);
-        rc = synth_dev_masked(precision, lines, &ray_syn, (int)nsv, nd, fdstate, nl, dlam, Sall, ss.needj, nv, st);
-        if (rc) goto done;
-        hipLaunchKernelGGL(fdjac_kernel<R>, dim3(nslot), dim3(256), 0, st, na, nslot, nl, Sall, hv, smask, ws[0], ws[1],
-                           ws[2], ws[3], S, J, ss.needj);
-      } else if (th) {
-        // build_from_nodes at the nodes and at each temperature node +- 2.5 K,
-        // then the response path's state and derivatives (the reference's
-        // Jacobian and the current chi^2 come from that path)
-        // with the analytic maps (one profile per slot) an accepted trial
-        // leaves the temperature and equilibrium of the current nodes in
-        // Tv, ltv and Ntv: only refilled slots march
-        const int *hmask = na.analytic ? ss.march : ss.needj;
-        hipLaunchKernelGGL(tprofiles_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ns, sstate, snodes, nvar,
-                           Tv, ltv, hmask);
-        rc = chem_hequilibrium(cdev_he, (long)nslot * nvar, nd, Tv, ltv, Ntv, hmask, nvar, st);
-        if (rc) goto done;
-        if (na.analytic) {
-          if (ntn) {
-            hipLaunchKernelGGL(tderiv_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, snodes, ltv, dTd,
-                               nmap, ss.needj);
-            rc = chem_he_tangent(cdev_he, nslot, nd, Tv, ltv, Ntv, ntn, dTd, dNd, hscr, ss.needj, st);
-            if (rc) goto done;
+      hipLaunchKernelGGL(active_count_kernel, dim3(1), dim3(1024), 0, st, nslot, ss.active, count, total);
+        // model -> state and Jacobian where the model changed
+        hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, snodes,
+                           (size_t)nn, sstate, ns, nmap, ss.needj);
+        if (th && jacobian == SNAPI_JAC_DIFFERENCES) {
+          // central differences of the synthesis path over the 2n + 1 node
+          // sets, every set with its own temperature, hydrostatic equilibrium
+          // and equation of state (forward_evaluate per set)
+          const long nsv = (long)nslot * nv;
+          hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, fdn, hv,
+                             ss.needj);
+          for (int k = 0; k < nv; ++k) {
+            if (hipMemcpy2DAsync(fdstate + (size_t)k * ns, nv * ns * es, sstate, ns * es, ns * es, nslot,
+                                    hipMemcpyDeviceToDevice, st) != hipSuccess) return SNAPI_ERR_DEVICE;
+            hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf,
+                               fdn + (size_t)k * nn, (size_t)nv * nn, fdstate + (size_t)k * ns, (size_t)nv * ns,
+                               (R *)nullptr, ss.needj);
           }
-          hipLaunchKernelGGL(tmap2_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, dNd, nmap2, ss.needj);
-          hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, 1, Tv, Ntv, sstate,
-                             (R *)nullptr, (R *)nullptr, sd, ss.needj);
+          hipLaunchKernelGGL(expand_active_kernel, dim3((unsigned)((nsv + 255) / 256)), dim3(256), 0, st, nsv, nv,
+                             ss.needj, fd_active);
+          hipLaunchKernelGGL(tprofiles_kernel<R>, dim3((unsigned)nsv), dim3(128), 0, st, na, (int)nsv, nd, ns, fdstate,
+                             fdn, 1, fdTv, fdltv, fd_active);
+          rc = chem_hequilibrium(cdev_he, nsv, nd, fdTv, fdltv, fdNtv, fd_active, 1, st);
+          if (rc) return rc;
+          hipLaunchKernelGGL(tmaps_kernel<R>, dim3((unsigned)nsv), dim3(128), 0, st, na, (int)nsv, nd, nf, 1, fdTv,
+                             fdNtv, fdstate, (R *)nullptr, (R *)nullptr, fdsd, fd_active);
+          rc = chem_eos(cdev, nsv, nd, fdsd, nullptr, nullptr, fdescr, fd_active, st);
+          if (rc) return rc;
+          hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nsv * ns, fdsd,
+                             fdstate);
+          rc = synth_dev_masked(precision, lines, &ray_syn, (int)nsv, nd, fdstate, nl, dlam, Sall, ss.needj, nv, st);
+          if (rc) return rc;
+          hipLaunchKernelGGL(fdjac_kernel<R>, dim3(nslot), dim3(256), 0, st, na, nslot, nl, Sall, hv, smask, ws[0], ws[1],
+                             ws[2], ws[3], S, J, ss.needj);
+        } else if (th) {
+          // build_from_nodes at the nodes and at each temperature node +- 2.5 K,
+          // then the response path's state and derivatives (the reference's
+          // Jacobian and the current chi^2 come from that path)
+          // with the analytic maps (one profile per slot) an accepted trial
+          // leaves the temperature and equilibrium of the current nodes in
+          // Tv, ltv and Ntv: only refilled slots march
+          const int *hmask = na.analytic ? ss.march : ss.needj;
+          hipLaunchKernelGGL(tprofiles_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ns, sstate, snodes, nvar,
+                             Tv, ltv, hmask);
+          rc = chem_hequilibrium(cdev_he, (long)nslot * nvar, nd, Tv, ltv, Ntv, hmask, nvar, st);
+          if (rc) return rc;
+          if (na.analytic) {
+            if (ntn) {
+              hipLaunchKernelGGL(tderiv_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, snodes, ltv, dTd,
+                                 nmap, ss.needj);
+              rc = chem_he_tangent(cdev_he, nslot, nd, Tv, ltv, Ntv, ntn, dTd, dNd, hscr, ss.needj, st);
+              if (rc) return rc;
+            }
+            hipLaunchKernelGGL(tmap2_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, dNd, nmap2, ss.needj);
+            hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, 1, Tv, Ntv, sstate,
+                               (R *)nullptr, (R *)nullptr, sd, ss.needj);
+          } else {
+            hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, nvar, Tv, Ntv, sstate,
+                               nmap, nmap2, sd, ss.needj);
+          }
+          rc = chem_eos(cdev, nslot, nd, sd, rsd, dsd, escr, ss.needj, st);
+          if (rc) return rc;
+          hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, rsd, rstate);
+          hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * 2 * ns, dsd,
+                             rdst);
+          if (jacobian == SNAPI_JAC_TANGENT)
+            rc = launch_tangent<R>(kl, a_resp, ra, nslot, rstate, rdst, dlam, S, J, st);
+          else
+            rc = launch_resp<R>(kl, a_resp, ra, nslot, rstate, rdst, dlam, S, J, scr, chunk, st);
+          if (rc) return rc;
+          // the Gaussian, or else the PSF, on the current spectrum and the
+          // Jacobian (atmos_fit.cc:104-115)
+          if (th->broadening > 0.0) {
+            rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, S, ss.needj,
+                                          st);
+            if (!rc)
+              rc = spectral_convolve_launch(precision, nslot, nl, 4 * nn, lam_host.data(), dlam, th->broadening, J,
+                                            ss.needj, st);
+            if (rc) return rc;
+            hipLaunchKernelGGL(mask_jacobian_kernel<R>, dim3(nslot), dim3(256), 0, st, nl, 4 * nn, smask, J, ss.needj);
+          } else if (th->npsf > 0) {
+            rc = psf_convolve_launch(precision, nslot, nl, 4, th->npsf, dpsf, S, ss.needj, st);
+            if (!rc) rc = psf_convolve_launch(precision, nslot, nl, 4 * nn, th->npsf, dpsf, J, ss.needj, st);
+            if (rc) return rc;
+            hipLaunchKernelGGL(mask_jacobian_kernel<R>, dim3(nslot), dim3(256), 0, st, nl, 4 * nn, smask, J, ss.needj);
+          }
+        } else if (jacobian == SNAPI_JAC_ADJOINT) {
+          rc = launch_resp<R>(kl, a, ra, nslot, sstate, dzero, dlam, S, J, scr, chunk, st);
+          if (rc) return rc;
+        } else if (jacobian == SNAPI_JAC_TANGENT) {
+          rc = launch_tangent<R>(kl, a, ra, nslot, sstate, nullptr, dlam, S, J, st);
+          if (rc) return rc;
         } else {
-          hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, nvar, Tv, Ntv, sstate,
-                             nmap, nmap2, sd, ss.needj);
+          hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, fdn, hv,
+                             ss.needj);
+          // (2n+1) copies of every slot's state with the node sets applied
+          for (int k = 0; k < nv; ++k) {
+            if (hipMemcpy2DAsync(fdstate + (size_t)k * ns, nv * ns * es, sstate, ns * es, ns * es, nslot,
+                                    hipMemcpyDeviceToDevice, st) != hipSuccess) return SNAPI_ERR_DEVICE;
+            hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf,
+                               fdn + (size_t)k * nn, (size_t)nv * nn, fdstate + (size_t)k * ns, (size_t)nv * ns,
+                               (R *)nullptr, ss.needj);
+          }
+          // fdstate is [slot][node set]: one synthesis over nslot * (2n+1) columns
+          rc = synth_dev_masked(precision, lines, ray, nslot * nv, nd, fdstate, nl, dlam, Sall, ss.needj, nv, st);
+          if (rc) return rc;
+          hipLaunchKernelGGL(fdjac_kernel<R>, dim3(nslot), dim3(256), 0, st, na, nslot, nl, Sall, hv, smask, ws[0], ws[1],
+                             ws[2], ws[3], S, J, ss.needj);
         }
-        rc = chem_eos(cdev, nslot, nd, sd, rsd, dsd, escr, ss.needj, st);
-        if (rc) goto done;
-        hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, rsd, rstate);
-        hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * 2 * ns, dsd,
-                           rdst);
-        if (jacobian == SNAPI_JAC_TANGENT)
-          rc = launch_tangent<R>(kl, a_resp, ra, nslot, rstate, rdst, dlam, S, J, st);
-        else
-          rc = launch_resp<R>(kl, a_resp, ra, nslot, rstate, rdst, dlam, S, J, scr, chunk, st);
-        if (rc) goto done;
-        // the Gaussian, or else the PSF, on the current spectrum and the
-        // Jacobian (atmos_fit.cc:104-115)
-        if (th->broadening > 0.0) {
-          rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, S, ss.needj,
+        // chi^2 of the current model (where it changed)
+        hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, S, sobs, smask, ws[0], ws[1],
+                           ws[2], ws[3], r, ss.chi2, ss.needj);
+        // normal equations and damped solve
+        launch_lm_normal<R>(nslot, nl * 4, nn, J, r, JTJ, rhs, nullptr, ss.active, st);
+        launch_lm_solve(nslot, nn, JTJ, rhs, ss.lm, corr, ss.active, st, dreg);
+        // trial model and its synthesis
+        hipLaunchKernelGGL(trial_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, corr, tnodes,
+                           ss.active);
+        hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, tnodes,
+                           (size_t)nn, tstate, ns, (R *)nullptr, ss.active);
+        if (th) {
+          // the trial's thermodynamics: temperature, hydrostatic equilibrium,
+          // the synthesis path's equation of state (forward_evaluate)
+          hipLaunchKernelGGL(tprofiles_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ns, tstate, tnodes, 1,
+                             Tv, ltv, ss.active);
+          rc = chem_hequilibrium(cdev_he, nslot, nd, Tv, ltv, Ntv, ss.active, 1, st);
+          if (rc) return rc;
+          hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, 1, Tv, Ntv, tstate,
+                             (R *)nullptr, (R *)nullptr, tsd, ss.active);
+          rc = chem_eos(cdev, nslot, nd, tsd, nullptr, nullptr, escr, ss.active, st);
+          if (rc) return rc;
+          hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, tsd, tstate);
+        }
+        rc = synth_dev_masked(precision, lines, th ? &ray_syn : ray, nslot, nd, tstate, nl, dlam, St, ss.active, 1, st);
+        if (rc) return rc;
+        if (th && th->broadening > 0.0) {  // forward_evaluate (atmos_fit.cc:717-737): Gaussian, then PSF
+          rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, St, ss.active,
                                         st);
-          if (!rc)
-            rc = spectral_convolve_launch(precision, nslot, nl, 4 * nn, lam_host.data(), dlam, th->broadening, J,
-                                          ss.needj, st);
-          if (rc) goto done;
-          hipLaunchKernelGGL(mask_jacobian_kernel<R>, dim3(nslot), dim3(256), 0, st, nl, 4 * nn, smask, J, ss.needj);
-        } else if (th->npsf > 0) {
-          rc = psf_convolve_launch(precision, nslot, nl, 4, th->npsf, dpsf, S, ss.needj, st);
-          if (!rc) rc = psf_convolve_launch(precision, nslot, nl, 4 * nn, th->npsf, dpsf, J, ss.needj, st);
-          if (rc) goto done;
-          hipLaunchKernelGGL(mask_jacobian_kernel<R>, dim3(nslot), dim3(256), 0, st, nl, 4 * nn, smask, J, ss.needj);
+          if (rc) return rc;
         }
-      } else if (jacobian == SNAPI_JAC_ADJOINT) {
-        rc = launch_resp<R>(kl, a, ra, nslot, sstate, dzero, dlam, S, J, scr, chunk, st);
-        if (rc) goto done;
-      } else if (jacobian == SNAPI_JAC_TANGENT) {
-        rc = launch_tangent<R>(kl, a, ra, nslot, sstate, nullptr, dlam, S, J, st);
-        if (rc) goto done;
-      } else {
-        hipLaunchKernelGGL(fd_nodes_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, fdn, hv,
-                           ss.needj);
-        // (2n+1) copies of every slot's state with the node sets applied
-        for (int k = 0; k < nv; ++k) {
-          HIPCHK(hipMemcpy2DAsync(fdstate + (size_t)k * ns, nv * ns * es, sstate, ns * es, ns * es, nslot,
-                                  hipMemcpyDeviceToDevice, st));
-          hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf,
-                             fdn + (size_t)k * nn, (size_t)nv * nn, fdstate + (size_t)k * ns, (size_t)nv * ns,
-                             (R *)nullptr, ss.needj);
+        if (th && th->npsf > 0) {
+          rc = psf_convolve_launch(precision, nslot, nl, 4, th->npsf, dpsf, St, ss.active, st);
+          if (rc) return rc;
         }
-        // fdstate is [slot][node set]: one synthesis over nslot * (2n+1) columns
-        rc = synth_dev_masked(precision, lines, ray, nslot * nv, nd, fdstate, nl, dlam, Sall, ss.needj, nv, st);
-        if (rc) goto done;
-        hipLaunchKernelGGL(fdjac_kernel<R>, dim3(nslot), dim3(256), 0, st, na, nslot, nl, Sall, hv, smask, ws[0], ws[1],
-                           ws[2], ws[3], S, J, ss.needj);
-      }
-      // chi^2 of the current model (where it changed)
-      hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, S, sobs, smask, ws[0], ws[1],
-                         ws[2], ws[3], r, ss.chi2, ss.needj);
-      // normal equations and damped solve
-      launch_lm_normal<R>(nslot, nl * 4, nn, J, r, JTJ, rhs, nullptr, ss.active, st);
-      launch_lm_solve(nslot, nn, JTJ, rhs, ss.lm, corr, ss.active, st, dreg);
-      // trial model and its synthesis
-      hipLaunchKernelGGL(trial_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, snodes, corr, tnodes,
-                         ss.active);
-      hipLaunchKernelGGL(nodes_to_state_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, tnodes,
-                         (size_t)nn, tstate, ns, (R *)nullptr, ss.active);
-      if (th) {
-        // the trial's thermodynamics: temperature, hydrostatic equilibrium,
-        // the synthesis path's equation of state (forward_evaluate)
-        hipLaunchKernelGGL(tprofiles_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ns, tstate, tnodes, 1,
-                           Tv, ltv, ss.active);
-        rc = chem_hequilibrium(cdev_he, nslot, nd, Tv, ltv, Ntv, ss.active, 1, st);
-        if (rc) goto done;
-        hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, 1, Tv, Ntv, tstate,
-                           (R *)nullptr, (R *)nullptr, tsd, ss.active);
-        rc = chem_eos(cdev, nslot, nd, tsd, nullptr, nullptr, escr, ss.active, st);
-        if (rc) goto done;
-        hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, tsd, tstate);
-      }
-      rc = synth_dev_masked(precision, lines, th ? &ray_syn : ray, nslot, nd, tstate, nl, dlam, St, ss.active, 1, st);
-      if (rc) goto done;
-      if (th && th->broadening > 0.0) {  // forward_evaluate (atmos_fit.cc:717-737): Gaussian, then PSF
-        rc = spectral_convolve_launch(precision, nslot, nl, 4, lam_host.data(), dlam, th->broadening, St, ss.active,
-                                      st);
-        if (rc) goto done;
-      }
-      if (th && th->npsf > 0) {
-        rc = psf_convolve_launch(precision, nslot, nl, 4, th->npsf, dpsf, St, ss.active, st);
-        if (rc) goto done;
-      }
-      hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, St, sobs, smask, ws[0], ws[1],
-                         ws[2], ws[3], (R *)nullptr, ss.chi2t, ss.active);
+        hipLaunchKernelGGL(residual_kernel<R>, dim3(nslot), dim3(256), 0, st, nslot, nl, St, sobs, smask, ws[0], ws[1],
+                           ws[2], ws[3], (R *)nullptr, ss.chi2t, ss.active);
       hipLaunchKernelGGL(update_kernel, dim3((nslot + 127) / 128), dim3(128), 0, st, na, nslot, max_iter, ss, snodes,
                          tnodes);
-      if (hipGetLastError() != hipSuccess) {
-        rc = SNAPI_ERR_DEVICE;
-        goto done;
-      }
-    }
+      return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+    };
+    rc = run_iterations(iteration, st, count, pinned);
+    if (rc) goto done;
+    long long pixel_iters = 0;
+    HIPCHK(hipMemcpyAsync(&pixel_iters, total, sizeof(long long), hipMemcpyDeviceToHost, st));
     if (hipStreamSynchronize(st) != hipSuccess) {
       rc = SNAPI_ERR_DEVICE;
       goto done;
@@ -1060,6 +1159,7 @@ done:
   hipFree(dTd); hipFree(dNd); hipFree(hscr);
   hipFree(rstate); hipFree(rdst); hipFree(nmap2);
   hipFree(dreg); hipFree(dpsf);
+  if (pinned) (void)hipHostFree(pinned);
   return rc;
 }
 
@@ -1383,7 +1483,7 @@ extern "C" int snapi_invert(int precision, int jacobian, const snapi_line_table 
   NodeArgs na;
   int rc = setup_nodes(lines, ray, model, npix, nd, nlambda, jacobian, max_iter, na);
   if (rc) return rc;
-  if (nslot <= 0 || nslot > npix) nslot = npix;
+  if (nslot > npix) nslot = npix;  // nslot <= 0: chosen from the free device memory
   double G[SNAPI_MAX_NODES * SNAPI_MAX_NODES];
   const double *reg = reg_matrix(model, na, G) ? G : nullptr;
   if (precision == SNAPI_FP64)
@@ -1408,7 +1508,7 @@ extern "C" int snapi_invert_dev(int precision, int jacobian, const snapi_line_ta
   NodeArgs na;
   int rc = setup_nodes(lines, ray, model, npix, nd, nlambda, jacobian, max_iter, na);
   if (rc) return rc;
-  if (nslot <= 0 || nslot > npix) nslot = npix;
+  if (nslot > npix) nslot = npix;  // nslot <= 0: chosen from the free device memory
   hipStream_t st = (hipStream_t)stream;
   double G[SNAPI_MAX_NODES * SNAPI_MAX_NODES];
   const double *reg = reg_matrix(model, na, G) ? G : nullptr;
@@ -1479,7 +1579,7 @@ extern "C" int snapi_invert_eos_dev(int precision, int jacobian, const snapi_lin
   int ntn = 0;
   for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
   if (ntn < 1) return SNAPI_ERR_ARG;
-  if (nslot <= 0 || nslot > npix) nslot = npix;
+  if (nslot > npix) nslot = npix;  // nslot <= 0: chosen from the free device memory
   ThermoArgs th{chem, chem_he, map, 0.0, 0, nullptr};
   double G[SNAPI_MAX_NODES * SNAPI_MAX_NODES];
   const double *reg = reg_matrix(model, na, G) ? G : nullptr;
@@ -1512,7 +1612,7 @@ extern "C" int snapi_invert_eos(int precision, int jacobian, const snapi_line_ta
   int ntn = 0;
   for (int n = 0; n < na.nnode; ++n) ntn += na.code[n] == NP_T;
   if (ntn < 1) return SNAPI_ERR_ARG;
-  if (nslot <= 0 || nslot > npix) nslot = npix;
+  if (nslot > npix) nslot = npix;  // nslot <= 0: chosen from the free device memory
   ThermoArgs th{chem, chem_he, map, 0.0, 0, nullptr};
   double G[SNAPI_MAX_NODES * SNAPI_MAX_NODES];
   const double *reg = reg_matrix(model, na, G) ? G : nullptr;

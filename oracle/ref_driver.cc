@@ -1048,6 +1048,147 @@ static int mode_nlteresp(int argc, char **argv, io_class &io) {
   return 0;
 }
 
+// nlterin <atmos.cfg>: the inputs of atom::add_response_contributions_new and
+// of the rate part of atom::compute_nlte_population_responses
+// (atmos/atmol/atom/atom_responses.cc:26-200, 668-1128), taken as
+// atmosphere::compute_nlte_population_responses (atmos/atmos_responses.cc:
+// 13-157) forms them after nltepops: per depth the state, the electron-density
+// and referent-opacity derivatives, the derivatives of the atom's total
+// population; per level the populations; per line the damping and its
+// temperature derivative; per level pair the collision rates and their full
+// derivatives; per ray and wavelength the projected velocity, the total
+// opacity and emissivity and the LTE background's perturbations.  At every
+// 40th wavelength also the atom's profiles and profile norms for checking.
+static int mode_nlterin(int argc, char **argv, io_class &io) {
+  if (argc < 3) return 2;
+  atmosphere *a = make_atmos(argv[2], io);
+  const int x1 = a->x1l, x2 = a->x2l;
+  a->set_grid(a->tau_grid);
+  a->boundary_condition_for_rt = -1;
+  a->popsetup();
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->set_parent_atmosphere(a);
+  if (a->tau_grid) a->compute_op_referent_derivative();
+  a->nltepops();
+  a->respsetup();
+  a->ne_lte_derivatives();
+  atom *at = 0;
+  for (int s = 0; s < a->natm; ++s) {
+    atom *t = dynamic_cast<atom *>(a->atml[s]);
+    if (t && t->NLTE) at = t;
+  }
+  if (!at) return 3;
+  fp_t *lambda = 0;
+  int32_t nlambda = 0;
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->rtsetup(a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  for (int s = 0; s < a->natm; ++s)
+    lambda = a->atml[s]->getlambda(lambda, nlambda, a->T[x1][x2][a->x3h], a->Nt[x1][x2][a->x3h], a->Ne[x1][x2][a->x3h]);
+  for (int s = 0; s < a->natm; ++s) a->atml[s]->rtinit();
+  int ntp;
+  fp_t *th, *ph;
+  fp_t *bin = a->anglesetup(th, ph, ntp);
+  fp_t *lambda_w = new fp_t[nlambda];
+  if (nlambda >= 2) {
+    lambda_w[0] = (lambda[1] - lambda[0]) * 0.5;
+    lambda_w[nlambda - 1] = (lambda[nlambda - 1] - lambda[nlambda - 2]) * 0.5;
+    for (int l = 1; l < nlambda - 1; ++l) lambda_w[l] = (lambda[l + 1] - lambda[l - 1]) * 0.5;
+  } else if (nlambda)
+    lambda_w[0] = 1.0;
+  const int nd = a->x3h - a->x3l + 1;
+  printf("nd %d\nntp %d\nnlam %d\nboundary %.17g\ntau_grid %d\n", nd, ntp, nlambda, a->boundary_condition_for_rt,
+         a->tau_grid);
+  printf("atom nmap %d ntr %d Z %d mass %.17g saha %.17g dT %.17g dN %.17g damp %d\n", at->nmap, at->ntr, (int)at->Z,
+         at->mass, saha_const, delta_T, delta_Nt_frac, (int)turn_on_damping);
+  for (int tp = 1; tp <= ntp; ++tp) printf("ray %d %.17g %.17g\n", tp, th[tp], bin[tp]);
+  for (int l = 0; l < nlambda; ++l) printf("lam %d %.17g %.17g\n", l, lambda[l], lambda_w[l]);
+  for (int i = 0; i < at->nmap; ++i) {
+    const int z = at->zmap[i], lv = at->lmap[i];
+    printf("level %d %d %d %.17g %d %.17g\n", i, z, lv, at->ee[z][lv], (int)at->g[z][lv], z < at->Z ? at->ip[z] : 0.0);
+  }
+  for (int tr = 1; tr <= (int)at->ntr; ++tr) {
+    const int z = at->inverse_tmap[tr][2], lo = at->inverse_tmap[tr][3], up = at->inverse_tmap[tr][4];
+    if (up < at->nl[z]) {
+      printf("trans %d line %d %d %d %d %.17g %.17g %.17g\n", tr, z, lo, up, (int)at->rmap[z][up], at->A[z][up][lo],
+             at->B[z][lo][up], at->B[z][up][lo]);
+    } else {
+      printf("trans %d bf %d %d %d %d %.17g\n", tr, z, lo, up, (int)at->rmap[z + 1][0],
+             at->bf[z][lo] ? at->bf[z][lo]->getlambda_crit() : 0.0);
+      for (int l = 0; l < nlambda; ++l)
+        printf("sigma %d %d %.17g\n", tr, l, at->bf[z][lo] ? at->bf[z][lo]->U(lambda[l]) : 0.0);
+    }
+  }
+  for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) {
+    const fp_t T = a->T[x1][x2][x3i], ne = a->Ne[x1][x2][x3i];
+    const fp_t dT = at->derivative_active_population(x1, x2, x3i);
+    const fp_t dN = at->derivative_active_population_density(x1, x2, x3i);
+    printf("depth %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", x3i - a->x3l,
+           T, a->Vt[x1][x2][x3i], ne, a->Nt[x1][x2][x3i], a->tau_grid ? a->op_referent[x1][x2][x3i] : 1.0,
+           a->rt_grid[x3i], a->get_ne_lte_derivative(1, x1, x2, x3i), a->get_ne_lte_derivative(2, x1, x2, x3i),
+           a->tau_grid ? a->get_op_referent_der(1, x3i, x1, x2, x3i) : 0.0,
+           a->tau_grid ? a->get_op_referent_der(2, x3i, x1, x2, x3i) : 0.0,
+           a->tau_grid ? a->get_op_referent_der(3, x3i, x1, x2, x3i) : 0.0, dT, dN,
+           at->fetch_population(x1, x2, x3i, 0, 0));
+    for (int i = 0; i < at->nmap; ++i)
+      printf("pop %d %d %.17g\n", x3i - a->x3l, i, at->pop[x1][x2][x3i].n[at->zmap[i]][at->lmap[i]]);
+    for (int tr = 1; tr <= (int)at->ntr; ++tr) {
+      const int z = at->inverse_tmap[tr][2], lo = at->inverse_tmap[tr][3], up = at->inverse_tmap[tr][4];
+      if (up < at->nl[z]) {
+        const fp_t lam0 = fabs(h * c / (at->ee[z][up] - at->ee[z][lo]));
+        printf("damp %d %d %.17g %.17g %.17g\n", tr, x3i - a->x3l, at->damp_rad(at->A[z], lo, up),
+               at->damp_col(x1, x2, x3i, z, up, lo, T, ne, lam0), at->damp_col_der_T(x1, x2, x3i, z, up, lo, T, ne, lam0));
+      }
+    }
+    for (int i = 0; i < at->nmap; ++i)
+      for (int j = 0; j < at->nmap; ++j)
+        printf("coll %d %d %d %.17g %.17g %.17g\n", x3i - a->x3l, i, j, at->collisional_rates(x1, x2, x3i, i, j),
+               at->derivative_collisions_full_temp(x1, x2, x3i, i, j),
+               at->derivative_collisions_full_density(x1, x2, x3i, i, j));
+  }
+  // per ray and wavelength, in the order of the responses' loop
+  for (int tp = 1; tp <= ntp; ++tp) {
+    fp_t ***Vr = a->project(a->Vx, a->Vy, a->Vz, th[tp], ph[tp], a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    fp_t ****Bv = a->transform(a->Bx, a->By, a->Bz, th[tp], ph[tp], a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    for (int x3i = a->x3l; x3i <= a->x3h; ++x3i) printf("vr %d %d %.17g\n", tp, x3i - a->x3l, Vr[x1][x2][x3i]);
+    at->compute_profile_norm(th[tp], ph[tp], lambda, lambda_w, Vr, nlambda);
+    for (int x3i = a->x3l; x3i <= a->x3h; ++x3i)
+      for (int tr = 1; tr <= (int)at->ntr; ++tr)
+        printf("norm %d %d %d %.17g %.17g %.17g %.17g\n", tp, x3i - a->x3l, tr, at->norm[x1][x2][x3i][tr],
+               at->norm_derivative[1][x1][x2][x3i][tr], at->norm_derivative[2][x1][x2][x3i][tr],
+               at->norm_derivative[3][x1][x2][x3i][tr]);
+    for (int l = 0; l < nlambda; ++l) {
+      for (int s = 0; s < a->natm; ++s) a->atml[s]->prof_init();
+      fp_t ***op = a->opacity(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]);
+      fp_t ***em = a->emissivity(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]);
+      fp_t *****opp = a->opacity_pert_lte(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]);
+      fp_t *****emp = a->emissivity_pert_lte(a->T, a->Ne, Vr, a->Vt, Bv, th[tp], ph[tp], lambda[l]);
+      for (int x3i = a->x3l; x3i <= a->x3h; ++x3i)
+        printf("pp %d %d %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", tp, l, x3i - a->x3l,
+               op[x1][x2][x3i], em[x1][x2][x3i], opp[1][x3i][x1][x2][x3i], opp[2][x3i][x1][x2][x3i],
+               opp[3][x3i][x1][x2][x3i], emp[1][x3i][x1][x2][x3i], emp[2][x3i][x1][x2][x3i], emp[3][x3i][x1][x2][x3i]);
+      if (l % 40 == 0)
+        for (int x3i = a->x3l; x3i <= a->x3h; ++x3i)
+          for (int tr = 1; tr <= (int)at->ntr; ++tr)
+            printf("prof %d %d %d %d %.17g\n", tp, l, x3i - a->x3l, tr, at->current_profile[x1][x2][x3i][tr]);
+      del_ft3dim(em, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+      del_ft3dim(op, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+      del_ft5dim(opp, 1, 7, a->x3l, a->x3h, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+      del_ft5dim(emp, 1, 7, a->x3l, a->x3h, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    }
+    del_ft4dim(Bv, 1, 3, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+    del_ft3dim(Vr, a->x1l, a->x1h, a->x2l, a->x2h, a->x3l, a->x3h);
+  }
+  // the system itself, for the record
+  a->compute_nlte_population_responses(0);
+  const int n = nd * at->nmap;
+  for (int r = 1; r <= n; ++r)
+    for (int cc = 1; cc <= n; ++cc)
+      if (at->response_matrix[r][cc] != 0.0) printf("M %d %d %.17g\n", r - 1, cc - 1, at->response_matrix[r][cc]);
+  for (int kk = 1; kk <= nd; ++kk)
+    for (int r = 1; r <= n; ++r)
+      printf("beta %d %d %.17g %.17g %.17g\n", kk - 1, r - 1, at->beta_Temp[kk][r], at->beta_density[kk][r],
+             at->beta_v_micro[kk][r]);
+  return 0;
+}
+
 // Node model from a text stream: "tau_min tau_max", the node count, then one
 // "code logtau value" line per node (code 1 T, 2 vt, 3 vLOS, 4 |B|,
 // 5 inclination [rad], 6 azimuth [rad]), grouped in that order as the
@@ -1282,6 +1423,7 @@ int main(int argc, char **argv) {
   if (!strcmp(argv[1], "nodes")) return mode_nodes(argc, argv, io);
   if (!strcmp(argv[1], "fit")) return mode_fit(argc, argv, io);
   if (!strcmp(argv[1], "psf")) return mode_psf(argc, argv, io);
+  if (!strcmp(argv[1], "nlterin")) return mode_nlterin(argc, argv, io);
   if (!strcmp(argv[1], "reg")) return mode_reg(argc, argv, io);
   if (!strcmp(argv[1], "time_synth")) return mode_time_synth(argc, argv, io);
   fprintf(stderr, "unknown mode %s\n", argv[1]);

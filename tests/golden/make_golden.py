@@ -927,6 +927,89 @@ def build_reg(d):
     print("reg: %d models" % len(REG_CASES))
 
 
+def build_nlterin(d):
+    """The inputs of the NLTE population-response system of the Ca II atom
+    in the cfg0 column (nltepops, then what compute_nlte_population_responses
+    feeds add_response_contributions_new and its rate part): state and
+    derivatives per depth, populations, line damping, collision rates and
+    their derivatives, per ray and wavelength the projected velocity, total
+    opacity and emissivity and the LTE background's perturbations; the
+    profiles and profile norms at every 40th wavelength; and the system
+    (M, beta) the reference assembles from them."""
+    nlte_dir(d)
+    out = run(d, ["nlterin", "atmos.cfg"])
+    head, rays, lams, levels, trans, sigma = {}, [], [], [], [], {}
+    depth, pops, damp, coll, vr, norm, pp, prof, Ms, betas = [], [], [], [], [], [], [], [], [], []
+    for ln in out:
+        t = ln.split()
+        k = t[0]
+        if k in ("nd", "ntp", "nlam", "tau_grid"):
+            head[k] = int(t[1])
+        elif k == "boundary":
+            head[k] = float(t[1])
+        elif k == "atom":
+            head.update(nmap=int(t[2]), ntr=int(t[4]), Z=int(t[6]), mass=float(t[8]), saha=float(t[10]),
+                        dT=float(t[12]), dN=float(t[14]), damp=int(t[16]))
+        elif k == "ray":
+            rays.append([float(t[2]), float(t[3])])
+        elif k == "lam":
+            lams.append([float(t[2]), float(t[3])])
+        elif k == "level":
+            levels.append([float(x) for x in t[1:]])
+        elif k == "trans":
+            if t[2] == "line":
+                trans.append([int(t[1]), 0, int(t[3]), int(t[4]), int(t[5]), int(t[6]), float(t[7]), float(t[8]),
+                              float(t[9]), 0.0])
+            else:
+                trans.append([int(t[1]), 1, int(t[3]), int(t[4]), int(t[5]), int(t[6]), 0.0, 0.0, 0.0, float(t[7])])
+        elif k == "sigma":
+            sigma.setdefault(int(t[1]), []).append(float(t[3]))
+        elif k == "depth":
+            depth.append([float(x) for x in t[2:]])
+        elif k == "pop":
+            pops.append(float(t[3]))
+        elif k == "damp":
+            damp.append([int(t[1]), int(t[2]), float(t[3]), float(t[4]), float(t[5])])
+        elif k == "coll":
+            coll.append([float(t[4]), float(t[5]), float(t[6])])
+        elif k == "vr":
+            vr.append(float(t[3]))
+        elif k == "norm":
+            norm.append([float(x) for x in t[4:]])
+        elif k == "pp":
+            pp.append([float(x) for x in t[4:]])
+        elif k == "prof":
+            prof.append(float(t[5]))
+        elif k == "M":
+            Ms.append((int(t[1]), int(t[2]), float(t[3])))
+        elif k == "beta":
+            betas.append((int(t[1]), int(t[2]), float(t[3]), float(t[4]), float(t[5])))
+    nd, ntp, nlam, nm, ntr = head["nd"], head["ntp"], head["nlam"], head["nmap"], head["ntr"]
+    n = nd * nm
+    M = np.zeros((n, n))
+    for r, c, v in Ms:
+        M[r, c] = v
+    beta = np.zeros((3, nd, n))
+    for kk, r, a, b, c in betas:
+        beta[:, kk, r] = (a, b, c)
+    dm = np.zeros((3, ntr, nd))
+    for tr, z, a, b, c in damp:
+        dm[:, tr - 1, z] = (a, b, c)
+    sg = np.zeros((ntr, nlam))
+    for tr, v in sigma.items():
+        sg[tr - 1] = v
+    nsample = len(range(0, nlam, 40))
+    recs = dict(head=np.array([nd, ntp, nlam, nm, ntr, head["Z"], head["tau_grid"], head["damp"]]),
+                consts=np.array([head["mass"], head["saha"], head["dT"], head["dN"], head["boundary"]]),
+                rays=np.array(rays), lam=np.array(lams), levels=np.array(levels), trans=np.array(trans), sigma=sg,
+                depth=np.array(depth), pops=np.array(pops).reshape(nd, nm), damp=dm,
+                coll=np.array(coll).reshape(nd, nm, nm, 3), vr=np.array(vr).reshape(ntp, nd),
+                norm=np.array(norm).reshape(ntp, nd, ntr, 4), pp=np.array(pp).reshape(ntp, nlam, nd, 8),
+                prof=np.array(prof).reshape(ntp, nsample, nd, ntr), M=M, beta=beta)
+    np.savez_compressed(os.path.join(HERE, "nlterin.npz"), **recs)
+    print("nlterin: %d rays x %d wavelengths, %d depths, %d levels" % (ntp, nlam, nd, nm))
+
+
 def build_tau(d):
     """The reference's own test atmosphere test/inverted_atmos.f0 (the
     falc_1000 column on its height grid as the reference wrote it back): its
@@ -940,7 +1023,7 @@ def build_tau(d):
 
 
 BUILDERS = ["cfg0", "cfg1_template", "resp", "formal_scalar", "lmstep", "interp", "conv", "ana", "nlte", "chemeq",
-            "nlteresp", "nodes", "fit", "tau", "psf", "reg", "fitreg"]
+            "nlteresp", "nodes", "fit", "tau", "psf", "reg", "fitreg", "nlterin"]
 
 
 def main():

@@ -48,6 +48,12 @@
  *   snapi_formal_responses*  atmos_ppbez::formal_with_responses    atmos/plpar/atmos_ppbez_rts.cc:349-778
  *   snapi_response_solve     the Crout solves of atom::compute_nlte_population_responses
  *                                                                   atmos/atmol/atom/atom_responses.cc:171-200
+ *   snapi_nlte_response_system* atmosphere::compute_nlte_population_responses with
+ *                            atom::compute_profile_norm, add_response_contributions_new and
+ *                            the rate part of atom::compute_nlte_population_responses
+ *                                                                   atmos/atmos_responses.cc:13-157,
+ *                                                                   atmos/atmol/atom/atom_rad.cc:885-930,
+ *                                                                   atmos/atmol/atom/atom_responses.cc:26-200, 668-1128
  *   snapi_chemeq*            atmosphere::ltepops: chemeq + atom::lte atmos/atmos_pop.cc:30-45,
  *                                                                   atmos/atmos_chm.cc:28-363,
  *                                                                   atmos/atmol/atom/atom.cc:779-787, 892-924
@@ -687,6 +693,100 @@ int snapi_formal_responses_dev(int ncol, int nprob, int nd, const double *grid_d
  *   M [nsys][n][n]; B, X [nsys][nrhs][n] (one right-hand side per row).
  */
 int snapi_response_solve(int nsys, int n, int nrhs, const double *M, const double *B, double *X);
+
+/*
+ * The NLTE population-response system of one NLTE atom:
+ * atmosphere::compute_nlte_population_responses (atmos/atmos_responses.cc:
+ * 13-157) with, per ray, atom::compute_profile_norm (atmos/atmol/atom/
+ * atom_rad.cc:885-930), per ray and wavelength the scalar formal solution
+ * with responses and atom::add_response_contributions_new
+ * (atom_responses.cc:668-1128), then the rate part of
+ * atom::compute_nlte_population_responses (atom_responses.cc:26-170).
+ *
+ * The atom: levels in map order, grouped by ion stage with consecutive
+ * indices within a stage; the top stage holds one level, the last.  Every
+ * pair of levels of one stage with nonzero Einstein coefficients is a line
+ * transition; each bound-free transition goes from a level of stage z to the
+ * ground level of stage z+1.
+ */
+typedef struct {
+  int nmap;                   /* levels */
+  int ntr;                    /* transitions */
+  int nstage;                 /* ion stages 0..nstage-1 (the reference's Z + 1) */
+  int damping;                /* turn_on_damping: collisional damping in the profiles (0/1) */
+  double mass;                /* atom mass [g] */
+  const int *level_stage;     /* [nmap] ion stage of each level */
+  const int *level_index;     /* [nmap] index of the level within its stage */
+  const double *level_energy; /* [nmap] excitation energy [erg] */
+  const double *level_g;      /* [nmap] statistical weight */
+  const double *stage_ip;     /* [nstage] ionisation energy of each stage [erg] */
+  const int *tr_type;         /* [ntr] SNAPI_TR_LINE or SNAPI_TR_BF */
+  const int *tr_lower;        /* [ntr] map index of the lower level */
+  const int *tr_upper;        /* [ntr] map index of the upper level (bound-free: next stage's ground) */
+  const double *tr_A;         /* [ntr] A_ul of the lines [s^-1] */
+  const double *tr_Blu;       /* [ntr] B_lu of the lines */
+  const double *tr_Bul;       /* [ntr] B_ul of the lines */
+  const double *tr_damp_rad;  /* [ntr] radiative damping of the lines (atom::damp_rad) */
+  const double *tr_lambda_crit; /* [ntr] edge wavelength of the bound-free transitions [cm] */
+} snapi_nlte_atom;
+
+/* Fields of the per-depth input of the response system, [SNAPI_NLTE_NFIELD][nd]
+ * per column: temperature [K], microturbulence [cm/s], electron density,
+ * total particle density Nt, referent opacity, the formal solver's grid
+ * (rt_grid), the electron-density derivatives d ne/dT and d ne/dNt
+ * (atmosphere::get_ne_lte_derivative 1, 2), the referent-opacity derivatives
+ * d op_ref/dT, /dNt, /dvt (get_op_referent_der 1-3, local), and the
+ * derivatives of the atom's total population d N/dT, d N/dNt
+ * (atom::derivative_active_population[_density]). */
+enum {
+  SNAPI_NLTE_T = 0, SNAPI_NLTE_VT, SNAPI_NLTE_NE, SNAPI_NLTE_NT, SNAPI_NLTE_OPREF, SNAPI_NLTE_GRID,
+  SNAPI_NLTE_DNE_DT, SNAPI_NLTE_DNE_DNT, SNAPI_NLTE_DOPREF_DT, SNAPI_NLTE_DOPREF_DNT, SNAPI_NLTE_DOPREF_DVT,
+  SNAPI_NLTE_DNACT_DT, SNAPI_NLTE_DNACT_DNT, SNAPI_NLTE_NFIELD
+};
+
+/*
+ * snapi_nlte_response_system: the system (M, beta) of ncol columns, and
+ * optionally its solution, the level responses.
+ *   atom: as above; tau_grid: 1 when the formal solver runs on the tau grid
+ *     (op and em normalised by the referent opacity, whose derivatives enter);
+ *   boundary: lower boundary of the formal solution (-1: I = S);
+ *   ray_theta, ray_weight [nray]: the angular quadrature (anglesetup);
+ *   lambda, lambda_weight [nlam]: the NLTE wavelength grid and its
+ *     trapezoidal weights;
+ *   sigma [ntr][nlam]: bound-free cross-section of each bound-free transition
+ *     at each wavelength (the bf object's U(lambda); rows of lines unused);
+ *   depth [ncol][SNAPI_NLTE_NFIELD][nd];
+ *   pops [ncol][nmap][nd]: level populations;
+ *   damp [ncol][2][ntr][nd]: collisional damping of each line
+ *     (atom::damp_col) and its temperature derivative (damp_col_der_T);
+ *   coll [ncol][3][nmap][nmap][nd]: collisional rate from level i to j
+ *     (atom::collisional_rates) and its full derivatives to T and Nt
+ *     (derivative_collisions_full_temp / _density);
+ *   vr [ncol][nray][nd]: line-of-sight velocity of each ray (project);
+ *   opem [ncol][nray][nlam][8][nd]: total opacity, emissivity, and the LTE
+ *     perturbations of opacity (T, Nt, vt) and emissivity (T, Nt, vt) at each
+ *     depth (opacity_pert_lte / emissivity_pert_lte, local);
+ *   M [ncol][n][n], beta [ncol][3][nd][n] out, n = nd * nmap: response_matrix
+ *     and beta_Temp / beta_density / beta_v_micro (one right-hand side per
+ *     perturbed depth);
+ *   resp [ncol][3][n][nd] out (may be NULL): level_responses, the solution
+ *     for every right-hand side (snapi_response_solve of M and beta).
+ * 2 <= nd <= 256, nmap <= 16, ntr <= 64.  Double throughout.
+ */
+int snapi_nlte_response_system(const snapi_nlte_atom *atom, int ncol, int nd, int nray, int nlam, int tau_grid,
+                               double boundary, const double *ray_theta, const double *ray_weight,
+                               const double *lambda, const double *lambda_weight, const double *sigma,
+                               const double *depth, const double *pops, const double *damp, const double *coll,
+                               const double *vr, const double *opem, double *M, double *beta, double *resp);
+/* The same on device arrays (every array argument except the atom's tables is
+ * a device pointer), enqueued on stream. */
+int snapi_nlte_response_system_dev(const snapi_nlte_atom *atom, int ncol, int nd, int nray, int nlam, int tau_grid,
+                                   double boundary, const double *ray_theta, const double *ray_weight,
+                                   const double *lambda_dev, const double *lambda_weight_dev,
+                                   const double *sigma_dev, const double *depth_dev, const double *pops_dev,
+                                   const double *damp_dev, const double *coll_dev, const double *vr_dev,
+                                   const double *opem_dev, double *M_dev, double *beta_dev, double *resp_dev,
+                                   void *stream);
 
 /*
  * Gaussian spectral convolution (convolve_spectra_with_gauss,

@@ -1083,6 +1083,15 @@ static int mode_nlterin(int argc, char **argv, io_class &io) {
   for (int s = 0; s < a->natm; ++s)
     lambda = a->atml[s]->getlambda(lambda, nlambda, a->T[x1][x2][a->x3h], a->Nt[x1][x2][a->x3h], a->Ne[x1][x2][a->x3h]);
   for (int s = 0; s < a->natm; ++s) a->atml[s]->rtinit();
+  // as atmos_responses.cc:35-44: the LTE atoms' population responses enter
+  // their opacity perturbations
+  for (int s = 0; s < a->natm; ++s)
+    if (a->atml[s]->check_if_nlte()) a->atml[s]->responses_init();
+  for (int s = 0; s < a->natm; ++s)
+    if (!a->atml[s]->check_if_nlte()) {
+      a->atml[s]->responses_init();
+      a->atml[s]->compute_lte_population_responses_analytical(a->T, a->Ne);
+    }
   int ntp;
   fp_t *th, *ph;
   fp_t *bin = a->anglesetup(th, ph, ntp);

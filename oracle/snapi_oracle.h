@@ -115,6 +115,21 @@ int orc_formal_responses(int nd, const double *grid, const double *op, const dou
                          double boundary, double *I, double *L, double *Rop, double *Rem);
 int orc_response_solve(int n, int nrhs, const double *M, const double *B, double *X);
 
+/* NLTE population-response system (snapi_oracle_nlteasm.c): the assembly of
+ * the reference's response matrix and right-hand sides for one column, with
+ * the layouts of snapi_nlte_response_system; and two of its pieces, the
+ * profile norms of one ray and the line profiles at one wavelength. */
+int orc_nlte_response_system(const snapi_nlte_atom *atom, int nd, int nray, int nlam, int tau_grid, double boundary,
+                             const double *ray_theta, const double *ray_weight, const double *lambda,
+                             const double *lambda_weight, const double *sigma, const double *depth,
+                             const double *pops, const double *damp, const double *coll, const double *vr,
+                             const double *opem, double *M, double *beta);
+int orc_nlte_profile_norm(const snapi_nlte_atom *atom, int nd, int nlam, const double *lambda,
+                          const double *lambda_weight, const double *depth, const double *damp, const double *vr,
+                          double *norm, double *normd);
+int orc_nlte_line_profiles(const snapi_nlte_atom *atom, int nd, double lambda, const double *depth,
+                           const double *damp, const double *vr, double *prof);
+
 #ifdef __cplusplus
 }
 #endif

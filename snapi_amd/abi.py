@@ -97,6 +97,42 @@ def make_ali_atom(nlevel, mass, trans):
     return a
 
 
+class SnapiNlteAtom(C.Structure):
+    _fields_ = [("nmap", C.c_int), ("ntr", C.c_int), ("nstage", C.c_int), ("damping", C.c_int),
+                ("mass", C.c_double), ("level_stage", C.POINTER(C.c_int)), ("level_index", C.POINTER(C.c_int)),
+                ("level_energy", C.POINTER(C.c_double)), ("level_g", C.POINTER(C.c_double)),
+                ("stage_ip", C.POINTER(C.c_double)), ("tr_type", C.POINTER(C.c_int)),
+                ("tr_lower", C.POINTER(C.c_int)), ("tr_upper", C.POINTER(C.c_int)), ("tr_A", C.POINTER(C.c_double)),
+                ("tr_Blu", C.POINTER(C.c_double)), ("tr_Bul", C.POINTER(C.c_double)),
+                ("tr_damp_rad", C.POINTER(C.c_double)), ("tr_lambda_crit", C.POINTER(C.c_double))]
+
+
+NLTE_FIELDS = ("T", "vt", "ne", "Nt", "opref", "grid", "dne_dT", "dne_dNt", "dopref_dT", "dopref_dNt", "dopref_dvt",
+               "dNact_dT", "dNact_dNt")
+
+
+def make_nlte_atom(level_stage, level_energy, level_g, stage_ip, tr_type, tr_lower, tr_upper, tr_A, tr_Blu, tr_Bul,
+                   tr_damp_rad, tr_lambda_crit, mass, damping=1):
+    """snapi_nlte_atom from per-level and per-transition arrays (map order;
+    level_index is derived from level_stage).  The arrays are kept alive on
+    the returned structure (attribute _keep)."""
+    st = np.ascontiguousarray(level_stage, dtype=np.int32)
+    idx = np.zeros_like(st)
+    for i in range(1, st.size):
+        idx[i] = idx[i - 1] + 1 if st[i] == st[i - 1] else 0
+    ints = [st, idx] + [np.ascontiguousarray(x, dtype=np.int32) for x in (tr_type, tr_lower, tr_upper)]
+    dbls = [np.ascontiguousarray(x, dtype=np.float64) for x in (level_energy, level_g, stage_ip, tr_A, tr_Blu, tr_Bul,
+                                                                 tr_damp_rad, tr_lambda_crit)]
+    a = SnapiNlteAtom()
+    a.nmap, a.ntr, a.nstage, a.damping, a.mass = st.size, ints[2].size, dbls[2].size, int(damping), float(mass)
+    ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    a.level_stage, a.level_index, a.tr_type, a.tr_lower, a.tr_upper = (ip(x) for x in ints)
+    (a.level_energy, a.level_g, a.stage_ip, a.tr_A, a.tr_Blu, a.tr_Bul, a.tr_damp_rad,
+     a.tr_lambda_crit) = (ptr(x) for x in dbls)
+    a._keep = ints + dbls
+    return a
+
+
 def make_ali_grid(lam, lamw, fudge, sigma, theta, wray, bg_per_ray, boundary=-1.0, tau_grid=1):
     """The arrays are kept alive on the returned structure (attribute _keep)."""
     arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (lam, lamw, fudge, sigma, theta, wray)]

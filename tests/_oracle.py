@@ -16,8 +16,8 @@ def lib():
     global _LIB
     if _LIB is None:
         path = os.path.join(REPO, "oracle", "liboracle.so")
-        if not os.path.exists(path):
-            subprocess.run(["make", "-s", "-f", os.path.join(REPO, "oracle", "Makefile")], check=True)
+        # make rebuilds only what a changed source needs
+        subprocess.run(["make", "-s", "-f", os.path.join(REPO, "oracle", "Makefile")], check=True)
         L = C.CDLL(path)
         dp = C.POINTER(C.c_double)
         L.orc_voigt.argtypes = [C.c_double, C.c_double, dp, dp]
@@ -887,3 +887,45 @@ def eos_state(table, smap, state, nthreads=0):
     L.orc_eos_state_batch(C.byref(table), C.byref(smap), ncol, nd, ptr(st), ptr(resp), ptr(ds),
                           nthreads if nthreads else (os.cpu_count() or 1))
     return st, resp, ds
+
+
+def nlte_response_system(atom, c, opem=None):
+    """The oracle's assembly of one column: (M [n][n], beta [3][nd][n]) for
+    the case dict of tests/_nlte_case.py (opem overrides the case's)."""
+    L_ = lib()
+    dp = C.POINTER(C.c_double)
+    L_.orc_nlte_response_system.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double] + [dp] * 13
+    L_.orc_nlte_response_system.restype = C.c_int
+    nd, n = c["nd"], c["nd"] * c["nmap"]
+    M = np.zeros((n, n))
+    beta = np.zeros((3, nd, n))
+    arrs = [_c(c[k]) for k in ("theta", "wray", "lam", "lamw", "sigma", "depth", "pops", "damp", "coll", "vr")]
+    oe = _c(c["opem"] if opem is None else opem)
+    rc = L_.orc_nlte_response_system(C.addressof(atom), nd, c["nray"], c["nlam"], c["tau_grid"], c["boundary"],
+                                     *[ptr(a) for a in arrs], ptr(oe), ptr(M), ptr(beta))
+    assert rc == 0, rc
+    return M, beta
+
+
+def nlte_profile_norm(atom, c, ray):
+    L_ = lib()
+    dp = C.POINTER(C.c_double)
+    L_.orc_nlte_profile_norm.argtypes = [C.c_void_p, C.c_int, C.c_int] + [dp] * 7
+    nd, ntr = c["nd"], c["ntr"]
+    norm = np.zeros((nd, ntr))
+    normd = np.zeros((3, nd, ntr))
+    arrs = [_c(c["lam"]), _c(c["lamw"]), _c(c["depth"]), _c(c["damp"]), _c(c["vr"][ray])]
+    assert L_.orc_nlte_profile_norm(C.addressof(atom), nd, c["nlam"], *[ptr(a) for a in arrs], ptr(norm),
+                                    ptr(normd)) == 0
+    return norm, normd
+
+
+def nlte_line_profiles(atom, c, ray, lam):
+    L_ = lib()
+    dp = C.POINTER(C.c_double)
+    L_.orc_nlte_line_profiles.argtypes = [C.c_void_p, C.c_int, C.c_double] + [dp] * 4
+    nd, ntr = c["nd"], c["ntr"]
+    prof = np.zeros((nd, ntr))
+    arrs = [_c(c["depth"]), _c(c["damp"]), _c(c["vr"][ray])]
+    assert L_.orc_nlte_line_profiles(C.addressof(atom), nd, float(lam), *[ptr(a) for a in arrs], ptr(prof)) == 0
+    return prof

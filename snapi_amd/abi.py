@@ -107,6 +107,7 @@ class SnapiNlteAtom(C.Structure):
                 ("tr_damp_rad", C.POINTER(C.c_double)), ("tr_lambda_crit", C.POINTER(C.c_double))]
 
 
+SNAPI_NLTE_NFIELD = 13
 NLTE_FIELDS = ("T", "vt", "ne", "Nt", "opref", "grid", "dne_dT", "dne_dNt", "dopref_dT", "dopref_dNt", "dopref_dvt",
                "dNact_dT", "dNact_dNt")
 

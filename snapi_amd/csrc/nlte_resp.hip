@@ -25,6 +25,7 @@
 
 #include <cmath>
 
+#include "nlte_device.h"
 #include "rt_common.h"
 
 #pragma clang fp contract(off)
@@ -256,6 +257,18 @@ __global__ __launch_bounds__(kSolveThreads) void crout_solve_kernel(int n, int n
   for (int i = 0; i < n; ++i) x[i] = Y[(size_t)i * nrhs + r];
 }
 
+#define NCHK(x)              \
+  do {                       \
+    if ((x) != hipSuccess) { \
+      rc = SNAPI_ERR_DEVICE; \
+      goto done;             \
+    }                        \
+  } while (0)
+
+}  // namespace
+
+namespace snapi {
+
 int launch_formal_resp(int ncol, int nprob, int nd, const double *grid, const double *op, const double *em,
                        const double *theta, double boundary, double *I, double *L, double *Rop, double *Rem,
                        hipStream_t st) {
@@ -276,15 +289,10 @@ int launch_response_solve(int nsys, int n, int nrhs, const double *M, const doub
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
-#define NCHK(x)              \
-  do {                       \
-    if ((x) != hipSuccess) { \
-      rc = SNAPI_ERR_DEVICE; \
-      goto done;             \
-    }                        \
-  } while (0)
+}  // namespace snapi
 
-}  // namespace
+using snapi::launch_formal_resp;
+using snapi::launch_response_solve;
 
 extern "C" int snapi_formal_responses_dev(int ncol, int nprob, int nd, const double *grid_dev, const double *op_dev,
                                           const double *em_dev, const double *theta_dev, double boundary,

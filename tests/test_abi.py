@@ -51,7 +51,8 @@ def test_struct_layout_matches_header():
              ("snapi_chem_table", abi.SnapiChemTable), ("snapi_state_map", abi.SnapiStateMap),
              ("snapi_level", __import__("snapi_amd.lines", fromlist=["SnapiLevel"]).SnapiLevel),
              ("snapi_node_model", __import__("snapi_amd.invert", fromlist=["SnapiNodeModel"]).SnapiNodeModel),
-             ("snapi_instrument", __import__("snapi_amd.invert", fromlist=["SnapiInstrument"]).SnapiInstrument)]
+             ("snapi_instrument", __import__("snapi_amd.invert", fromlist=["SnapiInstrument"]).SnapiInstrument),
+             ("snapi_nlte_atom", abi.SnapiNlteAtom)]
     lines = []
     want = []
     for cname, cls in pairs:

@@ -414,11 +414,75 @@ def nlteresp_leg(torch, dev, stream, ncol, nsys, with_cpu):
         solve["rel_err_vs_oracle"] = float(np.max(np.abs(X[0] - Xo)) / np.abs(Xo).max())
         log("nlteresp cpu oracle: %.4g problems/s, one solve %.3f s; GPU solve vs oracle %.2e"
             % (nref / dt, dts, solve["rel_err_vs_oracle"]))
-    return {"workload": "NLTE population responses, Ca II 5 levels + continuum in the 61-depth column: "
-                        "formal_with_responses for %d columns x %d (ray, wavelength) problems; %d solves of the "
+    system = nlte_system_leg(torch, dev, stream, ncol, with_cpu)
+    return {"workload": "NLTE population responses, Ca II 5 levels + continuum in the 61-depth column: the whole "
+                        "response system (profile norms, formal_with_responses, assembly, rate part, solve) for %d "
+                        "columns x %d (ray, wavelength) problems; formal_with_responses alone; %d solves of the "
                         "%d x %d response system" % (ncol, nprob, nsys, n, n),
-            "value": formal["problems_per_s"], "unit": "formal_with_responses problems/s", "formal": formal,
-            "solve": solve, "cpu_baseline": cpu}
+            "value": system["columns_per_s"], "unit": "NLTE response systems (columns)/s", "system": system,
+            "formal": formal, "solve": solve, "cpu_baseline": system["cpu_baseline"], "cpu_baseline_kernels": cpu}
+
+
+def nlte_system_leg(torch, dev, stream, ncol, with_cpu):
+    """snapi_nlte_response_system_dev on ncol perturbed copies of the Ca II
+    case recorded from the reference (workloads.nlte_system_problem), with
+    the solve, inputs resident on the device; the oracle's assembly of one
+    column timed beside it."""
+    from snapi_amd import nlteresp, workloads
+    atom, c, inp = workloads.nlte_system_problem(ncol)
+    nd, nm = c["nd"], c["nmap"]
+    n = nd * nm
+    tens = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in inp.items()}
+    lam_t = torch.from_numpy(c["lam"]).to(dev)
+    lamw_t = torch.from_numpy(c["lamw"]).to(dev)
+    M_t = torch.empty((ncol, n, n), dtype=torch.float64, device=dev)
+    b_t = torch.empty((ncol, 3, nd, n), dtype=torch.float64, device=dev)
+    r_t = torch.empty((ncol, 3, n, nd), dtype=torch.float64, device=dev)
+
+    def call():
+        nlteresp.response_system_dev(atom, c["theta"], c["wray"], lam_t, lamw_t, tens, M_t, b_t, r_t,
+                                     tau_grid=c["tau_grid"], boundary=c["boundary"], stream=stream)
+
+    call()
+    torch.cuda.synchronize()
+    reps = 3
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nprob = c["nray"] * c["nlam"]
+    res = {"columns": ncol, "depths": nd, "levels": nm, "problems_per_column": nprob, "ms_per_call": round(ms, 3),
+           "columns_per_s": ncol / (ms * 1e-3)}
+    log("NLTE response system: %d columns (Ca II, %d depths, %d problems each) in %.3f ms = %.4g columns/s"
+        % (ncol, nd, nprob, ms, res["columns_per_s"]))
+    cpu = None
+    if with_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import _oracle  # the FP64 CPU restatement: baseline and checker only
+        M0, b0 = M_t[0].cpu().numpy(), b_t[0].cpu().numpy()
+        ck = dict(c, pops=inp["pops"][0])
+        t1 = time.time()
+        nref = 0
+        while True:
+            Mo, bo = _oracle.nlte_response_system(atom, ck, opem=inp["opem"][0])
+            nref += 1
+            if time.time() - t1 > 10.0:
+                break
+        dt = time.time() - t1
+        rel = lambda a, b: float(np.max(np.abs(a - b) / np.maximum(np.abs(b).max(axis=-1, keepdims=True), 1e-300)))
+        res["rel_err_vs_oracle"] = {"M": rel(M0, Mo), "beta_T": rel(b0[0], bo[0]), "beta_Nt": rel(b0[1], bo[1]),
+                                    "beta_vt": rel(b0[2], bo[2])}
+        cpu = {"value": nref / dt, "unit": "NLTE response systems (columns)/s", "cores": 1, "kind": "port",
+               "sample": "the assembly of column 0, %d times (FP64 oracle: profile norms, formal_with_responses, "
+                         "contributions and rate part, one thread; no solve)" % nref, "seconds": round(dt, 2)}
+        log("NLTE response system cpu oracle: %.4g columns/s (one thread); GPU vs oracle %s"
+            % (cpu["value"], res["rel_err_vs_oracle"]))
+    res["cpu_baseline"] = cpu
+    return res
 
 
 def ali_leg(torch, dev, stream, side, with_cpu):

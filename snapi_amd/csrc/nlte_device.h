@@ -14,8 +14,9 @@ int launch_formal_resp(int ncol, int nprob, int nd, const double *grid, const do
                        hipStream_t st);
 
 // Pivot-free Crout factorisation and solves of nsys systems: M [nsys][n][n];
-// B, X [nsys][nrhs][n]; LU [nsys][n][n] and Y [nsys][n][nrhs] scratch.
-int launch_response_solve(int nsys, int n, int nrhs, const double *M, const double *B, double *LU, double *Y,
+// B, X [nsys][nrhs][n]; LU and LT [nsys][n][n] scratch (the factors, and the
+// lower factor column by column).
+int launch_response_solve(int nsys, int n, int nrhs, const double *M, const double *B, double *LU, double *LT,
                           double *X, hipStream_t st);
 
 }  // namespace snapi

@@ -23,6 +23,7 @@
 //                        right-hand side.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "nlte_device.h"
@@ -34,7 +35,7 @@ namespace {
 
 constexpr int kFrThreads = 64;   // one wavefront per formal problem
 constexpr int kFrMaxPer = 4;     // depths per lane: nd <= 256
-constexpr int kCroutThreads = 256;
+constexpr int kCroutThreads = 512;
 constexpr int kSolveThreads = 256;
 
 // Linear short-characteristics weights and their step derivatives
@@ -207,54 +208,101 @@ __global__ __launch_bounds__(kFrThreads) void formal_resp_kernel(int nprob, int 
 
 // Crout without pivoting (cmlib/mathtools.cc:487-500): column k of the lower
 // triangle, then row k of the unit upper triangle, each entry a dot product
-// over p < k in ascending order.
+// over p < k in ascending order.  The lower triangle is also kept column by
+// column in LT ([k][i] = L[i][k]) so that the column-k products read it
+// coalesced; the step's shared operands (column k of U, row k of L) are
+// staged in LDS, and each work-item loads its own operands kUnroll at a time
+// before adding the products in order.  D holds the factors as the
+// reference lays them out.
+constexpr int kUnroll = 16;
+
+__device__ __forceinline__ double ordered_dot(int k, const double *__restrict__ a, size_t astride,
+                                              const double *__restrict__ b) {
+  double sum = 0.;
+  int p = 0;
+  for (; p + kUnroll <= k; p += kUnroll) {
+    double va[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) va[u] = a[(size_t)(p + u) * astride];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) sum += va[u] * b[p + u];
+  }
+  for (; p < k; ++p) sum += a[(size_t)p * astride] * b[p];
+  return sum;
+}
+
 __global__ __launch_bounds__(kCroutThreads) void crout_kernel(int n, const double *__restrict__ S_all,
-                                                              double *__restrict__ D_all) {
+                                                              double *__restrict__ D_all,
+                                                              double *__restrict__ LT_all) {
+  extern __shared__ double stage[];  // column k of U [n], row k of L [n]
+  double *ucol = stage, *lrow = stage + n;
   const size_t off = (size_t)blockIdx.x * n * n;
   const double *S = S_all + off;
   double *D = D_all + off;
+  double *LT = LT_all + off;
   for (int k = 0; k < n; ++k) {
+    for (int p = threadIdx.x; p < k; p += blockDim.x) {
+      ucol[p] = D[(size_t)p * n + k];
+      lrow[p] = D[(size_t)k * n + p];
+    }
+    __syncthreads();
     for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
-      double sum = 0.;
-      for (int p = 0; p < k; ++p) sum += D[(size_t)i * n + p] * D[(size_t)p * n + k];
-      D[(size_t)i * n + k] = S[(size_t)i * n + k] - sum;
+      const double v = S[(size_t)i * n + k] - ordered_dot(k, LT + i, n, ucol);
+      D[(size_t)i * n + k] = v;
+      LT[(size_t)k * n + i] = v;
     }
     __syncthreads();
     const double dkk = D[(size_t)k * n + k];
-    for (int j = k + 1 + threadIdx.x; j < n; j += blockDim.x) {
-      double sum = 0.;
-      for (int p = 0; p < k; ++p) sum += D[(size_t)k * n + p] * D[(size_t)p * n + j];
-      D[(size_t)k * n + j] = (S[(size_t)k * n + j] - sum) / dkk;
-    }
+    for (int j = k + 1 + threadIdx.x; j < n; j += blockDim.x)
+      D[(size_t)k * n + j] = (S[(size_t)k * n + j] - ordered_dot(k, D + j, n, lrow)) / dkk;
     __syncthreads();
   }
 }
 
 // solveCrout (cmlib/mathtools.cc:502-514) for nrhs right-hand sides, one
-// work-item each; y lives in the solution array until the back substitution
-// overwrites it from the bottom.  B, X: [sys][nrhs][n]; Y scratch [sys][n][nrhs].
+// work-item each, blockDim.x of them per workgroup; each work-item's y, then
+// x, lives in its own LDS column ([row][blockDim.x]) while the factors' rows
+// are read uniformly.  B, X: [sys][nrhs][n].
 __global__ __launch_bounds__(kSolveThreads) void crout_solve_kernel(int n, int nrhs, const double *__restrict__ LU_all,
                                                                     const double *__restrict__ B_all,
-                                                                    double *__restrict__ Y_all,
                                                                     double *__restrict__ X_all) {
-  const int r = blockIdx.y * blockDim.x + threadIdx.x;
+  extern __shared__ double yl[];
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int r = blockIdx.y * T + tid;
   if (r >= nrhs) return;
   const double *LU = LU_all + (size_t)blockIdx.x * n * n;
   const double *b = B_all + ((size_t)blockIdx.x * nrhs + r) * n;
-  double *Y = Y_all + (size_t)blockIdx.x * n * nrhs;
   double *x = X_all + ((size_t)blockIdx.x * nrhs + r) * n;
+  double *y = yl + tid;
   for (int i = 0; i < n; ++i) {
     double sum = 0.;
-    for (int k = 0; k < i; ++k) sum += LU[(size_t)i * n + k] * Y[(size_t)k * nrhs + r];
-    Y[(size_t)i * nrhs + r] = (b[i] - sum) / LU[(size_t)i * n + i];
+    const double *row = LU + (size_t)i * n;
+    for (int k = 0; k < i; ++k) sum += row[k] * y[k * T];
+    y[i * T] = (b[i] - sum) / row[i];
   }
   // back substitution: x overwrites y from the last row up
   for (int i = n - 1; i >= 0; --i) {
     double sum = 0.;
-    for (int k = i + 1; k < n; ++k) sum += LU[(size_t)i * n + k] * Y[(size_t)k * nrhs + r];
-    Y[(size_t)i * nrhs + r] = (Y[(size_t)i * nrhs + r] - sum);
+    const double *row = LU + (size_t)i * n;
+    for (int k = i + 1; k < n; ++k) sum += row[k] * y[k * T];
+    y[i * T] = (y[i * T] - sum);
   }
-  for (int i = 0; i < n; ++i) x[i] = Y[(size_t)i * nrhs + r];
+  for (int i = 0; i < n; ++i) x[i] = y[i * T];
+}
+
+// Right-hand sides per solve workgroup: as many LDS columns of n doubles as
+// the device's per-workgroup LDS holds, at most kSolveThreads.
+int solve_tile(int n) {
+  static int lim = 0;
+  if (!lim) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      v = 64 * 1024;
+    lim = v;
+  }
+  const long t = (long)lim / ((long)n * (long)sizeof(double));
+  return (int)std::max(0L, std::min<long>(kSolveThreads, t));
 }
 
 #define NCHK(x)              \
@@ -280,12 +328,15 @@ int launch_formal_resp(int ncol, int nprob, int nd, const double *grid, const do
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
-int launch_response_solve(int nsys, int n, int nrhs, const double *M, const double *B, double *LU, double *Y,
+int launch_response_solve(int nsys, int n, int nrhs, const double *M, const double *B, double *LU, double *LT,
                           double *X, hipStream_t st) {
-  hipLaunchKernelGGL(crout_kernel, dim3(nsys), dim3(kCroutThreads), 0, st, n, M, LU);
+  const int T = solve_tile(n);
+  if (T < 1) return SNAPI_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(crout_kernel, dim3(nsys), dim3(kCroutThreads), 2 * (size_t)n * sizeof(double), st, n, M, LU,
+                     LT);
   if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;
-  const dim3 grid(nsys, (nrhs + kSolveThreads - 1) / kSolveThreads);
-  hipLaunchKernelGGL(crout_solve_kernel, grid, dim3(kSolveThreads), 0, st, n, nrhs, LU, B, Y, X);
+  const dim3 grid(nsys, (nrhs + T - 1) / T);
+  hipLaunchKernelGGL(crout_solve_kernel, grid, dim3(T), (size_t)T * n * sizeof(double), st, n, nrhs, LU, B, X);
   return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
@@ -347,7 +398,7 @@ extern "C" int snapi_response_solve(int nsys, int n, int nrhs, const double *M, 
   NCHK(hipMalloc((void **)&dM, nsys * nn * 8));
   NCHK(hipMalloc((void **)&dLU, nsys * nn * 8));
   NCHK(hipMalloc((void **)&dB, (size_t)nsys * nrhs * n * 8));
-  NCHK(hipMalloc((void **)&dY, (size_t)nsys * nrhs * n * 8));
+  NCHK(hipMalloc((void **)&dY, nsys * nn * 8));
   NCHK(hipMalloc((void **)&dX, (size_t)nsys * nrhs * n * 8));
   NCHK(hipMemcpy(dM, M, nsys * nn * 8, hipMemcpyHostToDevice));
   NCHK(hipMemcpy(dB, B, (size_t)nsys * nrhs * n * 8, hipMemcpyHostToDevice));

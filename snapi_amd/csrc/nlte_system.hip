@@ -57,7 +57,6 @@ namespace {
 constexpr int kMaxLev = 8;     // levels of the atom
 constexpr int kMaxTr = 32;     // transitions
 constexpr int kMaxRay = 16;    // rays of the angular quadrature
-constexpr int kLinesPerPass = 8;  // assembled lines per assembly pass
 constexpr int kRows = 8;       // rows l per assembly workgroup (one wavefront each)
 constexpr double kSaha = 2.0706615382E-16;  // cmlib/const.h:24
 constexpr double kDeltaT = 5.0;             // cmlib/const.h:27
@@ -494,11 +493,16 @@ __global__ __launch_bounds__(64) void sum_kernel(int nd, int nprob, int nsum, co
 }
 
 // ---------------------------------------------------------------- assembly
-// grid (ceil(nd/kRows), ncol), block (64, kRows): lane ll, wavefront l.
-// line0: first assembled line of this pass; dense: this pass also forms the
-// level-vector terms and writes the blocks (the first pass), later passes add
-// their lines' terms to what is there.
-template <int NM>
+// grid (ceil(nd/kRows), ncol), block (64, kRows): lane ll, wavefront l.  The
+// column record of one problem ([ncolrec][nd], the same for every row l) is
+// staged in LDS once per workgroup; each work-item prefetches its share of
+// the next problem's record into registers while it works on the current
+// one.  line0: first assembled line of this pass (NL lines per pass); dense:
+// this pass also forms the level-vector terms and writes the blocks (the
+// first pass), later passes add their lines' terms to what is there.
+constexpr int kPre = 9;  // prefetched doubles per work-item (records of up to 9 * 64 * kRows doubles)
+
+template <int NM, int NL>
 __global__ __launch_bounds__(64 * kRows) void assemble_kernel(NAtom at, int nd, int nprob, int line0, int dense,
                                                               const double *__restrict__ Rop,
                                                               const double *__restrict__ Rem,
@@ -506,16 +510,21 @@ __global__ __launch_bounds__(64 * kRows) void assemble_kernel(NAtom at, int nd, 
                                                               const double *__restrict__ crec, double *__restrict__ M,
                                                               double *__restrict__ beta) {
 #pragma clang fp contract(fast)
+  extern __shared__ double cbuf[];  // [ncolrec][nd]
   const int ll = threadIdx.x;
+  const int tid = threadIdx.y * 64 + threadIdx.x;
   const int l = __builtin_amdgcn_readfirstlane(blockIdx.x * kRows + threadIdx.y);
   const int col = blockIdx.y;
-  if (l >= nd) return;
+  const bool row_ok = l < nd;  // no early return: the workgroup meets at barriers
   const int llc = ll < nd ? ll : nd - 1;
+  const int lc = row_ok ? l : nd - 1;
   const int NLA = at.nla;
   const int nr = nrow_rec(at), nc = ncol_rec(at);
-  const int nline = min(kLinesPerPass, NLA - line0);
+  const int ncd = nc * nd;
+  const int nline = min(NL, NLA - line0);
+  const double *src = crec + (size_t)col * nprob * ncd;
   double Mc[NM][NM], Bc[3][NM];
-  double X[kLinesPerPass], Y[kLinesPerPass], Z[3][kLinesPerPass];
+  double X[NL], Y[NL], Z[3][NL];
 #pragma unroll
   for (int a = 0; a < NM; ++a) {
 #pragma unroll
@@ -524,29 +533,50 @@ __global__ __launch_bounds__(64 * kRows) void assemble_kernel(NAtom at, int nd, 
     for (int q = 0; q < 3; ++q) Bc[q][a] = 0.0;
   }
 #pragma unroll
-  for (int j = 0; j < kLinesPerPass; ++j) {
+  for (int j = 0; j < NL; ++j) {
     X[j] = Y[j] = 0.0;
     Z[0][j] = Z[1][j] = Z[2][j] = 0.0;
   }
+  double pre[kPre];
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) {
+    const int i = tid + k * 64 * kRows;
+    pre[k] = i < ncd ? src[i] : 0.0;
+  }
   const int s0 = 4 * NM, l0 = s0 + 10;
   for (int p = 0; p < nprob; ++p) {
+    __syncthreads();  // the previous problem's record is no longer read
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+      const int i = tid + k * 64 * kRows;
+      if (i < ncd) cbuf[i] = pre[k];
+    }
+    for (int i = tid + kPre * 64 * kRows; i < ncd; i += 64 * kRows) cbuf[i] = src[(size_t)p * ncd + i];
+    __syncthreads();
+    if (p + 1 < nprob) {
+#pragma unroll
+      for (int k = 0; k < kPre; ++k) {
+        const int i = tid + k * 64 * kRows;
+        if (i < ncd) pre[k] = src[(size_t)(p + 1) * ncd + i];
+      }
+    }
     const size_t pb = (size_t)col * nprob + p;
-    const double *R = rrec + (pb * nd + l) * nr;
-    const size_t ro_off = (pb * nd + l) * nd + llc;
+    const double *R = rrec + (pb * nd + lc) * nr;
+    const size_t ro_off = (pb * nd + lc) * nd + llc;
     const double ro = Rop[ro_off], re = Rem[ro_off];
-    const double *C = crec + pb * nc * nd + llc;
+    const double *C = cbuf + llc;
     if (dense) {
       double wl[NM], wb[NM];
 #pragma unroll
       for (int c = 0; c < NM; ++c) {
-        wl[c] = ro * C[(size_t)c * nd] + re * C[(size_t)(NM + c) * nd];
-        wb[c] = ro * C[(size_t)(2 * NM + c) * nd] + re * C[(size_t)(3 * NM + c) * nd];
+        wl[c] = ro * C[c * nd] + re * C[(NM + c) * nd];
+        wb[c] = ro * C[(2 * NM + c) * nd] + re * C[(3 * NM + c) * nd];
       }
       double tl[3], tb[2];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) tl[q] = ro * C[(size_t)(s0 + q) * nd] + re * C[(size_t)(s0 + 3 + q) * nd];
+      for (int q = 0; q < 3; ++q) tl[q] = ro * C[(s0 + q) * nd] + re * C[(s0 + 3 + q) * nd];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) tb[q] = ro * C[(size_t)(s0 + 6 + q) * nd] + re * C[(size_t)(s0 + 8 + q) * nd];
+      for (int q = 0; q < 2; ++q) tb[q] = ro * C[(s0 + 6 + q) * nd] + re * C[(s0 + 8 + q) * nd];
 #pragma unroll
       for (int a = 0; a < NM; ++a) {
         const double gl = R[NLA + a], gb = R[NLA + NM + a];
@@ -558,20 +588,20 @@ __global__ __launch_bounds__(64 * kRows) void assemble_kernel(NAtom at, int nd, 
       }
     }
 #pragma unroll
-    for (int j = 0; j < kLinesPerPass; ++j) {
+    for (int j = 0; j < NL; ++j) {
       if (j < nline) {
         const int jj = line0 + j;
         const double g = R[jj];
-        const double *Cj = C + (size_t)(l0 + 7 * jj) * nd;
+        const double *Cj = C + (l0 + 7 * jj) * nd;
         const double h = g * Cj[0];
         X[j] += h * ro;
         Y[j] += h * re;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) Z[q][j] += g * (ro * Cj[(size_t)(1 + q) * nd] + re * Cj[(size_t)(4 + q) * nd]);
+        for (int q = 0; q < 3; ++q) Z[q][j] += g * (ro * Cj[(1 + q) * nd] + re * Cj[(4 + q) * nd]);
       }
     }
   }
-  if (ll >= nd) return;
+  if (!row_ok || ll >= nd) return;
   const int n = nd * NM;
   double *Mrow = M + (size_t)col * n * n + (size_t)(l * NM) * n + ll * NM;  // row (l, a), column (ll, b)
   double *Bq = beta + (size_t)col * 3 * nd * n;                             // [q][ll][(l, a)]
@@ -585,7 +615,7 @@ __global__ __launch_bounds__(64 * kRows) void assemble_kernel(NAtom at, int nd, 
     }
   }
 #pragma unroll
-  for (int j = 0; j < kLinesPerPass; ++j) {
+  for (int j = 0; j < NL; ++j) {
     if (j >= nline) break;
     const int t = at.la[line0 + j], ri = at.up[t], rii = at.lo[t];
     const double dd = at.A[t] * Y[j] - at.Bul[t] * X[j];
@@ -775,31 +805,43 @@ int build_atom(const snapi_nlte_atom *a, int nray, const double *wray, int tau_g
   return SNAPI_OK;
 }
 
-template <int NM>
+template <int NM, int NL>
 int launch_assemble(const NAtom &at, int ncol, int nd, int nprob, const double *Rop, const double *Rem,
                     const double *rrec, const double *crec, double *M, double *beta, hipStream_t st) {
   const dim3 grid((nd + kRows - 1) / kRows, ncol), block(64, kRows);
+  const size_t lds = (size_t)ncol_rec(at) * nd * sizeof(double);
   int line0 = 0, dense = 1;
   do {
-    hipLaunchKernelGGL(assemble_kernel<NM>, grid, block, 0, st, at, nd, nprob, line0, dense, Rop, Rem, rrec, crec,
-                       M, beta);
+    hipLaunchKernelGGL((assemble_kernel<NM, NL>), grid, block, lds, st, at, nd, nprob, line0, dense, Rop, Rem, rrec,
+                       crec, M, beta);
     if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;
-    line0 += kLinesPerPass;
+    line0 += NL;
     dense = 0;
   } while (line0 < at.nla);
   return SNAPI_OK;
 }
 
+template <int NM>
+int launch_assemble_nm(const NAtom &at, int ncol, int nd, int nprob, const double *Rop, const double *Rem,
+                       const double *rrec, const double *crec, double *M, double *beta, hipStream_t st) {
+  // lines per pass: the assembled lines' count rounded up to an even number, at most 8
+  if (at.nla <= 2) return launch_assemble<NM, 2>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+  if (at.nla <= 4) return launch_assemble<NM, 4>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+  if (at.nla <= 6) return launch_assemble<NM, 6>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+  return launch_assemble<NM, 8>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+}
+
 int dispatch_assemble(const NAtom &at, int ncol, int nd, int nprob, const double *Rop, const double *Rem,
                       const double *rrec, const double *crec, double *M, double *beta, hipStream_t st) {
+  if ((size_t)ncol_rec(at) * nd * sizeof(double) > 64 * 1024) return SNAPI_ERR_UNSUPPORTED;
   switch (at.nmap) {
-    case 2: return launch_assemble<2>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
-    case 3: return launch_assemble<3>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
-    case 4: return launch_assemble<4>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
-    case 5: return launch_assemble<5>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
-    case 6: return launch_assemble<6>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
-    case 7: return launch_assemble<7>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
-    case 8: return launch_assemble<8>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+    case 2: return launch_assemble_nm<2>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+    case 3: return launch_assemble_nm<3>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+    case 4: return launch_assemble_nm<4>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+    case 5: return launch_assemble_nm<5>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+    case 6: return launch_assemble_nm<6>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+    case 7: return launch_assemble_nm<7>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
+    case 8: return launch_assemble_nm<8>(at, ncol, nd, nprob, Rop, Rem, rrec, crec, M, beta, st);
   }
   return SNAPI_ERR_UNSUPPORTED;
 }
@@ -818,7 +860,7 @@ size_t per_column_bytes(const NAtom &at, int nd, int nprob) {
 
 size_t solve_bytes(int nd, int nmap) {
   const size_t n = (size_t)nd * nmap;
-  return (n * n + 2 * 3 * (size_t)nd * n) * sizeof(double);  // LU, Y, X
+  return (2 * n * n + 3 * (size_t)nd * n) * sizeof(double);  // LU, LT, X
 }
 
 #define NCHK(x)              \
@@ -881,7 +923,7 @@ int system_dev(const NAtom &at, int ncol, int nd, int nlam, double boundary, con
     th_d = q, q += nprob;
     if (resp) {
       LU = q, q += (size_t)chunk * n * n;
-      Y = q, q += (size_t)chunk * 3 * nd * n;
+      Y = q, q += (size_t)chunk * n * n;
       X = q, q += (size_t)chunk * 3 * nd * n;
     }
   }

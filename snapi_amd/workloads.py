@@ -529,3 +529,55 @@ def eos_tables(dump_name, lam_mid):
     chem_he = abi.make_chem_table(d, referent=1)
     smap = abi.make_state_map(chem, d, g["lines"], g["bfsig"], lam_mid)
     return chem, chem_he, smap
+
+
+def nlte_system_case():
+    """The NLTE response case recorded from the reference
+    (tests/golden/nlterin.npz: the Ca II atom of the cfg0 column after
+    nltepops, with everything compute_nlte_population_responses feeds its
+    assembly) in the layouts of snapi_nlte_response_system: (atom, case) with
+    case holding the quadrature, the wavelength grid, sigma and one column's
+    depth, pops, damp, coll, vr and opem, plus the reference's own M, beta,
+    profile norms and profile samples."""
+    d = dict(np.load(os.path.join(GOLDEN, "nlterin.npz")))
+    nd, nray, nlam, nmap, ntr, Z, tau_grid, damping = (int(x) for x in d["head"])
+    mass, boundary = d["consts"][0], d["consts"][4]
+    lev, tr = d["levels"], d["trans"]
+    stage = lev[:, 1].astype(int)
+    stage_ip = np.zeros(Z + 1)
+    for i in range(nmap):
+        stage_ip[stage[i]] = lev[i, 5]
+    # transitions: index, kind (0 line, 1 bound-free), stage, lower and upper
+    # within the stage, map index of the upper level, A, Blu, Bul, edge
+    first = {z: int(np.argmax(stage == z)) for z in set(stage)}
+    lower = np.array([first[int(t[2])] + int(t[3]) for t in tr])
+    upper = np.array([int(t[5]) for t in tr])
+    atom = abi.make_nlte_atom(stage, lev[:, 3], lev[:, 4], stage_ip, tr[:, 1].astype(int), lower, upper, tr[:, 6],
+                              tr[:, 7], tr[:, 8], d["damp"][0, :, 0], tr[:, 9], mass, damping)
+    case = dict(nd=nd, nray=nray, nlam=nlam, nmap=nmap, ntr=ntr, tau_grid=tau_grid, boundary=float(boundary),
+                theta=d["rays"][:, 0].copy(), wray=d["rays"][:, 1].copy(), lam=d["lam"][:, 0].copy(),
+                lamw=d["lam"][:, 1].copy(), sigma=d["sigma"].copy(),
+                depth=np.ascontiguousarray(d["depth"][:, :abi.SNAPI_NLTE_NFIELD].T),
+                pops=np.ascontiguousarray(d["pops"].T), damp=np.ascontiguousarray(d["damp"][1:]),
+                coll=np.ascontiguousarray(d["coll"].transpose(3, 1, 2, 0)), vr=d["vr"].copy(),
+                opem=np.ascontiguousarray(d["pp"].transpose(0, 1, 3, 2)),
+                M=d["M"], beta=d["beta"], norm=d["norm"], prof=d["prof"])
+    return atom, case
+
+
+def nlte_system_problem(ncol, seed=14):
+    """ncol columns around the recorded Ca II case: opacity and emissivity of
+    every (ray, wavelength, depth) scaled by exp(N(0, 0.1)), the emissivity
+    again by exp(N(0, 0.05)), level populations by exp(N(0, 0.05)); the
+    per-depth state, damping and collision rates are the case's.  Returns
+    (atom, case, inputs) with inputs the per-column arrays (leading column
+    axis) and sigma."""
+    atom, c = nlte_system_case()
+    rng = np.random.default_rng(seed)
+    rep = lambda a: np.repeat(a[None], ncol, axis=0)  # noqa: E731
+    opem = c["opem"][None] * np.exp(rng.normal(0.0, 0.1, (ncol,) + c["opem"].shape[:2] + (1, c["nd"])))
+    opem[:, :, :, 1] *= np.exp(rng.normal(0.0, 0.05, (ncol,) + c["opem"].shape[:2] + (c["nd"],)))
+    pops = c["pops"][None] * np.exp(rng.normal(0.0, 0.05, (ncol,) + c["pops"].shape))
+    inputs = dict(sigma=c["sigma"], depth=rep(c["depth"]), pops=pops, damp=rep(c["damp"]), coll=rep(c["coll"]),
+                  vr=rep(c["vr"]), opem=np.ascontiguousarray(opem))
+    return atom, c, inputs

@@ -771,7 +771,11 @@ enum {
  *     perturbed depth);
  *   resp [ncol][3][n][nd] out (may be NULL): level_responses, the solution
  *     for every right-hand side (snapi_response_solve of M and beta).
- * 2 <= nd <= 256, nmap <= 16, ntr <= 64.  Double throughout.
+ * 2 <= nd <= 256, nmap <= 8, ntr <= 32, nray <= 16, nray * nlam <= 65535;
+ * SNAPI_ERR_UNSUPPORTED for atoms outside the layout above.  Double
+ * throughout.  The work area (about 128 MB per column of a 6-level,
+ * 15-transition atom on 2 x 444 problems) is allocated per call and
+ * covers as many columns per pass as half the free device memory holds.
  */
 int snapi_nlte_response_system(const snapi_nlte_atom *atom, int ncol, int nd, int nray, int nlam, int tau_grid,
                                double boundary, const double *ray_theta, const double *ray_weight,

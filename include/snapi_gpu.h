@@ -210,6 +210,34 @@ int snapi_version(void);
 int snapi_device_count(void);
 
 /*
+ * Launch-shape autotuner (no counterpart in the reference, whose solver runs
+ * one column per CPU process).  The synthesis kernels come in three
+ * occupancy bounds (float: 8, 7, 6 wavefronts per SIMD; double: the
+ * compiler's own, 3, 4) and the node-Jacobian tangent kernels in workgroups
+ * of 64, 128 and 256 work-items.  Every shape computes the same results bit
+ * for bit.  With tuning on (the default; SNAPI_AUTOTUNE=0 in the environment
+ * turns it off), the first call for a problem shape (kernel, precision,
+ * depths, lines, components, wavelengths, node pass) on a device times each
+ * shape on the call's own first columns (up to 16384 of them) with HIP
+ * events and keeps the fastest for the rest of the process; calls made
+ * while their stream is being captured into a graph, and calls with tuning
+ * off, take the first shape of the list unless the shape has been chosen
+ * already.
+ *   snapi_autotune_mode: 0 off, 1 on, -1 query; returns the previous mode.
+ *   snapi_autotune_force: kernel 0 synthesis, 1 tangent; candidate index to
+ *     launch from now on, -1 to return to the tuned choice.
+ *   snapi_autotune_clear: forgets every choice.
+ *   snapi_autotune_report: one JSON object per line and choice (device,
+ *     kernel, precision, nd, nline, ncomp, nlambda, extra, choice, ms of each
+ *     candidate, -1 for one that failed) into buf of n bytes
+ *     (SNAPI_ERR_CAPACITY when it does not fit).
+ */
+int snapi_autotune_mode(int mode);
+int snapi_autotune_force(int kernel, int candidate);
+int snapi_autotune_clear(void);
+int snapi_autotune_report(char *buf, size_t n);
+
+/*
  * Kernel status.  The kernels report what they cannot compute (a regime the
  * build does not implement, bit 1 -> SNAPI_ERR_UNSUPPORTED) and non-finite
  * results (bit 2 -> SNAPI_ERR_NONFINITE) by OR-ing bits into a status word on
@@ -267,11 +295,42 @@ int snapi_synth_stokes_dev(int precision, const snapi_line_table *lines, const s
  * SNAPI_MIXED), lam in double.  Valid for the mu = 1 ray (theta = pi), where
  * the ray and atmosphere frames coincide (atmosphere::transform_responses,
  * atmos/atmos_rad.cc:53-101, is the identity there); any other ray returns
- * SNAPI_ERR_UNSUPPORTED, as does snapi_stokes_adjoint.
+ * SNAPI_ERR_UNSUPPORTED, as does snapi_stokes_adjoint.  For other rays use
+ * snapi_stokes_responses_atm.
  */
 int snapi_stokes_responses(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
                            const void *state, const void *dstate, int nlambda, const double *lam, void *stokes,
                            void *resp);
+
+/*
+ * Per-depth responses for any ray, in the atmosphere's frame, as
+ * atmosphere::obs_stokes_responses returns them (atmos/atmos_rad_synthesis.cc:
+ * 166-399, transform_responses at :344): the responses of
+ * snapi_stokes_responses taken on the ray (theta = ray->theta, phi; the
+ * state holds the ray-frame projections, as snapi_cube_to_state_dev writes
+ * them), then transformed by snapi_transform_responses with the
+ * atmosphere-frame field bvec [ncol][3][nd] (Bx, By, Bz in G, double).
+ */
+int snapi_stokes_responses_atm(int precision, const snapi_line_table *lines, const snapi_ray *ray, double phi,
+                               const double *bvec, int ncol, int nd, const void *state, const void *dstate,
+                               int nlambda, const double *lam, void *stokes, void *resp);
+/*
+ * atmosphere::transform_responses (atmos/atmos_rad.cc:53-101) on per-depth
+ * responses resp [ncol][7][nd][nlambda][4] (precision's type), in place: per
+ * depth, with the atmosphere-frame field bvec [ncol][3][nd] (double), the
+ * inclination and azimuth responses become
+ *   R_incl' = R_incl d(incl_ray)/d(incl_atm) + R_azi d(azi_ray)/d(incl_atm),
+ *   R_azi'  = R_incl d(incl_ray)/d(azi_atm)  + R_azi d(azi_ray)/d(azi_atm),
+ * with incl = acos(B_z/|B|) and azi = atan(B_y/B_x) in each frame (the ray
+ * frame rotated by theta and phi as atmosphere::transform, :23-51), and the
+ * velocity response is multiplied by cos(phi), as the reference writes it
+ * (:89).  Where |B| = 0, the field lies along the ray or B_x = B_y = 0, where
+ * the reference divides by zero, both angle responses are set to zero.
+ */
+int snapi_transform_responses(int precision, int ncol, int nd, int nlambda, const double *bvec, double theta,
+                              double phi, void *resp);
+int snapi_transform_responses_dev(int precision, int ncol, int nd, int nlambda, const double *bvec_dev, double theta,
+                                  double phi, void *resp_dev, void *stream);
 
 /*
  * Gradient of chi^2/2 = sum_{l,s} w_s (S - S_obs)^2 / 2 with respect to the

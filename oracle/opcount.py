@@ -10,9 +10,11 @@ not re-derived at bench time but read from the table this script writes.
     python oracle/opcount.py            # writes profiles/synth_flops.json
 
 Workloads (the ones the synthesis roofline is quoted on):
-  cube512: the 512 x 512 x 71 cube of bench.py (seed 1, float32 state), 301
-           wavelengths; bench.py's 64-column sample (numpy
-           default_rng(5).choice(262144, 64, replace=False)).
+  cube512: the 512 x 512 x 71 cube of bench.py (seed 1, thermodynamics
+           solved: hydrostatic equilibrium and the equation of state, float32
+           state), 301 wavelengths; bench.py's 64-column sample (numpy
+           default_rng(5).choice(262144, 64, replace=False)), built by the
+           oracle's restatement of the same chain.
   cfg0:    the reference's one-column case (its depths and wavelength grid).
 """
 import json
@@ -77,11 +79,13 @@ def count(table, ray, states, lam):
 
 def cube512_sample():
     sys.path.insert(0, REPO)
+    sys.path.insert(0, os.path.join(REPO, "tests"))
     from snapi_amd import workloads
+    import _oracle
     t = workloads.template_cfg1()
-    st = workloads.cube_state(t, 512, 512, seed=1, dtype=np.float32)
-    samp = np.random.default_rng(5).choice(st.shape[0], 64, replace=False)
-    return t["table"], t["ray"], st[samp].astype(np.float64), t["lam_vac"]
+    samp = np.random.default_rng(5).choice(512 * 512, 64, replace=False)
+    st = _oracle.cube_state_eos(t, 512, 512, 1, samp).astype(np.float32)
+    return t["table"], t["ray"], st.astype(np.float64), t["lam_vac"]
 
 
 def cfg0_case():

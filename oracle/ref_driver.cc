@@ -342,6 +342,9 @@ static void dump_resp_state(atmosphere *a, fp_t el, fp_t az, fp_t *lambda, int n
   fp_t *lam_vac = airtovac(lambda + 1, nlambda) - 1;
   for (int l = 1; l <= nlambda; ++l) printf("lambda %d %.17g %.17g\n", l, lambda[l], lam_vac[l]);
   dump_state(a, Vr, Bv);
+  // the field in the atmosphere's frame, which transform_responses reads
+  for (int z = a->x3l; z <= a->x3h; ++z)
+    printf("bvec %d %.17g %.17g %.17g\n", z, a->Bx[x1][x2][z], a->By[x1][x2][z], a->Bz[x1][x2][z]);
   for (int z = a->x3l; z <= a->x3h; ++z) {
     printf("thermo %d %.17g", z, a->Nt[x1][x2][z]);
     for (int p = 1; p <= 2; ++p)

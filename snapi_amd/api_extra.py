@@ -60,6 +60,46 @@ def stokes_responses(table, ray, state, dstate, lam_vac, precision=abi.SNAPI_FP6
     return stokes, resp
 
 
+def stokes_responses_atm(table, ray, phi, bvec, state, dstate, lam_vac, precision=abi.SNAPI_FP64):
+    """Per-depth responses for any ray (ray.theta, phi) in the atmosphere's
+    frame, as atmosphere::obs_stokes_responses returns them: the state holds
+    the ray-frame projections, bvec [ncol][3][nd] the atmosphere-frame field
+    (Bx, By, Bz).  Returns (stokes, resp) as stokes_responses."""
+    dt = np.float64 if precision == abi.SNAPI_FP64 else np.float32
+    st = np.ascontiguousarray(state, dtype=dt)
+    ds = np.ascontiguousarray(dstate, dtype=dt)
+    bv = np.ascontiguousarray(bvec, dtype=np.float64)
+    if st.ndim == 2:
+        st, ds, bv = st[None], ds[None], bv[None]
+    ncol, nf, nd = st.shape
+    if bv.shape != (ncol, 3, nd):
+        raise ValueError("bvec must be [ncol][3][nd]")
+    lam = np.ascontiguousarray(lam_vac, dtype=np.float64)
+    stokes = np.zeros((ncol, len(lam), 4), dtype=dt)
+    resp = np.zeros((ncol, 7, nd, len(lam), 4), dtype=dt)
+    rc = lib().snapi_stokes_responses_atm(precision, C.byref(table), C.byref(ray), C.c_double(phi), abi.ptr(bv),
+                                          ncol, nd, abi.vptr(st), abi.vptr(ds), len(lam), abi.ptr(lam),
+                                          abi.vptr(stokes), abi.vptr(resp))
+    check("snapi_stokes_responses_atm", rc)
+    return stokes, resp
+
+
+def transform_responses(resp, bvec, theta, phi):
+    """atmosphere::transform_responses on responses [ncol][7][nd][nlambda][4]
+    (float64: FP64 kernel, float32: the same in the mixed kernel); returns
+    the transformed copy."""
+    r = np.array(resp, order="C", copy=True)
+    if r.dtype not in (np.float64, np.float32):
+        raise ValueError("responses must be float64 or float32")
+    prec = abi.SNAPI_FP64 if r.dtype == np.float64 else abi.SNAPI_MIXED
+    ncol, _, nd, nl, _ = r.shape
+    bv = np.ascontiguousarray(bvec, dtype=np.float64).reshape(ncol, 3, nd)
+    rc = lib().snapi_transform_responses(prec, ncol, nd, nl, abi.ptr(bv), C.c_double(theta), C.c_double(phi),
+                                         abi.vptr(r))
+    check("snapi_transform_responses", rc)
+    return r
+
+
 def stokes_adjoint(table, ray, state, dstate, lam_vac, obs, wstokes=(1.0, 1.0, 1.0, 1.0),
                    precision=abi.SNAPI_FP64):
     """Gradient [ncol][7][nd] of chi^2/2 = sum w_s (S - S_obs)^2 / 2 with

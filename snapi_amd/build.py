@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsnapi_gpu.so")
 SOURCES = ["runtime.hip", "synth.hip", "responses.hip", "tangent.hip", "lm.hip", "formal_scalar.hip", "invert.hip",
            "ali.hip", "chemeq.hip",
-           "nlte_resp.hip", "nlte_system.hip", "instrument.hip", "cube.hip", "atomdata.hip"]
+           "nlte_resp.hip", "nlte_system.hip", "instrument.hip", "cube.hip", "atomdata.hip", "autotune.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          # packed-FP32 instructions stay off: hipcc would otherwise form

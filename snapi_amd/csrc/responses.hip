@@ -579,9 +579,76 @@ template int launch_resp<double>(const KLines &, const SynthArgs &, RespArgs, in
 template int launch_resp<float>(const KLines &, const SynthArgs &, RespArgs, int, const void *, const void *,
                                 const double *, void *, void *, void *, int, hipStream_t);
 
+// Angle and velocity responses from the ray's frame to the atmosphere's
+// (atmosphere::transform_responses, atmos/atmos_rad.cc:53-101): per depth,
+// with the atmosphere-frame field (Bx, By, Bz), the derivatives of the ray
+// frame's inclination and azimuth (acos(B'_z/|B|), atan(B'_y/B'_x) of the
+// rotated field) with respect to the atmosphere frame's inclination and
+// azimuth (acos(Bz/|B|), atan(By/Bx)); the velocity response is multiplied
+// by cos(phi) as the reference does (:89).  Where the field vanishes or
+// points along the ray the reference divides by zero; the angle responses
+// are zero there.  One work-item per (column, depth, wavelength).
+template <typename R>
+__global__ __launch_bounds__(256) void transform_resp_kernel(int ncol, int nd, int nl, const double *__restrict__ bvec,
+                                                             double theta, double phi, R *__restrict__ resp) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)ncol * nd * nl;
+  if (i >= n) return;
+  const int l = (int)(i % nl);
+  const int z = (int)((i / nl) % nd);
+  const int col = (int)(i / ((long long)nl * nd));
+  const double cp = cos(phi), sp = sin(phi), ct = cos(kPi - theta), st = sin(kPi - theta);
+  const double *b = bvec + (size_t)col * 3 * nd;
+  const double Bx = b[z], By = b[nd + z], Bz = b[2 * nd + z];
+  const double nx = ct * cp * Bx + sp * By - st * cp * Bz;
+  const double ny = -ct * sp * Bx + cp * By + st * sp * Bz;
+  const double nz = st * Bx + ct * Bz;
+  const double bm = sqrt(Bx * Bx + By * By + Bz * Bz);
+  double ii_t = 0.0, ii_p = 0.0, az_t = 0.0, az_p = 0.0;
+  if (bm > 0.0 && nx * nx + ny * ny > 0.0 && (Bx != 0.0 || By != 0.0)) {
+    const double q = -1.0 / sqrt(1.0 - nz * nz / bm / bm);
+    const double dix = (st * bm - Bx / bm * nz) / bm / bm * q;
+    const double diy = (-By / bm * nz) / bm / bm * q;
+    const double diz = (ct * bm - Bz / bm * nz) / bm / bm * q;
+    const double r = 1.0 / (nx * nx) / (1.0 + ny * ny / nx / nx);
+    const double dax = (-ct * sp * nx - ct * cp * ny) * r;
+    const double day = (cp * nx - sp * ny) * r;
+    const double daz = (st * sp * nx + st * cp * ny) * r;
+    const double tb = acos(Bz / bm), pb = atan(By / Bx);
+    const double bxt = bm * cos(tb) * cos(pb), bxp = -bm * sin(tb) * sin(pb);
+    const double byt = bm * cos(tb) * sin(pb), byp = bm * sin(tb) * cos(pb);
+    const double bzt = -bm * sin(tb);
+    ii_t = dix * bxt + diy * byt + diz * bzt;
+    ii_p = dix * bxp + diy * byp;
+    az_t = dax * bxt + day * byt + daz * bzt;
+    az_p = dax * bxp + day * byp;
+  }
+  auto at = [&](int p) { return resp + (((((size_t)col * 7 + p) * nd + z) * nl + l) * 4); };
+  R *v = at(3), *ri = at(5), *ra = at(6);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const double to_in = (double)ri[s], to_az = (double)ra[s];
+    v[s] = (R)((double)v[s] * cp);
+    ri[s] = (R)(to_in * ii_t + to_az * az_t);
+    ra[s] = (R)(to_in * ii_p + to_az * az_p);
+  }
+}
+
+template <typename R>
+static int launch_transform(int ncol, int nd, int nl, const double *bvec, double theta, double phi, void *resp,
+                            hipStream_t stream) {
+  const long long n = (long long)ncol * nd * nl;
+  const long long blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffffLL) return SNAPI_ERR_ARG;
+  hipLaunchKernelGGL(transform_resp_kernel<R>, dim3((unsigned)blocks), dim3(256), 0, stream, ncol, nd, nl, bvec,
+                     theta, phi, static_cast<R *>(resp));
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+}
+
 static int run_resp(int precision, const snapi_line_table *lines, const snapi_ray *ray, int ncol, int nd,
                     const void *state, const void *dstate, int nlambda, const double *lam, void *stokes,
-                    void *out_host, int sink, const void *obs, const double *ws) {
+                    void *out_host, int sink, const void *obs, const double *ws, const double *bvec = nullptr,
+                    double phi = 0.0) {
   KLines kl;
   SynthArgs a;
   int rc = prepare_synth(lines, ray, ncol, nd, nlambda, kl, a);
@@ -591,7 +658,10 @@ static int run_resp(int precision, const snapi_line_table *lines, const snapi_ra
   // per-depth responses are formed in the atmosphere's frame only on the
   // mu = 1 ray (theta = pi), where atmosphere::transform_responses
   // (atmos/atmos_rad.cc:53-101) is the identity
-  if (!(cos(ray->theta) < -0.999999)) return SNAPI_ERR_UNSUPPORTED;
+  // (with the atmosphere-frame field given, any ray: the responses are
+  // transformed as the reference transforms them)
+  if (!bvec && !(cos(ray->theta) < -0.999999)) return SNAPI_ERR_UNSUPPORTED;
+  if (bvec && sink != SINK_RESP) return SNAPI_ERR_ARG;
   const size_t es = precision == SNAPI_FP64 ? 8 : 4;
   const size_t nf = a.nfield;
   const size_t sb = (size_t)ncol * nf * nd * es, db = 2 * sb, lb = (size_t)nlambda * 8;
@@ -603,18 +673,21 @@ static int run_resp(int precision, const snapi_line_table *lines, const snapi_ra
   if (chunk_s > (size_t)ncol) chunk_s = (size_t)ncol;
   const int chunk = (int)chunk_s;
   void *ds = nullptr, *dd = nullptr, *dstk = nullptr, *dout = nullptr, *dscr = nullptr, *dobs = nullptr;
-  double *dl = nullptr;
+  double *dl = nullptr, *dbv = nullptr;
+  const size_t bvb = bvec ? (size_t)ncol * 3 * nd * sizeof(double) : 0;
   rc = SNAPI_OK;
   if (hipMalloc(&ds, sb) != hipSuccess || hipMalloc(&dd, db) != hipSuccess ||
       hipMalloc((void **)&dl, lb) != hipSuccess || hipMalloc(&dstk, stb) != hipSuccess ||
       hipMalloc(&dout, ob) != hipSuccess || hipMalloc(&dscr, per_col * chunk) != hipSuccess ||
-      (sink == SINK_GRAD && hipMalloc(&dobs, stb) != hipSuccess)) {
+      (sink == SINK_GRAD && hipMalloc(&dobs, stb) != hipSuccess) ||
+      (bvec && hipMalloc((void **)&dbv, bvb) != hipSuccess)) {
     rc = SNAPI_ERR_DEVICE;
   } else {
     if (hipMemcpy(ds, state, sb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dd, dstate, db, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dl, lam, lb, hipMemcpyHostToDevice) != hipSuccess || hipMemset(dout, 0, ob) != hipSuccess ||
-        (dobs && hipMemcpy(dobs, obs, stb, hipMemcpyHostToDevice) != hipSuccess))
+        (dobs && hipMemcpy(dobs, obs, stb, hipMemcpyHostToDevice) != hipSuccess) ||
+        (dbv && hipMemcpy(dbv, bvec, bvb, hipMemcpyHostToDevice) != hipSuccess))
       rc = SNAPI_ERR_DEVICE;
     else
       rc = reset_status();
@@ -629,6 +702,9 @@ static int run_resp(int precision, const snapi_line_table *lines, const snapi_ra
       rc = launch_resp<double>(kl, a, ra, ncol, ds, dd, dl, dstk, dout, dscr, chunk, 0);
     else
       rc = launch_resp<float>(kl, a, ra, ncol, ds, dd, dl, dstk, dout, dscr, chunk, 0);
+    if (rc == SNAPI_OK && dbv)
+      rc = precision == SNAPI_FP64 ? launch_transform<double>(ncol, nd, nlambda, dbv, ray->theta, phi, dout, 0)
+                                   : launch_transform<float>(ncol, nd, nlambda, dbv, ray->theta, phi, dout, 0);
     if (rc == SNAPI_OK && hipDeviceSynchronize() != hipSuccess) rc = SNAPI_ERR_DEVICE;
     if (rc == SNAPI_OK) {
       if (hipMemcpy(stokes, dstk, stb, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -645,6 +721,7 @@ static int run_resp(int precision, const snapi_line_table *lines, const snapi_ra
   hipFree(dout);
   hipFree(dscr);
   hipFree(dobs);
+  hipFree(dbv);
   return rc;
 }
 
@@ -674,4 +751,45 @@ extern "C" int snapi_stokes_adjoint(int precision, const snapi_line_table *lines
   }
   return run_resp(precision, lines, ray, ncol, nd, state, dstate, nlambda, lam, tmp.data(), grad, SINK_GRAD, obs,
                   wstokes);
+}
+
+extern "C" int snapi_stokes_responses_atm(int precision, const snapi_line_table *lines, const snapi_ray *ray,
+                                          double phi, const double *bvec, int ncol, int nd, const void *state,
+                                          const void *dstate, int nlambda, const double *lam, void *stokes,
+                                          void *resp) {
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
+  if (!bvec) return SNAPI_ERR_ARG;
+  return run_resp(precision, lines, ray, ncol, nd, state, dstate, nlambda, lam, stokes, resp, SINK_RESP, nullptr,
+                  nullptr, bvec, phi);
+}
+
+extern "C" int snapi_transform_responses_dev(int precision, int ncol, int nd, int nlambda, const double *bvec_dev,
+                                             double theta, double phi, void *resp_dev, void *stream) {
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
+  if (ncol <= 0 || nd <= 0 || nlambda <= 0 || !bvec_dev || !resp_dev) return SNAPI_ERR_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return precision == SNAPI_FP64 ? launch_transform<double>(ncol, nd, nlambda, bvec_dev, theta, phi, resp_dev, st)
+                                 : launch_transform<float>(ncol, nd, nlambda, bvec_dev, theta, phi, resp_dev, st);
+}
+
+extern "C" int snapi_transform_responses(int precision, int ncol, int nd, int nlambda, const double *bvec,
+                                         double theta, double phi, void *resp) {
+  if (precision != SNAPI_FP64 && precision != SNAPI_MIXED) return SNAPI_ERR_ARG;
+  if (ncol <= 0 || nd <= 0 || nlambda <= 0 || !bvec || !resp) return SNAPI_ERR_ARG;
+  const size_t es = precision == SNAPI_FP64 ? 8 : 4;
+  const size_t rb = (size_t)ncol * 7 * nd * nlambda * 4 * es, bb = (size_t)ncol * 3 * nd * sizeof(double);
+  void *dr = nullptr;
+  double *db = nullptr;
+  int rc = SNAPI_OK;
+  if (hipMalloc(&dr, rb) != hipSuccess || hipMalloc((void **)&db, bb) != hipSuccess) rc = SNAPI_ERR_DEVICE;
+  else if (hipMemcpy(dr, resp, rb, hipMemcpyHostToDevice) != hipSuccess ||
+           hipMemcpy(db, bvec, bb, hipMemcpyHostToDevice) != hipSuccess)
+    rc = SNAPI_ERR_DEVICE;
+  if (rc == SNAPI_OK) rc = snapi_transform_responses_dev(precision, ncol, nd, nlambda, db, theta, phi, dr, nullptr);
+  if (rc == SNAPI_OK && (hipDeviceSynchronize() != hipSuccess ||
+                         hipMemcpy(resp, dr, rb, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = SNAPI_ERR_DEVICE;
+  if (dr) (void)hipFree(dr);
+  if (db) (void)hipFree(db);
+  return rc;
 }

@@ -17,6 +17,7 @@
 
 #include "rt_common.h"
 #include "synth_kernel.h"
+#include "autotune.h"
 
 namespace snapi {
 
@@ -56,29 +57,40 @@ __device__ __forceinline__ void synth_body(const KLines &lines, const SynthArgs 
 // four at 113 VGPRs without spills; on the 512 x 512 x 71 cube: 4 waves
 // 310 ms, 6 waves 214 ms, 7 waves 172 ms, 8 waves 160-166 ms.  The double
 // kernel keeps its registers (239 VGPRs; five wavefronts would spill and
-// measured 32 % slower).
-template <typename R> struct SynthKernel;
-__global__ __launch_bounds__(kSynthThreads, 8) void synth_kernel_f(const KLines lines, const SynthArgs args,
-                                                                  const float *__restrict__ state,
-                                                                  const double *__restrict__ lam,
-                                                                  float *__restrict__ stokes,
-                                                                  int *__restrict__ status,
-                                                                  const int *__restrict__ active, int active_div) {
-  synth_body<float>(lines, args, state, lam, stokes, status, active, active_div);
-}
-__global__ __launch_bounds__(kSynthThreads) void synth_kernel_d(const KLines lines, const SynthArgs args,
-                                                               const double *__restrict__ state,
-                                                               const double *__restrict__ lam,
-                                                               double *__restrict__ stokes, int *__restrict__ status,
-                                                               const int *__restrict__ active, int active_div) {
-  synth_body<double>(lines, args, state, lam, stokes, status, active, active_div);
-}
-template <> struct SynthKernel<float> {
-  static constexpr auto fn = synth_kernel_f;
+// measured 32 % slower).  Both come in a few occupancy bounds, among which
+// the autotuner picks per problem shape (autotune.h); every bound computes
+// the same results.
+#define SNAPI_SYNTH_KERNEL(NAME, R, BOUNDS)                                                                      \
+  __global__ BOUNDS void NAME(const KLines lines, const SynthArgs args, const R *__restrict__ state,             \
+                              const double *__restrict__ lam, R *__restrict__ stokes, int *__restrict__ status,  \
+                              const int *__restrict__ active, int active_div) {                                  \
+    synth_body<R>(lines, args, state, lam, stokes, status, active, active_div);                                  \
+  }
+SNAPI_SYNTH_KERNEL(synth_kernel_f, float, __launch_bounds__(kSynthThreads, 8))
+SNAPI_SYNTH_KERNEL(synth_kernel_f7, float, __launch_bounds__(kSynthThreads, 7))
+SNAPI_SYNTH_KERNEL(synth_kernel_f6, float, __launch_bounds__(kSynthThreads, 6))
+SNAPI_SYNTH_KERNEL(synth_kernel_d, double, __launch_bounds__(kSynthThreads))
+SNAPI_SYNTH_KERNEL(synth_kernel_d3, double, __launch_bounds__(kSynthThreads, 3))
+SNAPI_SYNTH_KERNEL(synth_kernel_d4, double, __launch_bounds__(kSynthThreads, 4))
+#undef SNAPI_SYNTH_KERNEL
+
+// Candidates of the autotuner, the default first.
+using SynthFnF = void (*)(const KLines, const SynthArgs, const float *, const double *, float *, int *, const int *,
+                          int);
+using SynthFnD = void (*)(const KLines, const SynthArgs, const double *, const double *, double *, int *,
+                          const int *, int);
+template <typename R> struct SynthKernels;
+template <> struct SynthKernels<float> {
+  static constexpr int n = 3;
+  static SynthFnF fn(int c) { return c == 1 ? synth_kernel_f7 : c == 2 ? synth_kernel_f6 : synth_kernel_f; }
 };
-template <> struct SynthKernel<double> {
-  static constexpr auto fn = synth_kernel_d;
+template <> struct SynthKernels<double> {
+  static constexpr int n = 3;
+  static SynthFnD fn(int c) { return c == 1 ? synth_kernel_d3 : c == 2 ? synth_kernel_d4 : synth_kernel_d; }
 };
+
+// Columns of the autotuner's sample: enough workgroups to fill the chip.
+constexpr int kSynthTuneColumns = 16384;
 
 template <typename R>
 static int launch_synth(const KLines &kl, const SynthArgs &a, const void *state, const double *lam, void *stokes,
@@ -86,10 +98,16 @@ static int launch_synth(const KLines &kl, const SynthArgs &a, const void *state,
   int threads = ((a.nlambda + 63) / 64) * 64;
   if (threads > kSynthThreads) threads = kSynthThreads;
   const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + RayGeo<R>::bytes(a.nd);
-  hipLaunchKernelGGL(SynthKernel<R>::fn, dim3(a.ncol), dim3(threads), smem, stream, kl, a,
-                     static_cast<const R *>(state), lam, static_cast<R *>(stokes), status, active,
-                     active_div > 0 ? active_div : 1);
-  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+  const int adiv = active_div > 0 ? active_div : 1;
+  auto launch = [&](int c, int ncol) {
+    hipLaunchKernelGGL(SynthKernels<R>::fn(c), dim3(ncol), dim3(threads), smem, stream, kl, a,
+                       static_cast<const R *>(state), lam, static_cast<R *>(stokes), status, active, adiv);
+    return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+  };
+  const TuneKey key{TUNE_SYNTH, sizeof(R) == 8 ? SNAPI_FP64 : SNAPI_MIXED, a.nd, kl.nline, kl.ncomp, a.nlambda, 0};
+  const int ns = a.ncol < kSynthTuneColumns ? a.ncol : kSynthTuneColumns;
+  const int c = tune_choose(key, SynthKernels<R>::n, 0, stream, true, [&](int cand) { return launch(cand, ns); });
+  return launch(c, a.ncol);
 }
 
 // Synthesis of the columns whose flag active[col / active_div] is set (the

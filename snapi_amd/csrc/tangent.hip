@@ -26,12 +26,12 @@
 #include <cmath>
 
 #include "resp_kernel.h"
+#include "autotune.h"
 #include "rt_common.h"
 #include "synth_kernel.h"
 
 namespace snapi {
 
-constexpr int kTanThreads = 64;
 // parameter rows of the local derivatives: the kinematic and magnetic ones,
 // then (kernels with thermodynamic directions) temperature and density
 enum { Q_VT = 0, Q_VL = 1, Q_B = 2, Q_TH = 3, Q_PH = 4, Q_T = 5, Q_N = 6, NQ = 5, NQ_TH = 7 };
@@ -729,7 +729,11 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
 // and the sweep waits on memory and LDS; measured 1.34x faster on the
 // inversion workload (four wavefronts, 128 VGPRs with 393 spilled, measured
 // 0.7x).  The double kernels keep their registers.  The _th variants carry the
-// temperature and density rows.
+// temperature and density rows.  Each comes in workgroups of 64, 128 and 256
+// work-items (one column, its wavelengths over the work-items), among which
+// the autotuner picks per problem shape (autotune.h): larger workgroups share
+// the column's staged tiles over more wavelengths at once, smaller ones keep
+// more columns resident when the tiles fill the LDS.
 #define SNAPI_TANGENT_KERNEL(NAME, R, NQR, BOUNDS)                                                                 \
   __global__ BOUNDS void NAME(const KLines lines, const SynthArgs args, const RespArgs ra,                        \
                               const R *__restrict__ state, const R *__restrict__ dstate,                          \
@@ -737,11 +741,38 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
                               int *__restrict__ status, int pass) {                                               \
     tangent_body<R, NQR>(lines, args, ra, state, dstate, lam, stokes, J, status, pass);                           \
   }
-SNAPI_TANGENT_KERNEL(tangent_kernel_f, float, NQ, __launch_bounds__(kTanThreads, 2))
-SNAPI_TANGENT_KERNEL(tangent_kernel_th_f, float, NQ_TH, __launch_bounds__(kTanThreads, 2))
-SNAPI_TANGENT_KERNEL(tangent_kernel_d, double, NQ, __launch_bounds__(kTanThreads))
-SNAPI_TANGENT_KERNEL(tangent_kernel_th_d, double, NQ_TH, __launch_bounds__(kTanThreads))
+SNAPI_TANGENT_KERNEL(tangent_kernel_f, float, NQ, __launch_bounds__(64, 2))
+SNAPI_TANGENT_KERNEL(tangent_kernel_th_f, float, NQ_TH, __launch_bounds__(64, 2))
+SNAPI_TANGENT_KERNEL(tangent_kernel_d, double, NQ, __launch_bounds__(64))
+SNAPI_TANGENT_KERNEL(tangent_kernel_th_d, double, NQ_TH, __launch_bounds__(64))
+SNAPI_TANGENT_KERNEL(tangent_kernel_f128, float, NQ, __launch_bounds__(128, 2))
+SNAPI_TANGENT_KERNEL(tangent_kernel_th_f128, float, NQ_TH, __launch_bounds__(128, 2))
+SNAPI_TANGENT_KERNEL(tangent_kernel_d128, double, NQ, __launch_bounds__(128))
+SNAPI_TANGENT_KERNEL(tangent_kernel_th_d128, double, NQ_TH, __launch_bounds__(128))
+SNAPI_TANGENT_KERNEL(tangent_kernel_f256, float, NQ, __launch_bounds__(256, 2))
+SNAPI_TANGENT_KERNEL(tangent_kernel_th_f256, float, NQ_TH, __launch_bounds__(256, 2))
+SNAPI_TANGENT_KERNEL(tangent_kernel_d256, double, NQ, __launch_bounds__(256))
+SNAPI_TANGENT_KERNEL(tangent_kernel_th_d256, double, NQ_TH, __launch_bounds__(256))
 #undef SNAPI_TANGENT_KERNEL
+
+template <typename R>
+using TanFn = void (*)(const KLines, const SynthArgs, const RespArgs, const R *, const R *, const double *, R *, R *,
+                       int *, int);
+// Candidates of the autotuner (workgroup sizes), the default first.
+constexpr int kTanCand = 3;
+constexpr int kTanCandThreads[kTanCand] = {64, 128, 256};
+static TanFn<float> tan_fn(float *, bool th, int c) {
+  if (c == 1) return th ? tangent_kernel_th_f128 : tangent_kernel_f128;
+  if (c == 2) return th ? tangent_kernel_th_f256 : tangent_kernel_f256;
+  return th ? tangent_kernel_th_f : tangent_kernel_f;
+}
+static TanFn<double> tan_fn(double *, bool th, int c) {
+  if (c == 1) return th ? tangent_kernel_th_d128 : tangent_kernel_d128;
+  if (c == 2) return th ? tangent_kernel_th_d256 : tangent_kernel_d256;
+  return th ? tangent_kernel_th_d : tangent_kernel_d;
+}
+// Columns of the autotuner's sample.
+constexpr int kTanTuneColumns = 16384;
 
 template <typename R>
 int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, const void *ds_, const void *dd_,
@@ -756,8 +787,10 @@ int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, 
   // and H- at the middle wavelength (the response path's) to keep the
   // background's derivatives wavelength-independent
   if (thermo && (!dd_ || (a.hminus && !a.hminus_mid))) return SNAPI_ERR_UNSUPPORTED;
-  int threads = ((a.nlambda + 63) / 64) * 64;
-  if (threads > kTanThreads) threads = kTanThreads;
+  const int wthreads = ((a.nlambda + 63) / 64) * 64;
+  // candidates whose workgroup the wavelengths fill
+  int ncand = 1;
+  while (ncand < kTanCand && kTanCandThreads[ncand] <= wthreads) ++ncand;
   const int npass = (ra.nnode + kTanMaxNodes - 1) / kTanMaxNodes;
   int *status = device_status_slot();
   if (!status) return SNAPI_ERR_DEVICE;
@@ -772,16 +805,18 @@ int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, 
     if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;
     const R *st = static_cast<const R *>(ds_), *dd = static_cast<const R *>(dd_);
     R *so = static_cast<R *>(dstk_), *Jo = static_cast<R *>(dJ_);
-    if (sizeof(R) == 4) {
-      auto k = th ? tangent_kernel_th_f : tangent_kernel_f;
-      hipLaunchKernelGGL(k, dim3(ncol), dim3(threads), smem, stream, kl, a, ra, (const float *)st, (const float *)dd,
-                         dl, (float *)so, (float *)Jo, status, pass);
-    } else {
-      auto k = th ? tangent_kernel_th_d : tangent_kernel_d;
-      hipLaunchKernelGGL(k, dim3(ncol), dim3(threads), smem, stream, kl, a, ra, (const double *)st,
-                         (const double *)dd, dl, (double *)so, (double *)Jo, status, pass);
-    }
-    if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;
+    auto launch = [&](int c, int nc) {
+      const int threads = kTanCandThreads[c] < wthreads ? kTanCandThreads[c] : wthreads;
+      hipLaunchKernelGGL(tan_fn((R *)nullptr, th, c), dim3(nc), dim3(threads), smem, stream, kl, a, ra, st, dd, dl,
+                         so, Jo, status, pass);
+      return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+    };
+    const TuneKey key{TUNE_TANGENT, sizeof(R) == 8 ? SNAPI_FP64 : SNAPI_MIXED, a.nd, kl.nline, kl.ncomp, a.nlambda,
+                      (th ? 16 : 0) + ng};
+    const int ns = ncol < kTanTuneColumns ? ncol : kTanTuneColumns;
+    const int c = tune_choose(key, ncand, 0, stream, true, [&](int cand) { return launch(cand, ns); });
+    const int rc = launch(c, ncol);
+    if (rc != SNAPI_OK) return rc;
   }
   return SNAPI_OK;
 }

@@ -16,7 +16,11 @@ uniform on [0, 2] km/s.  The thermodynamic fields of each column (electron
 and neutral-hydrogen densities, level populations, referent opacity,
 background) are the template's, rescaled with Boltzmann/Saha-type factors
 for the column's temperature: synthetic inputs of the right shape and
-magnitude, not a solution of the equation of state.
+magnitude, not a solution of the equation of state.  `cube_state_eos_dev`
+draws the same columns and solves their thermodynamics on the GPU as the
+reference builds a model atmosphere (hydrostatic equilibrium, the 12 x NX x
+NY x NZ model cube, the equation of state); that is the synthesis workload
+of bench.py.
 """
 import os
 
@@ -145,6 +149,91 @@ def cube_state(template, nx, ny, seed=1, dtype=np.float32):
         out[:, abi.NF_BASE + 2 * i] = st0[abi.NF_BASE + 2 * i][None] * fb
         out[:, abi.NF_BASE + 2 * i + 1] = st0[abi.NF_BASE + 2 * i + 1][None] * fb
     return np.ascontiguousarray(out.astype(dtype))
+
+
+def cube_draws(template, nx, ny, seed=1):
+    """The per-column draws of the cube workload (the same random sequence
+    as cube_state): temperature T [ncol][nd] (template + Gaussian random
+    field), |B| [G], inclination and azimuth [rad] in the atmosphere's frame,
+    vertical velocity vz [ncol][nd] [cm/s] and vmicro [cm/s] per column."""
+    rng = np.random.default_rng(seed)
+    st0 = template["state"]
+    ncol = nx * ny
+    dT = gaussian_random_field(nx, ny, 150.0, 8.0, rng).reshape(ncol)
+    logB = rng.uniform(np.log10(10.0), np.log10(3000.0), ncol)
+    cosi = rng.uniform(-1.0, 1.0, ncol)
+    azi = rng.uniform(0.0, 180.0, ncol)
+    v0 = rng.normal(0.0, 2e5, ncol)
+    vg = rng.normal(0.0, 0.5e5, ncol)
+    vmic = rng.uniform(0.0, 2e5, ncol)
+    logtau = template["logtau"]
+    T = st0[abi.F_T][None, :] + dT[:, None]
+    return dict(T=np.ascontiguousarray(T), B=10.0 ** logB, inc=np.arccos(cosi), azi=np.radians(azi),
+                vz=np.ascontiguousarray(v0[:, None] + vg[:, None] * logtau[None, :]), vmic=vmic, logtau=logtau)
+
+
+def model_cube(draws, nx, ny, Nt, ne, opref, rho, xp=np):
+    """The 12 x nx x ny x nd model cube (log tau, height, T, Pg, Pe, rho,
+    kappa_ref, |B|, vmicro, vz, inclination, azimuth: the parameters
+    atmosphere::read_spinor3d reads, atmos/atmos_fio.cc:79-127) of the drawn
+    columns with their hydrostatic densities Nt, ne, kappa_ref and rho
+    [ncol][nd].  xp: numpy, or torch for device tensors (then every input
+    array is a tensor on one device and the cube is float64 there)."""
+    T = draws["T"]
+    nd = T.shape[1]
+    ncol = nx * ny
+    if xp is np:
+        c = np.zeros((12, ncol, nd))
+        arr = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
+    else:
+        import torch
+        dev = T.device
+        c = torch.zeros((12, ncol, nd), dtype=torch.float64, device=dev)
+        arr = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev)  # noqa: E731
+    c[0] = arr(draws["logtau"])[None, :]
+    c[2] = T
+    c[3] = Nt * K_BOLTZ * T
+    c[4] = ne * K_BOLTZ * T
+    c[5] = rho
+    c[6] = opref
+    c[7] = arr(draws["B"])[:, None]
+    c[8] = arr(draws["vmic"])[:, None]
+    c[9] = arr(draws["vz"])
+    c[10] = arr(draws["inc"])[:, None]
+    c[11] = arr(draws["azi"])[:, None]
+    return c.reshape(12, nx, ny, nd)
+
+
+def cube_state_eos_dev(template, nx, ny, seed=1, stream=None):
+    """The cube workload with its thermodynamics solved on the GPU, as the
+    reference builds a model atmosphere: the drawn temperatures on the
+    template's log tau grid through hydrostatic equilibrium
+    (snapi_hequilibrium_dev, atmos_pp::enforce_hequilibrium) for Nt, ne,
+    kappa_ref and rho; the 12 x nx x ny x nd model cube in float32; the cube
+    to column states for the template's ray (snapi_cube_to_state_dev,
+    read_spinor3d with the ray projection); then the equation of state
+    (snapi_eos_state_dev: Ne, neutral H, referent opacity, Thomson and
+    bound-free background, LTE level populations of the lines, as popsetup
+    and compute_op_referent).  Returns (state [ncol][nfield][nd] float64 and
+    the float32 model cube [12][nx][ny][nd], both torch device tensors)."""
+    import torch
+    from . import chemeq, cube
+    ray = template["ray"]
+    dr = cube_draws(template, nx, ny, seed)
+    chem, chem_he, smap = eos_tables("cfg1_template", ray.lam_mid)
+    dev = torch.device("cuda")
+    ncol, nd = dr["T"].shape
+    T = torch.from_numpy(dr["T"]).to(dev)
+    lt = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(dr["logtau"], (ncol, nd)))).to(dev)
+    Nt, ne, kr, rho = (torch.empty((ncol, nd), dtype=torch.float64, device=dev) for _ in range(4))
+    chemeq.hequilibrium_dev(chem_he, T, lt, Nt, ne, kr, rho, stream=stream)
+    dd = dict(dr, T=T)
+    c32 = model_cube(dd, nx, ny, Nt, ne, kr, rho, xp=torch).float()
+    nf = abi.nfield(template["table"].nline)
+    st = torch.zeros((ncol, nf, nd), dtype=torch.float64, device=dev)
+    cube.cube_to_state_dev(c32.double(), nf, st, theta=ray.theta, phi=0.0, stream=stream)
+    chemeq.eos_state_dev(chem, smap, st, stream=stream)
+    return st, c32
 
 
 # the non-thermodynamic node set of the 14-node inversion workload (7 T

@@ -270,6 +270,49 @@ def build_resp(d):
     print("cfg0_resp: nlambda=%d" % nl)
 
 
+def build_resp_incl(d):
+    """Per-depth responses on an inclined ray (el = 2.4 rad, az = 0.6 rad) of
+    the 61-point column with a field and velocity varying with depth (|B|
+    300-1500 G, inclination 20-150 deg, azimuth -60..80 deg, vz 1 km/s):
+    the reference's obs_stokes_responses, whose angle and velocity responses
+    transform_responses (atmos/atmos_rad.cc:53-101) takes from the ray's
+    frame to the atmosphere's; with the atmosphere-frame field per depth."""
+    rows = []
+    for i, r in enumerate(cfg0_atmosphere()):
+        f = i / 60.0
+        r = list(r)
+        r[7] = 300.0 + 1200.0 * f
+        r[9] = 1e5
+        r[10] = math.radians(20.0 + 130.0 * f)
+        r[11] = math.radians(-60.0 + 140.0 * f)
+        rows.append(tuple(r))
+    lam = np.round(6301.5 + np.linspace(-0.3, 0.3, 31), 6)
+    setup_dir(d, rows, lam)
+    el, az = 2.4, 0.6
+    out = run(d, ["resp", "atmos.cfg", "lambda.txt", repr(el), repr(az)])
+    rec = parse_synth(out)
+    thermo, dpop, rs, bv = [], [], [], []
+    for ln in out:
+        t = ln.split()
+        if t and t[0] == "thermo":
+            thermo.append([float(x) for x in t[1:]])
+        elif t and t[0] == "dpop":
+            dpop.append([float(x) for x in t[1:]])
+        elif t and t[0] == "resp":
+            rs.append([float(x) for x in t[1:]])
+        elif t and t[0] == "bvec":
+            bv.append([float(x) for x in t[2:]])
+    rs = np.array(rs)
+    nd, nl = len(rows), len(lam)
+    resp = np.zeros((7, nd, nl, 4))
+    for r in rs:
+        resp[int(r[0]) - 1, int(r[1]) - 1, int(r[2]) - 1, :] = r[3:]
+    save_synth(rec, rows, "cfg0_resp_incl.npz",
+               extra=dict(thermo=np.array(thermo), dpop=np.array(dpop), resp=resp, bvec=np.array(bv).T.copy(),
+                          el=np.array(el), az=np.array(az)))
+    print("cfg0_resp_incl: nlambda=%d" % nl)
+
+
 def build_formal_scalar(d):
     """Scalar formal solution + Lambda* of the reference on its own scalar
     opacity (atmosphere::opacity/emissivity) for a few wavelengths, outgoing
@@ -1022,8 +1065,8 @@ def build_tau(d):
     print("tau_scale: %d depths" % a.shape[1])
 
 
-BUILDERS = ["cfg0", "cfg1_template", "resp", "formal_scalar", "lmstep", "interp", "conv", "ana", "nlte", "chemeq",
-            "nlteresp", "nodes", "fit", "tau", "psf", "reg", "fitreg", "nlterin"]
+BUILDERS = ["cfg0", "cfg1_template", "resp", "resp_incl", "formal_scalar", "lmstep", "interp", "conv", "ana", "nlte",
+            "chemeq", "nlteresp", "nodes", "fit", "tau", "psf", "reg", "fitreg", "nlterin"]
 
 
 def main():

@@ -105,3 +105,71 @@ def test_responses_cube_columns_against_oracle():
             print("  col %d %-5s max err %.2e, p99.9 %.2e, fraction above 2e-4: %.1e" % (
                 c, NAMES[p], e.max(), np.quantile(e, 0.999), frac), flush=True)
             assert frac < 1e-3 and e.max() < 3e-3, (c, NAMES[p])
+
+
+GOLD_INCL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cfg0_resp_incl.npz")
+
+
+def test_inclined_ray_responses_match_reference_and_oracle():
+    """snapi_stokes_responses_atm on an inclined ray (el 2.4, az 0.6) with a
+    field and velocity varying with depth: the responses in the atmosphere's
+    frame as the reference's obs_stokes_responses returns them."""
+    gi = workloads.response_column_from_dump(GOLD_INCL)
+    d = np.load(GOLD_INCL)
+    el, az = float(d["el"]), float(d["az"])
+    ray = abi.SnapiRay(theta=el, lam_mid=gi["ray"].lam_mid, tau_grid=1, hminus=1, hminus_mid=1)
+    print("inclined ray el %.2f az %.2f: nd=61, 31 wavelengths, 7 parameters" % (el, az), flush=True)
+    st, resp = api_extra.stokes_responses_atm(gi["table"], ray, az, d["bvec"], gi["state"], gi["dstate"],
+                                              gi["lam_vac"])
+    assert np.max(np.abs(st[0] - gi["stokes"])) / gi["stokes"][:, 0].max() < 1e-10
+    _, oresp, rc = _oracle.responses(gi["table"], ray, gi["state"][None], gi["dstate"][None], gi["lam_vac"],
+                                     fd_scale=0.1)
+    assert rc == 0
+    small = _oracle.transform_responses(oresp[0], d["bvec"], el, az)
+    _cmp(resp[0], small, 1e-4, "vs oracle (0.1 x reference steps)")
+    # the reference differences with unit steps; where such a quotient is
+    # off the derivative (on this column up to 1.1e-2 of the largest |B|
+    # response, where a step straddles a limiter switch), the oracle's own
+    # unit-step and 0.1-step quotients differ by the same amount
+    _, o1, _ = _oracle.responses(gi["table"], ray, gi["state"][None], gi["dstate"][None], gi["lam_vac"])
+    unit = _oracle.transform_responses(o1[0], d["bvec"], el, az)
+    ref = gi["resp"]
+    for p in range(7):
+        scale = np.maximum(np.abs(ref[p]).max(axis=(0, 1)), 1e-300)
+        e = np.abs(resp[0][p] - ref[p]) / scale
+        allow = np.abs(unit[p] - small[p]) / scale + 1e-4
+        print("  vs reference %-5s max err %.2e (its unit-step quotient is off by up to %.2e)"
+              % (NAMES[p], e.max(), (allow - 1e-4).max()), flush=True)
+        assert np.all(e <= allow), NAMES[p]
+    # the plain entry point refuses the inclined ray
+    with pytest.raises(RuntimeError):
+        api_extra.stokes_responses(gi["table"], ray, gi["state"], gi["dstate"], gi["lam_vac"])
+
+
+def test_transform_kernel_matches_oracle_and_is_the_identity_at_mu1(g):
+    rng = np.random.default_rng(6)
+    ncol, nd, nl = 3, 9, 5
+    bm = rng.uniform(10, 3000, (ncol, nd))
+    tb = rng.uniform(0.1, 3.0, (ncol, nd))
+    pb = rng.uniform(0.0, 2 * np.pi, (ncol, nd))
+    bv = np.stack([bm * np.sin(tb) * np.cos(pb), bm * np.sin(tb) * np.sin(pb), bm * np.cos(tb)], axis=1)
+    bv[0, :, 0] = 0.0  # a vanishing field: zero angle responses
+    r = rng.standard_normal((ncol, 7, nd, nl, 4))
+    print("transform of %d x 7 x %d x %d responses on the GPU" % (ncol, nd, nl), flush=True)
+    for theta, phi in ((2.4, 0.6), (np.pi, 0.0), (1.2, -2.0)):
+        got = api_extra.transform_responses(r, bv, theta, phi)
+        want = np.stack([_oracle.transform_responses(r[c], bv[c], theta, phi) for c in range(ncol)])
+        assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want)), (theta, phi)
+        got32 = api_extra.transform_responses(r.astype(np.float32), bv, theta, phi)
+        assert np.max(np.abs(got32 - want)) < 1e-5 * np.max(np.abs(want)), (theta, phi)
+        assert np.all(got[0, 5:7, 0] == 0.0)
+    # mu = 1, phi = 0: the atmosphere-frame entry point gives the plain one's responses
+    bvc = np.zeros((3, g["state"].shape[1]))
+    bm, inc, azi = (g["state"][f] for f in (abi.F_BMAG, abi.F_BINC, abi.F_BAZI))
+    bvc[0] = bm * np.sin(np.radians(inc)) * np.cos(np.radians(azi))
+    bvc[1] = bm * np.sin(np.radians(inc)) * np.sin(np.radians(azi))
+    bvc[2] = bm * np.cos(np.radians(inc))
+    s1, r1 = api_extra.stokes_responses(g["table"], g["rray"], g["state"], g["dstate"], g["lam_vac"])
+    s2, r2 = api_extra.stokes_responses_atm(g["table"], g["rray"], 0.0, bvc, g["state"], g["dstate"], g["lam_vac"])
+    assert np.array_equal(s1, s2)
+    assert np.max(np.abs(r1 - r2)) < 1e-12 * np.max(np.abs(r1))

@@ -643,9 +643,11 @@ def lm_leg(torch, dev, tmpl, lam_d, stream, side, nfd, tnode_side=48, tnode_legs
         nodes = torch.empty((n, nn), dtype=torch.float64, device=dev)
         chi2 = torch.empty(n, dtype=torch.float64, device=dev)
         iters = torch.empty(n, dtype=torch.int32, device=dev)
-        # warm-up on a few pixels (kernel loading), not timed
-        invert.invert_dev(table, ray, model, st_t[:64], lam_d, ob_t[:64], mk_t[:64], start[:64], nodes[:64],
-                          chi2[:64], iters[:64], wstokes=ws, max_iter=2, precision=precision, jacobian=jacobian,
+        # untimed: kernel loading, and the autotuner's choices for the launch
+        # shapes of a batch this large (one iteration of up to 16384 pixels)
+        nw = min(n, 16384)
+        invert.invert_dev(table, ray, model, st_t[:nw], lam_d, ob_t[:nw], mk_t[:nw], start[:nw], nodes[:nw],
+                          chi2[:nw], iters[:nw], wstokes=ws, max_iter=1, precision=precision, jacobian=jacobian,
                           stream=stream)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -732,8 +734,11 @@ def tnode_inversion(torch, dev, tmpl, lam_d, stream, ws, side=48, legs=("mixed_t
         st_p = st64.to(dt_).contiguous()
         ob_p = obs64.to(dt_).contiguous()
         mk_p = mask.to(dt_).contiguous()
-        invert.invert_eos_dev(table, ray, model, chem, chem_he, smap, st_p[:16], lam_d, ob_p[:16], mk_p[:16],
-                              start[:16], nodes[:16], chi2[:16], iters[:16], wstokes=ws, max_iter=2,
+        # untimed: kernel loading, and the autotuner's choices for the launch
+        # shapes of a batch this large (one iteration of up to 16384 pixels)
+        nw = min(npix, nslot if nslot > 0 else 16384, 16384)
+        invert.invert_eos_dev(table, ray, model, chem, chem_he, smap, st_p[:nw], lam_d, ob_p[:nw], mk_p[:nw],
+                              start[:nw], nodes[:nw], chi2[:nw], iters[:nw], wstokes=ws, max_iter=1,
                               precision=precision, stream=stream, analytic_maps=analytic, jacobian=jacobian)
         torch.cuda.synchronize()
         t1 = time.perf_counter()

@@ -29,4 +29,18 @@ struct TuneKey {
 int tune_choose(const TuneKey &key, int ncand, int dflt, hipStream_t stream, bool tune,
                 const std::function<int(int)> &run);
 
+// Problem-size class of a launch over ncol columns, for TuneKey::extra: 0 at
+// or above the sample size (the choice is made on a sample of that many
+// columns), else the bit length of ncol (the choice is made on the whole
+// call), so that a small call never decides for a large one.
+inline int tune_size_class(int ncol, int sample) {
+  if (ncol >= sample) return 0;
+  int b = 0;
+  while (ncol > 0) {
+    ++b;
+    ncol >>= 1;
+  }
+  return b;
+}
+
 }  // namespace snapi

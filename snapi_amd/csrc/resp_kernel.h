@@ -31,6 +31,7 @@ struct RespArgs {
   const void *nmap2;  // [ncol][nnode][nd] or null
   const void *mask;   // [ncol][nlambda] or null
   const int *active;  // [ncol] or null: columns with 0 are skipped
+  int staged_derivs;  // set by launch_resp: local derivatives from per-column LDS tiles
 };
 
 

@@ -27,6 +27,7 @@
 #include "rt_common.h"
 #include "synth_kernel.h"
 #include "resp_kernel.h"
+#include "deriv_kernel.h"
 
 namespace snapi {
 
@@ -296,7 +297,7 @@ template <typename R> __device__ __forceinline__ void kmul_adj(const R yb[4], co
   kb[5] += yb[2] * x[3] - yb[3] * x[2];   // f
 }
 
-template <typename R>
+template <typename R, int NS, bool STAGED>
 __global__ __launch_bounds__(kRespThreads) void resp_kernel(const KLines lines, const SynthArgs args,
                                                            const RespArgs ra, const R *__restrict__ state,
                                                            const R *__restrict__ dstate,
@@ -312,11 +313,23 @@ __global__ __launch_bounds__(kRespThreads) void resp_kernel(const KLines lines, 
   const R *st = state + (size_t)col * nf * nd;
   const R *dst = dstate + (size_t)col * 2 * nf * nd;
   ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
+  // the wavelength-independent derivative factors, staged once per column
+  // (when they fit and the background's T and Nt derivatives do not depend
+  // on the wavelength: H- at the middle wavelength, as the response path
+  // takes it); otherwise local_derivs evaluates everything per wavelength
+  R *sd = sm + ColumnTile<R>::bytes(nd, lines.nline) / sizeof(R);
+  DerivTile<R> dtile = DerivTile<R>::carve(sd, nd, lines.nline, 0);
+  ThermoTile<R> ttile = ThermoTile<R>::carve(sd + DerivTile<R>::bytes(nd, lines.nline, 0, false) / sizeof(R), nd,
+                                             lines.nline);
   stage_column<R, R>(lines, args, st, tile);
+  if (STAGED) {
+    stage_deriv<R, R>(lines, args, st, dtile);
+    stage_thermo<R, R>(lines, args, st, dst, ttile);
+  }
   __syncthreads();
   const bool up = args.ct < 0.0;
   const R mct = (R)(-args.ct);
-  const int ns = ra.nseed;
+  constexpr int ns = NS;  // seeds: 4 (the emergent Stokes components) or 1 (the gradient)
   auto zof = [&](int k) { return up ? nd - 1 - k : k; };
   for (int l = threadIdx.x; l < nl; l += blockDim.x) {
     const long long w = (long long)cl * nl + l;
@@ -510,6 +523,30 @@ __global__ __launch_bounds__(kRespThreads) void resp_kernel(const KLines lines, 
     // keep g[s][p] of every depth in the adjoint area of the scratch
     for (int k = 0; k < nd; ++k) {
       const int z = zof(k);
+      if constexpr (STAGED) {
+        R Kz[7], Ez[4], G[NQ_TH][11];
+        deriv_at<R, true, NQ_TH>(lines, args, tile, dtile, ttile, c, z, (1u << NQ_TH) - 1u, Kz, Ez, G);
+        for (int s = 0; s < ns; ++s) {
+          R Kb[7], Eb[4];
+          for (int i = 0; i < 7; ++i) Kb[i] = scr.adj(k, i, s, w);
+          for (int i = 0; i < 4; ++i) Eb[i] = scr.adj(k, 7 + i, s, w);
+#pragma unroll
+          for (int p = 0; p < 7; ++p) {
+            const int q = qrow(p);
+            R g = R(0);
+#pragma unroll
+            for (int i = 0; i < 7; ++i) g += Kb[i] * G[q][i];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) g += Eb[i] * G[q][7 + i];
+            if (ra.sink == SINK_RESP)
+              out[((((size_t)col * 7 + p) * nd + z) * nl + l) * 4 + s] = g;
+            else if (ra.sink == SINK_GRAD)
+              atomicAdd(&out[((size_t)col * 7 + p) * nd + z], g);
+            else
+              scr.adj(k, p, s, w) = g;  // entries 0..6 of depth k are no longer read
+          }
+        }
+      } else {
       double dK[7][7], dE[7][4];
       local_derivs<R>(lines, args, st, dst, z, lm, dK, dE);
       for (int s = 0; s < ns; ++s) {
@@ -527,6 +564,7 @@ __global__ __launch_bounds__(kRespThreads) void resp_kernel(const KLines lines, 
           else
             scr.adj(k, p, s, w) = (R)g;  // entries 0..6 of depth k are no longer read
         }
+      }
       }
     }
     if (ra.sink == SINK_NODE) {
@@ -553,7 +591,10 @@ int launch_resp(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, con
                        const double *dl, void *dstk_, void *dout_, void *dscr_, int chunk, hipStream_t stream) {
   int threads = ((a.nlambda + 63) / 64) * 64;
   if (threads > kRespThreads) threads = kRespThreads;
-  const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline);
+  size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline);
+  const size_t sderiv = DerivTile<R>::bytes(a.nd, kl.nline, 0, false) + ThermoTile<R>::bytes(a.nd, kl.nline);
+  ra.staged_derivs = (!a.hminus || a.hminus_mid) && smem + sderiv <= 65536;
+  if (ra.staged_derivs) smem += sderiv;
   int *status = device_status_slot();
   if (!status) return SNAPI_ERR_DEVICE;
   for (int c0 = 0; c0 < ncol; c0 += chunk) {
@@ -561,7 +602,9 @@ int launch_resp(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, con
     ra.col0 = c0;
     ra.nwork = (long long)nc * a.nlambda;
     Scratch<R> scr{static_cast<R *>(dscr_), ra.nwork, a.nd};
-    hipLaunchKernelGGL(resp_kernel<R>, dim3(nc), dim3(threads), smem, stream, kl, a, ra, static_cast<const R *>(ds_),
+    auto kern = ra.staged_derivs ? (ra.nseed == 1 ? resp_kernel<R, 1, true> : resp_kernel<R, 4, true>)
+                                 : (ra.nseed == 1 ? resp_kernel<R, 1, false> : resp_kernel<R, 4, false>);
+    hipLaunchKernelGGL(kern, dim3(nc), dim3(threads), smem, stream, kl, a, ra, static_cast<const R *>(ds_),
                        static_cast<const R *>(dd_), dl, scr, static_cast<R *>(dstk_), static_cast<R *>(dout_),
                        status);
     if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;

@@ -104,7 +104,8 @@ static int launch_synth(const KLines &kl, const SynthArgs &a, const void *state,
                        static_cast<const R *>(state), lam, static_cast<R *>(stokes), status, active, adiv);
     return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
   };
-  const TuneKey key{TUNE_SYNTH, sizeof(R) == 8 ? SNAPI_FP64 : SNAPI_MIXED, a.nd, kl.nline, kl.ncomp, a.nlambda, 0};
+  const TuneKey key{TUNE_SYNTH, sizeof(R) == 8 ? SNAPI_FP64 : SNAPI_MIXED, a.nd, kl.nline, kl.ncomp, a.nlambda,
+                    tune_size_class(a.ncol, kSynthTuneColumns)};
   const int ns = a.ncol < kSynthTuneColumns ? a.ncol : kSynthTuneColumns;
   const int c = tune_choose(key, SynthKernels<R>::n, 0, stream, true, [&](int cand) { return launch(cand, ns); });
   return launch(c, a.ncol);

@@ -63,3 +63,27 @@ def test_temperature_node_jacobian_against_oracle(pix):
         assert es < 1e-10
         assert np.quantile(e, 0.99) < 1e-4 and e.max() < 3e-3, i
     assert np.all(np.abs(J1[:, :, :7]).max(axis=1) > 0)
+
+
+@pytest.mark.parametrize("jacobian,analytic", [(abi.SNAPI_JAC_TANGENT, True), (abi.SNAPI_JAC_TANGENT, False),
+                                               (abi.SNAPI_JAC_ADJOINT, False)])
+def test_mixed_temperature_node_jacobian_against_fp64(pix, jacobian, analytic):
+    """The mixed-precision Jacobian of the 14-node model (float opacity,
+    formal solution and sweep) against the FP64 one at the same nodes: the
+    inversion's bar is 1e-3 of each (pixel, node) column's largest value, and
+    the Stokes vector within 1e-4 of the continuum."""
+    p = pix
+    t = p["t"]
+    print("14-node Jacobian, mixed vs FP64: jacobian %d, analytic maps %s" % (jacobian, analytic), flush=True)
+    args = (t["table"], t["ray"], p["model"], p["chem"], p["chem_he"], p["smap"])
+    s64, J64 = invert.node_jacobian_eos(*args, p["st"], p["v"], p["lam"], precision=abi.SNAPI_FP64,
+                                        analytic_maps=analytic, jacobian=jacobian)
+    s32, J32 = invert.node_jacobian_eos(*args, p["st"].astype(np.float32), p["v"], p["lam"],
+                                        precision=abi.SNAPI_MIXED, analytic_maps=analytic, jacobian=jacobian)
+    es = np.max(np.abs(s32.astype(np.float64) - s64)) / np.max(s64[..., 0])
+    scale = np.maximum(np.abs(J64).max(axis=1, keepdims=True), 1e-300)
+    e = np.abs(J32.astype(np.float64) - J64) / scale
+    print("  Stokes %.2e Ic; J max %.2e of each column's largest value (T nodes %.2e)"
+          % (es, e.max(), e[:, :, :7].max()), flush=True)
+    assert es < 1e-4
+    assert e.max() < 1e-3

@@ -227,6 +227,14 @@ def main():
     if not args.no_nlteresp:
         nresp = nlteresp_leg(torch, dev, stream, args.nlteresp_cols, args.nlteresp_systems, not args.no_cpu)
 
+    if lm is not None and he is not None and he.get("roofline"):
+        # the inversion's dominant kernel is the hydrostatic march (the
+        # kernel profile of the 14-node leg); its roofline as the
+        # hequilibrium leg of this run measures it
+        lm["roofline"] = dict(he["roofline"], kernel="hequilibrium_kernel",
+                              note="dominant kernel of the 14-node inversion (28 % of its GPU time, with the "
+                                   "equation-of-state pass 26 % and the tangent sweeps 28 %); measured in the "
+                                   "hequilibrium leg of this run")
     line = {"metric": METRIC, "value": value, "unit": "lambda-point formal solves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
@@ -238,6 +246,20 @@ def main():
             "cpu_baseline": cpu, "accuracy": acc, "fp64_gpu": fp64, "lm": lm, "ali": alires,
             "chemeq": chem, "hequilibrium": he, "nlte_responses": nresp}
     print(json.dumps(line))
+
+
+def chem_roofline(key, per, units, ms, kernel):
+    """FP64 VALU roofline of a chemistry kernel: the frozen operation count
+    (profiles/chem_flops.json, counted through the FP64 restatement of the
+    same workload) times the units of one call over the call's time."""
+    ff = os.path.join(REPO, "profiles", "chem_flops.json")
+    if not os.path.exists(ff):
+        return None
+    fpu = json.load(open(ff))[key][per]
+    achieved = fpu * units / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None, "kernel": kernel,
+            "flops_per_unit": round(fpu, 1), "unit_counted": per, "flops_source": "counted", "ms_per_call": ms}
 
 
 def chemeq_leg(torch, dev, stream, side, with_cpu):
@@ -279,6 +301,7 @@ def chemeq_leg(torch, dev, stream, side, with_cpu):
            "singular": int(np.sum(itn < 0))}
     log("chemeq: %d points in %.2f ms = %.4g points/s; Newton steps mean %.2f max %d"
         % (npt, ms, res["points_per_s"], itn.mean(), itn.max()))
+    roof = chem_roofline("chemeq", "flops_per_point", npt, ms, "chemeq_kernel")
     cpu = None
     if with_cpu:
         sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -300,7 +323,8 @@ def chemeq_leg(torch, dev, stream, side, with_cpu):
             % (cpu["value"], nthreads, res["ne_rel_err_vs_oracle"], res["steps_equal_fraction"]))
     return {"workload": "chemical equilibrium + LTE populations + referent continuum opacity, %d columns x %d depths, %d elements, %d molecules, "
                         "%d levels, FP64" % (ncol, nd, tb.nelement, tb.nmolecule, tb.nlevel),
-            "value": res["points_per_s"], "unit": "depth points/s", "fp64": res, "cpu_baseline": cpu}
+            "value": res["points_per_s"], "unit": "depth points/s", "fp64": res, "roofline": roof,
+            "cpu_baseline": cpu}
 
 
 def hequilibrium_leg(torch, dev, stream, side, with_cpu):
@@ -336,6 +360,8 @@ def hequilibrium_leg(torch, dev, stream, side, with_cpu):
            "mean_iterations": float(itn.mean()), "max_iterations": int(itn.max())}
     log("hequilibrium: %d columns in %.2f ms = %.4g columns/s (%.4g chemical equilibria/s); iterations mean %.2f max %d"
         % (ncol, ms, res["columns_per_s"], res["solves_per_s"], itn.mean(), itn.max()))
+    roof = chem_roofline("hequilibrium", "flops_per_equilibrium", solves, ms,
+                         "hequilibrium_kernel + chemeq_kernel (the final pass)")
     cpu = None
     if with_cpu:
         sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -356,7 +382,7 @@ def hequilibrium_leg(torch, dev, stream, side, with_cpu):
             % (cpu["value"], nthreads, res["Nt_rel_err_vs_oracle"], same.mean()))
     return {"workload": "hydrostatic equilibrium (chemical equilibrium + referent opacity per iteration), "
                         "%d columns x %d depths, FP64" % (ncol, nd),
-            "value": res["columns_per_s"], "unit": "columns/s", "fp64": res, "cpu_baseline": cpu}
+            "value": res["columns_per_s"], "unit": "columns/s", "fp64": res, "roofline": roof, "cpu_baseline": cpu}
 
 
 def nlteresp_leg(torch, dev, stream, ncol, nsys, with_cpu):

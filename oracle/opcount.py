@@ -8,6 +8,7 @@ roofline's algorithmic work: the synthesis kernel is timed, the count is
 not re-derived at bench time but read from the table this script writes.
 
     python oracle/opcount.py            # writes profiles/synth_flops.json
+    python oracle/opcount.py chem       # writes profiles/chem_flops.json
 
 Workloads (the ones the synthesis roofline is quoted on):
   cube512: the 512 x 512 x 71 cube of bench.py (seed 1, thermodynamics
@@ -16,6 +17,15 @@ Workloads (the ones the synthesis roofline is quoted on):
            default_rng(5).choice(262144, 64, replace=False)), built by the
            oracle's restatement of the same chain.
   cfg0:    the reference's one-column case (its depths and wavelength grid).
+
+Chemistry (oracle/opcount_chem.cc, the oracle's chemical equilibrium and
+hydrostatic march with the counting scalar):
+  chemeq:       bench.py's chemeq workload (65,536 columns of 61 depths),
+                its first 64 columns: operations per depth point.
+  hequilibrium: bench.py's hydrostatic-equilibrium workload (16,384 columns
+                of 61 depths), its first 64 columns: operations per column
+                (the march and the final pass) and per chemical
+                equilibrium solved.
 """
 import json
 import os
@@ -95,7 +105,83 @@ def cfg0_case():
     return t["table"], t["ray"], t["state"][None].astype(np.float64), t["lam_vac"]
 
 
+CHEM_BIN = os.path.join(HERE, "_ref", "opcount_chem")
+
+
+def build_chem():
+    os.makedirs(os.path.dirname(CHEM_BIN), exist_ok=True)
+    src = os.path.join(HERE, "opcount_chem.cc")
+    deps = [src, os.path.join(HERE, "opcount.cc"), os.path.join(HERE, "snapi_oracle_chem.c"),
+            os.path.join(HERE, "snapi_oracle.h")]
+    if os.path.exists(CHEM_BIN) and all(os.path.getmtime(CHEM_BIN) >= os.path.getmtime(d) for d in deps):
+        return CHEM_BIN
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas", "-o", CHEM_BIN, src],
+                   check=True)
+    return CHEM_BIN
+
+
+def count_chem(table, T, X, mode):
+    """Per column: (counted operations, raw operations, equilibria, Ne
+    (mode 0) or Nt (mode 1) [nd]).  mode 0: X = Nt, chemical equilibrium of
+    every point; mode 1: X = log tau, the hydrostatic march."""
+    import ctypes
+    exe = build_chem()
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    ncol, nd = T.shape
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "in.bin")
+        with open(p, "wb") as f:
+            f.write(struct.pack("<3i", mode, ncol, nd))
+            f.write(ctypes.string_at(ctypes.addressof(table), ctypes.sizeof(table)))
+            f.write(T.tobytes())
+            f.write(X.tobytes())
+        out = subprocess.run([exe, p], capture_output=True, text=True, check=True).stdout.split("\n")
+    res = []
+    for line in out:
+        if not line.strip():
+            continue
+        w = line.split()
+        res.append((int(w[1]), int(w[2]), int(w[3]), np.array([float.fromhex(x) for x in w[4:]])))
+    return res
+
+
+def chem_main():
+    sys.path.insert(0, REPO)
+    from snapi_amd import workloads
+    table = {}
+    p = workloads.chemeq_problem(256, 256)
+    res = count_chem(p["table"], p["T"][:64], p["Nt"][:64], 0)
+    c = np.array([r[0] for r in res], dtype=np.float64)
+    nd = p["T"].shape[1]
+    table["chemeq"] = {"flops_per_point": float(c.sum() / (64 * nd)),
+                       "raw_per_point": float(sum(r[1] for r in res) / (64 * nd)), "columns": 64, "depths": nd,
+                       "counted_per_column": [int(x) for x in c],
+                       "workload": "bench.py chemeq leg (chemeq_problem 256 x 256), first 64 columns"}
+    print("chemeq", table["chemeq"], flush=True)
+    p = workloads.hequilibrium_problem(128, 128)
+    res = count_chem(p["table"], p["T"][:64], p["logtau"][:64], 1)
+    c = np.array([r[0] for r in res], dtype=np.float64)
+    neq = np.array([r[2] for r in res], dtype=np.float64)
+    nd = p["T"].shape[1]
+    table["hequilibrium"] = {"flops_per_column": float(c.mean()), "flops_per_equilibrium": float(c.sum() / neq.sum()),
+                             "equilibria_per_column": float(neq.mean()),
+                             "raw_per_column": float(np.mean([r[1] for r in res])), "columns": 64, "depths": nd,
+                             "counted_per_column": [int(x) for x in c],
+                             "workload": "bench.py hequilibrium leg (hequilibrium_problem 128 x 128), first 64 "
+                                         "columns; march iterations plus the final pass"}
+    print("hequilibrium", table["hequilibrium"], flush=True)
+    table["method"] = ("oracle/opcount_chem.cc: snapi_oracle_chem.c's chemeq_point and orc_hequilibrium with the "
+                       "op-counting double of oracle/opcount.cc; add, sub, mul, div, sqrt, exp, log, log10 = 1, "
+                       "pow = 3; each (operation, operands) counted once per column")
+    table["command"] = "python oracle/opcount.py chem"
+    with open(os.path.join(REPO, "profiles", "chem_flops.json"), "w") as f:
+        json.dump(table, f, indent=1)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "chem":
+        return chem_main()
     table = {}
     for name, make in (("cfg0", cfg0_case), ("cube512", cube512_sample)):
         tb, ray, st, lam = make()

@@ -537,7 +537,7 @@ int orc_hequilibrium_tol(const snapi_chem_table *tb, int nd, const double *T, co
 int orc_he_tangent(const snapi_chem_table *tb, int nd, const double *T, const double *logtau, const double *Nt,
                    int ndir, const double *dT, double *dNt) {
   if (nd < 1 || ndir < 0 || !(tb->lambda_ref > 0.0)) return -1;
-  double *r = malloc(sizeof(double) * 3 * nd), *aT = r + nd, *aN = r + 2 * nd;
+  double *r = (double *)malloc(sizeof(double) * 3 * nd), *aT = r + nd, *aN = r + 2 * nd;
   for (int z = 0; z < nd; ++z) {
     double e, op, nh, rv[5];
     const double dNs = 1E-3 * Nt[z];

@@ -593,7 +593,13 @@ int launch_resp(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, con
   if (threads > kRespThreads) threads = kRespThreads;
   size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline);
   const size_t sderiv = DerivTile<R>::bytes(a.nd, kl.nline, 0, false) + ThermoTile<R>::bytes(a.nd, kl.nline);
-  ra.staged_derivs = (!a.hminus || a.hminus_mid) && smem + sderiv <= 65536;
+  // the staged T and Nt rows take H- at the middle wavelength; with H- at
+  // every wavelength they hold only when no output reads those rows (a node
+  // Jacobian of kinematic and magnetic nodes)
+  bool thermo_rows = ra.sink != SINK_NODE;
+  for (int n = 0; n < ra.nnode && !thermo_rows; ++n)
+    thermo_rows = ra.pidx[n] < 2 || (ra.nmap2 && ra.pidx2[n] >= 0);
+  ra.staged_derivs = (!a.hminus || a.hminus_mid || !thermo_rows) && smem + sderiv <= 65536;
   if (ra.staged_derivs) smem += sderiv;
   int *status = device_status_slot();
   if (!status) return SNAPI_ERR_DEVICE;

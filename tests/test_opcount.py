@@ -69,3 +69,44 @@ def test_frozen_table_matches_a_fresh_count(sample):
     fresh = opcount.count(tb, ray, st[:1], lam)[0][0]
     assert fresh == c["counted_per_column"][0]
     assert np.isclose(c["flops_per_lambda_point"], np.mean(c["counted_per_column"]) / lam.size)
+
+
+# ---- chemistry (oracle/opcount_chem.cc) ----
+
+@pytest.fixture(scope="module")
+def chem_cases():
+    from snapi_amd import workloads
+    return workloads.chemeq_problem(4, 4), workloads.hequilibrium_problem(4, 4)
+
+
+def test_chem_harness_reproduces_the_oracle_bit_for_bit(chem_cases):
+    pc, ph = chem_cases
+    res = opcount.count_chem(pc["table"], pc["T"][:2], pc["Nt"][:2], 0)
+    ne = _oracle.chemeq(pc["table"], pc["T"][:2], pc["Nt"][:2], nthreads=1)[0]
+    for c in range(2):
+        assert np.array_equal(res[c][3], ne[c])
+    res = opcount.count_chem(ph["table"], ph["T"][:2], ph["logtau"][:2], 1)
+    Nt, _, _, _, it = _oracle.hequilibrium(ph["table"], ph["T"][:2], ph["logtau"][:2], nthreads=1)
+    for c in range(2):
+        assert np.array_equal(res[c][3], Nt[c])
+        assert res[c][2] == int(it[c].sum()) + ph["T"].shape[1]  # the march's iterations plus the final pass
+
+
+def test_chem_repeats_are_counted_once(chem_cases):
+    """A column's points given twice: the raw count doubles, the counted
+    operations stay."""
+    pc, _ = chem_cases
+    one = opcount.count_chem(pc["table"], pc["T"][:1], pc["Nt"][:1], 0)[0]
+    two = opcount.count_chem(pc["table"], np.tile(pc["T"][:1], 2), np.tile(pc["Nt"][:1], 2), 0)[0]
+    assert two[0] == one[0] and two[1] == 2 * one[1]
+
+
+def test_frozen_chem_table_matches_a_fresh_count():
+    from snapi_amd import workloads
+    table = json.load(open(os.path.join(REPO, "profiles", "chem_flops.json")))
+    pc = workloads.chemeq_problem(256, 256)
+    assert opcount.count_chem(pc["table"], pc["T"][:1], pc["Nt"][:1], 0)[0][0] == \
+        table["chemeq"]["counted_per_column"][0]
+    ph = workloads.hequilibrium_problem(128, 128)
+    assert opcount.count_chem(ph["table"], ph["T"][:1], ph["logtau"][:1], 1)[0][0] == \
+        table["hequilibrium"]["counted_per_column"][0]

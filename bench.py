@@ -589,11 +589,26 @@ def ali_leg(torch, dev, stream, side, with_cpu):
             % (cpu["value"], nthreads, cpu["fp64_gpu_populations_rel_err_vs_oracle"],
                cpu["mixed_populations_rel_err_vs_oracle"]))
     syn = ca8542_leg(torch, dev, stream, p, pm, with_cpu)
+    # roofline of the accumulation (ali_acc_kernel: opacity, formal solution
+    # and atom::add sums per (column, ray, wavelength), float formal solution
+    # in the mixed path): the frozen operation count of one formal solution
+    # of this workload (profiles/ali_flops.json) times the whole iteration's
+    # rate, rate solves and convergence checks included, so a lower bound
+    roof = None
+    ff = os.path.join(REPO, "profiles", "ali_flops.json")
+    if os.path.exists(ff):
+        fa = json.load(open(ff))["ali"]
+        if (fa["depths"], fa["wavelengths"], fa["rays"]) == (nd, nlam, nray):
+            ach = fa["flops_per_formal_solve"] * mixed["formal_solves_per_s"] / 1e12
+            roof = {"bound": "valu", "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(ach / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": "ali_acc_kernel",
+                    "flops_per_formal_solve": round(fa["flops_per_formal_solve"], 1), "flops_source": "counted",
+                    "note": "rate of whole ALI iterations (rate solves and convergence checks included)"}
     return {"workload": "NLTE Ca II 5 levels + continuum by ALI, %d columns x %d depths, %d wavelengths, %d rays, "
                         "to relative change 1e-4" % (ncol, nd, nlam, nray),
             "value": mixed["formal_solves_per_s"], "unit": "ALI formal solves/s (column x ray x wavelength)",
-            "mixed": mixed, "fp64_gpu": fp64, "populations_rel_err_mixed_vs_fp64": err, "cpu_baseline": cpu,
-            "synthesis_8542": syn}
+            "mixed": mixed, "fp64_gpu": fp64, "populations_rel_err_mixed_vs_fp64": err, "roofline": roof,
+            "cpu_baseline": cpu, "synthesis_8542": syn}
 
 
 def ca8542_leg(torch, dev, stream, p, pops, with_cpu):

@@ -9,6 +9,7 @@ not re-derived at bench time but read from the table this script writes.
 
     python oracle/opcount.py            # writes profiles/synth_flops.json
     python oracle/opcount.py chem       # writes profiles/chem_flops.json
+    python oracle/opcount.py ali        # writes profiles/ali_flops.json
 
 Workloads (the ones the synthesis roofline is quoted on):
   cube512: the 512 x 512 x 71 cube of bench.py (seed 1, thermodynamics
@@ -26,6 +27,14 @@ hydrostatic march with the counting scalar):
                 of 61 depths), its first 64 columns: operations per column
                 (the march and the final pass) and per chemical
                 equilibrium solved.
+
+ALI (oracle/opcount_ali.cc, the oracle's accumulation pass with the counting
+scalar):
+  ali:          bench.py's ALI workload (64 x 64 columns of 81 depths, 220
+                wavelengths, 10 rays) at its starting populations, its first
+                8 columns: operations per formal solution (column x ray x
+                wavelength, with its opacity over the depths and its share
+                of the atom::add sums).
 """
 import json
 import os
@@ -179,9 +188,83 @@ def chem_main():
         json.dump(table, f, indent=1)
 
 
+ALI_BIN = os.path.join(HERE, "_ref", "opcount_ali")
+
+
+def build_ali():
+    os.makedirs(os.path.dirname(ALI_BIN), exist_ok=True)
+    src = os.path.join(HERE, "opcount_ali.cc")
+    deps = [src, os.path.join(HERE, "opcount.cc"), os.path.join(HERE, "snapi_oracle_ali.c"),
+            os.path.join(HERE, "snapi_oracle.c"), os.path.join(HERE, "snapi_oracle.h")]
+    if os.path.exists(ALI_BIN) and all(os.path.getmtime(ALI_BIN) >= os.path.getmtime(d) for d in deps):
+        return ALI_BIN
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas", "-o", ALI_BIN, src],
+                   check=True)
+    return ALI_BIN
+
+
+def count_ali(p, cols):
+    """Per column of `cols` of an ALI problem (workloads.ali_problem): (counted
+    operations, raw operations, acc [nd][ntrans][4])."""
+    import ctypes
+    exe = build_ali()
+    g = p["grid"]
+    lam, lamw, fudge, sigma, theta, wray = g._keep
+    atom = p["atom"]
+    cs = p["colstate"]
+    nd = cs.shape[2]
+    nbg = p["bgop"].shape[1]
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "in.bin")
+        with open(path, "wb") as f:
+            f.write(struct.pack("<7i", len(cols), nd, g.nlam, g.nray, g.bg_per_ray, g.tau_grid, nbg))
+            f.write(struct.pack("<d", g.boundary))
+            f.write(ctypes.string_at(ctypes.addressof(atom), ctypes.sizeof(atom)))
+            for a in (lam, lamw, fudge, sigma, theta, wray):
+                f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+            for c in cols:
+                for k in ("colstate", "vproj", "gamma", "bgop", "bgem", "pops"):
+                    f.write(np.ascontiguousarray(p[k][c], dtype=np.float64).tobytes())
+        out = subprocess.run([exe, path], capture_output=True, text=True, check=True).stdout.split("\n")
+    res = []
+    nt = atom.ntrans
+    for line in out:
+        if not line.strip():
+            continue
+        w = line.split()
+        res.append((int(w[1]), int(w[2]), np.array([float.fromhex(x) for x in w[3:]]).reshape(nd, nt, 4)))
+    return res
+
+
+def ali_main():
+    sys.path.insert(0, REPO)
+    from snapi_amd import workloads
+    p = workloads.ali_problem(64, 64)
+    ncs = 8
+    res = count_ali(p, list(range(ncs)))
+    c = np.array([r[0] for r in res], dtype=np.float64)
+    nsolve = p["nlam"] * p["nray"]
+    table = {"ali": {"flops_per_formal_solve": float(c.mean() / nsolve),
+                     "raw_per_formal_solve": float(np.mean([r[1] for r in res]) / nsolve),
+                     "columns": ncs, "depths": int(p["colstate"].shape[2]), "wavelengths": int(p["nlam"]),
+                     "rays": int(p["nray"]), "counted_per_column": [int(x) for x in c],
+                     "workload": "bench.py ALI leg (ali_problem 64 x 64) at its starting populations, first 8 "
+                                 "columns; one accumulation pass"},
+             "method": ("oracle/opcount_ali.cc: snapi_oracle_ali.c's orc_ali_accumulate (with snapi_oracle.c's "
+                        "Voigt function and scalar formal solution) with the op-counting double of "
+                        "oracle/opcount.cc; add, sub, mul, div, sqrt, exp, log, log10, sin, cos = 1, pow = 3; each "
+                        "(operation, operands) counted once per column"),
+             "command": "python oracle/opcount.py ali"}
+    print(table["ali"], flush=True)
+    with open(os.path.join(REPO, "profiles", "ali_flops.json"), "w") as f:
+        json.dump(table, f, indent=1)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "chem":
         return chem_main()
+    if len(sys.argv) > 1 and sys.argv[1] == "ali":
+        return ali_main()
     table = {}
     for name, make in (("cfg0", cfg0_case), ("cube512", cube512_sample)):
         tb, ray, st, lam = make()

@@ -38,10 +38,10 @@ int orc_ali_accumulate(const snapi_ali_atom *at, const snapi_ali_grid *g, int nd
                        const double *vproj, const double *gamma, const double *bgop, const double *bgem,
                        const double *pops, double *acc) {
   const int nl = g->nlam, nt = at->ntrans, nlev = at->nlevel;
-  double *op = malloc(sizeof(double) * nd), *em = malloc(sizeof(double) * nd);
-  double *opn = malloc(sizeof(double) * nd), *emn = malloc(sizeof(double) * nd);
-  double *I = malloc(sizeof(double) * nd), *L = malloc(sizeof(double) * nd);
-  double *prof = malloc(sizeof(double) * nd * nt);
+  double *op = (double *)malloc(sizeof(double) * nd), *em = (double *)malloc(sizeof(double) * nd);
+  double *opn = (double *)malloc(sizeof(double) * nd), *emn = (double *)malloc(sizeof(double) * nd);
+  double *I = (double *)malloc(sizeof(double) * nd), *L = (double *)malloc(sizeof(double) * nd);
+  double *prof = (double *)malloc(sizeof(double) * nd * nt);
   const double *T = cs + SNAPI_A_T * nd, *vt = cs + SNAPI_A_VT * nd, *ne = cs + SNAPI_A_NE * nd;
   const double *opref = cs + SNAPI_A_OPREF * nd, *grid = cs + SNAPI_A_GRID * nd;
   memset(acc, 0, sizeof(double) * nd * nt * 4);
@@ -122,7 +122,7 @@ int orc_ali_accumulate(const snapi_ali_atom *at, const snapi_ali_grid *g, int nd
 /* pivot-free Crout LU (cmlib/mathtools.cc:487-514: unit upper triangle,
  * diagonal in L) and the solve, for an n x n row-major matrix */
 static void crout_solve(int n, const double *A, const double *b, double *x) {
-  double *D = malloc(sizeof(double) * n * n), *y = malloc(sizeof(double) * n);
+  double *D = (double *)malloc(sizeof(double) * n * n), *y = (double *)malloc(sizeof(double) * n);
   for (int k = 0; k < n; ++k) {
     for (int i = k; i < n; ++i) {
       double sum = 0.0;
@@ -175,8 +175,8 @@ static double line_rate(const snapi_ali_atom *at, int t, int from, int to, doubl
 double orc_ali_newpops(const snapi_ali_atom *at, int nd, const double *cs, const double *coll, const double *pops,
                        const double *acc, int alo, double *newp) {
   const int n = at->nlevel, nt = at->ntrans;
-  double *M = malloc(sizeof(double) * n * n), *b = malloc(sizeof(double) * n);
-  int *bf_of = malloc(sizeof(int) * n), *stage = malloc(sizeof(int) * n);
+  double *M = (double *)malloc(sizeof(double) * n * n), *b = (double *)malloc(sizeof(double) * n);
+  int *bf_of = (int *)malloc(sizeof(int) * n), *stage = (int *)malloc(sizeof(int) * n);
   for (int i = 0; i < n; ++i) bf_of[i] = -1;
   for (int t = 0; t < nt; ++t)
     if (at->tr[t].kind == SNAPI_TR_BF) bf_of[at->tr[t].lower] = t;
@@ -254,7 +254,7 @@ int orc_ali_solve(const snapi_ali_atom *at, const snapi_ali_grid *g, int ncol, i
   for (int col = 0; col < ncol; ++col) {
     const double *cs = colstate + (size_t)col * SNAPI_A_NF * nd;
     double *p = pops + (size_t)col * nd * n;
-    double *acc = malloc(sizeof(double) * nd * nt * 4), *np = malloc(sizeof(double) * nd * n);
+    double *acc = (double *)malloc(sizeof(double) * nd * nt * 4), *np = (double *)malloc(sizeof(double) * nd * n);
     double rc = 1.0;
     int it;
     for (it = 0; it < max_iter; ++it) {

@@ -35,6 +35,14 @@ int chem_he_tangent(const ChemDevice &cd, long ncol, int nd, const double *T, co
 // state; resp/dstate may be null; columns with active[c] == 0 are skipped
 int chem_eos(const ChemDevice &cd, long ncol, int nd, double *state, double *resp, double *dstate, double *scratch,
              const int *active, hipStream_t st);
+// the same pass with derivatives, whose first five chemical equilibria per
+// point also give the hydrostatic tangent's partials at cd_he's referent
+// wavelength (he_scratch: chem_he_tangent_scratch bytes), then that tangent
+// (as chem_he_tangent) at the state's T and Nt, given as T, Nt [ncol][nd]
+int chem_eos_he_tangent(const ChemDevice &cd, const ChemDevice &cd_he, long ncol, int nd, double *state,
+                        double *resp, double *dstate, double *scratch, const double *T, const double *logtau,
+                        const double *Nt, int ndir, const double *dT, double *dN, double *he_scratch,
+                        const int *active, hipStream_t st);
 
 // Gaussian spectral convolution of nseries [nl][ncomp] series in place
 // (instrument.hip); lam_h the host copy of lam_d; series with active == 0

@@ -905,12 +905,20 @@ __host__ __device__ inline EosSeq eos_sequence(double T0, double Nt0) {
 // One team per (depth point, variant): chemical equilibrium at the variant's
 // (T, Nt); the record holds Ne, the referent opacity and the wanted stage
 // populations.  state [ncol][nf][nd] supplies T and NT.
+// With he_rec, the first five variants (the point, T +- 2.5 K, Nt +- half the
+// Nt step: the hydrostatic tangent's points, up to the rounding of the
+// reference's in-place steps) also give rho / kappa_ref at the referent
+// wavelength of the second table tb2 (the hydrostatic march's), in
+// he_partials_kernel's record layout [point][5].
 __global__ __launch_bounds__(kChemThreads) void eos_variants_kernel(const snapi_chem_table *__restrict__ tb,
                                                                     const ChemDerived *__restrict__ dv, long npt,
                                                                     int nd, int nf, int nvar,
                                                                     const double *__restrict__ state, EosWant want,
                                                                     double *__restrict__ rec,
-                                                                    const int *__restrict__ active) {
+                                                                    const int *__restrict__ active,
+                                                                    const snapi_chem_table *__restrict__ tb2,
+                                                                    const ChemDerived *__restrict__ dv2,
+                                                                    double *__restrict__ he_rec) {
   __shared__ ChemShared sh;
   extern __shared__ double region_s[];
   const int tid = threadIdx.x;
@@ -933,9 +941,13 @@ __global__ __launch_bounds__(kChemThreads) void eos_variants_kernel(const snapi_
   const int nrec = 2 + want.nwant;
   double *r = rec + t * nrec;
   const double op = (v <= 4) ? referent_opacity(tb, dv, cx, ct, sol) : 0.0;
+  const bool he = he_rec && v <= 4;  // uniform over the team
+  const double op2 = he ? referent_opacity(tb2, dv2, cx, ct, sol) : 1.0;
+  const double rho = mass_density(tshfl(sol.Nown, 1));
   if (cx.lane == 0) {
     r[0] = sol.N0;
     r[1] = op;
+    if (he) he_rec[pt * 5 + v] = rho / op2;
   }
   if (cx.is_el)
     for (int z = 0; z < cx.ns; ++z) {
@@ -1369,12 +1381,13 @@ int eos_want(const snapi_chem_table *tb, const snapi_state_map *map, EosWant &w)
 
 int launch_eos(const snapi_chem_table *dtb, const ChemDerived *ddv, const snapi_state_map *dmap, int nslot, long npt,
                int nd, int nf, int nvar, const EosWant &want, double *rec, double *state, double *resp,
-               double *dstate, hipStream_t st, const int *active = nullptr) {
+               double *dstate, hipStream_t st, const int *active = nullptr,
+               const snapi_chem_table *dtb2 = nullptr, const ChemDerived *ddv2 = nullptr, double *he_rec = nullptr) {
   long blocks;
   size_t lds;
   if (!team_grid(npt * nvar, nslot, blocks, lds)) return SNAPI_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(eos_variants_kernel, dim3((unsigned)blocks), dim3(kChemThreads), lds, st, dtb, ddv, npt, nd, nf,
-                     nvar, state, want, rec, active);
+                     nvar, state, want, rec, active, dtb2, ddv2, he_rec);
   if (hipGetLastError() != hipSuccess) return SNAPI_ERR_DEVICE;
   const long b2 = (npt + 255) / 256;
   if (b2 > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
@@ -1440,6 +1453,31 @@ int chem_he_tangent(const ChemDevice &cd, long ncol, int nd, const double *T, co
   return launch_he_tangent((const snapi_chem_table *)cd.buf,
                            (const ChemDerived *)((const char *)cd.buf + sizeof(snapi_chem_table)), cd.nslot, ncol, nd,
                            T, logtau, Nt, ndir, dT, dN, scratch, st, active);
+}
+
+int chem_eos_he_tangent(const ChemDevice &cd, const ChemDevice &cd_he, long ncol, int nd, double *state,
+                        double *resp, double *dstate, double *scratch, const double *T, const double *logtau,
+                        const double *Nt, int ndir, const double *dT, double *dN, double *he_scratch,
+                        const int *active, hipStream_t st) {
+  if (!resp && !dstate) return SNAPI_ERR_ARG;
+  // the two tables must hold the same species (they differ in the referent
+  // wavelength): the team layout of the first serves the second's opacity
+  if (cd.nslot != cd_he.nslot) return SNAPI_ERR_ARG;
+  EosWant w;
+  memcpy(&w, cd.want, sizeof(w));
+  const auto *tb2 = (const snapi_chem_table *)cd_he.buf;
+  const auto *dv2 = (const ChemDerived *)((const char *)cd_he.buf + sizeof(snapi_chem_table));
+  int rc = launch_eos((const snapi_chem_table *)cd.buf,
+                      (const ChemDerived *)((const char *)cd.buf + sizeof(snapi_chem_table)),
+                      (const snapi_state_map *)cd.map, cd.nslot, ncol * nd, nd, cd.nf, kEosVariants, w, scratch, state,
+                      resp, dstate, st, active, tb2, dv2, he_scratch);
+  if (rc) return rc;
+  const long b2 = (ncol * ndir + 127) / 128;
+  if (b2 > 0x7fffffffL) return SNAPI_ERR_UNSUPPORTED;
+  if (b2 > 0)
+    hipLaunchKernelGGL(he_tangent_kernel, dim3((unsigned)b2), dim3(128), 0, st, ncol, nd, ndir, T, logtau, Nt,
+                       he_scratch, dT, dN, active);
+  return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
 }
 
 int chem_eos(const ChemDevice &cd, long ncol, int nd, double *state, double *resp, double *dstate, double *scratch,

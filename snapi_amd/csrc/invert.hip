@@ -1027,21 +1027,27 @@ static int invert_impl(int precision, int jacobian, const snapi_line_table *line
           rc = chem_hequilibrium(cdev_he, (long)nslot * nvar, nd, Tv, ltv, Ntv, hmask, nvar, st);
           if (rc) return rc;
           if (na.analytic) {
-            if (ntn) {
-              hipLaunchKernelGGL(tderiv_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, snodes, ltv, dTd,
-                                 nmap, ss.needj);
-              rc = chem_he_tangent(cdev_he, nslot, nd, Tv, ltv, Ntv, ntn, dTd, dNd, hscr, ss.needj, st);
-              if (rc) return rc;
-            }
-            hipLaunchKernelGGL(tmap2_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, dNd, nmap2, ss.needj);
             hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, 1, Tv, Ntv, sstate,
                                (R *)nullptr, (R *)nullptr, sd, ss.needj);
+            if (ntn) {
+              // the hydrostatic tangent's partials come from the first five
+              // chemical equilibria of the equation-of-state pass at the same
+              // points (T, T +- 2.5 K, Nt +- its half step)
+              hipLaunchKernelGGL(tderiv_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, snodes, ltv, dTd,
+                                 nmap, ss.needj);
+              rc = chem_eos_he_tangent(cdev, cdev_he, nslot, nd, sd, rsd, dsd, escr, Tv, ltv, Ntv, ntn, dTd, dNd, hscr,
+                                       ss.needj, st);
+            } else {
+              rc = chem_eos(cdev, nslot, nd, sd, rsd, dsd, escr, ss.needj, st);
+            }
+            if (rc) return rc;
+            hipLaunchKernelGGL(tmap2_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, ntn, dNd, nmap2, ss.needj);
           } else {
             hipLaunchKernelGGL(tmaps_kernel<R>, dim3(nslot), dim3(128), 0, st, na, nslot, nd, nf, nvar, Tv, Ntv, sstate,
                                nmap, nmap2, sd, ss.needj);
+            rc = chem_eos(cdev, nslot, nd, sd, rsd, dsd, escr, ss.needj, st);
+            if (rc) return rc;
           }
-          rc = chem_eos(cdev, nslot, nd, sd, rsd, dsd, escr, ss.needj, st);
-          if (rc) return rc;
           hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * ns, rsd, rstate);
           hipLaunchKernelGGL((convert_kernel<double, R>), dim3(1024), dim3(256), 0, st, (size_t)nslot * 2 * ns, dsd,
                              rdst);

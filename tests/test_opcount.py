@@ -110,3 +110,19 @@ def test_frozen_chem_table_matches_a_fresh_count():
     ph = workloads.hequilibrium_problem(128, 128)
     assert opcount.count_chem(ph["table"], ph["T"][:1], ph["logtau"][:1], 1)[0][0] == \
         table["hequilibrium"]["counted_per_column"][0]
+
+
+# ---- ALI accumulation (oracle/opcount_ali.cc) ----
+
+def test_ali_harness_reproduces_the_oracle_bit_for_bit():
+    from snapi_amd import workloads
+    p = workloads.ali_problem(1, 2, nlam=12, nmu=1)
+    res = opcount.count_ali(p, [0, 1])
+    for c in range(2):
+        q = {k: (v[c] if k in ("colstate", "vproj", "gamma", "bgop", "bgem", "pops") else v) for k, v in p.items()}
+        q["nd"], q["ntrans"] = p["colstate"].shape[2], p["atom"].ntrans
+        acc = _oracle.ali_accumulate(q, p["pops"][c])
+        assert np.array_equal(acc, res[c][2])
+    # the wavelength- and ray-independent work (Doppler widths, damping,
+    # Planck and Saha factors per depth) is counted once per column
+    assert res[0][0] < res[0][1]

@@ -213,8 +213,10 @@ int snapi_device_count(void);
  * Launch-shape autotuner (no counterpart in the reference, whose solver runs
  * one column per CPU process).  The synthesis kernels come in three
  * occupancy bounds (float: 8, 7, 6 wavefronts per SIMD; double: the
- * compiler's own, 3, 4) and the node-Jacobian tangent kernels in workgroups
- * of 64, 128 and 256 work-items.  Every shape computes the same results bit
+ * compiler's own, 3, 4), the node-Jacobian tangent kernels in workgroups
+ * of 64, 128 and 256 work-items, and the float ALI accumulation of atoms of
+ * up to 10 lines and 6 bound-free transitions in bounds of 4, 3 and 2
+ * wavefronts per SIMD.  Every shape computes the same results bit
  * for bit.  With tuning on (the default; SNAPI_AUTOTUNE=0 in the environment
  * turns it off), the first call for a problem shape (kernel, precision,
  * depths, lines, components, wavelengths, node pass) on a device times each
@@ -224,8 +226,9 @@ int snapi_device_count(void);
  * off, take the first shape of the list unless the shape has been chosen
  * already.
  *   snapi_autotune_mode: 0 off, 1 on, -1 query; returns the previous mode.
- *   snapi_autotune_force: kernel 0 synthesis, 1 tangent; candidate index to
- *     launch from now on, -1 to return to the tuned choice.
+ *   snapi_autotune_force: kernel 0 synthesis, 1 tangent, 2 ALI
+ *     accumulation; candidate index to launch from now on, -1 to return to
+ *     the tuned choice.
  *   snapi_autotune_clear: forgets every choice.
  *   snapi_autotune_report: one JSON object per line and choice (device,
  *     kernel, precision, nd, nline, ncomp, nlambda, extra, choice, ms of each

@@ -1,13 +1,14 @@
 """Launch-shape autotuner of the library (snapi_autotune_*): the synthesis
-kernels in three occupancy bounds and the node-Jacobian tangent kernels in
-three workgroup sizes, timed on the first call of each problem shape and
-the fastest kept for the process.  Every shape gives the same results."""
+kernels and the float ALI accumulation in three occupancy bounds each, the
+node-Jacobian tangent kernels in three workgroup sizes, timed on the first
+call of each problem shape and the fastest kept for the process.  Every
+shape gives the same results."""
 import ctypes as C
 import json
 
 from ._native import lib, check
 
-SYNTH, TANGENT = 0, 1
+SYNTH, TANGENT, ALI = 0, 1, 2
 
 
 def _decl():
@@ -33,8 +34,8 @@ def mode(m=-1):
 
 
 def force(kernel, candidate):
-    """Launches candidate `candidate` of `kernel` (SYNTH or TANGENT) from now
-    on; -1 returns to the tuned choice."""
+    """Launches candidate `candidate` of `kernel` (SYNTH, TANGENT or ALI)
+    from now on; -1 returns to the tuned choice."""
     check("snapi_autotune_force", _decl().snapi_autotune_force(int(kernel), int(candidate)))
 
 

@@ -27,6 +27,7 @@
 #include <mutex>
 #include <vector>
 
+#include "autotune.h"
 #include "rt_common.h"
 #include "synth_kernel.h"
 
@@ -168,10 +169,13 @@ template <> struct AliNorm<float> {
 // registers.  On 4096
 // columns x 81 depths x 220 wavelengths x 10 rays: 2 waves (188 VGPRs)
 // 5.99 s, 4 waves 5.12 s, 5 waves 7.17 s, 8 waves 8.98 s.
+// The float variants of the small and mid atoms also come held to three and
+// two wavefronts, among which the autotuner picks (autotune.h); on the Ca II
+// workload five and six wavefronts measured 1.5x and 2x slower than four.
 template <typename R> constexpr int kAliWaves = 1;
 template <> constexpr int kAliWaves<float> = 4;
-template <typename R, bool kLds, int kL, int kB, int kS>
-__global__ __launch_bounds__(kAliThreads, kAliWaves<R>) void ali_acc_kernel(const AliAtom A, const AliGridArgs G, int nd, AliCol P) {
+template <typename R, bool kLds, int kL, int kB, int kS, int W = kAliWaves<R>>
+__global__ __launch_bounds__(kAliThreads, W) void ali_acc_kernel(const AliAtom A, const AliGridArgs G, int nd, AliCol P) {
   extern __shared__ unsigned char smem_raw[];
   const int col = blockIdx.x;
   if (P.active && !P.active[col]) return;  // uniform over the workgroup
@@ -663,6 +667,13 @@ static int convert_grid(const snapi_ali_grid *g, AliGridArgs &G) {
 // Atoms of at most kSmallL lines and kSmallB bound-free transitions run the
 // variant with 32 reduced slots and shorter register arrays.
 constexpr int kSmallL = 6, kSmallB = 6;
+// and atoms of at most kMidL lines and kMidB bound-free transitions (every
+// level pair of a 5-level ion is a line, as the reference's opacity takes
+// them: 10 lines and 5 bound-free for Ca II 5 + continuum) the 64-slot
+// reduction with register arrays of their own size
+constexpr int kMidL = 10, kMidB = 6;
+// columns of the autotuner's sample
+constexpr int kAliTuneColumns = 4096;
 
 static size_t acc_smem(int precision, const AliAtom &A, int nd, int threads, bool lds_sums) {
   const size_t es = precision == SNAPI_FP64 ? 8 : 4;
@@ -679,10 +690,40 @@ static int launch_acc(int precision, const AliAtom &A, const AliGridArgs &G, int
   const size_t smem = acc_smem(precision, A, nd, threads, lds);
   if (smem > 65536) return SNAPI_ERR_UNSUPPORTED;
   const bool small = A.nline <= kSmallL && A.nbf <= kSmallB && 3 * A.nline + 2 * A.nbf <= 32;
+  const bool mid = A.nline <= kMidL && A.nbf <= kMidB;
   const dim3 g(ncol), b(threads);
+  if (precision != SNAPI_FP64 && (small || mid) && lds) {
+    // float, small or mid atom: occupancy bound chosen by the autotuner; a
+    // timed run clears the sums first, as every caller does before the launch
+    using AccFn = void (*)(const AliAtom, const AliGridArgs, int, AliCol);
+    const AccFn fs[3] = {ali_acc_kernel<float, true, kSmallL, kSmallB, 32, 4>,
+                         ali_acc_kernel<float, true, kSmallL, kSmallB, 32, 3>,
+                         ali_acc_kernel<float, true, kSmallL, kSmallB, 32, 2>};
+    const AccFn fm[3] = {ali_acc_kernel<float, true, kMidL, kMidB, 64, 4>, ali_acc_kernel<float, true, kMidL, kMidB, 64, 3>,
+                         ali_acc_kernel<float, true, kMidL, kMidB, 64, 2>};
+    const AccFn *fn = small ? fs : fm;
+    const size_t per_col = (size_t)nd * A.ntrans * 4;
+    auto run = [&](int c, int nc, bool clear) {
+      if (clear)
+        hipLaunchKernelGGL(ali_zero_kernel, dim3(1024), dim3(256), 0, st, per_col, nc, P.active, P.acc);
+      hipLaunchKernelGGL(fn[c], dim3(nc), b, smem, st, A, G, nd, P);
+      return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
+    };
+    const int ns = ncol < kAliTuneColumns ? ncol : kAliTuneColumns;
+    const TuneKey key{TUNE_ALI, SNAPI_MIXED, nd, A.nline, A.nbf, G.nlam,
+                      (tune_size_class(ncol, kAliTuneColumns) << 8) + G.nray};
+    bool tuned = false;
+    const int c = tune_choose(key, 3, 0, st, true, [&](int cand) {
+      tuned = true;
+      return run(cand, ns, true);
+    });
+    return run(c, ncol, tuned);
+  }
   if (precision == SNAPI_FP64) {
     if (small && lds)
       hipLaunchKernelGGL((ali_acc_kernel<double, true, kSmallL, kSmallB, 32>), g, b, smem, st, A, G, nd, P);
+    else if (mid && lds)
+      hipLaunchKernelGGL((ali_acc_kernel<double, true, kMidL, kMidB, 64>), g, b, smem, st, A, G, nd, P);
     else if (lds)
       hipLaunchKernelGGL((ali_acc_kernel<double, true, kAliMaxLines, kAliMaxBf, 64>), g, b, smem, st, A, G, nd, P);
     else
@@ -690,6 +731,8 @@ static int launch_acc(int precision, const AliAtom &A, const AliGridArgs &G, int
   } else {
     if (small && lds)
       hipLaunchKernelGGL((ali_acc_kernel<float, true, kSmallL, kSmallB, 32>), g, b, smem, st, A, G, nd, P);
+    else if (mid && lds)
+      hipLaunchKernelGGL((ali_acc_kernel<float, true, kMidL, kMidB, 64>), g, b, smem, st, A, G, nd, P);
     else if (lds)
       hipLaunchKernelGGL((ali_acc_kernel<float, true, kAliMaxLines, kAliMaxBf, 64>), g, b, smem, st, A, G, nd, P);
     else

@@ -10,7 +10,7 @@
 
 namespace snapi {
 
-enum TuneKernel { TUNE_SYNTH = 0, TUNE_TANGENT = 1, TUNE_NKERNEL = 2 };
+enum TuneKernel { TUNE_SYNTH = 0, TUNE_TANGENT = 1, TUNE_ALI = 2, TUNE_NKERNEL = 3 };
 
 // Problem shape that a choice is kept for.
 struct TuneKey {

@@ -30,7 +30,7 @@ using Key = std::tuple<int, int, int, int, int, int, int, int>;  // device + Tun
 std::mutex g_mu;
 std::map<Key, Entry> g_table;
 int g_mode = -1;                 // -1: not read from the environment yet
-int g_force[TUNE_NKERNEL] = {-1, -1};
+int g_force[TUNE_NKERNEL] = {-1, -1, -1};
 
 int mode_locked() {
   if (g_mode < 0) {

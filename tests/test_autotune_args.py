@@ -12,7 +12,7 @@ def test_mode_force_and_report_arguments():
         assert autotune.mode() == 0
         assert autotune.mode(1) == 0
         assert L.snapi_autotune_mode(7) == abi.SNAPI_ERR_ARG
-        assert L.snapi_autotune_force(2, 0) == abi.SNAPI_ERR_ARG
+        assert L.snapi_autotune_force(3, 0) == abi.SNAPI_ERR_ARG
         assert L.snapi_autotune_force(0, -2) == abi.SNAPI_ERR_ARG
         assert L.snapi_autotune_force(0, 8) == abi.SNAPI_ERR_ARG
         autotune.force(autotune.SYNTH, 1)

@@ -74,3 +74,19 @@ def test_tuner_times_every_shape_once():
     b = api.synth_stokes(t["table"], t["ray"], st, lam, abi.SNAPI_MIXED)
     assert np.array_equal(a, b)
     assert len([x for x in autotune.report() if x["kernel"] == autotune.SYNTH and x["nlambda"] == len(lam)]) == 1
+
+
+def test_ali_accumulation_shapes_agree_bit_for_bit():
+    """The float ALI accumulation of the Ca II atom (10 lines, 5 bound-free)
+    in its three occupancy bounds: the same sums bit for bit."""
+    from snapi_amd import ali
+    p = workloads.ali_problem(4, 4, nlam=64, nmu=2)
+    args = [p[k] for k in ("colstate", "vproj", "gamma", "bgop", "bgem", "pops")]
+    res = []
+    for c in range(3):
+        print("ALI accumulation, mixed, occupancy shape %d: 16 columns" % c, flush=True)
+        autotune.force(autotune.ALI, c)
+        res.append(ali.accumulate(p["atom"], p["grid"], *args, precision=abi.SNAPI_MIXED))
+    autotune.force(autotune.ALI, -1)
+    for c in (1, 2):
+        assert np.array_equal(res[0], res[c]), c

@@ -19,6 +19,19 @@ namespace snapi {
 // then (kernels with thermodynamic directions) temperature and density
 enum { Q_VT = 0, Q_VL = 1, Q_B = 2, Q_TH = 3, Q_PH = 4, Q_T = 5, Q_N = 6, NQ = 5, NQ_TH = 7 };
 
+// A set of parameter rows as a bit mask (bit q: row q above), and the
+// position of row q among the rows of the set: kernels carry only the rows
+// of the set their node directions read, in that compact order.
+constexpr unsigned kRowsKin = 0x1Fu;  // vt, vLOS, |B|, inclination, azimuth
+constexpr unsigned kRowsAll = 0x7Fu;  // and temperature, density
+__host__ __device__ constexpr int nrows(unsigned m) { return m ? (int)(m & 1u) + nrows(m >> 1) : 0; }
+__host__ __device__ constexpr int rslot(unsigned m, int q) { return nrows(m & ((1u << q) - 1u)); }
+__host__ __device__ constexpr bool has_row(unsigned m, int q) { return (m >> q) & 1u; }
+// the row at position `slot` of the set
+__host__ __device__ constexpr int row_of(unsigned m, int slot, int q = 0) {
+  return q > 31 ? -1 : has_row(m, q) ? (slot == 0 ? q : row_of(m, slot - 1, q + 1)) : row_of(m, slot, q + 1);
+}
+
 // parameter row of a per-depth response index (0 T, 1 Nt, 2 vt, 3 vLOS,
 // 4 |B|, 5 inclination, 6 azimuth)
 __host__ __device__ inline int qrow(int pidx) { return pidx >= 2 ? pidx - 2 : (pidx == 0 ? Q_T : Q_N); }
@@ -214,10 +227,10 @@ __device__ void stage_deriv(const KLines &L, const SynthArgs &a, const S *__rest
 // Opacity (K, E) of depth z and its derivatives G[q] = d(K, E)/dq along the
 // parameters in qmask, entries 0..6 = K, 7..10 = E.  FULL = false computes
 // only the eta_I entry G[q][0] (the look-ahead of the optical-depth step).
-template <typename R, bool FULL, int NQR>
+template <typename R, bool FULL, unsigned ROWS>
 __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, const ColumnTile<R> &t,
                                          const DerivTile<R> &d, const ThermoTile<R> &th, const LamConst<R> &c, int z,
-                                         unsigned qmask, R K[7], R E[4], R G[NQR][11]) {
+                                         unsigned qmask, R K[7], R E[4], R G[nrows(ROWS)][11]) {
   R KI = t.bgop[z], EI = t.bgem[z];
   if (a.hminus) {
     const R tinv = t.tinv[z];
@@ -228,14 +241,27 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
     KI += o;
     EI += o * B;
   }
+  constexpr int NR = nrows(ROWS);
 #pragma unroll
-  for (int q = 0; q < NQR; ++q)
+  for (int q = 0; q < NR; ++q)
 #pragma unroll
     for (int i = 0; i < 11; ++i) G[q][i] = R(0);
-  const bool wvt = qmask & (1u << Q_VT), wvl = qmask & (1u << Q_VL), wb = qmask & (1u << Q_B);
-  const bool wth = qmask & (1u << Q_TH), wph = qmask & (1u << Q_PH);
-  constexpr bool kTh = NQR > Q_N;
-  const bool wT = kTh && (qmask & (1u << Q_T)), wN = kTh && (qmask & (1u << Q_N));
+  // each row's place in G (0 for a row outside the set, whose code is dead)
+  constexpr int sVT = has_row(ROWS, Q_VT) ? rslot(ROWS, Q_VT) : 0;
+  constexpr int sVL = has_row(ROWS, Q_VL) ? rslot(ROWS, Q_VL) : 0;
+  constexpr int sB = has_row(ROWS, Q_B) ? rslot(ROWS, Q_B) : 0;
+  constexpr int sTH = has_row(ROWS, Q_TH) ? rslot(ROWS, Q_TH) : 0;
+  constexpr int sPH = has_row(ROWS, Q_PH) ? rslot(ROWS, Q_PH) : 0;
+  constexpr int sT = has_row(ROWS, Q_T) ? rslot(ROWS, Q_T) : 0;
+  constexpr int sN = has_row(ROWS, Q_N) ? rslot(ROWS, Q_N) : 0;
+  const bool wvt = has_row(ROWS, Q_VT) && (qmask & (1u << Q_VT));
+  const bool wvl = has_row(ROWS, Q_VL) && (qmask & (1u << Q_VL));
+  const bool wb = has_row(ROWS, Q_B) && (qmask & (1u << Q_B));
+  const bool wth = has_row(ROWS, Q_TH) && (qmask & (1u << Q_TH));
+  const bool wph = has_row(ROWS, Q_PH) && (qmask & (1u << Q_PH));
+  constexpr bool kTh = has_row(ROWS, Q_T) || has_row(ROWS, Q_N);
+  const bool wT = has_row(ROWS, Q_T) && (qmask & (1u << Q_T));
+  const bool wN = has_row(ROWS, Q_N) && (qmask & (1u << Q_N));
   R KQ = 0, KU = 0, KV = 0, RQ = 0, RU = 0, RV = 0, EQ = 0, EU = 0, EV = 0;
   const int nd = a.nd;
   const R s2h = t.s2h[z], c2q = t.c2q[z], qf = t.qf[z], uf = t.uf[z], vf = t.vf[z];
@@ -255,24 +281,24 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
       EI += eml * H;
       const R gt = -gv * (H + x0 * Hx - av * Fx), gl = cvl * Hx;
       if (wvt) {
-        G[Q_VT][0] += opl * gt;
-        if (FULL) G[Q_VT][7] += eml * gt;
+        G[sVT][0] += opl * gt;
+        if (FULL) G[sVT][7] += eml * gt;
       }
       if (wvl) {
-        G[Q_VL][0] += opl * gl;
-        if (FULL) G[Q_VL][7] += eml * gl;
+        G[sVL][0] += opl * gl;
+        if (FULL) G[sVL][7] += eml * gl;
       }
       if constexpr (kTh) {
         if (wT) {
           const R gT = th.gT[i], adT = th.adT[i];
           const R gtt = -gT * (H + x0 * Hx - av * Fx) - adT * Fx;
-          G[Q_T][0] += opl * gtt + th.pT[i] * H;
-          if (FULL) G[Q_T][7] += eml * gtt + th.eT[i] * H;
+          G[sT][0] += opl * gtt + th.pT[i] * H;
+          if (FULL) G[sT][7] += eml * gtt + th.eT[i] * H;
         }
         if (wN) {
           const R gnn = -th.aN[i] * Fx;
-          G[Q_N][0] += opl * gnn + th.pN[i] * H;
-          if (FULL) G[Q_N][7] += eml * gnn + th.eN[i] * H;
+          G[sN][0] += opl * gnn + th.pN[i] * H;
+          if (FULL) G[sN][7] += eml * gnn + th.eN[i] * H;
         }
       }
       continue;
@@ -330,7 +356,7 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
         h[g] = -gv * (HS[g] + x0 * HX[g] + lbd * HXS[g] - av * FX[g]);
         f[g] = -gv * (FS[g] + x0 * FX[g] + lbd * FXS[g] + av * HX[g]);
       }
-      add(h, f, G[Q_VT], FULL);
+      add(h, f, G[sVT], FULL);
     }
     if (wvl) {
       R h[3], f[3];
@@ -339,7 +365,7 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
         h[g] = cvl * HX[g];
         f[g] = cvl * FX[g];
       }
-      add(h, f, G[Q_VL], FULL);
+      add(h, f, G[sVL], FULL);
     }
     if (wb) {
       R h[3], f[3];
@@ -348,7 +374,7 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
         h[g] = cB * HXS[g];
         f[g] = cB * FXS[g];
       }
-      add(h, f, G[Q_B], FULL);
+      add(h, f, G[sB], FULL);
     }
     if constexpr (kTh) {
       if (wT) {
@@ -359,8 +385,8 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
           h[g] = -gT * (HS[g] + x0 * HX[g] + lbd * HXS[g] - av * FX[g]) - adT * FX[g];
           f[g] = -gT * (FS[g] + x0 * FX[g] + lbd * FXS[g] + av * HX[g]) + adT * HX[g];
         }
-        add(h, f, G[Q_T], FULL);
-        addf(HS, FS, G[Q_T], FULL, th.pT[i], th.eT[i]);
+        add(h, f, G[sT], FULL);
+        addf(HS, FS, G[sT], FULL, th.pT[i], th.eT[i]);
       }
       if (wN) {
         const R aN = th.aN[i];
@@ -370,8 +396,8 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
           h[g] = -aN * FX[g];
           f[g] = aN * HX[g];
         }
-        add(h, f, G[Q_N], FULL);
-        addf(HS, FS, G[Q_N], FULL, th.pN[i], th.eN[i]);
+        add(h, f, G[sN], FULL);
+        addf(HS, FS, G[sN], FULL, th.pN[i], th.eN[i]);
       }
     }
     const R hl = HS[0] - R(0.5) * (HS[2] + HS[1]), fl = FS[0] - R(0.5) * (FS[2] + FS[1]);
@@ -379,29 +405,29 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
       // d/d inclination; rho_V as the reference writes it (cos, :999)
       const R sct = d.sct[z], dqt = d.dqt[z], dut = d.dut[z], mhs = d.mhs[z];
       const R eI = sct * hl;
-      G[Q_TH][0] += eI * opl;
+      G[sTH][0] += eI * opl;
       if (FULL) {
         const R eQ = dqt * hl, eU = dut * hl, eV = mhs * (HS[2] - HS[1]);
-        G[Q_TH][1] += eQ * opl;
-        G[Q_TH][2] += eU * opl;
-        G[Q_TH][3] += eV * opl;
-        G[Q_TH][4] += dqt * fl * opl;
-        G[Q_TH][5] += dut * fl * opl;
-        G[Q_TH][6] += vf * (FS[2] - FS[1]) * opl;
-        G[Q_TH][7] += eI * eml;
-        G[Q_TH][8] += eQ * eml;
-        G[Q_TH][9] += eU * eml;
-        G[Q_TH][10] += eV * eml;
+        G[sTH][1] += eQ * opl;
+        G[sTH][2] += eU * opl;
+        G[sTH][3] += eV * opl;
+        G[sTH][4] += dqt * fl * opl;
+        G[sTH][5] += dut * fl * opl;
+        G[sTH][6] += vf * (FS[2] - FS[1]) * opl;
+        G[sTH][7] += eI * eml;
+        G[sTH][8] += eQ * eml;
+        G[sTH][9] += eU * eml;
+        G[sTH][10] += eV * eml;
       }
     }
     if (wph && FULL) {
       // d/d azimuth: d qf = -2 uf, d uf = 2 qf
-      G[Q_PH][1] += R(-2) * uf * hl * opl;
-      G[Q_PH][2] += R(2) * qf * hl * opl;
-      G[Q_PH][4] += R(-2) * uf * fl * opl;
-      G[Q_PH][5] += R(2) * qf * fl * opl;
-      G[Q_PH][8] += R(-2) * uf * hl * eml;
-      G[Q_PH][9] += R(2) * qf * hl * eml;
+      G[sPH][1] += R(-2) * uf * hl * opl;
+      G[sPH][2] += R(2) * qf * hl * opl;
+      G[sPH][4] += R(-2) * uf * fl * opl;
+      G[sPH][5] += R(2) * qf * fl * opl;
+      G[sPH][8] += R(-2) * uf * hl * eml;
+      G[sPH][9] += R(2) * qf * hl * eml;
     }
   }
   K[0] = KI; K[1] = KQ; K[2] = KU; K[3] = KV; K[4] = RQ; K[5] = RU; K[6] = RV;
@@ -410,27 +436,27 @@ __device__ __forceinline__ void deriv_at(const KLines &L, const SynthArgs &a, co
     // the background's T and Nt terms, then the referent-opacity
     // normalisation: d(K / r) = dK / r - (K / r) d ln r
     if (wT) {
-      G[Q_T][0] += th.bKT[z];
-      if (FULL) G[Q_T][7] += th.bET[z];
+      G[sT][0] += th.bKT[z];
+      if (FULL) G[sT][7] += th.bET[z];
       const R rt = th.rT[z];
-      G[Q_T][0] -= rt * KI;
+      G[sT][0] -= rt * KI;
       if (FULL) {
 #pragma unroll
-        for (int k = 1; k < 7; ++k) G[Q_T][k] -= rt * K[k];
+        for (int k = 1; k < 7; ++k) G[sT][k] -= rt * K[k];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) G[Q_T][7 + k] -= rt * E[k];
+        for (int k = 0; k < 4; ++k) G[sT][7 + k] -= rt * E[k];
       }
     }
     if (wN) {
-      G[Q_N][0] += th.bKN[z];
-      if (FULL) G[Q_N][7] += th.bEN[z];
+      G[sN][0] += th.bKN[z];
+      if (FULL) G[sN][7] += th.bEN[z];
       const R rn = th.rN[z];
-      G[Q_N][0] -= rn * KI;
+      G[sN][0] -= rn * KI;
       if (FULL) {
 #pragma unroll
-        for (int k = 1; k < 7; ++k) G[Q_N][k] -= rn * K[k];
+        for (int k = 1; k < 7; ++k) G[sN][k] -= rn * K[k];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) G[Q_N][7 + k] -= rn * E[k];
+        for (int k = 0; k < 4; ++k) G[sN][7 + k] -= rn * E[k];
       }
     }
   }

@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kRespThreads) void resp_kernel(const KLines lines, 
       const int z = zof(k);
       if constexpr (STAGED) {
         R Kz[7], Ez[4], G[NQ_TH][11];
-        deriv_at<R, true, NQ_TH>(lines, args, tile, dtile, ttile, c, z, (1u << NQ_TH) - 1u, Kz, Ez, G);
+        deriv_at<R, true, kRowsAll>(lines, args, tile, dtile, ttile, c, z, kRowsAll, Kz, Ez, G);
         for (int s = 0; s < ns; ++s) {
           R Kb[7], Eb[4];
           for (int i = 0; i < 7; ++i) Kb[i] = scr.adj(k, i, s, w);

@@ -35,13 +35,13 @@ namespace snapi {
 
 // u_q = dS_q - dK~_q I of one depth for every parameter in qmask: the
 // derivative of the reduced source terms (K~ = K/eta_I - 1, S = E/eta_I).
-template <typename R, int NQR>
-__device__ __forceinline__ void reduced_u(const R G[NQR][11], R eta, const Kred<R> &k, const R S[4], const R I[4],
-                                          unsigned qmask, R u[NQR][4], R ds0[NQR]) {
+template <typename R, unsigned ROWS>
+__device__ __forceinline__ void reduced_u(const R G[nrows(ROWS)][11], R eta, const Kred<R> &k, const R S[4],
+                                          const R I[4], unsigned qmask, R u[nrows(ROWS)][4], R ds0[nrows(ROWS)]) {
   const R ie = rrcp<R>(eta);
 #pragma unroll
-  for (int q = 0; q < NQR; ++q) {
-    if (!(qmask & (1u << q))) {
+  for (int q = 0; q < nrows(ROWS); ++q) {  // q: position in the row set
+    if (!(qmask & (1u << row_of(ROWS, q)))) {
       u[q][0] = u[q][1] = u[q][2] = u[q][3] = R(0);
       ds0[q] = R(0);
       continue;
@@ -109,10 +109,10 @@ __device__ __forceinline__ R fb_deriv_p(R om, R o0, R op, R h1, R h2, R &gm, R &
 // One workgroup per column of one pass of up to kTanMaxNodes nodes,
 // work-items over wavelengths.  Writes the emergent Stokes vector (pass 0)
 // and J[col][l*4+s][n] = dI_s/dnode_n * nscale[n] * ws[s] * mask[col][l].
-// NQR = NQ_TH: the pass carries temperature or density rows, whose local
-// derivatives read dstate; each node's direction is
-// nmap[n] d/dq_pidx[n] + nmap2[n] d/dq_pidx2[n].
-template <typename R, int NQR>
+// ROWS: the parameter rows the kernel carries (a superset of the pass's own);
+// with the temperature or density row the local derivatives read dstate and
+// each node's direction is nmap[n] d/dq_pidx[n] + nmap2[n] d/dq_pidx2[n].
+template <typename R, unsigned ROWS>
 __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArgs &args, const RespArgs &ra,
                                              const R *__restrict__ state, const R *__restrict__ dstate,
                                              const double *__restrict__ lam, R *__restrict__ stokes,
@@ -123,7 +123,8 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
   const int col = ra.col0 + blockIdx.x;
   if (ra.active && !ra.active[col]) return;  // uniform over the workgroup
   const int n0 = pass * kTanMaxNodes;
-  constexpr bool kTh = NQR > Q_N;
+  constexpr int NR = nrows(ROWS);
+  constexpr bool kTh = has_row(ROWS, Q_T) || has_row(ROWS, Q_N);
   const int ng = min(ra.nnode - n0, kTanMaxNodes);
   const R *st = state + (size_t)col * nf * nd;
   ColumnTile<R> tile = ColumnTile<R>::carve(sm, nd, lines.nline);
@@ -146,18 +147,18 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
   __syncthreads();
   // one-hot parameter rows of each node of the pass (the second: the
   // density row of a temperature node)
-  R oh[kTanMaxNodes][NQR], oh2[kTh ? kTanMaxNodes : 1][NQR];
+  R oh[kTanMaxNodes][NR], oh2[kTh ? kTanMaxNodes : 1][NR];
   unsigned qmask = 0;
 #pragma unroll
   for (int j = 0; j < kTanMaxNodes; ++j) {
     const int q = j < ng ? qrow(ra.pidx[n0 + j]) : -1;
 #pragma unroll
-    for (int k = 0; k < NQR; ++k) oh[j][k] = (q == k) ? R(1) : R(0);
+    for (int k = 0; k < NR; ++k) oh[j][k] = (q == row_of(ROWS, k)) ? R(1) : R(0);
     if (q >= 0) qmask |= 1u << q;
     if constexpr (kTh) {
       const int q2 = (j < ng && ra.nmap2 && ra.pidx2[n0 + j] >= 0) ? qrow(ra.pidx2[n0 + j]) : -1;
 #pragma unroll
-      for (int k = 0; k < NQR; ++k) oh2[j][k] = (q2 == k) ? R(1) : R(0);
+      for (int k = 0; k < NR; ++k) oh2[j][k] = (q2 == row_of(ROWS, k)) ? R(1) : R(0);
       if (q2 >= 0) qmask |= 1u << q2;
     }
   }
@@ -175,20 +176,20 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
   for (int l = threadIdx.x; l < nl; l += blockDim.x) {
     LamConst<R> c;
     lambda_consts<R>(lines, args, lam[l], c);
-    R G[NQR][11], u[NQR][4], ds0[NQR];
+    R G[NR][11], u[NR][4], ds0[NR];
     R K0[7], E0[4], K1[7], E1[4];
     // bottom point: full derivatives, boundary I = (S_I, 0, 0, 0)
     const int z0 = zof(0), z1 = zof(1);
-    deriv_at<R, true, NQR>(lines, args, tile, dt, tht, c, z0, qmask, K0, E0, G);
+    deriv_at<R, true, ROWS>(lines, args, tile, dt, tht, c, z0, qmask, K0, E0, G);
     Kred<R> kp, kc;
     R Sp[4], Sc[4];
     reduce_ks<R>(K0, E0, kp, Sp);
     R Ip[4] = {up ? E0[0] / K0[0] : R(0), R(0), R(0), R(0)};
-    reduced_u<R, NQR>(G, K0[0], kp, Sp, Ip, qmask, u, ds0);
+    reduced_u<R, ROWS>(G, K0[0], kp, Sp, Ip, qmask, u, ds0);
     R dI[kTanMaxNodes][4], v[kTanMaxNodes][4], dem[kTanMaxNodes], de0[kTanMaxNodes], dDm[kTanMaxNodes];
-    R ds0r[NQR][1];
+    R ds0r[NR][1];
 #pragma unroll
-    for (int k = 0; k < NQR; ++k) ds0r[k][0] = ds0[k];
+    for (int k = 0; k < NR; ++k) ds0r[k][0] = ds0[k];
 #pragma unroll
     for (int j = 0; j < kTanMaxNodes; ++j) {
       const bool on = j < ng;
@@ -206,7 +207,7 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
       }
       dI[j][0] = up ? d0 : R(0);
     }
-    deriv_at<R, false, NQR>(lines, args, tile, dt, tht, c, z1, qmask, K1, E1, G);
+    deriv_at<R, false, ROWS>(lines, args, tile, dt, tht, c, z1, qmask, K1, E1, G);
     reduce_ks<R>(K1, E1, kc, Sc);
     R oom = K0[0], oo0 = K1[0];
     R g0 = tile.grid[z1];
@@ -221,7 +222,7 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
     for (int k = 1; k <= nd - 2; ++k) {
       const int zk = zof(k), zn = zof(k + 1);
       R Kn[7], En[4];
-      deriv_at<R, false, NQR>(lines, args, tile, dt, tht, c, zn, qmask, Kn, En, G);
+      deriv_at<R, false, ROWS>(lines, args, tile, dt, tht, c, zn, qmask, Kn, En, G);
       R dep[kTanMaxNodes];
 #pragma unroll
       for (int j = 0; j < kTanMaxNodes; ++j) dep[j] = j < ng ? ndir(j, zn, G, 0) : R(0);
@@ -253,8 +254,8 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
       // full derivatives of depth k, reduced with the new intensity
       {
         R Kd[7], Ed[4];
-        deriv_at<R, true, NQR>(lines, args, tile, dt, tht, c, zk, qmask, Kd, Ed, G);
-        reduced_u<R, NQR>(G, oo0, kc, Sc, In, qmask, u, ds0);
+        deriv_at<R, true, ROWS>(lines, args, tile, dt, tht, c, zk, qmask, Kd, Ed, G);
+        reduced_u<R, ROWS>(G, oo0, kc, Sc, In, qmask, u, ds0);
       }
       const R hh = h0 * h0 * R(1.0 / 12.0);
 #pragma unroll
@@ -305,34 +306,43 @@ __device__ __forceinline__ void tangent_body(const KLines &lines, const SynthArg
 }
 
 // The float kernels are held to two wavefronts per SIMD (256 registers, some
-// spilled): their 392 registers would otherwise leave one wavefront per SIMD,
-// and the sweep waits on memory and LDS; measured 1.34x faster on the
-// inversion workload (four wavefronts, 128 VGPRs with 393 spilled, measured
-// 0.7x).  The double kernels keep their registers.  The _th variants carry the
-// temperature and density rows.  Each comes in workgroups of 64, 128 and 256
-// work-items (one column, its wavelengths over the work-items), among which
-// the autotuner picks per problem shape (autotune.h): larger workgroups share
-// the column's staged tiles over more wavelengths at once, smaller ones keep
-// more columns resident when the tiles fill the LDS.
-#define SNAPI_TANGENT_KERNEL(NAME, R, NQR, BOUNDS)                                                                 \
+// spilled): their registers would otherwise leave one wavefront per SIMD, and
+// the sweep waits on memory and LDS; measured 1.34x faster on the inversion
+// workload than at four wavefronts (128 VGPRs, heavy spilling).  The double
+// kernels keep their registers.  Each kernel carries one of four row sets,
+// the smallest that holds the pass's parameters is launched (fewer rows, fewer
+// live registers): all five kinematic and magnetic rows, the field rows
+// without the turbulent velocity, all seven rows, or turbulent velocity with
+// temperature and density (a pass of temperature nodes).  Each comes in
+// workgroups of 64, 128 and 256 work-items (one column, its wavelengths over
+// the work-items), among which the autotuner picks per problem shape
+// (autotune.h): larger workgroups share the column's staged tiles over more
+// wavelengths at once, smaller ones keep more columns resident when the tiles
+// fill the LDS.
+constexpr unsigned kRowsField = (1u << Q_VL) | (1u << Q_B) | (1u << Q_TH) | (1u << Q_PH);
+constexpr unsigned kRowsTemp = (1u << Q_VT) | (1u << Q_T) | (1u << Q_N);
+constexpr int kTanSets = 4;
+constexpr unsigned kTanRowSets[kTanSets] = {kRowsKin, kRowsField, kRowsAll, kRowsTemp};
+
+#define SNAPI_TANGENT_KERNEL(NAME, R, ROWS, BOUNDS)                                                                \
   __global__ BOUNDS void NAME(const KLines lines, const SynthArgs args, const RespArgs ra,                        \
                               const R *__restrict__ state, const R *__restrict__ dstate,                          \
                               const double *__restrict__ lam, R *__restrict__ stokes, R *__restrict__ J,         \
                               int *__restrict__ status, int pass) {                                               \
-    tangent_body<R, NQR>(lines, args, ra, state, dstate, lam, stokes, J, status, pass);                           \
+    tangent_body<R, ROWS>(lines, args, ra, state, dstate, lam, stokes, J, status, pass);                          \
   }
-SNAPI_TANGENT_KERNEL(tangent_kernel_f, float, NQ, __launch_bounds__(64, 2))
-SNAPI_TANGENT_KERNEL(tangent_kernel_th_f, float, NQ_TH, __launch_bounds__(64, 2))
-SNAPI_TANGENT_KERNEL(tangent_kernel_d, double, NQ, __launch_bounds__(64))
-SNAPI_TANGENT_KERNEL(tangent_kernel_th_d, double, NQ_TH, __launch_bounds__(64))
-SNAPI_TANGENT_KERNEL(tangent_kernel_f128, float, NQ, __launch_bounds__(128, 2))
-SNAPI_TANGENT_KERNEL(tangent_kernel_th_f128, float, NQ_TH, __launch_bounds__(128, 2))
-SNAPI_TANGENT_KERNEL(tangent_kernel_d128, double, NQ, __launch_bounds__(128))
-SNAPI_TANGENT_KERNEL(tangent_kernel_th_d128, double, NQ_TH, __launch_bounds__(128))
-SNAPI_TANGENT_KERNEL(tangent_kernel_f256, float, NQ, __launch_bounds__(256, 2))
-SNAPI_TANGENT_KERNEL(tangent_kernel_th_f256, float, NQ_TH, __launch_bounds__(256, 2))
-SNAPI_TANGENT_KERNEL(tangent_kernel_d256, double, NQ, __launch_bounds__(256))
-SNAPI_TANGENT_KERNEL(tangent_kernel_th_d256, double, NQ_TH, __launch_bounds__(256))
+#define SNAPI_TANGENT_SET(SUF, ROWS)                                                                              \
+  SNAPI_TANGENT_KERNEL(tangent_kernel_##SUF##_f, float, ROWS, __launch_bounds__(64, 2))                          \
+  SNAPI_TANGENT_KERNEL(tangent_kernel_##SUF##_f128, float, ROWS, __launch_bounds__(128, 2))                      \
+  SNAPI_TANGENT_KERNEL(tangent_kernel_##SUF##_f256, float, ROWS, __launch_bounds__(256, 2))                      \
+  SNAPI_TANGENT_KERNEL(tangent_kernel_##SUF##_d, double, ROWS, __launch_bounds__(64))                            \
+  SNAPI_TANGENT_KERNEL(tangent_kernel_##SUF##_d128, double, ROWS, __launch_bounds__(128))                        \
+  SNAPI_TANGENT_KERNEL(tangent_kernel_##SUF##_d256, double, ROWS, __launch_bounds__(256))
+SNAPI_TANGENT_SET(kin, kRowsKin)
+SNAPI_TANGENT_SET(field, kRowsField)
+SNAPI_TANGENT_SET(all, kRowsAll)
+SNAPI_TANGENT_SET(temp, kRowsTemp)
+#undef SNAPI_TANGENT_SET
 #undef SNAPI_TANGENT_KERNEL
 
 template <typename R>
@@ -341,15 +351,31 @@ using TanFn = void (*)(const KLines, const SynthArgs, const RespArgs, const R *,
 // Candidates of the autotuner (workgroup sizes), the default first.
 constexpr int kTanCand = 3;
 constexpr int kTanCandThreads[kTanCand] = {64, 128, 256};
-static TanFn<float> tan_fn(float *, bool th, int c) {
-  if (c == 1) return th ? tangent_kernel_th_f128 : tangent_kernel_f128;
-  if (c == 2) return th ? tangent_kernel_th_f256 : tangent_kernel_f256;
-  return th ? tangent_kernel_th_f : tangent_kernel_f;
+static TanFn<float> tan_fn(float *, int set, int c) {
+  static const TanFn<float> t[kTanSets][kTanCand] = {
+      {tangent_kernel_kin_f, tangent_kernel_kin_f128, tangent_kernel_kin_f256},
+      {tangent_kernel_field_f, tangent_kernel_field_f128, tangent_kernel_field_f256},
+      {tangent_kernel_all_f, tangent_kernel_all_f128, tangent_kernel_all_f256},
+      {tangent_kernel_temp_f, tangent_kernel_temp_f128, tangent_kernel_temp_f256}};
+  return t[set][c];
 }
-static TanFn<double> tan_fn(double *, bool th, int c) {
-  if (c == 1) return th ? tangent_kernel_th_d128 : tangent_kernel_d128;
-  if (c == 2) return th ? tangent_kernel_th_d256 : tangent_kernel_d256;
-  return th ? tangent_kernel_th_d : tangent_kernel_d;
+static TanFn<double> tan_fn(double *, int set, int c) {
+  static const TanFn<double> t[kTanSets][kTanCand] = {
+      {tangent_kernel_kin_d, tangent_kernel_kin_d128, tangent_kernel_kin_d256},
+      {tangent_kernel_field_d, tangent_kernel_field_d128, tangent_kernel_field_d256},
+      {tangent_kernel_all_d, tangent_kernel_all_d128, tangent_kernel_all_d256},
+      {tangent_kernel_temp_d, tangent_kernel_temp_d128, tangent_kernel_temp_d256}};
+  return t[set][c];
+}
+// The row set to launch for a pass whose parameter rows are `rows`: the
+// smallest that holds them (the sets' sizes: 5, 4, 7, 3).
+static int tan_row_set(unsigned rows) {
+  int best = -1;
+  for (int s = 0; s < kTanSets; ++s)
+    if ((rows & ~kTanRowSets[s]) == 0 &&
+        (best < 0 || __builtin_popcount(kTanRowSets[s]) < __builtin_popcount(kTanRowSets[best])))
+      best = s;
+  return best;
 }
 // Columns of the autotuner's sample.
 constexpr int kTanTuneColumns = 16384;
@@ -376,9 +402,14 @@ int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, 
   if (!status) return SNAPI_ERR_DEVICE;
   ra.col0 = 0;
   for (int pass = 0; pass < npass; ++pass) {
-    bool th = false;
-    for (int n = pass * kTanMaxNodes; n < ra.nnode && n < (pass + 1) * kTanMaxNodes; ++n)
-      th |= ra.pidx[n] < 2 || (ra.nmap2 && ra.pidx2[n] >= 0);
+    unsigned rows = 0;
+    for (int n = pass * kTanMaxNodes; n < ra.nnode && n < (pass + 1) * kTanMaxNodes; ++n) {
+      rows |= 1u << qrow(ra.pidx[n]);
+      if (ra.nmap2 && ra.pidx2[n] >= 0) rows |= 1u << qrow(ra.pidx2[n]);
+    }
+    const int set = tan_row_set(rows);
+    if (set < 0) return SNAPI_ERR_ARG;
+    const bool th = (kTanRowSets[set] & ((1u << Q_T) | (1u << Q_N))) != 0;
     const int ng = ra.nnode - pass * kTanMaxNodes < kTanMaxNodes ? ra.nnode - pass * kTanMaxNodes : kTanMaxNodes;
     const size_t smem = ColumnTile<R>::bytes(a.nd, kl.nline) + DerivTile<R>::bytes(a.nd, kl.nline, ng, th) +
                         (th ? ThermoTile<R>::bytes(a.nd, kl.nline) : 0);
@@ -387,12 +418,12 @@ int launch_tangent(const KLines &kl, const SynthArgs &a, RespArgs ra, int ncol, 
     R *so = static_cast<R *>(dstk_), *Jo = static_cast<R *>(dJ_);
     auto launch = [&](int c, int nc) {
       const int threads = kTanCandThreads[c] < wthreads ? kTanCandThreads[c] : wthreads;
-      hipLaunchKernelGGL(tan_fn((R *)nullptr, th, c), dim3(nc), dim3(threads), smem, stream, kl, a, ra, st, dd, dl,
+      hipLaunchKernelGGL(tan_fn((R *)nullptr, set, c), dim3(nc), dim3(threads), smem, stream, kl, a, ra, st, dd, dl,
                          so, Jo, status, pass);
       return hipGetLastError() == hipSuccess ? SNAPI_OK : SNAPI_ERR_DEVICE;
     };
     const TuneKey key{TUNE_TANGENT, sizeof(R) == 8 ? SNAPI_FP64 : SNAPI_MIXED, a.nd, kl.nline, kl.ncomp, a.nlambda,
-                      (tune_size_class(ncol, kTanTuneColumns) << 8) + (th ? 16 : 0) + ng};
+                      (tune_size_class(ncol, kTanTuneColumns) << 8) + (set << 4) + ng};
     const int ns = ncol < kTanTuneColumns ? ncol : kTanTuneColumns;
     const int c = tune_choose(key, ncand, 0, stream, true, [&](int cand) { return launch(cand, ns); });
     const int rc = launch(c, ncol);

@@ -232,8 +232,8 @@ def main():
         # kernel profile of the 14-node leg); its roofline as the
         # hequilibrium leg of this run measures it
         lm["roofline"] = dict(he["roofline"], kernel="hequilibrium_kernel",
-                              note="dominant kernel of the 14-node inversion (28 % of its GPU time, with the "
-                                   "equation-of-state pass 26 % and the tangent sweeps 28 %); measured in the "
+                              note="dominant kernel of the 14-node inversion (37 % of its GPU time, with the "
+                                   "equation-of-state pass 35 % and the tangent sweeps 20 %); measured in the "
                                    "hequilibrium leg of this run")
     line = {"metric": METRIC, "value": value, "unit": "lambda-point formal solves/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,

@@ -87,3 +87,33 @@ def test_mixed_temperature_node_jacobian_against_fp64(pix, jacobian, analytic):
           % (es, e.max(), e[:, :, :7].max()), flush=True)
     assert es < 1e-4
     assert e.max() < 1e-3
+
+
+def test_tangent_row_sets_node_order(pix):
+    """The tangent Jacobian does not depend on the order of the nodes.  In
+    the natural order the two passes of up to 8 nodes launch the kernels
+    carrying the (vmicro, T, Nt) rows and the (vLOS, B, inclination,
+    azimuth) rows; reordered so that the first pass mixes temperature nodes
+    with a field node and the second holds vmicro with the field nodes, they
+    launch the kernels carrying all seven rows and the five kinematic rows.
+    The columns agree to rounding in FP64 and in mixed precision."""
+    p = pix
+    t = p["t"]
+    order = [0, 10, 1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13]  # each parameter's nodes stay ascending
+    model2 = invert.node_model([NODES[i] for i in order])
+    v2 = np.ascontiguousarray(p["v"][:, order])
+    for prec, st, tol in ((abi.SNAPI_FP64, p["st"], 1e-12), (abi.SNAPI_MIXED, p["st"].astype(np.float32), 1e-5)):
+        print("node order, precision %d" % prec, flush=True)
+        s1, J1 = invert.node_jacobian_eos(t["table"], t["ray"], p["model"], p["chem"], p["chem_he"], p["smap"], st,
+                                          p["v"], p["lam"], precision=prec, analytic_maps=True,
+                                          jacobian=abi.SNAPI_JAC_TANGENT)
+        s2, J2 = invert.node_jacobian_eos(t["table"], t["ray"], model2, p["chem"], p["chem_he"], p["smap"], st, v2,
+                                          p["lam"], precision=prec, analytic_maps=True,
+                                          jacobian=abi.SNAPI_JAC_TANGENT)
+        J1o = J1[..., order].astype(np.float64)
+        scale = np.maximum(np.abs(J1o).max(axis=1, keepdims=True), 1e-300)
+        e = np.abs(J2.astype(np.float64) - J1o) / scale
+        es = np.max(np.abs(s2.astype(np.float64) - s1)) / np.max(s1[..., 0])
+        print("  Stokes %.2e Ic; J max %.2e of each column's largest value" % (es, e.max()), flush=True)
+        assert es < tol
+        assert e.max() < tol
